@@ -1,0 +1,170 @@
+/*
+ * rtb_hip.h — C ABI of librtb_hip.so, the MI355X (gfx950) k-mesh hot path of
+ * the Rustb tight-binding crate (LiuyichenYanwushang/Rustb v0.7.2).
+ *
+ * Conventions (all entry points):
+ *   - Synchronous; return RTB_OK (0) or a negative RTB_ERR_* code; the message
+ *     of the last failure on the calling thread is rtb_last_error().
+ *   - Every host buffer is owned by the caller; outputs are caller-allocated.
+ *   - Complex numbers are interleaved (re, im) doubles; matrices are C-order.
+ *   - k-points are fractional reciprocal coordinates, one row of `dim` doubles.
+ *   - n = nsta = norb (spinless) or 2*norb (spinful, [up block, down block]).
+ *   - Thread safety: concurrent calls on one rtb_model are allowed (each call
+ *     uses its own HIP stream and device work buffers).
+ *   - No call allocates on the host beyond small bookkeeping; device memory is
+ *     stream-ordered (hipMallocAsync) and released before the call returns.
+ */
+#ifndef RTB_HIP_H
+#define RTB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: one per reference TbError variant raised on this path ---- */
+#define RTB_OK 0
+#define RTB_ERR_KVECTOR_LENGTH -1        /* TbError::KVectorLengthMismatch (error.rs:291) */
+#define RTB_ERR_DIMENSION_MISMATCH -2    /* TbError::DimensionMismatch (error.rs:99) */
+#define RTB_ERR_INVALID_KMESH -3         /* TbError::InvalidKmeshDimensions (error.rs:135) */
+#define RTB_ERR_INVALID_ENERGY_RANGE -4  /* TbError::InvalidEnergyRange (error.rs:138) */
+#define RTB_ERR_INVALID_DOS_PARAMETER -5 /* TbError::InvalidDosParameter (error.rs:185) */
+#define RTB_ERR_INVALID_RESPONSE -6      /* TbError::InvalidResponseParameter (error.rs:179) */
+#define RTB_ERR_SPIN_NOT_ALLOWED -7      /* TbError::SpinNotAllowed (error.rs:132) */
+#define RTB_ERR_EIGEN_FAILED -8          /* TbError::EigenvalueComputationFailed (error.rs:92) */
+#define RTB_ERR_ONSITE_NOT_REAL -9       /* TbError::OnsiteHoppingMustBeReal */
+#define RTB_ERR_R_LENGTH -10             /* TbError::RVectorLengthError */
+#define RTB_ERR_INVALID_MODEL -11        /* TbError::InvalidModelInvariant / LatticeDimensionError */
+#define RTB_ERR_INVALID_ARGUMENT -12     /* null pointer, zero size where one is required */
+#define RTB_ERR_DEVICE -13               /* a HIP runtime call failed */
+#define RTB_ERR_UNSUPPORTED -14          /* a size outside what the kernels implement */
+
+/* ---- enums ---- */
+#define RTB_GAUGE_LATTICE 0 /* Gauge::Lattice (model.rs:449-453) */
+#define RTB_GAUGE_ATOM 1    /* Gauge::Atom */
+
+#define RTB_SPIN_NONE 0 /* Option<SpinDirection>::None (model.rs:470-475) */
+#define RTB_SPIN_X 1
+#define RTB_SPIN_Y 2
+#define RTB_SPIN_Z 3
+
+#define RTB_PREC_FP64 0  /* plain form: the reference's formulas in FP64 */
+#define RTB_PREC_MIXED 1 /* mixed form: FP32 eigensolve + FP64 refinement */
+
+/* =====================================================================
+ * Model builder (host only, no GPU).  Mirrors the reference's hopping API
+ * so C/C++ hosts can build `Model` storage with the same semantics.
+ * ===================================================================== */
+typedef struct rtb_builder rtb_builder;
+
+/* Model::tb_model (model_build.rs:212-285): ham = one zero block at R = 0. */
+int rtb_builder_new(int dim, int spin, int norb, const double* lat /*dim*dim rows*/,
+                    const double* orb /*norb*dim*/, rtb_builder** out);
+void rtb_builder_destroy(rtb_builder* b);
+/* set_hop (model_build.rs:354-470) / add_hop (model_build.rs:477-558);
+ * pauli = RTB_SPIN_* decoration, ignored for spinless models. */
+int rtb_builder_set_hop(rtb_builder* b, double re, double im, int i, int j, const int64_t* R,
+                        int pauli);
+int rtb_builder_add_hop(rtb_builder* b, double re, double im, int i, int j, const int64_t* R,
+                        int pauli);
+/* add_element (model_build.rs:613-687): spin-doubled indices, no decoration. */
+int rtb_builder_add_element(rtb_builder* b, double re, double im, int i, int j, const int64_t* R);
+/* set_onsite (model_build.rs:689-700) / add_onsite (model_build.rs:737-750). */
+int rtb_builder_set_onsite(rtb_builder* b, const double* values /*norb*/, int pauli);
+int rtb_builder_add_onsite(rtb_builder* b, const double* values /*norb*/, int pauli);
+/* Storage queries and export (model.rs:150-158 layout). */
+int rtb_builder_n_R(const rtb_builder* b);
+int rtb_builder_nsta(const rtb_builder* b);
+int rtb_builder_export(const rtb_builder* b, int64_t* hamR /*n_R*dim*/, double* ham /*n_R*n*n*2*/);
+
+/* Seeded synthetic workloads (SplitMix64-seeded xoshiro256**, Box-Muller),
+ * identical on every host.  `kind` selects the recipe:
+ *   1: random spinful 3D model, |R_i|<=rmax, amplitude 0.5*exp(-|R|) (scale=0.5, decay=1)
+ *   2: Wannier-like 3D model, amplitude exp(-|R|/decay)
+ *   3: 2D model with n_R drawn R vectors within |R_i|<=rmax, amplitude exp(-|R|)
+ *   4: 2D slab of `layers` layers x norb_layer orbitals (see the synthetic-model notes)
+ * The sizes are returned first with ham == NULL, then the arrays are filled. */
+typedef struct rtb_synth_spec {
+  int kind;
+  int dim;
+  int spin;
+  int norb;        /* orbitals (per layer for kind 4) */
+  int rmax;        /* |R_i| bound */
+  int n_R_target;  /* kind 3: number of stored R (odd, includes R = 0) */
+  int layers;      /* kind 4 */
+  double scale;    /* amplitude prefactor */
+  double decay;    /* exp(-|R|/decay) */
+  uint64_t seed;
+} rtb_synth_spec;
+int rtb_synth_model(const rtb_synth_spec* spec, int* n_R, int* nsta, int64_t* hamR, double* ham,
+                    double* orb, double* lat);
+
+/* =====================================================================
+ * Device-resident model and the hot path
+ * ===================================================================== */
+typedef struct rtb_model rtb_model;
+
+/* Copies a1 storage (model.rs:150-158) to HBM once.  hamR row 0 must be R = 0.
+ * Replaces the per-k reads of `ham` by BLAS zaxpy (ndarray_lapack.rs:35-39). */
+int rtb_model_create(int dim, int spin, int norb, int n_R, const int64_t* hamR,
+                     const double* ham /*n_R*n*n complex*/, const double* orb /*norb*dim*/,
+                     const double* lat /*dim*dim rows*/, rtb_model** out);
+void rtb_model_destroy(rtb_model* m);
+int rtb_model_nsta(const rtb_model* m);
+
+/* Model::gen_ham (model_physics.rs:43-148) and Model::gen_v_projected
+ * (velocity.rs:470-731) for nk k-points.  H: nk*n*n complex.  If n_proj > 0,
+ * dirs is n_proj*dim and V receives nk*n_proj*n*n complex. */
+int rtb_gen_ham_batch(const rtb_model* m, const double* k, size_t nk, int gauge,
+                      const double* dirs, int n_proj, double* H, double* V);
+
+/* Solve::solve_band_all(_parallel) (solve_ham.rs:82-107): E nk*n ascending. */
+int rtb_solve_band_all(const rtb_model* m, const double* k, size_t nk, int precision, double* E);
+
+/* Solve::solve_all(_parallel) (solve_ham.rs:150-184): E nk*n and evec
+ * nk*n*n complex with the eigenvectors as ROWS (solve_ham.rs:127). */
+int rtb_solve_all(const rtb_model* m, const double* k, size_t nk, int precision, double* E,
+                  double* evec);
+
+/* BerryCurvature::berry_curvature_at (response/traits.rs:60-78,160-200) for
+ * nk k-points: omega, E nk*n.  dir is 2*dim (rows a, b). */
+int rtb_berry_curvature(const rtb_model* m, const double* k, size_t nk, const double* dir,
+                        int spin_dir, double eta, int precision, double* omega, double* E);
+
+/* BerryCurvature::occupied_berry_curvature_on (response/traits.rs:104-150):
+ * out[k] = sum_n Omega_n f(E_n, mu) at temperature T (kelvin). */
+int rtb_occupied_berry_curvature(const rtb_model* m, const double* k, size_t nk,
+                                 const double* dir, int spin_dir, double eta, double mu,
+                                 double T, int precision, double* out);
+
+/* Model::hall_conductivity, Integration::Direct (response/linear/mod.rs:160-225):
+ * sigma[n_mu] = sum_k sum_n Omega_n f(E_n, mu) / N_k / det(lat). kmesh has dim entries. */
+int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu,
+                    double T, double eta, const double* dir, int spin_dir, int precision,
+                    double* sigma);
+
+/* Model::dos (model_physics.rs:179-226): E[En] = linspace(Emin, Emax, En), rho[En]. */
+int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, size_t En,
+            double sigma, int precision, double* E, double* rho);
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char* rtb_last_error(void);
+
+/* Per-stage device time of the last hot-path call on this thread, measured
+ * with HIP events on the stream the kernels ran on.  Stages: 0 Fourier,
+ * 1 eigensolve, 2 Kubo/reduction, 3 DOS, 4 whole call (first to last event).
+ * ms[] in milliseconds; launches[] kernel launches per stage. */
+typedef struct rtb_stats {
+  double ms[5];
+  int launches[5];
+  size_t nk;
+  int nsta;
+} rtb_stats;
+int rtb_last_stats(rtb_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTB_HIP_H */

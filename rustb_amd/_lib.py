@@ -1,0 +1,121 @@
+"""ctypes binding of librtb_hip.so (the C ABI declared in include/rtb_hip.h).
+
+The product path has no fallback: if the in-tree library is missing or fails
+to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int64, c_size_t, c_uint64, c_void_p
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librtb_hip.so")
+
+RTB_OK = 0
+ERRORS = {
+    -1: "KVectorLengthMismatch",
+    -2: "DimensionMismatch",
+    -3: "InvalidKmeshDimensions",
+    -4: "InvalidEnergyRange",
+    -5: "InvalidDosParameter",
+    -6: "InvalidResponseParameter",
+    -7: "SpinNotAllowed",
+    -8: "EigenvalueComputationFailed",
+    -9: "OnsiteHoppingMustBeReal",
+    -10: "RVectorLengthError",
+    -11: "InvalidModelInvariant",
+    -12: "InvalidArgument",
+    -13: "Device",
+    -14: "Unsupported",
+}
+
+
+class TbError(RuntimeError):
+    """Error raised by the library; ``kind`` names the reference TbError variant."""
+
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.kind = ERRORS.get(code, f"Unknown({code})")
+        super().__init__(f"{self.kind}: {message}")
+
+
+class SynthSpec(ctypes.Structure):
+    _fields_ = [
+        ("kind", c_int),
+        ("dim", c_int),
+        ("spin", c_int),
+        ("norb", c_int),
+        ("rmax", c_int),
+        ("n_R_target", c_int),
+        ("layers", c_int),
+        ("scale", c_double),
+        ("decay", c_double),
+        ("seed", c_uint64),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ms", c_double * 5), ("launches", c_int * 5), ("nk", c_size_t), ("nsta", c_int)]
+
+
+_P = c_void_p
+SIGNATURES = {
+    "rtb_builder_new": (c_int, [c_int, c_int, c_int, _P, _P, POINTER(c_void_p)]),
+    "rtb_builder_destroy": (None, [c_void_p]),
+    "rtb_builder_set_hop": (c_int, [c_void_p, c_double, c_double, c_int, c_int, _P, c_int]),
+    "rtb_builder_add_hop": (c_int, [c_void_p, c_double, c_double, c_int, c_int, _P, c_int]),
+    "rtb_builder_add_element": (c_int, [c_void_p, c_double, c_double, c_int, c_int, _P]),
+    "rtb_builder_set_onsite": (c_int, [c_void_p, _P, c_int]),
+    "rtb_builder_add_onsite": (c_int, [c_void_p, _P, c_int]),
+    "rtb_builder_n_R": (c_int, [c_void_p]),
+    "rtb_builder_nsta": (c_int, [c_void_p]),
+    "rtb_builder_export": (c_int, [c_void_p, _P, _P]),
+    "rtb_synth_model": (c_int, [POINTER(SynthSpec), POINTER(c_int), POINTER(c_int), _P, _P, _P, _P]),
+    "rtb_model_create": (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P, _P, POINTER(c_void_p)]),
+    "rtb_model_destroy": (None, [c_void_p]),
+    "rtb_model_nsta": (c_int, [c_void_p]),
+    "rtb_gen_ham_batch": (c_int, [c_void_p, _P, c_size_t, c_int, _P, c_int, _P, _P]),
+    "rtb_solve_band_all": (c_int, [c_void_p, _P, c_size_t, c_int, _P]),
+    "rtb_solve_all": (c_int, [c_void_p, _P, c_size_t, c_int, _P, _P]),
+    "rtb_berry_curvature": (c_int, [c_void_p, _P, c_size_t, _P, c_int, c_double, c_int, _P, _P]),
+    "rtb_occupied_berry_curvature": (c_int, [c_void_p, _P, c_size_t, _P, c_int, c_double, c_double,
+                                             c_double, c_int, _P]),
+    "rtb_hall_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, _P]),
+    "rtb_dos": (c_int, [c_void_p, _P, c_double, c_double, c_size_t, c_double, c_int, _P, _P]),
+    "rtb_last_error": (c_char_p, []),
+    "rtb_last_stats": (c_int, [POINTER(Stats)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library (raises if it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m rustb_amd.build` (or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != RTB_OK:
+        msg = load().rtb_last_error().decode(errors="replace")
+        raise TbError(rc, msg)
+
+
+def ptr(a):
+    return a.ctypes.data_as(c_void_p) if a is not None else None
+
+
+def last_stats() -> dict:
+    s = Stats()
+    check(load().rtb_last_stats(ctypes.byref(s)))
+    return {"ms": list(s.ms), "launches": list(s.launches), "nk": s.nk, "nsta": s.nsta}

@@ -1,0 +1,757 @@
+// C ABI entry points: validation (response/config.rs, model_physics.rs),
+// k-chunking, device work buffers and the kernel pipeline of each call.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "eigh.hpp"
+
+namespace rtb {
+
+namespace {
+
+thread_local std::string g_error;
+thread_local rtb_stats g_stats;
+
+// Stream-ordered device allocations released at scope exit.
+class Work {
+ public:
+  explicit Work(hipStream_t s) : st_(s) {}
+  ~Work() {
+    for (void* p : ptrs_) (void)hipFreeAsync(p, st_);
+  }
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    *out = nullptr;
+    if (count == 0) count = 1;
+    void* p = nullptr;
+    hipError_t e = hipMallocAsync(&p, count * sizeof(T), st_);
+    if (e != hipSuccess) {
+      set_error("device allocation of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+      return RTB_ERR_DEVICE;
+    }
+    ptrs_.push_back(p);
+    *out = static_cast<T*>(p);
+    return RTB_OK;
+  }
+
+ private:
+  hipStream_t st_;
+  std::vector<void*> ptrs_;
+};
+
+// A non-blocking stream owned by one call (default priority).
+class Stream {
+ public:
+  Stream() { ok_ = hipStreamCreateWithFlags(&s_, hipStreamNonBlocking) == hipSuccess; }
+  ~Stream() {
+    if (ok_) {
+      (void)hipStreamSynchronize(s_);
+      (void)hipStreamDestroy(s_);
+    }
+  }
+  bool ok() const { return ok_; }
+  hipStream_t get() const { return s_; }
+
+ private:
+  hipStream_t s_{};
+  bool ok_ = false;
+};
+
+// HIP events around the stages of one call (stats for rtb_last_stats).
+class Timer {
+ public:
+  explicit Timer(hipStream_t s) : st_(s) {}
+  ~Timer() {
+    for (auto& r : recs_) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+  }
+  int begin(int stage) {
+    Rec r{stage, nullptr, nullptr, 0};
+    RTB_HIP(hipEventCreate(&r.a));
+    RTB_HIP(hipEventCreate(&r.b));
+    RTB_HIP(hipEventRecord(r.a, st_));
+    recs_.push_back(r);
+    return RTB_OK;
+  }
+  int end(int launches) {
+    recs_.back().launches = launches;
+    RTB_HIP(hipEventRecord(recs_.back().b, st_));
+    return RTB_OK;
+  }
+  // call after the stream is synchronised
+  int collect(size_t nk, int n) {
+    rtb_stats s;
+    std::memset(&s, 0, sizeof(s));
+    s.nk = nk;
+    s.nsta = n;
+    for (auto& r : recs_) {
+      float ms = 0.f;
+      RTB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      s.ms[r.stage] += ms;
+      s.launches[r.stage] += r.launches;
+    }
+    if (!recs_.empty()) {
+      float ms = 0.f;
+      RTB_HIP(hipEventElapsedTime(&ms, recs_.front().a, recs_.back().b));
+      s.ms[4] = ms;
+      for (int i = 0; i < 4; ++i) s.launches[4] += s.launches[i];
+    }
+    g_stats = s;
+    return RTB_OK;
+  }
+
+ private:
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+    int launches;
+  };
+  hipStream_t st_;
+  std::vector<Rec> recs_;
+};
+
+std::once_flag g_smem_once;
+int g_smem_rc = RTB_OK;
+
+int prepare(const rtb_model* m) {
+  if (!m) return fail(RTB_ERR_INVALID_ARGUMENT, "null model");
+  RTB_HIP(hipSetDevice(m->device));
+  std::call_once(g_smem_once, [] { g_smem_rc = eigh_set_smem_limits(); });
+  return g_smem_rc;
+}
+
+bool finite(double x) { return std::isfinite(x); }
+
+// validate_k_mesh (response/config.rs:246-259), validate_kmesh (kpoints.rs:39-47)
+int check_kmesh(const rtb_model* m, const size_t* kmesh, size_t* nk) {
+  if (!kmesh) return fail(RTB_ERR_INVALID_ARGUMENT, "null kmesh");
+  size_t t = 1;
+  for (int d = 0; d < m->dim; ++d) {
+    if (kmesh[d] == 0) return fail(RTB_ERR_INVALID_KMESH, "k-mesh entries must be at least 1");
+    t *= kmesh[d];
+  }
+  *nk = t;
+  return RTB_OK;
+}
+
+// validate_direction_matrix (response/config.rs:202-236), rank 2
+int check_direction(const rtb_model* m, const double* dir) {
+  if (!dir) return fail(RTB_ERR_INVALID_ARGUMENT, "null direction");
+  for (int p = 0; p < 2; ++p) {
+    bool allzero = true;
+    for (int d = 0; d < m->dim; ++d) {
+      double x = dir[p * m->dim + d];
+      if (!finite(x)) return fail(RTB_ERR_INVALID_RESPONSE, "direction: all components must be finite");
+      allzero = allzero && x == 0.0;
+    }
+    if (allzero) return fail(RTB_ERR_INVALID_RESPONSE, "direction: each direction row must not be the zero vector");
+  }
+  return RTB_OK;
+}
+
+int check_eta(double eta) {  // validate_broadening (config.rs:265-273)
+  if (!finite(eta) || eta < 0.0) return fail(RTB_ERR_INVALID_RESPONSE, "broadening: must be finite and non-negative");
+  return RTB_OK;
+}
+
+int check_mu(const double* mu, size_t n_mu) {  // validate_chemical_potentials (config.rs:275-289)
+  if (n_mu == 0 || !mu) return fail(RTB_ERR_INVALID_RESPONSE, "chemical_potentials: must contain at least one value");
+  for (size_t i = 0; i < n_mu; ++i)
+    if (!finite(mu[i])) return fail(RTB_ERR_INVALID_RESPONSE, "chemical_potentials: all values must be finite");
+  return RTB_OK;
+}
+
+int check_T(double T) {  // validate_temperature (config.rs:183-200)
+  if (!finite(T) || T < 0.0) return fail(RTB_ERR_INVALID_RESPONSE, "T: all values must be finite and non-negative");
+  return RTB_OK;
+}
+
+int check_spin(const rtb_model* m, int spin_dir) {
+  if (spin_dir < RTB_SPIN_NONE || spin_dir > RTB_SPIN_Z) return fail(RTB_ERR_INVALID_ARGUMENT, "spin_dir must be 0..3");
+  if (!m->spin && spin_dir != RTB_SPIN_NONE)
+    return fail(RTB_ERR_SPIN_NOT_ALLOWED, "spin current requested on a spinless model");
+  return RTB_OK;
+}
+
+int check_precision(int precision) {
+  if (precision != RTB_PREC_FP64 && precision != RTB_PREC_MIXED)
+    return fail(RTB_ERR_INVALID_ARGUMENT, "precision must be RTB_PREC_FP64 or RTB_PREC_MIXED");
+  return RTB_OK;
+}
+
+// Per-call constant tables: w_p . R_cart and w_p . tau_cart (velocity.rs:530-579)
+int upload_dirs(hipStream_t st, Work& w, const rtb_model* m, const double* dirs, int np, FourierArgs& a) {
+  a.dim = m->dim;
+  a.n = m->nsta;
+  a.nR = m->nR;
+  a.np = np;
+  a.ham = m->d_ham;
+  a.R = m->d_R;
+  a.tau = m->d_tau;
+  a.wR = nullptr;
+  a.wtau = nullptr;
+  if (np == 0) return RTB_OK;
+  std::vector<double> wR(static_cast<size_t>(np) * m->nR), wt(static_cast<size_t>(np) * m->nsta);
+  const int dim = m->dim;
+  for (int p = 0; p < np; ++p) {
+    const double* w = dirs + p * dim;
+    for (int r = 0; r < m->nR; ++r) {
+      double s = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        double rc = 0.0;  // R_cart[d] = sum_i R_i lat[i][d]
+        for (int i = 0; i < dim; ++i) rc += static_cast<double>(m->hamR[r * dim + i]) * m->lat[i * dim + d];
+        s += w[d] * rc;
+      }
+      wR[static_cast<size_t>(p) * m->nR + r] = s;
+    }
+    for (int o = 0; o < m->nsta; ++o) {
+      const int orb = o % m->norb;
+      double s = 0.0;
+      for (int d = 0; d < dim; ++d) {
+        double tc = 0.0;
+        for (int i = 0; i < dim; ++i) tc += m->orb[orb * dim + i] * m->lat[i * dim + d];
+        s += w[d] * tc;
+      }
+      wt[static_cast<size_t>(p) * m->nsta + o] = s;
+    }
+  }
+  double *dwR, *dwt;
+  RTB_TRY(w.alloc(&dwR, wR.size()));
+  RTB_TRY(w.alloc(&dwt, wt.size()));
+  RTB_HIP(hipMemcpyAsync(dwR, wR.data(), wR.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  RTB_HIP(hipMemcpyAsync(dwt, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  a.wR = dwR;
+  a.wtau = dwt;
+  return RTB_OK;
+}
+
+// k rows of `dim` doubles -> device [nk][3] (zero padded)
+int upload_k(hipStream_t st, Work& w, int dim, const double* k, size_t nk, double** dk) {
+  std::vector<double> kp(nk * 3, 0.0);
+  for (size_t i = 0; i < nk; ++i)
+    for (int d = 0; d < dim; ++d) kp[i * 3 + d] = k[i * dim + d];
+  RTB_TRY(w.alloc(dk, kp.size()));
+  RTB_HIP(hipMemcpyAsync(*dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  return RTB_OK;
+}
+
+size_t chunk_for(size_t nk, size_t per_k_bytes) {
+  const size_t budget = size_t(4) << 30;
+  size_t c = std::max<size_t>(1, budget / std::max<size_t>(per_k_bytes, 1));
+  return std::min(nk, c);
+}
+
+// The eigen + Kubo pipeline for one chunk of k-points whose H (and V when
+// omega is wanted) are already on the device.
+struct EigBufs {
+  double *Vg, *tau, *Y, *S;
+};
+
+int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
+  const size_t n2 = static_cast<size_t>(n) * n;
+  b.Vg = b.tau = nullptr;
+  if (vectors) {
+    RTB_TRY(w.alloc(&b.Vg, K * n2 * 2));
+    RTB_TRY(w.alloc(&b.tau, K * n * 2));
+    RTB_TRY(w.alloc(&b.Y, K * n2));
+    RTB_TRY(w.alloc(&b.S, K * n2 * 3));
+  } else {
+    b.Y = b.S = nullptr;
+  }
+  return RTB_OK;
+}
+
+// Kubo for a chunk: V = [K][2][n][n]; Z = [K][n][n] (columns); omega [K][n]
+//   A = Z^H J Z, B = Z^H v_b Z (response/traits.rs:177-180)
+int kubo_chunk(hipStream_t st, const rtb_model* m, size_t K, const double* V, const double* Z, const double* E,
+               int spin_dir, double eta, double* T1, double* A, double* B, double* J, double* omega) {
+  const int n = m->nsta;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const double* Ja = V;
+  size_t sJ = 2 * n2;  // complex elements between consecutive k in V
+  if (m->spin && spin_dir != RTB_SPIN_NONE) {
+    RTB_TRY(spin_current(st, m->norb, K, spin_dir, V, 2 * n2 * 2, J));
+    Ja = J;
+    sJ = n2;
+  }
+  RTB_TRY(zgemm_batched(st, n, K, Ja, sJ, false, Z, n2, T1, n2));
+  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, A, n2));
+  RTB_TRY(zgemm_batched(st, n, K, V + n2 * 2, 2 * n2, false, Z, n2, T1, n2));
+  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, B, n2));
+  RTB_TRY(kubo_omega(st, n, K, A, B, E, eta, omega));
+  return RTB_OK;
+}
+
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+}
+
+}  // namespace rtb
+
+using namespace rtb;
+
+extern "C" {
+
+const char* rtb_last_error(void) { return g_error.c_str(); }
+
+int rtb_last_stats(rtb_stats* out) {
+  if (!out) return fail(RTB_ERR_INVALID_ARGUMENT, "null stats");
+  *out = g_stats;
+  return RTB_OK;
+}
+
+int rtb_model_create(int dim, int spin, int norb, int n_R, const int64_t* hamR, const double* ham,
+                     const double* orb, const double* lat, rtb_model** out) {
+  if (!out || !hamR || !ham || !orb || !lat) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (dim < 1 || dim > 3) return fail(RTB_ERR_INVALID_MODEL, "dim must be 1, 2 or 3");
+  if (norb < 1) return fail(RTB_ERR_INVALID_MODEL, "the model has no orbitals");
+  if (n_R < 1) return fail(RTB_ERR_INVALID_MODEL, "hamR must hold at least R = 0");
+  for (int d = 0; d < dim; ++d)
+    if (hamR[d] != 0) return fail(RTB_ERR_INVALID_MODEL, "hamR row 0 must be R = 0");
+  for (int i = 0; i < dim * dim; ++i)
+    if (!std::isfinite(lat[i])) return fail(RTB_ERR_INVALID_MODEL, "lattice must be finite");
+  for (int i = 0; i < norb * dim; ++i)
+    if (!std::isfinite(orb[i])) return fail(RTB_ERR_INVALID_MODEL, "orbital positions must be finite");
+  const int n = spin ? 2 * norb : norb;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  auto* m = new rtb_model();
+  m->dim = dim;
+  m->spin = spin ? 1 : 0;
+  m->norb = norb;
+  m->nsta = n;
+  m->nR = n_R;
+  m->hamR.assign(hamR, hamR + static_cast<size_t>(n_R) * dim);
+  m->orb.assign(orb, orb + static_cast<size_t>(norb) * dim);
+  m->lat.assign(lat, lat + dim * dim);
+  if (dim == 1) {
+    m->det_lat = lat[0];
+  } else if (dim == 2) {
+    m->det_lat = lat[0] * lat[3] - lat[1] * lat[2];
+  } else {
+    m->det_lat = lat[0] * (lat[4] * lat[8] - lat[5] * lat[7]) - lat[1] * (lat[3] * lat[8] - lat[5] * lat[6]) +
+                 lat[2] * (lat[3] * lat[7] - lat[4] * lat[6]);
+  }
+  auto cleanup = [&](int rc) {
+    rtb_model_destroy(m);
+    return rc;
+  };
+  if (hipGetDevice(&m->device) != hipSuccess) return cleanup(fail(RTB_ERR_DEVICE, "no HIP device"));
+  std::vector<double> R(static_cast<size_t>(n_R) * 3, 0.0), Rc(static_cast<size_t>(n_R) * 3, 0.0);
+  for (int r = 0; r < n_R; ++r)
+    for (int d = 0; d < dim; ++d) {
+      R[r * 3 + d] = static_cast<double>(hamR[r * dim + d]);
+      double s = 0.0;
+      for (int i = 0; i < dim; ++i) s += static_cast<double>(hamR[r * dim + i]) * lat[i * dim + d];
+      Rc[r * 3 + d] = s;
+    }
+  std::vector<double> tau(static_cast<size_t>(n) * 3, 0.0), tauc(static_cast<size_t>(n) * 3, 0.0);
+  for (int o = 0; o < n; ++o) {
+    int b = o % norb;
+    for (int d = 0; d < dim; ++d) {
+      tau[o * 3 + d] = orb[b * dim + d];
+      double s = 0.0;
+      for (int i = 0; i < dim; ++i) s += orb[b * dim + i] * lat[i * dim + d];
+      tauc[o * 3 + d] = s;
+    }
+  }
+  if (hipMalloc(&m->d_ham, sizeof(double) * 2 * n2 * n_R) != hipSuccess ||
+      hipMalloc(&m->d_R, sizeof(double) * R.size()) != hipSuccess ||
+      hipMalloc(&m->d_Rcart, sizeof(double) * Rc.size()) != hipSuccess ||
+      hipMalloc(&m->d_tau, sizeof(double) * tau.size()) != hipSuccess ||
+      hipMalloc(&m->d_taucart, sizeof(double) * tauc.size()) != hipSuccess)
+    return cleanup(fail(RTB_ERR_DEVICE, "device allocation for the model failed"));
+  if (hipMemcpy(m->d_ham, ham, sizeof(double) * 2 * n2 * n_R, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_R, R.data(), sizeof(double) * R.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_Rcart, Rc.data(), sizeof(double) * Rc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_tau, tau.data(), sizeof(double) * tau.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(m->d_taucart, tauc.data(), sizeof(double) * tauc.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(RTB_ERR_DEVICE, "model upload failed"));
+  *out = m;
+  return RTB_OK;
+}
+
+void rtb_model_destroy(rtb_model* m) {
+  if (!m) return;
+  (void)hipFree(m->d_ham);
+  (void)hipFree(m->d_R);
+  (void)hipFree(m->d_Rcart);
+  (void)hipFree(m->d_tau);
+  (void)hipFree(m->d_taucart);
+  delete m;
+}
+
+int rtb_model_nsta(const rtb_model* m) { return m ? m->nsta : RTB_ERR_INVALID_ARGUMENT; }
+
+int rtb_gen_ham_batch(const rtb_model* m, const double* k, size_t nk, int gauge, const double* dirs, int n_proj,
+                      double* H, double* V) {
+  RTB_TRY(prepare(m));
+  if ((nk > 0 && (!k || !H)) || (n_proj > 0 && (!dirs || !V)))
+    return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (gauge != RTB_GAUGE_ATOM && gauge != RTB_GAUGE_LATTICE) return fail(RTB_ERR_INVALID_ARGUMENT, "bad gauge");
+  if (n_proj < 0 || n_proj > 3) return fail(RTB_ERR_UNSUPPORTED, "n_proj must be 0..3");
+  if (nk == 0) return RTB_OK;
+  Stream s;
+  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
+  hipStream_t st = s.get();
+  Work w(st);
+  Timer tmr(st);
+  FourierArgs a;
+  RTB_TRY(upload_dirs(st, w, m, dirs, n_proj, a));
+  a.gauge = gauge;
+  const int n = m->nsta;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const size_t K = chunk_for(nk, (1 + n_proj) * n2 * 16 + 24);
+  double *dk, *dH, *dV = nullptr;
+  RTB_TRY(w.alloc(&dk, K * 3));
+  RTB_TRY(w.alloc(&dH, K * n2 * 2));
+  if (n_proj) RTB_TRY(w.alloc(&dV, K * n_proj * n2 * 2));
+  std::vector<double> kp;
+  for (size_t k0 = 0; k0 < nk; k0 += K) {
+    const size_t c = std::min(K, nk - k0);
+    kp.assign(c * 3, 0.0);
+    for (size_t i = 0; i < c; ++i)
+      for (int d = 0; d < m->dim; ++d) kp[i * 3 + d] = k[(k0 + i) * m->dim + d];
+    RTB_HIP(hipMemcpyAsync(dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    RTB_TRY(tmr.begin(0));
+    RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
+    RTB_TRY(tmr.end(1));
+    RTB_HIP(hipMemcpyAsync(H + k0 * n2 * 2, dH, c * n2 * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (n_proj)
+      RTB_HIP(hipMemcpyAsync(V + k0 * n_proj * n2 * 2, dV, c * n_proj * n2 * 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, st));
+    RTB_HIP(hipStreamSynchronize(st));
+  }
+  RTB_TRY(tmr.collect(nk, n));
+  return RTB_OK;
+}
+
+static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precision, double* E, double* evec) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  if (nk > 0 && (!k || !E)) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (nk == 0) return RTB_OK;
+  const int n = m->nsta;
+  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  const bool vec = evec != nullptr;
+  Stream s;
+  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
+  hipStream_t st = s.get();
+  Work w(st);
+  Timer tmr(st);
+  FourierArgs a;
+  RTB_TRY(upload_dirs(st, w, m, nullptr, 0, a));
+  a.gauge = RTB_GAUGE_ATOM;  // solve_band_onek / solve_onek use Gauge::Atom
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const size_t per_k = 24 + n2 * 16 * 2 + n * 24 + (vec ? n2 * 32 + n2 * 16 : 0);
+  const size_t K = chunk_for(nk, per_k);
+  double *dk, *dH, *dE, *dZ = nullptr;
+  EigBufs eb;
+  RTB_TRY(w.alloc(&dk, K * 3));
+  RTB_TRY(w.alloc(&dH, K * n2 * 2));
+  RTB_TRY(w.alloc(&dE, K * n));
+  RTB_TRY(alloc_eig(w, n, K, vec, eb));
+  if (vec) RTB_TRY(w.alloc(&dZ, K * n2 * 2));
+  std::vector<double> kp;
+  for (size_t k0 = 0; k0 < nk; k0 += K) {
+    const size_t c = std::min(K, nk - k0);
+    kp.assign(c * 3, 0.0);
+    for (size_t i = 0; i < c; ++i)
+      for (int d = 0; d < m->dim; ++d) kp[i * 3 + d] = k[(k0 + i) * m->dim + d];
+    RTB_HIP(hipMemcpyAsync(dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    RTB_TRY(tmr.begin(0));
+    RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(tmr.begin(1));
+    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S));
+    if (vec) RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, true));
+    RTB_TRY(tmr.end(vec ? 2 : 1));
+    RTB_HIP(hipMemcpyAsync(E + k0 * n, dE, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (vec)
+      RTB_HIP(hipMemcpyAsync(evec + k0 * n2 * 2, dZ, c * n2 * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    RTB_HIP(hipStreamSynchronize(st));
+  }
+  RTB_TRY(tmr.collect(nk, n));
+  return RTB_OK;
+}
+
+int rtb_solve_band_all(const rtb_model* m, const double* k, size_t nk, int precision, double* E) {
+  return solve_impl(m, k, nk, precision, E, nullptr);
+}
+
+int rtb_solve_all(const rtb_model* m, const double* k, size_t nk, int precision, double* E, double* evec) {
+  if (nk > 0 && !evec) return fail(RTB_ERR_INVALID_ARGUMENT, "null evec");
+  return solve_impl(m, k, nk, precision, E, evec);
+}
+
+// Shared Berry-curvature pipeline over either a k list (k != null) or a mesh.
+//   mode 0: omega + E to host; mode 1: occupied sum per k to host;
+//   mode 2: Hall partial sums over mu
+struct BerryJob {
+  const double* k = nullptr;        // host k list, or null for the mesh
+  const size_t* kmesh = nullptr;
+  size_t nk = 0;
+  const double* dir = nullptr;
+  int spin_dir = RTB_SPIN_NONE;
+  double eta = 1e-3;
+  int mode = 0;
+  double* omega = nullptr;
+  double* E = nullptr;
+  double mu0 = 0.0;
+  double T = 0.0;
+  double* occ = nullptr;
+  const double* mu = nullptr;
+  size_t n_mu = 0;
+  double* sigma = nullptr;
+};
+
+static int berry_impl(const rtb_model* m, BerryJob& j) {
+  const int n = m->nsta;
+  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  Stream s;
+  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
+  hipStream_t st = s.get();
+  Work w(st);
+  Timer tmr(st);
+  FourierArgs a;
+  RTB_TRY(upload_dirs(st, w, m, j.dir, 2, a));
+  a.gauge = RTB_GAUGE_ATOM;  // berry_curvature_at_impl: gen_v_projected(k, Gauge::Atom, ..)
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const bool spinj = m->spin && j.spin_dir != RTB_SPIN_NONE;
+  // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
+  const size_t per_k = 24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 48;
+  const size_t K = chunk_for(j.nk, per_k);
+  double *dk, *dH, *dV, *dE, *dOm, *dJ = nullptr, *dZ;
+  EigBufs eb;
+  RTB_TRY(w.alloc(&dk, K * 3));
+  RTB_TRY(w.alloc(&dH, K * n2 * 2));
+  RTB_TRY(w.alloc(&dV, K * n2 * 4));
+  RTB_TRY(w.alloc(&dE, K * n));
+  RTB_TRY(w.alloc(&dOm, K * n));
+  RTB_TRY(alloc_eig(w, n, K, true, eb));
+  double* kub;
+  RTB_TRY(w.alloc(&kub, K * n2 * 6));  // T1, A, B (complex)
+  if (spinj) RTB_TRY(w.alloc(&dJ, K * n2 * 2));
+  dZ = dH;  // H is consumed by stage 1 before Z is written
+  double* T1 = kub;
+  double* A = kub + K * n2 * 2;
+  double* B = kub + K * n2 * 4;
+  double *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr;
+  int total_blk = 0;
+  if (j.mode == 2) {
+    for (size_t k0 = 0; k0 < j.nk; k0 += K) total_blk += hall_blocks(std::min(K, j.nk - k0));
+    RTB_TRY(w.alloc(&dmu, j.n_mu));
+    RTB_TRY(w.alloc(&dpart, static_cast<size_t>(total_blk) * j.n_mu));
+    RTB_TRY(w.alloc(&dsig, j.n_mu));
+    RTB_HIP(hipMemcpyAsync(dmu, j.mu, j.n_mu * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+  if (j.mode == 1) RTB_TRY(w.alloc(&docc, K));
+  std::vector<double> kp;
+  int blk_off = 0;
+  for (size_t k0 = 0; k0 < j.nk; k0 += K) {
+    const size_t c = std::min(K, j.nk - k0);
+    if (j.k) {
+      kp.assign(c * 3, 0.0);
+      for (size_t i = 0; i < c; ++i)
+        for (int d = 0; d < m->dim; ++d) kp[i * 3 + d] = j.k[(k0 + i) * m->dim + d];
+      RTB_HIP(hipMemcpyAsync(dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    RTB_TRY(tmr.begin(0));
+    int l0 = 1;
+    if (!j.k) {
+      RTB_TRY(mesh_points(st, m->dim, j.kmesh, k0, c, dk));
+      ++l0;
+    }
+    RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
+    RTB_TRY(tmr.end(l0));
+    RTB_TRY(tmr.begin(1));
+    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S));
+    RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, false));
+    RTB_TRY(tmr.end(2));
+    RTB_TRY(tmr.begin(2));
+    RTB_TRY(kubo_chunk(st, m, c, dV, dZ, dE, j.spin_dir, j.eta, T1, A, B, dJ, dOm));
+    int l2 = 5 + (spinj ? 1 : 0);
+    if (j.mode == 0) {
+      RTB_HIP(hipMemcpyAsync(j.omega + k0 * n, dOm, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
+      RTB_HIP(hipMemcpyAsync(j.E + k0 * n, dE, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
+    } else if (j.mode == 1) {
+      RTB_TRY(occupied_sum(st, n, c, dOm, dE, j.mu0, j.T, docc));
+      ++l2;
+      RTB_HIP(hipMemcpyAsync(j.occ + k0, docc, c * sizeof(double), hipMemcpyDeviceToHost, st));
+    } else {
+      const int nb = hall_blocks(c);
+      RTB_TRY(hall_partials(st, n, c, dOm, dE, dmu, static_cast<int>(j.n_mu), j.T,
+                            dpart + static_cast<size_t>(blk_off) * j.n_mu, nb));
+      ++l2;
+      blk_off += nb;
+    }
+    RTB_TRY(tmr.end(l2));
+    if (j.mode != 2) RTB_HIP(hipStreamSynchronize(st));
+  }
+  if (j.mode == 2) {
+    RTB_TRY(tmr.begin(2));
+    const double scale = 1.0 / static_cast<double>(j.nk) / m->det_lat;
+    RTB_TRY(reduce_partials(st, dpart, total_blk, static_cast<int>(j.n_mu), 1.0, dsig));
+    RTB_TRY(tmr.end(1));
+    std::vector<double> sums(j.n_mu);
+    RTB_HIP(hipMemcpyAsync(sums.data(), dsig, j.n_mu * sizeof(double), hipMemcpyDeviceToHost, st));
+    RTB_HIP(hipStreamSynchronize(st));
+    // sum / nk / det (response/linear/mod.rs:221)
+    for (size_t i = 0; i < j.n_mu; ++i) j.sigma[i] = sums[i] / static_cast<double>(j.nk) / m->det_lat;
+    (void)scale;
+  }
+  RTB_HIP(hipStreamSynchronize(st));
+  RTB_TRY(tmr.collect(j.nk, n));
+  return RTB_OK;
+}
+
+int rtb_berry_curvature(const rtb_model* m, const double* k, size_t nk, const double* dir, int spin_dir, double eta,
+                        int precision, double* omega, double* E) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  // berry_curvature_at (traits.rs:65-77): direction, broadening, spin
+  RTB_TRY(check_direction(m, dir));
+  RTB_TRY(check_eta(eta));
+  RTB_TRY(check_spin(m, spin_dir));
+  if (nk > 0 && (!k || !omega || !E)) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (nk == 0) return RTB_OK;
+  BerryJob j;
+  j.k = k;
+  j.nk = nk;
+  j.dir = dir;
+  j.spin_dir = spin_dir;
+  j.eta = eta;
+  j.mode = 0;
+  j.omega = omega;
+  j.E = E;
+  return berry_impl(m, j);
+}
+
+int rtb_occupied_berry_curvature(const rtb_model* m, const double* k, size_t nk, const double* dir, int spin_dir,
+                                 double eta, double mu, double T, int precision, double* out) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  // occupied_berry_curvature_on (traits.rs:109-129)
+  RTB_TRY(check_mu(&mu, 1));
+  RTB_TRY(check_T(T));
+  RTB_TRY(check_direction(m, dir));
+  RTB_TRY(check_eta(eta));
+  RTB_TRY(check_spin(m, spin_dir));
+  if (nk > 0 && (!k || !out)) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (nk == 0) return RTB_OK;
+  BerryJob j;
+  j.k = k;
+  j.nk = nk;
+  j.dir = dir;
+  j.spin_dir = spin_dir;
+  j.eta = eta;
+  j.mode = 1;
+  j.mu0 = mu;
+  j.T = T;
+  j.occ = out;
+  return berry_impl(m, j);
+}
+
+int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T, double eta,
+                    const double* dir, int spin_dir, int precision, double* sigma) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  // validate_rank2 (config.rs:141-147) then the spin check (linear/mod.rs:162-165)
+  size_t nk = 0;
+  RTB_TRY(check_kmesh(m, kmesh, &nk));
+  RTB_TRY(check_direction(m, dir));
+  RTB_TRY(check_mu(mu, n_mu));
+  RTB_TRY(check_eta(eta));
+  RTB_TRY(check_T(T));
+  RTB_TRY(check_spin(m, spin_dir));
+  if (!sigma) return fail(RTB_ERR_INVALID_ARGUMENT, "null sigma");
+  BerryJob j;
+  j.kmesh = kmesh;
+  j.nk = nk;
+  j.dir = dir;
+  j.spin_dir = spin_dir;
+  j.eta = eta;
+  j.mode = 2;
+  j.T = T;
+  j.mu = mu;
+  j.n_mu = n_mu;
+  j.sigma = sigma;
+  return berry_impl(m, j);
+}
+
+int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, size_t En, double sigma,
+            int precision, double* E, double* rho) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  // model_physics.rs:187-205
+  if (Emin >= Emax) return fail(RTB_ERR_INVALID_ENERGY_RANGE, "E_min must be below E_max");
+  if (En == 0) return fail(RTB_ERR_INVALID_DOS_PARAMETER, "E_n: number of energy bins must be at least 1");
+  if (!std::isfinite(sigma) || sigma <= 0.0)
+    return fail(RTB_ERR_INVALID_DOS_PARAMETER, "sigma: Gaussian smearing width must be finite and positive");
+  size_t nk = 0;
+  RTB_TRY(check_kmesh(m, kmesh, &nk));
+  if (!E || !rho) return fail(RTB_ERR_INVALID_ARGUMENT, "null output");
+  const int n = m->nsta;
+  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  const double step = En > 1 ? (Emax - Emin) / static_cast<double>(En - 1) : 0.0;
+  for (size_t i = 0; i < En; ++i) E[i] = Emin + step * static_cast<double>(i);
+  Stream s;
+  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
+  hipStream_t st = s.get();
+  Work w(st);
+  Timer tmr(st);
+  FourierArgs a;
+  RTB_TRY(upload_dirs(st, w, m, nullptr, 0, a));
+  a.gauge = RTB_GAUGE_ATOM;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const size_t K = chunk_for(nk, 24 + n2 * 32 + n * 24);
+  double *dk, *dH, *dE;
+  EigBufs eb;
+  RTB_TRY(w.alloc(&dk, K * 3));
+  RTB_TRY(w.alloc(&dH, K * n2 * 2));
+  RTB_TRY(w.alloc(&dE, K * n));
+  RTB_TRY(alloc_eig(w, n, K, false, eb));
+  int total_blk = 0;
+  for (size_t k0 = 0; k0 < nk; k0 += K) total_blk += dos_blocks(std::min(K, nk - k0) * n);
+  double *dpart, *drho;
+  RTB_TRY(w.alloc(&dpart, static_cast<size_t>(total_blk) * En));
+  RTB_TRY(w.alloc(&drho, En));
+  int blk_off = 0;
+  for (size_t k0 = 0; k0 < nk; k0 += K) {
+    const size_t c = std::min(K, nk - k0);
+    RTB_TRY(tmr.begin(0));
+    RTB_TRY(mesh_points(st, m->dim, kmesh, k0, c, dk));
+    RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
+    RTB_TRY(tmr.end(2));
+    RTB_TRY(tmr.begin(1));
+    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(tmr.begin(3));
+    const int nb = dos_blocks(c * n);
+    RTB_TRY(dos_partials(st, c * n, dE, Emin, step, static_cast<int>(En), sigma,
+                         dpart + static_cast<size_t>(blk_off) * En, nb));
+    RTB_TRY(tmr.end(1));
+    blk_off += nb;
+  }
+  RTB_TRY(tmr.begin(3));
+  RTB_TRY(reduce_partials(st, dpart, total_blk, static_cast<int>(En), 1.0, drho));
+  RTB_TRY(tmr.end(1));
+  RTB_HIP(hipMemcpyAsync(rho, drho, En * sizeof(double), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipStreamSynchronize(st));
+  for (size_t i = 0; i < En; ++i) rho[i] /= static_cast<double>(nk);  // model_physics.rs:224
+  RTB_TRY(tmr.collect(nk, n));
+  return RTB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// Launch wrappers for the hot-path kernels (one header for all units).
+#pragma once
+
+#include "rtb_internal.hpp"
+
+namespace rtb {
+
+// ---- eigensolver (eigh.hip) ----
+size_t eigh_stage1_smem(int n, int real_size);
+int eigh_set_smem_limits();
+// Householder + bisection (+ inverse iteration when `vectors`).
+//   H: [nmat][n][n] complex; E: [nmat][n]; Vg: [nmat][n][n] complex;
+//   taug: [nmat][n] complex; Y: [nmat][n][n]; S: [nmat][3][n][n]
+int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors,
+                     double* Vg, double* taug, double* Y, double* S);
+// Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
+int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* Vg,
+                            const double* taug, double* Z, bool rows_out);
+
+// ---- Fourier sum (fourier.hip) ----
+struct FourierArgs {
+  int dim, n, nR, np;     // np: number of projected velocity directions (0..3)
+  int gauge;              // RTB_GAUGE_*
+  const double* ham;      // [nR][n][n] complex
+  const double* R;        // [nR][3]
+  const double* wR;       // [np][nR]  w_p . R_cart
+  const double* tau;      // [n][3]    fractional
+  const double* wtau;     // [np][n]   w_p . tau_cart
+};
+// k: [nk][3] (zero padded); H: [nk][n][n]; V: [nk][np][n][n] (or null when np == 0)
+int fourier_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t nk, double* H,
+                  double* V);
+// Uniform mesh points k = (i/N1, j/N2, l/N3) for flat indices [k0, k0 + nk).
+int mesh_points(hipStream_t st, int dim, const size_t* kmesh, size_t k0, size_t nk, double* k);
+
+// ---- Kubo band sum, reductions, DOS (kubo.hip) ----
+// C[b] = op(A[b]) * B[b], n x n complex; conj_a: op = conjugate transpose.
+// sA, sB, sC: batch strides in complex elements.
+int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
+                  size_t sB, double* C, size_t sC);
+// J = (S v + v S) / 2 for the spin matrix S = sigma/2 (x) I (helpers.rs:11-39).
+int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride,
+                 double* J);
+// Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2)
+int kubo_omega(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E,
+               double eta, double* omega);
+// out[k] = sum_n omega[k][n] f(E[k][n], mu, T)
+int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const double* E, double mu,
+                 double T, double* out);
+// partial[blk][imu] += sum_{k in blk} sum_n omega f(E, mu_imu)   (fixed blocking)
+int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const double* E,
+                  const double* mu, int n_mu, double T, double* partial, int n_blk);
+int hall_blocks(size_t nk);
+// DOS partial sums: partial[blk][j] += sum_{eps in blk} g(E_j - eps)
+int dos_partials(hipStream_t st, size_t ne, const double* evals, double Emin, double dE, int En,
+                 double sigma, double* partial, int n_blk);
+int dos_blocks(size_t ne);
+// out[j] = scale * sum_blk partial[blk][j]
+int reduce_partials(hipStream_t st, const double* partial, int n_blk, int m, double scale, double* out);
+
+}  // namespace rtb

@@ -1,0 +1,303 @@
+// Kubo band sum (response/traits.rs:160-200), occupation reductions
+// (thermodynamics.rs:86-118, response/linear/mod.rs:204-221) and the
+// Gaussian DOS (model_physics.rs:179-226), FP64.
+#include "eigh.hpp"
+
+namespace rtb {
+
+namespace {
+
+constexpr int kT = 64;   // output tile (kT x kT) per workgroup
+constexpr int kK = 16;   // k-tile staged in LDS
+
+// C = op(A) B for one n x n complex matrix per blockIdx.z; 4x4 outputs per thread.
+__global__ void __launch_bounds__(256) zgemm_kernel(int n, const double2* __restrict__ A, size_t sA, int conj_a,
+                                                    const double2* __restrict__ B, size_t sB, double2* __restrict__ C,
+                                                    size_t sC) {
+  __shared__ double2 As[kK][kT + 1];
+  __shared__ double2 Bs[kK][kT + 1];
+  A += static_cast<size_t>(blockIdx.z) * sA;
+  B += static_cast<size_t>(blockIdx.z) * sB;
+  C += static_cast<size_t>(blockIdx.z) * sC;
+  const int row0 = blockIdx.y * kT, col0 = blockIdx.x * kT;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double cr[4][4], ci[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      cr[i][j] = 0.0;
+      ci[i][j] = 0.0;
+    }
+  for (int k0 = 0; k0 < n; k0 += kK) {
+    // As[kk][i] = op(A)[row0+i][k0+kk]
+    for (int q = threadIdx.x; q < kK * kT; q += 256) {
+      int i, kk;
+      double2 v = make_double2(0.0, 0.0);
+      if (conj_a) {  // op(A)[r][k] = conj(A[k][r]); read along r
+        kk = q / kT;
+        i = q % kT;
+        int r = row0 + i, kc = k0 + kk;
+        if (r < n && kc < n) {
+          v = A[static_cast<size_t>(kc) * n + r];
+          v.y = -v.y;
+        }
+      } else {  // read along k
+        i = q / kK;
+        kk = q % kK;
+        int r = row0 + i, kc = k0 + kk;
+        if (r < n && kc < n) v = A[static_cast<size_t>(r) * n + kc];
+      }
+      As[kk][i] = v;
+    }
+    for (int q = threadIdx.x; q < kK * kT; q += 256) {
+      int kk = q / kT, j = q % kT;
+      int kr = k0 + kk, c = col0 + j;
+      Bs[kk][j] = (kr < n && c < n) ? B[static_cast<size_t>(kr) * n + c] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < kK; ++kk) {
+      double2 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          cr[i][j] += a[i].x * b[j].x - a[i].y * b[j].y;
+          ci[i][j] += a[i].x * b[j].y + a[i].y * b[j].x;
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    int r = row0 + ty + 16 * i;
+    if (r >= n) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int c = col0 + tx + 16 * j;
+      if (c < n) C[static_cast<size_t>(r) * n + c] = make_double2(cr[i][j], ci[i][j]);
+    }
+  }
+}
+
+// helpers.rs:11-39 spin matrix S = sigma/2 (x) I_norb; J = (S v + v S) / 2
+__global__ void spin_current_kernel(int norb, size_t nb, int dir, const double2* __restrict__ v,
+                                    size_t v_stride, double2* __restrict__ J) {
+  const int n = 2 * norb;
+  const size_t total = nb * n * n;
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= total) return;
+  const size_t b = g / (static_cast<size_t>(n) * n);
+  const int e = static_cast<int>(g % (static_cast<size_t>(n) * n));
+  const int i = e / n, j = e % n;
+  const double2* vb = v + b * v_stride;
+  auto at = [&](int r, int c) { return vb[static_cast<size_t>(r) * n + c]; };
+  double2 sv = make_double2(0.0, 0.0), vs = make_double2(0.0, 0.0);
+  const int ip = i < norb ? i + norb : i - norb;  // spin partner of row i
+  const int jp = j < norb ? j + norb : j - norb;
+  if (dir == RTB_SPIN_Z) {
+    double si = i < norb ? 0.5 : -0.5, sj = j < norb ? 0.5 : -0.5;
+    double2 x = at(i, j);
+    sv = make_double2(si * x.x, si * x.y);
+    vs = make_double2(sj * x.x, sj * x.y);
+  } else if (dir == RTB_SPIN_X) {
+    double2 x = at(ip, j), y = at(i, jp);
+    sv = make_double2(0.5 * x.x, 0.5 * x.y);
+    vs = make_double2(0.5 * y.x, 0.5 * y.y);
+  } else {  // Y: S[i][i+norb] = -i/2, S[i+norb][i] = +i/2
+    double2 x = at(ip, j);
+    double f = i < norb ? -0.5 : 0.5;  // (S v)_ij = f * i * v_{ip j}
+    sv = make_double2(-f * x.y, f * x.x);
+    double2 y = at(i, jp);
+    double h = j < norb ? 0.5 : -0.5;  // (v S)_ij = v_{i jp} S_{jp j}; S_{j+norb, j} = +i/2
+    vs = make_double2(-h * y.y, h * y.x);
+  }
+  J[g] = make_double2(0.5 * (sv.x + vs.x), 0.5 * (sv.y + vs.y));
+}
+
+__global__ void kubo_omega_kernel(int n, size_t nk, const double2* __restrict__ A, const double2* __restrict__ B,
+                                  const double* __restrict__ E, double eta2, double* __restrict__ omega) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= nk * n) return;
+  const size_t k = g / n;
+  const int b = static_cast<int>(g % n);
+  const double2* Ak = A + k * n * n;
+  const double2* Bk = B + k * n * n;
+  const double* Ek = E + k * n;
+  double acc = 0.0;
+  for (int m = 0; m < n; ++m) {
+    if (m == b) continue;
+    const double2 a = Ak[static_cast<size_t>(b) * n + m];
+    const double2 c = Bk[static_cast<size_t>(m) * n + b];
+    const double im = a.x * c.y + a.y * c.x;  // Im(A_bm B_mb)
+    const double dE = Ek[b] - Ek[m];
+    acc += -2.0 * im / (dE * dE + eta2);
+  }
+  omega[g] = acc;
+}
+
+__device__ inline double fermi(double e, double mu, double width) {
+  // fermi_from_width (thermodynamics.rs:100-118)
+  if (width == 0.0) return e < mu ? 1.0 : (e > mu ? 0.0 : 0.5);
+  const double x = (e - mu) / width;
+  if (x > 50.0) return 0.0;
+  if (x < -50.0) return 1.0;
+  return 1.0 / (1.0 + exp(x));
+}
+
+__device__ inline double width_of(double T) {
+  return T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10
+}
+
+__global__ void occupied_sum_kernel(int n, size_t nk, const double* __restrict__ omega,
+                                    const double* __restrict__ E, double mu, double width,
+                                    double* __restrict__ out) {
+  const size_t k = blockIdx.x * 256ull + threadIdx.x;
+  if (k >= nk) return;
+  double s = 0.0;
+  for (int b = 0; b < n; ++b) s += omega[k * n + b] * fermi(E[k * n + b], mu, width);
+  out[k] = s;
+}
+
+constexpr int kHallK = 256;  // k-points per hall partial block
+
+__global__ void hall_partials_kernel(int n, size_t nk, const double* __restrict__ omega,
+                                     const double* __restrict__ E, const double* __restrict__ mu,
+                                     int n_mu, double width, double* __restrict__ partial) {
+  // block = (k-block, mu); 256 threads over k, then a fixed tree
+  __shared__ double red[256];
+  const int imu = blockIdx.y;
+  const size_t k = blockIdx.x * static_cast<size_t>(kHallK) + threadIdx.x;
+  double s = 0.0;
+  if (k < nk) {
+    const double m = mu[imu];
+    for (int b = 0; b < n; ++b) s += omega[k * n + b] * fermi(E[k * n + b], m, width);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[static_cast<size_t>(blockIdx.x) * n_mu + imu] = red[0];
+}
+
+constexpr int kDosE = 256;   // energies per block
+constexpr int kDosEig = 2048; // eigenvalues per block
+
+__global__ void dos_partials_kernel(size_t ne, const double* __restrict__ ev, double Emin, double dE, int En,
+                                    double sigma0, double pi0, double* __restrict__ partial) {
+  __shared__ double sev[kDosEig];
+  const int j = blockIdx.x * kDosE + threadIdx.x;
+  const size_t e0 = static_cast<size_t>(blockIdx.y) * kDosEig;
+  const int cnt = static_cast<int>(ne - e0 < static_cast<size_t>(kDosEig) ? ne - e0 : kDosEig);
+  for (int q = threadIdx.x; q < cnt; q += kDosE) sev[q] = ev[e0 + q];
+  __syncthreads();
+  if (j >= En) return;
+  // linspace: a + step * i (ndarray::Array1::linspace)
+  const double Ej = Emin + dE * j;
+  double acc = 0.0;
+  for (int q = 0; q < cnt; ++q) {
+    const double A = (Ej - sev[q]) * sigma0;
+    acc += exp(-A * A / 2.0) * sigma0 * pi0;
+  }
+  partial[static_cast<size_t>(blockIdx.y) * En + j] = acc;
+}
+
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int n_blk, int m, double scale,
+                                       double* __restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= m) return;
+  double s = 0.0;
+  for (int b = 0; b < n_blk; ++b) s += partial[static_cast<size_t>(b) * m + j];
+  out[j] = s * scale;
+}
+
+}  // namespace
+
+int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
+                  size_t sB, double* C, size_t sC) {
+  if (nb == 0) return RTB_OK;
+  const unsigned t = static_cast<unsigned>((n + kT - 1) / kT);
+  for (size_t off = 0; off < nb; off += 65535) {
+    size_t cnt = nb - off < 65535 ? nb - off : 65535;
+    hipLaunchKernelGGL(zgemm_kernel, dim3(t, t, static_cast<unsigned>(cnt)), dim3(256), 0, st, n,
+                       reinterpret_cast<const double2*>(A) + off * sA, sA, conj_a ? 1 : 0,
+                       reinterpret_cast<const double2*>(B) + off * sB, sB,
+                       reinterpret_cast<double2*>(C) + off * sC, sC);
+  }
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride, double* J) {
+  const size_t n = 2 * static_cast<size_t>(norb);
+  const size_t total = nb * n * n;
+  if (total == 0) return RTB_OK;
+  hipLaunchKernelGGL(spin_current_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st,
+                     norb, nb, spin_dir, reinterpret_cast<const double2*>(v), v_stride / 2,
+                     reinterpret_cast<double2*>(J));
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int kubo_omega(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E, double eta,
+               double* omega) {
+  const size_t total = nk * n;
+  if (total == 0) return RTB_OK;
+  hipLaunchKernelGGL(kubo_omega_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, n,
+                     nk, reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B), E, eta * eta,
+                     omega);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const double* E, double mu, double T,
+                 double* out) {
+  if (nk == 0) return RTB_OK;
+  hipLaunchKernelGGL(occupied_sum_kernel, dim3(static_cast<unsigned>((nk + 255) / 256)), dim3(256), 0, st, n, nk,
+                     omega, E, mu, T <= 0.0 ? 0.0 : T * 8.617333262145e-5, out);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int hall_blocks(size_t nk) { return static_cast<int>((nk + kHallK - 1) / kHallK); }
+
+int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const double* E, const double* mu,
+                  int n_mu, double T, double* partial, int n_blk) {
+  if (nk == 0) return RTB_OK;
+  hipLaunchKernelGGL(hall_partials_kernel, dim3(static_cast<unsigned>(n_blk), static_cast<unsigned>(n_mu)),
+                     dim3(256), 0, st, n, nk, omega, E, mu, n_mu, T <= 0.0 ? 0.0 : T * 8.617333262145e-5,
+                     partial);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int dos_blocks(size_t ne) { return static_cast<int>((ne + kDosEig - 1) / kDosEig); }
+
+int dos_partials(hipStream_t st, size_t ne, const double* evals, double Emin, double dE, int En, double sigma,
+                 double* partial, int n_blk) {
+  if (ne == 0) return RTB_OK;
+  const double sigma0 = 1.0 / sigma;
+  const double pi0 = 1.0 / sqrt(2.0 * M_PI);
+  hipLaunchKernelGGL(dos_partials_kernel, dim3(static_cast<unsigned>((En + kDosE - 1) / kDosE),
+                                               static_cast<unsigned>(n_blk)),
+                     dim3(kDosE), 0, st, ne, evals, Emin, dE, En, sigma0, pi0, partial);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int reduce_partials(hipStream_t st, const double* partial, int n_blk, int m, double scale, double* out) {
+  if (m == 0) return RTB_OK;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(static_cast<unsigned>((m + 255) / 256)), dim3(256), 0, st,
+                     partial, n_blk, m, scale, out);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+}  // namespace rtb

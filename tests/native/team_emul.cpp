@@ -1,0 +1,70 @@
+// CPU emulation of one eigensolver workgroup: the team-generic device source
+// (rustb_amd/csrc/eigh_core.hpp) run by `nt` host threads with a barrier.
+// Test-only: lets the indexing and numerics be checked on the host (with
+// AddressSanitizer) for any team size.
+#include <barrier>
+#include <thread>
+#include <vector>
+
+#include "../../rustb_amd/csrc/eigh_core.hpp"
+
+namespace {
+
+struct CpuTeam {
+  int id, nt;
+  std::barrier<>* bar;
+  void sync() { bar->arrive_and_wait(); }
+  template <typename T>
+  T reduce_sum(T x, T* red) {
+    red[id] = x;
+    sync();
+    T s = T(0);
+    for (int i = 0; i < nt; ++i) s += red[i];
+    sync();
+    return s;
+  }
+};
+
+template <typename T>
+int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
+  using namespace rtb;
+  std::vector<cplx<T>> P(static_cast<size_t>(n) * (n + 1) / 2), v(n), w(n);
+  std::vector<T> d(n), e(n), e2(n), Ev(n), red(nt + 64);
+  std::vector<cplx<T>> Vg(static_cast<size_t>(n) * n), taug(n);
+  std::vector<T> Y(static_cast<size_t>(n) * n), S(static_cast<size_t>(n) * n * 3);
+  std::vector<cplx<T>> Zb(static_cast<size_t>(n) * n), vs(n), sv(n);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c <= r; ++c) {
+      const double* h = H + (static_cast<size_t>(r) * n + c) * 2;
+      P[pidx(r, c)] = {static_cast<T>(h[0]), static_cast<T>(h[1])};
+    }
+  std::barrier<> bar(nt);
+  auto body = [&](int id) {
+    CpuTeam tm{id, nt, &bar};
+    tridiagonalize<T>(tm, n, P.data(), v.data(), w.data(), d.data(), e.data(), red.data(), Vg.data(),
+                      taug.data());
+    bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data());
+    if (vectors) {
+      tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, S.data());
+      for (int f = id; f < n * n; f += nt) Zb[f] = {Y[f], T(0)};
+      tm.sync();
+      apply_q<T>(tm, n, n, Zb.data(), Vg.data(), taug.data(), vs.data(), sv.data());
+    }
+  };
+  std::vector<std::thread> th;
+  for (int i = 0; i < nt; ++i) th.emplace_back(body, i);
+  for (auto& t : th) t.join();
+  for (int j = 0; j < n; ++j) E[j] = static_cast<double>(Ev[j]);
+  if (vectors)
+    for (int f = 0; f < n * n; ++f) {
+      Z[2 * f] = static_cast<double>(Zb[f].re);
+      Z[2 * f + 1] = static_cast<double>(Zb[f].im);
+    }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int emul_eigh(int n, int nt, int fp32, const double* H, double* E, double* Z, int vectors) {
+  return fp32 ? run<float>(n, nt, H, E, Z, vectors) : run<double>(n, nt, H, E, Z, vectors);
+}

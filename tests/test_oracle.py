@@ -1,0 +1,244 @@
+"""Pins the CPU oracle against the reference's own numeric tests, its committed
+band fixture, closed forms and invariants (CPU only)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import refmodels as RM
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def make(lat, orb, spin):
+    return O.Model(lat, orb, spin)
+
+
+# ---------------------------------------------------------------- test_gen_v
+def test_gen_v_pins():
+    """lib.rs:671-755 — exact Gamma eigenvalues, Dirac point, tabulated v."""
+    m = RM.gen_v_model(make)
+    assert np.array_equal(O.solve_band_onek(m, [0.0, 0.0]), np.array([-3.0, 3.0]))
+    assert np.allclose(O.solve_band_onek(m, [1 / 3, 2 / 3]), [0.0, 0.0], atol=1e-5)
+    v, _ = O.gen_v(m, [1 / 3, 1 / 3], "atom")
+    vy = np.array([[0, -0.4698463103929542 - 0.17101007166283436j],
+                   [-0.4698463103929542 + 0.17101007166283436j, 0]])
+    vx = np.array([[0, -0.8137976813493737 - 0.2961981327260237j],
+                   [-0.8137976813493737 + 0.2961981327260237j, 0]])
+    assert np.abs(v[0] - vx).max() < 1e-8
+    assert np.abs(v[1] - vy).max() < 1e-8
+    v, _ = O.gen_v(m, [1 / 3, 1 / 3], "lattice")
+    t = 1.0
+    vx_l = np.array([[0, -3 * math.sqrt(3) / 4 * t + 0.75j * t],
+                     [-3 * math.sqrt(3) / 4 * t - 0.75j * t, 0]])
+    assert np.abs(v[0] - vx_l).max() < 1e-8
+    # lib.rs:743-755: conj(evec) H evec^T is diagonal with the bands
+    k = [1 / 3, 1 / 3]
+    band, evec = O.solve_onek(m, k)
+    H = O.gen_ham(m, k)
+    new_band = np.diag(evec.conj() @ H @ evec.T).real
+    assert np.allclose(new_band, band, atol=1e-5)
+
+
+def test_evec_transform_sanity():
+    """lib.rs:1231-1277 — E_vec^T H E_vec^* = diag(E) with E_vec = conj(U)."""
+    m = RM.haldane(make)
+    H = O.gen_ham(m, [0.3, 0.4])
+    E, U = O.eigh(H)
+    evec = U.conj()  # what ndarray-linalg's eigh hands back
+    D = evec.T @ H @ evec.conj()
+    assert np.abs(np.diag(D).real - E).max() < 1e-10
+    assert np.abs(D - np.diag(np.diag(D))).max() < 1e-10
+
+
+# ------------------------------------------------------ conductivity_test
+def _wilson_phase(m, klist, band):
+    """Single-band Wilson loop of geometry.rs:156-209 (1x1 SVD = phase)."""
+    _, vecs = O.solve_all(m, klist)
+    prod = 1.0 + 0j
+    for j in range(len(klist) - 1):
+        prod *= np.vdot(vecs[j, band], vecs[j + 1, band])
+    return -np.angle(prod)
+
+
+def test_kubo_vs_wilson_plaquette():
+    """lib.rs:757-823: 2 pi sum_occ Omega vs a small Wilson loop, 1e-4."""
+    m = RM.haldane(make)
+    k = np.array([1 / 3, 2 / 3])
+    d1, d2 = np.array([1.0, 0.0]), np.array([0.0, 1.0])
+    om, E = O.berry_curvature_at(m, k, [d1, d2], 1e-3)
+    result1 = om[E <= 0.0].sum() * 2 * math.pi
+    dk = 1e-4
+    steps = [d1, d1 + d2, d2, -d1 + d2, -d1, -d1 - d2, -d2, d1 - d2, d1]
+    klist = np.array([k + dk * s for s in steps])
+    result2 = _wilson_phase(m, klist, 0) / dk**2 / 4 / (2 * math.pi) * math.sqrt(3) / 2
+    assert abs(result2 - result1) < 1e-4
+
+
+def test_hall_single_vs_multi_mu():
+    """lib.rs:825-845 (a 40x40 mesh keeps the CPU suite short)."""
+    m = RM.haldane(make)
+    a1 = O.hall_conductivity_direct(m, [40, 40], [[0, 1], [1, 0]], [-1.0], 0.0, 1e-3)
+    a2 = O.hall_conductivity_direct(m, [40, 40], [[0, 1], [1, 0]], [-1.0, 0.0], 0.0, 1e-3)
+    assert abs(a1[0] - a2[0]) < 1e-5
+
+
+def test_haldane_chern_number():
+    """lib.rs:926-998: 2 pi sigma(mu=0) = 1 +- 1e-3 at 31x31, eta=1e-3."""
+    m = RM.haldane(make)
+    s = O.hall_conductivity_direct(m, [31, 31], [[1, 0], [0, 1]], [0.0, -2.0, 2.0], 0.0, 1e-3)
+    assert abs(s[0] * 2 * math.pi - 1.0) < 1e-3
+    grid = O.hall_conductivity_direct(m, [31, 31], [[1, 0], [0, 1]], np.linspace(-2, 2, 101), 0.0, 1e-3)
+    assert abs(grid[50] - s[0]) < 1e-3
+    assert abs(grid[0] - s[1]) < 1e-3 and abs(grid[100] - s[2]) < 1e-3
+
+
+def test_hall_time_reversal():
+    """lib.rs:2665-2740: sigma(t2) + sigma(-t2) <= 1e-10 (Direct), 51x51."""
+    mu = np.linspace(-3, 3, 101)
+    a = O.hall_conductivity_direct(RM.haldane(make, -0.3), [51, 51], np.eye(2), mu, 0.0, 0.05)
+    b = O.hall_conductivity_direct(RM.haldane(make, 0.3), [51, 51], np.eye(2), mu, 0.0, 0.05)
+    assert np.abs(a + b).max() < 1e-10
+    assert np.abs(a).max() > 1e-3
+
+
+def test_qwz_3d_plateau():
+    """lib.rs:3219-3270: stacked QWZ, eta=0.1; the EC plateau is 1/(2 pi)
+    within 3e-3 and direct-vs-EC within 5e-2, so direct is within 5.3e-2."""
+    m = RM.qwz_stacked_3d(make)
+    for nk in (10, 14):
+        s = O.hall_conductivity_direct(m, [nk, nk, 4], [[1, 0, 0], [0, 1, 0]], [0.0], 0.0, 0.1)
+        assert abs(abs(s[0]) - 1 / (2 * math.pi)) < 5.3e-2
+
+
+# --------------------------------------------------------- band fixture
+def test_graphene_band_fixture():
+    """examples/graphene/BAND.dat (2001 k, 4 bands, 6 decimals) from the
+    spinful SOC graphene of examples/graphene/main.rs:11-47 along G-K-K'-G."""
+    data = np.loadtxt(os.path.join(GOLDEN, "graphene_soc_BAND.dat"))
+    m = RM.graphene_soc_example(make)
+    path = [[0, 0], [1 / 3, 2 / 3], [2 / 3, 1 / 3], [0, 0]]
+    kvec, kdist, _ = O.k_path(m.lat, path, 2001)
+    E = np.stack([O.eigvalsh(H) for H in O.gen_ham_batch(m, kvec)])
+    assert np.abs(kdist - data[:, 0]).max() < 6e-7
+    assert np.abs(E - data[:, 1:]).max() < 6e-7
+
+
+# ------------------------------------------------- closed forms, invariants
+def test_graphene_closed_form_and_gauge():
+    m = RM.graphene_config1(make)
+    rng = np.random.default_rng(0)
+    for k in rng.random((20, 2)):
+        f = abs(1 + np.exp(-2j * np.pi * k[0]) + np.exp(-2j * np.pi * k[1]))
+        e = O.solve_band_onek(m, k)
+        assert np.allclose(e, [-2.85 * f, 2.85 * f][::-1] if -2.85 * f > 2.85 * f else [-2.85 * f, 2.85 * f], atol=1e-12)
+        el = O.eigvalsh(O.gen_ham(m, k, "lattice"))
+        assert np.allclose(e, el, atol=1e-12)
+        # lattice gauge periodic under k -> k + G
+        assert np.allclose(O.gen_ham(m, k, "lattice"), O.gen_ham(m, k + [1, -2], "lattice"), atol=1e-12)
+
+
+def _random_model(n_orb, spin, dim, nR_half, seed):
+    rng = np.random.default_rng(seed)
+    lat = np.eye(dim) + 0.1 * rng.standard_normal((dim, dim))
+    m = O.Model(lat, rng.random((n_orb, dim)), spin)
+    nsta = m.nsta
+    m.set_onsite(rng.uniform(-1, 1, n_orb))
+    for i in range(nsta):
+        for j in range(i + 1, nsta):
+            m.add_element(complex(*rng.standard_normal(2)), i, j, np.zeros(dim, int))
+    for _ in range(nR_half):
+        R = rng.integers(-2, 3, dim)
+        if not np.any(R):
+            continue
+        for i in range(nsta):
+            for j in range(nsta):
+                m.add_element(complex(*rng.standard_normal(2)) * 0.3, i, j, R)
+    return m
+
+
+def test_hermitian_storage_and_brute_force():
+    m = _random_model(3, True, 3, 4, 1)
+    for iR, R in enumerate(m.hamR):
+        inv = m.find_R(-R)
+        assert np.allclose(m.ham[inv], m.ham[iR].conj().T)
+    k = np.array([0.1, 0.27, -0.4])
+    H = O.gen_ham(m, k, "atom")
+    tau = m.orb_sta()
+    Hb = np.zeros_like(H)
+    for a in range(m.nsta):
+        for b in range(m.nsta):
+            for iR, R in enumerate(m.hamR):
+                Hb[a, b] += m.ham[iR, a, b] * np.exp(2j * np.pi * k @ (R - tau[a] + tau[b]))
+    assert np.abs(H - Hb).max() < 1e-12
+    assert np.abs(H - H.conj().T).max() < 1e-12
+    assert np.allclose(O.gen_ham_batch(m, k[None])[0], H, atol=1e-12)
+
+
+def test_velocity_is_finite_difference():
+    """Lattice gauge: v_p = (1/2pi) sum_d (lat w_p)_d dH/dk_d."""
+    m = _random_model(2, False, 2, 5, 2)
+    k = np.array([0.13, 0.71])
+    w = np.array([[0.6, -0.8], [1.0, 2.0]])
+    v, _ = O.gen_v_projected(m, k, "lattice", w)
+    h = 1e-6
+    for p in range(2):
+        c = m.lat @ w[p]
+        fd = sum(c[d] * (O.gen_ham(m, k + h * np.eye(2)[d], "lattice") -
+                         O.gen_ham(m, k - h * np.eye(2)[d], "lattice")) / (2 * h)
+                 for d in range(2)) / (2 * np.pi)
+        assert np.abs(v[p] - fd).max() < 1e-6
+    # linearity in the direction rows
+    v2, _ = O.gen_v_projected(m, k, "atom", [w[0] + 2 * w[1], w[1]])
+    va, _ = O.gen_v_projected(m, k, "atom", w)
+    assert np.abs(v2[0] - (va[0] + 2 * va[1])).max() < 1e-12
+    vb, Hb = O.gen_v_projected_batch(m, k[None], w)
+    assert np.abs(vb[0] - va).max() < 1e-12
+
+
+def test_berry_sum_rule_and_batch():
+    m = _random_model(3, False, 2, 4, 3)
+    ks = np.random.default_rng(4).random((5, 2))
+    for k in ks:
+        om, _ = O.berry_curvature_at(m, k, np.eye(2), 1e-3)
+        assert abs(om.sum()) < 1e-9 * max(1.0, np.abs(om).max())
+    ob, Eb = O.berry_curvature_batch(m, ks, np.eye(2), 1e-3)
+    oa = np.stack([O.berry_curvature_at(m, k, np.eye(2), 1e-3)[0] for k in ks])
+    assert np.abs(ob - oa).max() < 1e-9 * max(1.0, np.abs(oa).max())
+
+
+def test_dos_normalisation_and_single_level():
+    m = RM.graphene_config1(make)
+    E, rho = O.dos(m, [21, 21], -10.0, 10.0, 2001, 0.1)
+    assert abs(np.trapezoid(rho, E) - 2.0) < 1e-6
+    ev = np.array([0.3])
+    Eg = np.linspace(-1, 1, 11)
+    g = O.dos_from_eigenvalues(ev, Eg, 0.2)
+    ref = np.exp(-((Eg - 0.3) / 0.2) ** 2 / 2) / (0.2 * math.sqrt(2 * math.pi))
+    assert np.allclose(g, ref, rtol=1e-14, atol=0)
+
+
+def test_fermi_cases():
+    """thermodynamics.rs:152-172."""
+    assert O.fermi(-1.0, 0.0, 0.0) == 1.0
+    assert O.fermi(0.0, 0.0, 0.0) == 0.5
+    assert O.fermi(1.0, 0.0, 0.0) == 0.0
+    a = O.fermi(0.02, 0.0, 300.0)
+    assert abs(a - 1.0 / (1.0 + math.exp(0.02 / (300 * O.BOLTZMANN_CONSTANT_EV_PER_K)))) < 1e-14
+    assert O.fermi(10.0, 0.0, 1.0) == 0.0 and O.fermi(-10.0, 0.0, 1.0) == 1.0
+
+
+def test_kmesh_order_and_errors():
+    k = O.gen_kmesh([2, 3])
+    assert k.shape == (6, 2)
+    assert np.allclose(k[:3], [[0, 0], [0, 1 / 3], [0, 2 / 3]])
+    with pytest.raises(O.OracleError):
+        O.gen_kmesh([0, 3])
+    with pytest.raises(O.OracleError):
+        O.dos(RM.graphene_config1(make), [3, 3], 1.0, 1.0, 10, 0.1)
+    with pytest.raises(O.OracleError):
+        O.berry_curvature_at(RM.haldane(make), [0, 0], [[0, 0], [0, 1]], 1e-3)
+    with pytest.raises(O.OracleError):
+        RM.haldane(make).set_hop(1j, 0, 0, [0, 0])
