@@ -1,0 +1,206 @@
+"""Benchmark of the k-mesh hot path on one MI355X.
+
+Default workload: wannier3d-n128 (spinful 3D model, 64 orbitals -> n = 128,
+|R_i| <= 5 -> N_R = 1331, 48^3 = 110592 k-points): H(k) + dH/dk_{x,y} ->
+eigh -> Kubo Omega_z -> AHC at mu = 0, T = 0, eta = 1e-4, all on the device
+(one rtb_hall_direct call = one step; inputs resident in HBM, only the mu
+values and sigma cross PCIe).
+
+Prints ONE JSON line.  `value` is k-points/s of the whole job.  `roofline`
+is the dominant kernel's algorithmic work per launch over its average launch
+time, measured with HIP events inside the library on the kernel's stream.
+`cpu_baseline` is the reference-equivalent CPU path (oracle/cpu_ref.cpp:
+OpenMP over k, MKL/OpenBLAS zheev + zgemm, zaxpy-order Fourier sum) on a
+bounded sample of the same mesh.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "k-points/s (H(k)+eigh+Berry Ω), mixed vs FP64 GPU form at ≤1e-9 eV eig error"
+
+# published MI355X peaks
+PEAK_FP64 = 78.6  # TFLOP/s (vector and matrix)
+PEAK_FP32 = 157.3
+PEAK_HBM = 8000.0  # GB/s
+
+
+def device_sync():
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+
+
+def algorithmic_flops(stage, n, nR, d):
+    """Per-k algorithmic work (LAPACK/zaxpy conventions, independent of the
+    algorithm actually run): Fourier 8 N_R n^2 (1+d); eigensolve with vectors
+    (16/3 + 8) n^3 (tridiagonalisation + back-transform share the count);
+    Kubo 16 d n^3."""
+    if stage == "fourier":
+        return 8.0 * nR * n * n * (1 + d)
+    if stage == "eigh":
+        return 16.0 / 3.0 * n**3
+    if stage == "backtransform":
+        return 8.0 * n**3
+    if stage == "kubo":
+        return 16.0 * d * n**3
+    return 0.0
+
+
+def cpu_baseline(model, kmesh, dirs, eta, target_s):
+    from oracle import cpu_ref
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    rng = np.random.default_rng(123)
+    nk_total = int(np.prod(kmesh))
+    done, t_used, batch = 0, 0.0, threads
+    idx_all = []
+    while t_used < target_s and done < nk_total:
+        idx = rng.integers(0, nk_total, batch)
+        ks = np.stack(np.unravel_index(idx, kmesh), axis=1) / np.asarray(kmesh, float)
+        t0 = time.perf_counter()
+        cpu_ref.run(2, model.lat, model.orb, model.spin, model.hamR, model.ham, ks, dirs, eta, threads)
+        t_used += time.perf_counter() - t0
+        done += batch
+        idx_all.append(idx)
+        batch = min(batch * 2, threads * 16)
+    return {
+        "value": done / t_used,
+        "unit": "k-points/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} seeded random k-points of the same {'x'.join(map(str, kmesh))} mesh "
+                  f"(H + dH/dk zaxpy sum, zheev 'V', 4 zgemm, Kubo loop) in {t_used:.1f} s on {threads} threads, "
+                  f"LAPACK {cpu_ref.lapack_name()}",
+    }, np.concatenate(idx_all)
+
+
+def accuracy_check(model, kmesh, idx, dirs, eta, precision):
+    """GPU vs the CPU FP64 zheev path on the first sampled k-points."""
+    import rustb_amd as ra
+    from oracle import cpu_ref
+
+    idx = idx[:64]
+    ks = np.stack(np.unravel_index(idx, kmesh), axis=1) / np.asarray(kmesh, float)
+    p = ra.Parameters(kmesh=tuple(kmesh), direction=np.asarray(dirs, float), mu=np.array([0.0]), eta=eta,
+                      precision=precision)
+    om, E = model.berry_curvature_on(ks, p)
+    omr, Er = cpu_ref.run(2, model.lat, model.orb, model.spin, model.hamR, model.ham, ks, dirs, eta)
+    occ = (om * (E < 0)).sum(axis=1)
+    occr = (omr * (Er < 0)).sum(axis=1)
+    return {
+        "max_abs_dE_eV": float(np.abs(E - Er).max()),
+        "omega_occ_max_rel_err": float(np.abs(occ - occr).max() / max(np.abs(occr).max(), 1e-300)),
+        "k_points": int(len(idx)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--kmesh", default="48,48,48")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(k-points shard trivially; see the sharding notes). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import rustb_amd as ra
+    from rustb_amd import _lib
+
+    kmesh = tuple(int(x) for x in args.kmesh.split(","))
+    eta = 1e-4
+    dirs = [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]
+    model = ra.workloads.wannier3d_n128()
+    n, nR = model.nsta, model.hamR.shape[0]
+    nk = int(np.prod(kmesh))
+    params = ra.Parameters(kmesh=kmesh, direction=np.array(dirs), mu=np.array([0.0]), eta=eta,
+                           precision=args.precision)
+    model._device()  # upload outside the timed region
+
+    for _ in range(args.warmup):
+        model.hall_conductivity(params)
+    device_sync()
+    stage_ms = np.zeros(6)
+    stage_launch = np.zeros(6)
+    t0 = time.perf_counter()
+    sigma = None
+    for _ in range(args.steps):
+        sigma = model.hall_conductivity(params).conductivity[0]
+        st = _lib.last_stats()
+        stage_ms += st["ms"]
+        stage_launch += st["launches"]
+    device_sync()
+    elapsed = time.perf_counter() - t0
+    value = nk * args.steps / elapsed
+
+    names = _lib.STAGES
+    dom = int(np.argmax(stage_ms[:4]))
+    dname = names[dom]
+    per_launch_ms = stage_ms[dom] / max(stage_launch[dom], 1)
+    launches_per_step = stage_launch[dom] / args.steps
+    k_per_launch = nk / launches_per_step if dname != "kubo" else nk / (launches_per_step / 5.0)
+    if dname == "kubo":  # 4 GEMMs + Omega per chunk; report the per-chunk group
+        per_launch_ms = stage_ms[dom] / max(stage_launch[dom] / 5.0, 1)
+    flops = algorithmic_flops(dname, n, nR, 2) * k_per_launch
+    achieved = flops / (per_launch_ms * 1e-3) / 1e12
+    peak = PEAK_FP64 if args.precision == "fp64" else PEAK_FP32
+    roofline = {
+        "bound": "mfma",
+        "kernel": dname,
+        "achieved": round(achieved, 3),
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 4),
+        "traffic": None,
+        "per_launch_ms": round(per_launch_ms, 3),
+        "k_per_launch": int(k_per_launch),
+        "algorithmic_flop_per_k": algorithmic_flops(dname, n, nR, 2),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "k-points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "wannier3d-n128", "nsta": n, "n_R": nR, "kmesh": list(kmesh), "n_k": nk,
+                   "precision": args.precision, "eta": eta, "mu": [0.0], "T": 0.0, "outputs": "E, Omega_z, AHC"},
+        "stage_ms_per_step": {names[i]: round(stage_ms[i] / args.steps, 2) for i in range(6)},
+        "sigma_xy_mu0": float(sigma),
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        cb, idx = cpu_baseline(model, kmesh, dirs, eta, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = round(value / cb["value"], 1)
+        out["accuracy_vs_fp64_zheev"] = accuracy_check(model, kmesh, idx, dirs, eta, args.precision)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -152,13 +152,25 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
 /* Thread-local message of the last failure on this thread ("" if none). */
 const char* rtb_last_error(void);
 
-/* Per-stage device time of the last hot-path call on this thread, measured
- * with HIP events on the stream the kernels ran on.  Stages: 0 Fourier,
- * 1 eigensolve, 2 Kubo/reduction, 3 DOS, 4 whole call (first to last event).
+/* Per-kernel device time of the last hot-path call on this thread, measured
+ * with HIP events on the stream the kernels ran on.  Stages:
+ *   RTB_STAGE_FOURIER    Fourier sum kernel (H and projected velocities)
+ *   RTB_STAGE_EIGH       eigensolver kernel (tridiagonalisation, eigenvalues,
+ *                        tridiagonal eigenvectors)
+ *   RTB_STAGE_BACKTRANS  eigenvector back-transformation kernel
+ *   RTB_STAGE_KUBO       Kubo band transforms and Omega
+ *   RTB_STAGE_REDUCE     mesh points, occupation/DOS reductions
+ *   RTB_STAGE_TOTAL      first to last event of the call
  * ms[] in milliseconds; launches[] kernel launches per stage. */
+#define RTB_STAGE_FOURIER 0
+#define RTB_STAGE_EIGH 1
+#define RTB_STAGE_BACKTRANS 2
+#define RTB_STAGE_KUBO 3
+#define RTB_STAGE_REDUCE 4
+#define RTB_STAGE_TOTAL 5
 typedef struct rtb_stats {
-  double ms[5];
-  int launches[5];
+  double ms[6];
+  int launches[6];
   size_t nk;
   int nsta;
 } rtb_stats;

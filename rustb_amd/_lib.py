@@ -56,7 +56,7 @@ class SynthSpec(ctypes.Structure):
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [("ms", c_double * 5), ("launches", c_int * 5), ("nk", c_size_t), ("nsta", c_int)]
+    _fields_ = [("ms", c_double * 6), ("launches", c_int * 6), ("nk", c_size_t), ("nsta", c_int)]
 
 
 _P = c_void_p
@@ -119,3 +119,6 @@ def last_stats() -> dict:
     s = Stats()
     check(load().rtb_last_stats(ctypes.byref(s)))
     return {"ms": list(s.ms), "launches": list(s.launches), "nk": s.nk, "nsta": s.nsta}
+
+
+STAGES = ["fourier", "eigh", "backtransform", "kubo", "reduce", "total"]

@@ -97,8 +97,8 @@ class Timer {
     if (!recs_.empty()) {
       float ms = 0.f;
       RTB_HIP(hipEventElapsedTime(&ms, recs_.front().a, recs_.back().b));
-      s.ms[4] = ms;
-      for (int i = 0; i < 4; ++i) s.launches[4] += s.launches[i];
+      s.ms[RTB_STAGE_TOTAL] = ms;
+      for (int i = 0; i < RTB_STAGE_TOTAL; ++i) s.launches[RTB_STAGE_TOTAL] += s.launches[i];
     }
     g_stats = s;
     return RTB_OK;
@@ -424,7 +424,7 @@ int rtb_gen_ham_batch(const rtb_model* m, const double* k, size_t nk, int gauge,
     for (size_t i = 0; i < c; ++i)
       for (int d = 0; d < m->dim; ++d) kp[i * 3 + d] = k[(k0 + i) * m->dim + d];
     RTB_HIP(hipMemcpyAsync(dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    RTB_TRY(tmr.begin(0));
+    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
     RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
     RTB_TRY(tmr.end(1));
     RTB_HIP(hipMemcpyAsync(H + k0 * n2 * 2, dH, c * n2 * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -470,13 +470,17 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     for (size_t i = 0; i < c; ++i)
       for (int d = 0; d < m->dim; ++d) kp[i * 3 + d] = k[(k0 + i) * m->dim + d];
     RTB_HIP(hipMemcpyAsync(dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
-    RTB_TRY(tmr.begin(0));
+    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
     RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
     RTB_TRY(tmr.end(1));
-    RTB_TRY(tmr.begin(1));
+    RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
     RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S));
-    if (vec) RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, true));
-    RTB_TRY(tmr.end(vec ? 2 : 1));
+    RTB_TRY(tmr.end(1));
+    if (vec) {
+      RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
+      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, true));
+      RTB_TRY(tmr.end(1));
+    }
     RTB_HIP(hipMemcpyAsync(E + k0 * n, dE, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
     if (vec)
       RTB_HIP(hipMemcpyAsync(evec + k0 * n2 * 2, dZ, c * n2 * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -567,21 +571,25 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
         for (int d = 0; d < m->dim; ++d) kp[i * 3 + d] = j.k[(k0 + i) * m->dim + d];
       RTB_HIP(hipMemcpyAsync(dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
     }
-    RTB_TRY(tmr.begin(0));
-    int l0 = 1;
     if (!j.k) {
+      RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
       RTB_TRY(mesh_points(st, m->dim, j.kmesh, k0, c, dk));
-      ++l0;
+      RTB_TRY(tmr.end(1));
     }
+    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
     RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
-    RTB_TRY(tmr.end(l0));
-    RTB_TRY(tmr.begin(1));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
     RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
     RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, false));
-    RTB_TRY(tmr.end(2));
-    RTB_TRY(tmr.begin(2));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
     RTB_TRY(kubo_chunk(st, m, c, dV, dZ, dE, j.spin_dir, j.eta, T1, A, B, dJ, dOm));
-    int l2 = 5 + (spinj ? 1 : 0);
+    RTB_TRY(tmr.end(5 + (spinj ? 1 : 0)));
+    int l2 = 0;
+    RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     if (j.mode == 0) {
       RTB_HIP(hipMemcpyAsync(j.omega + k0 * n, dOm, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
       RTB_HIP(hipMemcpyAsync(j.E + k0 * n, dE, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -600,8 +608,7 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     if (j.mode != 2) RTB_HIP(hipStreamSynchronize(st));
   }
   if (j.mode == 2) {
-    RTB_TRY(tmr.begin(2));
-    const double scale = 1.0 / static_cast<double>(j.nk) / m->det_lat;
+    RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     RTB_TRY(reduce_partials(st, dpart, total_blk, static_cast<int>(j.n_mu), 1.0, dsig));
     RTB_TRY(tmr.end(1));
     std::vector<double> sums(j.n_mu);
@@ -609,7 +616,6 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     RTB_HIP(hipStreamSynchronize(st));
     // sum / nk / det (response/linear/mod.rs:221)
     for (size_t i = 0; i < j.n_mu; ++i) j.sigma[i] = sums[i] / static_cast<double>(j.nk) / m->det_lat;
-    (void)scale;
   }
   RTB_HIP(hipStreamSynchronize(st));
   RTB_TRY(tmr.collect(j.nk, n));
@@ -730,21 +736,23 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   int blk_off = 0;
   for (size_t k0 = 0; k0 < nk; k0 += K) {
     const size_t c = std::min(K, nk - k0);
-    RTB_TRY(tmr.begin(0));
+    RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     RTB_TRY(mesh_points(st, m->dim, kmesh, k0, c, dk));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
     RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
-    RTB_TRY(tmr.end(2));
-    RTB_TRY(tmr.begin(1));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
     RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr));
     RTB_TRY(tmr.end(1));
-    RTB_TRY(tmr.begin(3));
+    RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     const int nb = dos_blocks(c * n);
     RTB_TRY(dos_partials(st, c * n, dE, Emin, step, static_cast<int>(En), sigma,
                          dpart + static_cast<size_t>(blk_off) * En, nb));
     RTB_TRY(tmr.end(1));
     blk_off += nb;
   }
-  RTB_TRY(tmr.begin(3));
+  RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
   RTB_TRY(reduce_partials(st, dpart, total_blk, static_cast<int>(En), 1.0, drho));
   RTB_TRY(tmr.end(1));
   RTB_HIP(hipMemcpyAsync(rho, drho, En * sizeof(double), hipMemcpyDeviceToHost, st));
