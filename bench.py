@@ -111,7 +111,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--kmesh", default="48,48,48")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -160,7 +161,7 @@ def main():
         per_launch_ms = stage_ms[dom] / max(stage_launch[dom] / 5.0, 1)
     flops = algorithmic_flops(dname, n, nR, 2) * k_per_launch
     achieved = flops / (per_launch_ms * 1e-3) / 1e12
-    peak = PEAK_FP64 if args.precision == "fp64" else PEAK_FP32
+    peak = PEAK_FP64  # every stage computes in FP64
     roofline = {
         "bound": "mfma",
         "kernel": dname,
@@ -184,7 +185,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "wannier3d-n128", "nsta": n, "n_R": nR, "kmesh": list(kmesh), "n_k": nk,
                    "precision": args.precision, "eta": eta, "mu": [0.0], "T": 0.0, "outputs": "E, Omega_z, AHC"},
@@ -192,6 +193,19 @@ def main():
         "sigma_xy_mu0": float(sigma),
         "roofline": roofline,
     }
+    if args.precision == "mixed" and not args.no_fp64_compare:
+        # the plain FP64 GPU form on the same workload, same process
+        p64 = ra.Parameters(kmesh=kmesh, direction=np.array(dirs), mu=np.array([0.0]), eta=eta, precision="fp64")
+        model.hall_conductivity(p64)
+        device_sync()
+        t0 = time.perf_counter()
+        s64 = model.hall_conductivity(p64).conductivity[0]
+        device_sync()
+        t64 = time.perf_counter() - t0
+        out["fp64_plain"] = {"value": round(nk / t64, 2), "ms_per_step": round(t64 * 1e3, 2),
+                             "stage_ms": {names[i]: round(_lib.last_stats()["ms"][i], 2) for i in range(6)}}
+        out["mixed_over_fp64"] = round(value / (nk / t64), 3)
+        out["ahc_rel_err_mixed_vs_fp64"] = float(abs(sigma - s64) / abs(s64))
     if not args.no_cpu:
         cb, idx = cpu_baseline(model, kmesh, dirs, eta, args.cpu_seconds)
         out["cpu_baseline"] = cb

@@ -120,7 +120,10 @@ int g_smem_rc = RTB_OK;
 int prepare(const rtb_model* m) {
   if (!m) return fail(RTB_ERR_INVALID_ARGUMENT, "null model");
   RTB_HIP(hipSetDevice(m->device));
-  std::call_once(g_smem_once, [] { g_smem_rc = eigh_set_smem_limits(); });
+  std::call_once(g_smem_once, [] {
+    g_smem_rc = eigh_set_smem_limits();
+    if (g_smem_rc == RTB_OK) g_smem_rc = mesh_set_smem_limits();
+  });
   return g_smem_rc;
 }
 
@@ -248,12 +251,13 @@ size_t chunk_for(size_t nk, size_t per_k_bytes) {
 // The eigen + Kubo pipeline for one chunk of k-points whose H (and V when
 // omega is wanted) are already on the device.
 struct EigBufs {
-  double *Vg, *tau, *Y, *S;
+  double *Vg, *tau, *Y, *S, *DE;
 };
 
 int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
   const size_t n2 = static_cast<size_t>(n) * n;
   b.Vg = b.tau = nullptr;
+  RTB_TRY(w.alloc(&b.DE, 2 * K * n));
   if (vectors) {
     RTB_TRY(w.alloc(&b.Vg, K * n2 * 2));
     RTB_TRY(w.alloc(&b.tau, K * n * 2));
@@ -474,7 +478,7 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
     RTB_TRY(tmr.end(1));
     RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S));
+    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
     RTB_TRY(tmr.end(1));
     if (vec) {
       RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
@@ -518,7 +522,44 @@ struct BerryJob {
   const double* mu = nullptr;
   size_t n_mu = 0;
   double* sigma = nullptr;
+  int precision = RTB_PREC_FP64;
 };
+
+// A chunk of consecutive pipeline k-points; for the mesh path it covers the
+// pairs [p0, p0 + np) with every i1 (k1-innermost order).
+struct Chunk {
+  size_t k0, c, p0, np;
+};
+
+// Mesh intermediates for the separable Bloch sum (mixed form on mesh calls).
+static int mesh_setup(hipStream_t st, Work& w, Timer& tmr, const rtb_model* m, const size_t* kmesh, int precision,
+                      size_t nk, size_t K, bool want_v, MeshPlan& mp, double** G, std::vector<Chunk>& chunks) {
+  bool use_mesh = false;
+  *G = nullptr;
+  if (kmesh && precision == RTB_PREC_MIXED) {
+    RTB_TRY(mesh_plan(m, kmesh, mp));
+    use_mesh = mp.ok && static_cast<size_t>(mp.N[0]) <= K;
+  }
+  chunks.clear();
+  if (!use_mesh) {
+    for (size_t k0 = 0; k0 < nk; k0 += K) chunks.push_back({k0, std::min(K, nk - k0), 0, 0});
+    return RTB_OK;
+  }
+  const size_t KP = K / mp.N[0];
+  for (size_t p0 = 0; p0 < mp.P; p0 += KP) {
+    const size_t np = std::min(KP, mp.P - p0);
+    chunks.push_back({p0 * mp.N[0], np * mp.N[0], p0, np});
+  }
+  double* work;
+  int* bidx;
+  RTB_TRY(w.alloc(&work, 2 * (mp.box_elems + mp.ga_elems)));
+  RTB_TRY(w.alloc(G, 2 * mp.gb_elems));
+  RTB_TRY(w.alloc(&bidx, static_cast<size_t>(m->nR)));
+  RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+  RTB_TRY(mesh_prepare(st, m, mp, bidx, work, *G, want_v));
+  RTB_TRY(tmr.end(m->dim == 3 ? (want_v ? 4 : 3) : 2));
+  return RTB_OK;
+}
 
 static int berry_impl(const rtb_model* m, BerryJob& j) {
   const int n = m->nsta;
@@ -536,6 +577,10 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const size_t per_k = 24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 48;
   const size_t K = chunk_for(j.nk, per_k);
+  MeshPlan mp;
+  double* G = nullptr;
+  std::vector<Chunk> chunks;
+  RTB_TRY(mesh_setup(st, w, tmr, m, j.k ? nullptr : j.kmesh, j.precision, j.nk, K, true, mp, &G, chunks));
   double *dk, *dH, *dV, *dE, *dOm, *dJ = nullptr, *dZ;
   EigBufs eb;
   RTB_TRY(w.alloc(&dk, K * 3));
@@ -554,7 +599,7 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   double *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr;
   int total_blk = 0;
   if (j.mode == 2) {
-    for (size_t k0 = 0; k0 < j.nk; k0 += K) total_blk += hall_blocks(std::min(K, j.nk - k0));
+    for (const Chunk& ch : chunks) total_blk += hall_blocks(ch.c);
     RTB_TRY(w.alloc(&dmu, j.n_mu));
     RTB_TRY(w.alloc(&dpart, static_cast<size_t>(total_blk) * j.n_mu));
     RTB_TRY(w.alloc(&dsig, j.n_mu));
@@ -563,24 +608,30 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   if (j.mode == 1) RTB_TRY(w.alloc(&docc, K));
   std::vector<double> kp;
   int blk_off = 0;
-  for (size_t k0 = 0; k0 < j.nk; k0 += K) {
-    const size_t c = std::min(K, j.nk - k0);
+  for (const Chunk& ch : chunks) {
+    const size_t k0 = ch.k0, c = ch.c;
     if (j.k) {
       kp.assign(c * 3, 0.0);
       for (size_t i = 0; i < c; ++i)
         for (int d = 0; d < m->dim; ++d) kp[i * 3 + d] = j.k[(k0 + i) * m->dim + d];
       RTB_HIP(hipMemcpyAsync(dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
     }
-    if (!j.k) {
-      RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
-      RTB_TRY(mesh_points(st, m->dim, j.kmesh, k0, c, dk));
+    if (G) {
+      RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+      RTB_TRY(mesh_final(st, m, mp, G, a, j.dir, ch.p0, ch.np, dH, dV));
+      RTB_TRY(tmr.end(1));
+    } else {
+      if (!j.k) {
+        RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
+        RTB_TRY(mesh_points(st, m->dim, j.kmesh, k0, c, dk));
+        RTB_TRY(tmr.end(1));
+      }
+      RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+      RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
       RTB_TRY(tmr.end(1));
     }
-    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
-    RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
-    RTB_TRY(tmr.end(1));
     RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S));
+    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
     RTB_TRY(tmr.end(1));
     RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
     RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, false));
@@ -693,6 +744,7 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
   j.mu = mu;
   j.n_mu = n_mu;
   j.sigma = sigma;
+  j.precision = precision;
   return berry_impl(m, j);
 }
 
@@ -722,6 +774,10 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   a.gauge = RTB_GAUGE_ATOM;
   const size_t n2 = static_cast<size_t>(n) * n;
   const size_t K = chunk_for(nk, 24 + n2 * 32 + n * 24);
+  MeshPlan mp;
+  double* G = nullptr;
+  std::vector<Chunk> chunks;
+  RTB_TRY(mesh_setup(st, w, tmr, m, kmesh, precision, nk, K, false, mp, &G, chunks));
   double *dk, *dH, *dE;
   EigBufs eb;
   RTB_TRY(w.alloc(&dk, K * 3));
@@ -729,21 +785,27 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   RTB_TRY(w.alloc(&dE, K * n));
   RTB_TRY(alloc_eig(w, n, K, false, eb));
   int total_blk = 0;
-  for (size_t k0 = 0; k0 < nk; k0 += K) total_blk += dos_blocks(std::min(K, nk - k0) * n);
+  for (const Chunk& ch : chunks) total_blk += dos_blocks(ch.c * n);
   double *dpart, *drho;
   RTB_TRY(w.alloc(&dpart, static_cast<size_t>(total_blk) * En));
   RTB_TRY(w.alloc(&drho, En));
   int blk_off = 0;
-  for (size_t k0 = 0; k0 < nk; k0 += K) {
-    const size_t c = std::min(K, nk - k0);
-    RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
-    RTB_TRY(mesh_points(st, m->dim, kmesh, k0, c, dk));
-    RTB_TRY(tmr.end(1));
-    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
-    RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
-    RTB_TRY(tmr.end(1));
+  for (const Chunk& ch : chunks) {
+    const size_t k0 = ch.k0, c = ch.c;
+    if (G) {
+      RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+      RTB_TRY(mesh_final(st, m, mp, G, a, nullptr, ch.p0, ch.np, dH, nullptr));
+      RTB_TRY(tmr.end(1));
+    } else {
+      RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
+      RTB_TRY(mesh_points(st, m->dim, kmesh, k0, c, dk));
+      RTB_TRY(tmr.end(1));
+      RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+      RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
+      RTB_TRY(tmr.end(1));
+    }
     RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr));
+    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr, eb.DE));
     RTB_TRY(tmr.end(1));
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     const int nb = dos_blocks(c * n);

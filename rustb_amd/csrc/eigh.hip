@@ -1,4 +1,10 @@
-// Batched Hermitian eigensolver kernels (gfx950): one workgroup per matrix.
+// Batched Hermitian eigensolver kernels (gfx950).
+//
+//   eigh_tridiag_kernel     one workgroup per matrix: Householder reduction
+//                           with the packed lower triangle in LDS
+//   eigh_trieig_kernel      one thread per eigenvalue, 256/n matrices per
+//                           workgroup: bisection (+ inverse iteration)
+//   eigh_backtransform_kernel  Z = Q Y per block of columns
 #include "eigh.hpp"
 
 #include "eigh_core.hpp"
@@ -9,16 +15,16 @@ namespace rtb {
 namespace {
 
 template <typename T>
-struct Stage1Smem {
+struct TriSmem {
   cplx<T>* P;
   cplx<T>* v;
   cplx<T>* w;
-  T *d, *e, *e2, *E, *red;
+  T *d, *e, *red;
 };
 
 template <typename T>
-__device__ Stage1Smem<T> carve(void* base, int n) {
-  Stage1Smem<T> s;
+__device__ TriSmem<T> carve_tri(void* base, int n) {
+  TriSmem<T> s;
   char* p = static_cast<char*>(base);
   s.P = reinterpret_cast<cplx<T>*>(p);
   p += sizeof(cplx<T>) * (static_cast<size_t>(n) * (n + 1) / 2);
@@ -28,57 +34,88 @@ __device__ Stage1Smem<T> carve(void* base, int n) {
   p += sizeof(cplx<T>) * n;
   s.d = reinterpret_cast<T*>(p);
   s.e = s.d + n;
-  s.e2 = s.e + n;
-  s.E = s.e2 + n;
-  s.red = s.E + n;
+  s.red = s.e + n;
   return s;
 }
 
-// H: [nmat][n][n] complex FP64 (row-major); reads the lower triangle.
-template <typename T, bool VEC>
-__global__ void __launch_bounds__(256) eigh_stage1_kernel(int n, const double* __restrict__ H,
-                                                          double* __restrict__ Eout,
-                                                          cplx<T>* __restrict__ Vg,
-                                                          cplx<T>* __restrict__ taug,
-                                                          T* __restrict__ Y, T* __restrict__ S) {
+// H: [nmat][n][n] complex FP64 (row-major; the lower triangle is read).
+// Dg, Eg: [nmat][n] tridiagonal (diagonal, sub-diagonal); Vg, taug reflectors.
+template <typename T>
+__global__ void __launch_bounds__(256) eigh_tridiag_kernel(int n, const double* __restrict__ H, T* __restrict__ Dg,
+                                                           T* __restrict__ Eg, cplx<T>* __restrict__ Vg,
+                                                           cplx<T>* __restrict__ taug) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   GpuTeam tm;
   const size_t mat = blockIdx.x;
-  Stage1Smem<T> s = carve<T>(smem_raw, n);
+  TriSmem<T> s = carve_tri<T>(smem_raw, n);
   const double* Hm = H + mat * static_cast<size_t>(n) * n * 2;
-  // lower triangle into packed storage
+  // lower triangle into packed storage, rows walked incrementally
   const int np = n * (n + 1) / 2;
-  for (int f = tm.id; f < np; f += tm.nt) {
-    int r = static_cast<int>((sqrtf(8.0f * f + 1.0f) - 1.0f) * 0.5f);
-    if ((r + 1) * (r + 2) / 2 <= f) ++r;
-    if (r * (r + 1) / 2 > f) --r;
-    int c = f - r * (r + 1) / 2;
-    const double* h = Hm + (static_cast<size_t>(r) * n + c) * 2;
-    s.P[f].re = static_cast<T>(h[0]);
-    s.P[f].im = static_cast<T>(h[1]);
+  {
+    int r = 0, rs = 0;
+    for (int f = tm.id; f < np; f += tm.nt) {
+      while (f >= rs + r + 1) {
+        rs += r + 1;
+        ++r;
+      }
+      const int c = f - rs;
+      const double2 h = reinterpret_cast<const double2*>(Hm)[static_cast<size_t>(r) * n + c];
+      s.P[f].re = static_cast<T>(h.x);
+      s.P[f].im = static_cast<T>(h.y);
+    }
   }
   tm.sync();
   cplx<T>* Vm = Vg ? Vg + mat * static_cast<size_t>(n) * n : nullptr;
-  cplx<T>* tm_tau = taug ? taug + mat * static_cast<size_t>(n) : nullptr;
-  tridiagonalize<T>(tm, n, s.P, s.v, s.w, s.d, s.e, s.red, Vm, tm_tau);
-  bisect_all<T>(tm, n, s.d, s.e, s.e2, s.E);
-  for (int j = tm.id; j < n; j += tm.nt) Eout[mat * n + j] = static_cast<double>(s.E[j]);
+  cplx<T>* tu = taug ? taug + mat * static_cast<size_t>(n) : nullptr;
+  tridiagonalize<T>(tm, n, s.P, s.v, s.w, s.d, s.e, s.red, Vm, tu);
+  for (int i = tm.id; i < n; i += tm.nt) {
+    Dg[mat * n + i] = s.d[i];
+    Eg[mat * n + i] = s.e[i];
+  }
+}
+
+constexpr int kTeBlock = 256;
+
+// Tridiagonal eigenvalues (+ eigenvectors): sub-team of n threads per matrix.
+template <typename T, bool VEC>
+__global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nmat, const T* __restrict__ Dg,
+                                                              const T* __restrict__ Eg, double* __restrict__ Eout,
+                                                              T* __restrict__ Y, T* __restrict__ S) {
+  __shared__ T sh[4 * kTeBlock];
+  const int per = kTeBlock / n;  // matrices per workgroup (n <= 128)
+  const int sub = threadIdx.x / n;
+  const size_t mat = static_cast<size_t>(blockIdx.x) * per + sub;
+  const bool live = sub < per && mat < nmat;
+  // idle threads take id >= n and sub-team 0's LDS (read-only for them)
+  const int so = live ? sub : 0;
+  GpuSubTeam tm(live ? static_cast<int>(threadIdx.x % n) : n, n);
+  T* d = sh + so * 4 * n;
+  T* e = d + n;
+  T* e2 = e + n;
+  T* E = e2 + n;
+  if (live) {
+    d[tm.id] = Dg[mat * n + tm.id];
+    e[tm.id] = Eg[mat * n + tm.id];
+  }
+  __syncthreads();
+  bisect_all<T>(tm, n, d, e, e2, E);
+  if (live) Eout[mat * n + tm.id] = static_cast<double>(E[tm.id]);
   if (VEC) {
-    T* Ym = Y + mat * static_cast<size_t>(n) * n;
-    T* Sm = S + mat * static_cast<size_t>(n) * n * 3;
-    tridiag_vectors<T>(tm, n, s.d, s.e, s.E, Ym, n, Sm);
+    const size_t mm = live ? mat : 0;
+    T* Ym = Y + mm * static_cast<size_t>(n) * n;
+    T* Sm = S + mm * static_cast<size_t>(n) * n * 3;
+    tridiag_vectors<T>(tm, n, d, e, E, Ym, n, Sm);
   }
 }
 
 // Z[:, c0:c0+cb] = Q Y[:, c0:c0+cb]; output complex FP64.
-//   rows_out = false: Z[mat][r][c] (eigenvectors as columns)
-//   rows_out = true : Z[mat][c][r] (eigenvectors as rows, solve_all layout)
+//   rows_out = 0: Z[mat][r][c] (eigenvectors as columns)
+//   rows_out = 1: Z[mat][c][r] (eigenvectors as rows, solve_all layout)
 template <typename T>
 __global__ void __launch_bounds__(256) eigh_backtransform_kernel(int n, int cb, const T* __restrict__ Y,
                                                                  const cplx<T>* __restrict__ Vg,
                                                                  const cplx<T>* __restrict__ taug,
-                                                                 double* __restrict__ Zout,
-                                                                 int rows_out) {
+                                                                 double* __restrict__ Zout, int rows_out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   GpuTeam tm;
   const size_t mat = blockIdx.x;
@@ -86,7 +123,8 @@ __global__ void __launch_bounds__(256) eigh_backtransform_kernel(int n, int cb, 
   const int w = min(cb, n - c0);
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* vs = Z + static_cast<size_t>(n) * cb;
-  cplx<T>* sv = vs + n;
+  cplx<T>* part = vs + 2 * n;
+  cplx<T>* sv = part + tm.nt;
   const T* Ym = Y + mat * static_cast<size_t>(n) * n;
   for (int f = tm.id; f < n * cb; f += tm.nt) {
     int r = f / cb, c = f % cb;
@@ -94,8 +132,8 @@ __global__ void __launch_bounds__(256) eigh_backtransform_kernel(int n, int cb, 
     Z[f] = {y, T(0)};
   }
   tm.sync();
-  apply_q<T>(tm, n, cb, Z, Vg + mat * static_cast<size_t>(n) * n, taug + mat * static_cast<size_t>(n),
-             vs, sv);
+  apply_q<T>(tm, n, cb, Z, Vg + mat * static_cast<size_t>(n) * n, taug + mat * static_cast<size_t>(n), vs, part,
+             sv);
   double* Zm = Zout + mat * static_cast<size_t>(n) * n * 2;
   for (int f = tm.id; f < n * w; f += tm.nt) {
     int r, c;
@@ -108,29 +146,48 @@ __global__ void __launch_bounds__(256) eigh_backtransform_kernel(int n, int cb, 
     }
     cplx<T> z = Z[r * cb + c];
     size_t o = rows_out ? (static_cast<size_t>(c0 + c) * n + r) : (static_cast<size_t>(r) * n + c0 + c);
-    Zm[2 * o] = static_cast<double>(z.re);
-    Zm[2 * o + 1] = static_cast<double>(z.im);
+    reinterpret_cast<double2*>(Zm)[o] = make_double2(static_cast<double>(z.re), static_cast<double>(z.im));
   }
 }
 
-}  // namespace
-
-size_t eigh_stage1_smem(int n, int real_size) {
-  return EighSmem::bytes(n, 256, static_cast<size_t>(real_size));
+size_t tri_smem(int n, size_t rs) {
+  return sizeof(double) * 0 + (static_cast<size_t>(n) * (n + 1) / 2 + 2 * n) * 2 * rs + 2 * n * rs +
+         (256 + 64) * rs;
 }
 
-int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S) {
+int bt_cb(int n) { return n <= 64 ? n : 32; }
+size_t bt_smem(int n, int cb, size_t rs) { return 2 * rs * (static_cast<size_t>(n) * cb + 2 * n + 256 + cb); }
+
+}  // namespace
+
+int eigh_set_smem_limits() {
+  // up to 160 KB of dynamic LDS per workgroup on gfx950
+  const int lim = 160 * 1024;
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_tridiag_kernel<double>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_backtransform_kernel<double>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  return RTB_OK;
+}
+
+int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors, double* Vg,
+                     double* taug, double* Y, double* S, double* DE) {
   if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
   if (nmat == 0) return RTB_OK;
-  const size_t sm = eigh_stage1_smem(n, sizeof(double));
-  dim3 grid(static_cast<unsigned>(nmat));
+  double* Dg = DE;
+  double* Eg = DE + nmat * n;
+  hipLaunchKernelGGL((eigh_tridiag_kernel<double>), dim3(static_cast<unsigned>(nmat)), dim3(256),
+                     tri_smem(n, sizeof(double)), st, n, H, Dg, Eg, reinterpret_cast<cplx<double>*>(Vg),
+                     reinterpret_cast<cplx<double>*>(taug));
+  RTB_HIP(hipGetLastError());
+  const int per = kTeBlock / n;
+  const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
   if (vectors) {
-    hipLaunchKernelGGL((eigh_stage1_kernel<double, true>), grid, dim3(256), sm, st, n, H, E,
-                       reinterpret_cast<cplx<double>*>(Vg), reinterpret_cast<cplx<double>*>(taug), Y, S);
+    hipLaunchKernelGGL((eigh_trieig_kernel<double, true>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg, E,
+                       Y, S);
   } else {
-    hipLaunchKernelGGL((eigh_stage1_kernel<double, false>), grid, dim3(256), sm, st, n, H, E,
-                       reinterpret_cast<cplx<double>*>(Vg), reinterpret_cast<cplx<double>*>(taug), Y, S);
+    hipLaunchKernelGGL((eigh_trieig_kernel<double, false>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg,
+                       E, Y, S);
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -139,25 +196,12 @@ int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* Vg,
                             const double* taug, double* Z, bool rows_out) {
   if (nmat == 0) return RTB_OK;
-  const int cb = n <= 64 ? n : 32;
-  const size_t sm = sizeof(cplx<double>) * (static_cast<size_t>(n) * cb + n + cb);
+  const int cb = bt_cb(n);
   dim3 grid(static_cast<unsigned>(nmat), static_cast<unsigned>((n + cb - 1) / cb));
-  hipLaunchKernelGGL((eigh_backtransform_kernel<double>), grid, dim3(256), sm, st, n, cb, Y,
-                     reinterpret_cast<const cplx<double>*>(Vg), reinterpret_cast<const cplx<double>*>(taug),
+  hipLaunchKernelGGL((eigh_backtransform_kernel<double>), grid, dim3(256), bt_smem(n, cb, sizeof(double)), st, n,
+                     cb, Y, reinterpret_cast<const cplx<double>*>(Vg), reinterpret_cast<const cplx<double>*>(taug),
                      Z, rows_out ? 1 : 0);
   RTB_HIP(hipGetLastError());
-  return RTB_OK;
-}
-
-int eigh_set_smem_limits() {
-  // allow up to 160 KB of dynamic LDS per workgroup on gfx950
-  const int lim = 160 * 1024;
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_stage1_kernel<double, true>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_stage1_kernel<double, false>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_backtransform_kernel<double>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RTB_OK;
 }
 

@@ -6,13 +6,13 @@
 namespace rtb {
 
 // ---- eigensolver (eigh.hip) ----
-size_t eigh_stage1_smem(int n, int real_size);
 int eigh_set_smem_limits();
 // Householder + bisection (+ inverse iteration when `vectors`).
 //   H: [nmat][n][n] complex; E: [nmat][n]; Vg: [nmat][n][n] complex;
-//   taug: [nmat][n] complex; Y: [nmat][n][n]; S: [nmat][3][n][n]
+//   taug: [nmat][n] complex; Y: [nmat][n][n]; S: [nmat][3][n][n];
+//   DE: [2][nmat][n] tridiagonal scratch
 int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S);
+                     double* Vg, double* taug, double* Y, double* S, double* DE);
 // Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* Vg,
                             const double* taug, double* Z, bool rows_out);
@@ -30,6 +30,23 @@ struct FourierArgs {
 // k: [nk][3] (zero padded); H: [nk][n][n]; V: [nk][np][n][n] (or null when np == 0)
 int fourier_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t nk, double* H,
                   double* V);
+// Axis-separable Bloch sum on a uniform mesh (fourier_mesh.hip).
+struct MeshPlan {
+  int dim = 0;
+  int lo[3], L[3], N[3];
+  size_t box = 0, P = 0;               // box cells, pairs (N2*N3)
+  size_t box_elems = 0, ga_elems = 0, gb_elems = 0;  // complex element counts
+  bool ok = false;
+};
+int mesh_plan(const rtb_model* m, const size_t* kmesh, MeshPlan& p);
+// passes A/B: work >= box_elems + ga_elems complex; G = gb_elems complex
+int mesh_prepare(hipStream_t st, const rtb_model* m, const MeshPlan& p, int* d_boxidx, double* work, double* G,
+                 bool want_v);
+// pass C for pairs [p0, p0 + np_pairs): H [np_pairs*N1][n][n], V [..][2][n][n]
+// in k1-innermost order: local k = (pair - p0) * N1 + i1
+int mesh_final(hipStream_t st, const rtb_model* m, const MeshPlan& p, const double* G, const FourierArgs& a,
+               const double* dirs, size_t p0, size_t np_pairs, double* H, double* V);
+int mesh_set_smem_limits();
 // Uniform mesh points k = (i/N1, j/N2, l/N3) for flat indices [k0, k0 + nk).
 int mesh_points(hipStream_t st, int dim, const size_t* kmesh, size_t k0, size_t nk, double* k);
 

@@ -142,27 +142,48 @@ RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, 
     }
     tm.sync();
     if (tau.re != T(0) || tau.im != T(0)) {
-      // p = tau * A22 v (rows j+1..n-1), and the partial of p^H v
+      // p = tau * A22 v (rows j+1..n-1) and the partial of p^H v.  With
+      // Team::kPairs two threads share a row (even/odd columns) and combine
+      // through pair_sum; the trip count is uniform across the team.
       cplx<T> dot = {T(0), T(0)};
-      for (int r = j + 1 + id; r < n; r += nt) {
+      const int half = Team::kPairs ? (id & 1) : 0;
+      const int cstep = Team::kPairs ? 2 : 1;
+      const int rid = Team::kPairs ? (id >> 1) : id;
+      const int rnt = Team::kPairs ? (nt >> 1) : nt;
+      const int m = n - j - 1;
+      for (int base = 0; base < m; base += rnt) {
+        const int r = j + 1 + base + rid;
+        const bool act = (base + rid < m) && (rid < rnt);
         cplx<T> s = {T(0), T(0)};
-        for (int c = j + 1; c <= r; ++c) {
-          cplx<T> a = P[pidx(r, c)];
-          cplx<T> vc = v[c];
-          s.re += a.re * vc.re - a.im * vc.im;
-          s.im += a.re * vc.im + a.im * vc.re;
+        if (act) {
+          const cplx<T>* Prow = P + pidx(r, 0);
+          for (int c = j + 1 + half; c <= r; c += cstep) {
+            cplx<T> a = Prow[c];
+            cplx<T> vc = v[c];
+            s.re += a.re * vc.re - a.im * vc.im;
+            s.im += a.re * vc.im + a.im * vc.re;
+          }
+          // same column parity class as the first loop (c = j+1+half mod 2)
+          int c = r + 1;
+          if (Team::kPairs && ((c - (j + 1 + half)) & 1) != 0) ++c;
+          for (; c < n; c += cstep) {
+            cplx<T> a = P[pidx(c, r)];  // A(r,c) = conj(A(c,r))
+            cplx<T> vc = v[c];
+            s.re += a.re * vc.re + a.im * vc.im;
+            s.im += a.re * vc.im - a.im * vc.re;
+          }
         }
-        for (int c = r + 1; c < n; ++c) {
-          cplx<T> a = P[pidx(c, r)];  // A(r,c) = conj(A(c,r))
-          cplx<T> vc = v[c];
-          s.re += a.re * vc.re + a.im * vc.im;
-          s.im += a.re * vc.im - a.im * vc.re;
+        if (Team::kPairs) {
+          s.re = tm.pair_sum(s.re, red);
+          s.im = tm.pair_sum(s.im, red);
         }
-        cplx<T> p = cmul(tau, s);
-        w[r] = p;
-        cplx<T> pv = cmulc(p, v[r]);
-        dot.re += pv.re;
-        dot.im += pv.im;
+        if (act && half == 0) {
+          cplx<T> p = cmul(tau, s);
+          w[r] = p;
+          cplx<T> pv = cmulc(p, v[r]);
+          dot.re += pv.re;
+          dot.im += pv.im;
+        }
       }
       const T dre = tm.reduce_sum(dot.re, red);
       const T dim_ = tm.reduce_sum(dot.im, red);
@@ -176,13 +197,16 @@ RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, 
         w[r].im += t.im;
       }
       tm.sync();
-      // A22 -= v w^H + w v^H on the packed lower triangle
+      // A22 -= v w^H + w v^H on the packed lower triangle (rows j+1..n-1),
+      // row index tracked incrementally along each thread's strided walk
       const int f0 = pidx(j + 1, 0), f1 = n * (n + 1) / 2;
+      int r = j + 1, rs = f0;  // current row and its packed start
       for (int f = f0 + id; f < f1; f += nt) {
-        int r = static_cast<int>((sqrt(8.0 * f + 1.0) - 1.0) * 0.5);
-        if ((r + 1) * (r + 2) / 2 <= f) ++r;
-        if (r * (r + 1) / 2 > f) --r;
-        int c = f - r * (r + 1) / 2;
+        while (f >= rs + r + 1) {
+          rs += r + 1;
+          ++r;
+        }
+        const int c = f - rs;
         if (c <= j) continue;
         cplx<T> vr = v[r], wr = w[r], vc = v[c], wc = w[c];
         cplx<T> t1 = cmul(vr, conjc(wc));
@@ -204,6 +228,29 @@ RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, 
 // Stage 2: Sturm-count bisection (dstebz rule).  Thread t computes the
 // eigenvalues t, t+nt, ... ascending into E.  e2[i] = e[i]^2 must be filled.
 // --------------------------------------------------------------------------
+// 1/q: on the GPU the hardware reciprocal refined by two Newton steps (error
+// well below the bisection tolerance); on the host a plain division.
+RTB_HD inline double recip(double q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(q);
+  double e = fma(-q, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-q, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / q;
+#endif
+}
+RTB_HD inline float recip(float q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  float r = __builtin_amdgcn_rcpf(q);
+  float e = fmaf(-q, r, 1.0f);
+  return fmaf(r, e, r);
+#else
+  return 1.0f / q;
+#endif
+}
+
 template <typename T>
 RTB_HD inline int sturm_count(int n, const T* d, const T* e2, T x, T pivmin) {
   int cnt = 0;
@@ -211,7 +258,7 @@ RTB_HD inline int sturm_count(int n, const T* d, const T* e2, T x, T pivmin) {
   if (fabs(q) < pivmin) q = -pivmin;
   cnt += q < T(0);
   for (int i = 1; i < n; ++i) {
-    q = d[i] - x - e2[i - 1] / q;
+    q = d[i] - x - e2[i - 1] * recip(q);
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < T(0);
   }
@@ -413,37 +460,63 @@ RTB_HD void tridiag_vectors(Team& tm, int n, const T* d, const T* e, const T* E,
 
 // --------------------------------------------------------------------------
 // Stage 4: back-transformation of a block of columns, Z = Q Z,
-//   Z : team-shared complex [n][cb] (row r, column c), holds Y on entry
+//   Z    : team-shared complex [n][cb] (row r, column c), holds Y on entry
 //   Vg/taug : reflectors from stage 1
-//   s : team-shared complex [cb]
+//   vs   : team-shared complex [2][n] (double-buffered reflector)
+//   part : team-shared complex [max(nt, cb)] (row-slice partial dot products)
+//   s    : team-shared complex [cb]
+// Per reflector: s_c = tau sum_r conj(v_r) Z[r][c] (row slices in parallel,
+// then a fixed-order sum of the slices), Z[r][c] -= v_r s_c.
 // --------------------------------------------------------------------------
 template <typename T, typename Team>
-RTB_HD void apply_q(Team& tm, int n, int cb, cplx<T>* Z, const cplx<T>* Vg, const cplx<T>* taug,
-                    cplx<T>* vs, cplx<T>* s) {
+RTB_HD void apply_q(Team& tm, int n, int cb, cplx<T>* Z, const cplx<T>* Vg, const cplx<T>* taug, cplx<T>* vs,
+                    cplx<T>* part, cplx<T>* s) {
+  const int S = tm.nt / cb > 0 ? tm.nt / cb : 1;  // row slices per column
+  int buf = 0;
+  if (n >= 2)
+    for (int r = tm.id; r < n; r += tm.nt) vs[r] = Vg[static_cast<size_t>(n - 2) * n + r];
+  tm.sync();
   for (int j = n - 2; j >= 0; --j) {
+    const cplx<T>* v = vs + buf * n;
     const cplx<T> tau = taug[j];
-    if (tau.re == T(0) && tau.im == T(0)) continue;
-    for (int r = tm.id; r < n; r += tm.nt) vs[r] = Vg[static_cast<size_t>(j) * n + r];
-    tm.sync();
-    // s_c = sum_r conj(v_r) Z[r][c]
-    for (int c = tm.id; c < cb; c += tm.nt) {
-      cplx<T> acc = {T(0), T(0)};
-      for (int r = j + 1; r < n; ++r) {
-        cplx<T> t = cmulc(vs[r], Z[r * cb + c]);
-        acc.re += t.re;
-        acc.im += t.im;
+    const bool active = !(tau.re == T(0) && tau.im == T(0));
+    if (active) {
+      for (int q = tm.id; q < S * cb; q += tm.nt) {
+        const int c = q % cb, sl = q / cb;
+        cplx<T> acc = {T(0), T(0)};
+        for (int r = j + 1 + sl; r < n; r += S) {
+          cplx<T> t = cmulc(v[r], Z[r * cb + c]);
+          acc.re += t.re;
+          acc.im += t.im;
+        }
+        part[q] = acc;
       }
-      s[c] = cmul(tau, acc);
     }
     tm.sync();
-    const int tot = (n - j - 1) * cb;
-    for (int f = tm.id; f < tot; f += tm.nt) {
-      int r = j + 1 + f / cb, c = f % cb;
-      cplx<T> t = cmul(vs[r], s[c]);
-      Z[r * cb + c].re -= t.re;
-      Z[r * cb + c].im -= t.im;
+    if (active) {
+      for (int c = tm.id; c < cb; c += tm.nt) {
+        cplx<T> acc = {T(0), T(0)};
+        for (int sl = 0; sl < S; ++sl) {
+          acc.re += part[sl * cb + c].re;
+          acc.im += part[sl * cb + c].im;
+        }
+        s[c] = cmul(tau, acc);
+      }
+    }
+    if (j >= 1)  // prefetch the next reflector into the other buffer
+      for (int r = tm.id; r < n; r += tm.nt) vs[(buf ^ 1) * n + r] = Vg[static_cast<size_t>(j - 1) * n + r];
+    tm.sync();
+    if (active) {
+      const int tot = (n - j - 1) * cb;
+      for (int f = tm.id; f < tot; f += tm.nt) {
+        int r = j + 1 + f / cb, c = f % cb;
+        cplx<T> t = cmul(v[r], s[c]);
+        Z[r * cb + c].re -= t.re;
+        Z[r * cb + c].im -= t.im;
+      }
     }
     tm.sync();
+    buf ^= 1;
   }
 }
 

@@ -6,9 +6,16 @@
 namespace rtb {
 
 struct GpuTeam {
+  static constexpr bool kPairs = true;  // lanes (2i, 2i+1) share rows
   int id, nt;
   __device__ GpuTeam() : id(threadIdx.x), nt(blockDim.x) {}
   __device__ void sync() { __syncthreads(); }
+
+  // x + partner's x for the lane pair (2i, 2i+1); both lanes must be active.
+  template <typename T>
+  __device__ T pair_sum(T x, T*) {
+    return x + __shfl_xor(x, 1, 64);
+  }
 
   // Sum over the workgroup, returned to every thread.  Deterministic: a fixed
   // butterfly inside each wave, then the per-wave partials in wave order.
@@ -24,6 +31,17 @@ struct GpuTeam {
     __syncthreads();
     return s;
   }
+};
+
+// `nt` threads of a workgroup working on one matrix while the other threads
+// of the workgroup work on others; sync() is the whole-workgroup barrier, so
+// every sub-team must execute the same barrier sequence (idle sub-teams take
+// id >= n and skip all per-index work).
+struct GpuSubTeam {
+  static constexpr bool kPairs = false;
+  int id, nt;
+  __device__ GpuSubTeam(int id_, int nt_) : id(id_), nt(nt_) {}
+  __device__ void sync() { __syncthreads(); }
 };
 
 }  // namespace rtb

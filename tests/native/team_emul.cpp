@@ -10,10 +10,20 @@
 
 namespace {
 
+template <bool Pairs>
 struct CpuTeam {
+  static constexpr bool kPairs = Pairs;
   int id, nt;
   std::barrier<>* bar;
   void sync() { bar->arrive_and_wait(); }
+  template <typename T>
+  T pair_sum(T x, T* red) {
+    red[id] = x;
+    sync();
+    T y = x + red[id ^ 1];
+    sync();
+    return y;
+  }
   template <typename T>
   T reduce_sum(T x, T* red) {
     red[id] = x;
@@ -25,14 +35,14 @@ struct CpuTeam {
   }
 };
 
-template <typename T>
+template <typename T, bool Pairs>
 int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
   using namespace rtb;
   std::vector<cplx<T>> P(static_cast<size_t>(n) * (n + 1) / 2), v(n), w(n);
   std::vector<T> d(n), e(n), e2(n), Ev(n), red(nt + 64);
   std::vector<cplx<T>> Vg(static_cast<size_t>(n) * n), taug(n);
   std::vector<T> Y(static_cast<size_t>(n) * n), S(static_cast<size_t>(n) * n * 3);
-  std::vector<cplx<T>> Zb(static_cast<size_t>(n) * n), vs(n), sv(n);
+  std::vector<cplx<T>> Zb(static_cast<size_t>(n) * n), vs(2 * n), part(nt > n ? nt : n), sv(n);
   for (int r = 0; r < n; ++r)
     for (int c = 0; c <= r; ++c) {
       const double* h = H + (static_cast<size_t>(r) * n + c) * 2;
@@ -40,7 +50,7 @@ int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
     }
   std::barrier<> bar(nt);
   auto body = [&](int id) {
-    CpuTeam tm{id, nt, &bar};
+    CpuTeam<Pairs> tm{id, nt, &bar};
     tridiagonalize<T>(tm, n, P.data(), v.data(), w.data(), d.data(), e.data(), red.data(), Vg.data(),
                       taug.data());
     bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data());
@@ -48,7 +58,7 @@ int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
       tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, S.data());
       for (int f = id; f < n * n; f += nt) Zb[f] = {Y[f], T(0)};
       tm.sync();
-      apply_q<T>(tm, n, n, Zb.data(), Vg.data(), taug.data(), vs.data(), sv.data());
+      apply_q<T>(tm, n, n, Zb.data(), Vg.data(), taug.data(), vs.data(), part.data(), sv.data());
     }
   };
   std::vector<std::thread> th;
@@ -66,5 +76,8 @@ int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
 }  // namespace
 
 extern "C" int emul_eigh(int n, int nt, int fp32, const double* H, double* E, double* Z, int vectors) {
-  return fp32 ? run<float>(n, nt, H, E, Z, vectors) : run<double>(n, nt, H, E, Z, vectors);
+  // even team sizes exercise the paired-row matvec used on the GPU
+  if (nt % 2 == 0)
+    return fp32 ? run<float, true>(n, nt, H, E, Z, vectors) : run<double, true>(n, nt, H, E, Z, vectors);
+  return fp32 ? run<float, false>(n, nt, H, E, Z, vectors) : run<double, false>(n, nt, H, E, Z, vectors);
 }

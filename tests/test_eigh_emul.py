@@ -40,7 +40,7 @@ def rand_herm(rng, n):
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 33, 64])
-@pytest.mark.parametrize("nt", [1, 4, 13])
+@pytest.mark.parametrize("nt", [1, 4, 13, 64])
 def test_fp64_matches_zheev(emul, n, nt):
     rng = np.random.default_rng(n * 100 + nt)
     H = rand_herm(rng, n)
