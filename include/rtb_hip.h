@@ -160,6 +160,7 @@ const char* rtb_last_error(void);
  *   RTB_STAGE_BACKTRANS  eigenvector back-transformation kernel
  *   RTB_STAGE_KUBO       Kubo band transforms and Omega
  *   RTB_STAGE_REDUCE     mesh points, occupation/DOS reductions
+ *   RTB_STAGE_REFINE     FP64 refinement of FP32 eigenpairs (mixed form)
  *   RTB_STAGE_TOTAL      first to last event of the call
  * ms[] in milliseconds; launches[] kernel launches per stage. */
 #define RTB_STAGE_FOURIER 0
@@ -167,10 +168,11 @@ const char* rtb_last_error(void);
 #define RTB_STAGE_BACKTRANS 2
 #define RTB_STAGE_KUBO 3
 #define RTB_STAGE_REDUCE 4
-#define RTB_STAGE_TOTAL 5
+#define RTB_STAGE_REFINE 5
+#define RTB_STAGE_TOTAL 6
 typedef struct rtb_stats {
-  double ms[6];
-  int launches[6];
+  double ms[7];
+  int launches[7];
   size_t nk;
   int nsta;
 } rtb_stats;

@@ -56,7 +56,7 @@ class SynthSpec(ctypes.Structure):
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [("ms", c_double * 6), ("launches", c_int * 6), ("nk", c_size_t), ("nsta", c_int)]
+    _fields_ = [("ms", c_double * 7), ("launches", c_int * 7), ("nk", c_size_t), ("nsta", c_int)]
 
 
 _P = c_void_p
@@ -121,4 +121,4 @@ def last_stats() -> dict:
     return {"ms": list(s.ms), "launches": list(s.launches), "nk": s.nk, "nsta": s.nsta}
 
 
-STAGES = ["fourier", "eigh", "backtransform", "kubo", "reduce", "total"]
+STAGES = ["fourier", "eigh", "backtransform", "kubo", "reduce", "refine", "total"]

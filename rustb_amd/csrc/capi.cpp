@@ -269,6 +269,30 @@ int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
   return RTB_OK;
 }
 
+// Mixed-form eigenpairs of a chunk: FP32 Householder / bisection / inverse
+// iteration / back-transformation (the FP64 buffers of `eb` reused as FP32
+// storage), then FP64 refinement against the FP64 H.  Z: columns.
+int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, double* E, EigBufs& eb, double* Z,
+                    double* W1, double* W2, double* W3) {
+  float* Vg = reinterpret_cast<float*>(eb.Vg);
+  float* tau = reinterpret_cast<float*>(eb.tau);
+  float* Y = reinterpret_cast<float*>(eb.Y);
+  float* S = reinterpret_cast<float*>(eb.S);
+  float* DE = reinterpret_cast<float*>(eb.DE);
+  RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
+  RTB_TRY(eigh_launch_fp32(st, n, c, H, E, true, Vg, tau, Y, S, DE));
+  RTB_TRY(tmr.end(2));
+  RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
+  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, Vg, tau, Z, false));
+  RTB_TRY(tmr.end(1));
+  // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
+  // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius
+  RTB_TRY(tmr.begin(RTB_STAGE_REFINE));
+  RTB_TRY(refine_eigenpairs(st, n, c, H, Z, W1, W2, W3, E, 1e-3));
+  RTB_TRY(tmr.end(8));
+  return RTB_OK;
+}
+
 // Kubo for a chunk: V = [K][2][n][n]; Z = [K][n][n] (columns); omega [K][n]
 //   A = Z^H J Z, B = Z^H v_b Z (response/traits.rs:177-180)
 int kubo_chunk(hipStream_t st, const rtb_model* m, size_t K, const double* V, const double* Z, const double* E,
@@ -458,15 +482,22 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
   RTB_TRY(upload_dirs(st, w, m, nullptr, 0, a));
   a.gauge = RTB_GAUGE_ATOM;  // solve_band_onek / solve_onek use Gauge::Atom
   const size_t n2 = static_cast<size_t>(n) * n;
-  const size_t per_k = 24 + n2 * 16 * 2 + n * 24 + (vec ? n2 * 32 + n2 * 16 : 0);
+  // the mixed form runs the FP32 route (with FP64 refinement) when
+  // eigenvectors are requested; eigenvalue-only solves stay FP64
+  const bool f32 = vec && precision == RTB_PREC_MIXED;
+  const size_t per_k = 24 + n2 * 16 * 2 + n * 24 + (vec ? n2 * 32 + n2 * 16 : 0) + (f32 ? n2 * 32 : 0);
   const size_t K = chunk_for(nk, per_k);
-  double *dk, *dH, *dE, *dZ = nullptr;
+  double *dk, *dH, *dE, *dZ = nullptr, *dZc = nullptr, *dHZ = nullptr;
   EigBufs eb;
   RTB_TRY(w.alloc(&dk, K * 3));
   RTB_TRY(w.alloc(&dH, K * n2 * 2));
   RTB_TRY(w.alloc(&dE, K * n));
   RTB_TRY(alloc_eig(w, n, K, vec, eb));
   if (vec) RTB_TRY(w.alloc(&dZ, K * n2 * 2));
+  if (f32) {
+    RTB_TRY(w.alloc(&dZc, K * n2 * 2));
+    RTB_TRY(w.alloc(&dHZ, K * n2 * 2));
+  }
   std::vector<double> kp;
   for (size_t k0 = 0; k0 < nk; k0 += K) {
     const size_t c = std::min(K, nk - k0);
@@ -477,13 +508,19 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
     RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
     RTB_TRY(tmr.end(1));
-    RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
-    RTB_TRY(tmr.end(1));
-    if (vec) {
-      RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, true));
+    if (f32) {
+      // W2, W3: the reflector and inverse-iteration scratch, dead by then
+      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S));
+      RTB_TRY(transpose_batched(st, n, c, dZc, dZ));
+    } else {
+      RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
+      RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
       RTB_TRY(tmr.end(1));
+      if (vec) {
+        RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
+        RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, true));
+        RTB_TRY(tmr.end(1));
+      }
     }
     RTB_HIP(hipMemcpyAsync(E + k0 * n, dE, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
     if (vec)
@@ -575,7 +612,9 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   const size_t n2 = static_cast<size_t>(n) * n;
   const bool spinj = m->spin && j.spin_dir != RTB_SPIN_NONE;
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
-  const size_t per_k = 24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 48;
+  const bool f32 = j.precision == RTB_PREC_MIXED;  // FP32 eigensolve + FP64 refinement
+  const size_t per_k =
+      24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 48 + (f32 ? n2 * 16 : 0);
   const size_t K = chunk_for(j.nk, per_k);
   MeshPlan mp;
   double* G = nullptr;
@@ -592,7 +631,11 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   double* kub;
   RTB_TRY(w.alloc(&kub, K * n2 * 6));  // T1, A, B (complex)
   if (spinj) RTB_TRY(w.alloc(&dJ, K * n2 * 2));
-  dZ = dH;  // H is consumed by stage 1 before Z is written
+  if (f32) {
+    RTB_TRY(w.alloc(&dZ, K * n2 * 2));  // the refinement needs H after the solve
+  } else {
+    dZ = dH;  // H is consumed by the Householder kernel before Z is written
+  }
   double* T1 = kub;
   double* A = kub + K * n2 * 2;
   double* B = kub + K * n2 * 4;
@@ -630,12 +673,16 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
       RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
       RTB_TRY(tmr.end(1));
     }
-    RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
-    RTB_TRY(tmr.end(1));
-    RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-    RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, false));
-    RTB_TRY(tmr.end(1));
+    if (f32) {
+      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZ, T1, A, B));
+    } else {
+      RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
+      RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
+      RTB_TRY(tmr.end(1));
+      RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
+      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, false));
+      RTB_TRY(tmr.end(1));
+    }
     RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
     RTB_TRY(kubo_chunk(st, m, c, dV, dZ, dE, j.spin_dir, j.eta, T1, A, B, dJ, dOm));
     RTB_TRY(tmr.end(5 + (spinj ? 1 : 0)));
@@ -692,6 +739,7 @@ int rtb_berry_curvature(const rtb_model* m, const double* k, size_t nk, const do
   j.mode = 0;
   j.omega = omega;
   j.E = E;
+  j.precision = precision;
   return berry_impl(m, j);
 }
 
@@ -717,6 +765,7 @@ int rtb_occupied_berry_curvature(const rtb_model* m, const double* k, size_t nk,
   j.mu0 = mu;
   j.T = T;
   j.occ = out;
+  j.precision = precision;
   return berry_impl(m, j);
 }
 

@@ -167,6 +167,46 @@ int eigh_set_smem_limits() {
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_backtransform_kernel<double>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_tridiag_kernel<float>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_backtransform_kernel<float>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  return RTB_OK;
+}
+
+// FP32 route of the mixed form: same kernels instantiated for float.
+int eigh_launch_fp32(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors, float* Vg,
+                     float* taug, float* Y, float* S, float* DE) {
+  if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
+  if (nmat == 0) return RTB_OK;
+  float* Dg = DE;
+  float* Eg = DE + nmat * n;
+  hipLaunchKernelGGL((eigh_tridiag_kernel<float>), dim3(static_cast<unsigned>(nmat)), dim3(256),
+                     tri_smem(n, sizeof(float)), st, n, H, Dg, Eg, reinterpret_cast<cplx<float>*>(Vg),
+                     reinterpret_cast<cplx<float>*>(taug));
+  RTB_HIP(hipGetLastError());
+  const int per = kTeBlock / n;
+  const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
+  if (vectors) {
+    hipLaunchKernelGGL((eigh_trieig_kernel<float, true>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg, E,
+                       Y, S);
+  } else {
+    hipLaunchKernelGGL((eigh_trieig_kernel<float, false>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg,
+                       E, Y, S);
+  }
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* Vg, const float* taug,
+                            double* Z, bool rows_out) {
+  if (nmat == 0) return RTB_OK;
+  const int cb = n <= 128 ? (n <= 64 ? n : 64) : 32;
+  dim3 grid(static_cast<unsigned>(nmat), static_cast<unsigned>((n + cb - 1) / cb));
+  hipLaunchKernelGGL((eigh_backtransform_kernel<float>), grid, dim3(256), bt_smem(n, cb, sizeof(float)), st, n, cb,
+                     Y, reinterpret_cast<const cplx<float>*>(Vg), reinterpret_cast<const cplx<float>*>(taug), Z,
+                     rows_out ? 1 : 0);
+  RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
 

@@ -13,6 +13,19 @@ int eigh_set_smem_limits();
 //   DE: [2][nmat][n] tridiagonal scratch
 int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors,
                      double* Vg, double* taug, double* Y, double* S, double* DE);
+// FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
+int eigh_launch_fp32(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors, float* Vg,
+                     float* taug, float* Y, float* S, float* DE);
+int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* Vg, const float* taug,
+                            double* Z, bool rows_out);
+// FP64 refinement of FP32 eigenpairs (refine.hip): one Ogita-Aishima step,
+// Rayleigh quotients, unit columns, Rayleigh-Ritz in clusters with gaps
+// below gap_rel * max(1, spectral radius).  Z: [nk][n][n] columns (in/out);
+// H is overwritten; W1..W3 scratch of the same size.
+int refine_eigenpairs(hipStream_t st, int n, size_t nk, double* H, double* Z, double* W1, double* W2, double* W3,
+                      double* E, double gap_rel);
+// out[k][r][c] = Z[k][c][r] (eigenvector columns to rows)
+int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, double* out);
 // Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* Vg,
                             const double* taug, double* Z, bool rows_out);

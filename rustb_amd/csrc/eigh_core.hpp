@@ -100,7 +100,14 @@ struct EighSmem {
 template <typename T, typename Team>
 RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, T* d, T* e,
                            T* red, cplx<T>* Vg, cplx<T>* taug) {
+  // Four barriers per column: the norm reduction, the reflector broadcast,
+  // the dot reduction (which also publishes p), and the end of the rank-2
+  // update.  Each reduction has its own scratch, and every scratch is
+  // rewritten only after at least one further barrier.
   const int id = tm.id, nt = tm.nt;
+  T* red_norm = red;
+  T* red_dot = red + Team::kRedStride;
+  T* red_pair = red + 3 * Team::kRedStride;
   for (int j = 0; j + 1 < n; ++j) {
     // zlarfg on alpha = A(j+1, j), x = A(j+2:n, j)
     T part = T(0);
@@ -108,7 +115,7 @@ RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, 
       cplx<T> x = P[pidx(r, j)];
       part += x.re * x.re + x.im * x.im;
     }
-    const T xnorm2 = tm.reduce_sum(part, red);
+    const T xnorm2 = tm.reduce_sum_ns(part, red_norm);
     const cplx<T> alpha = P[pidx(j + 1, j)];
     cplx<T> tau = {T(0), T(0)};
     cplx<T> scale = {T(0), T(0)};
@@ -142,9 +149,9 @@ RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, 
     }
     tm.sync();
     if (tau.re != T(0) || tau.im != T(0)) {
-      // p = tau * A22 v (rows j+1..n-1) and the partial of p^H v.  With
-      // Team::kPairs two threads share a row (even/odd columns) and combine
-      // through pair_sum; the trip count is uniform across the team.
+      // p = tau * A22 v (rows j+1..n-1) into w, and the partial of p^H v.
+      // With Team::kPairs two threads share a row (even/odd columns) and
+      // combine through pair_sum; the trip count is uniform across the team.
       cplx<T> dot = {T(0), T(0)};
       const int half = Team::kPairs ? (id & 1) : 0;
       const int cstep = Team::kPairs ? 2 : 1;
@@ -174,8 +181,8 @@ RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, 
           }
         }
         if (Team::kPairs) {
-          s.re = tm.pair_sum(s.re, red);
-          s.im = tm.pair_sum(s.im, red);
+          s.re = tm.pair_sum(s.re, red_pair);
+          s.im = tm.pair_sum(s.im, red_pair);
         }
         if (act && half == 0) {
           cplx<T> p = cmul(tau, s);
@@ -185,30 +192,44 @@ RTB_HD void tridiagonalize(Team& tm, int n, cplx<T>* P, cplx<T>* v, cplx<T>* w, 
           dot.im += pv.im;
         }
       }
-      const T dre = tm.reduce_sum(dot.re, red);
-      const T dim_ = tm.reduce_sum(dot.im, red);
-      // alpha2 = -1/2 tau (p^H v);  w = p + alpha2 v
+      T dre = dot.re, dim_ = dot.im;
+      tm.reduce_sum2_ns(dre, dim_, red_dot);
+      // alpha2 = -1/2 tau (p^H v); the update uses w = p + alpha2 v on the fly
       cplx<T> a2 = cmul(tau, cplx<T>{dre, dim_});
       a2.re *= T(-0.5);
       a2.im *= T(-0.5);
-      for (int r = j + 1 + id; r < n; r += nt) {
-        cplx<T> t = cmul(a2, v[r]);
-        w[r].re += t.re;
-        w[r].im += t.im;
-      }
-      tm.sync();
       // A22 -= v w^H + w v^H on the packed lower triangle (rows j+1..n-1),
       // row index tracked incrementally along each thread's strided walk
       const int f0 = pidx(j + 1, 0), f1 = n * (n + 1) / 2;
       int r = j + 1, rs = f0;  // current row and its packed start
+      cplx<T> vr = v[r], wr = w[r];
+      {
+        cplx<T> t = cmul(a2, vr);
+        wr.re += t.re;
+        wr.im += t.im;
+      }
       for (int f = f0 + id; f < f1; f += nt) {
+        bool moved = false;
         while (f >= rs + r + 1) {
           rs += r + 1;
           ++r;
+          moved = true;
+        }
+        if (moved) {
+          vr = v[r];
+          wr = w[r];
+          cplx<T> t = cmul(a2, vr);
+          wr.re += t.re;
+          wr.im += t.im;
         }
         const int c = f - rs;
         if (c <= j) continue;
-        cplx<T> vr = v[r], wr = w[r], vc = v[c], wc = w[c];
+        cplx<T> vc = v[c], wc = w[c];
+        {
+          cplx<T> t = cmul(a2, vc);
+          wc.re += t.re;
+          wc.im += t.im;
+        }
         cplx<T> t1 = cmul(vr, conjc(wc));
         cplx<T> t2 = cmul(wr, conjc(vc));
         P[f].re -= t1.re + t2.re;

@@ -6,7 +6,8 @@
 namespace rtb {
 
 struct GpuTeam {
-  static constexpr bool kPairs = true;  // lanes (2i, 2i+1) share rows
+  static constexpr bool kPairs = true;   // lanes (2i, 2i+1) share rows
+  static constexpr int kRedStride = 32;  // >= 2 x the waves of a workgroup
   int id, nt;
   __device__ GpuTeam() : id(threadIdx.x), nt(blockDim.x) {}
   __device__ void sync() { __syncthreads(); }
@@ -30,6 +31,43 @@ struct GpuTeam {
     for (int w = 0; w < nw; ++w) s += red[w];
     __syncthreads();
     return s;
+  }
+
+  // The same sum without a trailing barrier: `red` must not be rewritten
+  // until every thread has passed another barrier.
+  template <typename T>
+  __device__ T reduce_sum_ns(T x, T* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    const int wave = id >> 6, nw = (nt + 63) >> 6;
+    if ((id & 63) == 0) red[wave] = x;
+    __syncthreads();
+    T s = T(0);
+    for (int w = 0; w < nw; ++w) s += red[w];
+    return s;
+  }
+
+  // Two sums at once (no trailing barrier), red holds interleaved pairs.
+  template <typename T>
+  __device__ void reduce_sum2_ns(T& x, T& y, T* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      x += __shfl_xor(x, off, 64);
+      y += __shfl_xor(y, off, 64);
+    }
+    const int wave = id >> 6, nw = (nt + 63) >> 6;
+    if ((id & 63) == 0) {
+      red[2 * wave] = x;
+      red[2 * wave + 1] = y;
+    }
+    __syncthreads();
+    T sx = T(0), sy = T(0);
+    for (int w = 0; w < nw; ++w) {
+      sx += red[2 * w];
+      sy += red[2 * w + 1];
+    }
+    x = sx;
+    y = sy;
   }
 };
 

@@ -1,0 +1,168 @@
+// FP64 refinement of FP32 eigenpairs (mixed form).
+//
+// One Ogita-Aishima step (Ogita & Aishima, Japan J. Indust. Appl. Math. 35,
+// 2018: iterative refinement for symmetric eigendecompositions) on FP64
+// matrix cores lifts the FP32 eigenvectors to FP64 accuracy outside
+// clusters; then, with HZ = H Z in FP64:
+//   * every column is renormalised and its eigenvalue replaced by the FP64
+//     Rayleigh quotient  lambda_j = Re(z_j^H H z_j) / (z_j^H z_j), whose error
+//     is quadratic in the FP32 eigenvector error;
+//   * inside clusters of close eigenvalues (where the FP32 vectors mix the
+//     members), an FP64 Rayleigh-Ritz step: orthonormalise the cluster's
+//     columns (Cholesky of the Gram matrix), diagonalise the projected
+//     Hermitian matrix by cyclic Jacobi, rotate the columns.
+#include "eigh.hpp"
+#include "refine_core.hpp"
+
+namespace rtb {
+
+namespace {
+
+
+__global__ void rayleigh_kernel(int n, size_t nk, double2* __restrict__ Z, double2* __restrict__ HZ,
+                                double* __restrict__ E) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= nk * n) return;
+  const size_t k = g / n;
+  const int j = static_cast<int>(g % n);
+  double2* Zk = Z + k * n * n;
+  double2* Hk = HZ + k * n * n;
+  double num = 0.0, den = 0.0;
+  for (int r = 0; r < n; ++r) {
+    const double2 z = Zk[static_cast<size_t>(r) * n + j];
+    const double2 h = Hk[static_cast<size_t>(r) * n + j];
+    num += z.x * h.x + z.y * h.y;  // Re(conj(z) h)
+    den += z.x * z.x + z.y * z.y;
+  }
+  E[g] = num / den;
+  const double s = 1.0 / sqrt(den);
+  for (int r = 0; r < n; ++r) {  // keep HZ = H Z for the unit columns
+    double2& z = Zk[static_cast<size_t>(r) * n + j];
+    z.x *= s;
+    z.y *= s;
+    double2& h = Hk[static_cast<size_t>(r) * n + j];
+    h.x *= s;
+    h.y *= s;
+  }
+}
+
+// One wave per k-point: the lanes test every eigenvalue gap in parallel; only
+// a k-point with a gap below the threshold runs the serial Rayleigh-Ritz
+// (lane 0, small matrices in LDS).  HZ = H Z for the unit columns.
+__global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restrict__ Z,
+                                                        const double2* __restrict__ HZ, double* __restrict__ E,
+                                                        double gap_rel) {
+  __shared__ C2 lds[7][kMaxCluster][kMaxCluster];
+  const size_t k = blockIdx.x;
+  double* Ek = E + k * n;
+  double m = 0.0;
+  for (int j = threadIdx.x; j < n; j += 64) m = fmax(m, fabs(Ek[j]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  const double thr = gap_rel * fmax(m, 1.0);
+  int close = 0;
+  for (int j = threadIdx.x; j + 1 < n; j += 64) close |= (Ek[j + 1] - Ek[j] <= thr);
+  if (!__any(close)) return;
+  if (threadIdx.x != 0) return;
+  cluster_rr_one(n, reinterpret_cast<Cd*>(Z + k * n * n), reinterpret_cast<const Cd*>(HZ + k * n * n), Ek, gap_rel,
+                 lds);
+}
+
+// One workgroup per k-point: the Rayleigh-like diagonal lt_i = S_ii / G_ii and
+// the spectral scale max |lt_i| are formed once in LDS, then every element
+// of the correction matrix F' is written.
+__global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2* __restrict__ S,
+                                                            const double2* __restrict__ G, double2* __restrict__ F,
+                                                            double gap_rel) {
+  __shared__ double lt[kMaxEighN];
+  __shared__ double gd[kMaxEighN];
+  __shared__ double red[4];
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const size_t k = blockIdx.x;
+  const double2* Sk = S + k * n2;
+  const double2* Gk = G + k * n2;
+  double2* Fk = F + k * n2;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double g = Gk[static_cast<size_t>(i) * n + i].x;
+    gd[i] = g;
+    lt[i] = Sk[static_cast<size_t>(i) * n + i].x / g;
+    m = fmax(m, fabs(lt[i]));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  const double scale = fmax(fmax(fmax(red[0], red[1]), fmax(red[2], red[3])), 1.0);
+  const double thr = gap_rel * scale;
+  for (size_t e = threadIdx.x; e < n2; e += 256) {
+    const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
+    const double2 s = Sk[e], gg = Gk[e];
+    double2 f;
+    if (i == j) {
+      f = make_double2(1.0 + 0.5 * (1.0 - gd[i]), 0.0);
+    } else if (fabs(lt[i] - lt[j]) > thr) {
+      const double inv = 1.0 / (lt[j] - lt[i]);
+      f = make_double2((s.x - lt[j] * gg.x) * inv, (s.y - lt[j] * gg.y) * inv);
+    } else {
+      f = make_double2(-0.5 * gg.x, -0.5 * gg.y);
+    }
+    Fk[e] = f;
+  }
+}
+
+__global__ void transpose_kernel(int n, size_t nk, const double2* __restrict__ Z, double2* __restrict__ out) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  if (g >= nk * n2) return;
+  const size_t k = g / n2, e = g % n2;
+  const size_t r = e / n, c = e % n;  // out[k][r][c] = Z[k][c][r]
+  out[g] = Z[k * n2 + c * n + r];
+}
+
+}  // namespace
+
+int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, double* out) {
+  const size_t tot = nk * static_cast<size_t>(n) * n;
+  if (tot == 0) return RTB_OK;
+  hipLaunchKernelGGL(transpose_kernel, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, st, n, nk,
+                     reinterpret_cast<const double2*>(Z), reinterpret_cast<double2*>(out));
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+// One FP64 Ogita-Aishima refinement step of the FP32 eigenpairs, then Rayleigh
+// quotients and Rayleigh-Ritz inside clusters:
+//   HZ = H Z, S = Z^H HZ, G = Z^H Z, lt_i = S_ii / G_ii,
+//   F'_ij = (S_ij - lt_j G_ij) / (lt_j - lt_i)  (i != j, |lt_i - lt_j| > thr)
+//         = -G_ij / 2                           (i != j, inside a cluster)
+//   F'_ii = 1 + (1 - G_ii) / 2,
+//   Z' = Z F', HZ' = HZ F'  (the FP32 vector error enters squared).
+// Buffers (each [nk][n][n] complex): H in/out scratch, Z in/out, W1..W3.
+int refine_eigenpairs(hipStream_t st, int n, size_t nk, double* H, double* Z, double* W1, double* W2, double* W3,
+                      double* E, double gap_rel) {
+  if (nk == 0) return RTB_OK;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  RTB_TRY(zgemm_batched(st, n, nk, H, n2, false, Z, n2, W1, n2));   // HZ
+  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, W1, n2, W2, n2));   // S
+  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, Z, n2, W3, n2));    // G
+  const size_t tot = nk * n2;
+  hipLaunchKernelGGL(oa_correction_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
+                     reinterpret_cast<const double2*>(W2), reinterpret_cast<const double2*>(W3),
+                     reinterpret_cast<double2*>(H), gap_rel);  // F' into H (H is dead)
+  RTB_HIP(hipGetLastError());
+  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, false, H, n2, W2, n2));   // Z' = Z F'
+  RTB_TRY(zgemm_batched(st, n, nk, W1, n2, false, H, n2, W3, n2));  // HZ' = HZ F'
+  const size_t tk = nk * n;
+  hipLaunchKernelGGL(rayleigh_kernel, dim3(static_cast<unsigned>((tk + 255) / 256)), dim3(256), 0, st, n, nk,
+                     reinterpret_cast<double2*>(W2), reinterpret_cast<double2*>(W3), E);
+  RTB_HIP(hipGetLastError());
+  hipLaunchKernelGGL(cluster_rr_kernel, dim3(static_cast<unsigned>(nk)), dim3(64), 0, st, n,
+                     reinterpret_cast<double2*>(W2),
+                     reinterpret_cast<const double2*>(W3), E, gap_rel);
+  RTB_HIP(hipGetLastError());
+  RTB_HIP(hipMemcpyAsync(Z, W2, tot * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  return RTB_OK;
+}
+
+}  // namespace rtb

@@ -13,6 +13,7 @@ namespace {
 template <bool Pairs>
 struct CpuTeam {
   static constexpr bool kPairs = Pairs;
+  static constexpr int kRedStride = 2048;  // >= 2 x the largest emulated team
   int id, nt;
   std::barrier<>* bar;
   void sync() { bar->arrive_and_wait(); }
@@ -23,6 +24,27 @@ struct CpuTeam {
     T y = x + red[id ^ 1];
     sync();
     return y;
+  }
+  template <typename T>
+  T reduce_sum_ns(T x, T* red) {
+    red[id] = x;
+    sync();
+    T s = T(0);
+    for (int i = 0; i < nt; ++i) s += red[i];
+    return s;
+  }
+  template <typename T>
+  void reduce_sum2_ns(T& x, T& y, T* red) {
+    red[2 * id] = x;
+    red[2 * id + 1] = y;
+    sync();
+    T sx = T(0), sy = T(0);
+    for (int i = 0; i < nt; ++i) {
+      sx += red[2 * i];
+      sy += red[2 * i + 1];
+    }
+    x = sx;
+    y = sy;
   }
   template <typename T>
   T reduce_sum(T x, T* red) {
@@ -39,7 +61,7 @@ template <typename T, bool Pairs>
 int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
   using namespace rtb;
   std::vector<cplx<T>> P(static_cast<size_t>(n) * (n + 1) / 2), v(n), w(n);
-  std::vector<T> d(n), e(n), e2(n), Ev(n), red(nt + 64);
+  std::vector<T> d(n), e(n), e2(n), Ev(n), red(4 * 2048 + nt);
   std::vector<cplx<T>> Vg(static_cast<size_t>(n) * n), taug(n);
   std::vector<T> Y(static_cast<size_t>(n) * n), S(static_cast<size_t>(n) * n * 3);
   std::vector<cplx<T>> Zb(static_cast<size_t>(n) * n), vs(2 * n), part(nt > n ? nt : n), sv(n);
