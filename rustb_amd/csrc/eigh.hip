@@ -8,6 +8,7 @@
 #include "eigh.hpp"
 
 #include "eigh_core.hpp"
+#include "eigh_reg.hpp"
 #include "gpu_team.hpp"
 
 namespace rtb {
@@ -71,6 +72,90 @@ __global__ void __launch_bounds__(256) eigh_tridiag_kernel(int n, const double* 
   for (int i = tm.id; i < n; i += tm.nt) {
     Dg[mat * n + i] = s.d[i];
     Eg[mat * n + i] = s.e[i];
+  }
+}
+
+// Register-resident tridiagonalization: one workgroup of NB * S threads per
+// matrix (n <= NB), each thread holding NB / S entries of one row.
+template <typename T, int NB, int S>
+__global__ void __launch_bounds__(NB * S) eigh_tridiag_reg_kernel(int n, const double* __restrict__ H,
+                                                                  T* __restrict__ Dg, T* __restrict__ Eg,
+                                                                  cplx<T>* __restrict__ Vg,
+                                                                  cplx<T>* __restrict__ taug) {
+  __shared__ cplx<T> xc[NB], vp[NB], wp[NB], yp[S * NB];
+  __shared__ T red[2 * GpuTeam::kRedStride];
+  GpuTeam tm;
+  const size_t mat = blockIdx.x;
+  const size_t nn = static_cast<size_t>(n) * n;
+  RegSmem<T> sm{xc, vp, wp, yp, red};
+  tridiag_reg<T, NB, S>(tm, n, H + mat * nn * 2, sm, Dg + mat * n, Eg + mat * n, Vg ? Vg + mat * nn : nullptr,
+                        taug ? taug + mat * n : nullptr);
+}
+
+// Register-resident back-transformation Z = Q Y: one workgroup per matrix,
+// each thread holding NB / S entries of one column of Z.
+template <typename T, int NB, int S>
+__global__ void __launch_bounds__(NB * S) eigh_backtrans_reg_kernel(int n, const T* __restrict__ Y,
+                                                                    const cplx<T>* __restrict__ Vg,
+                                                                    const cplx<T>* __restrict__ taug,
+                                                                    double* __restrict__ Zout, int rows_out) {
+  __shared__ cplx<T> vp[3 * NB], yp[2 * S * NB];
+  GpuTeam tm;
+  const size_t mat = blockIdx.x;
+  const size_t nn = static_cast<size_t>(n) * n;
+  RegSmem<T> sm{nullptr, vp, nullptr, yp, nullptr};
+  backtrans_reg<T, NB, S>(tm, n, Y + mat * nn, n, Vg + mat * nn, taug + mat * n, sm, Zout + mat * nn * 2,
+                          rows_out);
+}
+
+// Slice counts: FP32 rows of 128 fit in 2 x 64 complex registers per thread;
+// FP64 rows of 128 are split over 4 slices (32 complex doubles per thread).
+template <typename T, int NB>
+struct RegShape;
+template <> struct RegShape<float, 32> { static constexpr int S = 2; };
+template <> struct RegShape<float, 64> { static constexpr int S = 2; };
+template <> struct RegShape<float, 128> { static constexpr int S = 2; };
+template <> struct RegShape<double, 32> { static constexpr int S = 2; };
+template <> struct RegShape<double, 64> { static constexpr int S = 2; };
+template <> struct RegShape<double, 128> { static constexpr int S = 4; };
+
+template <typename T, int NB>
+void launch_tridiag_reg(hipStream_t st, int n, size_t nmat, const double* H, T* Dg, T* Eg, cplx<T>* Vg,
+                        cplx<T>* taug) {
+  constexpr int S = RegShape<T, NB>::S;
+  hipLaunchKernelGGL((eigh_tridiag_reg_kernel<T, NB, S>), dim3(static_cast<unsigned>(nmat)), dim3(NB * S), 0, st, n,
+                     H, Dg, Eg, Vg, taug);
+}
+
+template <typename T>
+void tridiag_reg_dispatch(hipStream_t st, int n, size_t nmat, const double* H, T* Dg, T* Eg, cplx<T>* Vg,
+                          cplx<T>* taug) {
+  if (n <= 32) {
+    launch_tridiag_reg<T, 32>(st, n, nmat, H, Dg, Eg, Vg, taug);
+  } else if (n <= 64) {
+    launch_tridiag_reg<T, 64>(st, n, nmat, H, Dg, Eg, Vg, taug);
+  } else {
+    launch_tridiag_reg<T, 128>(st, n, nmat, H, Dg, Eg, Vg, taug);
+  }
+}
+
+template <typename T, int NB>
+void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const cplx<T>* Vg, const cplx<T>* taug,
+                          double* Z, int rows_out) {
+  constexpr int S = RegShape<T, NB>::S;
+  hipLaunchKernelGGL((eigh_backtrans_reg_kernel<T, NB, S>), dim3(static_cast<unsigned>(nmat)), dim3(NB * S), 0, st,
+                     n, Y, Vg, taug, Z, rows_out);
+}
+
+template <typename T>
+void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const cplx<T>* Vg, const cplx<T>* taug,
+                            double* Z, int rows_out) {
+  if (n <= 32) {
+    launch_backtrans_reg<T, 32>(st, n, nmat, Y, Vg, taug, Z, rows_out);
+  } else if (n <= 64) {
+    launch_backtrans_reg<T, 64>(st, n, nmat, Y, Vg, taug, Z, rows_out);
+  } else {
+    launch_backtrans_reg<T, 128>(st, n, nmat, Y, Vg, taug, Z, rows_out);
   }
 }
 
@@ -181,9 +266,8 @@ int eigh_launch_fp32(hipStream_t st, int n, size_t nmat, const double* H, double
   if (nmat == 0) return RTB_OK;
   float* Dg = DE;
   float* Eg = DE + nmat * n;
-  hipLaunchKernelGGL((eigh_tridiag_kernel<float>), dim3(static_cast<unsigned>(nmat)), dim3(256),
-                     tri_smem(n, sizeof(float)), st, n, H, Dg, Eg, reinterpret_cast<cplx<float>*>(Vg),
-                     reinterpret_cast<cplx<float>*>(taug));
+  tridiag_reg_dispatch<float>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<float>*>(Vg),
+                              reinterpret_cast<cplx<float>*>(taug));
   RTB_HIP(hipGetLastError());
   const int per = kTeBlock / n;
   const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
@@ -201,11 +285,8 @@ int eigh_launch_fp32(hipStream_t st, int n, size_t nmat, const double* H, double
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* Vg, const float* taug,
                             double* Z, bool rows_out) {
   if (nmat == 0) return RTB_OK;
-  const int cb = n <= 128 ? (n <= 64 ? n : 64) : 32;
-  dim3 grid(static_cast<unsigned>(nmat), static_cast<unsigned>((n + cb - 1) / cb));
-  hipLaunchKernelGGL((eigh_backtransform_kernel<float>), grid, dim3(256), bt_smem(n, cb, sizeof(float)), st, n, cb,
-                     Y, reinterpret_cast<const cplx<float>*>(Vg), reinterpret_cast<const cplx<float>*>(taug), Z,
-                     rows_out ? 1 : 0);
+  backtrans_reg_dispatch<float>(st, n, nmat, Y, reinterpret_cast<const cplx<float>*>(Vg),
+                                reinterpret_cast<const cplx<float>*>(taug), Z, rows_out ? 1 : 0);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
@@ -216,9 +297,8 @@ int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double
   if (nmat == 0) return RTB_OK;
   double* Dg = DE;
   double* Eg = DE + nmat * n;
-  hipLaunchKernelGGL((eigh_tridiag_kernel<double>), dim3(static_cast<unsigned>(nmat)), dim3(256),
-                     tri_smem(n, sizeof(double)), st, n, H, Dg, Eg, reinterpret_cast<cplx<double>*>(Vg),
-                     reinterpret_cast<cplx<double>*>(taug));
+  tridiag_reg_dispatch<double>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<double>*>(Vg),
+                               reinterpret_cast<cplx<double>*>(taug));
   RTB_HIP(hipGetLastError());
   const int per = kTeBlock / n;
   const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
@@ -236,11 +316,8 @@ int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* Vg,
                             const double* taug, double* Z, bool rows_out) {
   if (nmat == 0) return RTB_OK;
-  const int cb = bt_cb(n);
-  dim3 grid(static_cast<unsigned>(nmat), static_cast<unsigned>((n + cb - 1) / cb));
-  hipLaunchKernelGGL((eigh_backtransform_kernel<double>), grid, dim3(256), bt_smem(n, cb, sizeof(double)), st, n,
-                     cb, Y, reinterpret_cast<const cplx<double>*>(Vg), reinterpret_cast<const cplx<double>*>(taug),
-                     Z, rows_out ? 1 : 0);
+  backtrans_reg_dispatch<double>(st, n, nmat, Y, reinterpret_cast<const cplx<double>*>(Vg),
+                                 reinterpret_cast<const cplx<double>*>(taug), Z, rows_out ? 1 : 0);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../rustb_amd/csrc/eigh_core.hpp"
+#include "../../rustb_amd/csrc/eigh_reg.hpp"
 
 namespace {
 
@@ -95,7 +96,52 @@ int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
   return 0;
 }
 
+// Register-resident route: tridiag_reg and backtrans_reg with NB * S threads,
+// the tridiagonal eigenpairs with the team-generic routines in between.
+template <typename T, int NB, int S>
+int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_out) {
+  using namespace rtb;
+  constexpr int NT = NB * S;
+  std::vector<cplx<T>> xc(NB), vp(3 * NB), wp(NB), yp(2 * S * NB);
+  std::vector<T> red(2 * 2048), d(n), e(n), e2(n), Ev(n), red2(4 * 2048 + n);
+  std::vector<cplx<T>> Vg(static_cast<size_t>(n) * n), taug(n);
+  std::vector<T> Y(static_cast<size_t>(n) * n), Sw(static_cast<size_t>(n) * n * 3);
+  RegSmem<T> sm{xc.data(), vp.data(), wp.data(), yp.data(), red.data()};
+  auto team = [](int nt, auto&& f) {
+    std::barrier<> bar(nt);
+    std::vector<std::thread> th;
+    for (int i = 0; i < nt; ++i) th.emplace_back([&, i] {
+        CpuTeam<false> tm{i, nt, &bar};
+        f(tm);
+      });
+    for (auto& t : th) t.join();
+  };
+  team(NT, [&](auto& tm) { tridiag_reg<T, NB, S>(tm, n, H, sm, d.data(), e.data(), Vg.data(), taug.data()); });
+  team(n, [&](auto& tm) {
+    bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data());
+    if (vectors) tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, Sw.data());
+  });
+  for (int j = 0; j < n; ++j) E[j] = static_cast<double>(Ev[j]);
+  if (vectors)
+    team(NT, [&](auto& tm) {
+      backtrans_reg<T, NB, S>(tm, n, Y.data(), n, Vg.data(), taug.data(), sm, Z, rows_out);
+    });
+  return 0;
+}
+
 }  // namespace
+
+extern "C" int emul_eigh_reg(int n, int fp32, const double* H, double* E, double* Z, int vectors, int rows_out) {
+  if (n < 1 || n > 128) return -1;
+  if (fp32) {
+    if (n <= 32) return run_reg<float, 32, 2>(n, H, E, Z, vectors, rows_out);
+    if (n <= 64) return run_reg<float, 64, 2>(n, H, E, Z, vectors, rows_out);
+    return run_reg<float, 128, 2>(n, H, E, Z, vectors, rows_out);
+  }
+  if (n <= 32) return run_reg<double, 32, 2>(n, H, E, Z, vectors, rows_out);
+  if (n <= 64) return run_reg<double, 64, 2>(n, H, E, Z, vectors, rows_out);
+  return run_reg<double, 128, 4>(n, H, E, Z, vectors, rows_out);
+}
 
 extern "C" int emul_eigh(int n, int nt, int fp32, const double* H, double* E, double* Z, int vectors) {
   // even team sizes exercise the paired-row matvec used on the GPU

@@ -31,6 +31,17 @@ def emul(tmp_path_factory):
                       Z.ctypes.data_as(ctypes.c_void_p), vectors)
         return E, Z
 
+    def run_reg(H, fp32=0, rows_out=0):
+        n = H.shape[0]
+        Hc = np.ascontiguousarray(H, np.complex128)
+        E = np.zeros(n)
+        Z = np.zeros((n, n), np.complex128)
+        rc = lib.emul_eigh_reg(n, fp32, Hc.ctypes.data_as(ctypes.c_void_p), E.ctypes.data_as(ctypes.c_void_p),
+                               Z.ctypes.data_as(ctypes.c_void_p), 1, rows_out)
+        assert rc == 0
+        return E, (Z.T if rows_out else Z)
+
+    run.reg = run_reg
     return run
 
 
@@ -83,3 +94,49 @@ def test_fp32_variant(emul):
     E, Z = emul(H, 8, fp32=1)
     assert np.abs(E - w).max() < 5e-5 * np.abs(w).max()
     assert np.abs(Z.conj().T @ Z - np.eye(32)).max() < 1e-4
+
+
+# Register-resident route (tridiag_reg / backtrans_reg in eigh_reg.hpp): the
+# slice shapes used on the GPU (32x2, 64x2, 128x2 FP32; 128x4 FP64) with
+# padding (n < NB), both output layouts, and degenerate spectra.
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 32, 33, 64, 65, 100, 128])
+def test_reg_fp64_matches_zheev(emul, n):
+    rng = np.random.default_rng(n + 7)
+    H = rand_herm(rng, n)
+    w, _, _ = L.zheev(H)
+    E, Z = emul.reg(H, rows_out=n % 2)
+    scale = max(1.0, np.abs(w).max())
+    assert np.abs(E - w).max() <= 4e-15 * scale * n
+    assert np.abs(H @ Z - Z * E).max() <= 4e-15 * scale * n
+    assert np.abs(Z.conj().T @ Z - np.eye(n)).max() <= 1e-14 * n
+
+
+@pytest.mark.parametrize("n", [5, 40, 128])
+def test_reg_fp32_matches_zheev(emul, n):
+    rng = np.random.default_rng(n + 70)
+    H = rand_herm(rng, n)
+    w, _, _ = L.zheev(H)
+    E, Z = emul.reg(H, fp32=1)
+    scale = np.abs(w).max()
+    assert np.abs(E - w).max() < 5e-6 * scale
+    assert np.abs(H @ Z - Z * E).max() < 5e-6 * scale
+    assert np.abs(Z.conj().T @ Z - np.eye(n)).max() < 1e-4
+
+
+@pytest.mark.parametrize("case", ["degenerate", "block", "zero", "diagonal"])
+def test_reg_special_spectra(emul, case):
+    rng = np.random.default_rng(6)
+    if case == "degenerate":
+        H = np.diag([1.0] * 30 + [2.0] * 30 + [3.0] * 10).astype(complex)
+    elif case == "block":
+        H = np.kron(np.eye(9), rand_herm(rng, 8))
+    elif case == "zero":
+        H = np.zeros((70, 70), complex)
+    else:
+        H = np.diag(rng.standard_normal(66)).astype(complex)
+    w, _, _ = L.zheev(H)
+    E, Z = emul.reg(H)
+    n = H.shape[0]
+    assert np.abs(E - w).max() < 1e-13
+    assert np.abs(H @ Z - Z * E).max() < 1e-13
+    assert np.abs(Z.conj().T @ Z - np.eye(n)).max() < 1e-13
