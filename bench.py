@@ -8,7 +8,8 @@ values and sigma cross PCIe).
 
 Prints ONE JSON line.  `value` is k-points/s of the whole job.  `roofline`
 is the dominant kernel's algorithmic work per launch over its average launch
-time, measured with HIP events inside the library on the kernel's stream.
+time, measured with HIP events inside the library on the kernel's stream
+(rtb_last_stats: one event pair per launch group of that kernel).
 `cpu_baseline` is the reference-equivalent CPU path (oracle/cpu_ref.cpp:
 OpenMP over k, MKL/OpenBLAS zheev + zgemm, zaxpy-order Fourier sum) on a
 bounded sample of the same mesh.
@@ -41,22 +42,28 @@ def device_sync():
     hip.hipDeviceSynchronize()
 
 
-def algorithmic_flops(stage, n, nR, d):
-    """Per-k algorithmic work (LAPACK/zaxpy conventions, independent of the
-    algorithm actually run): Fourier 8 N_R n^2 (1+d); eigensolve with vectors
-    (16/3 + 8) n^3 (tridiagonalisation + back-transform share the count);
-    Kubo 16 d n^3."""
-    if stage == "fourier":
-        return 8.0 * nR * n * n * (1 + d)
-    if stage == "eigh":
-        return 16.0 / 3.0 * n**3
-    if stage == "backtransform":
-        return 8.0 * n**3
-    if stage == "kubo":
-        return 16.0 * d * n**3
-    if stage == "refine":  # five n^3 complex GEMMs of the refinement step
-        return 40.0 * n**3
-    return 0.0
+# Stages with a closed-form algorithmic flop count (the library reports the
+# count per launch group in rtb_stats.flop): the kernel each one times, the
+# arithmetic it runs on and that unit's published peak.
+ROOFLINE_STAGES = {
+    "gemm64": ("zgemm3m_kernel<double, false>", "FP64 matrix cores", PEAK_FP64),
+    "gemm32": ("zgemm3m_kernel<float, true>", "FP32 matrix cores", PEAK_FP32),
+    "tridiag": ("eigh_tridiag_reg_kernel", "vector ALU", None),
+    "backtransform": ("eigh_backtrans_reg_kernel", "vector ALU", None),
+}
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), as
+    committed under profiles/; None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec.get("bytes_per_launch")
 
 
 def cpu_baseline(model, kmesh, dirs, eta, target_s):
@@ -140,8 +147,11 @@ def main():
     for _ in range(args.warmup):
         model.hall_conductivity(params)
     device_sync()
-    stage_ms = np.zeros(7)
-    stage_launch = np.zeros(7)
+    names = _lib.STAGES
+    ns = len(names)
+    stage_ms = np.zeros(ns)
+    stage_launch = np.zeros(ns)
+    stage_flop = np.zeros(ns)
     t0 = time.perf_counter()
     sigma = None
     for _ in range(args.steps):
@@ -149,34 +159,36 @@ def main():
         st = _lib.last_stats()
         stage_ms += st["ms"]
         stage_launch += st["launches"]
+        stage_flop += st["flop"]
     device_sync()
     elapsed = time.perf_counter() - t0
     value = nk * args.steps / elapsed
 
-    names = _lib.STAGES
-    dom = int(np.argmax(stage_ms[:6]))
+    # dominant kernel among the stages with an algorithmic flop count
+    cand = [names.index(k) for k in ROOFLINE_STAGES if stage_launch[names.index(k)] > 0]
+    dom = max(cand, key=lambda i: stage_ms[i])
     dname = names[dom]
-    per_launch_ms = stage_ms[dom] / max(stage_launch[dom], 1)
-    launches_per_step = stage_launch[dom] / args.steps
-    k_per_launch = nk / launches_per_step if dname != "kubo" else nk / (launches_per_step / 5.0)
-    if dname == "kubo":  # 4 GEMMs + Omega per chunk; report the per-chunk group
-        per_launch_ms = stage_ms[dom] / max(stage_launch[dom] / 5.0, 1)
-    flops = algorithmic_flops(dname, n, nR, 2) * k_per_launch
-    achieved = flops / (per_launch_ms * 1e-3) / 1e12
-    # the mixed form's eigensolver stages compute in FP32; all others in FP64
-    f32_stage = args.precision == "mixed" and dname in ("eigh", "backtransform")
-    peak = PEAK_FP32 if f32_stage else PEAK_FP64
+    kernel, unit_name, peak = ROOFLINE_STAGES[dname]
+    f32 = args.precision == "mixed" and dname in ("tridiag", "backtransform")
+    if peak is None:  # vector ALU: FP32 in the mixed form, FP64 otherwise
+        peak = PEAK_FP32 if f32 else PEAK_FP64
+    per_launch_ms = stage_ms[dom] / stage_launch[dom]
+    flop_per_launch = stage_flop[dom] / stage_launch[dom]
+    achieved = flop_per_launch / (per_launch_ms * 1e-3) / 1e12
     roofline = {
         "bound": "mfma",
-        "kernel": dname,
         "achieved": round(achieved, 3),
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 4),
-        "traffic": None,
-        "per_launch_ms": round(per_launch_ms, 3),
-        "k_per_launch": int(k_per_launch),
-        "algorithmic_flop_per_k": algorithmic_flops(dname, n, nR, 2),
+        "traffic": traffic_from_profiles(kernel),
+        "kernel": kernel,
+        "stage": dname,
+        "arithmetic": unit_name,
+        "per_launch_ms": round(per_launch_ms, 4),
+        "launches_per_step": int(round(stage_launch[dom] / args.steps)),
+        "algorithmic_flop_per_launch": flop_per_launch,
+        "dominant_stage_by_time": names[int(np.argmax(stage_ms[:ns - 1]))],
     }
     out = {
         "metric": METRIC,
@@ -193,7 +205,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "wannier3d-n128", "nsta": n, "n_R": nR, "kmesh": list(kmesh), "n_k": nk,
                    "precision": args.precision, "eta": eta, "mu": [0.0], "T": 0.0, "outputs": "E, Omega_z, AHC"},
-        "stage_ms_per_step": {names[i]: round(stage_ms[i] / args.steps, 2) for i in range(7)},
+        "stage_ms_per_step": {names[i]: round(stage_ms[i] / args.steps, 2) for i in range(ns)},
         "sigma_xy_mu0": float(sigma),
         "roofline": roofline,
     }
@@ -207,7 +219,7 @@ def main():
         device_sync()
         t64 = time.perf_counter() - t0
         out["fp64_plain"] = {"value": round(nk / t64, 2), "ms_per_step": round(t64 * 1e3, 2),
-                             "stage_ms": {names[i]: round(_lib.last_stats()["ms"][i], 2) for i in range(7)}}
+                             "stage_ms": {names[i]: round(_lib.last_stats()["ms"][i], 2) for i in range(ns)}}
         out["mixed_over_fp64"] = round(value / (nk / t64), 3)
         out["ahc_rel_err_mixed_vs_fp64"] = float(abs(sigma - s64) / abs(s64))
     if not args.no_cpu:

@@ -153,26 +153,38 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
 const char* rtb_last_error(void);
 
 /* Per-kernel device time of the last hot-path call on this thread, measured
- * with HIP events on the stream the kernels ran on.  Stages:
- *   RTB_STAGE_FOURIER    Fourier sum kernel (H and projected velocities)
- *   RTB_STAGE_EIGH       eigensolver kernel (tridiagonalisation, eigenvalues,
- *                        tridiagonal eigenvectors)
+ * with HIP events on the stream the kernels ran on (one event pair around
+ * each launch or group of launches of one kernel kind).  Stages:
+ *   RTB_STAGE_FOURIER    Fourier sum kernels (H and projected velocities)
+ *   RTB_STAGE_TRIDIAG    Householder tridiagonalisation kernel
+ *   RTB_STAGE_TRIEIG     tridiagonal eigenvalues (+ eigenvectors) kernel
  *   RTB_STAGE_BACKTRANS  eigenvector back-transformation kernel
- *   RTB_STAGE_KUBO       Kubo band transforms and Omega
- *   RTB_STAGE_REDUCE     mesh points, occupation/DOS reductions
- *   RTB_STAGE_REFINE     FP64 refinement of FP32 eigenpairs (mixed form)
+ *   RTB_STAGE_GEMM64     FP64 matrix-core products (Kubo transforms and the
+ *                        refinement products)
+ *   RTB_STAGE_GEMM32     FP32 matrix-core correction products (refinement)
+ *   RTB_STAGE_REFINE     other refinement kernels (mixed form)
+ *   RTB_STAGE_KUBO       spin current and the Omega band sum
+ *   RTB_STAGE_REDUCE     mesh points, occupation/DOS reductions, copies
  *   RTB_STAGE_TOTAL      first to last event of the call
- * ms[] in milliseconds; launches[] kernel launches per stage. */
+ * ms[] in milliseconds; launches[] kernel launches per stage; flop[] the
+ * algorithmic floating-point operations of those launches where the stage
+ * has a closed form (complex product 8 n^3, Hermitian product 4 n^2 (n+1),
+ * tridiagonalisation 16/3 n^3, back-transformation 8 n^3; 0 otherwise). */
 #define RTB_STAGE_FOURIER 0
-#define RTB_STAGE_EIGH 1
-#define RTB_STAGE_BACKTRANS 2
-#define RTB_STAGE_KUBO 3
-#define RTB_STAGE_REDUCE 4
-#define RTB_STAGE_REFINE 5
-#define RTB_STAGE_TOTAL 6
+#define RTB_STAGE_TRIDIAG 1
+#define RTB_STAGE_TRIEIG 2
+#define RTB_STAGE_BACKTRANS 3
+#define RTB_STAGE_GEMM64 4
+#define RTB_STAGE_GEMM32 5
+#define RTB_STAGE_REFINE 6
+#define RTB_STAGE_KUBO 7
+#define RTB_STAGE_REDUCE 8
+#define RTB_STAGE_TOTAL 9
+#define RTB_N_STAGES 10
 typedef struct rtb_stats {
-  double ms[7];
-  int launches[7];
+  double ms[RTB_N_STAGES];
+  int launches[RTB_N_STAGES];
+  double flop[RTB_N_STAGES];
   size_t nk;
   int nsta;
 } rtb_stats;

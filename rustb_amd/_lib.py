@@ -56,7 +56,8 @@ class SynthSpec(ctypes.Structure):
 
 
 class Stats(ctypes.Structure):
-    _fields_ = [("ms", c_double * 7), ("launches", c_int * 7), ("nk", c_size_t), ("nsta", c_int)]
+    _fields_ = [("ms", c_double * 10), ("launches", c_int * 10), ("flop", c_double * 10), ("nk", c_size_t),
+                ("nsta", c_int)]
 
 
 _P = c_void_p
@@ -118,7 +119,7 @@ def ptr(a):
 def last_stats() -> dict:
     s = Stats()
     check(load().rtb_last_stats(ctypes.byref(s)))
-    return {"ms": list(s.ms), "launches": list(s.launches), "nk": s.nk, "nsta": s.nsta}
+    return {"ms": list(s.ms), "launches": list(s.launches), "flop": list(s.flop), "nk": s.nk, "nsta": s.nsta}
 
 
-STAGES = ["fourier", "eigh", "backtransform", "kubo", "reduce", "refine", "total"]
+STAGES = ["fourier", "tridiag", "trieig", "backtransform", "gemm64", "gemm32", "refine", "kubo", "reduce", "total"]

@@ -59,61 +59,6 @@ class Stream {
   bool ok_ = false;
 };
 
-// HIP events around the stages of one call (stats for rtb_last_stats).
-class Timer {
- public:
-  explicit Timer(hipStream_t s) : st_(s) {}
-  ~Timer() {
-    for (auto& r : recs_) {
-      (void)hipEventDestroy(r.a);
-      (void)hipEventDestroy(r.b);
-    }
-  }
-  int begin(int stage) {
-    Rec r{stage, nullptr, nullptr, 0};
-    RTB_HIP(hipEventCreate(&r.a));
-    RTB_HIP(hipEventCreate(&r.b));
-    RTB_HIP(hipEventRecord(r.a, st_));
-    recs_.push_back(r);
-    return RTB_OK;
-  }
-  int end(int launches) {
-    recs_.back().launches = launches;
-    RTB_HIP(hipEventRecord(recs_.back().b, st_));
-    return RTB_OK;
-  }
-  // call after the stream is synchronised
-  int collect(size_t nk, int n) {
-    rtb_stats s;
-    std::memset(&s, 0, sizeof(s));
-    s.nk = nk;
-    s.nsta = n;
-    for (auto& r : recs_) {
-      float ms = 0.f;
-      RTB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-      s.ms[r.stage] += ms;
-      s.launches[r.stage] += r.launches;
-    }
-    if (!recs_.empty()) {
-      float ms = 0.f;
-      RTB_HIP(hipEventElapsedTime(&ms, recs_.front().a, recs_.back().b));
-      s.ms[RTB_STAGE_TOTAL] = ms;
-      for (int i = 0; i < RTB_STAGE_TOTAL; ++i) s.launches[RTB_STAGE_TOTAL] += s.launches[i];
-    }
-    g_stats = s;
-    return RTB_OK;
-  }
-
- private:
-  struct Rec {
-    int stage;
-    hipEvent_t a, b;
-    int launches;
-  };
-  hipStream_t st_;
-  std::vector<Rec> recs_;
-};
-
 std::once_flag g_smem_once;
 int g_smem_rc = RTB_OK;
 
@@ -271,7 +216,8 @@ int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
 
 // Mixed-form eigenpairs of a chunk: FP32 Householder / bisection / inverse
 // iteration / back-transformation (the FP64 buffers of `eb` reused as FP32
-// storage), then FP64 refinement against the FP64 H.  Z: columns.
+// storage) into Z, then FP64 refinement against the FP64 H.  The refined
+// eigenvectors (columns) end in W2; Z, W1 and W3 are scratch afterwards.
 int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, double* E, EigBufs& eb, double* Z,
                     double* W1, double* W2, double* W3) {
   float* Vg = reinterpret_cast<float*>(eb.Vg);
@@ -279,42 +225,47 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, doub
   float* Y = reinterpret_cast<float*>(eb.Y);
   float* S = reinterpret_cast<float*>(eb.S);
   float* DE = reinterpret_cast<float*>(eb.DE);
-  RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-  RTB_TRY(eigh_launch_fp32(st, n, c, H, E, true, Vg, tau, Y, S, DE));
-  RTB_TRY(tmr.end(2));
+  RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE));
   RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
   RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, Vg, tau, Z, false));
-  RTB_TRY(tmr.end(1));
+  RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
   // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
   // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius
-  RTB_TRY(tmr.begin(RTB_STAGE_REFINE));
-  RTB_TRY(refine_eigenpairs(st, n, c, H, Z, W1, W2, W3, E, 1e-3));
-  RTB_TRY(tmr.end(8));
+  RTB_TRY(refine_eigenpairs(st, &tmr, n, c, H, Z, W1, W2, W3, E, 1e-3));
   return RTB_OK;
 }
 
 // Kubo for a chunk: V = [K][2][n][n]; Z = [K][n][n] (columns); omega [K][n]
 //   A = Z^H J Z, B = Z^H v_b Z (response/traits.rs:177-180)
-int kubo_chunk(hipStream_t st, const rtb_model* m, size_t K, const double* V, const double* Z, const double* E,
-               int spin_dir, double eta, double* T1, double* A, double* B, double* J, double* omega) {
+int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const double* V, const double* Z,
+               const double* E, int spin_dir, double eta, double* T1, double* A, double* B, double* J,
+               double* omega) {
   const int n = m->nsta;
   const size_t n2 = static_cast<size_t>(n) * n;
   const double* Ja = V;
   size_t sJ = 2 * n2;  // complex elements between consecutive k in V
   if (m->spin && spin_dir != RTB_SPIN_NONE) {
+    RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
     RTB_TRY(spin_current(st, m->norb, K, spin_dir, V, 2 * n2 * 2, J));
+    RTB_TRY(tmr.end(1));
     Ja = J;
     sJ = n2;
   }
+  RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
   RTB_TRY(zgemm_batched(st, n, K, Ja, sJ, false, Z, n2, T1, n2));
-  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, A, n2));
+  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, A, n2, true));
   RTB_TRY(zgemm_batched(st, n, K, V + n2 * 2, 2 * n2, false, Z, n2, T1, n2));
-  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, B, n2));
+  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, B, n2, true));
+  RTB_TRY(tmr.end(4, 2 * flop_zgemm(n, K) + 2 * flop_zgemm_herm(n, K)));
+  RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
   RTB_TRY(kubo_omega(st, n, K, A, B, E, eta, omega));
+  RTB_TRY(tmr.end(1));
   return RTB_OK;
 }
 
 }  // namespace
+
+void publish_stats(const rtb_stats& s) { g_stats = s; }
 
 void set_error(const char* fmt, ...) {
   char buf[1024];
@@ -511,15 +462,15 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     if (f32) {
       // W2, W3: the reflector and inverse-iteration scratch, dead by then
       RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S));
-      RTB_TRY(transpose_batched(st, n, c, dZc, dZ));
-    } else {
-      RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-      RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
+      RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
+      RTB_TRY(transpose_batched(st, n, c, eb.Vg, dZ));
       RTB_TRY(tmr.end(1));
+    } else {
+      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
       if (vec) {
         RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
         RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, true));
-        RTB_TRY(tmr.end(1));
+        RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
       }
     }
     RTB_HIP(hipMemcpyAsync(E + k0 * n, dE, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -673,19 +624,19 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
       RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
       RTB_TRY(tmr.end(1));
     }
+    double* Zc = dZ;  // eigenvectors (columns) for the Kubo products
+    double* Ab = A;   // scratch of the Kubo products
     if (f32) {
       RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZ, T1, A, B));
+      Zc = A;  // refined vectors end in the A buffer; dZ is free
+      Ab = dZ;
     } else {
-      RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-      RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
-      RTB_TRY(tmr.end(1));
+      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
       RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
       RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, false));
-      RTB_TRY(tmr.end(1));
+      RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     }
-    RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
-    RTB_TRY(kubo_chunk(st, m, c, dV, dZ, dE, j.spin_dir, j.eta, T1, A, B, dJ, dOm));
-    RTB_TRY(tmr.end(5 + (spinj ? 1 : 0)));
+    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dE, j.spin_dir, j.eta, T1, Ab, B, dJ, dOm));
     int l2 = 0;
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     if (j.mode == 0) {
@@ -853,9 +804,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
       RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
       RTB_TRY(tmr.end(1));
     }
-    RTB_TRY(tmr.begin(RTB_STAGE_EIGH));
-    RTB_TRY(eigh_launch_fp64(st, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr, eb.DE));
-    RTB_TRY(tmr.end(1));
+    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr, eb.DE));
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     const int nb = dos_blocks(c * n);
     RTB_TRY(dos_partials(st, c * n, dE, Emin, step, static_cast<int>(En), sigma,

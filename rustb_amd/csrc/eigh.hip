@@ -260,15 +260,18 @@ int eigh_set_smem_limits() {
 }
 
 // FP32 route of the mixed form: same kernels instantiated for float.
-int eigh_launch_fp32(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors, float* Vg,
-                     float* taug, float* Y, float* S, float* DE) {
+int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
+                     float* Vg, float* taug, float* Y, float* S, float* DE) {
   if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
   if (nmat == 0) return RTB_OK;
   float* Dg = DE;
   float* Eg = DE + nmat * n;
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIDIAG));
   tridiag_reg_dispatch<float>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<float>*>(Vg),
                               reinterpret_cast<cplx<float>*>(taug));
   RTB_HIP(hipGetLastError());
+  if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIEIG));
   const int per = kTeBlock / n;
   const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
   if (vectors) {
@@ -279,6 +282,7 @@ int eigh_launch_fp32(hipStream_t st, int n, size_t nmat, const double* H, double
                        E, Y, S);
   }
   RTB_HIP(hipGetLastError());
+  if (tmr) RTB_TRY(tmr->end(1));
   return RTB_OK;
 }
 
@@ -291,15 +295,18 @@ int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, 
   return RTB_OK;
 }
 
-int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors, double* Vg,
-                     double* taug, double* Y, double* S, double* DE) {
+int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
+                     double* Vg, double* taug, double* Y, double* S, double* DE) {
   if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
   if (nmat == 0) return RTB_OK;
   double* Dg = DE;
   double* Eg = DE + nmat * n;
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIDIAG));
   tridiag_reg_dispatch<double>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<double>*>(Vg),
                                reinterpret_cast<cplx<double>*>(taug));
   RTB_HIP(hipGetLastError());
+  if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIEIG));
   const int per = kTeBlock / n;
   const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
   if (vectors) {
@@ -310,6 +317,7 @@ int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double
                        E, Y, S);
   }
   RTB_HIP(hipGetLastError());
+  if (tmr) RTB_TRY(tmr->end(1));
   return RTB_OK;
 }
 

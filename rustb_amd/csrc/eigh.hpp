@@ -11,19 +11,20 @@ int eigh_set_smem_limits();
 //   H: [nmat][n][n] complex; E: [nmat][n]; Vg: [nmat][n][n] complex;
 //   taug: [nmat][n] complex; Y: [nmat][n][n]; S: [nmat][3][n][n];
 //   DE: [2][nmat][n] tridiagonal scratch
-int eigh_launch_fp64(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors,
+int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
                      double* Vg, double* taug, double* Y, double* S, double* DE);
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
-int eigh_launch_fp32(hipStream_t st, int n, size_t nmat, const double* H, double* E, bool vectors, float* Vg,
-                     float* taug, float* Y, float* S, float* DE);
+int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
+                     float* Vg, float* taug, float* Y, float* S, float* DE);
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* Vg, const float* taug,
                             double* Z, bool rows_out);
 // FP64 refinement of FP32 eigenpairs (refine.hip): one Ogita-Aishima step,
 // Rayleigh quotients, unit columns, Rayleigh-Ritz in clusters with gaps
-// below gap_rel * max(1, spectral radius).  Z: [nk][n][n] columns (in/out);
-// H is overwritten; W1..W3 scratch of the same size.
-int refine_eigenpairs(hipStream_t st, int n, size_t nk, double* H, double* Z, double* W1, double* W2, double* W3,
-                      double* E, double gap_rel);
+// below gap_rel * max(1, spectral radius).  Z: [nk][n][n] columns (in, then
+// scratch); H is overwritten; the refined columns are written to W2; W1, W3
+// scratch of the same size.
+int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, double* Z, double* W1, double* W2,
+                      double* W3, double* E, double gap_rel);
 // out[k][r][c] = Z[k][c][r] (eigenvector columns to rows)
 int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, double* out);
 // Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
@@ -63,15 +64,24 @@ int mesh_set_smem_limits();
 // Uniform mesh points k = (i/N1, j/N2, l/N3) for flat indices [k0, k0 + nk).
 int mesh_points(hipStream_t st, int dim, const size_t* kmesh, size_t k0, size_t nk, double* k);
 
-// ---- Kubo band sum, reductions, DOS (kubo.hip) ----
+// ---- batched complex products (zgemm.hip) ----
 // C[b] = op(A[b]) * B[b], n x n complex; conj_a: op = conjugate transpose.
-// sA, sB, sC: batch strides in complex elements.
+// sA, sB, sC: batch strides in complex elements.  herm: the product is known
+// to be Hermitian; the upper triangle is filled from the lower one.
 int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
-                  size_t sB, double* C, size_t sC);
+                  size_t sB, double* C, size_t sC, bool herm = false);
+// C[b] = A[b] + A[b] * B[b] with the product on the FP32 matrix cores (FP64
+// storage): for small corrections B, whose product needs ~1e-7 relative
+// accuracy only.  C must not alias A or B.
+int zgemm_add_f32_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, const double* B, size_t sB,
+                          double* C, size_t sC);
+
+// ---- Kubo band sum, reductions, DOS (kubo.hip) ----
 // J = (S v + v S) / 2 for the spin matrix S = sigma/2 (x) I (helpers.rs:11-39).
 int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride,
                  double* J);
-// Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2)
+// Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2) for
+// Hermitian A, B (A_nm is read as conj(A_mn))
 int kubo_omega(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E,
                double eta, double* omega);
 // out[k] = sum_n omega[k][n] f(E[k][n], mu, T)

@@ -7,187 +7,6 @@ namespace rtb {
 
 namespace {
 
-constexpr int kT = 64;   // output tile (kT x kT) per workgroup
-constexpr int kK = 16;   // k-tile staged in LDS
-
-// C = op(A) B for one n x n complex matrix per blockIdx.z; 4x4 outputs per thread.
-__global__ void __launch_bounds__(256) zgemm_kernel(int n, const double2* __restrict__ A, size_t sA, int conj_a,
-                                                    const double2* __restrict__ B, size_t sB, double2* __restrict__ C,
-                                                    size_t sC) {
-  __shared__ double2 As[kK][kT + 1];
-  __shared__ double2 Bs[kK][kT + 1];
-  A += static_cast<size_t>(blockIdx.z) * sA;
-  B += static_cast<size_t>(blockIdx.z) * sB;
-  C += static_cast<size_t>(blockIdx.z) * sC;
-  const int row0 = blockIdx.y * kT, col0 = blockIdx.x * kT;
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  double cr[4][4], ci[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      cr[i][j] = 0.0;
-      ci[i][j] = 0.0;
-    }
-  for (int k0 = 0; k0 < n; k0 += kK) {
-    // As[kk][i] = op(A)[row0+i][k0+kk]
-    for (int q = threadIdx.x; q < kK * kT; q += 256) {
-      int i, kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (conj_a) {  // op(A)[r][k] = conj(A[k][r]); read along r
-        kk = q / kT;
-        i = q % kT;
-        int r = row0 + i, kc = k0 + kk;
-        if (r < n && kc < n) {
-          v = A[static_cast<size_t>(kc) * n + r];
-          v.y = -v.y;
-        }
-      } else {  // read along k
-        i = q / kK;
-        kk = q % kK;
-        int r = row0 + i, kc = k0 + kk;
-        if (r < n && kc < n) v = A[static_cast<size_t>(r) * n + kc];
-      }
-      As[kk][i] = v;
-    }
-    for (int q = threadIdx.x; q < kK * kT; q += 256) {
-      int kk = q / kT, j = q % kT;
-      int kr = k0 + kk, c = col0 + j;
-      Bs[kk][j] = (kr < n && c < n) ? B[static_cast<size_t>(kr) * n + c] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < kK; ++kk) {
-      double2 a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          cr[i][j] += a[i].x * b[j].x - a[i].y * b[j].y;
-          ci[i][j] += a[i].x * b[j].y + a[i].y * b[j].x;
-        }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    int r = row0 + ty + 16 * i;
-    if (r >= n) continue;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      int c = col0 + tx + 16 * j;
-      if (c < n) C[static_cast<size_t>(r) * n + c] = make_double2(cr[i][j], ci[i][j]);
-    }
-  }
-}
-
-typedef double d4_t __attribute__((ext_vector_type(4)));
-
-// C = op(A) B on FP64 matrix cores (v_mfma_f64_16x16x4f64).  Each of the four
-// waves computes a WT x WT sub-tile as (WT/16)^2 blocks of 16x16, so the
-// workgroup tile is 2WT x 2WT (WT = 64: one workgroup per 128 x 128 matrix,
-// every operand element read once).  A complex product is four real MFMAs:
-// Cre += Ar Br - Ai Bi, Cim += Ar Bi + Ai Br.  Operand layout of the
-// instruction (lane l): A[l%16][l/16], B[l/16][l%16]; accumulator register r
-// of lane l holds D[4r + l/16][l%16].
-template <int WT>
-__global__ void __launch_bounds__(256) zgemm_mfma_kernel(int n, const double2* __restrict__ A, size_t sA,
-                                                         int conj_a, const double2* __restrict__ B, size_t sB,
-                                                         double2* __restrict__ C, size_t sC) {
-  constexpr int TT = 2 * WT;             // workgroup tile
-  constexpr int NB = WT / 16;            // blocks per wave side
-  constexpr int KC = WT == 64 ? 8 : 16;  // k-chunk staged in LDS
-  __shared__ double Ar[TT][KC + 1], Ai[TT][KC + 1];
-  __shared__ double Br[KC][TT + 1], Bi[KC][TT + 1];
-  A += static_cast<size_t>(blockIdx.z) * sA;
-  B += static_cast<size_t>(blockIdx.z) * sB;
-  C += static_cast<size_t>(blockIdx.z) * sC;
-  const int row0 = blockIdx.y * TT, col0 = blockIdx.x * TT;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * WT, wc = (wave & 1) * WT;  // wave sub-tile origin
-  const bool busy = row0 + wr < n && col0 + wc < n;
-  d4_t cre[NB][NB], cim[NB][NB];
-#pragma unroll
-  for (int a = 0; a < NB; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      cre[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
-      cim[a][b] = d4_t{0.0, 0.0, 0.0, 0.0};
-    }
-  for (int k0 = 0; k0 < n; k0 += KC) {
-    for (int q = threadIdx.x; q < TT * KC; q += 256) {
-      int i, kk;
-      double2 v = make_double2(0.0, 0.0);
-      if (conj_a) {  // op(A)[r][k] = conj(A[k][r]), read along r
-        kk = q / TT;
-        i = q % TT;
-        const int r = row0 + i, kc = k0 + kk;
-        if (r < n && kc < n) {
-          v = A[static_cast<size_t>(kc) * n + r];
-          v.y = -v.y;
-        }
-      } else {
-        i = q / KC;
-        kk = q % KC;
-        const int r = row0 + i, kc = k0 + kk;
-        if (r < n && kc < n) v = A[static_cast<size_t>(r) * n + kc];
-      }
-      Ar[i][kk] = v.x;
-      Ai[i][kk] = v.y;
-    }
-    for (int q = threadIdx.x; q < TT * KC; q += 256) {
-      const int kk = q / TT, j = q % TT;
-      const int kr = k0 + kk, c = col0 + j;
-      const double2 v = (kr < n && c < n) ? B[static_cast<size_t>(kr) * n + c] : make_double2(0.0, 0.0);
-      Br[kk][j] = v.x;
-      Bi[kk][j] = v.y;
-    }
-    __syncthreads();
-    if (busy) {
-#pragma unroll
-      for (int kk = 0; kk < KC; kk += 4) {
-        double ar[NB], ai[NB], nai[NB], br[NB], bi[NB];
-#pragma unroll
-        for (int a = 0; a < NB; ++a) {
-          ar[a] = Ar[wr + a * 16 + (lane & 15)][kk + (lane >> 4)];
-          ai[a] = Ai[wr + a * 16 + (lane & 15)][kk + (lane >> 4)];
-          nai[a] = -ai[a];
-        }
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          br[b] = Br[kk + (lane >> 4)][wc + b * 16 + (lane & 15)];
-          bi[b] = Bi[kk + (lane >> 4)][wc + b * 16 + (lane & 15)];
-        }
-#pragma unroll
-        for (int a = 0; a < NB; ++a)
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], cre[a][b], 0, 0, 0);
-            cre[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[a], bi[b], cre[a][b], 0, 0, 0);
-            cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], cim[a][b], 0, 0, 0);
-            cim[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], cim[a][b], 0, 0, 0);
-          }
-      }
-    }
-    __syncthreads();
-  }
-  if (!busy) return;
-#pragma unroll
-  for (int a = 0; a < NB; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = row0 + wr + a * 16 + 4 * r + (lane >> 4);
-        const int j = col0 + wc + b * 16 + (lane & 15);
-        if (i < n && j < n) C[static_cast<size_t>(i) * n + j] = make_double2(cre[a][b][r], cim[a][b][r]);
-      }
-}
-
 // helpers.rs:11-39 spin matrix S = sigma/2 (x) I_norb; J = (S v + v S) / 2
 __global__ void spin_current_kernel(int norb, size_t nb, int dir, const double2* __restrict__ v,
                                     size_t v_stride, double2* __restrict__ J) {
@@ -233,11 +52,13 @@ __global__ void kubo_omega_kernel(int n, size_t nk, const double2* __restrict__ 
   const double2* Bk = B + k * n * n;
   const double* Ek = E + k * n;
   double acc = 0.0;
+  // A is Hermitian: A_bm = conj(A_mb), so both operands are read down
+  // column b (consecutive threads, consecutive addresses)
   for (int m = 0; m < n; ++m) {
     if (m == b) continue;
-    const double2 a = Ak[static_cast<size_t>(b) * n + m];
+    const double2 a = Ak[static_cast<size_t>(m) * n + b];
     const double2 c = Bk[static_cast<size_t>(m) * n + b];
-    const double im = a.x * c.y + a.y * c.x;  // Im(A_bm B_mb)
+    const double im = a.x * c.y - a.y * c.x;  // Im(A_bm B_mb) = Im(conj(A_mb) B_mb)
     const double dE = Ek[b] - Ek[m];
     acc += -2.0 * im / (dE * dE + eta2);
   }
@@ -322,30 +143,6 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
 }
 
 }  // namespace
-
-int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
-                  size_t sB, double* C, size_t sC) {
-  if (nb == 0) return RTB_OK;
-  // n <= 64: one 64 x 64 tile per matrix; otherwise 128 x 128 tiles
-  const int tt = n <= 64 ? 64 : 128;
-  const unsigned t = static_cast<unsigned>((n + tt - 1) / tt);
-  for (size_t off = 0; off < nb; off += 65535) {
-    size_t cnt = nb - off < 65535 ? nb - off : 65535;
-    if (tt == 64) {
-      hipLaunchKernelGGL((zgemm_mfma_kernel<32>), dim3(t, t, static_cast<unsigned>(cnt)), dim3(256), 0, st, n,
-                         reinterpret_cast<const double2*>(A) + off * sA, sA, conj_a ? 1 : 0,
-                         reinterpret_cast<const double2*>(B) + off * sB, sB,
-                         reinterpret_cast<double2*>(C) + off * sC, sC);
-    } else {
-      hipLaunchKernelGGL((zgemm_mfma_kernel<64>), dim3(t, t, static_cast<unsigned>(cnt)), dim3(256), 0, st, n,
-                         reinterpret_cast<const double2*>(A) + off * sA, sA, conj_a ? 1 : 0,
-                         reinterpret_cast<const double2*>(B) + off * sB, sB,
-                         reinterpret_cast<double2*>(C) + off * sC, sC);
-    }
-  }
-  RTB_HIP(hipGetLastError());
-  return RTB_OK;
-}
 
 int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride, double* J) {
   const size_t n = 2 * static_cast<size_t>(norb);

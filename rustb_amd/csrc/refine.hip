@@ -46,31 +46,97 @@ __global__ void rayleigh_kernel(int n, size_t nk, double2* __restrict__ Z, doubl
   }
 }
 
-// One wave per k-point: the lanes test every eigenvalue gap in parallel; only
-// a k-point with a gap below the threshold runs the serial Rayleigh-Ritz
-// (lane 0, small matrices in LDS).  HZ = H Z for the unit columns.
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// One wave per k-point: the lanes test every eigenvalue gap in parallel; a
+// k-point with a cluster forms the cluster's Gram and projection sums with
+// the lanes over rows, lane 0 solves the c x c problem (LDS), and the lanes
+// rotate the cluster's columns row by row.  HZ = H Z for the unit columns.
 __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restrict__ Z,
                                                         const double2* __restrict__ HZ, double* __restrict__ E,
                                                         double gap_rel) {
   __shared__ C2 lds[7][kMaxCluster][kMaxCluster];
+  __shared__ double ev[kMaxCluster];
+  __shared__ int solved;
   const size_t k = blockIdx.x;
+  const int lane = threadIdx.x;
   double* Ek = E + k * n;
   double m = 0.0;
-  for (int j = threadIdx.x; j < n; j += 64) m = fmax(m, fabs(Ek[j]));
+  for (int j = lane; j < n; j += 64) m = fmax(m, fabs(Ek[j]));
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
   const double thr = gap_rel * fmax(m, 1.0);
   int close = 0;
-  for (int j = threadIdx.x; j + 1 < n; j += 64) close |= (Ek[j + 1] - Ek[j] <= thr);
+  for (int j = lane; j + 1 < n; j += 64) close |= (Ek[j + 1] - Ek[j] <= thr);
   if (!__any(close)) return;
-  if (threadIdx.x != 0) return;
-  cluster_rr_one(n, reinterpret_cast<Cd*>(Z + k * n * n), reinterpret_cast<const Cd*>(HZ + k * n * n), Ek, gap_rel,
-                 lds);
+  double2* Zk = Z + k * n * n;
+  const double2* Hk = HZ + k * n * n;
+  int j0 = 0;
+  while (j0 < n) {  // the same cluster sequence in every lane
+    const int j1 = cluster_end(n, Ek, j0, thr);
+    const int c = j1 - j0 + 1;
+    if (c >= 2) {
+      for (int p = 0; p < c * c; ++p) {
+        const int a = p / c, b = p % c;
+        double gr = 0.0, gi = 0.0, mr = 0.0, mi = 0.0;
+        for (int r = lane; r < n; r += 64) {
+          const double2 za = Zk[static_cast<size_t>(r) * n + j0 + a];
+          const double2 zb = Zk[static_cast<size_t>(r) * n + j0 + b];
+          const double2 hb = Hk[static_cast<size_t>(r) * n + j0 + b];
+          gr += za.x * zb.x + za.y * zb.y;  // conj(za) zb
+          gi += za.x * zb.y - za.y * zb.x;
+          mr += za.x * hb.x + za.y * hb.y;  // conj(za) hb
+          mi += za.x * hb.y - za.y * hb.x;
+        }
+        gr = wave_sum(gr);
+        gi = wave_sum(gi);
+        mr = wave_sum(mr);
+        mi = wave_sum(mi);
+        if (lane == 0) {
+          lds[0][a][b] = {gr, gi};
+          lds[1][a][b] = {mr, mi};
+        }
+      }
+      if (lane == 0) solved = cluster_rr_small(c, lds, ev) ? 1 : 0;
+      __syncthreads();
+      if (solved) {
+        for (int r = lane; r < n; r += 64) {
+          double2* zr = Zk + static_cast<size_t>(r) * n + j0;
+          C2 row[kMaxCluster];
+#pragma unroll
+          for (int q = 0; q < kMaxCluster; ++q)
+            if (q < c) row[q] = {zr[q].x, zr[q].y};
+#pragma unroll
+          for (int bb = 0; bb < kMaxCluster; ++bb) {
+            if (bb < c) {
+              C2 acc = {0.0, 0.0};
+#pragma unroll
+              for (int q = 0; q < kMaxCluster; ++q) {
+                if (q < c) {
+                  const C2 t = cm(row[q], lds[6][q][bb]);
+                  acc.re += t.re;
+                  acc.im += t.im;
+                }
+              }
+              zr[bb] = make_double2(acc.re, acc.im);
+            }
+          }
+        }
+        if (lane < c) Ek[j0 + lane] = ev[lane];
+      }
+      __syncthreads();
+    }
+    j0 = j1 + 1;
+  }
 }
 
 // One workgroup per k-point: the Rayleigh-like diagonal lt_i = S_ii / G_ii and
 // the spectral scale max |lt_i| are formed once in LDS, then every element
-// of the correction matrix F' is written.
+// of the correction F' - I is written.
 __global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2* __restrict__ S,
                                                             const double2* __restrict__ G, double2* __restrict__ F,
                                                             double gap_rel) {
@@ -100,7 +166,7 @@ __global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2
     const double2 s = Sk[e], gg = Gk[e];
     double2 f;
     if (i == j) {
-      f = make_double2(1.0 + 0.5 * (1.0 - gd[i]), 0.0);
+      f = make_double2(0.5 * (1.0 - gd[i]), 0.0);
     } else if (fabs(lt[i] - lt[j]) > thr) {
       const double inv = 1.0 / (lt[j] - lt[i]);
       f = make_double2((s.x - lt[j] * gg.x) * inv, (s.y - lt[j] * gg.y) * inv);
@@ -137,31 +203,41 @@ int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, double*
 //   F'_ij = (S_ij - lt_j G_ij) / (lt_j - lt_i)  (i != j, |lt_i - lt_j| > thr)
 //         = -G_ij / 2                           (i != j, inside a cluster)
 //   F'_ii = 1 + (1 - G_ii) / 2,
-//   Z' = Z F', HZ' = HZ F'  (the FP32 vector error enters squared).
-// Buffers (each [nk][n][n] complex): H in/out scratch, Z in/out, W1..W3.
-int refine_eigenpairs(hipStream_t st, int n, size_t nk, double* H, double* Z, double* W1, double* W2, double* W3,
-                      double* E, double gap_rel) {
+//   Z' = Z + Z (F' - I), HZ' = HZ + HZ (F' - I)  (the FP32 vector error enters
+//   squared).  F' - I is of the size of that error, so its two products run
+//   on the FP32 matrix cores: their rounding (~1e-7 relative to a correction
+//   already ~1e-6 or smaller) stays below the FP64 level of the sums.
+// Buffers (each [nk][n][n] complex): H (consumed), Z (in; scratch after),
+// W1, W3 scratch; the refined eigenvectors (columns) are left in W2.
+int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, double* Z, double* W1, double* W2,
+                      double* W3, double* E, double gap_rel) {
   if (nk == 0) return RTB_OK;
   const size_t n2 = static_cast<size_t>(n) * n;
-  RTB_TRY(zgemm_batched(st, n, nk, H, n2, false, Z, n2, W1, n2));   // HZ
-  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, W1, n2, W2, n2));   // S
-  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, Z, n2, W3, n2));    // G
-  const size_t tot = nk * n2;
+  // FP64 products: HZ, S = Z^H HZ, G = Z^H Z
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM64));
+  RTB_TRY(zgemm_batched(st, n, nk, H, n2, false, Z, n2, W1, n2));
+  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, W1, n2, W2, n2, true));
+  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, Z, n2, W3, n2, true));
+  if (tmr) RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk)));
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
   hipLaunchKernelGGL(oa_correction_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
                      reinterpret_cast<const double2*>(W2), reinterpret_cast<const double2*>(W3),
-                     reinterpret_cast<double2*>(H), gap_rel);  // F' into H (H is dead)
+                     reinterpret_cast<double2*>(H), gap_rel);  // F' - I into H (H is dead)
   RTB_HIP(hipGetLastError());
-  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, false, H, n2, W2, n2));   // Z' = Z F'
-  RTB_TRY(zgemm_batched(st, n, nk, W1, n2, false, H, n2, W3, n2));  // HZ' = HZ F'
+  if (tmr) RTB_TRY(tmr->end(1));
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM32));
+  RTB_TRY(zgemm_add_f32_batched(st, n, nk, Z, n2, H, n2, W2, n2));   // Z' = Z + Z (F' - I)
+  RTB_TRY(zgemm_add_f32_batched(st, n, nk, W1, n2, H, n2, W3, n2));  // HZ' = HZ + HZ (F' - I)
+  if (tmr) RTB_TRY(tmr->end(2, 2 * flop_zgemm(n, nk)));
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
   const size_t tk = nk * n;
   hipLaunchKernelGGL(rayleigh_kernel, dim3(static_cast<unsigned>((tk + 255) / 256)), dim3(256), 0, st, n, nk,
                      reinterpret_cast<double2*>(W2), reinterpret_cast<double2*>(W3), E);
   RTB_HIP(hipGetLastError());
   hipLaunchKernelGGL(cluster_rr_kernel, dim3(static_cast<unsigned>(nk)), dim3(64), 0, st, n,
-                     reinterpret_cast<double2*>(W2),
-                     reinterpret_cast<const double2*>(W3), E, gap_rel);
+                     reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W3), E, gap_rel);
   RTB_HIP(hipGetLastError());
-  RTB_HIP(hipMemcpyAsync(Z, W2, tot * sizeof(double2), hipMemcpyDeviceToDevice, st));
+  if (tmr) RTB_TRY(tmr->end(2));
   return RTB_OK;
 }
 

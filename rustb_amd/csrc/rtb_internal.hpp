@@ -38,6 +38,76 @@ inline int fail(int code, const char* msg) {
     }                        \
   } while (0)
 
+// Event-pair timing of kernel launches on one stream, by stage (see
+// rtb_stats).  collect() runs after the stream is synchronised and publishes
+// the sums as the calling thread's rtb_last_stats().
+void publish_stats(const rtb_stats& s);
+class Timer {
+ public:
+  explicit Timer(hipStream_t s) : st_(s) {}
+  Timer(const Timer&) = delete;
+  Timer& operator=(const Timer&) = delete;
+  ~Timer() {
+    for (auto& r : recs_) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+  }
+  int begin(int stage) {
+    Rec r{stage, nullptr, nullptr, 0, 0.0};
+    RTB_HIP(hipEventCreate(&r.a));
+    RTB_HIP(hipEventCreate(&r.b));
+    RTB_HIP(hipEventRecord(r.a, st_));
+    recs_.push_back(r);
+    return RTB_OK;
+  }
+  int end(int launches, double flop = 0.0) {
+    recs_.back().launches = launches;
+    recs_.back().flop = flop;
+    RTB_HIP(hipEventRecord(recs_.back().b, st_));
+    return RTB_OK;
+  }
+  int collect(size_t nk, int n) {
+    rtb_stats s{};
+    s.nk = nk;
+    s.nsta = n;
+    for (auto& r : recs_) {
+      float ms = 0.f;
+      RTB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      s.ms[r.stage] += ms;
+      s.launches[r.stage] += r.launches;
+      s.flop[r.stage] += r.flop;
+    }
+    if (!recs_.empty()) {
+      float ms = 0.f;
+      RTB_HIP(hipEventElapsedTime(&ms, recs_.front().a, recs_.back().b));
+      s.ms[RTB_STAGE_TOTAL] = ms;
+      for (int i = 0; i < RTB_STAGE_TOTAL; ++i) {
+        s.launches[RTB_STAGE_TOTAL] += s.launches[i];
+        s.flop[RTB_STAGE_TOTAL] += s.flop[i];
+      }
+    }
+    publish_stats(s);
+    return RTB_OK;
+  }
+
+ private:
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+    int launches;
+    double flop;
+  };
+  hipStream_t st_;
+  std::vector<Rec> recs_;
+};
+
+// Algorithmic flop counts used in the stats.
+inline double flop_zgemm(int n, size_t nb) { return 8.0 * n * n * static_cast<double>(n) * nb; }
+inline double flop_zgemm_herm(int n, size_t nb) { return 4.0 * n * n * (n + 1.0) * nb; }
+inline double flop_tridiag(int n, size_t nb) { return 16.0 / 3.0 * n * n * static_cast<double>(n) * nb; }
+inline double flop_backtrans(int n, size_t nb) { return 8.0 * n * n * static_cast<double>(n) * nb; }
+
 // Largest nsta the batched eigensolver handles (packed lower triangle of the
 // FP64 matrix in LDS: n(n+1)/2 * 16 B <= 160 KB).
 constexpr int kMaxEighN = 128;

@@ -1,0 +1,238 @@
+// Batched complex matrix products on the matrix cores (gfx950), FP64 storage.
+//
+// C = op(A) B (FP64 arithmetic) or C = A + A B (FP32 arithmetic, for small
+// corrections B) for a batch of n x n complex matrices (n <= 128 in practice,
+// any n works), row-major interleaved (re, im).  One workgroup computes one
+// 64 x 64 output tile; its four waves own 32 x 32 quarters (2 x 2 blocks of
+// the 16 x 16 x 4 MFMA).
+//
+//   * 3M product: Re = Ar Br - Ai Bi, Im = (Ar + Ai)(Br + Bi) - Ar Br - Ai Bi,
+//     three real MFMAs per block and k-step instead of four;
+//   * the next 16-wide k-chunk is loaded into registers while the current
+//     one is multiplied from LDS;
+//   * herm = 1: op(A) B is Hermitian (X^H Y with X^H Y = Y^H X, e.g. U^H J U);
+//     only tiles on or below the diagonal are computed and the off-diagonal
+//     ones are also stored conjugate-transposed;
+//   * workgroup w runs on XCD w % 8; the tiles of one matrix are given to
+//     workgroups on the same XCD so that they share its L2.
+//
+// MFMA operand layout (lane l): A[l % 16][l / 16], B[l / 16][l % 16];
+// accumulator register r of lane l holds D[4r + l / 16][l % 16] (FP64) or
+// D[4(l / 16) + r][l % 16] (FP32).
+#include "eigh.hpp"
+
+namespace rtb {
+
+namespace {
+
+constexpr int kTile = 64;  // output tile side
+constexpr int kKc = 16;    // k-chunk
+constexpr int kPad = 1;    // LDS row padding (doubles)
+constexpr int kLd = kTile + kPad;
+constexpr int kXcd = 8;
+
+struct TileMap {
+  int tiles;       // tiles per matrix
+  int side;        // tiles per matrix side
+  int herm;
+};
+
+// Tile t of a side x side grid: row-major, or the lower triangle row by row.
+__device__ inline void tile_coords(int t, int side, int herm, int& ti, int& tj) {
+  if (!herm) {
+    ti = t / side;
+    tj = t % side;
+    return;
+  }
+  ti = 0;
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  tj = t - ti * (ti + 1) / 2;
+}
+
+// Arithmetic type of the products: FP64 (v_mfma_f64_16x16x4f64) or FP32
+// (v_mfma_f32_16x16x4f32).  Operands are staged in LDS in that type; output
+// and addend are FP64.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ v4 mac(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  // row of accumulator register r in lane group fk = lane / 16
+  static __device__ int row(int r, int fk) { return 4 * r + fk; }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ int row(int r, int fk) { return 4 * fk + r; }
+};
+
+// C = [D +] op(A) B; D may alias A (it is read only), C aliases nothing.
+template <typename T, bool ADD>
+__global__ void __launch_bounds__(256) zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A,
+                                                      size_t sA, int conj_a, const double2* __restrict__ B, size_t sB,
+                                                      const double2* D, size_t sD, double2* __restrict__ C,
+                                                      size_t sC) {
+  typedef typename Mfma<T>::v4 v4;
+  __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
+  __shared__ T sB3[3][kKc][kLd];  // B tile, k-major
+  // XCD-aware decomposition of the workgroup index
+  const size_t g = blockIdx.x;
+  const size_t loc = g / kXcd;
+  const size_t mat = (loc / tm.tiles) * kXcd + g % kXcd;
+  if (mat >= nmat) return;
+  int ti, tj;
+  tile_coords(static_cast<int>(loc % tm.tiles), tm.side, tm.herm, ti, tj);
+  A += mat * sA;
+  B += mat * sB;
+  C += mat * sC;
+  if (ADD) D += mat * sD;
+  const int row0 = ti * kTile, col0 = tj * kTile;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+
+  // global -> register staging: 4 elements of op(A) and 4 of B per thread
+  double2 pa[4], pb[4];
+  auto load_chunk = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = tid + 256 * u;
+      double2 v = make_double2(0.0, 0.0);
+      if (conj_a) {  // op(A)[r][k] = conj(A[k][r]); lanes along r
+        const int kk = q / kTile, i = q % kTile;
+        const int r = row0 + i, k = k0 + kk;
+        if (r < n && k < n) v = A[static_cast<size_t>(k) * n + r];  // conjugated when staged
+      } else {  // lanes along k within a row
+        const int i = q / kKc, kk = q % kKc;
+        const int r = row0 + i, k = k0 + kk;
+        if (r < n && k < n) v = A[static_cast<size_t>(r) * n + k];
+      }
+      pa[u] = v;
+      const int kk = q / kTile, j = q % kTile;
+      const int k = k0 + kk, c = col0 + j;
+      pb[u] = (k < n && c < n) ? B[static_cast<size_t>(k) * n + c] : make_double2(0.0, 0.0);
+    }
+  };
+  auto store_chunk = [&]() {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = tid + 256 * u;
+      int i, kk;
+      if (conj_a) {
+        kk = q / kTile;
+        i = q % kTile;
+      } else {
+        i = q / kKc;
+        kk = q % kKc;
+      }
+      const T ar = static_cast<T>(pa[u].x);
+      const T ai = static_cast<T>(conj_a ? -pa[u].y : pa[u].y);
+      sA3[0][kk][i] = ar;
+      sA3[1][kk][i] = ai;
+      sA3[2][kk][i] = ar + ai;
+      const int kb = q / kTile, j = q % kTile;
+      const T br = static_cast<T>(pb[u].x), bi = static_cast<T>(pb[u].y);
+      sB3[0][kb][j] = br;
+      sB3[1][kb][j] = bi;
+      sB3[2][kb][j] = br + bi;
+    }
+  };
+
+  v4 acc[3][2][2];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[p][a][b] = v4{T(0), T(0), T(0), T(0)};
+
+  const int fr = lane & 15, fk = lane >> 4;
+  load_chunk(0);
+  for (int k0 = 0; k0 < n; k0 += kKc) {
+    __syncthreads();  // previous chunk fully consumed
+    store_chunk();
+    __syncthreads();
+    if (k0 + kKc < n) load_chunk(k0 + kKc);  // in flight during the MFMAs
+    // one k-step at a time keeps the operand registers few (two waves per
+    // SIMD); each step is 12 MFMAs against 12 LDS reads
+#pragma unroll 1
+    for (int kk = 0; kk < kKc; kk += 4) {
+      T av[3][2], bv[3][2];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          av[p][a] = sA3[p][kk + fk][wr + 16 * a + fr];
+          bv[p][a] = sB3[p][kk + fk][wc + 16 * a + fr];
+        }
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) acc[p][a][b] = Mfma<T>::mac(av[p][a], bv[p][b], acc[p][a][b]);
+    }
+  }
+  const bool mirror = tm.herm && ti != tj;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + wr + 16 * a + Mfma<T>::row(r, fk);
+        const int j = col0 + wc + 16 * b + fr;
+        if (i < n && j < n) {
+          const double p1 = acc[0][a][b][r], p2 = acc[1][a][b][r], p3 = acc[2][a][b][r];
+          double re = p1 - p2, im = p3 - p1 - p2;
+          const size_t o = static_cast<size_t>(i) * n + j;
+          if (ADD) {
+            const double2 d = D[o];
+            re += d.x;
+            im += d.y;
+          }
+          C[o] = make_double2(re, im);
+          if (mirror) C[static_cast<size_t>(j) * n + i] = make_double2(re, -im);
+        }
+      }
+}
+
+template <typename T, bool ADD>
+int zgemm_launch(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
+                 size_t sB, const double* D, size_t sD, double* C, size_t sC, bool herm) {
+  if (nb == 0 || n <= 0) return RTB_OK;
+  TileMap tm;
+  tm.side = (n + kTile - 1) / kTile;
+  tm.herm = herm ? 1 : 0;
+  tm.tiles = herm ? tm.side * (tm.side + 1) / 2 : tm.side * tm.side;
+  // grid: kXcd interleaved streams of (matrix group, tile); one launch per
+  // slice of at most 2^24 workgroups (grid size in work-items < 2^32)
+  const size_t per_launch = (static_cast<size_t>(1) << 24) / (kXcd * tm.tiles) * kXcd;
+  for (size_t off = 0; off < nb; off += per_launch) {
+    const size_t cnt = nb - off < per_launch ? nb - off : per_launch;
+    const size_t groups = (cnt + kXcd - 1) / kXcd;
+    const size_t wgs = groups * tm.tiles * kXcd;
+    hipLaunchKernelGGL((zgemm3m_kernel<T, ADD>), dim3(static_cast<unsigned>(wgs)), dim3(256), 0, st, n, cnt, tm,
+                       reinterpret_cast<const double2*>(A) + off * sA, sA, conj_a ? 1 : 0,
+                       reinterpret_cast<const double2*>(B) + off * sB, sB,
+                       D ? reinterpret_cast<const double2*>(D) + off * sD : nullptr, sD,
+                       reinterpret_cast<double2*>(C) + off * sC, sC);
+  }
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+}  // namespace
+
+int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
+                  size_t sB, double* C, size_t sC, bool herm) {
+  return zgemm_launch<double, false>(st, n, nb, A, sA, conj_a, B, sB, nullptr, 0, C, sC, herm);
+}
+
+int zgemm_add_f32_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, const double* B, size_t sB,
+                          double* C, size_t sC) {
+  return zgemm_launch<float, true>(st, n, nb, A, sA, false, B, sB, A, sA, C, sC, false);
+}
+
+}  // namespace rtb

@@ -235,7 +235,19 @@ def test_validation_errors_gpu():
 
 
 def test_last_stats_reports_stages():
+    from rustb_amd._lib import STAGES
     m = ra.workloads.random3d_n32()
     m.solve_band_all(np.random.default_rng(0).random((64, 3)))
     st = ra.last_stats()
-    assert st["nk"] == 64 and st["nsta"] == 32 and st["ms"][6] > 0 and st["launches"][1] >= 1
+    i = STAGES.index
+    assert st["nk"] == 64 and st["nsta"] == 32 and st["ms"][i("total")] > 0
+    assert st["launches"][i("tridiag")] == 1 and st["launches"][i("trieig")] == 1
+    assert st["flop"][i("tridiag")] == pytest.approx(16.0 / 3.0 * 32**3 * 64, rel=1e-12)
+    # the mixed Hall path reports its matrix-core products with their flops
+    p = ra.Parameters(kmesh=(4, 4, 4), direction=np.eye(3)[:2], mu=np.array([0.0]), eta=1e-3, precision="mixed")
+    m.hall_conductivity(p)
+    st = ra.last_stats()
+    assert st["launches"][i("gemm64")] == 7 and st["launches"][i("gemm32")] == 2
+    herm = 4.0 * 32 * 32 * 33 * 64
+    assert st["flop"][i("gemm64")] == pytest.approx(3 * 8.0 * 32**3 * 64 + 4 * herm, rel=1e-12)
+    assert all(st["ms"][i(s)] > 0 for s in ("fourier", "tridiag", "trieig", "backtransform", "gemm64", "refine"))
