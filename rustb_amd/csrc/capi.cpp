@@ -217,9 +217,10 @@ int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
 // Mixed-form eigenpairs of a chunk: FP32 Householder / bisection / inverse
 // iteration / back-transformation (the FP64 buffers of `eb` reused as FP32
 // storage) into Z, then FP64 refinement against the FP64 H.  The refined
-// eigenvectors (columns) end in W2; Z, W1 and W3 are scratch afterwards.
+// eigenvectors end in W2 as columns that are unit after multiplication by
+// scale[k][j]; Z, W1 and W3 are scratch afterwards.
 int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, double* E, EigBufs& eb, double* Z,
-                    double* W1, double* W2, double* W3) {
+                    double* W1, double* W2, double* W3, double* scale) {
   float* Vg = reinterpret_cast<float*>(eb.Vg);
   float* tau = reinterpret_cast<float*>(eb.tau);
   float* Y = reinterpret_cast<float*>(eb.Y);
@@ -227,19 +228,19 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, doub
   float* DE = reinterpret_cast<float*>(eb.DE);
   RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE));
   RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, Vg, tau, Z, false));
+  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, Vg, tau, Z, false));
   RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
   // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
   // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius
-  RTB_TRY(refine_eigenpairs(st, &tmr, n, c, H, Z, W1, W2, W3, E, 1e-3));
+  RTB_TRY(refine_eigenpairs(st, &tmr, n, c, H, Z, W1, W2, W3, E, scale, 1e-3));
   return RTB_OK;
 }
 
 // Kubo for a chunk: V = [K][2][n][n]; Z = [K][n][n] (columns); omega [K][n]
 //   A = Z^H J Z, B = Z^H v_b Z (response/traits.rs:177-180)
 int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const double* V, const double* Z,
-               const double* E, int spin_dir, double eta, double* T1, double* A, double* B, double* J,
-               double* omega) {
+               const double* zscale, const double* E, int spin_dir, double eta, double* T1, double* A, double* B,
+               double* J, double* omega) {
   const int n = m->nsta;
   const size_t n2 = static_cast<size_t>(n) * n;
   const double* Ja = V;
@@ -252,10 +253,11 @@ int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const d
     sJ = n2;
   }
   RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
+  // Z diag(zscale) are the unit eigenvectors (zscale null: Z is unit)
   RTB_TRY(zgemm_batched(st, n, K, Ja, sJ, false, Z, n2, T1, n2));
-  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, A, n2, true));
+  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, A, n2, true, zscale));
   RTB_TRY(zgemm_batched(st, n, K, V + n2 * 2, 2 * n2, false, Z, n2, T1, n2));
-  RTB_TRY(zgemm_batched(st, n, K, Z, n2, true, T1, n2, B, n2, true));
+  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, B, n2, true, zscale));
   RTB_TRY(tmr.end(4, 2 * flop_zgemm(n, K) + 2 * flop_zgemm_herm(n, K)));
   RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
   RTB_TRY(kubo_omega(st, n, K, A, B, E, eta, omega));
@@ -436,9 +438,9 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
   // the mixed form runs the FP32 route (with FP64 refinement) when
   // eigenvectors are requested; eigenvalue-only solves stay FP64
   const bool f32 = vec && precision == RTB_PREC_MIXED;
-  const size_t per_k = 24 + n2 * 16 * 2 + n * 24 + (vec ? n2 * 32 + n2 * 16 : 0) + (f32 ? n2 * 32 : 0);
+  const size_t per_k = 24 + n2 * 16 * 2 + n * 24 + (vec ? n2 * 32 + n2 * 16 : 0) + (f32 ? n2 * 32 + n * 8 : 0);
   const size_t K = chunk_for(nk, per_k);
-  double *dk, *dH, *dE, *dZ = nullptr, *dZc = nullptr, *dHZ = nullptr;
+  double *dk, *dH, *dE, *dZ = nullptr, *dZc = nullptr, *dHZ = nullptr, *dsc = nullptr;
   EigBufs eb;
   RTB_TRY(w.alloc(&dk, K * 3));
   RTB_TRY(w.alloc(&dH, K * n2 * 2));
@@ -448,6 +450,7 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
   if (f32) {
     RTB_TRY(w.alloc(&dZc, K * n2 * 2));
     RTB_TRY(w.alloc(&dHZ, K * n2 * 2));
+    RTB_TRY(w.alloc(&dsc, K * n));
   }
   std::vector<double> kp;
   for (size_t k0 = 0; k0 < nk; k0 += K) {
@@ -461,15 +464,15 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     RTB_TRY(tmr.end(1));
     if (f32) {
       // W2, W3: the reflector and inverse-iteration scratch, dead by then
-      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S));
+      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S, dsc));
       RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
-      RTB_TRY(transpose_batched(st, n, c, eb.Vg, dZ));
+      RTB_TRY(transpose_batched(st, n, c, eb.Vg, dsc, dZ));
       RTB_TRY(tmr.end(1));
     } else {
       RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
       if (vec) {
         RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-        RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, true));
+        RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.Vg, eb.tau, dZ, true));
         RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
       }
     }
@@ -565,7 +568,7 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const bool f32 = j.precision == RTB_PREC_MIXED;  // FP32 eigensolve + FP64 refinement
   const size_t per_k =
-      24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 48 + (f32 ? n2 * 16 : 0);
+      24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 48 + (f32 ? n2 * 16 + n * 8 : 0);
   const size_t K = chunk_for(j.nk, per_k);
   MeshPlan mp;
   double* G = nullptr;
@@ -582,8 +585,10 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   double* kub;
   RTB_TRY(w.alloc(&kub, K * n2 * 6));  // T1, A, B (complex)
   if (spinj) RTB_TRY(w.alloc(&dJ, K * n2 * 2));
+  double* dsc = nullptr;  // column scales of the refined eigenvectors
   if (f32) {
     RTB_TRY(w.alloc(&dZ, K * n2 * 2));  // the refinement needs H after the solve
+    RTB_TRY(w.alloc(&dsc, K * n));
   } else {
     dZ = dH;  // H is consumed by the Householder kernel before Z is written
   }
@@ -627,16 +632,16 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     double* Zc = dZ;  // eigenvectors (columns) for the Kubo products
     double* Ab = A;   // scratch of the Kubo products
     if (f32) {
-      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZ, T1, A, B));
+      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZ, T1, A, B, dsc));
       Zc = A;  // refined vectors end in the A buffer; dZ is free
       Ab = dZ;
     } else {
       RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
       RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.Vg, eb.tau, dZ, false));
+      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.Vg, eb.tau, dZ, false));
       RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     }
-    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dE, j.spin_dir, j.eta, T1, Ab, B, dJ, dOm));
+    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, B, dJ, dOm));
     int l2 = 0;
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     if (j.mode == 0) {

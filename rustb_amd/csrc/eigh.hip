@@ -96,6 +96,7 @@ __global__ void __launch_bounds__(NB * S) eigh_tridiag_reg_kernel(int n, const d
 // each thread holding NB / S entries of one column of Z.
 template <typename T, int NB, int S>
 __global__ void __launch_bounds__(NB * S) eigh_backtrans_reg_kernel(int n, const T* __restrict__ Y,
+                                                                    const T* __restrict__ ysc,
                                                                     const cplx<T>* __restrict__ Vg,
                                                                     const cplx<T>* __restrict__ taug,
                                                                     double* __restrict__ Zout, int rows_out) {
@@ -104,20 +105,22 @@ __global__ void __launch_bounds__(NB * S) eigh_backtrans_reg_kernel(int n, const
   const size_t mat = blockIdx.x;
   const size_t nn = static_cast<size_t>(n) * n;
   RegSmem<T> sm{nullptr, vp, nullptr, yp, nullptr};
-  backtrans_reg<T, NB, S>(tm, n, Y + mat * nn, n, Vg + mat * nn, taug + mat * n, sm, Zout + mat * nn * 2,
+  backtrans_reg<T, NB, S>(tm, n, Y + mat * nn, n, ysc + mat * n, Vg + mat * nn, taug + mat * n, sm, Zout + mat * nn * 2,
                           rows_out);
 }
 
-// Slice counts: FP32 rows of 128 fit in 2 x 64 complex registers per thread;
-// FP64 rows of 128 are split over 4 slices (32 complex doubles per thread).
+// Slice counts (threads per row or column = S).  Tridiagonalisation at
+// n = 128: 4 slices (32 complex entries per thread, 8 waves per matrix, 4
+// waves per SIMD) beat 2 slices (64 entries, 2 waves per SIMD) by ~20% in
+// FP32; the back-transformation is indifferent and keeps 2 in FP32.
 template <typename T, int NB>
 struct RegShape;
-template <> struct RegShape<float, 32> { static constexpr int S = 2; };
-template <> struct RegShape<float, 64> { static constexpr int S = 2; };
-template <> struct RegShape<float, 128> { static constexpr int S = 2; };
-template <> struct RegShape<double, 32> { static constexpr int S = 2; };
-template <> struct RegShape<double, 64> { static constexpr int S = 2; };
-template <> struct RegShape<double, 128> { static constexpr int S = 4; };
+template <> struct RegShape<float, 32> { static constexpr int S = 2, Sbt = 2; };
+template <> struct RegShape<float, 64> { static constexpr int S = 2, Sbt = 2; };
+template <> struct RegShape<float, 128> { static constexpr int S = 4, Sbt = 2; };
+template <> struct RegShape<double, 32> { static constexpr int S = 2, Sbt = 2; };
+template <> struct RegShape<double, 64> { static constexpr int S = 2, Sbt = 2; };
+template <> struct RegShape<double, 128> { static constexpr int S = 4, Sbt = 4; };
 
 template <typename T, int NB>
 void launch_tridiag_reg(hipStream_t st, int n, size_t nmat, const double* H, T* Dg, T* Eg, cplx<T>* Vg,
@@ -140,30 +143,32 @@ void tridiag_reg_dispatch(hipStream_t st, int n, size_t nmat, const double* H, T
 }
 
 template <typename T, int NB>
-void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const cplx<T>* Vg, const cplx<T>* taug,
-                          double* Z, int rows_out) {
-  constexpr int S = RegShape<T, NB>::S;
+void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const T* ysc, const cplx<T>* Vg,
+                          const cplx<T>* taug, double* Z, int rows_out) {
+  constexpr int S = RegShape<T, NB>::Sbt;
   hipLaunchKernelGGL((eigh_backtrans_reg_kernel<T, NB, S>), dim3(static_cast<unsigned>(nmat)), dim3(NB * S), 0, st,
-                     n, Y, Vg, taug, Z, rows_out);
+                     n, Y, ysc, Vg, taug, Z, rows_out);
 }
 
 template <typename T>
-void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const cplx<T>* Vg, const cplx<T>* taug,
-                            double* Z, int rows_out) {
+void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const T* ysc, const cplx<T>* Vg,
+                            const cplx<T>* taug, double* Z, int rows_out) {
   if (n <= 32) {
-    launch_backtrans_reg<T, 32>(st, n, nmat, Y, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 32>(st, n, nmat, Y, ysc, Vg, taug, Z, rows_out);
   } else if (n <= 64) {
-    launch_backtrans_reg<T, 64>(st, n, nmat, Y, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 64>(st, n, nmat, Y, ysc, Vg, taug, Z, rows_out);
   } else {
-    launch_backtrans_reg<T, 128>(st, n, nmat, Y, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 128>(st, n, nmat, Y, ysc, Vg, taug, Z, rows_out);
   }
 }
 
 constexpr int kTeBlock = 256;
 
 // Tridiagonal eigenvalues (+ eigenvectors): sub-team of n threads per matrix.
+// With VEC, the per-eigenvector scale of Y (see tridiag_vectors) overwrites
+// the diagonal Dg once it has been read.
 template <typename T, bool VEC>
-__global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nmat, const T* __restrict__ Dg,
+__global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nmat, T* __restrict__ Dg,
                                                               const T* __restrict__ Eg, double* __restrict__ Eout,
                                                               T* __restrict__ Y, T* __restrict__ S) {
   __shared__ T sh[4 * kTeBlock];
@@ -189,7 +194,7 @@ __global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nma
     const size_t mm = live ? mat : 0;
     T* Ym = Y + mm * static_cast<size_t>(n) * n;
     T* Sm = S + mm * static_cast<size_t>(n) * n * 3;
-    tridiag_vectors<T>(tm, n, d, e, E, Ym, n, Sm);
+    tridiag_vectors<T>(tm, n, d, e, E, Ym, n, Sm, Dg + mm * n);
   }
 }
 
@@ -286,10 +291,10 @@ int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
   return RTB_OK;
 }
 
-int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* Vg, const float* taug,
-                            double* Z, bool rows_out) {
+int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* Vg,
+                            const float* taug, double* Z, bool rows_out) {
   if (nmat == 0) return RTB_OK;
-  backtrans_reg_dispatch<float>(st, n, nmat, Y, reinterpret_cast<const cplx<float>*>(Vg),
+  backtrans_reg_dispatch<float>(st, n, nmat, Y, DE, reinterpret_cast<const cplx<float>*>(Vg),
                                 reinterpret_cast<const cplx<float>*>(taug), Z, rows_out ? 1 : 0);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -321,10 +326,10 @@ int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
   return RTB_OK;
 }
 
-int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* Vg,
-                            const double* taug, double* Z, bool rows_out) {
+int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE,
+                            const double* Vg, const double* taug, double* Z, bool rows_out) {
   if (nmat == 0) return RTB_OK;
-  backtrans_reg_dispatch<double>(st, n, nmat, Y, reinterpret_cast<const cplx<double>*>(Vg),
+  backtrans_reg_dispatch<double>(st, n, nmat, Y, DE, reinterpret_cast<const cplx<double>*>(Vg),
                                  reinterpret_cast<const cplx<double>*>(taug), Z, rows_out ? 1 : 0);
   RTB_HIP(hipGetLastError());
   return RTB_OK;

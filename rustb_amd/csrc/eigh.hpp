@@ -16,20 +16,21 @@ int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
                      float* Vg, float* taug, float* Y, float* S, float* DE);
-int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* Vg, const float* taug,
-                            double* Z, bool rows_out);
+int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* Vg,
+                            const float* taug, double* Z, bool rows_out);
 // FP64 refinement of FP32 eigenpairs (refine.hip): one Ogita-Aishima step,
 // Rayleigh quotients, unit columns, Rayleigh-Ritz in clusters with gaps
 // below gap_rel * max(1, spectral radius).  Z: [nk][n][n] columns (in, then
-// scratch); H is overwritten; the refined columns are written to W2; W1, W3
-// scratch of the same size.
+// scratch); H is overwritten; the refined columns are written to W2, unit
+// after multiplication by scale[k][j] ([nk][n], written); W1, W3 scratch.
 int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, double* Z, double* W1, double* W2,
-                      double* W3, double* E, double gap_rel);
-// out[k][r][c] = Z[k][c][r] (eigenvector columns to rows)
-int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, double* out);
+                      double* W3, double* E, double* scale, double gap_rel);
+// out[k][r][c] = Z[k][c][r] * scale[k][r] (eigenvector columns to rows;
+// scale may be null)
+int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, double* out);
 // Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
-int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* Vg,
-                            const double* taug, double* Z, bool rows_out);
+int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE,
+                            const double* Vg, const double* taug, double* Z, bool rows_out);
 
 // ---- Fourier sum (fourier.hip) ----
 struct FourierArgs {
@@ -70,6 +71,10 @@ int mesh_points(hipStream_t st, int dim, const size_t* kmesh, size_t k0, size_t 
 // to be Hermitian; the upper triangle is filled from the lower one.
 int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
                   size_t sB, double* C, size_t sC, bool herm = false);
+// The same product with C[b][i][j] multiplied by s[b][i] s[b][j] (s: [nb][n],
+// stride n): diag(s) op(A) B diag(s), e.g. U^H J U for unit columns U diag(s).
+int zgemm_batched_scaled(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
+                         size_t sB, double* C, size_t sC, bool herm, const double* s);
 // C[b] = A[b] + A[b] * B[b] with the product on the FP32 matrix cores (FP64
 // storage): for small corrections B, whose product needs ~1e-7 relative
 // accuracy only.  C must not alias A or B.

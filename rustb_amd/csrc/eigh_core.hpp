@@ -447,19 +447,122 @@ RTB_HD void mgs_column(int n, T* Y, int ldy, int jj0, int jj) {
   for (int k = 0; k < n; ++k) Y[static_cast<size_t>(k) * ldy + jj] *= inv;
 }
 
-// Inverse iteration for all eigenvectors; cluster threshold ortol*tnorm as
-// in dstein (ortol = 1e-3).  Two solves, each followed by MGS in clusters.
+// Eigenvector of an isolated eigenvalue lam by a twisted factorisation
+// (Parlett & Dhillon, LAA 309 (2000) 121-151; Fernando, SIMAX 18 (1997)):
+// T - lam I = L+ D+ L+^T (top-down) = U- D- U-^T (bottom-up); the twist
+// index r minimises |gamma_k| = |D+_k + D-_k - (d_k - lam)|, and
+//   z_r = 1,  z_k = -(e_k / D+_k) z_{k+1} (k < r),
+//   z_k = -(e_{k-1} / D-_k) z_{k-1} (k > r)
+// solves (T - lam I) z = gamma_r e_r: one solve, no iteration, residual
+// |gamma_r| / ||z||.  Pivots below pivmin are replaced by -pivmin (the Sturm
+// count rule).  S: [2][n] scratch columns (D+ and D-), same layout as Y.
+// z is left unnormalised in Y; the return value is 1 / ||z||.
+template <typename T>
+RTB_HD T twisted_vector(int n, const T* d, const T* e, T lam, T pivmin, T* Y, int ldy, int j, T* S) {
+  // The recurrences are serial; the scratch reads are not, so they are
+  // issued kB at a time ahead of the arithmetic (one memory latency per
+  // block instead of one per step).
+  constexpr int kB = 16;
+  const size_t nn = static_cast<size_t>(n) * ldy;
+  T* Dp = S;
+  T* Dm = S + nn;
+#define Yat(i) Y[static_cast<size_t>(i) * ldy + j]
+#define SAt(arr, i) arr[static_cast<size_t>(i) * ldy + j]
+  T dp = d[0] - lam;
+  if (fabs(dp) < pivmin) dp = -pivmin;
+  SAt(Dp, 0) = dp;
+  for (int k = 0; k + 1 < n; ++k) {
+    dp = d[k + 1] - lam - e[k] * e[k] * recip(dp);
+    if (fabs(dp) < pivmin) dp = -pivmin;
+    SAt(Dp, k + 1) = dp;
+  }
+  T dm = d[n - 1] - lam;
+  if (fabs(dm) < pivmin) dm = -pivmin;
+  SAt(Dm, n - 1) = dm;
+  int r = n - 1;
+  T gmin = fabs(dp);  // gamma_{n-1} = D+_{n-1}
+  for (int k0 = n - 2; k0 >= 0; k0 -= kB) {
+    T pre[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) pre[u] = k0 - u >= 0 ? SAt(Dp, k0 - u) : T(0);
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        const T dk = d[k] - lam;
+        dm = dk - e[k] * e[k] * recip(dm);
+        if (fabs(dm) < pivmin) dm = -pivmin;
+        SAt(Dm, k) = dm;
+        const T g = fabs(pre[u] + dm - dk);
+        if (g < gmin) {
+          gmin = g;
+          r = k;
+        }
+      }
+    }
+  }
+  T z = T(1), nrm2 = T(1);
+  Yat(r) = z;
+  for (int k0 = r - 1; k0 >= 0; k0 -= kB) {
+    T pre[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) pre[u] = k0 - u >= 0 ? SAt(Dp, k0 - u) : T(1);
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        z = -e[k] * recip(pre[u]) * z;
+        Yat(k) = z;
+        nrm2 += z * z;
+      }
+    }
+  }
+  z = T(1);
+  for (int k0 = r + 1; k0 < n; k0 += kB) {
+    T pre[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) pre[u] = k0 + u < n ? SAt(Dm, k0 + u) : T(1);
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int k = k0 + u;
+      if (k < n) {
+        z = -e[k - 1] * recip(pre[u]) * z;
+        Yat(k) = z;
+        nrm2 += z * z;
+      }
+    }
+  }
+#undef Yat
+#undef SAt
+  return T(1) / sqrt(nrm2);
+}
+
+// Eigenvectors of T: eigenvalues farther than ortol * tnorm from both
+// neighbours (ortol = 1e-3, dstein's cluster threshold) by the twisted
+// factorisation; members of a cluster by inverse iteration (dstein: a
+// pivoted LU solve from a fixed pseudo-random start, twice, each followed
+// by modified Gram-Schmidt inside the cluster).  Column j of Y times ysc[j]
+// is the unit eigenvector (twisted vectors are stored unnormalised).
 template <typename T, typename Team>
 RTB_HD void tridiag_vectors(Team& tm, int n, const T* d, const T* e, const T* E, T* Y, int ldy,
-                            T* S) {
+                            T* S, T* ysc) {
   T gl, gu;
   gershgorin(n, d, e, gl, gu);
   const T tnorm = fmax(fabs(gl), fabs(gu));
   const T eps = Lim<T>::eps();
   const T tol = fmax(eps * tnorm, Lim<T>::safmin() * T(1e4));
   const T ortol = T(1e-3) * tnorm;
+  T emax2 = T(0);
+  for (int i = 0; i + 1 < n; ++i) emax2 = fmax(emax2, e[i] * e[i]);
+  const T pivmin = Lim<T>::safmin() * fmax(T(1), emax2);
   for (int pass = 0; pass < 2; ++pass) {
     for (int j = tm.id; j < n; j += tm.nt) {
+      const bool clustered = (j > 0 && E[j] - E[j - 1] <= ortol) || (j + 1 < n && E[j + 1] - E[j] <= ortol);
+      if (!clustered) {
+        if (pass == 0) ysc[j] = twisted_vector(n, d, e, E[j], pivmin, Y, ldy, j, S);
+        continue;
+      }
+      ysc[j] = T(1);
       if (pass == 0) {
         for (int i = 0; i < n; ++i)
           Y[static_cast<size_t>(i) * ldy + j] = static_cast<T>(rtb_hash_uniform(j, i));

@@ -226,14 +226,15 @@ RTB_HD void tridiag_reg(Team& tm, int n, const double* Hm, RegSmem<T> sm, T* Dg,
 
 // --------------------------------------------------------------------------
 // Z = Q Y for the reflectors of tridiag_reg (zunmtr, SIDE = 'L', 'N'):
-// Y real [n][ldy] (eigenvector c in column c), Z complex FP64 written as
+// Y real [n][ldy] (eigenvector c in column c, scaled by ysc[c] on load),
+// Z complex FP64 written as
 //   rows_out = 0: Zout[r][c] (eigenvectors as columns)
 //   rows_out = 1: Zout[c][r] (eigenvectors as rows)
 // Per reflector j = n-2 .. 0: s_c = tau_j v_j^H Z[:, c], Z[:, c] -= v_j s_c.
 // --------------------------------------------------------------------------
 template <typename T, int NB, int S, typename Team>
-RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const cplx<T>* Vg, const cplx<T>* taug,
-                          RegSmem<T> sm, double* Zout, int rows_out) {
+RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* ysc, const cplx<T>* Vg,
+                          const cplx<T>* taug, RegSmem<T> sm, double* Zout, int rows_out) {
   constexpr int C = NB / S;
   constexpr int BL = C < 8 ? C : 8;
   constexpr int NBL = C / BL;
@@ -242,10 +243,11 @@ RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const cplx<T>* V
   {
     const T* yb = Y + static_cast<size_t>(s) * ldy + (c < n ? c : 0);  // Y[s + S q][c]
     const size_t ystep = static_cast<size_t>(S) * ldy;
+    const T sc = c < n ? ysc[c] : T(0);
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int r = s + S * q;
-      z[q] = {(r < n && c < n) ? yb[q * ystep] : T(0), T(0)};
+      z[q] = {(r < n && c < n) ? yb[q * ystep] * sc : T(0), T(0)};
 #if defined(__HIP_DEVICE_COMPILE__)
       // keep the zero imaginary parts opaque: otherwise the no-reflector path
       // is specialised with 2 * C constant doubles live at the stores

@@ -19,14 +19,17 @@ namespace rtb {
 namespace {
 
 
-__global__ void rayleigh_kernel(int n, size_t nk, double2* __restrict__ Z, double2* __restrict__ HZ,
-                                double* __restrict__ E) {
+// Rayleigh quotient E_j = Re(z_j^H (H z_j)) / (z_j^H z_j) and the column
+// scale 1 / ||z_j||; the columns themselves are left unnormalised (their
+// consumers apply the scale).
+__global__ void rayleigh_kernel(int n, size_t nk, const double2* __restrict__ Z, const double2* __restrict__ HZ,
+                                double* __restrict__ E, double* __restrict__ scale) {
   const size_t g = blockIdx.x * 256ull + threadIdx.x;
   if (g >= nk * n) return;
   const size_t k = g / n;
   const int j = static_cast<int>(g % n);
-  double2* Zk = Z + k * n * n;
-  double2* Hk = HZ + k * n * n;
+  const double2* Zk = Z + k * n * n;
+  const double2* Hk = HZ + k * n * n;
   double num = 0.0, den = 0.0;
   for (int r = 0; r < n; ++r) {
     const double2 z = Zk[static_cast<size_t>(r) * n + j];
@@ -35,15 +38,7 @@ __global__ void rayleigh_kernel(int n, size_t nk, double2* __restrict__ Z, doubl
     den += z.x * z.x + z.y * z.y;
   }
   E[g] = num / den;
-  const double s = 1.0 / sqrt(den);
-  for (int r = 0; r < n; ++r) {  // keep HZ = H Z for the unit columns
-    double2& z = Zk[static_cast<size_t>(r) * n + j];
-    z.x *= s;
-    z.y *= s;
-    double2& h = Hk[static_cast<size_t>(r) * n + j];
-    h.x *= s;
-    h.y *= s;
-  }
+  scale[g] = 1.0 / sqrt(den);
 }
 
 __device__ inline double wave_sum(double x) {
@@ -55,10 +50,11 @@ __device__ inline double wave_sum(double x) {
 // One wave per k-point: the lanes test every eigenvalue gap in parallel; a
 // k-point with a cluster forms the cluster's Gram and projection sums with
 // the lanes over rows, lane 0 solves the c x c problem (LDS), and the lanes
-// rotate the cluster's columns row by row.  HZ = H Z for the unit columns.
+// rotate the cluster's columns row by row.  Column j of Z (and of HZ = H Z)
+// times scale[j] is the unit vector; rotated columns are written unit.
 __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restrict__ Z,
                                                         const double2* __restrict__ HZ, double* __restrict__ E,
-                                                        double gap_rel) {
+                                                        double* __restrict__ scale, double gap_rel) {
   __shared__ C2 lds[7][kMaxCluster][kMaxCluster];
   __shared__ double ev[kMaxCluster];
   __shared__ int solved;
@@ -75,6 +71,7 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
   if (!__any(close)) return;
   double2* Zk = Z + k * n * n;
   const double2* Hk = HZ + k * n * n;
+  double* sk = scale + k * n;
   int j0 = 0;
   while (j0 < n) {  // the same cluster sequence in every lane
     const int j1 = cluster_end(n, Ek, j0, thr);
@@ -92,10 +89,12 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
           mr += za.x * hb.x + za.y * hb.y;  // conj(za) hb
           mi += za.x * hb.y - za.y * hb.x;
         }
-        gr = wave_sum(gr);
-        gi = wave_sum(gi);
-        mr = wave_sum(mr);
-        mi = wave_sum(mi);
+        // sums over the unit columns s_a z_a, s_b z_b (H z_b scales with z_b)
+        const double sab = sk[j0 + a] * sk[j0 + b];
+        gr = wave_sum(gr) * sab;
+        gi = wave_sum(gi) * sab;
+        mr = wave_sum(mr) * sab;
+        mi = wave_sum(mi) * sab;
         if (lane == 0) {
           lds[0][a][b] = {gr, gi};
           lds[1][a][b] = {mr, mi};
@@ -109,7 +108,7 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
           C2 row[kMaxCluster];
 #pragma unroll
           for (int q = 0; q < kMaxCluster; ++q)
-            if (q < c) row[q] = {zr[q].x, zr[q].y};
+            if (q < c) row[q] = {zr[q].x * sk[j0 + q], zr[q].y * sk[j0 + q]};
 #pragma unroll
           for (int bb = 0; bb < kMaxCluster; ++bb) {
             if (bb < c) {
@@ -126,7 +125,11 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
             }
           }
         }
-        if (lane < c) Ek[j0 + lane] = ev[lane];
+        __syncthreads();  // every lane has read the old scales
+        if (lane < c) {
+          Ek[j0 + lane] = ev[lane];
+          sk[j0 + lane] = 1.0;  // the rotated columns are unit vectors
+        }
       }
       __syncthreads();
     }
@@ -177,22 +180,25 @@ __global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2
   }
 }
 
-__global__ void transpose_kernel(int n, size_t nk, const double2* __restrict__ Z, double2* __restrict__ out) {
+__global__ void transpose_kernel(int n, size_t nk, const double2* __restrict__ Z, const double* __restrict__ scale,
+                                 double2* __restrict__ out) {
   const size_t g = blockIdx.x * 256ull + threadIdx.x;
   const size_t n2 = static_cast<size_t>(n) * n;
   if (g >= nk * n2) return;
   const size_t k = g / n2, e = g % n2;
-  const size_t r = e / n, c = e % n;  // out[k][r][c] = Z[k][c][r]
-  out[g] = Z[k * n2 + c * n + r];
+  const size_t r = e / n, c = e % n;  // out[k][r][c] = Z[k][c][r] * scale[k][r]
+  const double s = scale ? scale[k * n + r] : 1.0;
+  const double2 z = Z[k * n2 + c * n + r];
+  out[g] = make_double2(z.x * s, z.y * s);
 }
 
 }  // namespace
 
-int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, double* out) {
+int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, double* out) {
   const size_t tot = nk * static_cast<size_t>(n) * n;
   if (tot == 0) return RTB_OK;
   hipLaunchKernelGGL(transpose_kernel, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, st, n, nk,
-                     reinterpret_cast<const double2*>(Z), reinterpret_cast<double2*>(out));
+                     reinterpret_cast<const double2*>(Z), scale, reinterpret_cast<double2*>(out));
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
@@ -208,9 +214,10 @@ int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, double*
 //   on the FP32 matrix cores: their rounding (~1e-7 relative to a correction
 //   already ~1e-6 or smaller) stays below the FP64 level of the sums.
 // Buffers (each [nk][n][n] complex): H (consumed), Z (in; scratch after),
-// W1, W3 scratch; the refined eigenvectors (columns) are left in W2.
+// W1, W3 scratch; the refined eigenvectors are left in W2 as columns that
+// become unit vectors when multiplied by scale[k][j] ([nk][n]).
 int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, double* Z, double* W1, double* W2,
-                      double* W3, double* E, double gap_rel) {
+                      double* W3, double* E, double* scale, double gap_rel) {
   if (nk == 0) return RTB_OK;
   const size_t n2 = static_cast<size_t>(n) * n;
   // FP64 products: HZ, S = Z^H HZ, G = Z^H Z
@@ -232,10 +239,10 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, d
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
   const size_t tk = nk * n;
   hipLaunchKernelGGL(rayleigh_kernel, dim3(static_cast<unsigned>((tk + 255) / 256)), dim3(256), 0, st, n, nk,
-                     reinterpret_cast<double2*>(W2), reinterpret_cast<double2*>(W3), E);
+                     reinterpret_cast<const double2*>(W2), reinterpret_cast<const double2*>(W3), E, scale);
   RTB_HIP(hipGetLastError());
   hipLaunchKernelGGL(cluster_rr_kernel, dim3(static_cast<unsigned>(nk)), dim3(64), 0, st, n,
-                     reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W3), E, gap_rel);
+                     reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W3), E, scale, gap_rel);
   RTB_HIP(hipGetLastError());
   if (tmr) RTB_TRY(tmr->end(2));
   return RTB_OK;

@@ -13,6 +13,8 @@
 //   * herm = 1: op(A) B is Hermitian (X^H Y with X^H Y = Y^H X, e.g. U^H J U);
 //     only tiles on or below the diagonal are computed and the off-diagonal
 //     ones are also stored conjugate-transposed;
+//   * an optional per-matrix vector s scales the result to diag(s) C diag(s)
+//     (products with unnormalised eigenvector columns U, unit after U diag(s));
 //   * workgroup w runs on XCD w % 8; the tiles of one matrix are given to
 //     workgroups on the same XCD so that they share its L2.
 //
@@ -68,12 +70,13 @@ struct Mfma<float> {
   static __device__ int row(int r, int fk) { return 4 * fk + r; }
 };
 
-// C = [D +] op(A) B; D may alias A (it is read only), C aliases nothing.
+// C = [D +] op(A) B, optionally scaled to diag(s) op(A) B diag(s) (s: n per
+// matrix); D may alias A (it is read only), C aliases nothing.
 template <typename T, bool ADD>
 __global__ void __launch_bounds__(256) zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A,
                                                       size_t sA, int conj_a, const double2* __restrict__ B, size_t sB,
                                                       const double2* D, size_t sD, double2* __restrict__ C,
-                                                      size_t sC) {
+                                                      size_t sC, const double* __restrict__ sc) {
   typedef typename Mfma<T>::v4 v4;
   __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
   __shared__ T sB3[3][kKc][kLd];  // B tile, k-major
@@ -88,6 +91,7 @@ __global__ void __launch_bounds__(256) zgemm3m_kernel(int n, size_t nmat, TileMa
   B += mat * sB;
   C += mat * sC;
   if (ADD) D += mat * sD;
+  if (sc) sc += mat * n;
   const int row0 = ti * kTile, col0 = tj * kTile;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -186,6 +190,11 @@ __global__ void __launch_bounds__(256) zgemm3m_kernel(int n, size_t nmat, TileMa
         if (i < n && j < n) {
           const double p1 = acc[0][a][b][r], p2 = acc[1][a][b][r], p3 = acc[2][a][b][r];
           double re = p1 - p2, im = p3 - p1 - p2;
+          if (sc) {
+            const double f = sc[i] * sc[j];
+            re *= f;
+            im *= f;
+          }
           const size_t o = static_cast<size_t>(i) * n + j;
           if (ADD) {
             const double2 d = D[o];
@@ -200,7 +209,7 @@ __global__ void __launch_bounds__(256) zgemm3m_kernel(int n, size_t nmat, TileMa
 
 template <typename T, bool ADD>
 int zgemm_launch(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
-                 size_t sB, const double* D, size_t sD, double* C, size_t sC, bool herm) {
+                 size_t sB, const double* D, size_t sD, double* C, size_t sC, bool herm, const double* sc) {
   if (nb == 0 || n <= 0) return RTB_OK;
   TileMap tm;
   tm.side = (n + kTile - 1) / kTile;
@@ -217,7 +226,7 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const double* A, size_t sA, b
                        reinterpret_cast<const double2*>(A) + off * sA, sA, conj_a ? 1 : 0,
                        reinterpret_cast<const double2*>(B) + off * sB, sB,
                        D ? reinterpret_cast<const double2*>(D) + off * sD : nullptr, sD,
-                       reinterpret_cast<double2*>(C) + off * sC, sC);
+                       reinterpret_cast<double2*>(C) + off * sC, sC, sc ? sc + off * n : nullptr);
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -227,12 +236,17 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const double* A, size_t sA, b
 
 int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
                   size_t sB, double* C, size_t sC, bool herm) {
-  return zgemm_launch<double, false>(st, n, nb, A, sA, conj_a, B, sB, nullptr, 0, C, sC, herm);
+  return zgemm_launch<double, false>(st, n, nb, A, sA, conj_a, B, sB, nullptr, 0, C, sC, herm, nullptr);
+}
+
+int zgemm_batched_scaled(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
+                         size_t sB, double* C, size_t sC, bool herm, const double* s) {
+  return zgemm_launch<double, false>(st, n, nb, A, sA, conj_a, B, sB, nullptr, 0, C, sC, herm, s);
 }
 
 int zgemm_add_f32_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, const double* B, size_t sB,
                           double* C, size_t sC) {
-  return zgemm_launch<float, true>(st, n, nb, A, sA, false, B, sB, A, sA, C, sC, false);
+  return zgemm_launch<float, true>(st, n, nb, A, sA, false, B, sB, A, sA, C, sC, false, nullptr);
 }
 
 }  // namespace rtb

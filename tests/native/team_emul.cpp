@@ -64,7 +64,7 @@ int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
   std::vector<cplx<T>> P(static_cast<size_t>(n) * (n + 1) / 2), v(n), w(n);
   std::vector<T> d(n), e(n), e2(n), Ev(n), red(4 * 2048 + nt);
   std::vector<cplx<T>> Vg(static_cast<size_t>(n) * n), taug(n);
-  std::vector<T> Y(static_cast<size_t>(n) * n), S(static_cast<size_t>(n) * n * 3);
+  std::vector<T> Y(static_cast<size_t>(n) * n), S(static_cast<size_t>(n) * n * 3), ysc(n);
   std::vector<cplx<T>> Zb(static_cast<size_t>(n) * n), vs(2 * n), part(nt > n ? nt : n), sv(n);
   for (int r = 0; r < n; ++r)
     for (int c = 0; c <= r; ++c) {
@@ -78,8 +78,8 @@ int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
                       taug.data());
     bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data());
     if (vectors) {
-      tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, S.data());
-      for (int f = id; f < n * n; f += nt) Zb[f] = {Y[f], T(0)};
+      tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, S.data(), ysc.data());
+      for (int f = id; f < n * n; f += nt) Zb[f] = {Y[f] * ysc[f % n], T(0)};
       tm.sync();
       apply_q<T>(tm, n, n, Zb.data(), Vg.data(), taug.data(), vs.data(), part.data(), sv.data());
     }
@@ -98,14 +98,15 @@ int run(int n, int nt, const double* H, double* E, double* Z, int vectors) {
 
 // Register-resident route: tridiag_reg and backtrans_reg with NB * S threads,
 // the tridiagonal eigenpairs with the team-generic routines in between.
-template <typename T, int NB, int S>
+template <typename T, int NB, int S, int Sbt>
 int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_out) {
   using namespace rtb;
-  constexpr int NT = NB * S;
-  std::vector<cplx<T>> xc(NB), vp(3 * NB), wp(NB), yp(2 * S * NB);
+  constexpr int NT = NB * S, NTb = NB * Sbt;
+  constexpr int SM = S > Sbt ? S : Sbt;
+  std::vector<cplx<T>> xc(NB), vp(3 * NB), wp(NB), yp(2 * SM * NB);
   std::vector<T> red(2 * 2048), d(n), e(n), e2(n), Ev(n), red2(4 * 2048 + n);
   std::vector<cplx<T>> Vg(static_cast<size_t>(n) * n), taug(n);
-  std::vector<T> Y(static_cast<size_t>(n) * n), Sw(static_cast<size_t>(n) * n * 3);
+  std::vector<T> Y(static_cast<size_t>(n) * n), Sw(static_cast<size_t>(n) * n * 3), ysc(n);
   RegSmem<T> sm{xc.data(), vp.data(), wp.data(), yp.data(), red.data()};
   auto team = [](int nt, auto&& f) {
     std::barrier<> bar(nt);
@@ -119,12 +120,12 @@ int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_
   team(NT, [&](auto& tm) { tridiag_reg<T, NB, S>(tm, n, H, sm, d.data(), e.data(), Vg.data(), taug.data()); });
   team(n, [&](auto& tm) {
     bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data());
-    if (vectors) tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, Sw.data());
+    if (vectors) tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, Sw.data(), ysc.data());
   });
   for (int j = 0; j < n; ++j) E[j] = static_cast<double>(Ev[j]);
   if (vectors)
-    team(NT, [&](auto& tm) {
-      backtrans_reg<T, NB, S>(tm, n, Y.data(), n, Vg.data(), taug.data(), sm, Z, rows_out);
+    team(NTb, [&](auto& tm) {
+      backtrans_reg<T, NB, Sbt>(tm, n, Y.data(), n, ysc.data(), Vg.data(), taug.data(), sm, Z, rows_out);
     });
   return 0;
 }
@@ -133,14 +134,15 @@ int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_
 
 extern "C" int emul_eigh_reg(int n, int fp32, const double* H, double* E, double* Z, int vectors, int rows_out) {
   if (n < 1 || n > 128) return -1;
+  // slice shapes of the GPU dispatch (eigh.hip RegShape)
   if (fp32) {
-    if (n <= 32) return run_reg<float, 32, 2>(n, H, E, Z, vectors, rows_out);
-    if (n <= 64) return run_reg<float, 64, 2>(n, H, E, Z, vectors, rows_out);
-    return run_reg<float, 128, 2>(n, H, E, Z, vectors, rows_out);
+    if (n <= 32) return run_reg<float, 32, 2, 2>(n, H, E, Z, vectors, rows_out);
+    if (n <= 64) return run_reg<float, 64, 2, 2>(n, H, E, Z, vectors, rows_out);
+    return run_reg<float, 128, 4, 2>(n, H, E, Z, vectors, rows_out);
   }
-  if (n <= 32) return run_reg<double, 32, 2>(n, H, E, Z, vectors, rows_out);
-  if (n <= 64) return run_reg<double, 64, 2>(n, H, E, Z, vectors, rows_out);
-  return run_reg<double, 128, 4>(n, H, E, Z, vectors, rows_out);
+  if (n <= 32) return run_reg<double, 32, 2, 2>(n, H, E, Z, vectors, rows_out);
+  if (n <= 64) return run_reg<double, 64, 2, 2>(n, H, E, Z, vectors, rows_out);
+  return run_reg<double, 128, 4, 4>(n, H, E, Z, vectors, rows_out);
 }
 
 extern "C" int emul_eigh(int n, int nt, int fp32, const double* H, double* E, double* Z, int vectors) {
