@@ -66,6 +66,22 @@ def traffic_from_profiles(kernel):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """Device-to-device copies of a known size (HIP's blit kernel), the
+    calibration run for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
+
 def cpu_baseline(model, kmesh, dirs, eta, target_s):
     from oracle import cpu_ref
 
@@ -125,11 +141,17 @@ def main():
     ap.add_argument("--kmesh", default="48,48,48")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 3 device copies of 1 GiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(k-points shard trivially; see the sharding notes). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        copy_probe()
+        return
 
     import rustb_amd as ra
     from rustb_amd import _lib
