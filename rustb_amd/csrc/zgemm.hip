@@ -71,12 +71,15 @@ struct Mfma<float> {
 };
 
 // C = [D +] op(A) B, optionally scaled to diag(s) op(A) B diag(s) (s: n per
-// matrix); D may alias A (it is read only), C aliases nothing.
+// matrix); D may alias A (it is read only), C aliases nothing.  The
+// waves-per-SIMD attribute (FP64 2, FP32 4) bounds the registers so that the
+// software-pipelined fragments fit next to the accumulators without spills.
 template <typename T, bool ADD>
-__global__ void __launch_bounds__(256) zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A,
-                                                      size_t sA, int conj_a, const double2* __restrict__ B, size_t sB,
-                                                      const double2* D, size_t sD, double2* __restrict__ C,
-                                                      size_t sC, const double* __restrict__ sc) {
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4, sizeof(T) == 8 ? 2 : 4)))
+zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A, size_t sA, int conj_a,
+               const double2* __restrict__ B, size_t sB, const double2* D, size_t sD, double2* __restrict__ C,
+               size_t sC, const double* __restrict__ sc) {
   typedef typename Mfma<T>::v4 v4;
   __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
   __shared__ T sB3[3][kKc][kLd];  // B tile, k-major
@@ -158,24 +161,30 @@ __global__ void __launch_bounds__(256) zgemm3m_kernel(int n, size_t nmat, TileMa
     store_chunk();
     __syncthreads();
     if (k0 + kKc < n) load_chunk(k0 + kKc);  // in flight during the MFMAs
-    // one k-step at a time keeps the operand registers few (two waves per
-    // SIMD); each step is 12 MFMAs against 12 LDS reads
-#pragma unroll 1
-    for (int kk = 0; kk < kKc; kk += 4) {
-      T av[3][2], bv[3][2];
+    // the fragments of k-step s + 1 are read from LDS while the 12 MFMAs of
+    // k-step s issue (two fragment register sets)
+    T av[2][3][2], bv[2][3][2];
+    auto frag = [&](int set, int kk) {
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-          av[p][a] = sA3[p][kk + fk][wr + 16 * a + fr];
-          bv[p][a] = sB3[p][kk + fk][wc + 16 * a + fr];
+          av[set][p][a] = sA3[p][kk + fk][wr + 16 * a + fr];
+          bv[set][p][a] = sB3[p][kk + fk][wc + 16 * a + fr];
         }
+    };
+    frag(0, 0);
+#pragma unroll
+    for (int ks = 0; ks < kKc / 4; ++ks) {
+      const int cur = ks & 1;
+      if (ks + 1 < kKc / 4) frag(cur ^ 1, 4 * (ks + 1));
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-          for (int b = 0; b < 2; ++b) acc[p][a][b] = Mfma<T>::mac(av[p][a], bv[p][b], acc[p][a][b]);
+          for (int b = 0; b < 2; ++b)
+            acc[p][a][b] = Mfma<T>::mac(av[cur][p][a], bv[cur][p][b], acc[p][a][b]);
     }
   }
   const bool mirror = tm.herm && ti != tj;
