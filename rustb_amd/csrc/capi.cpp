@@ -2,6 +2,7 @@
 // k-chunking, device work buffers and the kernel pipeline of each call.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -65,10 +66,7 @@ int g_smem_rc = RTB_OK;
 int prepare(const rtb_model* m) {
   if (!m) return fail(RTB_ERR_INVALID_ARGUMENT, "null model");
   RTB_HIP(hipSetDevice(m->device));
-  std::call_once(g_smem_once, [] {
-    g_smem_rc = eigh_set_smem_limits();
-    if (g_smem_rc == RTB_OK) g_smem_rc = mesh_set_smem_limits();
-  });
+  std::call_once(g_smem_once, [] { g_smem_rc = mesh_set_smem_limits(); });
   return g_smem_rc;
 }
 
@@ -187,8 +185,21 @@ int upload_k(hipStream_t st, Work& w, int dim, const double* k, size_t nk, doubl
   return RTB_OK;
 }
 
+// Device bytes one call may hold for its k-point chunk: RTB_WORKSPACE_MB
+// when set, else an eighth of the device memory (36 GB on a 288 GB part),
+// at most 32 GiB.  Larger chunks mean fewer, fuller launches.
+size_t workspace_budget() {
+  if (const char* env = std::getenv("RTB_WORKSPACE_MB")) {
+    const long long mb = std::atoll(env);
+    if (mb > 0) return static_cast<size_t>(mb) << 20;
+  }
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || total_b == 0) return size_t(4) << 30;
+  return std::min(total_b / 8, size_t(32) << 30);
+}
+
 size_t chunk_for(size_t nk, size_t per_k_bytes) {
-  const size_t budget = size_t(4) << 30;
+  const size_t budget = workspace_budget();
   size_t c = std::max<size_t>(1, budget / std::max<size_t>(per_k_bytes, 1));
   return std::min(nk, c);
 }
@@ -202,7 +213,7 @@ struct EigBufs {
 int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
   const size_t n2 = static_cast<size_t>(n) * n;
   b.Vg = b.tau = nullptr;
-  RTB_TRY(w.alloc(&b.DE, 2 * K * n));
+  RTB_TRY(w.alloc(&b.DE, 3 * K * n));
   if (vectors) {
     RTB_TRY(w.alloc(&b.Vg, K * n2 * 2));
     RTB_TRY(w.alloc(&b.tau, K * n * 2));
@@ -228,7 +239,7 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, doub
   float* DE = reinterpret_cast<float*>(eb.DE);
   RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE));
   RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, Vg, tau, Z, false));
+  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Z, false));
   RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
   // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
   // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius
@@ -236,31 +247,32 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, doub
   return RTB_OK;
 }
 
-// Kubo for a chunk: V = [K][2][n][n]; Z = [K][n][n] (columns); omega [K][n]
-//   A = Z^H J Z, B = Z^H v_b Z (response/traits.rs:177-180)
+// Kubo for a chunk: V = [K][n][n] (V0 + i V1) or [K][2][n][n] (spin current);
+// Z = [K][n][n] (columns); omega [K][n].  The reference forms A = Z^H J Z and
+// B = Z^H v_b Z (response/traits.rs:177-180); here C = Z^H (J + i v_b) Z = A + i B,
+// two products instead of four, and the band sum reads A, B out of C.
 int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const double* V, const double* Z,
-               const double* zscale, const double* E, int spin_dir, double eta, double* T1, double* A, double* B,
-               double* J, double* omega) {
+               const double* zscale, const double* E, int spin_dir, double eta, double* T1, double* C, double* J,
+               double* omega) {
   const int n = m->nsta;
   const size_t n2 = static_cast<size_t>(n) * n;
-  const double* Ja = V;
-  size_t sJ = 2 * n2;  // complex elements between consecutive k in V
+  // the operator pair as one matrix M = A_op + i B_op: V holds V0 + i V1
+  // (combined by the Fourier stage), or V0, V1 apart when the first
+  // operator is the spin current J(V0)
+  const double* M = V;
   if (m->spin && spin_dir != RTB_SPIN_NONE) {
     RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
-    RTB_TRY(spin_current(st, m->norb, K, spin_dir, V, 2 * n2 * 2, J));
+    RTB_TRY(spin_current(st, m->norb, K, spin_dir, V, 2 * n2 * 2, V + n2 * 2, J));
     RTB_TRY(tmr.end(1));
-    Ja = J;
-    sJ = n2;
+    M = J;
   }
   RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
-  // Z diag(zscale) are the unit eigenvectors (zscale null: Z is unit)
-  RTB_TRY(zgemm_batched(st, n, K, Ja, sJ, false, Z, n2, T1, n2));
-  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, A, n2, true, zscale));
-  RTB_TRY(zgemm_batched(st, n, K, V + n2 * 2, 2 * n2, false, Z, n2, T1, n2));
-  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, B, n2, true, zscale));
-  RTB_TRY(tmr.end(4, 2 * flop_zgemm(n, K) + 2 * flop_zgemm_herm(n, K)));
+  // C = U^H M U for the unit eigenvectors U = Z diag(zscale) (zscale null: Z is unit)
+  RTB_TRY(zgemm_batched(st, n, K, M, n2, false, Z, n2, T1, n2));
+  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, C, n2, false, zscale));
+  RTB_TRY(tmr.end(2, 2 * flop_zgemm(n, K)));
   RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
-  RTB_TRY(kubo_omega(st, n, K, A, B, E, eta, omega));
+  RTB_TRY(kubo_omega(st, n, K, C, E, eta, omega));
   RTB_TRY(tmr.end(1));
   return RTB_OK;
 }
@@ -438,7 +450,7 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
   // the mixed form runs the FP32 route (with FP64 refinement) when
   // eigenvectors are requested; eigenvalue-only solves stay FP64
   const bool f32 = vec && precision == RTB_PREC_MIXED;
-  const size_t per_k = 24 + n2 * 16 * 2 + n * 24 + (vec ? n2 * 32 + n2 * 16 : 0) + (f32 ? n2 * 32 + n * 8 : 0);
+  const size_t per_k = 24 + n2 * 16 * 2 + n * 32 + (vec ? n2 * 32 + n2 * 16 : 0) + (f32 ? n2 * 32 + n * 8 : 0);
   const size_t K = chunk_for(nk, per_k);
   double *dk, *dH, *dE, *dZ = nullptr, *dZc = nullptr, *dHZ = nullptr, *dsc = nullptr;
   EigBufs eb;
@@ -472,7 +484,7 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
       RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
       if (vec) {
         RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-        RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.Vg, eb.tau, dZ, true));
+        RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, true));
         RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
       }
     }
@@ -532,11 +544,13 @@ static int mesh_setup(hipStream_t st, Work& w, Timer& tmr, const rtb_model* m, c
     use_mesh = mp.ok && static_cast<size_t>(mp.N[0]) <= K;
   }
   chunks.clear();
+  // equal chunks (a short last chunk would run its launches under-filled)
   if (!use_mesh) {
-    for (size_t k0 = 0; k0 < nk; k0 += K) chunks.push_back({k0, std::min(K, nk - k0), 0, 0});
+    const size_t nc = (nk + K - 1) / K, Kb = (nk + nc - 1) / nc;
+    for (size_t k0 = 0; k0 < nk; k0 += Kb) chunks.push_back({k0, std::min(Kb, nk - k0), 0, 0});
     return RTB_OK;
   }
-  const size_t KP = K / mp.N[0];
+  const size_t KP0 = K / mp.N[0], ncp = (mp.P + KP0 - 1) / KP0, KP = (mp.P + ncp - 1) / ncp;
   for (size_t p0 = 0; p0 < mp.P; p0 += KP) {
     const size_t np = std::min(KP, mp.P - p0);
     chunks.push_back({p0 * mp.N[0], np * mp.N[0], p0, np});
@@ -565,10 +579,11 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   a.gauge = RTB_GAUGE_ATOM;  // berry_curvature_at_impl: gen_v_projected(k, Gauge::Atom, ..)
   const size_t n2 = static_cast<size_t>(n) * n;
   const bool spinj = m->spin && j.spin_dir != RTB_SPIN_NONE;
+  a.combine = spinj ? 0 : 1;  // V0 + i V1 straight from the Fourier stage
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const bool f32 = j.precision == RTB_PREC_MIXED;  // FP32 eigensolve + FP64 refinement
   const size_t per_k =
-      24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 48 + (f32 ? n2 * 16 + n * 8 : 0);
+      24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 + (f32 ? n2 * 16 + n * 8 : 0);
   const size_t K = chunk_for(j.nk, per_k);
   MeshPlan mp;
   double* G = nullptr;
@@ -638,10 +653,10 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     } else {
       RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
       RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.Vg, eb.tau, dZ, false));
+      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, false));
       RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     }
-    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, B, dJ, dOm));
+    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, dJ, dOm));
     int l2 = 0;
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     if (j.mode == 0) {
@@ -778,7 +793,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   RTB_TRY(upload_dirs(st, w, m, nullptr, 0, a));
   a.gauge = RTB_GAUGE_ATOM;
   const size_t n2 = static_cast<size_t>(n) * n;
-  const size_t K = chunk_for(nk, 24 + n2 * 32 + n * 24);
+  const size_t K = chunk_for(nk, 24 + n2 * 32 + n * 32);
   MeshPlan mp;
   double* G = nullptr;
   std::vector<Chunk> chunks;

@@ -1,10 +1,16 @@
 // Batched Hermitian eigensolver kernels (gfx950).
 //
-//   eigh_tridiag_kernel     one workgroup per matrix: Householder reduction
-//                           with the packed lower triangle in LDS
-//   eigh_trieig_kernel      one thread per eigenvalue, 256/n matrices per
-//                           workgroup: bisection (+ inverse iteration)
-//   eigh_backtransform_kernel  Z = Q Y per block of columns
+//   eigh_tridiag_reg_kernel    one workgroup of NB * S threads per matrix:
+//                              register-resident Householder reduction
+//   eigh_trieig_kernel         one thread per eigenvalue, 256/n matrices per
+//                              workgroup: bisection, then the twisted
+//                              factorisations of the isolated eigenvalues
+//   eigh_cluster_kernel        the same layout: inverse iteration for the
+//                              members of eigenvalue clusters (a workgroup
+//                              without clusters returns at once)
+//   eigh_backtrans_reg_kernel  one workgroup per matrix: the eigenvectors of
+//                              T (generated from the twisted factorisations)
+//                              and Z = Q Y, register-resident
 #include "eigh.hpp"
 
 #include "eigh_core.hpp"
@@ -14,66 +20,6 @@
 namespace rtb {
 
 namespace {
-
-template <typename T>
-struct TriSmem {
-  cplx<T>* P;
-  cplx<T>* v;
-  cplx<T>* w;
-  T *d, *e, *red;
-};
-
-template <typename T>
-__device__ TriSmem<T> carve_tri(void* base, int n) {
-  TriSmem<T> s;
-  char* p = static_cast<char*>(base);
-  s.P = reinterpret_cast<cplx<T>*>(p);
-  p += sizeof(cplx<T>) * (static_cast<size_t>(n) * (n + 1) / 2);
-  s.v = reinterpret_cast<cplx<T>*>(p);
-  p += sizeof(cplx<T>) * n;
-  s.w = reinterpret_cast<cplx<T>*>(p);
-  p += sizeof(cplx<T>) * n;
-  s.d = reinterpret_cast<T*>(p);
-  s.e = s.d + n;
-  s.red = s.e + n;
-  return s;
-}
-
-// H: [nmat][n][n] complex FP64 (row-major; the lower triangle is read).
-// Dg, Eg: [nmat][n] tridiagonal (diagonal, sub-diagonal); Vg, taug reflectors.
-template <typename T>
-__global__ void __launch_bounds__(256) eigh_tridiag_kernel(int n, const double* __restrict__ H, T* __restrict__ Dg,
-                                                           T* __restrict__ Eg, cplx<T>* __restrict__ Vg,
-                                                           cplx<T>* __restrict__ taug) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  GpuTeam tm;
-  const size_t mat = blockIdx.x;
-  TriSmem<T> s = carve_tri<T>(smem_raw, n);
-  const double* Hm = H + mat * static_cast<size_t>(n) * n * 2;
-  // lower triangle into packed storage, rows walked incrementally
-  const int np = n * (n + 1) / 2;
-  {
-    int r = 0, rs = 0;
-    for (int f = tm.id; f < np; f += tm.nt) {
-      while (f >= rs + r + 1) {
-        rs += r + 1;
-        ++r;
-      }
-      const int c = f - rs;
-      const double2 h = reinterpret_cast<const double2*>(Hm)[static_cast<size_t>(r) * n + c];
-      s.P[f].re = static_cast<T>(h.x);
-      s.P[f].im = static_cast<T>(h.y);
-    }
-  }
-  tm.sync();
-  cplx<T>* Vm = Vg ? Vg + mat * static_cast<size_t>(n) * n : nullptr;
-  cplx<T>* tu = taug ? taug + mat * static_cast<size_t>(n) : nullptr;
-  tridiagonalize<T>(tm, n, s.P, s.v, s.w, s.d, s.e, s.red, Vm, tu);
-  for (int i = tm.id; i < n; i += tm.nt) {
-    Dg[mat * n + i] = s.d[i];
-    Eg[mat * n + i] = s.e[i];
-  }
-}
 
 // Register-resident tridiagonalization: one workgroup of NB * S threads per
 // matrix (n <= NB), each thread holding NB / S entries of one row.
@@ -87,26 +33,30 @@ __global__ void __launch_bounds__(NB * S) eigh_tridiag_reg_kernel(int n, const d
   GpuTeam tm;
   const size_t mat = blockIdx.x;
   const size_t nn = static_cast<size_t>(n) * n;
-  RegSmem<T> sm{xc, vp, wp, yp, red};
+  RegSmem<T> sm{xc, vp, wp, yp, red, nullptr};
   tridiag_reg<T, NB, S>(tm, n, H + mat * nn * 2, sm, Dg + mat * n, Eg + mat * n, Vg ? Vg + mat * nn : nullptr,
                         taug ? taug + mat * n : nullptr);
 }
 
 // Register-resident back-transformation Z = Q Y: one workgroup per matrix,
-// each thread holding NB / S entries of one column of Z.
+// each thread holding NB / S entries of one column of Z.  twist / e: the
+// trieig outputs ([nmat][n] each), Ts: its scratch ([nmat][3][n][n]).
 template <typename T, int NB, int S>
 __global__ void __launch_bounds__(NB * S) eigh_backtrans_reg_kernel(int n, const T* __restrict__ Y,
-                                                                    const T* __restrict__ ysc,
+                                                                    const T* __restrict__ twist,
+                                                                    const T* __restrict__ e,
+                                                                    const T* __restrict__ Ts,
                                                                     const cplx<T>* __restrict__ Vg,
                                                                     const cplx<T>* __restrict__ taug,
                                                                     double* __restrict__ Zout, int rows_out) {
   __shared__ cplx<T> vp[3 * NB], yp[2 * S * NB];
+  __shared__ T es[NB];
   GpuTeam tm;
   const size_t mat = blockIdx.x;
   const size_t nn = static_cast<size_t>(n) * n;
-  RegSmem<T> sm{nullptr, vp, nullptr, yp, nullptr};
-  backtrans_reg<T, NB, S>(tm, n, Y + mat * nn, n, ysc + mat * n, Vg + mat * nn, taug + mat * n, sm, Zout + mat * nn * 2,
-                          rows_out);
+  RegSmem<T> sm{nullptr, vp, nullptr, yp, nullptr, es};
+  backtrans_reg<T, NB, S>(tm, n, Y + mat * nn, n, twist + mat * n, e + mat * n, Ts + mat * nn * 3, Vg + mat * nn,
+                          taug + mat * n, sm, Zout + mat * nn * 2, rows_out);
 }
 
 // Slice counts (threads per row or column = S).  Tridiagonalisation at
@@ -143,34 +93,37 @@ void tridiag_reg_dispatch(hipStream_t st, int n, size_t nmat, const double* H, T
 }
 
 template <typename T, int NB>
-void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const T* ysc, const cplx<T>* Vg,
-                          const cplx<T>* taug, double* Z, int rows_out) {
+void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* Ts,
+                          const cplx<T>* Vg, const cplx<T>* taug, double* Z, int rows_out) {
   constexpr int S = RegShape<T, NB>::Sbt;
   hipLaunchKernelGGL((eigh_backtrans_reg_kernel<T, NB, S>), dim3(static_cast<unsigned>(nmat)), dim3(NB * S), 0, st,
-                     n, Y, ysc, Vg, taug, Z, rows_out);
+                     n, Y, DE + 2 * nmat * n, DE + nmat * n, Ts, Vg, taug, Z, rows_out);
 }
 
 template <typename T>
-void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const T* ysc, const cplx<T>* Vg,
-                            const cplx<T>* taug, double* Z, int rows_out) {
+void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* Ts,
+                            const cplx<T>* Vg, const cplx<T>* taug, double* Z, int rows_out) {
   if (n <= 32) {
-    launch_backtrans_reg<T, 32>(st, n, nmat, Y, ysc, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 32>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
   } else if (n <= 64) {
-    launch_backtrans_reg<T, 64>(st, n, nmat, Y, ysc, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 64>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
   } else {
-    launch_backtrans_reg<T, 128>(st, n, nmat, Y, ysc, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 128>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
   }
 }
 
 constexpr int kTeBlock = 256;
+// Relative eigenvalue gap below which eigenvectors are formed by inverse
+// iteration inside the cluster (dstein's threshold).
+constexpr double kClusterRel = 1e-3;
 
-// Tridiagonal eigenvalues (+ eigenvectors): sub-team of n threads per matrix.
-// With VEC, the per-eigenvector scale of Y (see tridiag_vectors) overwrites
-// the diagonal Dg once it has been read.
+// Tridiagonal eigenvalues (+ the twisted factorisations of the isolated ones
+// when VEC): sub-team of n threads per matrix, 256 / n matrices per
+// workgroup.  TW: [nmat][n] twist indices (-1: cluster member).
 template <typename T, bool VEC>
-__global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nmat, T* __restrict__ Dg,
+__global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nmat, const T* __restrict__ Dg,
                                                               const T* __restrict__ Eg, double* __restrict__ Eout,
-                                                              T* __restrict__ Y, T* __restrict__ S) {
+                                                              T* __restrict__ S, T* __restrict__ TW) {
   __shared__ T sh[4 * kTeBlock];
   const int per = kTeBlock / n;  // matrices per workgroup (n <= 128)
   const int sub = threadIdx.x / n;
@@ -192,147 +145,107 @@ __global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nma
   if (live) Eout[mat * n + tm.id] = static_cast<double>(E[tm.id]);
   if (VEC) {
     const size_t mm = live ? mat : 0;
-    T* Ym = Y + mm * static_cast<size_t>(n) * n;
-    T* Sm = S + mm * static_cast<size_t>(n) * n * 3;
-    tridiag_vectors<T>(tm, n, d, e, E, Ym, n, Sm, Dg + mm * n);
+    twisted_all<T>(tm, n, d, e, E, n, S + mm * static_cast<size_t>(n) * n * 3, TW + mm * n, T(kClusterRel));
   }
 }
 
-// Z[:, c0:c0+cb] = Q Y[:, c0:c0+cb]; output complex FP64.
-//   rows_out = 0: Z[mat][r][c] (eigenvectors as columns)
-//   rows_out = 1: Z[mat][c][r] (eigenvectors as rows, solve_all layout)
+// Eigenvectors of the cluster members (inverse iteration), same layout as
+// eigh_trieig_kernel; a workgroup whose matrices have no cluster returns at
+// once.  The eigenvalues are read back from Eval (exact: they were T).
 template <typename T>
-__global__ void __launch_bounds__(256) eigh_backtransform_kernel(int n, int cb, const T* __restrict__ Y,
-                                                                 const cplx<T>* __restrict__ Vg,
-                                                                 const cplx<T>* __restrict__ taug,
-                                                                 double* __restrict__ Zout, int rows_out) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  GpuTeam tm;
-  const size_t mat = blockIdx.x;
-  const int c0 = blockIdx.y * cb;
-  const int w = min(cb, n - c0);
-  cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
-  cplx<T>* vs = Z + static_cast<size_t>(n) * cb;
-  cplx<T>* part = vs + 2 * n;
-  cplx<T>* sv = part + tm.nt;
-  const T* Ym = Y + mat * static_cast<size_t>(n) * n;
-  for (int f = tm.id; f < n * cb; f += tm.nt) {
-    int r = f / cb, c = f % cb;
-    T y = c < w ? Ym[static_cast<size_t>(r) * n + c0 + c] : T(0);
-    Z[f] = {y, T(0)};
+__global__ void __launch_bounds__(kTeBlock) eigh_cluster_kernel(int n, size_t nmat, const T* __restrict__ Dg,
+                                                               const T* __restrict__ Eg, const double* __restrict__ Eval,
+                                                               const T* __restrict__ TW, T* __restrict__ Y,
+                                                               T* __restrict__ S) {
+  __shared__ T sh[3 * kTeBlock];
+  const int per = kTeBlock / n;
+  const int sub = threadIdx.x / n;
+  const size_t mat = static_cast<size_t>(blockIdx.x) * per + sub;
+  const bool live = sub < per && mat < nmat;
+  const int so = live ? sub : 0;
+  GpuSubTeam tm(live ? static_cast<int>(threadIdx.x % n) : n, n);
+  const bool member = live && TW[mat * n + tm.id] < T(0);
+  if (!__syncthreads_or(member)) return;
+  T* d = sh + so * 3 * n;
+  T* e = d + n;
+  T* E = e + n;
+  if (live) {
+    d[tm.id] = Dg[mat * n + tm.id];
+    e[tm.id] = Eg[mat * n + tm.id];
+    E[tm.id] = static_cast<T>(Eval[mat * n + tm.id]);
   }
-  tm.sync();
-  apply_q<T>(tm, n, cb, Z, Vg + mat * static_cast<size_t>(n) * n, taug + mat * static_cast<size_t>(n), vs, part,
-             sv);
-  double* Zm = Zout + mat * static_cast<size_t>(n) * n * 2;
-  for (int f = tm.id; f < n * w; f += tm.nt) {
-    int r, c;
-    if (rows_out) {  // coalesce along r
-      c = f / n;
-      r = f % n;
-    } else {
-      r = f / w;
-      c = f % w;
-    }
-    cplx<T> z = Z[r * cb + c];
-    size_t o = rows_out ? (static_cast<size_t>(c0 + c) * n + r) : (static_cast<size_t>(r) * n + c0 + c);
-    reinterpret_cast<double2*>(Zm)[o] = make_double2(static_cast<double>(z.re), static_cast<double>(z.im));
-  }
+  __syncthreads();
+  const size_t mm = live ? mat : 0;
+  cluster_vectors<T>(tm, n, d, e, E, Y + mm * static_cast<size_t>(n) * n, n, S + mm * static_cast<size_t>(n) * n * 3,
+                     T(kClusterRel));
 }
 
-size_t tri_smem(int n, size_t rs) {
-  return sizeof(double) * 0 + (static_cast<size_t>(n) * (n + 1) / 2 + 2 * n) * 2 * rs + 2 * n * rs +
-         (256 + 64) * rs;
-}
-
-int bt_cb(int n) { return n <= 64 ? n : 32; }
-size_t bt_smem(int n, int cb, size_t rs) { return 2 * rs * (static_cast<size_t>(n) * cb + 2 * n + 256 + cb); }
-
-}  // namespace
-
-int eigh_set_smem_limits() {
-  // up to 160 KB of dynamic LDS per workgroup on gfx950
-  const int lim = 160 * 1024;
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_tridiag_kernel<double>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_backtransform_kernel<double>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_tridiag_kernel<float>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&eigh_backtransform_kernel<float>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  return RTB_OK;
-}
-
-// FP32 route of the mixed form: same kernels instantiated for float.
-int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
-                     float* Vg, float* taug, float* Y, float* S, float* DE) {
-  if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
-  if (nmat == 0) return RTB_OK;
-  float* Dg = DE;
-  float* Eg = DE + nmat * n;
-  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIDIAG));
-  tridiag_reg_dispatch<float>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<float>*>(Vg),
-                              reinterpret_cast<cplx<float>*>(taug));
-  RTB_HIP(hipGetLastError());
-  if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
-  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIEIG));
+template <typename T>
+int launch_trieig(hipStream_t st, int n, size_t nmat, const T* Dg, const T* Eg, double* E, bool vectors, T* Y, T* S,
+                  T* TW) {
   const int per = kTeBlock / n;
   const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
   if (vectors) {
-    hipLaunchKernelGGL((eigh_trieig_kernel<float, true>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg, E,
-                       Y, S);
+    hipLaunchKernelGGL((eigh_trieig_kernel<T, true>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg, E, S,
+                       TW);
+    RTB_HIP(hipGetLastError());
+    hipLaunchKernelGGL((eigh_cluster_kernel<T>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg, E, TW, Y, S);
   } else {
-    hipLaunchKernelGGL((eigh_trieig_kernel<float, false>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg,
-                       E, Y, S);
+    hipLaunchKernelGGL((eigh_trieig_kernel<T, false>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg, E, S,
+                       TW);
   }
   RTB_HIP(hipGetLastError());
-  if (tmr) RTB_TRY(tmr->end(1));
   return RTB_OK;
 }
 
-int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* Vg,
-                            const float* taug, double* Z, bool rows_out) {
+template <typename T>
+int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors, T* Vg,
+                T* taug, T* Y, T* S, T* DE) {
+  if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
   if (nmat == 0) return RTB_OK;
-  backtrans_reg_dispatch<float>(st, n, nmat, Y, DE, reinterpret_cast<const cplx<float>*>(Vg),
-                                reinterpret_cast<const cplx<float>*>(taug), Z, rows_out ? 1 : 0);
+  T* Dg = DE;
+  T* Eg = DE + nmat * n;
+  T* TW = DE + 2 * nmat * n;
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIDIAG));
+  tridiag_reg_dispatch<T>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<T>*>(Vg), reinterpret_cast<cplx<T>*>(taug));
+  RTB_HIP(hipGetLastError());
+  if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
+  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIEIG));
+  RTB_TRY(launch_trieig<T>(st, n, nmat, Dg, Eg, E, vectors, Y, S, TW));
+  if (tmr) RTB_TRY(tmr->end(vectors ? 2 : 1));
+  return RTB_OK;
+}
+
+template <typename T>
+int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* S, const T* Vg,
+                       const T* taug, double* Z, bool rows_out) {
+  if (nmat == 0) return RTB_OK;
+  backtrans_reg_dispatch<T>(st, n, nmat, Y, DE, S, reinterpret_cast<const cplx<T>*>(Vg),
+                            reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
+}
+
+}  // namespace
+
+int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
+                     float* Vg, float* taug, float* Y, float* S, float* DE) {
+  return eigh_launch<float>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE);
+}
+
+int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
+                            const float* Vg, const float* taug, double* Z, bool rows_out) {
+  return eigh_backtransform<float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
 }
 
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
                      double* Vg, double* taug, double* Y, double* S, double* DE) {
-  if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
-  if (nmat == 0) return RTB_OK;
-  double* Dg = DE;
-  double* Eg = DE + nmat * n;
-  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIDIAG));
-  tridiag_reg_dispatch<double>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<double>*>(Vg),
-                               reinterpret_cast<cplx<double>*>(taug));
-  RTB_HIP(hipGetLastError());
-  if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
-  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIEIG));
-  const int per = kTeBlock / n;
-  const unsigned blocks = static_cast<unsigned>((nmat + per - 1) / per);
-  if (vectors) {
-    hipLaunchKernelGGL((eigh_trieig_kernel<double, true>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg, E,
-                       Y, S);
-  } else {
-    hipLaunchKernelGGL((eigh_trieig_kernel<double, false>), dim3(blocks), dim3(kTeBlock), 0, st, n, nmat, Dg, Eg,
-                       E, Y, S);
-  }
-  RTB_HIP(hipGetLastError());
-  if (tmr) RTB_TRY(tmr->end(1));
-  return RTB_OK;
+  return eigh_launch<double>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE);
 }
 
-int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE,
+int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
                             const double* Vg, const double* taug, double* Z, bool rows_out) {
-  if (nmat == 0) return RTB_OK;
-  backtrans_reg_dispatch<double>(st, n, nmat, Y, DE, reinterpret_cast<const cplx<double>*>(Vg),
-                                 reinterpret_cast<const cplx<double>*>(taug), Z, rows_out ? 1 : 0);
-  RTB_HIP(hipGetLastError());
-  return RTB_OK;
+  return eigh_backtransform<double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
 }
 
 }  // namespace rtb

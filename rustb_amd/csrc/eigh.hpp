@@ -6,18 +6,19 @@
 namespace rtb {
 
 // ---- eigensolver (eigh.hip) ----
-int eigh_set_smem_limits();
-// Householder + bisection (+ inverse iteration when `vectors`).
+// Householder + bisection (+ the tridiagonal eigenvector data when
+// `vectors`: twisted factorisations, inverse iteration inside clusters).
 //   H: [nmat][n][n] complex; E: [nmat][n]; Vg: [nmat][n][n] complex;
 //   taug: [nmat][n] complex; Y: [nmat][n][n]; S: [nmat][3][n][n];
-//   DE: [2][nmat][n] tridiagonal scratch
+//   DE: [3][nmat][n] the tridiagonal (diagonal, off-diagonal) and the twist
+//   indices; Vg, taug, Y, S, DE are the inputs of the back-transformation.
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
                      double* Vg, double* taug, double* Y, double* S, double* DE);
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
                      float* Vg, float* taug, float* Y, float* S, float* DE);
-int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* Vg,
-                            const float* taug, double* Z, bool rows_out);
+int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
+                            const float* Vg, const float* taug, double* Z, bool rows_out);
 // FP64 refinement of FP32 eigenpairs (refine.hip): one Ogita-Aishima step,
 // Rayleigh quotients, unit columns, Rayleigh-Ritz in clusters with gaps
 // below gap_rel * max(1, spectral radius).  Z: [nk][n][n] columns (in, then
@@ -29,20 +30,22 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, d
 // scale may be null)
 int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, double* out);
 // Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
-int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE,
+int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
                             const double* Vg, const double* taug, double* Z, bool rows_out);
 
 // ---- Fourier sum (fourier.hip) ----
 struct FourierArgs {
   int dim, n, nR, np;     // np: number of projected velocity directions (0..3)
   int gauge;              // RTB_GAUGE_*
+  int combine = 0;        // np == 2: write V0 + i V1 as one matrix per k
   const double* ham;      // [nR][n][n] complex
   const double* R;        // [nR][3]
   const double* wR;       // [np][nR]  w_p . R_cart
   const double* tau;      // [n][3]    fractional
   const double* wtau;     // [np][n]   w_p . tau_cart
 };
-// k: [nk][3] (zero padded); H: [nk][n][n]; V: [nk][np][n][n] (or null when np == 0)
+// k: [nk][3] (zero padded); H: [nk][n][n]; V: [nk][np][n][n] (or null when
+// np == 0; [nk][n][n] with a.combine)
 int fourier_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t nk, double* H,
                   double* V);
 // Axis-separable Bloch sum on a uniform mesh (fourier_mesh.hip).
@@ -58,7 +61,8 @@ int mesh_plan(const rtb_model* m, const size_t* kmesh, MeshPlan& p);
 int mesh_prepare(hipStream_t st, const rtb_model* m, const MeshPlan& p, int* d_boxidx, double* work, double* G,
                  bool want_v);
 // pass C for pairs [p0, p0 + np_pairs): H [np_pairs*N1][n][n], V [..][2][n][n]
-// in k1-innermost order: local k = (pair - p0) * N1 + i1
+// ([..][n][n] holding V0 + i V1 with a.combine) in k1-innermost order: local
+// k = (pair - p0) * N1 + i1
 int mesh_final(hipStream_t st, const rtb_model* m, const MeshPlan& p, const double* G, const FourierArgs& a,
                const double* dirs, size_t p0, size_t np_pairs, double* H, double* V);
 int mesh_set_smem_limits();
@@ -82,13 +86,13 @@ int zgemm_add_f32_batched(hipStream_t st, int n, size_t nb, const double* A, siz
                           double* C, size_t sC);
 
 // ---- Kubo band sum, reductions, DOS (kubo.hip) ----
-// J = (S v + v S) / 2 for the spin matrix S = sigma/2 (x) I (helpers.rs:11-39).
+// J = (S v + v S) / 2 for the spin matrix S = sigma/2 (x) I (helpers.rs:11-39);
+// with w (stride v_stride like v) the output is J + i w.
 int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride,
-                 double* J);
+                 const double* w, double* J);
 // Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2) for
-// Hermitian A, B (A_nm is read as conj(A_mn))
-int kubo_omega(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E,
-               double eta, double* omega);
+// Hermitian A, B, from C = A + i B: Im(A_nm B_mn) = (|C_nm|^2 - |C_mn|^2) / 4.
+int kubo_omega(hipStream_t st, int n, size_t nk, const double* C, const double* E, double eta, double* omega);
 // out[k] = sum_n omega[k][n] f(E[k][n], mu, T)
 int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const double* E, double mu,
                  double T, double* out);

@@ -27,6 +27,7 @@ struct RegSmem {
   cplx<T>* wp;  // [NB]      w, slice-major
   cplx<T>* yp;  // [2 * S * NB]  per-slice partial sums
   T* red;       // [2 * Team::kRedStride]
+  T* es;        // [NB]      sub-diagonal of T (back-transformation)
 };
 
 // a[q] for a team-uniform q: a select chain the compiler keeps in registers
@@ -225,35 +226,103 @@ RTB_HD void tridiag_reg(Team& tm, int n, const double* Hm, RegSmem<T> sm, T* Dg,
 }
 
 // --------------------------------------------------------------------------
-// Z = Q Y for the reflectors of tridiag_reg (zunmtr, SIDE = 'L', 'N'):
-// Y real [n][ldy] (eigenvector c in column c, scaled by ysc[c] on load),
+// Z = Q Y for the reflectors of tridiag_reg (zunmtr, SIDE = 'L', 'N'), Y the
+// eigenvectors of T (column c = eigenvector c):
+//   twist[c] >= 0: generated here from the twisted factorisation stored by
+//                  tridiag_vectors (Ts = [D+ | D-], [2][n][ldy]) and the
+//                  sub-diagonal e; every thread of column c walks all k
+//                  (in step across the wave: coalesced reads) and keeps its
+//                  rows;
+//   twist[c] < 0:  read from the unit column c of Y ([n][ldy]).
 // Z complex FP64 written as
 //   rows_out = 0: Zout[r][c] (eigenvectors as columns)
 //   rows_out = 1: Zout[c][r] (eigenvectors as rows)
 // Per reflector j = n-2 .. 0: s_c = tau_j v_j^H Z[:, c], Z[:, c] -= v_j s_c.
 // --------------------------------------------------------------------------
 template <typename T, int NB, int S, typename Team>
-RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* ysc, const cplx<T>* Vg,
-                          const cplx<T>* taug, RegSmem<T> sm, double* Zout, int rows_out) {
+RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* twist, const T* e, const T* Ts,
+                          const cplx<T>* Vg, const cplx<T>* taug, RegSmem<T> sm, double* Zout, int rows_out) {
   constexpr int C = NB / S;
   constexpr int BL = C < 8 ? C : 8;
   constexpr int NBL = C / BL;
   const int c = tm.id % NB, s = slice_of<NB>(tm.id);
+  // e staged in team-shared memory: read per k by every thread, it would
+  // otherwise occupy one scalar register per k
+  for (int i = tm.id; i < n; i += tm.nt) sm.es[i] = e[i];
+  tm.sync();
+  const T* es = sm.es;
   cplx<T> z[C];
   {
-    const T* yb = Y + static_cast<size_t>(s) * ldy + (c < n ? c : 0);  // Y[s + S q][c]
-    const size_t ystep = static_cast<size_t>(S) * ldy;
-    const T sc = c < n ? ysc[c] : T(0);
+    const int r = c < n ? static_cast<int>(twist[c]) : -1;
+#pragma unroll
+    for (int q = 0; q < C; ++q) z[q] = {T(0), T(0)};
+    {
+      // branch-free: every step computes the next chain value and keeps it
+      // only inside the chain's range (k < r going up, r < k < n going down);
+      // the scratch loads are scheduled kGenGroup register rows at a time.
+      // Columns without a twist (r < 0) are overwritten from Y below.
+      constexpr int kGenGroup = 4;
+      const int cc = c < n ? c : 0;
+      const T* Dp = Ts + cc;
+      const T* Dm = Ts + static_cast<size_t>(n) * ldy + cc;
+      T nrm2 = T(1), v = T(1);
+#pragma unroll
+      for (int q = C - 1; q >= 0; --q) {  // k < r, descending
+#pragma unroll
+        for (int t = S - 1; t >= 0; --t) {
+          const int k = t + S * q;
+          if (k + 1 < NB) {
+            const int kc = k < n ? k : n - 1;
+            const T nv = -es[kc] * recip(Dp[static_cast<size_t>(kc) * ldy]) * v;
+            const bool in = k < r;
+            v = in ? nv : v;
+            nrm2 += in ? nv * nv : T(0);
+            if (t == s) z[q].re = in ? nv : (k == r ? T(1) : z[q].re);
+          } else if (t == s) {
+            z[q].re = k == r ? T(1) : z[q].re;
+          }
+        }
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (q % kGenGroup == 0) __builtin_amdgcn_sched_barrier(0);  // bounded load look-ahead
+#endif
+      }
+      v = T(1);
+#pragma unroll
+      for (int q = 0; q < C; ++q) {  // k > r, ascending
+#pragma unroll
+        for (int t = 0; t < S; ++t) {
+          const int k = t + S * q;
+          if (k >= 1) {
+            const int kc = k < n ? k : n - 1;
+            const T nv = -es[kc - 1] * recip(Dm[static_cast<size_t>(kc) * ldy]) * v;
+            const bool in = r >= 0 && k > r && k < n;
+            v = in ? nv : v;
+            nrm2 += in ? nv * nv : T(0);
+            if (t == s && in) z[q].re = nv;
+          }
+        }
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (q % kGenGroup == kGenGroup - 1) __builtin_amdgcn_sched_barrier(0);
+#endif
+      }
+      const T inv = T(1) / sqrt(nrm2);
+#pragma unroll
+      for (int q = 0; q < C; ++q) z[q].re *= inv;
+    }
+    if (r < 0 && c < n) {  // a cluster member: the unit column of Y
+      const T* yb = Y + static_cast<size_t>(s) * ldy + c;  // Y[s + S q][c]
+      const size_t ystep = static_cast<size_t>(S) * ldy;
+#pragma unroll
+      for (int q = 0; q < C; ++q) z[q].re = s + S * q < n ? yb[q * ystep] : T(0);
+    }
+#if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      const int r = s + S * q;
-      z[q] = {(r < n && c < n) ? yb[q * ystep] * sc : T(0), T(0)};
-#if defined(__HIP_DEVICE_COMPILE__)
       // keep the zero imaginary parts opaque: otherwise the no-reflector path
       // is specialised with 2 * C constant doubles live at the stores
       asm volatile("" : "+v"(z[q].im));
-#endif
     }
+#endif
   }
   // reflector buffers rotate over three slots: a slot is rewritten two
   // barriers after its last read; the partial sums alternate over two.

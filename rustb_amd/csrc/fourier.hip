@@ -111,6 +111,7 @@ __global__ void __launch_bounds__(kBlock) fourier_dense_kernel(FourierArgs a, co
     const size_t ho = (static_cast<size_t>(kg) * n2 + e) * 2;
     H[ho] = hr[kk] * gr - hi[kk] * gi;
     H[ho + 1] = hr[kk] * gi + hi[kk] * gr;
+    double cr = 0.0, ci = 0.0;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       // i * acc  (+ i (w.tau_c - w.tau_m) H_lat in the Atom gauge)
@@ -123,9 +124,19 @@ __global__ void __launch_bounds__(kBlock) fourier_dense_kernel(FourierArgs a, co
         xr = yr;
         xi = yi;
       }
-      const size_t vo = ((static_cast<size_t>(kg) * NP + p) * n2 + e) * 2;
-      V[vo] = xr;
-      V[vo + 1] = xi;
+      if (NP == 2 && a.combine) {  // V0 + i V1
+        cr += p == 0 ? xr : -xi;
+        ci += p == 0 ? xi : xr;
+      } else {
+        const size_t vo = ((static_cast<size_t>(kg) * NP + p) * n2 + e) * 2;
+        V[vo] = xr;
+        V[vo + 1] = xi;
+      }
+    }
+    if (NP == 2 && a.combine) {
+      const size_t vo = (static_cast<size_t>(kg) * n2 + e) * 2;
+      V[vo] = cr;
+      V[vo + 1] = ci;
     }
   }
 }

@@ -109,6 +109,7 @@ struct FinalArgs {
   const double* wtau;     // [2][n]
   int np;                 // 0 or 2
   int gauge;
+  int combine;            // V0 + i V1 into one matrix per k
 };
 
 // Pass C for pairs [p0, p0 + gridDim.y): outputs in k1-innermost order.
@@ -189,6 +190,7 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
     }
     const size_t kl = pl * a.N1 + i1;
     H[kl * n2 + e] = make_double2(hr * gr - hi * gi, hr * gi + hi * gr);
+    double cr = 0.0, ci = 0.0;
     for (int p = 0; p < a.np; ++p) {
       const double sr = a.alpha[p][0] * d1r + a.alpha[p][1] * g1r + a.alpha[p][2] * g2r;
       const double si = a.alpha[p][0] * d1i + a.alpha[p][1] * g1i + a.alpha[p][2] * g2i;
@@ -201,8 +203,14 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
         xr = yr;
         xi = yi;
       }
-      V[(kl * 2 + p) * n2 + e] = make_double2(xr, xi);
+      if (a.combine) {
+        cr += p == 0 ? xr : -xi;
+        ci += p == 0 ? xi : xr;
+      } else {
+        V[(kl * 2 + p) * n2 + e] = make_double2(xr, xi);
+      }
     }
+    if (a.combine && a.np == 2) V[kl * n2 + e] = make_double2(cr, ci);
   }
 }
 
@@ -326,6 +334,7 @@ int mesh_final(hipStream_t st, const rtb_model* m, const MeshPlan& p, const doub
   f.G1 = m->dim >= 2 ? GB + gsz : nullptr;
   f.G2 = m->dim >= 3 ? GB + 2 * gsz : nullptr;
   f.np = a.np;
+  f.combine = a.combine && a.np == 2;
   for (int q = 0; q < 2; ++q)
     for (int d = 0; d < 3; ++d) {
       double s = 0.0;

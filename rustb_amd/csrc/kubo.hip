@@ -7,9 +7,10 @@ namespace rtb {
 
 namespace {
 
-// helpers.rs:11-39 spin matrix S = sigma/2 (x) I_norb; J = (S v + v S) / 2
+// helpers.rs:11-39 spin matrix S = sigma/2 (x) I_norb; J = (S v + v S) / 2,
+// plus i w when w is given (the combined operator of kubo_omega)
 __global__ void spin_current_kernel(int norb, size_t nb, int dir, const double2* __restrict__ v,
-                                    size_t v_stride, double2* __restrict__ J) {
+                                    size_t v_stride, const double2* __restrict__ w, double2* __restrict__ J) {
   const int n = 2 * norb;
   const size_t total = nb * n * n;
   const size_t g = blockIdx.x * 256ull + threadIdx.x;
@@ -39,28 +40,36 @@ __global__ void spin_current_kernel(int norb, size_t nb, int dir, const double2*
     double h = j < norb ? 0.5 : -0.5;  // (v S)_ij = v_{i jp} S_{jp j}; S_{j+norb, j} = +i/2
     vs = make_double2(-h * y.y, h * y.x);
   }
-  J[g] = make_double2(0.5 * (sv.x + vs.x), 0.5 * (sv.y + vs.y));
+  double2 out = make_double2(0.5 * (sv.x + vs.x), 0.5 * (sv.y + vs.y));
+  if (w) {
+    const double2 x = w[b * v_stride + e];
+    out.x -= x.y;
+    out.y += x.x;
+  }
+  J[g] = out;
 }
 
-__global__ void kubo_omega_kernel(int n, size_t nk, const double2* __restrict__ A, const double2* __restrict__ B,
-                                  const double* __restrict__ E, double eta2, double* __restrict__ omega) {
+// Omega_b = sum_{m != b} (|C_mb|^2 - |C_bm|^2) / 2 / ((E_b - E_m)^2 + eta^2) for
+// C = A + i B (A, B Hermitian): -2 Im(A_bm B_mb) = (|C_mb|^2 - |C_bm|^2) / 2.
+// Column b is read coalesced across threads; each thread streams its row b.
+__global__ void kubo_omega_kernel(int n, size_t nk, const double2* __restrict__ C, const double* __restrict__ E,
+                                  double eta2, double* __restrict__ omega) {
   const size_t g = blockIdx.x * 256ull + threadIdx.x;
   if (g >= nk * n) return;
   const size_t k = g / n;
   const int b = static_cast<int>(g % n);
-  const double2* Ak = A + k * n * n;
-  const double2* Bk = B + k * n * n;
+  const double2* Ck = C + k * n * n;
+  const double2* row = Ck + static_cast<size_t>(b) * n;
   const double* Ek = E + k * n;
+  const double eb = Ek[b];
   double acc = 0.0;
-  // A is Hermitian: A_bm = conj(A_mb), so both operands are read down
-  // column b (consecutive threads, consecutive addresses)
   for (int m = 0; m < n; ++m) {
     if (m == b) continue;
-    const double2 a = Ak[static_cast<size_t>(m) * n + b];
-    const double2 c = Bk[static_cast<size_t>(m) * n + b];
-    const double im = a.x * c.y - a.y * c.x;  // Im(A_bm B_mb) = Im(conj(A_mb) B_mb)
-    const double dE = Ek[b] - Ek[m];
-    acc += -2.0 * im / (dE * dE + eta2);
+    const double2 c = Ck[static_cast<size_t>(m) * n + b];
+    const double2 r = row[m];
+    const double num = 0.5 * ((c.x * c.x + c.y * c.y) - (r.x * r.x + r.y * r.y));
+    const double dE = eb - Ek[m];
+    acc += num / (dE * dE + eta2);
   }
   omega[g] = acc;
 }
@@ -144,24 +153,23 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
 
 }  // namespace
 
-int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride, double* J) {
+int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride,
+                 const double* w, double* J) {
   const size_t n = 2 * static_cast<size_t>(norb);
   const size_t total = nb * n * n;
   if (total == 0) return RTB_OK;
   hipLaunchKernelGGL(spin_current_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st,
                      norb, nb, spin_dir, reinterpret_cast<const double2*>(v), v_stride / 2,
-                     reinterpret_cast<double2*>(J));
+                     reinterpret_cast<const double2*>(w), reinterpret_cast<double2*>(J));
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
 
-int kubo_omega(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E, double eta,
-               double* omega) {
+int kubo_omega(hipStream_t st, int n, size_t nk, const double* C, const double* E, double eta, double* omega) {
   const size_t total = nk * n;
   if (total == 0) return RTB_OK;
   hipLaunchKernelGGL(kubo_omega_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, n,
-                     nk, reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B), E, eta * eta,
-                     omega);
+                     nk, reinterpret_cast<const double2*>(C), E, eta * eta, omega);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -1,9 +1,12 @@
 """Host-side check of the eigensolver's team-generic device source.
 
-tests/native/team_emul.cpp runs rustb_amd/csrc/eigh_core.hpp (the exact code
-the GPU workgroups execute) with `nt` host threads and a barrier; this checks
-it against LAPACK zheev for several sizes and team sizes, including
-degenerate and block-diagonal spectra (CPU only)."""
+tests/native/team_emul.cpp runs rustb_amd/csrc/eigh_core.hpp and
+eigh_reg.hpp (the exact code the GPU workgroups execute: register-resident
+Householder reduction, bisection, twisted factorisations / inverse
+iteration, back-transformation) with host threads and a barrier, in the
+slice shapes of the GPU dispatch, and checks it against LAPACK zheev for
+several sizes, padding (n < NB), both output layouts, and degenerate,
+block-diagonal and clustered spectra (CPU only)."""
 import ctypes
 import os
 import subprocess
@@ -22,16 +25,7 @@ def emul(tmp_path_factory):
                     "-o", out], check=True)
     lib = ctypes.CDLL(out)
 
-    def run(H, nt, fp32=0, vectors=1):
-        n = H.shape[0]
-        Hc = np.ascontiguousarray(H, np.complex128)
-        E = np.zeros(n)
-        Z = np.zeros((n, n), np.complex128)
-        lib.emul_eigh(n, nt, fp32, Hc.ctypes.data_as(ctypes.c_void_p), E.ctypes.data_as(ctypes.c_void_p),
-                      Z.ctypes.data_as(ctypes.c_void_p), vectors)
-        return E, Z
-
-    def run_reg(H, fp32=0, rows_out=0):
+    def run(H, fp32=0, rows_out=0):
         n = H.shape[0]
         Hc = np.ascontiguousarray(H, np.complex128)
         E = np.zeros(n)
@@ -41,60 +35,12 @@ def emul(tmp_path_factory):
         assert rc == 0
         return E, (Z.T if rows_out else Z)
 
-    run.reg = run_reg
     return run
 
 
 def rand_herm(rng, n):
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     return A + A.conj().T
-
-
-@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 33, 64])
-@pytest.mark.parametrize("nt", [1, 4, 13, 64])
-def test_fp64_matches_zheev(emul, n, nt):
-    rng = np.random.default_rng(n * 100 + nt)
-    H = rand_herm(rng, n)
-    w, _, _ = L.zheev(H)
-    E, Z = emul(H, nt)
-    scale = max(1.0, np.abs(w).max())
-    assert np.abs(E - w).max() <= 4e-15 * scale * n
-    assert np.abs(H @ Z - Z * E).max() <= 4e-15 * scale * n
-    assert np.abs(Z.conj().T @ Z - np.eye(n)).max() <= 1e-14 * n
-
-
-def test_exact_two_by_two(emul):
-    E, _ = emul(np.array([[0, 3], [3, 0]], complex), 4)
-    assert np.array_equal(E, [-3.0, 3.0])
-
-
-@pytest.mark.parametrize("case", ["degenerate", "block", "zero", "near_degenerate"])
-def test_degenerate_spectra(emul, case):
-    rng = np.random.default_rng(5)
-    if case == "degenerate":
-        H = np.diag([1.0] * 4 + [2.0] * 4 + [3.0] * 4).astype(complex)
-    elif case == "block":
-        H = np.kron(np.eye(3), rand_herm(rng, 4))
-    elif case == "zero":
-        H = np.zeros((9, 9), complex)
-    else:
-        H = np.kron(rand_herm(rng, 3), np.eye(4)) + 1e-9 * np.diag(rng.standard_normal(12))
-    w, _, _ = L.zheev(H)
-    E, Z = emul(H, 5)
-    n = H.shape[0]
-    assert np.abs(E - w).max() < 1e-13
-    assert np.abs(H @ Z - Z * E).max() < 1e-13
-    assert np.abs(Z.conj().T @ Z - np.eye(n)).max() < 1e-13
-
-
-def test_fp32_variant(emul):
-    rng = np.random.default_rng(11)
-    H = rand_herm(rng, 32)
-    w, _, _ = L.zheev(H)
-    E, Z = emul(H, 8, fp32=1)
-    assert np.abs(E - w).max() < 5e-5 * np.abs(w).max()
-    assert np.abs(Z.conj().T @ Z - np.eye(32)).max() < 1e-4
-
 
 # Register-resident route (tridiag_reg / backtrans_reg in eigh_reg.hpp): the
 # slice shapes used on the GPU (32x2, 64x2, 128x2 FP32; 128x4 FP64) with
@@ -104,7 +50,7 @@ def test_reg_fp64_matches_zheev(emul, n):
     rng = np.random.default_rng(n + 7)
     H = rand_herm(rng, n)
     w, _, _ = L.zheev(H)
-    E, Z = emul.reg(H, rows_out=n % 2)
+    E, Z = emul(H, rows_out=n % 2)
     scale = max(1.0, np.abs(w).max())
     assert np.abs(E - w).max() <= 4e-15 * scale * n
     assert np.abs(H @ Z - Z * E).max() <= 4e-15 * scale * n
@@ -116,7 +62,7 @@ def test_reg_fp32_matches_zheev(emul, n):
     rng = np.random.default_rng(n + 70)
     H = rand_herm(rng, n)
     w, _, _ = L.zheev(H)
-    E, Z = emul.reg(H, fp32=1)
+    E, Z = emul(H, fp32=1)
     scale = np.abs(w).max()
     assert np.abs(E - w).max() < 5e-6 * scale
     assert np.abs(H @ Z - Z * E).max() < 5e-6 * scale
@@ -135,8 +81,32 @@ def test_reg_special_spectra(emul, case):
     else:
         H = np.diag(rng.standard_normal(66)).astype(complex)
     w, _, _ = L.zheev(H)
-    E, Z = emul.reg(H)
+    E, Z = emul(H)
     n = H.shape[0]
+    assert np.abs(E - w).max() < 1e-13
+    assert np.abs(H @ Z - Z * E).max() < 1e-13
+    assert np.abs(Z.conj().T @ Z - np.eye(n)).max() < 1e-13
+
+
+def test_reg_exact_two_by_two(emul):
+    E, _ = emul(np.array([[0, 3], [3, 0]], complex))
+    assert np.array_equal(E, [-3.0, 3.0])
+
+
+@pytest.mark.parametrize("case", ["near_degenerate", "clusters_and_isolated"])
+def test_reg_clustered_spectra(emul, case):
+    """Clusters (inverse iteration + MGS) next to isolated eigenvalues (twisted)."""
+    rng = np.random.default_rng(15)
+    if case == "near_degenerate":
+        H = np.kron(rand_herm(rng, 3), np.eye(4)) + 1e-9 * np.diag(rng.standard_normal(12))
+    else:
+        w = np.concatenate([np.full(5, 1.0), np.linspace(-3, 3, 40), np.full(3, -2.5), 2.0 + 1e-7 * np.arange(4)])
+        Q, _ = np.linalg.qr(rng.standard_normal((w.size, w.size)) + 1j * rng.standard_normal((w.size, w.size)))
+        H = (Q * w) @ Q.conj().T
+        H = 0.5 * (H + H.conj().T)
+    n = H.shape[0]
+    w, _, _ = L.zheev(H)
+    E, Z = emul(H)
     assert np.abs(E - w).max() < 1e-13
     assert np.abs(H @ Z - Z * E).max() < 1e-13
     assert np.abs(Z.conj().T @ Z - np.eye(n)).max() < 1e-13
