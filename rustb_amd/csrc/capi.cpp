@@ -42,10 +42,34 @@ class Work {
   std::vector<void*> ptrs_;
 };
 
+// The workspace of a call (tens of GB for a large mesh) comes from the
+// device's default stream-ordered pool.  By default the pool returns freed
+// memory to the driver at every synchronisation, and mapping it again costs
+// more than a call's computation; the first call on a device therefore lets
+// the pool keep what it has (the process holds its peak workspace until it
+// exits).
+void retain_pool_memory() {
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done[dev]) return;
+  hipMemPool_t pool;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
+  done[dev] = true;
+}
+
 // A non-blocking stream owned by one call (default priority).
 class Stream {
  public:
-  Stream() { ok_ = hipStreamCreateWithFlags(&s_, hipStreamNonBlocking) == hipSuccess; }
+  Stream() {
+    retain_pool_memory();
+    ok_ = hipStreamCreateWithFlags(&s_, hipStreamNonBlocking) == hipSuccess;
+  }
   ~Stream() {
     if (ok_) {
       (void)hipStreamSynchronize(s_);

@@ -113,6 +113,13 @@ struct FinalArgs {
 };
 
 // Pass C for pairs [p0, p0 + gridDim.y): outputs in k1-innermost order.
+// A workgroup owns kTW matrix elements of one pair for all N1 values of k1:
+// the kTW columns of G0..G2 are staged in LDS once, and each thread forms
+// kFB values of k1 per pass, so one LDS read of G serves kFB outputs.  The
+// Atom-gauge factor conj(phi_m) phi_c = exp(i 2 pi (tau_c - tau_m).k) is one
+// sincospi per element and k.
+constexpr int kFB = 2;
+
 __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, double2* __restrict__ H,
                                                         double2* __restrict__ V) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -148,69 +155,78 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
   const double k3 = a.dim >= 3 ? static_cast<double>(i3) / static_cast<double>(a.N3) : 0.0;
   const double* tm = a.tau + static_cast<size_t>(mrow) * 3;
   const double* tc = a.tau + static_cast<size_t>(mcol) * 3;
-  for (int i1 = threadIdx.x / kTW; i1 < a.N1; i1 += 256 / kTW) {
-    double hr = 0.0, hi = 0.0, d1r = 0.0, d1i = 0.0, g1r = 0.0, g1i = 0.0, g2r = 0.0, g2i = 0.0;
-    int m = dmod(i1 * a.lo1, a.N1);
-    const int step = i1 % a.N1;
+  const double dt1 = tc[0] - tm[0], dt23 = (tc[1] - tm[1]) * k2 + (tc[2] - tm[2]) * k3;
+  double wd[2] = {0.0, 0.0};  // w_p . (tau_c - tau_m)
+  for (int p = 0; p < a.np; ++p)
+    wd[p] = a.wtau[static_cast<size_t>(p) * a.n + mcol] - a.wtau[static_cast<size_t>(p) * a.n + mrow];
+  constexpr int kGroups = 256 / kTW;
+  for (int ib = (threadIdx.x / kTW) * kFB; ib < a.N1; ib += kGroups * kFB) {
+    double hr[kFB], hi[kFB], d1r[kFB], d1i[kFB], g1r[kFB], g1i[kFB], g2r[kFB], g2i[kFB];
+    int m[kFB], step[kFB];
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) {
+      const int i1 = min(ib + u, a.N1 - 1);  // lanes past N1 repeat the last k1, unstored
+      m[u] = dmod(i1 * a.lo1, a.N1);
+      step[u] = i1;
+      hr[u] = hi[u] = d1r[u] = d1i[u] = g1r[u] = g1i[u] = g2r[u] = g2i[u] = 0.0;
+    }
     for (int l = 0; l < a.L1; ++l) {
-      const double2 w = W[m];
       const double2 x = X0[l * kTW + t];
-      const double tr = w.x * x.x - w.y * x.y, ti = w.x * x.y + w.y * x.x;
-      hr += tr;
-      hi += ti;
-      if (a.np) {
-        const double r1 = static_cast<double>(a.lo1 + l);
-        d1r += r1 * tr;
-        d1i += r1 * ti;
-        if (ng > 1) {
-          const double2 y = X1[l * kTW + t];
-          g1r += w.x * y.x - w.y * y.y;
-          g1i += w.x * y.y + w.y * y.x;
+      const double2 y = ng > 1 ? X1[l * kTW + t] : make_double2(0.0, 0.0);
+      const double2 z = ng > 2 ? X2[l * kTW + t] : make_double2(0.0, 0.0);
+      const double r1 = static_cast<double>(a.lo1 + l);
+#pragma unroll
+      for (int u = 0; u < kFB; ++u) {
+        const double2 w = W[m[u]];
+        const double tr = w.x * x.x - w.y * x.y, ti = w.x * x.y + w.y * x.x;
+        hr[u] += tr;
+        hi[u] += ti;
+        if (a.np) {
+          d1r[u] += r1 * tr;
+          d1i[u] += r1 * ti;
+          g1r[u] += w.x * y.x - w.y * y.y;
+          g1i[u] += w.x * y.y + w.y * y.x;
+          g2r[u] += w.x * z.x - w.y * z.y;
+          g2i[u] += w.x * z.y + w.y * z.x;
         }
-        if (ng > 2) {
-          const double2 z = X2[l * kTW + t];
-          g2r += w.x * z.x - w.y * z.y;
-          g2i += w.x * z.y + w.y * z.x;
-        }
+        m[u] += step[u];
+        if (m[u] >= a.N1) m[u] -= a.N1;
       }
-      m += step;
-      if (m >= a.N1) m -= a.N1;
     }
-    const double k1 = static_cast<double>(i1) / static_cast<double>(a.N1);
-    double gr = 1.0, gi = 0.0;
-    if (a.gauge == RTB_GAUGE_ATOM) {
-      double cm, sm, cc, sc;
-      double am = tm[0] * k1 + tm[1] * k2 + tm[2] * k3, ac = tc[0] * k1 + tc[1] * k2 + tc[2] * k3;
-      am -= rint(am);
-      ac -= rint(ac);
-      sincospi(2.0 * am, &sm, &cm);
-      sincospi(2.0 * ac, &sc, &cc);
-      gr = cm * cc + sm * sc;
-      gi = cm * sc - sm * cc;
-    }
-    const size_t kl = pl * a.N1 + i1;
-    H[kl * n2 + e] = make_double2(hr * gr - hi * gi, hr * gi + hi * gr);
-    double cr = 0.0, ci = 0.0;
-    for (int p = 0; p < a.np; ++p) {
-      const double sr = a.alpha[p][0] * d1r + a.alpha[p][1] * g1r + a.alpha[p][2] * g2r;
-      const double si = a.alpha[p][0] * d1i + a.alpha[p][1] * g1i + a.alpha[p][2] * g2i;
-      double xr = -si, xi = sr;  // i * sum
+#pragma unroll
+    for (int u = 0; u < kFB; ++u) {
+      const int i1 = ib + u;
+      if (i1 >= a.N1) break;
+      const double k1 = static_cast<double>(i1) / static_cast<double>(a.N1);
+      double gr = 1.0, gi = 0.0;
       if (a.gauge == RTB_GAUGE_ATOM) {
-        const double diff = a.wtau[static_cast<size_t>(p) * a.n + mcol] - a.wtau[static_cast<size_t>(p) * a.n + mrow];
-        xr += -diff * hi;
-        xi += diff * hr;
-        const double yr = xr * gr - xi * gi, yi = xr * gi + xi * gr;
-        xr = yr;
-        xi = yi;
+        double arg = dt1 * k1 + dt23;
+        arg -= rint(arg);
+        sincospi(2.0 * arg, &gi, &gr);
       }
-      if (a.combine) {
-        cr += p == 0 ? xr : -xi;
-        ci += p == 0 ? xi : xr;
-      } else {
-        V[(kl * 2 + p) * n2 + e] = make_double2(xr, xi);
+      const size_t kl = pl * a.N1 + i1;
+      H[kl * n2 + e] = make_double2(hr[u] * gr - hi[u] * gi, hr[u] * gi + hi[u] * gr);
+      double cr = 0.0, ci = 0.0;
+      for (int p = 0; p < a.np; ++p) {
+        const double sr = a.alpha[p][0] * d1r[u] + a.alpha[p][1] * g1r[u] + a.alpha[p][2] * g2r[u];
+        const double si = a.alpha[p][0] * d1i[u] + a.alpha[p][1] * g1i[u] + a.alpha[p][2] * g2i[u];
+        double xr = -si, xi = sr;  // i * sum
+        if (a.gauge == RTB_GAUGE_ATOM) {
+          xr += -wd[p] * hi[u];
+          xi += wd[p] * hr[u];
+          const double yr = xr * gr - xi * gi, yi = xr * gi + xi * gr;
+          xr = yr;
+          xi = yi;
+        }
+        if (a.combine) {
+          cr += p == 0 ? xr : -xi;
+          ci += p == 0 ? xi : xr;
+        } else {
+          V[(kl * 2 + p) * n2 + e] = make_double2(xr, xi);
+        }
       }
+      if (a.combine && a.np == 2) V[kl * n2 + e] = make_double2(cr, ci);
     }
-    if (a.combine && a.np == 2) V[kl * n2 + e] = make_double2(cr, ci);
   }
 }
 
