@@ -41,14 +41,14 @@ __global__ void __launch_bounds__(NB * S) eigh_tridiag_reg_kernel(int n, const d
 // Register-resident back-transformation Z = Q Y: one workgroup per matrix,
 // each thread holding NB / S entries of one column of Z.  twist / e: the
 // trieig outputs ([nmat][n] each), Ts: its scratch ([nmat][3][n][n]).
-template <typename T, int NB, int S>
+template <typename T, int NB, int S, typename TO>
 __global__ void __launch_bounds__(NB * S) eigh_backtrans_reg_kernel(int n, const T* __restrict__ Y,
                                                                     const T* __restrict__ twist,
                                                                     const T* __restrict__ e,
                                                                     const T* __restrict__ Ts,
                                                                     const cplx<T>* __restrict__ Vg,
                                                                     const cplx<T>* __restrict__ taug,
-                                                                    double* __restrict__ Zout, int rows_out) {
+                                                                    TO* __restrict__ Zout, int rows_out) {
   __shared__ cplx<T> vp[3 * NB], yp[2 * S * NB];
   __shared__ T es[NB];
   GpuTeam tm;
@@ -92,23 +92,23 @@ void tridiag_reg_dispatch(hipStream_t st, int n, size_t nmat, const double* H, T
   }
 }
 
-template <typename T, int NB>
+template <typename T, int NB, typename TO>
 void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* Ts,
-                          const cplx<T>* Vg, const cplx<T>* taug, double* Z, int rows_out) {
+                          const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out) {
   constexpr int S = RegShape<T, NB>::Sbt;
-  hipLaunchKernelGGL((eigh_backtrans_reg_kernel<T, NB, S>), dim3(static_cast<unsigned>(nmat)), dim3(NB * S), 0, st,
+  hipLaunchKernelGGL((eigh_backtrans_reg_kernel<T, NB, S, TO>), dim3(static_cast<unsigned>(nmat)), dim3(NB * S), 0, st,
                      n, Y, DE + 2 * nmat * n, DE + nmat * n, Ts, Vg, taug, Z, rows_out);
 }
 
-template <typename T>
+template <typename T, typename TO>
 void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* Ts,
-                            const cplx<T>* Vg, const cplx<T>* taug, double* Z, int rows_out) {
+                            const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out) {
   if (n <= 32) {
-    launch_backtrans_reg<T, 32>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 32, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
   } else if (n <= 64) {
-    launch_backtrans_reg<T, 64>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 64, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
   } else {
-    launch_backtrans_reg<T, 128>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 128, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
   }
 }
 
@@ -216,11 +216,11 @@ int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H,
   return RTB_OK;
 }
 
-template <typename T>
+template <typename T, typename TO>
 int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* S, const T* Vg,
-                       const T* taug, double* Z, bool rows_out) {
+                       const T* taug, TO* Z, bool rows_out) {
   if (nmat == 0) return RTB_OK;
-  backtrans_reg_dispatch<T>(st, n, nmat, Y, DE, S, reinterpret_cast<const cplx<T>*>(Vg),
+  backtrans_reg_dispatch<T, TO>(st, n, nmat, Y, DE, S, reinterpret_cast<const cplx<T>*>(Vg),
                             reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -234,8 +234,8 @@ int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
 }
 
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
-                            const float* Vg, const float* taug, double* Z, bool rows_out) {
-  return eigh_backtransform<float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
+                            const float* Vg, const float* taug, float* Z, bool rows_out) {
+  return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
 }
 
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
@@ -245,7 +245,7 @@ int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
 
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
                             const double* Vg, const double* taug, double* Z, bool rows_out) {
-  return eigh_backtransform<double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
+  return eigh_backtransform<double, double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
 }
 
 }  // namespace rtb

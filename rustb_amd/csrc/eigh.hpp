@@ -17,15 +17,16 @@ int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
                      float* Vg, float* taug, float* Y, float* S, float* DE);
+// Z: [nmat][n][n] complex FP32 (the values of the FP32 route, exact)
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
-                            const float* Vg, const float* taug, double* Z, bool rows_out);
+                            const float* Vg, const float* taug, float* Z, bool rows_out);
 // FP64 refinement of FP32 eigenpairs (refine.hip): one Ogita-Aishima step,
 // Rayleigh quotients, unit columns, Rayleigh-Ritz in clusters with gaps
-// below gap_rel * max(1, spectral radius).  Z: [nk][n][n] columns (in, then
-// scratch); H is overwritten; the refined columns are written to W2, unit
+// below gap_rel * max(1, spectral radius).  Z: [nk][n][n] FP32 columns (in);
+// H is overwritten; the refined columns are written to W2 (FP64), unit
 // after multiplication by scale[k][j] ([nk][n], written); W1, W3 scratch.
-int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, double* Z, double* W1, double* W2,
-                      double* W3, double* E, double* scale, double gap_rel);
+int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, const float* Z, double* W1,
+                      double* W2, double* W3, double* E, double* scale, double gap_rel);
 // out[k][r][c] = Z[k][c][r] * scale[k][r] (eigenvector columns to rows;
 // scale may be null)
 int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, double* out);
@@ -70,6 +71,21 @@ int mesh_set_smem_limits();
 int mesh_points(hipStream_t st, int dim, const size_t* kmesh, size_t k0, size_t nk, double* k);
 
 // ---- batched complex products (zgemm.hip) ----
+// A batched complex operand, (re, im) pairs in FP64 or in FP32.
+struct ZOperand {
+  const void* p = nullptr;
+  size_t stride = 0;  // complex elements between consecutive matrices
+  bool f32 = false;
+};
+inline ZOperand zop(const double* p, size_t stride) { return {p, stride, false}; }
+inline ZOperand zop(const float* p, size_t stride) { return {p, stride, true}; }
+// C = op(A) B (FP64 arithmetic), diag(s) op(A) B diag(s) when s ([nb][n]) is
+// given; herm: the product is Hermitian (lower tiles computed, mirrored).
+int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B, double* C, size_t sC, bool herm,
+          const double* s);
+// C = A + A B with the product on the FP32 matrix cores, for a correction B
+// stored in FP32 (A in either storage; FP64 sums).  C must not alias A or B.
+int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC);
 // C[b] = op(A[b]) * B[b], n x n complex; conj_a: op = conjugate transpose.
 // sA, sB, sC: batch strides in complex elements.  herm: the product is known
 // to be Hermitian; the upper triangle is filled from the lower one.
@@ -79,11 +95,6 @@ int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, 
 // stride n): diag(s) op(A) B diag(s), e.g. U^H J U for unit columns U diag(s).
 int zgemm_batched_scaled(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
                          size_t sB, double* C, size_t sC, bool herm, const double* s);
-// C[b] = A[b] + A[b] * B[b] with the product on the FP32 matrix cores (FP64
-// storage): for small corrections B, whose product needs ~1e-7 relative
-// accuracy only.  C must not alias A or B.
-int zgemm_add_f32_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, const double* B, size_t sB,
-                          double* C, size_t sC);
 
 // ---- Kubo band sum, reductions, DOS (kubo.hip) ----
 // J = (S v + v S) / 2 for the spin matrix S = sigma/2 (x) I (helpers.rs:11-39);

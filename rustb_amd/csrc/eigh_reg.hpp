@@ -234,14 +234,14 @@ RTB_HD void tridiag_reg(Team& tm, int n, const double* Hm, RegSmem<T> sm, T* Dg,
 //                  (in step across the wave: coalesced reads) and keeps its
 //                  rows;
 //   twist[c] < 0:  read from the unit column c of Y ([n][ldy]).
-// Z complex FP64 written as
+// Z complex, written in type TO (FP64, or FP32 when T is: exact) as
 //   rows_out = 0: Zout[r][c] (eigenvectors as columns)
 //   rows_out = 1: Zout[c][r] (eigenvectors as rows)
 // Per reflector j = n-2 .. 0: s_c = tau_j v_j^H Z[:, c], Z[:, c] -= v_j s_c.
 // --------------------------------------------------------------------------
-template <typename T, int NB, int S, typename Team>
+template <typename T, int NB, int S, typename Team, typename TO>
 RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* twist, const T* e, const T* Ts,
-                          const cplx<T>* Vg, const cplx<T>* taug, RegSmem<T> sm, double* Zout, int rows_out) {
+                          const cplx<T>* Vg, const cplx<T>* taug, RegSmem<T> sm, TO* Zout, int rows_out) {
   constexpr int C = NB / S;
   constexpr int BL = C < 8 ? C : 8;
   constexpr int NBL = C / BL;
@@ -391,14 +391,14 @@ RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* twist, 
   }
   if (c < n) {
     // Zout element (r = s + S q, c): base + q * stride
-    double* zb = Zout + (rows_out ? static_cast<size_t>(c) * n + s : static_cast<size_t>(s) * n + c) * 2;
+    TO* zb = Zout + (rows_out ? static_cast<size_t>(c) * n + s : static_cast<size_t>(s) * n + c) * 2;
     const size_t zstep = rows_out ? static_cast<size_t>(S) * 2 : static_cast<size_t>(S) * n * 2;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       if (s + S * q < n) {
-        double* o = zb + q * zstep;
-        o[0] = static_cast<double>(z[q].re);
-        o[1] = static_cast<double>(z[q].im);
+        TO* o = zb + q * zstep;
+        o[0] = static_cast<TO>(z[q].re);
+        o[1] = static_cast<TO>(z[q].im);
       }
     }
   }

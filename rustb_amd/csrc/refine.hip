@@ -139,9 +139,9 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
 
 // One workgroup per k-point: the Rayleigh-like diagonal lt_i = S_ii / G_ii and
 // the spectral scale max |lt_i| are formed once in LDS, then every element
-// of the correction F' - I is written.
+// of the correction F' - I is written in FP32 (its products run in FP32).
 __global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2* __restrict__ S,
-                                                            const double2* __restrict__ G, double2* __restrict__ F,
+                                                            const double2* __restrict__ G, float2* __restrict__ F,
                                                             double gap_rel) {
   __shared__ double lt[kMaxEighN];
   __shared__ double gd[kMaxEighN];
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2
   const size_t k = blockIdx.x;
   const double2* Sk = S + k * n2;
   const double2* Gk = G + k * n2;
-  double2* Fk = F + k * n2;
+  float2* Fk = F + k * n2;
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) {
     const double g = Gk[static_cast<size_t>(i) * n + i].x;
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2
     } else {
       f = make_double2(-0.5 * gg.x, -0.5 * gg.y);
     }
-    Fk[e] = f;
+    Fk[e] = make_float2(static_cast<float>(f.x), static_cast<float>(f.y));
   }
 }
 
@@ -213,28 +213,29 @@ int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const d
 //   squared).  F' - I is of the size of that error, so its two products run
 //   on the FP32 matrix cores: their rounding (~1e-7 relative to a correction
 //   already ~1e-6 or smaller) stays below the FP64 level of the sums.
-// Buffers (each [nk][n][n] complex): H (consumed), Z (in; scratch after),
+// Buffers (each [nk][n][n] complex): H (consumed; F' - I in FP32), Z (in, FP32),
 // W1, W3 scratch; the refined eigenvectors are left in W2 as columns that
 // become unit vectors when multiplied by scale[k][j] ([nk][n]).
-int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, double* Z, double* W1, double* W2,
-                      double* W3, double* E, double* scale, double gap_rel) {
+int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, const float* Z, double* W1,
+                      double* W2, double* W3, double* E, double* scale, double gap_rel) {
   if (nk == 0) return RTB_OK;
   const size_t n2 = static_cast<size_t>(n) * n;
-  // FP64 products: HZ, S = Z^H HZ, G = Z^H Z
+  // FP64 products: HZ, S = Z^H HZ, G = Z^H Z (Z read from its exact FP32 storage)
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM64));
-  RTB_TRY(zgemm_batched(st, n, nk, H, n2, false, Z, n2, W1, n2));
-  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, W1, n2, W2, n2, true));
-  RTB_TRY(zgemm_batched(st, n, nk, Z, n2, true, Z, n2, W3, n2, true));
+  RTB_TRY(zgemm(st, n, nk, zop(H, n2), false, zop(Z, n2), W1, n2, false, nullptr));
+  RTB_TRY(zgemm(st, n, nk, zop(Z, n2), true, zop(W1, n2), W2, n2, true, nullptr));
+  RTB_TRY(zgemm(st, n, nk, zop(Z, n2), true, zop(Z, n2), W3, n2, true, nullptr));
   if (tmr) RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
+  float* F = reinterpret_cast<float*>(H);  // F' - I into H (H is dead)
   hipLaunchKernelGGL(oa_correction_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
                      reinterpret_cast<const double2*>(W2), reinterpret_cast<const double2*>(W3),
-                     reinterpret_cast<double2*>(H), gap_rel);  // F' - I into H (H is dead)
+                     reinterpret_cast<float2*>(F), gap_rel);
   RTB_HIP(hipGetLastError());
   if (tmr) RTB_TRY(tmr->end(1));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM32));
-  RTB_TRY(zgemm_add_f32_batched(st, n, nk, Z, n2, H, n2, W2, n2));   // Z' = Z + Z (F' - I)
-  RTB_TRY(zgemm_add_f32_batched(st, n, nk, W1, n2, H, n2, W3, n2));  // HZ' = HZ + HZ (F' - I)
+  RTB_TRY(zgemm_add_f32(st, n, nk, zop(Z, n2), zop(F, n2), W2, n2));   // Z' = Z + Z (F' - I)
+  RTB_TRY(zgemm_add_f32(st, n, nk, zop(W1, n2), zop(F, n2), W3, n2));  // HZ' = HZ + HZ (F' - I)
   if (tmr) RTB_TRY(tmr->end(2, 2 * flop_zgemm(n, nk)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
   const size_t tk = nk * n;

@@ -1,8 +1,10 @@
-// Batched complex matrix products on the matrix cores (gfx950), FP64 storage.
+// Batched complex matrix products on the matrix cores (gfx950).
 //
 // C = op(A) B (FP64 arithmetic) or C = A + A B (FP32 arithmetic, for small
 // corrections B) for a batch of n x n complex matrices (n <= 128 in practice,
-// any n works), row-major interleaved (re, im).  One workgroup computes one
+// any n works), row-major interleaved (re, im).  Each operand is stored in
+// FP64 or FP32 (values exact in FP32, or needed only to FP32 accuracy: half
+// the memory traffic); C is FP64.  One workgroup computes one
 // 64 x 64 output tile; its four waves own 32 x 32 quarters (2 x 2 blocks of
 // the 16 x 16 x 4 MFMA).
 //
@@ -74,11 +76,20 @@ struct Mfma<float> {
 // matrix); D may alias A (it is read only), C aliases nothing.  The
 // waves-per-SIMD attribute (FP64 2, FP32 4) bounds the registers so that the
 // software-pipelined fragments fit next to the accumulators without spills.
-template <typename T, bool ADD>
+template <typename V>
+__device__ inline V zero2() {
+  V v;
+  v.x = 0;
+  v.y = 0;
+  return v;
+}
+
+// SA, SB: storage of A (and D) and B, double2 or float2.
+template <typename T, bool ADD, typename SA, typename SB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4, sizeof(T) == 8 ? 2 : 4)))
-zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A, size_t sA, int conj_a,
-               const double2* __restrict__ B, size_t sB, const double2* D, size_t sD, double2* __restrict__ C,
+zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t sA, int conj_a,
+               const SB* __restrict__ B, size_t sB, const SA* D, size_t sD, double2* __restrict__ C,
                size_t sC, const double* __restrict__ sc) {
   typedef typename Mfma<T>::v4 v4;
   __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
@@ -100,12 +111,13 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A, si
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 
   // global -> register staging: 4 elements of op(A) and 4 of B per thread
-  double2 pa[4], pb[4];
+  SA pa[4];
+  SB pb[4];
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int q = tid + 256 * u;
-      double2 v = make_double2(0.0, 0.0);
+      SA v = zero2<SA>();
       if (conj_a) {  // op(A)[r][k] = conj(A[k][r]); lanes along r
         const int kk = q / kTile, i = q % kTile;
         const int r = row0 + i, k = k0 + kk;
@@ -118,7 +130,7 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A, si
       pa[u] = v;
       const int kk = q / kTile, j = q % kTile;
       const int k = k0 + kk, c = col0 + j;
-      pb[u] = (k < n && c < n) ? B[static_cast<size_t>(k) * n + c] : make_double2(0.0, 0.0);
+      pb[u] = (k < n && c < n) ? B[static_cast<size_t>(k) * n + c] : zero2<SB>();
     }
   };
   auto store_chunk = [&]() {
@@ -206,9 +218,9 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A, si
           }
           const size_t o = static_cast<size_t>(i) * n + j;
           if (ADD) {
-            const double2 d = D[o];
-            re += d.x;
-            im += d.y;
+            const SA d = D[o];
+            re += static_cast<double>(d.x);
+            im += static_cast<double>(d.y);
           }
           C[o] = make_double2(re, im);
           if (mirror) C[static_cast<size_t>(j) * n + i] = make_double2(re, -im);
@@ -216,14 +228,17 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const double2* __restrict__ A, si
       }
 }
 
-template <typename T, bool ADD>
-int zgemm_launch(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
-                 size_t sB, const double* D, size_t sD, double* C, size_t sC, bool herm, const double* sc) {
+template <typename T, bool ADD, typename SA, typename SB>
+int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_a, const ZOperand& B,
+                 const ZOperand& D, double* C, size_t sC, bool herm, const double* sc) {
   if (nb == 0 || n <= 0) return RTB_OK;
   TileMap tm;
   tm.side = (n + kTile - 1) / kTile;
   tm.herm = herm ? 1 : 0;
   tm.tiles = herm ? tm.side * (tm.side + 1) / 2 : tm.side * tm.side;
+  const SA* pA = static_cast<const SA*>(A.p);
+  const SB* pB = static_cast<const SB*>(B.p);
+  const SA* pD = static_cast<const SA*>(D.p);
   // grid: kXcd interleaved streams of (matrix group, tile); one launch per
   // slice of at most 2^24 workgroups (grid size in work-items < 2^32)
   const size_t per_launch = (static_cast<size_t>(1) << 24) / (kXcd * tm.tiles) * kXcd;
@@ -231,11 +246,10 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const double* A, size_t sA, b
     const size_t cnt = nb - off < per_launch ? nb - off : per_launch;
     const size_t groups = (cnt + kXcd - 1) / kXcd;
     const size_t wgs = groups * tm.tiles * kXcd;
-    hipLaunchKernelGGL((zgemm3m_kernel<T, ADD>), dim3(static_cast<unsigned>(wgs)), dim3(256), 0, st, n, cnt, tm,
-                       reinterpret_cast<const double2*>(A) + off * sA, sA, conj_a ? 1 : 0,
-                       reinterpret_cast<const double2*>(B) + off * sB, sB,
-                       D ? reinterpret_cast<const double2*>(D) + off * sD : nullptr, sD,
-                       reinterpret_cast<double2*>(C) + off * sC, sC, sc ? sc + off * n : nullptr);
+    hipLaunchKernelGGL((zgemm3m_kernel<T, ADD, SA, SB>), dim3(static_cast<unsigned>(wgs)), dim3(256), 0, st, n, cnt,
+                       tm, pA + off * A.stride, A.stride, conj_a ? 1 : 0, pB + off * B.stride, B.stride,
+                       pD ? pD + off * D.stride : nullptr, D.stride, reinterpret_cast<double2*>(C) + off * sC, sC,
+                       sc ? sc + off * n : nullptr);
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -243,19 +257,29 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const double* A, size_t sA, b
 
 }  // namespace
 
+int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B, double* C, size_t sC, bool herm,
+          const double* s) {
+  const ZOperand none{};
+  if (!A.f32 && !B.f32) return zgemm_launch<double, false, double2, double2>(st, n, nb, A, conj_a, B, none, C, sC, herm, s);
+  if (!A.f32 && B.f32) return zgemm_launch<double, false, double2, float2>(st, n, nb, A, conj_a, B, none, C, sC, herm, s);
+  if (A.f32 && !B.f32) return zgemm_launch<double, false, float2, double2>(st, n, nb, A, conj_a, B, none, C, sC, herm, s);
+  return zgemm_launch<double, false, float2, float2>(st, n, nb, A, conj_a, B, none, C, sC, herm, s);
+}
+
+int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC) {
+  if (!B.f32) return fail(RTB_ERR_UNSUPPORTED, "zgemm_add_f32: the correction operand is stored in FP32");
+  if (A.f32) return zgemm_launch<float, true, float2, float2>(st, n, nb, A, false, B, A, C, sC, false, nullptr);
+  return zgemm_launch<float, true, double2, float2>(st, n, nb, A, false, B, A, C, sC, false, nullptr);
+}
+
 int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
                   size_t sB, double* C, size_t sC, bool herm) {
-  return zgemm_launch<double, false>(st, n, nb, A, sA, conj_a, B, sB, nullptr, 0, C, sC, herm, nullptr);
+  return zgemm(st, n, nb, zop(A, sA), conj_a, zop(B, sB), C, sC, herm, nullptr);
 }
 
 int zgemm_batched_scaled(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
                          size_t sB, double* C, size_t sC, bool herm, const double* s) {
-  return zgemm_launch<double, false>(st, n, nb, A, sA, conj_a, B, sB, nullptr, 0, C, sC, herm, s);
-}
-
-int zgemm_add_f32_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, const double* B, size_t sB,
-                          double* C, size_t sC) {
-  return zgemm_launch<float, true>(st, n, nb, A, sA, false, B, sB, A, sA, C, sC, false, nullptr);
+  return zgemm(st, n, nb, zop(A, sA), conj_a, zop(B, sB), C, sC, herm, s);
 }
 
 }  // namespace rtb
