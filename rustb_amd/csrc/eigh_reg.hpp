@@ -130,12 +130,13 @@ RTB_HD void tridiag_reg(Team& tm, int n, const double* Hm, RegSmem<T> sm, T* Dg,
     if (!(xnorm2 == T(0) && alpha.im == T(0))) {  // zlarfg
       T nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xnorm2);
       beta = alpha.re >= T(0) ? -nrm : nrm;
-      tau.re = (beta - alpha.re) / beta;
-      tau.im = -alpha.im / beta;
+      const T rb = recip(beta);
+      tau.re = (beta - alpha.re) * rb;
+      tau.im = -alpha.im * rb;
       T ar = alpha.re - beta, ai = alpha.im;
-      T den = ar * ar + ai * ai;
-      scale.re = ar / den;
-      scale.im = -ai / den;
+      const T rden = recip(ar * ar + ai * ai);
+      scale.re = ar * rden;
+      scale.im = -ai * rden;
     }
     if (s == 0) {
       cplx<T> v = {T(0), T(0)};

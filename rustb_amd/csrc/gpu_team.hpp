@@ -5,69 +5,63 @@
 
 namespace rtb {
 
+// Sum over the 16 lanes of each DPP row, returned to every lane of the row:
+// quad swaps (xor 1, xor 2), then half-row and row mirrors.  Register-to-
+// register moves, no LDS round trips (cf. a shuffle through ds_bpermute).
+template <int Ctrl>
+__device__ inline float dpp_mov(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), Ctrl, 0xF, 0xF, false));
+}
+template <int Ctrl>
+__device__ inline double dpp_mov(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), Ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), Ctrl, 0xF, 0xF, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <typename T>
+__device__ inline T row_sum16(T x) {
+  x += dpp_mov<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+  x += dpp_mov<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+  x += dpp_mov<0x141>(x);  // row_half_mirror
+  x += dpp_mov<0x140>(x);  // row_mirror
+  return x;
+}
+__device__ inline float lane_value(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ inline double lane_value(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+// Sum over a full wave of 64 active lanes, wave-uniform; fixed order.
+template <typename T>
+__device__ inline T wave_sum64(T x) {
+  x = row_sum16(x);
+  return (lane_value(x, 0) + lane_value(x, 16)) + (lane_value(x, 32) + lane_value(x, 48));
+}
+
 struct GpuTeam {
-  static constexpr bool kPairs = true;   // lanes (2i, 2i+1) share rows
   static constexpr int kRedStride = 32;  // >= 2 x the waves of a workgroup
   int id, nt;
   __device__ GpuTeam() : id(threadIdx.x), nt(blockDim.x) {}
   __device__ void sync() { __syncthreads(); }
 
-  // x + partner's x for the lane pair (2i, 2i+1); both lanes must be active.
-  template <typename T>
-  __device__ T pair_sum(T x, T*) {
-    return x + __shfl_xor(x, 1, 64);
-  }
-
-  // Sum over the workgroup, returned to every thread.  Deterministic: a fixed
-  // butterfly inside each wave, then the per-wave partials in wave order.
-  template <typename T>
-  __device__ T reduce_sum(T x, T* red) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    const int wave = id >> 6, nw = (nt + 63) >> 6;
-    if ((id & 63) == 0) red[wave] = x;
-    __syncthreads();
-    T s = T(0);
-    for (int w = 0; w < nw; ++w) s += red[w];
-    __syncthreads();
-    return s;
-  }
-
-  // The same sum without a trailing barrier: `red` must not be rewritten
-  // until every thread has passed another barrier.
+  // Sum over the workgroup, returned to every thread, without a trailing
+  // barrier: `red` must not be rewritten until every thread has passed
+  // another barrier.  Deterministic: each wave in a fixed order, then the
+  // per-wave partials in wave order.  Every lane of the workgroup calls it.
   template <typename T>
   __device__ T reduce_sum_ns(T x, T* red) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    x = wave_sum64(x);
     const int wave = id >> 6, nw = (nt + 63) >> 6;
     if ((id & 63) == 0) red[wave] = x;
     __syncthreads();
     T s = T(0);
     for (int w = 0; w < nw; ++w) s += red[w];
     return s;
-  }
-
-  // Two sums at once (no trailing barrier), red holds interleaved pairs.
-  template <typename T>
-  __device__ void reduce_sum2_ns(T& x, T& y, T* red) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      x += __shfl_xor(x, off, 64);
-      y += __shfl_xor(y, off, 64);
-    }
-    const int wave = id >> 6, nw = (nt + 63) >> 6;
-    if ((id & 63) == 0) {
-      red[2 * wave] = x;
-      red[2 * wave + 1] = y;
-    }
-    __syncthreads();
-    T sx = T(0), sy = T(0);
-    for (int w = 0; w < nw; ++w) {
-      sx += red[2 * w];
-      sy += red[2 * w + 1];
-    }
-    x = sx;
-    y = sy;
   }
 };
 
@@ -76,7 +70,6 @@ struct GpuTeam {
 // every sub-team must execute the same barrier sequence (idle sub-teams take
 // id >= n and skip all per-index work).
 struct GpuSubTeam {
-  static constexpr bool kPairs = false;
   int id, nt;
   __device__ GpuSubTeam(int id_, int nt_) : id(id_), nt(nt_) {}
   __device__ void sync() { __syncthreads(); }

@@ -12,49 +12,17 @@
 
 namespace {
 
-template <bool Pairs>
 struct CpuTeam {
-  static constexpr bool kPairs = Pairs;
   static constexpr int kRedStride = 2048;  // >= 2 x the largest emulated team
   int id, nt;
   std::barrier<>* bar;
   void sync() { bar->arrive_and_wait(); }
-  template <typename T>
-  T pair_sum(T x, T* red) {
-    red[id] = x;
-    sync();
-    T y = x + red[id ^ 1];
-    sync();
-    return y;
-  }
   template <typename T>
   T reduce_sum_ns(T x, T* red) {
     red[id] = x;
     sync();
     T s = T(0);
     for (int i = 0; i < nt; ++i) s += red[i];
-    return s;
-  }
-  template <typename T>
-  void reduce_sum2_ns(T& x, T& y, T* red) {
-    red[2 * id] = x;
-    red[2 * id + 1] = y;
-    sync();
-    T sx = T(0), sy = T(0);
-    for (int i = 0; i < nt; ++i) {
-      sx += red[2 * i];
-      sy += red[2 * i + 1];
-    }
-    x = sx;
-    y = sy;
-  }
-  template <typename T>
-  T reduce_sum(T x, T* red) {
-    red[id] = x;
-    sync();
-    T s = T(0);
-    for (int i = 0; i < nt; ++i) s += red[i];
-    sync();
     return s;
   }
 };
@@ -76,7 +44,7 @@ int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_
     std::barrier<> bar(nt);
     std::vector<std::thread> th;
     for (int i = 0; i < nt; ++i) th.emplace_back([&, i] {
-        CpuTeam<false> tm{i, nt, &bar};
+        CpuTeam tm{i, nt, &bar};
         f(tm);
       });
     for (auto& t : th) t.join();
