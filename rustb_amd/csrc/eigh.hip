@@ -40,9 +40,13 @@ __global__ void __launch_bounds__(NB * S) eigh_tridiag_reg_kernel(int n, const d
 
 // Register-resident back-transformation Z = Q Y: one workgroup per matrix,
 // each thread holding NB / S entries of one column of Z.  twist / e: the
-// trieig outputs ([nmat][n] each), Ts: its scratch ([nmat][3][n][n]).
+// trieig outputs ([nmat][n] each), Ts: its scratch ([nmat][3][n][n]).  The
+// FP32 kernels are held to 3 waves per SIMD (168 registers): the few spills
+// this costs sit in the eigenvector generation before the reflector loop,
+// and the third workgroup per CU hides the per-reflector barrier (FP32,
+// n = 128: ~10% faster than 217 registers at 2 waves).
 template <typename T, int NB, int S, typename TO>
-__global__ void __launch_bounds__(NB * S) eigh_backtrans_reg_kernel(int n, const T* __restrict__ Y,
+__global__ void __launch_bounds__(NB * S) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1, 8))) eigh_backtrans_reg_kernel(int n, const T* __restrict__ Y,
                                                                     const T* __restrict__ twist,
                                                                     const T* __restrict__ e,
                                                                     const T* __restrict__ Ts,

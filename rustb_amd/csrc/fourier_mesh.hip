@@ -119,26 +119,27 @@ struct FinalArgs {
 // Atom-gauge factor conj(phi_m) phi_c = exp(i 2 pi (tau_c - tau_m).k) is one
 // sincospi per element and k.
 constexpr int kFB = 2;
+constexpr int kTWF = 64;  // matrix elements per workgroup of the final pass
 
 __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, double2* __restrict__ H,
                                                         double2* __restrict__ V) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double2* W = reinterpret_cast<double2*>(smem_raw);  // [N1]
-  double2* X0 = W + a.N1;                             // [L1][kTW]
-  double2* X1 = X0 + a.L1 * kTW;
-  double2* X2 = X1 + a.L1 * kTW;
+  double2* X0 = W + a.N1;                             // [L1][kTWF]
+  double2* X1 = X0 + a.L1 * kTWF;
+  double2* X2 = X1 + a.L1 * kTWF;
   const size_t n2 = static_cast<size_t>(a.n) * a.n;
   const size_t pl = blockIdx.y, pg = p0 + pl;
-  const size_t t0 = static_cast<size_t>(blockIdx.x) * kTW;
-  const int tw = static_cast<int>(n2 - t0 < static_cast<size_t>(kTW) ? n2 - t0 : kTW);
+  const size_t t0 = static_cast<size_t>(blockIdx.x) * kTWF;
+  const int tw = static_cast<int>(n2 - t0 < static_cast<size_t>(kTWF) ? n2 - t0 : kTWF);
   const int ng = a.np ? a.dim : 1;  // inputs needed
   for (int m = threadIdx.x; m < a.N1; m += 256) {
     double s, c;
     sincospi(2.0 * static_cast<double>(m) / static_cast<double>(a.N1), &s, &c);
     W[m] = make_double2(c, s);
   }
-  for (int q = threadIdx.x; q < a.L1 * kTW; q += 256) {
-    const int l = q / kTW, t = q % kTW;
+  for (int q = threadIdx.x; q < a.L1 * kTWF; q += 256) {
+    const int l = q / kTWF, t = q % kTWF;
     const size_t src = (static_cast<size_t>(l) * a.P + pg) * n2 + t0 + t;
     const bool ok = t < tw;
     X0[q] = ok ? a.G0[src] : make_double2(0.0, 0.0);
@@ -146,7 +147,7 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
     if (ng > 2) X2[q] = ok ? a.G2[src] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const int t = threadIdx.x % kTW;
+  const int t = threadIdx.x % kTWF;
   if (t >= tw) return;
   const size_t e = t0 + t;
   const int mrow = static_cast<int>(e / a.n), mcol = static_cast<int>(e % a.n);
@@ -159,8 +160,8 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
   double wd[2] = {0.0, 0.0};  // w_p . (tau_c - tau_m)
   for (int p = 0; p < a.np; ++p)
     wd[p] = a.wtau[static_cast<size_t>(p) * a.n + mcol] - a.wtau[static_cast<size_t>(p) * a.n + mrow];
-  constexpr int kGroups = 256 / kTW;
-  for (int ib = (threadIdx.x / kTW) * kFB; ib < a.N1; ib += kGroups * kFB) {
+  constexpr int kGroups = 256 / kTWF;
+  for (int ib = (threadIdx.x / kTWF) * kFB; ib < a.N1; ib += kGroups * kFB) {
     double hr[kFB], hi[kFB], d1r[kFB], d1i[kFB], g1r[kFB], g1i[kFB], g2r[kFB], g2i[kFB];
     int m[kFB], step[kFB];
 #pragma unroll
@@ -171,9 +172,9 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
       hr[u] = hi[u] = d1r[u] = d1i[u] = g1r[u] = g1i[u] = g2r[u] = g2i[u] = 0.0;
     }
     for (int l = 0; l < a.L1; ++l) {
-      const double2 x = X0[l * kTW + t];
-      const double2 y = ng > 1 ? X1[l * kTW + t] : make_double2(0.0, 0.0);
-      const double2 z = ng > 2 ? X2[l * kTW + t] : make_double2(0.0, 0.0);
+      const double2 x = X0[l * kTWF + t];
+      const double2 y = ng > 1 ? X1[l * kTWF + t] : make_double2(0.0, 0.0);
+      const double2 z = ng > 2 ? X2[l * kTWF + t] : make_double2(0.0, 0.0);
       const double r1 = static_cast<double>(a.lo1 + l);
 #pragma unroll
       for (int u = 0; u < kFB; ++u) {
@@ -361,13 +362,14 @@ int mesh_final(hipStream_t st, const rtb_model* m, const MeshPlan& p, const doub
   f.tau = a.tau;
   f.wtau = a.wtau;
   f.gauge = a.gauge;
-  const size_t smem = sizeof(double2) * (static_cast<size_t>(p.N[0]) + 3 * static_cast<size_t>(p.L[0]) * kTW);
-  const size_t tiles = (n2 + kTW - 1) / kTW;
+  const size_t smem = sizeof(double2) * (static_cast<size_t>(p.N[0]) + 3 * static_cast<size_t>(p.L[0]) * kTWF);
+  const size_t tiles = (n2 + kTWF - 1) / kTWF;
+  const size_t vk = f.combine ? 1 : 2;  // V matrices per k
   for (size_t q0 = 0; q0 < np_pairs; q0 += 65535) {
     const size_t qc = np_pairs - q0 < 65535 ? np_pairs - q0 : 65535;
     hipLaunchKernelGGL(dft_final_kernel, dim3(static_cast<unsigned>(tiles), static_cast<unsigned>(qc)), dim3(256),
                        smem, st, f, p0 + q0, reinterpret_cast<double2*>(H) + q0 * p.N[0] * n2,
-                       V ? reinterpret_cast<double2*>(V) + q0 * p.N[0] * 2 * n2 : nullptr);
+                       V ? reinterpret_cast<double2*>(V) + q0 * p.N[0] * vk * n2 : nullptr);
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
