@@ -264,7 +264,7 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, doub
   RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE));
   RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
   float* Zf = reinterpret_cast<float*>(Z);  // exact: the route's values are FP32
-  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Zf, false));
+  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Zf, false, scale));  // |z_j|^2 into scale
   RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
   // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
   // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius

@@ -52,7 +52,8 @@ __global__ void __launch_bounds__(NB * S) __attribute__((amdgpu_waves_per_eu(siz
                                                                     const T* __restrict__ Ts,
                                                                     const cplx<T>* __restrict__ Vg,
                                                                     const cplx<T>* __restrict__ taug,
-                                                                    TO* __restrict__ Zout, int rows_out) {
+                                                                    TO* __restrict__ Zout, int rows_out,
+                                                                    double* __restrict__ gdiag) {
   __shared__ cplx<T> vp[3 * NB], yp[2 * S * NB];
   __shared__ T es[NB];
   GpuTeam tm;
@@ -60,7 +61,7 @@ __global__ void __launch_bounds__(NB * S) __attribute__((amdgpu_waves_per_eu(siz
   const size_t nn = static_cast<size_t>(n) * n;
   RegSmem<T> sm{nullptr, vp, nullptr, yp, nullptr, es};
   backtrans_reg<T, NB, S>(tm, n, Y + mat * nn, n, twist + mat * n, e + mat * n, Ts + mat * nn * 3, Vg + mat * nn,
-                          taug + mat * n, sm, Zout + mat * nn * 2, rows_out);
+                          taug + mat * n, sm, Zout + mat * nn * 2, rows_out, gdiag ? gdiag + mat * n : nullptr);
 }
 
 // Slice counts (threads per row or column = S).  Tridiagonalisation at
@@ -98,21 +99,21 @@ void tridiag_reg_dispatch(hipStream_t st, int n, size_t nmat, const double* H, T
 
 template <typename T, int NB, typename TO>
 void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* Ts,
-                          const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out) {
+                          const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out, double* gdiag) {
   constexpr int S = RegShape<T, NB>::Sbt;
   hipLaunchKernelGGL((eigh_backtrans_reg_kernel<T, NB, S, TO>), dim3(static_cast<unsigned>(nmat)), dim3(NB * S), 0, st,
-                     n, Y, DE + 2 * nmat * n, DE + nmat * n, Ts, Vg, taug, Z, rows_out);
+                     n, Y, DE + 2 * nmat * n, DE + nmat * n, Ts, Vg, taug, Z, rows_out, gdiag);
 }
 
 template <typename T, typename TO>
 void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* Ts,
-                            const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out) {
+                            const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out, double* gdiag) {
   if (n <= 32) {
-    launch_backtrans_reg<T, 32, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 32, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out, gdiag);
   } else if (n <= 64) {
-    launch_backtrans_reg<T, 64, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 64, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out, gdiag);
   } else {
-    launch_backtrans_reg<T, 128, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out);
+    launch_backtrans_reg<T, 128, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out, gdiag);
   }
 }
 
@@ -222,10 +223,10 @@ int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H,
 
 template <typename T, typename TO>
 int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* S, const T* Vg,
-                       const T* taug, TO* Z, bool rows_out) {
+                       const T* taug, TO* Z, bool rows_out, double* gdiag) {
   if (nmat == 0) return RTB_OK;
   backtrans_reg_dispatch<T, TO>(st, n, nmat, Y, DE, S, reinterpret_cast<const cplx<T>*>(Vg),
-                            reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0);
+                                reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0, gdiag);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
@@ -238,8 +239,8 @@ int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
 }
 
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
-                            const float* Vg, const float* taug, float* Z, bool rows_out) {
-  return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
+                            const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag) {
+  return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, gdiag);
 }
 
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
@@ -249,7 +250,7 @@ int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
 
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
                             const double* Vg, const double* taug, double* Z, bool rows_out) {
-  return eigh_backtransform<double, double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out);
+  return eigh_backtransform<double, double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, nullptr);
 }
 
 }  // namespace rtb

@@ -17,14 +17,16 @@ int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const doubl
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
                      float* Vg, float* taug, float* Y, float* S, float* DE);
-// Z: [nmat][n][n] complex FP32 (the values of the FP32 route, exact)
+// Z: [nmat][n][n] complex FP32 (the values of the FP32 route, exact);
+// gdiag (optional, [nmat][n]): the squared column norms of Z in FP64.
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
-                            const float* Vg, const float* taug, float* Z, bool rows_out);
+                            const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag);
 // FP64 refinement of FP32 eigenpairs (refine.hip): one Ogita-Aishima step,
 // Rayleigh quotients, unit columns, Rayleigh-Ritz in clusters with gaps
 // below gap_rel * max(1, spectral radius).  Z: [nk][n][n] FP32 columns (in);
 // H is overwritten; the refined columns are written to W2 (FP64), unit
-// after multiplication by scale[k][j] ([nk][n], written); W1, W3 scratch.
+// after multiplication by scale[k][j] ([nk][n]: the squared column norms of
+// Z on entry, as eigh_backtransform_fp32 writes them); W1, W3 scratch.
 int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, const float* Z, double* W1,
                       double* W2, double* W3, double* E, double* scale, double gap_rel);
 // out[k][r][c] = Z[k][c][r] * scale[k][r] (eigenvector columns to rows;
@@ -83,6 +85,12 @@ inline ZOperand zop(const float* p, size_t stride) { return {p, stride, true}; }
 // given; herm: the product is Hermitian (lower tiles computed, mirrored).
 int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B, double* C, size_t sC, bool herm,
           const double* s);
+// The Gram product G = Z^H Z (FP64, Z stored in FP32, stride n^2) of the
+// Ogita-Aishima step with G consumed in the epilogue: F (FP32, [nb][n][n])
+// receives the correction F' - I from G, S = Z^H H Z ([nb][n][n]) and the
+// exact diagonal gd ([nb][n]) of G (n <= 128).
+int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double* S, const double* gd, float* F,
+                  double gap_rel);
 // C = A + A B with the product on the FP32 matrix cores, for a correction B
 // stored in FP32 (A in either storage; FP64 sums).  C must not alias A or B.
 int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC);

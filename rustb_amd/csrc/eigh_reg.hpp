@@ -239,10 +239,12 @@ RTB_HD void tridiag_reg(Team& tm, int n, const double* Hm, RegSmem<T> sm, T* Dg,
 //   rows_out = 0: Zout[r][c] (eigenvectors as columns)
 //   rows_out = 1: Zout[c][r] (eigenvectors as rows)
 // Per reflector j = n-2 .. 0: s_c = tau_j v_j^H Z[:, c], Z[:, c] -= v_j s_c.
+// gdiag (optional, [n]): the squared 2-norms of the stored columns.
 // --------------------------------------------------------------------------
 template <typename T, int NB, int S, typename Team, typename TO>
 RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* twist, const T* e, const T* Ts,
-                          const cplx<T>* Vg, const cplx<T>* taug, RegSmem<T> sm, TO* Zout, int rows_out) {
+                          const cplx<T>* Vg, const cplx<T>* taug, RegSmem<T> sm, TO* Zout, int rows_out,
+                          double* gdiag = nullptr) {
   constexpr int C = NB / S;
   constexpr int BL = C < 8 ? C : 8;
   constexpr int NBL = C / BL;
@@ -388,6 +390,26 @@ RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* twist, 
           }
         }
       }
+    }
+  }
+  if (gdiag) {
+    // squared 2-norm of column c of the stored Z, in FP64 (exact products of
+    // the stored values), reduced over the S slices in shared memory
+    double g = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const double zr = static_cast<double>(static_cast<TO>(z[q].re));
+      const double zi = static_cast<double>(static_cast<TO>(z[q].im));
+      if (s + S * q < n) g += zr * zr + zi * zi;
+    }
+    tm.sync();  // the partial-sum buffer is free
+    double* gp = reinterpret_cast<double*>(sm.yp);
+    gp[s * NB + c] = g;
+    tm.sync();
+    if (s == 0 && c < n) {
+      double t = 0.0;
+      for (int u = 0; u < S; ++u) t += gp[u * NB + c];
+      gdiag[c] = t;
     }
   }
   if (c < n) {

@@ -137,49 +137,6 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
   }
 }
 
-// One workgroup per k-point: the Rayleigh-like diagonal lt_i = S_ii / G_ii and
-// the spectral scale max |lt_i| are formed once in LDS, then every element
-// of the correction F' - I is written in FP32 (its products run in FP32).
-__global__ void __launch_bounds__(256) oa_correction_kernel(int n, const double2* __restrict__ S,
-                                                            const double2* __restrict__ G, float2* __restrict__ F,
-                                                            double gap_rel) {
-  __shared__ double lt[kMaxEighN];
-  __shared__ double gd[kMaxEighN];
-  __shared__ double red[4];
-  const size_t n2 = static_cast<size_t>(n) * n;
-  const size_t k = blockIdx.x;
-  const double2* Sk = S + k * n2;
-  const double2* Gk = G + k * n2;
-  float2* Fk = F + k * n2;
-  double m = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    const double g = Gk[static_cast<size_t>(i) * n + i].x;
-    gd[i] = g;
-    lt[i] = Sk[static_cast<size_t>(i) * n + i].x / g;
-    m = fmax(m, fabs(lt[i]));
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-  __syncthreads();
-  const double scale = fmax(fmax(fmax(red[0], red[1]), fmax(red[2], red[3])), 1.0);
-  const double thr = gap_rel * scale;
-  for (size_t e = threadIdx.x; e < n2; e += 256) {
-    const int i = static_cast<int>(e / n), j = static_cast<int>(e % n);
-    const double2 s = Sk[e], gg = Gk[e];
-    double2 f;
-    if (i == j) {
-      f = make_double2(0.5 * (1.0 - gd[i]), 0.0);
-    } else if (fabs(lt[i] - lt[j]) > thr) {
-      const double inv = 1.0 / (lt[j] - lt[i]);
-      f = make_double2((s.x - lt[j] * gg.x) * inv, (s.y - lt[j] * gg.y) * inv);
-    } else {
-      f = make_double2(-0.5 * gg.x, -0.5 * gg.y);
-    }
-    Fk[e] = make_float2(static_cast<float>(f.x), static_cast<float>(f.y));
-  }
-}
-
 __global__ void transpose_kernel(int n, size_t nk, const double2* __restrict__ Z, const double* __restrict__ scale,
                                  double2* __restrict__ out) {
   const size_t g = blockIdx.x * 256ull + threadIdx.x;
@@ -205,7 +162,8 @@ int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const d
 
 // One FP64 Ogita-Aishima refinement step of the FP32 eigenpairs, then Rayleigh
 // quotients and Rayleigh-Ritz inside clusters:
-//   HZ = H Z, S = Z^H HZ, G = Z^H Z, lt_i = S_ii / G_ii,
+//   HZ = H Z, S = Z^H HZ, G = Z^H Z, lt_i = S_ii / G_ii (G_ii exact, from the
+//   back-transformation; G is consumed in its product's epilogue),
 //   F'_ij = (S_ij - lt_j G_ij) / (lt_j - lt_i)  (i != j, |lt_i - lt_j| > thr)
 //         = -G_ij / 2                           (i != j, inside a cluster)
 //   F'_ii = 1 + (1 - G_ii) / 2,
@@ -215,24 +173,21 @@ int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const d
 //   already ~1e-6 or smaller) stays below the FP64 level of the sums.
 // Buffers (each [nk][n][n] complex): H (consumed; F' - I in FP32), Z (in, FP32),
 // W1, W3 scratch; the refined eigenvectors are left in W2 as columns that
-// become unit vectors when multiplied by scale[k][j] ([nk][n]).
+// become unit vectors when multiplied by scale[k][j] ([nk][n]; on entry the
+// squared column norms of Z).
 int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, const float* Z, double* W1,
                       double* W2, double* W3, double* E, double* scale, double gap_rel) {
   if (nk == 0) return RTB_OK;
   const size_t n2 = static_cast<size_t>(n) * n;
-  // FP64 products: HZ, S = Z^H HZ, G = Z^H Z (Z read from its exact FP32 storage)
+  // FP64 products: HZ, S = Z^H HZ, and G = Z^H Z consumed by its epilogue,
+  // which writes F' - I (Z read from its exact FP32 storage; the exact
+  // diagonal of G comes in `scale`)
+  float* F = reinterpret_cast<float*>(H);  // F' - I into H (H is dead after HZ)
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM64));
   RTB_TRY(zgemm(st, n, nk, zop(H, n2), false, zop(Z, n2), W1, n2, false, nullptr));
   RTB_TRY(zgemm(st, n, nk, zop(Z, n2), true, zop(W1, n2), W2, n2, true, nullptr));
-  RTB_TRY(zgemm(st, n, nk, zop(Z, n2), true, zop(Z, n2), W3, n2, true, nullptr));
+  RTB_TRY(zgemm_gram_oa(st, n, nk, Z, W2, scale, F, gap_rel));
   if (tmr) RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk)));
-  if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
-  float* F = reinterpret_cast<float*>(H);  // F' - I into H (H is dead)
-  hipLaunchKernelGGL(oa_correction_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
-                     reinterpret_cast<const double2*>(W2), reinterpret_cast<const double2*>(W3),
-                     reinterpret_cast<float2*>(F), gap_rel);
-  RTB_HIP(hipGetLastError());
-  if (tmr) RTB_TRY(tmr->end(1));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM32));
   RTB_TRY(zgemm_add_f32(st, n, nk, zop(Z, n2), zop(F, n2), W2, n2));   // Z' = Z + Z (F' - I)
   RTB_TRY(zgemm_add_f32(st, n, nk, zop(W1, n2), zop(F, n2), W3, n2));  // HZ' = HZ + HZ (F' - I)

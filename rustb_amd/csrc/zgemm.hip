@@ -84,14 +84,30 @@ __device__ inline V zero2() {
   return v;
 }
 
+// Optional epilogue of the Gram product G = Z^H Z of the Ogita-Aishima step:
+// G itself is not stored; each element becomes the FP32 correction
+//   F'_ij - I_ij = (S_ij - lt_j G_ij) / (lt_j - lt_i)   (i != j, |lt_i - lt_j| > thr)
+//                = -G_ij / 2                           (i != j, inside a cluster)
+//                = (1 - gd_i) / 2                      (i == j)
+// from S = Z^H H Z (already stored), the exact diagonal gd of G and the
+// Rayleigh quotients lt_i = S_ii / gd_i; thr = gap_rel * max(1, max |lt|).
+struct OaEpilogue {
+  const double2* S = nullptr;  // [nmat][n][n], stride sC
+  const double* gd = nullptr;  // [nmat][n]
+  float2* F = nullptr;         // [nmat][n][n], stride sC
+  double gap_rel = 0.0;
+};
+
 // SA, SB: storage of A (and D) and B, double2 or float2.
 template <typename T, bool ADD, typename SA, typename SB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4, sizeof(T) == 8 ? 2 : 4)))
 zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t sA, int conj_a,
                const SB* __restrict__ B, size_t sB, const SA* D, size_t sD, double2* __restrict__ C,
-               size_t sC, const double* __restrict__ sc) {
+               size_t sC, const double* __restrict__ sc, OaEpilogue oa) {
   typedef typename Mfma<T>::v4 v4;
+  __shared__ double s_lt[kMaxEighN];  // Rayleigh quotients (OA epilogue)
+  __shared__ double s_red[4];
   __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
   __shared__ T sB3[3][kKc][kLd];  // B tile, k-major
   // XCD-aware decomposition of the workgroup index
@@ -108,6 +124,38 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
   if (sc) sc += mat * n;
   const int row0 = ti * kTile, col0 = tj * kTile;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double thr = 0.0;
+  if (oa.F) {  // workgroup-uniform
+    oa.S += mat * sC;
+    oa.gd += mat * n;
+    oa.F += mat * sC;
+    double m = 0.0;
+    for (int i = tid; i < n; i += 256) {
+      const double l = oa.S[static_cast<size_t>(i) * n + i].x / oa.gd[i];
+      s_lt[i] = l;
+      m = fmax(m, fabs(l));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (lane == 0) s_red[wave] = m;
+    __syncthreads();
+    thr = oa.gap_rel * fmax(fmax(fmax(s_red[0], s_red[1]), fmax(s_red[2], s_red[3])), 1.0);
+  }
+  auto oa_entry = [&](int i, int j, double gr, double gi, double2 sij) -> float2 {
+    double fr, fi;
+    if (i == j) {
+      fr = 0.5 * (1.0 - oa.gd[i]);
+      fi = 0.0;
+    } else if (fabs(s_lt[i] - s_lt[j]) > thr) {
+      const double inv = 1.0 / (s_lt[j] - s_lt[i]);
+      fr = (sij.x - s_lt[j] * gr) * inv;
+      fi = (sij.y - s_lt[j] * gi) * inv;
+    } else {
+      fr = -0.5 * gr;
+      fi = -0.5 * gi;
+    }
+    return make_float2(static_cast<float>(fr), static_cast<float>(fi));
+  };
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
 
   // global -> register staging: 4 elements of op(A) and 4 of B per thread
@@ -217,6 +265,13 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
             im *= f;
           }
           const size_t o = static_cast<size_t>(i) * n + j;
+          if (oa.F) {  // the Ogita-Aishima correction instead of G (S Hermitian)
+            const double2 sij = oa.S[o];
+            oa.F[o] = oa_entry(i, j, re, im, sij);
+            if (mirror)
+              oa.F[static_cast<size_t>(j) * n + i] = oa_entry(j, i, re, -im, make_double2(sij.x, -sij.y));
+            continue;
+          }
           if (ADD) {
             const SA d = D[o];
             re += static_cast<double>(d.x);
@@ -230,7 +285,8 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
 
 template <typename T, bool ADD, typename SA, typename SB>
 int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_a, const ZOperand& B,
-                 const ZOperand& D, double* C, size_t sC, bool herm, const double* sc) {
+                 const ZOperand& D, double* C, size_t sC, bool herm, const double* sc,
+                 const OaEpilogue& oa = OaEpilogue{}) {
   if (nb == 0 || n <= 0) return RTB_OK;
   TileMap tm;
   tm.side = (n + kTile - 1) / kTile;
@@ -248,8 +304,10 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_
     const size_t wgs = groups * tm.tiles * kXcd;
     hipLaunchKernelGGL((zgemm3m_kernel<T, ADD, SA, SB>), dim3(static_cast<unsigned>(wgs)), dim3(256), 0, st, n, cnt,
                        tm, pA + off * A.stride, A.stride, conj_a ? 1 : 0, pB + off * B.stride, B.stride,
-                       pD ? pD + off * D.stride : nullptr, D.stride, reinterpret_cast<double2*>(C) + off * sC, sC,
-                       sc ? sc + off * n : nullptr);
+                       pD ? pD + off * D.stride : nullptr, D.stride,
+                       C ? reinterpret_cast<double2*>(C) + off * sC : nullptr, sC, sc ? sc + off * n : nullptr,
+                       OaEpilogue{oa.S ? oa.S + off * sC : nullptr, oa.gd ? oa.gd + off * n : nullptr,
+                                  oa.F ? oa.F + off * sC : nullptr, oa.gap_rel});
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -264,6 +322,19 @@ int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B,
   if (!A.f32 && B.f32) return zgemm_launch<double, false, double2, float2>(st, n, nb, A, conj_a, B, none, C, sC, herm, s);
   if (A.f32 && !B.f32) return zgemm_launch<double, false, float2, double2>(st, n, nb, A, conj_a, B, none, C, sC, herm, s);
   return zgemm_launch<double, false, float2, float2>(st, n, nb, A, conj_a, B, none, C, sC, herm, s);
+}
+
+int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double* S, const double* gd, float* F,
+                  double gap_rel) {
+  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "zgemm_gram_oa supports n <= 128");
+  const size_t n2 = static_cast<size_t>(n) * n;
+  OaEpilogue oa;
+  oa.S = reinterpret_cast<const double2*>(S);
+  oa.gd = gd;
+  oa.F = reinterpret_cast<float2*>(F);
+  oa.gap_rel = gap_rel;
+  return zgemm_launch<double, false, float2, float2>(st, n, nb, zop(Z, n2), true, zop(Z, n2), ZOperand{}, nullptr,
+                                                     n2, true, nullptr, oa);
 }
 
 int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC) {
