@@ -43,27 +43,41 @@ def device_sync():
 
 
 # Stages with a closed-form algorithmic flop count (the library reports the
-# count per launch group in rtb_stats.flop): the kernel each one times, the
-# arithmetic it runs on and that unit's published peak.
+# count per launch group in rtb_stats.flop): the kernels each one times, the
+# arithmetic they run on and that unit's published peak.
 ROOFLINE_STAGES = {
-    "gemm64": ("zgemm3m_kernel<double, false>", "FP64 matrix cores", PEAK_FP64),
-    "gemm32": ("zgemm3m_kernel<float, true>", "FP32 matrix cores", PEAK_FP32),
+    "gemm64": ("zgemm3m_kernel<double, ...>", "FP64 matrix cores", PEAK_FP64),
+    "gemm32": ("zgemm3m_kernel<float, ...>", "FP32 matrix cores", PEAK_FP32),
     "tridiag": ("eigh_tridiag_reg_kernel", "vector ALU", None),
     "backtransform": ("eigh_backtrans_reg_kernel", "vector ALU", None),
 }
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), as
+def traffic_from_profiles(stage):
+    """HBM bytes per launch of the kernels of `stage` measured with rocprofv3
+    --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), as
     committed under profiles/; None when no such measurement exists."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f).get(kernel)
+            rec = json.load(f).get("stage:" + stage)
     except (OSError, ValueError):
         return None
     return None if rec is None else rec.get("bytes_per_launch")
+
+
+def mfma_ceiling():
+    """Attainable FP64 and FP32 matrix-core rates of this device (TFLOP/s),
+    measured by rustb_amd/csrc/probe.hip: independent MFMA chains, no memory
+    traffic.  None when the probe library is absent or fails."""
+    try:
+        lib = ctypes.CDLL(os.path.join(ROOT, "rustb_amd", "librtb_probe.so"))
+    except OSError:
+        return None
+    f64, f32 = ctypes.c_double(), ctypes.c_double()
+    if lib.rtb_probe_mfma(ctypes.byref(f64), ctypes.byref(f32)) != 0:
+        return None
+    return {"fp64_mfma": round(f64.value, 2), "fp32_mfma": round(f32.value, 2)}
 
 
 def copy_probe(nbytes=1 << 30, reps=3):
@@ -174,6 +188,7 @@ def main():
     stage_ms = np.zeros(ns)
     stage_launch = np.zeros(ns)
     stage_flop = np.zeros(ns)
+    stage_mflop = np.zeros(ns)
     t0 = time.perf_counter()
     sigma = None
     for _ in range(args.steps):
@@ -182,6 +197,7 @@ def main():
         stage_ms += st["ms"]
         stage_launch += st["launches"]
         stage_flop += st["flop"]
+        stage_mflop += st["mflop"]
     device_sync()
     elapsed = time.perf_counter() - t0
     value = nk * args.steps / elapsed
@@ -203,7 +219,7 @@ def main():
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 4),
-        "traffic": traffic_from_profiles(kernel),
+        "traffic": traffic_from_profiles(dname),
         "kernel": kernel,
         "stage": dname,
         "arithmetic": unit_name,
@@ -212,6 +228,16 @@ def main():
         "algorithmic_flop_per_launch": flop_per_launch,
         "dominant_stage_by_time": names[int(np.argmax(stage_ms[:ns - 1]))],
     }
+    if stage_mflop[dom] > 0:
+        # what the matrix cores executed (3M products, padded tiles) against
+        # the rate this device attains on independent MFMA chains
+        ceil = mfma_ceiling()
+        executed = stage_mflop[dom] / stage_launch[dom] / (per_launch_ms * 1e-3) / 1e12
+        roofline["mfma_executed"] = round(executed, 3)
+        if ceil:
+            att = ceil["fp64_mfma" if dname == "gemm64" else "fp32_mfma"]
+            roofline["attainable_peak"] = att
+            roofline["frac_of_attainable"] = round(executed / att, 4)
     out = {
         "metric": METRIC,
         "value": round(value, 2),

@@ -169,7 +169,9 @@ const char* rtb_last_error(void);
  * ms[] in milliseconds; launches[] kernel launches per stage; flop[] the
  * algorithmic floating-point operations of those launches where the stage
  * has a closed form (complex product 8 n^3, Hermitian product 4 n^2 (n+1),
- * tridiagonalisation 16/3 n^3, back-transformation 8 n^3; 0 otherwise). */
+ * tridiagonalisation 16/3 n^3, back-transformation 8 n^3; 0 otherwise);
+ * mflop[] the matrix-core operations the product kernels executed (three
+ * real products per complex one, 64 x 64 tiles, the k loop padded to 16). */
 #define RTB_STAGE_FOURIER 0
 #define RTB_STAGE_TRIDIAG 1
 #define RTB_STAGE_TRIEIG 2
@@ -187,6 +189,7 @@ typedef struct rtb_stats {
   double flop[RTB_N_STAGES];
   size_t nk;
   int nsta;
+  double mflop[RTB_N_STAGES];
 } rtb_stats;
 int rtb_last_stats(rtb_stats* out);
 

@@ -57,7 +57,7 @@ class SynthSpec(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("ms", c_double * 10), ("launches", c_int * 10), ("flop", c_double * 10), ("nk", c_size_t),
-                ("nsta", c_int)]
+                ("nsta", c_int), ("mflop", c_double * 10)]
 
 
 _P = c_void_p
@@ -119,7 +119,8 @@ def ptr(a):
 def last_stats() -> dict:
     s = Stats()
     check(load().rtb_last_stats(ctypes.byref(s)))
-    return {"ms": list(s.ms), "launches": list(s.launches), "flop": list(s.flop), "nk": s.nk, "nsta": s.nsta}
+    return {"ms": list(s.ms), "launches": list(s.launches), "flop": list(s.flop), "nk": s.nk, "nsta": s.nsta,
+            "mflop": list(s.mflop)}
 
 
 STAGES = ["fourier", "tridiag", "trieig", "backtransform", "gemm64", "gemm32", "refine", "kubo", "reduce", "total"]

@@ -295,7 +295,7 @@ int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const d
   // C = U^H M U for the unit eigenvectors U = Z diag(zscale) (zscale null: Z is unit)
   RTB_TRY(zgemm_batched(st, n, K, M, n2, false, Z, n2, T1, n2));
   RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, C, n2, false, zscale));
-  RTB_TRY(tmr.end(2, 2 * flop_zgemm(n, K)));
+  RTB_TRY(tmr.end(2, 2 * flop_zgemm(n, K), 2 * mflop_zgemm(n, K, false)));
   RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
   RTB_TRY(kubo_omega(st, n, K, C, E, eta, omega));
   RTB_TRY(tmr.end(1));

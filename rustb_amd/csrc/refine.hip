@@ -187,11 +187,13 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, c
   RTB_TRY(zgemm(st, n, nk, zop(H, n2), false, zop(Z, n2), W1, n2, false, nullptr));
   RTB_TRY(zgemm(st, n, nk, zop(Z, n2), true, zop(W1, n2), W2, n2, true, nullptr));
   RTB_TRY(zgemm_gram_oa(st, n, nk, Z, W2, scale, F, gap_rel));
-  if (tmr) RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk)));
+  if (tmr)
+    RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk),
+                     mflop_zgemm(n, nk, false) + 2 * mflop_zgemm(n, nk, true)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM32));
   RTB_TRY(zgemm_add_f32(st, n, nk, zop(Z, n2), zop(F, n2), W2, n2));   // Z' = Z + Z (F' - I)
   RTB_TRY(zgemm_add_f32(st, n, nk, zop(W1, n2), zop(F, n2), W3, n2));  // HZ' = HZ + HZ (F' - I)
-  if (tmr) RTB_TRY(tmr->end(2, 2 * flop_zgemm(n, nk)));
+  if (tmr) RTB_TRY(tmr->end(2, 2 * flop_zgemm(n, nk), 2 * mflop_zgemm(n, nk, false)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
   const size_t tk = nk * n;
   hipLaunchKernelGGL(rayleigh_kernel, dim3(static_cast<unsigned>((tk + 255) / 256)), dim3(256), 0, st, n, nk,

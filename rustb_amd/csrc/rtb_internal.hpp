@@ -54,16 +54,17 @@ class Timer {
     }
   }
   int begin(int stage) {
-    Rec r{stage, nullptr, nullptr, 0, 0.0};
+    Rec r{stage, nullptr, nullptr, 0, 0.0, 0.0};
     RTB_HIP(hipEventCreate(&r.a));
     RTB_HIP(hipEventCreate(&r.b));
     RTB_HIP(hipEventRecord(r.a, st_));
     recs_.push_back(r);
     return RTB_OK;
   }
-  int end(int launches, double flop = 0.0) {
+  int end(int launches, double flop = 0.0, double mflop = 0.0) {
     recs_.back().launches = launches;
     recs_.back().flop = flop;
+    recs_.back().mflop = mflop;
     RTB_HIP(hipEventRecord(recs_.back().b, st_));
     return RTB_OK;
   }
@@ -77,6 +78,7 @@ class Timer {
       s.ms[r.stage] += ms;
       s.launches[r.stage] += r.launches;
       s.flop[r.stage] += r.flop;
+      s.mflop[r.stage] += r.mflop;
     }
     if (!recs_.empty()) {
       float ms = 0.f;
@@ -85,6 +87,7 @@ class Timer {
       for (int i = 0; i < RTB_STAGE_TOTAL; ++i) {
         s.launches[RTB_STAGE_TOTAL] += s.launches[i];
         s.flop[RTB_STAGE_TOTAL] += s.flop[i];
+        s.mflop[RTB_STAGE_TOTAL] += s.mflop[i];
       }
     }
     publish_stats(s);
@@ -96,7 +99,7 @@ class Timer {
     int stage;
     hipEvent_t a, b;
     int launches;
-    double flop;
+    double flop, mflop;
   };
   hipStream_t st_;
   std::vector<Rec> recs_;
@@ -107,6 +110,14 @@ inline double flop_zgemm(int n, size_t nb) { return 8.0 * n * n * static_cast<do
 inline double flop_zgemm_herm(int n, size_t nb) { return 4.0 * n * n * (n + 1.0) * nb; }
 inline double flop_tridiag(int n, size_t nb) { return 16.0 / 3.0 * n * n * static_cast<double>(n) * nb; }
 inline double flop_backtrans(int n, size_t nb) { return 8.0 * n * n * static_cast<double>(n) * nb; }
+// Matrix-core operations the product kernel executes for nb n x n complex
+// products: 64 x 64 output tiles (the lower triangle of them when herm), the
+// k loop padded to 16, three real products (3M) of 2 flops per multiply-add.
+inline double mflop_zgemm(int n, size_t nb, bool herm) {
+  const double side = (n + 63) / 64, kpad = ((n + 15) / 16) * 16.0;
+  const double tiles = herm ? side * (side + 1) / 2 : side * side;
+  return tiles * 64.0 * 64.0 * kpad * 6.0 * static_cast<double>(nb);
+}
 
 // Largest nsta the batched eigensolver handles (packed lower triangle of the
 // FP64 matrix in LDS: n(n+1)/2 * 16 B <= 160 KB).
