@@ -121,6 +121,9 @@ struct FinalArgs {
 constexpr int kFB = 2;
 constexpr int kTWF = 64;  // matrix elements per workgroup of the final pass
 
+// NG: partial sums staged (G0 .. G_{NG-1}); VEL: the velocities are formed
+// (d1 from G0, then G1, G2 for NG = 2, 3).
+template <int NG, bool VEL>
 __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, double2* __restrict__ H,
                                                         double2* __restrict__ V) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -132,7 +135,6 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
   const size_t pl = blockIdx.y, pg = p0 + pl;
   const size_t t0 = static_cast<size_t>(blockIdx.x) * kTWF;
   const int tw = static_cast<int>(n2 - t0 < static_cast<size_t>(kTWF) ? n2 - t0 : kTWF);
-  const int ng = a.np ? a.dim : 1;  // inputs needed
   for (int m = threadIdx.x; m < a.N1; m += 256) {
     double s, c;
     sincospi(2.0 * static_cast<double>(m) / static_cast<double>(a.N1), &s, &c);
@@ -143,8 +145,8 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
     const size_t src = (static_cast<size_t>(l) * a.P + pg) * n2 + t0 + t;
     const bool ok = t < tw;
     X0[q] = ok ? a.G0[src] : make_double2(0.0, 0.0);
-    if (ng > 1) X1[q] = ok ? a.G1[src] : make_double2(0.0, 0.0);
-    if (ng > 2) X2[q] = ok ? a.G2[src] : make_double2(0.0, 0.0);
+    if (NG > 1) X1[q] = ok ? a.G1[src] : make_double2(0.0, 0.0);
+    if (NG > 2) X2[q] = ok ? a.G2[src] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   const int t = threadIdx.x % kTWF;
@@ -171,27 +173,38 @@ __global__ void __launch_bounds__(256) dft_final_kernel(FinalArgs a, size_t p0, 
       step[u] = i1;
       hr[u] = hi[u] = d1r[u] = d1i[u] = g1r[u] = g1i[u] = g2r[u] = g2i[u] = 0.0;
     }
+    // unrolled so that the LDS reads of the next terms are issued while the
+    // current ones are multiplied
+#pragma unroll 4
     for (int l = 0; l < a.L1; ++l) {
       const double2 x = X0[l * kTWF + t];
-      const double2 y = ng > 1 ? X1[l * kTWF + t] : make_double2(0.0, 0.0);
-      const double2 z = ng > 2 ? X2[l * kTWF + t] : make_double2(0.0, 0.0);
+      const double2 y = NG > 1 ? X1[l * kTWF + t] : make_double2(0.0, 0.0);
+      const double2 z = NG > 2 ? X2[l * kTWF + t] : make_double2(0.0, 0.0);
       const double r1 = static_cast<double>(a.lo1 + l);
+      double2 w[kFB];
 #pragma unroll
       for (int u = 0; u < kFB; ++u) {
-        const double2 w = W[m[u]];
-        const double tr = w.x * x.x - w.y * x.y, ti = w.x * x.y + w.y * x.x;
-        hr[u] += tr;
-        hi[u] += ti;
-        if (a.np) {
-          d1r[u] += r1 * tr;
-          d1i[u] += r1 * ti;
-          g1r[u] += w.x * y.x - w.y * y.y;
-          g1i[u] += w.x * y.y + w.y * y.x;
-          g2r[u] += w.x * z.x - w.y * z.y;
-          g2i[u] += w.x * z.y + w.y * z.x;
-        }
+        w[u] = W[m[u]];
         m[u] += step[u];
         if (m[u] >= a.N1) m[u] -= a.N1;
+      }
+#pragma unroll
+      for (int u = 0; u < kFB; ++u) {
+        const double tr = w[u].x * x.x - w[u].y * x.y, ti = w[u].x * x.y + w[u].y * x.x;
+        hr[u] += tr;
+        hi[u] += ti;
+        if (VEL) {
+          d1r[u] += r1 * tr;
+          d1i[u] += r1 * ti;
+        }
+        if (NG > 1) {
+          g1r[u] += w[u].x * y.x - w[u].y * y.y;
+          g1i[u] += w[u].x * y.y + w[u].y * y.x;
+        }
+        if (NG > 2) {
+          g2r[u] += w[u].x * z.x - w[u].y * z.y;
+          g2i[u] += w[u].x * z.y + w[u].y * z.x;
+        }
       }
     }
 #pragma unroll
@@ -252,7 +265,13 @@ int mesh_set_smem_limits() {
   const int lim = 160 * 1024;
   RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&dft_axis_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&dft_final_kernel),
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&dft_final_kernel<1, false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&dft_final_kernel<1, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&dft_final_kernel<2, true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&dft_final_kernel<3, true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, lim));
   return RTB_OK;
 }
@@ -367,9 +386,18 @@ int mesh_final(hipStream_t st, const rtb_model* m, const MeshPlan& p, const doub
   const size_t vk = f.combine ? 1 : 2;  // V matrices per k
   for (size_t q0 = 0; q0 < np_pairs; q0 += 65535) {
     const size_t qc = np_pairs - q0 < 65535 ? np_pairs - q0 : 65535;
-    hipLaunchKernelGGL(dft_final_kernel, dim3(static_cast<unsigned>(tiles), static_cast<unsigned>(qc)), dim3(256),
-                       smem, st, f, p0 + q0, reinterpret_cast<double2*>(H) + q0 * p.N[0] * n2,
-                       V ? reinterpret_cast<double2*>(V) + q0 * p.N[0] * vk * n2 : nullptr);
+    const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(qc));
+    double2* Hq = reinterpret_cast<double2*>(H) + q0 * p.N[0] * n2;
+    double2* Vq = V ? reinterpret_cast<double2*>(V) + q0 * p.N[0] * vk * n2 : nullptr;
+    if (!f.np) {
+      hipLaunchKernelGGL((dft_final_kernel<1, false>), grid, dim3(256), smem, st, f, p0 + q0, Hq, Vq);
+    } else if (f.dim == 1) {
+      hipLaunchKernelGGL((dft_final_kernel<1, true>), grid, dim3(256), smem, st, f, p0 + q0, Hq, Vq);
+    } else if (f.dim == 2) {
+      hipLaunchKernelGGL((dft_final_kernel<2, true>), grid, dim3(256), smem, st, f, p0 + q0, Hq, Vq);
+    } else {
+      hipLaunchKernelGGL((dft_final_kernel<3, true>), grid, dim3(256), smem, st, f, p0 + q0, Hq, Vq);
+    }
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
