@@ -91,6 +91,11 @@ int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B,
 // exact diagonal gd ([nb][n]) of G (n <= 128).
 int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double* S, const double* gd, float* F,
                   double gap_rel);
+// The same product (C = A + A B) consumed by its epilogue: not stored; the
+// Rayleigh sums of the columns Zp ([nb][n][n]) against it, Re(zp_j^H C_j)
+// and zp_j^H zp_j, are written per 64-row tile to part ([nb][ceil(n/64)][2][n]).
+int zgemm_add_f32_rayleigh(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, const double* Zp,
+                           double* part);
 // C = A + A B with the product on the FP32 matrix cores, for a correction B
 // stored in FP32 (A in either storage; FP64 sums).  C must not alias A or B.
 int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC);

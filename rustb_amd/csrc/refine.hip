@@ -19,48 +19,48 @@ namespace rtb {
 namespace {
 
 
-// Rayleigh quotient E_j = Re(z_j^H (H z_j)) / (z_j^H z_j) and the column
-// scale 1 / ||z_j||; the columns themselves are left unnormalised (their
-// consumers apply the scale).
-__global__ void rayleigh_kernel(int n, size_t nk, const double2* __restrict__ Z, const double2* __restrict__ HZ,
-                                double* __restrict__ E, double* __restrict__ scale) {
-  const size_t g = blockIdx.x * 256ull + threadIdx.x;
-  if (g >= nk * n) return;
-  const size_t k = g / n;
-  const int j = static_cast<int>(g % n);
-  const double2* Zk = Z + k * n * n;
-  const double2* Hk = HZ + k * n * n;
-  double num = 0.0, den = 0.0;
-  for (int r = 0; r < n; ++r) {
-    const double2 z = Zk[static_cast<size_t>(r) * n + j];
-    const double2 h = Hk[static_cast<size_t>(r) * n + j];
-    num += z.x * h.x + z.y * h.y;  // Re(conj(z) h)
-    den += z.x * z.x + z.y * z.y;
-  }
-  E[g] = num / den;
-  scale[g] = 1.0 / sqrt(den);
-}
-
 __device__ inline double wave_sum(double x) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
   return x;
 }
 
-// One wave per k-point: the lanes test every eigenvalue gap in parallel; a
+// One wave per k-point: first the Rayleigh quotients and column scales from
+// the per-tile sums; then the lanes test every eigenvalue gap in parallel; a
 // k-point with a cluster forms the cluster's Gram and projection sums with
 // the lanes over rows, lane 0 solves the c x c problem (LDS), and the lanes
 // rotate the cluster's columns row by row.  Column j of Z (and of HZ = H Z)
 // times scale[j] is the unit vector; rotated columns are written unit.
 __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restrict__ Z,
-                                                        const double2* __restrict__ HZ, double* __restrict__ E,
-                                                        double* __restrict__ scale, double gap_rel) {
+                                                        const double2* __restrict__ HZ,
+                                                        const float2* __restrict__ F,
+                                                        const double* __restrict__ part, int tiles,
+                                                        double* __restrict__ E, double* __restrict__ scale,
+                                                        double gap_rel) {
   __shared__ C2 lds[7][kMaxCluster][kMaxCluster];
+  __shared__ double2 hzc[kMaxCluster][kMaxEighN];  // refined H z' of the cluster's columns
   __shared__ double ev[kMaxCluster];
   __shared__ int solved;
   const size_t k = blockIdx.x;
   const int lane = threadIdx.x;
   double* Ek = E + k * n;
+  double* sk = scale + k * n;
+  {
+    // Rayleigh quotients and column scales from the per-tile sums of the
+    // correction product's epilogue
+    const double* pk = part + k * tiles * 2 * static_cast<size_t>(n);
+    for (int j = lane; j < n; j += 64) {
+      double num = 0.0, den = 0.0;
+      for (int t = 0; t < tiles; ++t) {
+        num += pk[(2 * t) * static_cast<size_t>(n) + j];
+        den += pk[(2 * t + 1) * static_cast<size_t>(n) + j];
+      }
+      Ek[j] = num / den;
+      sk[j] = 1.0 / sqrt(den);
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
   double m = 0.0;
   for (int j = lane; j < n; j += 64) m = fmax(m, fabs(Ek[j]));
 #pragma unroll
@@ -71,19 +71,33 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
   if (!__any(close)) return;
   double2* Zk = Z + k * n * n;
   const double2* Hk = HZ + k * n * n;
-  double* sk = scale + k * n;
+  const float2* Fk = F + k * n * n;
   int j0 = 0;
   while (j0 < n) {  // the same cluster sequence in every lane
     const int j1 = cluster_end(n, Ek, j0, thr);
     const int c = j1 - j0 + 1;
-    if (c >= 2) {
+    if (c >= 2 && c <= kMaxCluster) {
+      // H z'_b = HZ (e_b + (F' - I)_b) for the cluster's columns, in FP64
+      for (int b = 0; b < c; ++b)
+        for (int r = lane; r < n; r += 64) {
+          const double2* hr = Hk + static_cast<size_t>(r) * n;
+          double2 acc = hr[j0 + b];
+          for (int q = 0; q < n; ++q) {
+            const double2 h = hr[q];
+            const float2 f = Fk[static_cast<size_t>(q) * n + j0 + b];
+            acc.x += h.x * f.x - h.y * f.y;
+            acc.y += h.x * f.y + h.y * f.x;
+          }
+          hzc[b][r] = acc;
+        }
+      __syncthreads();
       for (int p = 0; p < c * c; ++p) {
         const int a = p / c, b = p % c;
         double gr = 0.0, gi = 0.0, mr = 0.0, mi = 0.0;
         for (int r = lane; r < n; r += 64) {
           const double2 za = Zk[static_cast<size_t>(r) * n + j0 + a];
           const double2 zb = Zk[static_cast<size_t>(r) * n + j0 + b];
-          const double2 hb = Hk[static_cast<size_t>(r) * n + j0 + b];
+          const double2 hb = hzc[b][r];
           gr += za.x * zb.x + za.y * zb.y;  // conj(za) zb
           gi += za.x * zb.y - za.y * zb.x;
           mr += za.x * hb.x + za.y * hb.y;  // conj(za) hb
@@ -170,11 +184,14 @@ int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const d
 //   Z' = Z + Z (F' - I), HZ' = HZ + HZ (F' - I)  (the FP32 vector error enters
 //   squared).  F' - I is of the size of that error, so its two products run
 //   on the FP32 matrix cores: their rounding (~1e-7 relative to a correction
-//   already ~1e-6 or smaller) stays below the FP64 level of the sums.
-// Buffers (each [nk][n][n] complex): H (consumed; F' - I in FP32), Z (in, FP32),
-// W1, W3 scratch; the refined eigenvectors are left in W2 as columns that
-// become unit vectors when multiplied by scale[k][j] ([nk][n]; on entry the
-// squared column norms of Z).
+//   already ~1e-6 or smaller) stays below the FP64 level of the sums.  HZ'
+//   is never stored: its product's epilogue forms the Rayleigh sums
+//   Re(z'^H HZ') and z'^H z' per column, and the cluster kernel rebuilds
+//   H z' for the few columns inside clusters.
+// Buffers (each [nk][n][n] complex): H (consumed; F' - I in FP32), Z (in,
+// FP32), W1 (HZ), W3 (the Rayleigh sums); the refined eigenvectors are left in
+// W2 as columns that become unit vectors when multiplied by scale[k][j]
+// ([nk][n]; on entry the squared column norms of Z).
 int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, const float* Z, double* W1,
                       double* W2, double* W3, double* E, double* scale, double gap_rel) {
   if (nk == 0) return RTB_OK;
@@ -190,19 +207,20 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, c
   if (tmr)
     RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk),
                      mflop_zgemm(n, nk, false) + 2 * mflop_zgemm(n, nk, true)));
+  // Z' = Z + Z (F' - I); HZ' = HZ + HZ (F' - I) is consumed by its epilogue,
+  // which leaves the Rayleigh sums of the Z' columns per 64-row tile in W3
+  const int tiles = (n + 63) / 64;
+  double* part = W3;
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM32));
-  RTB_TRY(zgemm_add_f32(st, n, nk, zop(Z, n2), zop(F, n2), W2, n2));   // Z' = Z + Z (F' - I)
-  RTB_TRY(zgemm_add_f32(st, n, nk, zop(W1, n2), zop(F, n2), W3, n2));  // HZ' = HZ + HZ (F' - I)
+  RTB_TRY(zgemm_add_f32(st, n, nk, zop(Z, n2), zop(F, n2), W2, n2));
+  RTB_TRY(zgemm_add_f32_rayleigh(st, n, nk, zop(W1, n2), zop(F, n2), W2, part));
   if (tmr) RTB_TRY(tmr->end(2, 2 * flop_zgemm(n, nk), 2 * mflop_zgemm(n, nk, false)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
-  const size_t tk = nk * n;
-  hipLaunchKernelGGL(rayleigh_kernel, dim3(static_cast<unsigned>((tk + 255) / 256)), dim3(256), 0, st, n, nk,
-                     reinterpret_cast<const double2*>(W2), reinterpret_cast<const double2*>(W3), E, scale);
-  RTB_HIP(hipGetLastError());
   hipLaunchKernelGGL(cluster_rr_kernel, dim3(static_cast<unsigned>(nk)), dim3(64), 0, st, n,
-                     reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W3), E, scale, gap_rel);
+                     reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W1),
+                     reinterpret_cast<const float2*>(F), part, tiles, E, scale, gap_rel);
   RTB_HIP(hipGetLastError());
-  if (tmr) RTB_TRY(tmr->end(2));
+  if (tmr) RTB_TRY(tmr->end(1));
   return RTB_OK;
 }
 

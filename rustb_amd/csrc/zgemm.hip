@@ -98,16 +98,26 @@ struct OaEpilogue {
   double gap_rel = 0.0;
 };
 
+// Optional epilogue of the correction product HZ' = HZ + HZ (F' - I): the
+// Rayleigh sums of the refined columns, num_j = Re(z'_j^H HZ'_j) and
+// den_j = z'_j^H z'_j, over the tile's rows, written per row tile to
+// part[mat][tile row][2][n] (Z' read from memory); C may then be null.
+struct RqEpilogue {
+  const double2* Zp = nullptr;  // [nmat][n][n], stride sC
+  double* part = nullptr;       // [nmat][tiles per side][2][n]
+};
+
 // SA, SB: storage of A (and D) and B, double2 or float2.
 template <typename T, bool ADD, typename SA, typename SB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4, sizeof(T) == 8 ? 2 : 4)))
 zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t sA, int conj_a,
                const SB* __restrict__ B, size_t sB, const SA* D, size_t sD, double2* __restrict__ C,
-               size_t sC, const double* __restrict__ sc, OaEpilogue oa) {
+               size_t sC, const double* __restrict__ sc, OaEpilogue oa, RqEpilogue rq) {
   typedef typename Mfma<T>::v4 v4;
   __shared__ double s_lt[kMaxEighN];  // Rayleigh quotients (OA epilogue)
   __shared__ double s_red[4];
+  __shared__ double s_rq[2][2][kTile];  // Rayleigh sums by wave row (RQ epilogue)
   __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
   __shared__ T sB3[3][kKc][kLd];  // B tile, k-major
   // XCD-aware decomposition of the workgroup index
@@ -119,7 +129,7 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
   tile_coords(static_cast<int>(loc % tm.tiles), tm.side, tm.herm, ti, tj);
   A += mat * sA;
   B += mat * sB;
-  C += mat * sC;
+  if (C) C += mat * sC;  // null when an epilogue consumes the product
   if (ADD) D += mat * sD;
   if (sc) sc += mat * n;
   const int row0 = ti * kTile, col0 = tj * kTile;
@@ -248,6 +258,8 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
     }
   }
   const bool mirror = tm.herm && ti != tj;
+  if (rq.part) rq.Zp += mat * sC;
+  double rq_num[2] = {0.0, 0.0}, rq_den[2] = {0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -277,16 +289,43 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
             re += static_cast<double>(d.x);
             im += static_cast<double>(d.y);
           }
-          C[o] = make_double2(re, im);
-          if (mirror) C[static_cast<size_t>(j) * n + i] = make_double2(re, -im);
+          if (C) {
+            C[o] = make_double2(re, im);
+            if (mirror) C[static_cast<size_t>(j) * n + i] = make_double2(re, -im);
+          }
+          if (rq.part) {
+            const double2 z = rq.Zp[o];
+            rq_num[b] += z.x * re + z.y * im;  // Re(conj(z) hz)
+            rq_den[b] += z.x * z.x + z.y * z.y;
+          }
         }
       }
+  if (rq.part) {  // workgroup-uniform: column sums over the tile's 64 rows
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      // the four lane groups of a wave hold the same 16 columns
+      rq_num[b] += __shfl_xor(rq_num[b], 16, 64);
+      rq_num[b] += __shfl_xor(rq_num[b], 32, 64);
+      rq_den[b] += __shfl_xor(rq_den[b], 16, 64);
+      rq_den[b] += __shfl_xor(rq_den[b], 32, 64);
+      if (fk == 0) {
+        s_rq[wave >> 1][0][wc + 16 * b + fr] = rq_num[b];
+        s_rq[wave >> 1][1][wc + 16 * b + fr] = rq_den[b];
+      }
+    }
+    __syncthreads();
+    if (tid < kTile && col0 + tid < n) {
+      double* pt = rq.part + (mat * tm.side + ti) * 2 * static_cast<size_t>(n);
+      pt[col0 + tid] = s_rq[0][0][tid] + s_rq[1][0][tid];
+      pt[n + col0 + tid] = s_rq[0][1][tid] + s_rq[1][1][tid];
+    }
+  }
 }
 
 template <typename T, bool ADD, typename SA, typename SB>
 int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_a, const ZOperand& B,
                  const ZOperand& D, double* C, size_t sC, bool herm, const double* sc,
-                 const OaEpilogue& oa = OaEpilogue{}) {
+                 const OaEpilogue& oa = OaEpilogue{}, const RqEpilogue& rq = RqEpilogue{}) {
   if (nb == 0 || n <= 0) return RTB_OK;
   TileMap tm;
   tm.side = (n + kTile - 1) / kTile;
@@ -307,7 +346,9 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_
                        pD ? pD + off * D.stride : nullptr, D.stride,
                        C ? reinterpret_cast<double2*>(C) + off * sC : nullptr, sC, sc ? sc + off * n : nullptr,
                        OaEpilogue{oa.S ? oa.S + off * sC : nullptr, oa.gd ? oa.gd + off * n : nullptr,
-                                  oa.F ? oa.F + off * sC : nullptr, oa.gap_rel});
+                                  oa.F ? oa.F + off * sC : nullptr, oa.gap_rel},
+                       RqEpilogue{rq.Zp ? rq.Zp + off * sC : nullptr,
+                                  rq.part ? rq.part + off * tm.side * 2 * static_cast<size_t>(n) : nullptr});
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -335,6 +376,20 @@ int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double
   oa.gap_rel = gap_rel;
   return zgemm_launch<double, false, float2, float2>(st, n, nb, zop(Z, n2), true, zop(Z, n2), ZOperand{}, nullptr,
                                                      n2, true, nullptr, oa);
+}
+
+int zgemm_add_f32_rayleigh(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, const double* Zp,
+                           double* part) {
+  if (!B.f32) return fail(RTB_ERR_UNSUPPORTED, "zgemm_add_f32: the correction operand is stored in FP32");
+  const size_t n2 = static_cast<size_t>(n) * n;
+  RqEpilogue rq;
+  rq.Zp = reinterpret_cast<const double2*>(Zp);
+  rq.part = part;
+  if (A.f32)
+    return zgemm_launch<float, true, float2, float2>(st, n, nb, A, false, B, A, nullptr, n2, false, nullptr,
+                                                     OaEpilogue{}, rq);
+  return zgemm_launch<float, true, double2, float2>(st, n, nb, A, false, B, A, nullptr, n2, false, nullptr,
+                                                    OaEpilogue{}, rq);
 }
 
 int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC) {
