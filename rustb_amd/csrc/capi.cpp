@@ -292,12 +292,14 @@ int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const d
     M = J;
   }
   RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
-  // C = U^H M U for the unit eigenvectors U = Z diag(zscale) (zscale null: Z is unit)
+  // C = U^H M U for the unit eigenvectors U = Z diag(zscale) (zscale null: Z
+  // is unit), consumed by the product's epilogue: per-tile sums of the band-
+  // sum weights into C's buffer, then Omega from them
   RTB_TRY(zgemm_batched(st, n, K, M, n2, false, Z, n2, T1, n2));
-  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, C, n2, false, zscale));
+  RTB_TRY(zgemm_kubo(st, n, K, Z, T1, zscale, E, eta, C));
   RTB_TRY(tmr.end(2, 2 * flop_zgemm(n, K), 2 * mflop_zgemm(n, K, false)));
   RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
-  RTB_TRY(kubo_omega(st, n, K, C, E, eta, omega));
+  RTB_TRY(kubo_omega_from_parts(st, n, K, C, omega));
   RTB_TRY(tmr.end(1));
   return RTB_OK;
 }

@@ -91,6 +91,12 @@ int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B,
 // exact diagonal gd ([nb][n]) of G (n <= 128).
 int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double* S, const double* gd, float* F,
                   double gap_rel);
+// The Kubo product C = diag(s) U^H T1 diag(s) (U, T1: [nb][n][n] FP64; s
+// may be null) consumed by its epilogue: with Q_ij = |C_ij|^2 /
+// ((E_i - E_j)^2 + eta^2), i != j, part ([nb][2][ceil(n/64)][n]) receives the
+// per-tile column sums ([0][tile row]) and row sums ([1][tile column]) of Q.
+int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* T1, const double* s,
+               const double* E, double eta, double* part);
 // The same product (C = A + A B) consumed by its epilogue: not stored; the
 // Rayleigh sums of the columns Zp ([nb][n][n]) against it, Re(zp_j^H C_j)
 // and zp_j^H zp_j, are written per 64-row tile to part ([nb][ceil(n/64)][2][n]).
@@ -115,8 +121,10 @@ int zgemm_batched_scaled(hipStream_t st, int n, size_t nb, const double* A, size
 int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride,
                  const double* w, double* J);
 // Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2) for
-// Hermitian A, B, from C = A + i B: Im(A_nm B_mn) = (|C_nm|^2 - |C_mn|^2) / 4.
-int kubo_omega(hipStream_t st, int n, size_t nk, const double* C, const double* E, double eta, double* omega);
+// Hermitian A, B, from C = A + i B (Im(A_nm B_mn) = (|C_nm|^2 - |C_mn|^2) / 4):
+// (sum over tiles of the column sums - sum of the row sums) / 2 from the
+// partial sums zgemm_kubo writes (tiles = ceil(n/64)).
+int kubo_omega_from_parts(hipStream_t st, int n, size_t nk, const double* part, double* omega);
 // out[k] = sum_n omega[k][n] f(E[k][n], mu, T)
 int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const double* E, double mu,
                  double T, double* out);

@@ -8,7 +8,7 @@ namespace rtb {
 namespace {
 
 // helpers.rs:11-39 spin matrix S = sigma/2 (x) I_norb; J = (S v + v S) / 2,
-// plus i w when w is given (the combined operator of kubo_omega)
+// plus i w when w is given (the combined operator of the Kubo product)
 __global__ void spin_current_kernel(int norb, size_t nb, int dir, const double2* __restrict__ v,
                                     size_t v_stride, const double2* __restrict__ w, double2* __restrict__ J) {
   const int n = 2 * norb;
@@ -49,50 +49,24 @@ __global__ void spin_current_kernel(int norb, size_t nb, int dir, const double2*
   J[g] = out;
 }
 
-// Omega_b = sum_{m != b} (|C_mb|^2 - |C_bm|^2) / 2 / ((E_b - E_m)^2 + eta^2) for
-// C = A + i B (A, B Hermitian): -2 Im(A_bm B_mb) = (|C_mb|^2 - |C_bm|^2) / 2.
-// With Q_rc = |C_rc|^2 / ((E_r - E_c)^2 + eta^2) (Q_bb = 0), Omega_b is half
-// the column sum minus the row sum of Q.  One workgroup per k reads C once,
-// row-major and coalesced, in tiles of kOmRows rows staged as Q in LDS;
-// threads < n keep the column sums, all threads share the row sums.
-constexpr int kOmRows = 32;
-constexpr int kOmParts = 256 / kOmRows;  // threads per row in the row sums
-
-__global__ void __launch_bounds__(256) kubo_omega_kernel(int n, const double2* __restrict__ C,
-                                                         const double* __restrict__ E, double eta2,
-                                                         double* __restrict__ omega) {
-  __shared__ double Q[kOmRows][kMaxEighN + 1];
-  __shared__ double Es[kMaxEighN];
-  __shared__ double rs[kMaxEighN];
-  const size_t k = blockIdx.x;
-  const int t = threadIdx.x;
-  const double2* Ck = C + k * n * n;
-  for (int i = t; i < n; i += 256) Es[i] = E[k * n + i];
-  double col = 0.0;
-  const int plen = (n + kOmParts - 1) / kOmParts;
-  for (int r0 = 0; r0 < n; r0 += kOmRows) {
-    const int rows = min(kOmRows, n - r0);
-    __syncthreads();  // Es ready / previous tile consumed
-    for (int q = t; q < rows * n; q += 256) {
-      const int r = q / n, c = q % n;
-      const double2 x = Ck[static_cast<size_t>(r0) * n + q];
-      const double d = Es[r0 + r] - Es[c];
-      Q[r][c] = (r0 + r == c) ? 0.0 : (x.x * x.x + x.y * x.y) / (d * d + eta2);
-    }
-    __syncthreads();
-    if (t < n)
-      for (int r = 0; r < rows; ++r) col += Q[r][t];
-    // row sums: kOmParts consecutive lanes per row, then a lane reduction
-    const int r = t / kOmParts, part = t % kOmParts;
-    double s = 0.0;
-    if (r < rows)
-      for (int c = part * plen; c < min(n, (part + 1) * plen); ++c) s += Q[r][c];
-#pragma unroll
-    for (int off = 1; off < kOmParts; off <<= 1) s += __shfl_xor(s, off, 64);
-    if (r < rows && part == 0) rs[r0 + r] = s;
+// Omega_j = sum_{m != j} (|C_mj|^2 - |C_jm|^2) / 2 / ((E_j - E_m)^2 + eta^2) for
+// C = A + i B (A, B Hermitian): -2 Im(A_jm B_mj) = (|C_mj|^2 - |C_jm|^2) / 2,
+// i.e. half the column sum minus the row sum of Q_rc = |C_rc|^2 /
+// ((E_r - E_c)^2 + eta^2); the product kernel leaves those sums per 64 x 64
+// tile (zgemm_kubo), summed here over the tiles.
+__global__ void kubo_parts_kernel(int n, size_t nk, int tiles, const double* __restrict__ part,
+                                  double* __restrict__ omega) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= nk * n) return;
+  const size_t k = g / n;
+  const int j = static_cast<int>(g % n);
+  const double* pk = part + k * 2 * tiles * static_cast<size_t>(n);
+  double col = 0.0, row = 0.0;
+  for (int t = 0; t < tiles; ++t) {
+    col += pk[static_cast<size_t>(t) * n + j];
+    row += pk[static_cast<size_t>(tiles + t) * n + j];
   }
-  __syncthreads();
-  if (t < n) omega[k * n + t] = 0.5 * (col - rs[t]);
+  omega[g] = 0.5 * (col - row);
 }
 
 __device__ inline double fermi(double e, double mu, double width) {
@@ -186,12 +160,11 @@ int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double
   return RTB_OK;
 }
 
-int kubo_omega(hipStream_t st, int n, size_t nk, const double* C, const double* E, double eta, double* omega) {
+int kubo_omega_from_parts(hipStream_t st, int n, size_t nk, const double* part, double* omega) {
   const size_t total = nk * n;
   if (total == 0) return RTB_OK;
-  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "kubo_omega supports nsta <= 128");
-  hipLaunchKernelGGL(kubo_omega_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
-                     reinterpret_cast<const double2*>(C), E, eta * eta, omega);
+  hipLaunchKernelGGL(kubo_parts_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, n, nk,
+                     (n + 63) / 64, part, omega);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -24,6 +24,7 @@
 // accumulator register r of lane l holds D[4r + l / 16][l % 16] (FP64) or
 // D[4(l / 16) + r][l % 16] (FP32).
 #include "eigh.hpp"
+#include "gpu_team.hpp"
 
 namespace rtb {
 
@@ -107,17 +108,29 @@ struct RqEpilogue {
   double* part = nullptr;       // [nmat][tiles per side][2][n]
 };
 
+// Optional epilogue of the Kubo product C = U^H (v_a + i v_b) U: C is not
+// stored; with Q_ij = |C_ij|^2 / ((E_i - E_j)^2 + eta^2) (Q_ii = 0), the tile
+// writes its column sums of Q (over its rows) to part[mat][0][tile row][n]
+// and its row sums (over its columns) to part[mat][1][tile column][n];
+// Omega_j = (sum of column sums - sum of row sums) / 2.
+struct KuboEpilogue {
+  const double* E = nullptr;  // [nmat][n]
+  double eta2 = 0.0;
+  double* part = nullptr;     // [nmat][2][tiles per side][n]
+};
+
 // SA, SB: storage of A (and D) and B, double2 or float2.
 template <typename T, bool ADD, typename SA, typename SB>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4, sizeof(T) == 8 ? 2 : 4)))
 zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t sA, int conj_a,
                const SB* __restrict__ B, size_t sB, const SA* D, size_t sD, double2* __restrict__ C,
-               size_t sC, const double* __restrict__ sc, OaEpilogue oa, RqEpilogue rq) {
+               size_t sC, const double* __restrict__ sc, OaEpilogue oa, RqEpilogue rq, KuboEpilogue kb) {
   typedef typename Mfma<T>::v4 v4;
   __shared__ double s_lt[kMaxEighN];  // Rayleigh quotients (OA epilogue)
   __shared__ double s_red[4];
-  __shared__ double s_rq[2][2][kTile];  // Rayleigh sums by wave row (RQ epilogue)
+  __shared__ double s_rq[2][2][kTile];  // Rayleigh sums / Kubo column sums by wave row
+  __shared__ double s_kr[2][kTile];     // Kubo row sums by wave column
   __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
   __shared__ T sB3[3][kKc][kLd];  // B tile, k-major
   // XCD-aware decomposition of the workgroup index
@@ -260,6 +273,23 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
   const bool mirror = tm.herm && ti != tj;
   if (rq.part) rq.Zp += mat * sC;
   double rq_num[2] = {0.0, 0.0}, rq_den[2] = {0.0, 0.0};
+  double kb_row[2][4] = {};  // Kubo: per (a, r) row of this lane
+  double kb_er[2][4] = {}, kb_ec[2] = {};  // eigenvalues of this lane's rows and columns
+  if (kb.part) {
+    kb.E += mat * n;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + wr + 16 * a + Mfma<T>::row(r, fk);
+        kb_er[a][r] = i < n ? kb.E[i] : 0.0;
+      }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = col0 + wc + 16 * b + fr;
+      kb_ec[b] = j < n ? kb.E[j] : 0.0;
+    }
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -277,6 +307,20 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
             im *= f;
           }
           const size_t o = static_cast<size_t>(i) * n + j;
+          if (kb.part) {  // the Kubo band-sum weights instead of C
+            if (i != j) {
+              // 1 / (d^2 + eta^2): hardware reciprocal and two Newton steps
+              const double d = kb_er[a][r] - kb_ec[b];
+              const double den = fma(d, d, kb.eta2);
+              double rc = __builtin_amdgcn_rcp(den);
+              rc = fma(rc, fma(-den, rc, 1.0), rc);
+              rc = fma(rc, fma(-den, rc, 1.0), rc);
+              const double q = (re * re + im * im) * rc;
+              rq_num[b] += q;  // column j
+              kb_row[a][r] += q;
+            }
+            continue;
+          }
           if (oa.F) {  // the Ogita-Aishima correction instead of G (S Hermitian)
             const double2 sij = oa.S[o];
             oa.F[o] = oa_entry(i, j, re, im, sij);
@@ -300,6 +344,28 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
           }
         }
       }
+  if (kb.part) {  // workgroup-uniform: column and row sums of Q over the tile
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      rq_num[b] += __shfl_xor(rq_num[b], 16, 64);
+      rq_num[b] += __shfl_xor(rq_num[b], 32, 64);
+      if (fk == 0) s_rq[wave >> 1][0][wc + 16 * b + fr] = rq_num[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // the 16 lanes of a DPP row hold the 16 columns of one row
+        const double x = row_sum16(kb_row[a][r]);
+        if (fr == 0) s_kr[wave & 1][wr + 16 * a + Mfma<T>::row(r, fk)] = x;
+      }
+    __syncthreads();
+    if (tid < kTile) {
+      const size_t base = mat * 2 * static_cast<size_t>(tm.side);
+      if (col0 + tid < n) kb.part[(base + ti) * n + col0 + tid] = s_rq[0][0][tid] + s_rq[1][0][tid];
+      if (row0 + tid < n) kb.part[(base + tm.side + tj) * n + row0 + tid] = s_kr[0][tid] + s_kr[1][tid];
+    }
+  }
   if (rq.part) {  // workgroup-uniform: column sums over the tile's 64 rows
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
@@ -325,7 +391,8 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
 template <typename T, bool ADD, typename SA, typename SB>
 int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_a, const ZOperand& B,
                  const ZOperand& D, double* C, size_t sC, bool herm, const double* sc,
-                 const OaEpilogue& oa = OaEpilogue{}, const RqEpilogue& rq = RqEpilogue{}) {
+                 const OaEpilogue& oa = OaEpilogue{}, const RqEpilogue& rq = RqEpilogue{},
+                 const KuboEpilogue& kb = KuboEpilogue{}) {
   if (nb == 0 || n <= 0) return RTB_OK;
   TileMap tm;
   tm.side = (n + kTile - 1) / kTile;
@@ -348,7 +415,9 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_
                        OaEpilogue{oa.S ? oa.S + off * sC : nullptr, oa.gd ? oa.gd + off * n : nullptr,
                                   oa.F ? oa.F + off * sC : nullptr, oa.gap_rel},
                        RqEpilogue{rq.Zp ? rq.Zp + off * sC : nullptr,
-                                  rq.part ? rq.part + off * tm.side * 2 * static_cast<size_t>(n) : nullptr});
+                                  rq.part ? rq.part + off * tm.side * 2 * static_cast<size_t>(n) : nullptr},
+                       KuboEpilogue{kb.E ? kb.E + off * n : nullptr, kb.eta2,
+                                    kb.part ? kb.part + off * tm.side * 2 * static_cast<size_t>(n) : nullptr});
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
@@ -376,6 +445,17 @@ int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double
   oa.gap_rel = gap_rel;
   return zgemm_launch<double, false, float2, float2>(st, n, nb, zop(Z, n2), true, zop(Z, n2), ZOperand{}, nullptr,
                                                      n2, true, nullptr, oa);
+}
+
+int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* T1, const double* s,
+               const double* E, double eta, double* part) {
+  const size_t n2 = static_cast<size_t>(n) * n;
+  KuboEpilogue kb;
+  kb.E = E;
+  kb.eta2 = eta * eta;
+  kb.part = part;
+  return zgemm_launch<double, false, double2, double2>(st, n, nb, zop(U, n2), true, zop(T1, n2), ZOperand{},
+                                                       nullptr, n2, false, s, OaEpilogue{}, RqEpilogue{}, kb);
 }
 
 int zgemm_add_f32_rayleigh(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, const double* Zp,
