@@ -97,6 +97,11 @@ int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double
 // per-tile column sums ([0][tile row]) and row sums ([1][tile column]) of Q.
 int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* T1, const double* s,
                const double* E, double eta, double* part);
+// Both refinement corrections in one pass: Zp = Z + Z F (FP64 out) and the
+// Rayleigh sums of the Zp columns against HZ + HZ F (not stored), per 64-row
+// tile into part ([nb][ceil(n/64)][2][n]); products on the FP32 matrix cores.
+int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, const double* HZ, const float* F,
+                          double* Zp, double* part);
 // The same product (C = A + A B) consumed by its epilogue: not stored; the
 // Rayleigh sums of the columns Zp ([nb][n][n]) against it, Re(zp_j^H C_j)
 // and zp_j^H zp_j, are written per 64-row tile to part ([nb][ceil(n/64)][2][n]).

@@ -207,14 +207,14 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, c
   if (tmr)
     RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk),
                      mflop_zgemm(n, nk, false) + 2 * mflop_zgemm(n, nk, true)));
-  // Z' = Z + Z (F' - I); HZ' = HZ + HZ (F' - I) is consumed by its epilogue,
-  // which leaves the Rayleigh sums of the Z' columns per 64-row tile in W3
+  // Z' = Z + Z (F' - I) and HZ' = HZ + HZ (F' - I) in one kernel; HZ' is
+  // consumed by its epilogue, which leaves the Rayleigh sums of the Z'
+  // columns per 64-row tile in W3
   const int tiles = (n + 63) / 64;
   double* part = W3;
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM32));
-  RTB_TRY(zgemm_add_f32(st, n, nk, zop(Z, n2), zop(F, n2), W2, n2));
-  RTB_TRY(zgemm_add_f32_rayleigh(st, n, nk, zop(W1, n2), zop(F, n2), W2, part));
-  if (tmr) RTB_TRY(tmr->end(2, 2 * flop_zgemm(n, nk), 2 * mflop_zgemm(n, nk, false)));
+  RTB_TRY(zgemm_corrections_f32(st, n, nk, Z, W1, F, W2, part));  // both products, one pass over F
+  if (tmr) RTB_TRY(tmr->end(1, 2 * flop_zgemm(n, nk), 2 * mflop_zgemm(n, nk, false)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
   hipLaunchKernelGGL(cluster_rr_kernel, dim3(static_cast<unsigned>(nk)), dim3(64), 0, st, n,
                      reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W1),

@@ -388,6 +388,154 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
   }
 }
 
+// Both correction products of the refinement in one pass over F' - I:
+// Z' = Z + Z F and (HZ)' = HZ + HZ F on the FP32 matrix cores (Z, F in FP32
+// storage, HZ in FP64), one 64 x 64 tile of each per workgroup sharing the
+// staged F tile.  Z' is stored (FP64); (HZ)' is consumed in the epilogue:
+// the Rayleigh sums Re(z'^H (HZ)') and z'^H z' of the tile's columns over its
+// rows go to part[mat][tile row][2][n].  Same lane layouts and pipelining as
+// zgemm3m_kernel.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z, const double2* __restrict__ HZ,
+                    const float2* __restrict__ F, double2* __restrict__ Zp, double* __restrict__ part) {
+  typedef Mfma<float>::v4 v4;
+  __shared__ float sA3[3][kKc][kLd];  // Z tile, k-major: re, im, re + im
+  __shared__ float sH3[3][kKc][kLd];  // HZ tile (rounded to FP32)
+  __shared__ float sB3[3][kKc][kLd];  // F tile
+  __shared__ double s_rq[2][2][kTile];
+  const size_t g = blockIdx.x;
+  const size_t loc = g / kXcd;
+  const size_t mat = (loc / tm.tiles) * kXcd + g % kXcd;
+  if (mat >= nmat) return;
+  int ti, tj;
+  tile_coords(static_cast<int>(loc % tm.tiles), tm.side, 0, ti, tj);
+  const size_t n2 = static_cast<size_t>(n) * n;
+  Z += mat * n2;
+  HZ += mat * n2;
+  F += mat * n2;
+  Zp += mat * n2;
+  const int row0 = ti * kTile, col0 = tj * kTile;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  float2 pz[4], pf[4];
+  double2 ph[4];
+  auto load_chunk = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = tid + 256 * u;
+      {  // lanes along k within a row
+        const int i = q / kKc, kk = q % kKc;
+        const int r = row0 + i, k = k0 + kk;
+        const bool ok = r < n && k < n;
+        pz[u] = ok ? Z[static_cast<size_t>(r) * n + k] : zero2<float2>();
+        ph[u] = ok ? HZ[static_cast<size_t>(r) * n + k] : zero2<double2>();
+      }
+      const int kk = q / kTile, j = q % kTile;
+      const int k = k0 + kk, c = col0 + j;
+      pf[u] = (k < n && c < n) ? F[static_cast<size_t>(k) * n + c] : zero2<float2>();
+    }
+  };
+  auto store_chunk = [&]() {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = tid + 256 * u;
+      const int i = q / kKc, kk = q % kKc;
+      sA3[0][kk][i] = pz[u].x;
+      sA3[1][kk][i] = pz[u].y;
+      sA3[2][kk][i] = pz[u].x + pz[u].y;
+      const float hr = static_cast<float>(ph[u].x), hi = static_cast<float>(ph[u].y);
+      sH3[0][kk][i] = hr;
+      sH3[1][kk][i] = hi;
+      sH3[2][kk][i] = hr + hi;
+      const int kb = q / kTile, j = q % kTile;
+      sB3[0][kb][j] = pf[u].x;
+      sB3[1][kb][j] = pf[u].y;
+      sB3[2][kb][j] = pf[u].x + pf[u].y;
+    }
+  };
+  v4 acz[3][2][2], ach[3][2][2];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acz[p][a][b] = v4{0.f, 0.f, 0.f, 0.f};
+        ach[p][a][b] = v4{0.f, 0.f, 0.f, 0.f};
+      }
+  const int fr = lane & 15, fk = lane >> 4;
+  load_chunk(0);
+  for (int k0 = 0; k0 < n; k0 += kKc) {
+    __syncthreads();
+    store_chunk();
+    __syncthreads();
+    if (k0 + kKc < n) load_chunk(k0 + kKc);
+#pragma unroll
+    for (int ks = 0; ks < kKc / 4; ++ks) {
+      const int kk = 4 * ks;
+      float az[3][2], ah[3][2], bv[3][2];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          az[p][a] = sA3[p][kk + fk][wr + 16 * a + fr];
+          ah[p][a] = sH3[p][kk + fk][wr + 16 * a + fr];
+          bv[p][a] = sB3[p][kk + fk][wc + 16 * a + fr];
+        }
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            acz[p][a][b] = Mfma<float>::mac(az[p][a], bv[p][b], acz[p][a][b]);
+            ach[p][a][b] = Mfma<float>::mac(ah[p][a], bv[p][b], ach[p][a][b]);
+          }
+    }
+  }
+  double num[2] = {0.0, 0.0}, den[2] = {0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + wr + 16 * a + Mfma<float>::row(r, fk);
+        const int j = col0 + wc + 16 * b + fr;
+        if (i < n && j < n) {
+          const size_t o = static_cast<size_t>(i) * n + j;
+          const float2 z = Z[o];
+          const double2 h = HZ[o];
+          double p1 = acz[0][a][b][r], p2 = acz[1][a][b][r], p3 = acz[2][a][b][r];
+          const double zr = z.x + (p1 - p2), zi = z.y + (p3 - p1 - p2);
+          p1 = ach[0][a][b][r];
+          p2 = ach[1][a][b][r];
+          p3 = ach[2][a][b][r];
+          const double hr = h.x + (p1 - p2), hi = h.y + (p3 - p1 - p2);
+          Zp[o] = make_double2(zr, zi);
+          num[b] += zr * hr + zi * hi;  // Re(conj(z') h')
+          den[b] += zr * zr + zi * zi;
+        }
+      }
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    num[b] += __shfl_xor(num[b], 16, 64);
+    num[b] += __shfl_xor(num[b], 32, 64);
+    den[b] += __shfl_xor(den[b], 16, 64);
+    den[b] += __shfl_xor(den[b], 32, 64);
+    if (fk == 0) {
+      s_rq[wave >> 1][0][wc + 16 * b + fr] = num[b];
+      s_rq[wave >> 1][1][wc + 16 * b + fr] = den[b];
+    }
+  }
+  __syncthreads();
+  if (tid < kTile && col0 + tid < n) {
+    double* pt = part + (mat * tm.side + ti) * 2 * static_cast<size_t>(n);
+    pt[col0 + tid] = s_rq[0][0][tid] + s_rq[1][0][tid];
+    pt[n + col0 + tid] = s_rq[0][1][tid] + s_rq[1][1][tid];
+  }
+}
+
 template <typename T, bool ADD, typename SA, typename SB>
 int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_a, const ZOperand& B,
                  const ZOperand& D, double* C, size_t sC, bool herm, const double* sc,
@@ -456,6 +604,27 @@ int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* 
   kb.part = part;
   return zgemm_launch<double, false, double2, double2>(st, n, nb, zop(U, n2), true, zop(T1, n2), ZOperand{},
                                                        nullptr, n2, false, s, OaEpilogue{}, RqEpilogue{}, kb);
+}
+
+int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, const double* HZ, const float* F,
+                          double* Zp, double* part) {
+  if (nb == 0 || n <= 0) return RTB_OK;
+  TileMap tm;
+  tm.side = (n + kTile - 1) / kTile;
+  tm.herm = 0;
+  tm.tiles = tm.side * tm.side;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const size_t per_launch = (static_cast<size_t>(1) << 24) / (kXcd * tm.tiles) * kXcd;
+  for (size_t off = 0; off < nb; off += per_launch) {
+    const size_t cnt = nb - off < per_launch ? nb - off : per_launch;
+    const size_t wgs = (cnt + kXcd - 1) / kXcd * tm.tiles * kXcd;
+    hipLaunchKernelGGL(zgemm3m_dual_kernel, dim3(static_cast<unsigned>(wgs)), dim3(256), 0, st, n, cnt, tm,
+                       reinterpret_cast<const float2*>(Z) + off * n2, reinterpret_cast<const double2*>(HZ) + off * n2,
+                       reinterpret_cast<const float2*>(F) + off * n2, reinterpret_cast<double2*>(Zp) + off * n2,
+                       part + off * tm.side * 2 * static_cast<size_t>(n));
+  }
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
 }
 
 int zgemm_add_f32_rayleigh(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, const double* Zp,
