@@ -102,14 +102,6 @@ int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* 
 // tile into part ([nb][ceil(n/64)][2][n]); products on the FP32 matrix cores.
 int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, const double* HZ, const float* F,
                           double* Zp, double* part);
-// The same product (C = A + A B) consumed by its epilogue: not stored; the
-// Rayleigh sums of the columns Zp ([nb][n][n]) against it, Re(zp_j^H C_j)
-// and zp_j^H zp_j, are written per 64-row tile to part ([nb][ceil(n/64)][2][n]).
-int zgemm_add_f32_rayleigh(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, const double* Zp,
-                           double* part);
-// C = A + A B with the product on the FP32 matrix cores, for a correction B
-// stored in FP32 (A in either storage; FP64 sums).  C must not alias A or B.
-int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC);
 // C[b] = op(A[b]) * B[b], n x n complex; conj_a: op = conjugate transpose.
 // sA, sB, sC: batch strides in complex elements.  herm: the product is known
 // to be Hermitian; the upper triangle is filled from the lower one.

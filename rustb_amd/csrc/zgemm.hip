@@ -1,10 +1,12 @@
 // Batched complex matrix products on the matrix cores (gfx950).
 //
-// C = op(A) B (FP64 arithmetic) or C = A + A B (FP32 arithmetic, for small
-// corrections B) for a batch of n x n complex matrices (n <= 128 in practice,
-// any n works), row-major interleaved (re, im).  Each operand is stored in
-// FP64 or FP32 (values exact in FP32, or needed only to FP32 accuracy: half
-// the memory traffic); C is FP64.  One workgroup computes one
+// zgemm3m_kernel: C = op(A) B in FP64 arithmetic for a batch of n x n
+// complex matrices (n <= 128 in practice, any n works), row-major
+// interleaved (re, im).  Each operand is stored in FP64 or FP32 (values exact
+// in FP32: half the memory traffic); C is FP64, or consumed by an epilogue
+// (the Ogita-Aishima correction from the Gram product, the Kubo band-sum
+// weights from the velocity product).  zgemm3m_dual_kernel: the two FP32
+// correction products of the refinement.  One workgroup computes one
 // 64 x 64 output tile; its four waves own 32 x 32 quarters (2 x 2 blocks of
 // the 16 x 16 x 4 MFMA).
 //
@@ -99,15 +101,6 @@ struct OaEpilogue {
   double gap_rel = 0.0;
 };
 
-// Optional epilogue of the correction product HZ' = HZ + HZ (F' - I): the
-// Rayleigh sums of the refined columns, num_j = Re(z'_j^H HZ'_j) and
-// den_j = z'_j^H z'_j, over the tile's rows, written per row tile to
-// part[mat][tile row][2][n] (Z' read from memory); C may then be null.
-struct RqEpilogue {
-  const double2* Zp = nullptr;  // [nmat][n][n], stride sC
-  double* part = nullptr;       // [nmat][tiles per side][2][n]
-};
-
 // Optional epilogue of the Kubo product C = U^H (v_a + i v_b) U: C is not
 // stored; with Q_ij = |C_ij|^2 / ((E_i - E_j)^2 + eta^2) (Q_ii = 0), the tile
 // writes its column sums of Q (over its rows) to part[mat][0][tile row][n]
@@ -125,11 +118,11 @@ __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 2 : 4, sizeof(T) == 8 ? 2 : 4)))
 zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t sA, int conj_a,
                const SB* __restrict__ B, size_t sB, const SA* D, size_t sD, double2* __restrict__ C,
-               size_t sC, const double* __restrict__ sc, OaEpilogue oa, RqEpilogue rq, KuboEpilogue kb) {
+               size_t sC, const double* __restrict__ sc, OaEpilogue oa, KuboEpilogue kb) {
   typedef typename Mfma<T>::v4 v4;
   __shared__ double s_lt[kMaxEighN];  // Rayleigh quotients (OA epilogue)
   __shared__ double s_red[4];
-  __shared__ double s_rq[2][2][kTile];  // Rayleigh sums / Kubo column sums by wave row
+  __shared__ double s_rq[2][kTile];  // Kubo column sums by wave row
   __shared__ double s_kr[2][kTile];     // Kubo row sums by wave column
   __shared__ T sA3[3][kKc][kLd];  // op(A) tile, k-major: re, im, re + im
   __shared__ T sB3[3][kKc][kLd];  // B tile, k-major
@@ -271,9 +264,8 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
     }
   }
   const bool mirror = tm.herm && ti != tj;
-  if (rq.part) rq.Zp += mat * sC;
-  double rq_num[2] = {0.0, 0.0}, rq_den[2] = {0.0, 0.0};
-  double kb_row[2][4] = {};  // Kubo: per (a, r) row of this lane
+  double kb_col[2] = {0.0, 0.0};  // Kubo: per b column of this lane
+  double kb_row[2][4] = {};       // Kubo: per (a, r) row of this lane
   double kb_er[2][4] = {}, kb_ec[2] = {};  // eigenvalues of this lane's rows and columns
   if (kb.part) {
     kb.E += mat * n;
@@ -316,7 +308,7 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
               rc = fma(rc, fma(-den, rc, 1.0), rc);
               rc = fma(rc, fma(-den, rc, 1.0), rc);
               const double q = (re * re + im * im) * rc;
-              rq_num[b] += q;  // column j
+              kb_col[b] += q;  // column j
               kb_row[a][r] += q;
             }
             continue;
@@ -337,19 +329,14 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
             C[o] = make_double2(re, im);
             if (mirror) C[static_cast<size_t>(j) * n + i] = make_double2(re, -im);
           }
-          if (rq.part) {
-            const double2 z = rq.Zp[o];
-            rq_num[b] += z.x * re + z.y * im;  // Re(conj(z) hz)
-            rq_den[b] += z.x * z.x + z.y * z.y;
-          }
         }
       }
   if (kb.part) {  // workgroup-uniform: column and row sums of Q over the tile
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      rq_num[b] += __shfl_xor(rq_num[b], 16, 64);
-      rq_num[b] += __shfl_xor(rq_num[b], 32, 64);
-      if (fk == 0) s_rq[wave >> 1][0][wc + 16 * b + fr] = rq_num[b];
+      kb_col[b] += __shfl_xor(kb_col[b], 16, 64);
+      kb_col[b] += __shfl_xor(kb_col[b], 32, 64);
+      if (fk == 0) s_rq[wave >> 1][wc + 16 * b + fr] = kb_col[b];
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -362,28 +349,8 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
     __syncthreads();
     if (tid < kTile) {
       const size_t base = mat * 2 * static_cast<size_t>(tm.side);
-      if (col0 + tid < n) kb.part[(base + ti) * n + col0 + tid] = s_rq[0][0][tid] + s_rq[1][0][tid];
+      if (col0 + tid < n) kb.part[(base + ti) * n + col0 + tid] = s_rq[0][tid] + s_rq[1][tid];
       if (row0 + tid < n) kb.part[(base + tm.side + tj) * n + row0 + tid] = s_kr[0][tid] + s_kr[1][tid];
-    }
-  }
-  if (rq.part) {  // workgroup-uniform: column sums over the tile's 64 rows
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      // the four lane groups of a wave hold the same 16 columns
-      rq_num[b] += __shfl_xor(rq_num[b], 16, 64);
-      rq_num[b] += __shfl_xor(rq_num[b], 32, 64);
-      rq_den[b] += __shfl_xor(rq_den[b], 16, 64);
-      rq_den[b] += __shfl_xor(rq_den[b], 32, 64);
-      if (fk == 0) {
-        s_rq[wave >> 1][0][wc + 16 * b + fr] = rq_num[b];
-        s_rq[wave >> 1][1][wc + 16 * b + fr] = rq_den[b];
-      }
-    }
-    __syncthreads();
-    if (tid < kTile && col0 + tid < n) {
-      double* pt = rq.part + (mat * tm.side + ti) * 2 * static_cast<size_t>(n);
-      pt[col0 + tid] = s_rq[0][0][tid] + s_rq[1][0][tid];
-      pt[n + col0 + tid] = s_rq[0][1][tid] + s_rq[1][1][tid];
     }
   }
 }
@@ -470,26 +437,32 @@ zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z
     store_chunk();
     __syncthreads();
     if (k0 + kKc < n) load_chunk(k0 + kKc);
-#pragma unroll
-    for (int ks = 0; ks < kKc / 4; ++ks) {
-      const int kk = 4 * ks;
-      float az[3][2], ah[3][2], bv[3][2];
+    // fragments of k-step s + 1 read from LDS while the 24 MFMAs of k-step s
+    // issue (two register sets)
+    float az[2][3][2], ah[2][3][2], bv[2][3][2];
+    auto frag = [&](int set, int kk) {
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
-          az[p][a] = sA3[p][kk + fk][wr + 16 * a + fr];
-          ah[p][a] = sH3[p][kk + fk][wr + 16 * a + fr];
-          bv[p][a] = sB3[p][kk + fk][wc + 16 * a + fr];
+          az[set][p][a] = sA3[p][kk + fk][wr + 16 * a + fr];
+          ah[set][p][a] = sH3[p][kk + fk][wr + 16 * a + fr];
+          bv[set][p][a] = sB3[p][kk + fk][wc + 16 * a + fr];
         }
+    };
+    frag(0, 0);
+#pragma unroll
+    for (int ks = 0; ks < kKc / 4; ++ks) {
+      const int cur = ks & 1;
+      if (ks + 1 < kKc / 4) frag(cur ^ 1, 4 * (ks + 1));
 #pragma unroll
       for (int p = 0; p < 3; ++p)
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
-            acz[p][a][b] = Mfma<float>::mac(az[p][a], bv[p][b], acz[p][a][b]);
-            ach[p][a][b] = Mfma<float>::mac(ah[p][a], bv[p][b], ach[p][a][b]);
+            acz[p][a][b] = Mfma<float>::mac(az[cur][p][a], bv[cur][p][b], acz[p][a][b]);
+            ach[p][a][b] = Mfma<float>::mac(ah[cur][p][a], bv[cur][p][b], ach[p][a][b]);
           }
     }
   }
@@ -539,8 +512,7 @@ zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z
 template <typename T, bool ADD, typename SA, typename SB>
 int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_a, const ZOperand& B,
                  const ZOperand& D, double* C, size_t sC, bool herm, const double* sc,
-                 const OaEpilogue& oa = OaEpilogue{}, const RqEpilogue& rq = RqEpilogue{},
-                 const KuboEpilogue& kb = KuboEpilogue{}) {
+                 const OaEpilogue& oa = OaEpilogue{}, const KuboEpilogue& kb = KuboEpilogue{}) {
   if (nb == 0 || n <= 0) return RTB_OK;
   TileMap tm;
   tm.side = (n + kTile - 1) / kTile;
@@ -562,8 +534,6 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_
                        C ? reinterpret_cast<double2*>(C) + off * sC : nullptr, sC, sc ? sc + off * n : nullptr,
                        OaEpilogue{oa.S ? oa.S + off * sC : nullptr, oa.gd ? oa.gd + off * n : nullptr,
                                   oa.F ? oa.F + off * sC : nullptr, oa.gap_rel},
-                       RqEpilogue{rq.Zp ? rq.Zp + off * sC : nullptr,
-                                  rq.part ? rq.part + off * tm.side * 2 * static_cast<size_t>(n) : nullptr},
                        KuboEpilogue{kb.E ? kb.E + off * n : nullptr, kb.eta2,
                                     kb.part ? kb.part + off * tm.side * 2 * static_cast<size_t>(n) : nullptr});
   }
@@ -603,7 +573,7 @@ int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* 
   kb.eta2 = eta * eta;
   kb.part = part;
   return zgemm_launch<double, false, double2, double2>(st, n, nb, zop(U, n2), true, zop(T1, n2), ZOperand{},
-                                                       nullptr, n2, false, s, OaEpilogue{}, RqEpilogue{}, kb);
+                                                       nullptr, n2, false, s, OaEpilogue{}, kb);
 }
 
 int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, const double* HZ, const float* F,
@@ -625,26 +595,6 @@ int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, cons
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
-}
-
-int zgemm_add_f32_rayleigh(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, const double* Zp,
-                           double* part) {
-  if (!B.f32) return fail(RTB_ERR_UNSUPPORTED, "zgemm_add_f32: the correction operand is stored in FP32");
-  const size_t n2 = static_cast<size_t>(n) * n;
-  RqEpilogue rq;
-  rq.Zp = reinterpret_cast<const double2*>(Zp);
-  rq.part = part;
-  if (A.f32)
-    return zgemm_launch<float, true, float2, float2>(st, n, nb, A, false, B, A, nullptr, n2, false, nullptr,
-                                                     OaEpilogue{}, rq);
-  return zgemm_launch<float, true, double2, float2>(st, n, nb, A, false, B, A, nullptr, n2, false, nullptr,
-                                                    OaEpilogue{}, rq);
-}
-
-int zgemm_add_f32(hipStream_t st, int n, size_t nb, ZOperand A, ZOperand B, double* C, size_t sC) {
-  if (!B.f32) return fail(RTB_ERR_UNSUPPORTED, "zgemm_add_f32: the correction operand is stored in FP32");
-  if (A.f32) return zgemm_launch<float, true, float2, float2>(st, n, nb, A, false, B, A, C, sC, false, nullptr);
-  return zgemm_launch<float, true, double2, float2>(st, n, nb, A, false, B, A, C, sC, false, nullptr);
 }
 
 int zgemm_batched(hipStream_t st, int n, size_t nb, const double* A, size_t sA, bool conj_a, const double* B,
