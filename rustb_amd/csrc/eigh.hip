@@ -129,7 +129,7 @@ template <typename T, bool VEC>
 __global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nmat, const T* __restrict__ Dg,
                                                               const T* __restrict__ Eg, double* __restrict__ Eout,
                                                               T* __restrict__ S, T* __restrict__ TW) {
-  __shared__ T sh[4 * kTeBlock];
+  __shared__ T sh[5 * kTeBlock];
   const int per = kTeBlock / n;  // matrices per workgroup (n <= 128)
   const int sub = threadIdx.x / n;
   const size_t mat = static_cast<size_t>(blockIdx.x) * per + sub;
@@ -137,16 +137,17 @@ __global__ void __launch_bounds__(kTeBlock) eigh_trieig_kernel(int n, size_t nma
   // idle threads take id >= n and sub-team 0's LDS (read-only for them)
   const int so = live ? sub : 0;
   GpuSubTeam tm(live ? static_cast<int>(threadIdx.x % n) : n, n);
-  T* d = sh + so * 4 * n;
+  T* d = sh + so * 5 * n;
   T* e = d + n;
   T* e2 = e + n;
   T* E = e2 + n;
+  T* cnt = E + n;
   if (live) {
     d[tm.id] = Dg[mat * n + tm.id];
     e[tm.id] = Eg[mat * n + tm.id];
   }
   __syncthreads();
-  bisect_all<T>(tm, n, d, e, e2, E);
+  bisect_all<T>(tm, n, d, e, e2, E, cnt);
   if (live) Eout[mat * n + tm.id] = static_cast<double>(E[tm.id]);
   if (VEC) {
     const size_t mm = live ? mat : 0;

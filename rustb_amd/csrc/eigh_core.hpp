@@ -121,8 +121,12 @@ RTB_HD inline void gershgorin(int n, const T* d, const T* e, T& gl, T& gu) {
   }
 }
 
+// The brackets start from one multisection sweep: the team counts at the n - 1
+// points splitting the Gershgorin interval into n equal parts (one point per
+// thread, counts in cnt[n]), and eigenvalue j starts from the part holding
+// it, log2(n) bisection steps shorter.
 template <typename T, typename Team>
-RTB_HD void bisect_all(Team& tm, int n, const T* d, const T* e, T* e2, T* E) {
+RTB_HD void bisect_all(Team& tm, int n, const T* d, const T* e, T* e2, T* E, T* cnt) {
   for (int i = tm.id; i < n; i += tm.nt) e2[i] = (i + 1 < n) ? e[i] * e[i] : T(0);
   tm.sync();
   T gl, gu;
@@ -151,8 +155,22 @@ RTB_HD void bisect_all(Team& tm, int n, const T* d, const T* e, T* e2, T* E) {
     tm.sync();
     return;
   }
+  const T width = gu - gl;
+  auto point = [&](int m) { return gl + width * static_cast<T>(m + 1) / static_cast<T>(n); };
+  for (int m = tm.id; m + 1 < n; m += tm.nt) cnt[m] = static_cast<T>(sturm_count(n, d, e2, point(m), pivmin));
+  tm.sync();
   for (int j = tm.id; j < n; j += tm.nt) {
-    T lo = gl, hi = gu;
+    // first point whose count exceeds j (counts are non-decreasing)
+    int a = 0, b = n - 1;
+    while (a < b) {
+      const int mid = (a + b) / 2;
+      if (cnt[mid] > static_cast<T>(j)) {
+        b = mid;
+      } else {
+        a = mid + 1;
+      }
+    }
+    T lo = a > 0 ? point(a - 1) : gl, hi = a < n - 1 ? point(a) : gu;
     for (int it = 0; it < 200; ++it) {
       T tol = atol + T(2) * eps * fmax(fabs(lo), fabs(hi));
       if (hi - lo <= tol) break;

@@ -35,7 +35,7 @@ int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_
   constexpr int NT = NB * S, NTb = NB * Sbt;
   constexpr int SM = S > Sbt ? S : Sbt;
   std::vector<cplx<T>> xc(NB), vp(3 * NB), wp(NB), yp(2 * SM * NB);
-  std::vector<T> red(2 * 2048), d(n), e(n), e2(n), Ev(n);
+  std::vector<T> red(2 * 2048), d(n), e(n), e2(n), Ev(n), cnt(n);
   std::vector<cplx<T>> Vg(static_cast<size_t>(n) * n), taug(n);
   std::vector<T> Y(static_cast<size_t>(n) * n), Sw(static_cast<size_t>(n) * n * 3), twist(n);
   std::vector<T> es(NB);
@@ -51,7 +51,7 @@ int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_
   };
   team(NT, [&](auto& tm) { tridiag_reg<T, NB, S>(tm, n, H, sm, d.data(), e.data(), Vg.data(), taug.data()); });
   team(n, [&](auto& tm) {
-    bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data());
+    bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data(), cnt.data());
     if (vectors) tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, Sw.data(), twist.data());
   });
   for (int j = 0; j < n; ++j) E[j] = static_cast<double>(Ev[j]);
