@@ -93,14 +93,19 @@ def test_reg_exact_two_by_two(emul):
     assert np.array_equal(E, [-3.0, 3.0])
 
 
-@pytest.mark.parametrize("case", ["near_degenerate", "clusters_and_isolated"])
+@pytest.mark.parametrize("case", ["near_degenerate", "clusters_and_isolated", "few_members"])
 def test_reg_clustered_spectra(emul, case):
-    """Clusters (inverse iteration + MGS) next to isolated eigenvalues (twisted)."""
+    """Clusters (inverse iteration + MGS) next to isolated eigenvalues (twisted):
+    three clusters of 4, clusters of 5, 3 and 4 among 40 isolated levels, and
+    a pair and a triple among 30."""
     rng = np.random.default_rng(15)
     if case == "near_degenerate":
         H = np.kron(rand_herm(rng, 3), np.eye(4)) + 1e-9 * np.diag(rng.standard_normal(12))
     else:
-        w = np.concatenate([np.full(5, 1.0), np.linspace(-3, 3, 40), np.full(3, -2.5), 2.0 + 1e-7 * np.arange(4)])
+        if case == "few_members":
+            w = np.concatenate([[1.0, 1.0 + 1e-8], np.linspace(-3, 3, 30), np.full(3, -2.5)])
+        else:
+            w = np.concatenate([np.full(5, 1.0), np.linspace(-3, 3, 40), np.full(3, -2.5), 2.0 + 1e-7 * np.arange(4)])
         Q, _ = np.linalg.qr(rng.standard_normal((w.size, w.size)) + 1j * rng.standard_normal((w.size, w.size)))
         H = (Q * w) @ Q.conj().T
         H = 0.5 * (H + H.conj().T)
