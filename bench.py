@@ -6,6 +6,12 @@ eigh -> Kubo Omega_z -> AHC at mu = 0, T = 0, eta = 1e-4, all on the device
 (one rtb_hall_direct call = one step; inputs resident in HBM, only the mu
 values and sigma cross PCIe).
 
+`--ingest` measures the input side instead: the same model written as
+Wannier90 files (seedname_hr.dat in Wannier90's '(5I5,2F12.6)' layout, 21.8 M
+lines, 1.09 GB) read by the library's loader and uploaded as a device model;
+its JSON line compares the parse rate with the host's read ceiling for that
+file (rtb_probe_read) and the oracle's line-by-line reading.
+
 Prints ONE JSON line.  `value` is k-points/s of the whole job.  `roofline`
 is the dominant kernel's algorithmic work per launch over its average launch
 time, measured with HIP events inside the library on the kernel's stream
@@ -145,6 +151,91 @@ def accuracy_check(model, kmesh, idx, dirs, eta, precision):
     }
 
 
+def ingest_bench(args):
+    """Wannier90 _hr.dat -> device model (Wannier90::from_hr, wannier90.rs:49-1228)."""
+    import rustb_amd as ra
+    from rustb_amd import _lib, workloads as W
+
+    d = args.ingest_dir.rstrip("/") + "/"
+    os.makedirs(d, exist_ok=True)
+    hr = d + "wan_hr.dat"
+    model = W.wannier3d_n128()
+    n, nR = model.nsta, model.hamR.shape[0]
+    expect = 3 + (nR + 14) // 15 + nR * n * n  # lines
+    if not (os.path.exists(hr) and os.path.exists(d + "wan.win")
+            and os.path.getsize(hr) >= 50 * nR * n * n):
+        t = time.perf_counter()
+        W.write_wannier90(model, d + "wan", centres=True)
+        print(f"[ingest] wrote {hr} in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    size = os.path.getsize(hr)
+    threads = min(16, os.cpu_count() or 1)
+    lib = _lib.load()
+    probe = ctypes.CDLL(os.path.join(ROOT, "rustb_amd", "librtb_probe.so"))
+    probe.rtb_probe_read.restype = ctypes.c_int
+    probe.rtb_probe_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    ceil = ctypes.c_double()
+    for _ in range(2):  # the second scan is from the page cache
+        probe.rtb_probe_read(hr.encode(), threads, ctypes.byref(ceil))
+    h = ctypes.c_void_p()
+    dm = ctypes.c_void_p()
+    for _ in range(args.warmup):
+        _lib.check(lib.rtb_w90_read(d.encode(), b"wan", 0.0, -1, threads, ctypes.byref(h)))
+        lib.rtb_w90_destroy(h)
+    t_parse = t_up = 0.0
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _lib.check(lib.rtb_w90_read(d.encode(), b"wan", 0.0, -1, threads, ctypes.byref(h)))
+        t1 = time.perf_counter()
+        _lib.check(lib.rtb_w90_model(h, ctypes.byref(dm)))
+        device_sync()
+        t2 = time.perf_counter()
+        t_parse += t1 - t0
+        t_up += t2 - t1
+        lib.rtb_model_destroy(dm)
+        lib.rtb_w90_destroy(h)
+    t_parse /= args.steps
+    t_up /= args.steps
+    parse_gbps = size / t_parse / 1e9
+    out = {
+        "metric": "Wannier90 _hr.dat ingest (file -> device model), MB/s",
+        "value": round(size / (t_parse + t_up) / 1e6, 1),
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round((t_parse + t_up) * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "text -> f64",
+        "data": "synthetic",
+        "config": {"workload": "wannier3d-n128 _hr.dat", "bytes": size, "lines": expect, "nsta": n, "n_R": nR,
+                   "threads": threads, "files": "_hr.dat, .win, _centres.xyz"},
+        "parse_ms": round(t_parse * 1e3, 1),
+        "upload_ms": round(t_up * 1e3, 1),
+        "roofline": {"bound": "host-read", "achieved": round(parse_gbps, 3), "peak": round(ceil.value, 3),
+                     "unit": "GB/s", "frac": round(parse_gbps / ceil.value, 4) if ceil.value else None,
+                     "traffic": None, "kernel": "rtb_w90_read (host threads)",
+                     "peak_is": f"newline scan of the page-cached file on {threads} threads (rtb_probe_read)"},
+    }
+    if not args.no_cpu:
+        # the oracle (line-by-line, one core) on a bounded sample: the same
+        # model cut to |R_i| <= 1 (27 R, 442k lines)
+        from oracle import wannier90_ref as OW
+        keep = np.abs(model.hamR).max(axis=1) <= 1
+        small = ra.Model.from_arrays(model.lat, model.orb, model.spin, model.hamR[keep], model.ham[keep])
+        W.write_wannier90(small, d + "sample", centres=True)
+        ss = os.path.getsize(d + "sample_hr.dat")
+        t0 = time.perf_counter()
+        OW.from_hr(d, "sample", 0.0)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(ss / tc / 1e6, 2), "unit": "MB/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle/wannier90_ref.py on the |R_i|<=1 cut ({ss} bytes, {tc:.1f} s)"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +246,9 @@ def main():
     ap.add_argument("--kmesh", default="48,48,48")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ingest", action="store_true",
+                    help="measure Wannier90 _hr.dat ingestion of the same model instead of the k-mesh path")
+    ap.add_argument("--ingest-dir", default="/tmp/rtb_ingest")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device copies of 1 GiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -165,6 +259,9 @@ def main():
 
     if args.copy_probe:
         copy_probe()
+        return
+    if args.ingest:
+        ingest_bench(args)
         return
 
     import rustb_amd as ra

@@ -40,6 +40,13 @@ extern "C" {
 #define RTB_ERR_INVALID_ARGUMENT -12     /* null pointer, zero size where one is required */
 #define RTB_ERR_DEVICE -13               /* a HIP runtime call failed */
 #define RTB_ERR_UNSUPPORTED -14          /* a size outside what the kernels implement */
+#define RTB_ERR_FILE_OPEN -15            /* TbError::FileCreation (error.rs:79), an input file missing */
+#define RTB_ERR_FILE_PARSE -16           /* TbError::FileParse (error.rs:64) */
+#define RTB_ERR_ORBITAL_PROJECTION -17   /* TbError::InvalidOrbitalProjection (error.rs:67) */
+#define RTB_ERR_SPIN_MISMATCH -18        /* TbError::Other "Spin mismatch" (wannier90.rs:559-564) */
+#define RTB_ERR_IO -19                   /* TbError::Io (error.rs:61), e.g. text that is not UTF-8 */
+#define RTB_ERR_NO_ORBITALS -20          /* TbError::NoOrbitals (error.rs:279) */
+#define RTB_ERR_LINALG -21               /* TbError::Linalg (error.rs:83): a singular lattice */
 
 /* ---- enums ---- */
 #define RTB_GAUGE_LATTICE 0 /* Gauge::Lattice (model.rs:449-453) */
@@ -102,6 +109,34 @@ int rtb_synth_model(const rtb_synth_spec* spec, int* n_R, int* nsta, int64_t* ha
                     double* orb, double* lat);
 
 /* =====================================================================
+ * Wannier90 input (host only).  Wannier90::from_hr (wannier90.rs:49-1228)
+ * for Model<SPIN, 3, NoRMatrix>: reads {path}{seedname}_hr.dat and .win,
+ * and _centres.xyz and _wsvec.dat when present.  Storage follows model.rs:
+ * hamR row 0 is R = 0 (Fermi level `zero_energy` subtracted from its
+ * diagonal), then the other R in file order (or in order of first appearance
+ * after the Wigner-Seitz shifts of _wsvec.dat).
+ * ===================================================================== */
+typedef struct rtb_w90 rtb_w90;
+/* spin: 0 or 1 is the caller's SPIN (a different `spinors` setting in .win is
+ * RTB_ERR_SPIN_MISMATCH), -1 takes the file's.  n_threads: parse threads for
+ * the _hr.dat body (0: up to 16). */
+int rtb_w90_read(const char* path, const char* seedname, double zero_energy, int spin, int n_threads,
+                 rtb_w90** out);
+void rtb_w90_destroy(rtb_w90* m);
+/* Model::norb / nsta / hamR.nrows() / natom (model.rs:697-709). */
+int rtb_w90_info(const rtb_w90* m, int* spin, int* norb, int* nsta, int* n_R, int* natom);
+/* Model fields lat (3x3 rows, Angstrom), orb (norb x 3 fractional),
+ * orb_projection (norb OrbProj codes in the enum's order, atom_struct.rs:15-77),
+ * hamR (n_R x 3), ham (n_R x nsta x nsta complex).  Null outputs are skipped. */
+int rtb_w90_export(const rtb_w90* m, int64_t* hamR, double* ham, double* orb, double* lat, int* orb_proj);
+/* Model::atoms[i] (atom_struct.rs Atom): species as its atomic number
+ * (AtomType, H = 1 .. Ra = 88), fractional position, orbitals
+ * [first_orb, first_orb + n_orb). */
+int rtb_w90_atom(const rtb_w90* m, int i, int* species, double* pos, int* first_orb, int* n_orb);
+const char* rtb_element_symbol(int z);
+const char* rtb_orbital_projection_name(int code);
+
+/* =====================================================================
  * Device-resident model and the hot path
  * ===================================================================== */
 typedef struct rtb_model rtb_model;
@@ -112,6 +147,8 @@ int rtb_model_create(int dim, int spin, int norb, int n_R, const int64_t* hamR,
                      const double* ham /*n_R*n*n complex*/, const double* orb /*norb*dim*/,
                      const double* lat /*dim*dim rows*/, rtb_model** out);
 void rtb_model_destroy(rtb_model* m);
+/* The device model of a loaded Wannier90 model (as rtb_model_create). */
+int rtb_w90_model(const rtb_w90* w, rtb_model** out);
 int rtb_model_nsta(const rtb_model* m);
 
 /* Model::gen_ham (model_physics.rs:43-148) and Model::gen_v_projected

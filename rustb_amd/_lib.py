@@ -28,6 +28,13 @@ ERRORS = {
     -12: "InvalidArgument",
     -13: "Device",
     -14: "Unsupported",
+    -15: "FileCreation",
+    -16: "FileParse",
+    -17: "InvalidOrbitalProjection",
+    -18: "SpinMismatch",
+    -19: "Io",
+    -20: "NoOrbitals",
+    -21: "Linalg",
 }
 
 
@@ -84,6 +91,15 @@ SIGNATURES = {
                                              c_double, c_int, _P]),
     "rtb_hall_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, _P]),
     "rtb_dos": (c_int, [c_void_p, _P, c_double, c_double, c_size_t, c_double, c_int, _P, _P]),
+    "rtb_w90_read": (c_int, [c_char_p, c_char_p, c_double, c_int, c_int, POINTER(c_void_p)]),
+    "rtb_w90_destroy": (None, [c_void_p]),
+    "rtb_w90_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int),
+                             POINTER(c_int)]),
+    "rtb_w90_export": (c_int, [c_void_p, _P, _P, _P, _P, _P]),
+    "rtb_w90_atom": (c_int, [c_void_p, c_int, POINTER(c_int), _P, POINTER(c_int), POINTER(c_int)]),
+    "rtb_element_symbol": (c_char_p, [c_int]),
+    "rtb_orbital_projection_name": (c_char_p, [c_int]),
+    "rtb_w90_model": (c_int, [c_void_p, POINTER(c_void_p)]),
     "rtb_last_error": (c_char_p, []),
     "rtb_last_stats": (c_int, [POINTER(Stats)]),
 }

@@ -264,7 +264,7 @@ RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* twist, 
       // only inside the chain's range (k < r going up, r < k < n going down);
       // the scratch loads are scheduled kGenGroup register rows at a time.
       // Columns without a twist (r < 0) are overwritten from Y below.
-      constexpr int kGenGroup = 4;
+      [[maybe_unused]] constexpr int kGenGroup = 4;  // unused in the host pass
       const int cc = c < n ? c : 0;
       const T* Dp = Ts + cc;
       const T* Dm = Ts + static_cast<size_t>(n) * ldy + cc;

@@ -2,7 +2,15 @@
 // independent MFMA accumulator chains with no memory traffic, timed with HIP
 // events; reports the attainable FP64 and FP32 matrix-core rates of the
 // device, against which the FP64 / FP32 product kernels are judged.
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <thread>
+#include <vector>
 
 namespace {
 
@@ -74,4 +82,42 @@ extern "C" int rtb_probe_mfma(double* tflops_f64, double* tflops_f32) {
   check(hipEventDestroy(e1));
   check(hipFree(out));
   return ok ? 0 : -1;
+}
+
+// Host read ceiling for file ingestion: `threads` host threads scan equal
+// parts of the memory-mapped file (newline count, vectorised), as the
+// Wannier90 loader's first pass does, with nothing else per byte.  Run after
+// the file is in the page cache; *gbps receives bytes / wall time.
+extern "C" int rtb_probe_read(const char* path, int threads, double* gbps) {
+  *gbps = 0.0;
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) return -1;
+  struct stat st;
+  if (fstat(fd, &st) != 0 || st.st_size == 0) {
+    close(fd);
+    return -1;
+  }
+  const size_t S = static_cast<size_t>(st.st_size);
+  void* map = mmap(nullptr, S, PROT_READ, MAP_PRIVATE, fd, 0);
+  close(fd);
+  if (map == MAP_FAILED) return -1;
+  const unsigned char* p = static_cast<const unsigned char*>(map);
+  const int T = threads > 0 ? threads : 1;
+  std::vector<size_t> count(T, 0);
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&, t] {
+      const size_t a = S / T * t, b = t == T - 1 ? S : S / T * (t + 1);
+      size_t c = 0;
+      for (size_t j = a; j < b; ++j) c += p[j] == '\n';
+      count[t] = c;
+    });
+  for (auto& x : th) x.join();
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  munmap(map, S);
+  size_t total = 0;
+  for (size_t c : count) total += c;
+  *gbps = total > 0 && dt > 0 ? S / dt / 1e9 : 0.0;
+  return 0;
 }

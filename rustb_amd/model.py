@@ -68,6 +68,19 @@ class HallConductivityResult:
         return float(self.conductivity[0]) if self.conductivity.size == 1 else None
 
 
+@dataclass
+class Atom:
+    """An atomic site (atom_struct.rs Atom): species, fractional position and
+    the indices of its orbitals."""
+
+    species: str
+    position: np.ndarray
+    orbitals: list
+
+    def norb(self):
+        return len(self.orbitals)
+
+
 class Model:
     """Tight-binding model (model.rs:150-158) with device-resident storage."""
 
@@ -107,6 +120,47 @@ class Model:
             raise TbError(-11, "hamiltonian_shape")
         m._arrays = (hamR, ham)
         return m
+
+    @classmethod
+    def from_hr(cls, path: str, file_name: str, zero_energy: float, spin=None, n_threads: int = 0) -> "Model":
+        """Wannier90::from_hr (wannier90.rs:49-1228), R = NoRMatrix: reads
+        {path}{file_name}_hr.dat, .win and, when present, _centres.xyz and
+        _wsvec.dat.  spin: the Model's SPIN (True/False; a different `spinors`
+        setting in the .win file raises), or None to take the file's.
+        Adds ``atoms`` (species, fractional position, orbital indices) and
+        ``orb_projection`` (orbital names) to the model."""
+        lib = load()
+        h = ctypes.c_void_p()
+        sp = -1 if spin is None else int(bool(spin))
+        check(lib.rtb_w90_read(path.encode(), file_name.encode(), float(zero_energy), sp, int(n_threads),
+                               ctypes.byref(h)))
+        try:
+            v = [ctypes.c_int() for _ in range(5)]
+            check(lib.rtb_w90_info(h, *[ctypes.byref(x) for x in v]))
+            file_spin, norb, nsta, nR, natom = (x.value for x in v)
+            hamR = np.zeros((nR, 3), np.int64)
+            ham = np.empty((nR, nsta, nsta), np.complex128)
+            orb = np.zeros((norb, 3))
+            lat = np.zeros((3, 3))
+            proj = np.zeros(norb, np.int32)
+            check(lib.rtb_w90_export(h, ptr(hamR), ptr(ham), ptr(orb), ptr(lat), ptr(proj)))
+            atoms = []
+            for i in range(natom):
+                z, first, cnt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+                pos = np.zeros(3)
+                check(lib.rtb_w90_atom(h, i, ctypes.byref(z), ptr(pos), ctypes.byref(first), ctypes.byref(cnt)))
+                atoms.append(Atom(lib.rtb_element_symbol(z.value).decode(), pos,
+                                  list(range(first.value, first.value + cnt.value))))
+        finally:
+            lib.rtb_w90_destroy(h)
+        m = cls(lat, orb, bool(file_spin), _builder=False)
+        m._arrays = (hamR, ham)
+        m.atoms = atoms
+        m.orb_projection = [lib.rtb_orbital_projection_name(int(c)).decode() for c in proj]
+        return m
+
+    def natom(self):
+        return len(getattr(self, "atoms", []))
 
     def _R(self, R):
         R = np.ascontiguousarray(R, dtype=np.int64)

@@ -12,6 +12,10 @@
   amplitudes exp(-|R|) CN(0,1); mesh n x n, n in [16,128]; Omega_z, Chern.
 5 slab2d-n256: 32 layers x 8 orbitals, |R_i|<=3 (49 R); 200x200 mesh.
 Generators are SplitMix64-seeded xoshiro256** (rtb_synth_model).
+
+write_wannier90 writes a model as Wannier90 files (seedname_hr.dat in
+Wannier90's own '(5I5,2F12.6)' layout, seedname.win, optionally
+seedname_centres.xyz), the input Model.from_hr reads.
 """
 from __future__ import annotations
 
@@ -61,3 +65,115 @@ CONFIGS = {
     4: dict(name="batch2d-64models"),
     5: dict(name="slab2d-n256", kmesh=(200, 200), eta=1e-4),
 }
+
+
+# ------------------------------------------------------- Wannier90 files
+def _digits_right(out, col_end, x, min_digits=1):
+    """Writes the decimal digits of x (non-negative int64) right-aligned so the
+    last digit sits at column col_end; returns the digit count per row."""
+    nd = np.full(x.shape, min_digits, np.int64)
+    for k in range(1, 19):
+        nd = np.maximum(nd, np.where(x >= 10 ** k, k + 1, 0))
+    rem = x.copy()
+    for k in range(int(nd.max())):
+        put = k < nd
+        rows = np.nonzero(put)[0]
+        out[rows, col_end - k] = (48 + rem[rows] % 10).astype(np.uint8)
+        rem //= 10
+    return nd
+
+
+def _fmt_ints(out, col0, width, v):
+    """Fortran I<width>: right-aligned signed integers in columns [col0, col0 + width)."""
+    v = np.asarray(v, np.int64)
+    if v.size and (v.max() >= 10 ** (width - 1) or v.min() <= -(10 ** (width - 2))):
+        raise ValueError("integer does not fit its field")
+    nd = _digits_right(out, col0 + width - 1, np.abs(v))
+    neg = np.nonzero(v < 0)[0]
+    out[neg, col0 + width - 1 - nd[neg]] = ord("-")
+
+
+def _fmt_f12_6(out, col0, v):
+    """Fortran F12.6 (leading zero, no sign for values that print as zero)."""
+    v = np.asarray(v, np.float64)
+    if v.size and np.abs(v).max() >= 9999.0:
+        raise ValueError("value does not fit F12.6")
+    x = np.rint(np.abs(v) * 1e6).astype(np.int64)
+    ip, fr = x // 1000000, x % 1000000
+    for k in range(6):
+        out[:, col0 + 11 - k] = (48 + fr % 10).astype(np.uint8)
+        fr //= 10
+    out[:, col0 + 5] = ord(".")
+    nd = _digits_right(out, col0 + 4, ip)
+    neg = np.nonzero((v < 0) & (x > 0))[0]
+    out[neg, col0 + 4 - nd[neg]] = ord("-")
+
+
+def write_hr(path, hamR, ham, ndegen=None, fermi=0.0, header="written by rustb_amd.workloads.write_hr"):
+    """seedname_hr.dat in Wannier90's layout: nsta, n_R, the degeneracies 15
+    per line, then for each R, n outer and m inner (1-based), the line
+    'R1 R2 R3 m n Re Im' of ndegen * <m,0|H|n,R> (+ fermi on the R = 0
+    diagonal), '(5I5,2F12.6)'.  Rows are written in the order given."""
+    hamR = np.asarray(hamR, np.int64)
+    nR, n = hamR.shape[0], ham.shape[1]
+    ndegen = np.ones(nR, np.int64) if ndegen is None else np.asarray(ndegen, np.int64)
+    with open(path, "wb") as f:
+        f.write(f"{header}\n{n:12d}\n{nR:12d}\n".encode())
+        for a in range(0, nR, 15):
+            f.write(("".join(f"{int(w):5d}" for w in ndegen[a:a + 15]) + "\n").encode())
+        mm, nn = np.meshgrid(np.arange(1, n + 1), np.arange(1, n + 1))  # n outer, m inner
+        mm, nn = mm.reshape(-1), nn.reshape(-1)
+        per = max(1, (1 << 20) // (n * n))
+        for r0 in range(0, nR, per):
+            r1 = min(nR, r0 + per)
+            blocks = np.ascontiguousarray(np.asarray(ham[r0:r1]).transpose(0, 2, 1))  # [r][n][m]
+            blocks = blocks * ndegen[r0:r1, None, None]
+            zero = np.nonzero(~hamR[r0:r1].any(axis=1))[0]
+            for z in zero:
+                blocks[z] = blocks[z] + fermi * np.eye(n)
+            L = (r1 - r0) * n * n
+            out = np.full((L, 50), ord(" "), np.uint8)
+            out[:, 49] = ord("\n")
+            Rl = np.repeat(hamR[r0:r1], n * n, axis=0)
+            for d in range(3):
+                _fmt_ints(out, 5 * d, 5, Rl[:, d])
+            _fmt_ints(out, 15, 5, np.tile(mm, r1 - r0))
+            _fmt_ints(out, 20, 5, np.tile(nn, r1 - r0))
+            flat = blocks.reshape(-1)
+            _fmt_f12_6(out, 25, flat.real)
+            _fmt_f12_6(out, 37, flat.imag)
+            f.write(out.tobytes())
+
+
+def write_wannier90(model, prefix, ndegen=None, fermi=0.0, centres=False, species="C"):
+    """A 3D model as Wannier90 files at {prefix}_hr.dat, {prefix}.win (Angstrom
+    lattice, spinors, one '<species>:s' projection, one atom per orbital in
+    atoms_frac) and, with centres=True, {prefix}_centres.xyz (the orbitals as
+    Wannier centres, then the atoms).  R rows are written sorted (R = 0 in the
+    middle for a symmetric R set, as Wannier90 lists them)."""
+    hamR, ham = model.hamR, model.ham
+    if hamR.shape[1] != 3:
+        raise ValueError("Wannier90 files describe 3D models")
+    order = np.lexsort((hamR[:, 2], hamR[:, 1], hamR[:, 0]))
+    write_hr(prefix + "_hr.dat", hamR[order], ham[order], None if ndegen is None else np.asarray(ndegen)[order],
+             fermi)
+    lat, orb = model.lat, model.orb
+    lines = ["begin unit_cell_cart", "Ang"] + [" ".join(f"{x:.16f}" for x in row) for row in lat]
+    lines += ["end unit_cell_cart", ""]
+    if model.spin:
+        lines += ["spinors = .true.", ""]
+    lines += ["begin projections", f"{species}:s", "end projections", "", "begin atoms_frac"]
+    lines += [f"{species} " + " ".join(f"{x:.17g}" for x in p) for p in orb]
+    lines += ["end atoms_frac", ""]
+    with open(prefix + ".win", "w") as f:
+        f.write("\n".join(lines))
+    if centres:
+        cart = orb @ lat
+        n = model.nsta
+        rows = [f"{n + orb.shape[0]}", "Wannier centres, written by rustb_amd.workloads"]
+        rows += ["X " + " ".join(f"{x:.17g}" for x in c) for c in cart]
+        if model.spin:
+            rows += ["X " + " ".join(f"{x:.17g}" for x in c) for c in cart]
+        rows += [f"{species} " + " ".join(f"{x:.17g}" for x in c) for c in cart]
+        with open(prefix + "_centres.xyz", "w") as f:
+            f.write("\n".join(rows) + "\n")

@@ -1,0 +1,74 @@
+"""GPU parity for models read from Wannier90 files: the loader's storage
+through the device hot path (Fourier sum, eigensolve, Berry curvature, AHC)
+against the oracle's reading of the same files through the CPU oracle.
+
+Tolerances as in test_gpu_parity.py (FP64): H(k) 1e-12 relative to
+max|h| N_R, eigenvalues 1e-12 relative to the spectral radius, Omega 1e-8
+relative to max|Omega|; the mixed form 1e-9 eV on eigenvalues."""
+import numpy as np
+import pytest
+
+import rustb_amd as ra
+from oracle import oracle as O
+from oracle import wannier90_ref as OW
+from rustb_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def written_model(tmp_path, norb=6, rmax=2, seed=41, wsvec=False):
+    m0 = ra.synth_model(2, 3, True, norb, rmax, seed, scale=1.0, decay=1.5)
+    ndegen = np.random.default_rng(seed).integers(1, 5, m0.hamR.shape[0])
+    prefix = str(tmp_path / "wan")
+    W.write_wannier90(m0, prefix, ndegen=ndegen, fermi=0.75, centres=True)
+    if wsvec:
+        rng = np.random.default_rng(seed + 1)
+        lines = ["## written by test_gpu_wannier90"]
+        n = m0.nsta
+        for R in m0.hamR:
+            for i in range(1, n + 1):
+                for j in range(1, n + 1):
+                    k = int(rng.integers(1, 3))
+                    lines += [f"{R[0]} {R[1]} {R[2]} {i} {j}", str(k)]
+                    lines += ["0 0 0"] + [f"{int(rng.integers(-1, 2)) * 4} 0 0" for _ in range(k - 1)]
+        with open(prefix + "_wsvec.dat", "w") as f:
+            f.write("\n".join(lines) + "\n")
+    d = str(tmp_path) + "/"
+    m = ra.Model.from_hr(d, "wan", 0.75)
+    w = OW.from_hr(d, "wan", 0.75)
+    o = O.model_from_arrays(w.lat, w.orb, w.spin, w.hamR, w.ham)
+    return m, o, w
+
+
+@pytest.mark.parametrize("wsvec", [False, True])
+def test_hamiltonian_and_bands_of_loaded_model(wsvec, tmp_path):
+    m, o, w = written_model(tmp_path, wsvec=wsvec)
+    print(f"[w90] nsta={m.nsta} n_R={m.hamR.shape[0]} wsvec={wsvec}", flush=True)
+    assert np.array_equal(m.hamR, w.hamR)
+    assert np.array_equal(m.ham.view(np.float64), w.ham.view(np.float64))
+    ks = np.random.default_rng(7).random((24, 3))
+    H = m.gen_ham_all(ks, "atom")
+    Hr = O.gen_ham_batch(o, ks, "atom")
+    scale = np.abs(w.ham).max() * w.hamR.shape[0]
+    assert np.abs(H - Hr).max() <= 1e-12 * scale
+    E = m.solve_band_all(ks)
+    Er = O.solve_band_all(o, ks)
+    assert np.abs(E - Er).max() <= 1e-12 * max(1.0, np.abs(Er).max())
+    Em = m.solve_band_all(ks, precision="mixed")
+    assert np.abs(Em - Er).max() <= 1e-9
+
+
+def test_berry_curvature_and_hall_of_loaded_model(tmp_path):
+    m, o, _ = written_model(tmp_path, norb=5, rmax=1, seed=43)
+    print(f"[w90] berry nsta={m.nsta}", flush=True)
+    ks = np.random.default_rng(8).random((12, 3))
+    dirs = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    p = ra.Parameters(kmesh=(1, 1, 1), direction=dirs, mu=np.array([0.0]), eta=1e-3)
+    om, E = m.berry_curvature_on(ks, p)
+    ob, Eb = O.berry_curvature_batch(o, ks, dirs, 1e-3)
+    assert np.abs(E - Eb).max() < 1e-12 * max(1.0, np.abs(Eb).max())
+    assert np.abs(om - ob).max() <= 1e-8 * max(1.0, np.abs(ob).max())
+    mu = np.array([-0.5, 0.0, 0.5])
+    s = m.hall_conductivity(ra.Parameters.rank2([4, 4, 3], [1, 0, 0], [0, 1, 0], mu, eta=0.01)).conductivity
+    r = O.hall_conductivity_direct(o, [4, 4, 3], dirs, mu, 0.0, 0.01)
+    assert np.abs(s - r).max() <= 1e-9 * max(1.0, np.abs(r).max())

@@ -1,0 +1,513 @@
+"""Wannier90 input (Wannier90::from_hr, wannier90.rs:49-1228): the C++ loader
+in librtb_hip.so (host code, no GPU) and the oracle (oracle/wannier90_ref.py),
+each against the reference's own regression datasets (wannier90.rs:1230-1518),
+closed forms and each other.  Storage comparisons are bit for bit (the loader
+parses numbers with correct rounding, the oracle with Python's float);
+fractional positions, which go through a 3 x 3 inverse, agree to 1e-14."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import rustb_amd as ra
+from oracle import wannier90_ref as OW
+from rustb_amd import workloads as W
+
+UNIT_WIN = "begin unit_cell_cart\n1.0 0.0 0.0\n0.0 1.0 0.0\n0.0 0.0 1.0\nend unit_cell_cart\n\n"
+ONE_S_HR = "generated\n1\n1\n1\n0 0 0 1 1 0.0 0.0\n"
+
+
+def write(d, files, seed="seedname"):
+    os.makedirs(d, exist_ok=True)
+    for suffix, content in files.items():
+        mode = "wb" if isinstance(content, bytes) else "w"
+        with open(os.path.join(d, seed + suffix), mode) as f:
+            f.write(content)
+    return d + "/"
+
+
+class Loaded:
+    """One view of a loaded model from either implementation."""
+
+    def __init__(self, impl, path, seed="seedname", ze=0.0, spin=None, threads=0):
+        if impl == "loader":
+            m = ra.Model.from_hr(path, seed, ze, spin=spin, n_threads=threads)
+            self.spin, self.lat, self.orb, self.hamR, self.ham = m.spin, m.lat, m.orb, m.hamR, m.ham
+            self.atoms = [(a.species, a.position, a.orbitals) for a in m.atoms]
+            self.orb_projection = m.orb_projection
+        else:
+            m = OW.from_hr(path, seed, ze, spin=spin)
+            self.spin, self.lat, self.orb, self.hamR, self.ham = m.spin, m.lat, m.orb, m.hamR, m.ham
+            self.atoms = m.atoms
+            self.orb_projection = m.orb_projection
+        self.norb = self.orb.shape[0]
+        self.natom = len(self.atoms)
+
+
+def load_error(impl, path, seed="seedname", ze=0.0, spin=None):
+    """(kind, message) of the failure."""
+    try:
+        Loaded(impl, path, seed, ze, spin)
+    except ra.TbError as e:
+        return e.kind, str(e)
+    except OW.W90Error as e:
+        return e.kind, str(e)
+    raise AssertionError("expected an error")
+
+
+IMPLS = ["loader", "oracle"]
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def cdiv(b, w):
+    """num_complex's Complex / f64: each part divided (numpy's complex / float
+    divides as complex numbers and can differ in the last bit)."""
+    out = np.empty(b.shape, complex)
+    out.real = b.real / float(w)
+    out.imag = b.imag / float(w)
+    return out
+
+
+# ------------------------------------------- the reference's regression tests
+@pytest.mark.parametrize("impl", IMPLS)
+def test_loads_minimal_dataset(impl, tmp_path):
+    """wannier90.rs:1236-1245."""
+    xyz = "2\nWannier centres\nC 0.0 0.0 0.0\nC 0.0 0.0 0.0\n"
+    p = write(str(tmp_path), {".win": UNIT_WIN + "begin projections\nC:s\nend projections\n", "_hr.dat": ONE_S_HR,
+                              "_centres.xyz": xyz})
+    m = Loaded(impl, p)
+    assert (m.norb, m.natom) == (1, 1)
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("xyz", ["3\nWannier centres\nX 0.0 0.0 0.0\nC 0.1 0.0 0.0\nCs 0.2 0.0 0.0\n",
+                                 "4\nWannier centres\nX 0.0 0.0 0.0\nC 0.1 0.0 0.0\nCs 0.2 0.0 0.0\nCs 0.3 0.0 0.0\n"])
+def test_drops_xyz_atoms_without_orbitals(impl, xyz, tmp_path):
+    """wannier90.rs:1247-1294: Cs atoms without projections are dropped."""
+    p = write(str(tmp_path), {".win": UNIT_WIN + "begin projections\nC:s\nend projections\n", "_hr.dat": ONE_S_HR,
+                              "_centres.xyz": xyz})
+    m = Loaded(impl, p)
+    assert (m.norb, m.natom) == (1, 1)
+    assert m.atoms[0][0] == "C"
+    assert np.allclose(m.atoms[0][1], [0.1, 0, 0], atol=1e-15)
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_without_xyz_drops_atom_without_orbitals(impl, tmp_path):
+    """wannier90.rs:1296-1312."""
+    win = UNIT_WIN + "begin atoms_cart\nC 0.0 0.0 0.0\nCs 0.1 0.0 0.0\nend atoms_cart\n\nbegin projections\nC:s\nend projections\n"
+    m = Loaded(impl, write(str(tmp_path), {".win": win, "_hr.dat": ONE_S_HR}))
+    assert (m.norb, m.natom) == (1, 1)
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_rejects_projection_species_missing_from_xyz(impl, tmp_path):
+    """wannier90.rs:1314-1328."""
+    xyz = "2\nWannier centres\nC 0.0 0.0 0.0\nFe 0.0 0.0 0.0\n"
+    p = write(str(tmp_path), {".win": UNIT_WIN + "begin projections\nC:s\nend projections\n", "_hr.dat": ONE_S_HR,
+                              "_centres.xyz": xyz})
+    kind, msg = load_error(impl, p)
+    assert kind == "FileParse" and "species mismatch" in msg
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_rejects_unknown_projection_species(impl, tmp_path):
+    """wannier90.rs:1330-1350."""
+    p = write(str(tmp_path), {".win": UNIT_WIN + "begin projections\nXx:s\nend projections\n", "_hr.dat": ONE_S_HR,
+                              "_centres.xyz": "1\nWannier centres\nC 0.0 0.0 0.0\n"})
+    kind, msg = load_error(impl, p)
+    assert kind == "FileParse" and "Unknown atomic species 'Xx' in begin projections" in msg
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("line,spin", [("spinors = .true.", True), ("# spinors = .true.", False),
+                                       ("#spinors=true", False), ("spinors = .true. ! enable spin", True),
+                                       ("spinors = .false.", False)])
+def test_win_comment_stripping(impl, line, spin, tmp_path):
+    """wannier90.rs:1352-1365 (win_data_line), seen through the spin flag."""
+    win = UNIT_WIN + f"{line}\nbegin atoms_cart\nFe 0.0 0.0 0.0 # position\nend atoms_cart\n" \
+                     "begin projections\nFe:s\nend projections\n"
+    hr = "generated\n2\n1\n1\n" + "".join(f"0 0 0 {i} {j} 0.0 0.0\n" for j in (1, 2) for i in (1, 2)) if spin \
+        else ONE_S_HR
+    m = Loaded(impl, write(str(tmp_path), {".win": win, "_hr.dat": hr}))
+    assert m.spin == spin and m.norb == 1
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_without_centres_uses_atoms_cart(impl, tmp_path):
+    """wannier90.rs:1367-1383."""
+    win = UNIT_WIN + "begin atoms_cart\nFe 0.0 0.0 0.0\nend atoms_cart\n\nbegin projections\nFe:s\nend projections\n"
+    m = Loaded(impl, write(str(tmp_path), {".win": win, "_hr.dat": ONE_S_HR}))
+    assert (m.norb, m.natom) == (1, 1)
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_without_xyz_multiple_atoms_of_one_species(impl, tmp_path):
+    """wannier90.rs:1385-1413: two Fe atoms with Fe:s and Fe:p own 4 orbitals each."""
+    win = UNIT_WIN + "begin atoms_cart\nFe 0.0 0.0 0.0\nFe 1.0 0.0 0.0\nend atoms_cart\n\n" \
+                     "begin projections\nFe:s\nFe:p\nend projections\n"
+    hr = "generated\n8\n1\n1\n0 0 0 1 1 0.0 0.0\n" + "".join(
+        f"0 0 0 {i} {j} 0.0 0.0\n" for i in range(1, 9) for j in range(1, 9) if (i, j) != (1, 1))
+    m = Loaded(impl, write(str(tmp_path), {".win": win, "_hr.dat": hr}))
+    assert (m.norb, m.natom) == (8, 2)
+    assert [len(a[2]) for a in m.atoms] == [4, 4]
+    assert m.orb_projection == ["s", "pz", "px", "py"] * 2
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("unit,block,expected", [
+    ("BoHr", "ATOMS_CART\nBOHR\nFe 1.0 0.0 0.0 ! inline comment\nEND ATOMS_CART", 2.0 * OW.BOHR_TO_ANGSTROM),
+    ("AnG", "atoms_cart\nAngstrom\nFe 1.0 0.0 0.0\nend atoms_cart", 2.0),
+    ("Ang", "atoms_frac\nFe 0.5 0.0 0.0\nend atoms_frac", 2.0)])
+def test_coordinate_blocks_and_units(impl, unit, block, expected, tmp_path):
+    """wannier90.rs:1415-1455."""
+    win = f"BEGIN UNIT_CELL_CART\n{unit}\n2.0 0.0 0.0\n0.0 2.0 0.0\n0.0 0.0 2.0\nEND UNIT_CELL_CART\n\n" \
+          f"BEGIN {block}\n\nbegin projections\nFe:s\nend projections\n"
+    m = Loaded(impl, write(str(tmp_path), {".win": win, "_hr.dat": ONE_S_HR}))
+    assert (m.norb, m.natom) == (1, 1)
+    assert abs(m.lat[0, 0] - expected) < 1e-12
+    assert abs(m.atoms[0][1][0] - 0.5) < 1e-12
+    assert abs(m.orb[0, 0] - 0.5) < 1e-12
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_rejects_malformed_atom_rows(impl, tmp_path):
+    """wannier90.rs:1457-1471."""
+    win = "begin unit_cell_cart\n1 0 0\n0 1 0\n0 0 1\nend unit_cell_cart\n\nbegin atoms_cart\nFe 0.0 0.0\n" \
+          "end atoms_cart\n\nbegin projections\nFe:s\nend projections\n"
+    kind, _ = load_error(impl, write(str(tmp_path), {".win": win, "_hr.dat": ONE_S_HR}))
+    assert kind == "FileParse"
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_rejects_truncated_centres(impl, tmp_path):
+    """wannier90.rs:1473-1497."""
+    win = "begin unit_cell_cart\n1 0 0\n0 1 0\n0 0 1\nend unit_cell_cart\n\nbegin projections\nFe:s,p\nend projections\n"
+    hr = "generated\n4\n1\n1\n" + "".join(f"0 0 0 {i} {j} 0.0 0.0\n" for i in range(1, 5) for j in range(1, 5))
+    xyz = "5\nWannier centres\nX 0.0 0.0 0.0\nFe 0.0 0.0 0.0\n"
+    kind, msg = load_error(impl, write(str(tmp_path), {".win": win, "_hr.dat": hr, "_centres.xyz": xyz}))
+    assert kind == "FileParse" and "Truncated" in msg
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_merges_multi_line_projections(impl, tmp_path):
+    """wannier90.rs:1499-1517."""
+    win = UNIT_WIN + "begin atoms_cart\nFe 0.0 0.0 0.0\nend atoms_cart\n\nbegin projections\nFe:s\nFe:p\nend projections\n"
+    hr = "generated\n4\n1\n1\n" + "".join(f"0 0 0 {i} {j} 0.0 0.0\n" for i in range(1, 5) for j in range(1, 5))
+    m = Loaded(impl, write(str(tmp_path), {".win": win, "_hr.dat": hr}))
+    assert (m.norb, m.natom) == (4, 1)
+    assert len(m.atoms[0][2]) == 4
+
+
+# ------------------------------------------------------------ closed forms
+def hr_text(nsta, Rs, blocks, weights, fmt=repr, comment="closed form"):
+    """_hr.dat of blocks[r][m][n] (values as written, before the division),
+    lines n outer, m inner."""
+    out = [comment, str(nsta), str(len(Rs))]
+    for a in range(0, len(weights), 15):
+        out.append(" ".join(str(w) for w in weights[a:a + 15]))
+    for R, b in zip(Rs, blocks):
+        for n in range(nsta):
+            for m in range(nsta):
+                v = b[m, n]
+                out.append(f"{R[0]} {R[1]} {R[2]} {m + 1} {n + 1} {fmt(float(v.real))} {fmt(float(v.imag))}")
+    return "\n".join(out) + "\n"
+
+
+def one_atom_win(nsta):
+    return UNIT_WIN + "begin atoms_frac\nFe 0.0 0.0 0.0\nend atoms_frac\n" \
+                      "begin projections\nFe:" + ",".join(["s"] * nsta) + "\nend projections\n"
+
+
+@pytest.mark.parametrize("zero_at", ["middle", "first", "last", "absent", "twice"])
+def test_storage_closed_form(zero_at, tmp_path):
+    """Non-Hermitian blocks, random degeneracies and a Fermi level: the element
+    written on line (n outer, m inner) of R is ham[R][m][n] / ndegen[R], R = 0
+    is row 0 (the last R = 0 block when repeated; zeros when absent), the other
+    R follow in file order (a repeated nonzero R is kept twice), and E_F is
+    subtracted on the R = 0 diagonal only.  Bit for bit, both implementations."""
+    rng = np.random.default_rng(hash(zero_at) % 1000)
+    n = 5
+    Rs = [tuple(int(x) for x in rng.integers(-3, 4, 3)) for _ in range(8)]
+    Rs = [R for R in Rs if R != (0, 0, 0)][:6]
+    Rs.append(Rs[1])  # a repeated nonzero R
+    if zero_at == "middle":
+        Rs.insert(3, (0, 0, 0))
+    elif zero_at == "first":
+        Rs.insert(0, (0, 0, 0))
+    elif zero_at == "last":
+        Rs.append((0, 0, 0))
+    elif zero_at == "twice":
+        Rs.insert(1, (0, 0, 0))
+        Rs.insert(5, (0, 0, 0))
+    blocks = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in Rs]
+    blocks[0][1, 2] = -0.0  # signed zeros survive
+    weights = [int(w) for w in rng.integers(1, 9, len(Rs))]
+    ze = 0.3712
+    p = write(str(tmp_path), {"_hr.dat": hr_text(n, Rs, blocks, weights), ".win": one_atom_win(n)})
+    exp_R, exp_H = [(0, 0, 0)], [np.zeros((n, n), complex)]
+    for R, b, w in zip(Rs, blocks, weights):
+        if R == (0, 0, 0):
+            exp_H[0] = cdiv(b, w)
+        else:
+            exp_R.append(R)
+            exp_H.append(cdiv(b, w))
+    exp_H = np.array(exp_H)
+    exp_H[0][np.diag_indices(n)] -= ze
+    for impl in IMPLS:
+        m = Loaded(impl, p, ze=ze, threads=1)
+        assert np.array_equal(m.hamR, np.array(exp_R)), impl
+        assert np.array_equal(bits(m.ham), bits(exp_H)), impl
+
+
+def test_storage_matches_oracle_any_thread_count(tmp_path):
+    """A file big enough to be split over threads: every thread count gives the
+    same bits, equal to the oracle's, with R = 0 away from the middle (the
+    reordering path) and in the middle (the in-place path)."""
+    rng = np.random.default_rng(5)
+    n = 12
+    Rs = [(a, b, c) for a in range(-2, 3) for b in range(-2, 3) for c in range(-1, 2)]  # 75 R, 0 in the middle
+    blocks = [np.round(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)), 6) for _ in Rs]
+    weights = [int(w) for w in rng.integers(1, 5, len(Rs))]
+    for order in ("sorted", "shuffled"):
+        perm = np.arange(len(Rs)) if order == "sorted" else rng.permutation(len(Rs))
+        p = write(str(tmp_path / order), {
+            "_hr.dat": hr_text(n, [Rs[i] for i in perm], [blocks[i] for i in perm], [weights[i] for i in perm],
+                               fmt=lambda x: f"{x:12.6f}"),
+            ".win": one_atom_win(n)})
+        ref = Loaded("oracle", p, ze=0.25)
+        for T in (1, 2, 3, 8, 16):
+            m = Loaded("loader", p, ze=0.25, threads=T)
+            assert np.array_equal(m.hamR, ref.hamR), (order, T)
+            assert np.array_equal(bits(m.ham), bits(ref.ham)), (order, T)
+
+
+def test_simple_cubic_band_closed_form(tmp_path):
+    """One s orbital, on-site eps and hopping t to the 6 neighbours, written in
+    Wannier90's own layout: E(k) = eps - 2t(cos 2pi k1 + cos 2pi k2 + cos 2pi k3)
+    from the stored blocks, both implementations."""
+    eps, t, ef = 0.3, 0.25, 0.1
+    Rs = [(0, 0, -1), (0, -1, 0), (-1, 0, 0), (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    hamR = np.array(Rs)
+    ham = np.array([[[eps]] if R == (0, 0, 0) else [[-t]] for R in Rs], complex)
+    prefix = str(tmp_path / "sc")
+    W.write_hr(prefix + "_hr.dat", hamR, ham, fermi=ef)
+    with open(prefix + ".win", "w") as f:
+        f.write(one_atom_win(1))
+    k = np.random.default_rng(1).random((50, 3))
+    exact = eps - 2 * t * np.cos(2 * np.pi * k).sum(axis=1)
+    for impl in IMPLS:
+        m = Loaded(impl, str(tmp_path) + "/", "sc", ze=ef)
+        Hk = np.einsum("rij,kr->kij", m.ham, np.exp(2j * np.pi * k @ m.hamR.T))
+        assert np.abs(Hk[:, 0, 0] - exact).max() < 1e-14, impl
+
+
+def test_writer_layout_is_wannier90s(tmp_path):
+    """write_hr emits '(5I5,2F12.6)' lines, n outer and m inner, 15 weights per
+    line, and reads back to the values rounded to 6 decimals."""
+    rng = np.random.default_rng(2)
+    hamR = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0]])
+    ham = rng.standard_normal((3, 2, 2)) + 1j * rng.standard_normal((3, 2, 2))
+    W.write_hr(str(tmp_path / "x_hr.dat"), hamR, ham, ndegen=[1, 2, 3])
+    lines = open(tmp_path / "x_hr.dat").read().split("\n")
+    assert lines[3] == "    1    2    3"
+    assert lines[5] == f"    0    0    0    2    1{ham[0, 1, 0].real:12.6f}{ham[0, 1, 0].imag:12.6f}"
+    assert len(lines[5]) == 49
+    with open(tmp_path / "x.win", "w") as f:
+        f.write(one_atom_win(2))
+    m = Loaded("loader", str(tmp_path) + "/", "x")
+    assert np.abs(m.ham - ham).max() < 5e-7 * 3
+
+
+# ------------------------------------------------------------ number grammar
+GOOD = ["1.", ".5", "-0.0", "+2.5e-3", "1E2", "0.1", "3.14159265358979323846264338327950288",
+        "12345678901234567890123e-20", "4.9e-324", "2.2250738585072011e-308", "1e400", "-1e-400",
+        "9007199254740993.0", "9007199254740995.0", "1e22", "1e23", "123456.789e-3", "0.000000",
+        "1.7976931348623157e308", "00000000000000000000000001.5", "inf", "-Infinity", "NaN", "7"]
+BAD = ["1.0D-03", "0x10", "1e", "e5", ".", "+", "1_0", "1.2.3", "--1", "1e+", "infinit", "１.0"]
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+def test_float_grammar_and_rounding(impl, tmp_path):
+    """Tokens Rust's f64::from_str accepts, rounded correctly (against Python's
+    correctly rounded float()), through the real part of the R = 0 element."""
+    n = len(GOOD)
+    lines = ["g", str(n), "1", "1"]
+    for c in range(n):
+        for r in range(n):
+            lines.append(f"0 0 0 {r + 1} {c + 1} {GOOD[c] if r == c else '0.5'} -0.0")
+    p = write(str(tmp_path), {"_hr.dat": "\n".join(lines) + "\n", ".win": one_atom_win(n)})
+    m = Loaded(impl, p)
+    got = np.diag(m.ham[0]).real
+    want = np.array([float(g) for g in GOOD])
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(bits(got[~nan]), bits(want[~nan]))
+    assert np.all(np.signbit(np.diag(m.ham[0]).imag))
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("tok", BAD)
+def test_float_grammar_rejects(impl, tok, tmp_path):
+    hr = f"g\n1\n1\n1\n0 0 0 1 1 {tok} 0.0\n"
+    kind, msg = load_error(impl, write(str(tmp_path), {"_hr.dat": hr, ".win": one_atom_win(1)}))
+    assert kind == "FileParse" and "Failed to parse Hamiltonian real part: invalid float literal" in msg
+
+
+# ------------------------------------------------------------ error paths
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("case", ["missing_hr", "missing_win", "bad_nsta", "bad_weight", "short_line",
+                                  "bad_R", "truncated", "not_utf8", "bad_projection", "spin_mismatch",
+                                  "far_orbital", "singular_lattice", "odd_spin_nsta"])
+def test_errors(impl, case, tmp_path):
+    win1 = one_atom_win(1)
+    files = {"_hr.dat": ONE_S_HR, ".win": win1}
+    spin = None
+    expect = "FileParse"
+    if case == "missing_hr":
+        files.pop("_hr.dat")
+        expect = "FileCreation"
+    elif case == "missing_win":
+        files.pop(".win")
+        expect = "FileCreation"
+    elif case == "bad_nsta":
+        files["_hr.dat"] = "g\n1x\n1\n1\n0 0 0 1 1 0.0 0.0\n"
+    elif case == "bad_weight":
+        files["_hr.dat"] = "g\n1\n1\n-1\n0 0 0 1 1 0.0 0.0\n"
+    elif case == "short_line":
+        files["_hr.dat"] = "g\n1\n1\n1\n0 0 0 1 1 0.0\n"
+    elif case == "bad_R":
+        files["_hr.dat"] = "g\n1\n1\n1\n0 0.5 0 1 1 0.0 0.0\n"
+    elif case == "truncated":
+        files["_hr.dat"] = "g\n2\n2\n1 1\n" + "".join(f"0 0 0 {i} {j} 0.0 0.0\n" for i in (1, 2) for j in (1, 2))
+        files[".win"] = one_atom_win(2)
+    elif case == "not_utf8":
+        files["_hr.dat"] = ONE_S_HR.encode() + b"\xff\n"
+        expect = "Io"
+    elif case == "bad_projection":
+        files[".win"] = UNIT_WIN + "begin projections\nFe:q\nend projections\n"
+        expect = "InvalidOrbitalProjection"
+    elif case == "spin_mismatch":
+        spin = True
+        expect = "SpinMismatch" if impl == "loader" else "Other"
+    elif case == "far_orbital":
+        files["_centres.xyz"] = "2\nc\nX 0.5 0.0 0.0\nFe 0.0 0.0 0.0\n"
+        files[".win"] = UNIT_WIN + "begin projections\nFe:s\nend projections\n"
+        expect = "InvalidModelInvariant"
+    elif case == "singular_lattice":
+        files[".win"] = "begin unit_cell_cart\n1 0 0\n2 0 0\n0 0 1\nend unit_cell_cart\n" + \
+                        win1[win1.index("begin atoms_frac"):]
+        expect = "Linalg"
+    elif case == "odd_spin_nsta":
+        files["_hr.dat"] = "g\n3\n1\n1\n" + "".join(f"0 0 0 {i} {j} 0.0 0.0\n" for i in (1, 2, 3) for j in (1, 2, 3))
+        files[".win"] = "spinors = true\n" + one_atom_win(1)
+        expect = "InvalidModelInvariant"
+    kind, msg = load_error(impl, write(str(tmp_path), files), spin=spin)
+    assert kind == expect, msg
+
+
+def test_first_error_in_file_order_across_threads(tmp_path):
+    """Two bad lines far apart: every thread count reports the earlier one; on
+    one line the R fields fail before the matrix fields."""
+    n = 16
+    rows = [f"{R} 0 0 {m} {k} 0.1 0.2" for R in range(-20, 21) for k in range(1, n + 1) for m in range(1, n + 1)]
+    first = 7 * n * n + 5  # a matrix field
+    later = 30 * n * n  # the R field of a block start
+    rows[first] = rows[first].replace("0.1", "0.1x")
+    rows[later] = "1.5 " + rows[later].split(" ", 1)[1]
+    body = "\n".join(rows) + "\n"
+    hdr = f"g\n{n}\n41\n" + " ".join(["1"] * 41) + "\n"
+    p = write(str(tmp_path), {"_hr.dat": hdr + body, ".win": one_atom_win(n)})
+    for T in (1, 4, 16):
+        with pytest.raises(ra.TbError, match="real part: invalid float literal"):
+            ra.Model.from_hr(p, "seedname", 0.0, n_threads=T)
+    rows[first] = rows[first].replace("0.1x", "0.1")
+    rows[later] = rows[later].replace("0.1", "0.1x")  # same line: R fails first
+    p = write(str(tmp_path), {"_hr.dat": hdr + "\n".join(rows) + "\n"})
+    for T in (1, 16):
+        with pytest.raises(ra.TbError, match="Failed to parse R vector"):
+            ra.Model.from_hr(p, "seedname", 0.0, n_threads=T)
+
+
+# ------------------------------------------------------------ _wsvec.dat
+def test_wsvec_hand_computed(tmp_path):
+    """Two orbitals, R in {0, +x, -x}; the (+x, 1->2) hopping is split over
+    shifts (0,0,0) and (-2,0,0) (equivalent in a 2-cell supercell sense), and
+    an entry with a new R (+2x) appears: the storage is what the shifts say,
+    new R appended in order of first appearance, both implementations."""
+    n = 2
+    Rs = [(-1, 0, 0), (0, 0, 0), (1, 0, 0)]
+    blocks = [np.array([[1 + 1j, 2 - 1j], [3, 4j]]), np.array([[5, 6 + 2j], [6 - 2j, 7]], complex),
+              np.array([[1 - 1j, 3], [2 + 1j, -4j]])]
+    ws = ["## header"]
+    for R in Rs:
+        for i in (1, 2):
+            for j in (1, 2):
+                ws.append(f"{R[0]} {R[1]} {R[2]} {i} {j}")
+                if R == (1, 0, 0) and (i, j) == (1, 2):
+                    ws += ["2", "0 0 0", "-2 0 0"]
+                elif R == (-1, 0, 0) and (i, j) == (2, 2):
+                    ws += ["3", "0 0 0", "3 0 0", "0 0 0"]
+                else:
+                    ws += ["1", "0 0 0"]
+    p = write(str(tmp_path), {"_hr.dat": hr_text(n, Rs, blocks, [1, 1, 1]), ".win": one_atom_win(n),
+                              "_wsvec.dat": "\n".join(ws) + "\n"})
+    # expected, by hand: new R order 0, -x, +2x, +x
+    exp_R = [(0, 0, 0), (-1, 0, 0), (2, 0, 0), (1, 0, 0)]
+    exp = np.zeros((4, n, n), complex)
+    exp[0] = blocks[1]
+    exp[1] = blocks[0]
+    exp[1, 1, 1] = 2 * (blocks[0][1, 1] / 3)  # two of three shifts stay at -x
+    exp[2, 1, 1] = blocks[0][1, 1] / 3
+    exp[3] = blocks[2]
+    exp[3, 0, 1] = blocks[2][0, 1] / 2
+    exp[1, 0, 1] += blocks[2][0, 1] / 2  # +x - 2x = -x
+    for impl in IMPLS:
+        m = Loaded(impl, p)
+        assert [tuple(r) for r in m.hamR] == exp_R, impl
+        assert np.allclose(m.ham, exp, atol=1e-15, rtol=0), impl
+    lo, orc = Loaded("loader", p), Loaded("oracle", p)
+    assert np.array_equal(bits(lo.ham), bits(orc.ham))
+
+
+def test_wsvec_preserves_hamiltonian_at_gamma(tmp_path):
+    """Shifts redistribute each hopping over equivalent R, so H(k = 0) (the
+    plain sum over R) is unchanged; loader and oracle agree bit for bit."""
+    rng = np.random.default_rng(9)
+    n = 4
+    Rs = [(a, b, 0) for a in (-1, 0, 1) for b in (-1, 0, 1)]
+    blocks = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in Rs]
+    ws = ["## header"]
+    for R in Rs:
+        for i in range(1, n + 1):
+            for j in range(1, n + 1):
+                k = int(rng.integers(1, 4))
+                ws += [f"{R[0]} {R[1]} {R[2]} {i} {j}", str(k)]
+                ws += [f"{int(rng.integers(-1, 2)) * 3} {int(rng.integers(-1, 2)) * 3} 0" for _ in range(k)]
+    p = write(str(tmp_path), {"_hr.dat": hr_text(n, Rs, blocks, [1] * 9), ".win": one_atom_win(n),
+                              "_wsvec.dat": "\n".join(ws) + "\n"})
+    lo, orc = Loaded("loader", p), Loaded("oracle", p)
+    assert np.array_equal(lo.hamR, orc.hamR)
+    assert np.array_equal(bits(lo.ham), bits(orc.ham))
+    assert np.abs(lo.ham.sum(axis=0) - sum(blocks)).max() < 1e-13
+
+
+# ------------------------------------------------------------ spinful, centres
+def test_spinful_model_with_centres_round_trip(tmp_path):
+    """A spinful synthetic model written with _centres.xyz: the loaded
+    storage is the model's own rounded to F12.6, orbitals at the centres."""
+    m0 = ra.synth_model(2, 3, True, 4, 1, 21, scale=1.0, decay=1.5)
+    prefix = str(tmp_path / "sp")
+    W.write_wannier90(m0, prefix, centres=True, fermi=1.25)
+    lo = Loaded("loader", str(tmp_path) + "/", "sp", ze=1.25)
+    orc = Loaded("oracle", str(tmp_path) + "/", "sp", ze=1.25)
+    assert lo.spin and lo.norb == 4 and lo.natom == 4
+    assert np.array_equal(bits(lo.ham), bits(orc.ham)) and np.array_equal(lo.hamR, orc.hamR)
+    assert np.abs(lo.orb - m0.orb).max() < 1e-14 and np.abs(lo.orb - orc.orb).max() < 1e-14
+    index = {tuple(R): i for i, R in enumerate(m0.hamR)}
+    err = max(np.abs(lo.ham[i] - m0.ham[index[tuple(R)]]).max() for i, R in enumerate(lo.hamR))
+    assert err <= 5e-7 * math.sqrt(2) + 1e-12
