@@ -320,8 +320,12 @@ def gen_v(model: Model, k, gauge: str = "atom"):
 # Eigensolve (ndarray_lapack.rs:412-448, solve_ham.rs:52-63,110-129)
 # --------------------------------------------------------------------------
 def eigvalsh(H):
-    """eigvalsh_v -> LAPACK zheev('N') (ndarray_lapack.rs:412-448)."""
-    w, _, info = _lapack.zheev(np.asarray(H, np.complex128), compute_v=0)
+    """eigvalsh_v(&hamk, UPLO::Upper) -> LAPACK zheev('N', 'U')
+    (solve_ham.rs:61, ndarray_lapack.rs:412-448).  The row-major buffer is
+    handed to LAPACK as column-major, i.e. H^T, so 'U' reads the LOWER
+    triangle of H (H[i][j], i >= j): for an H that is Hermitian only to
+    rounding this is the matrix whose eigenvalues the reference returns."""
+    w, _, info = _lapack.zheev(np.asarray(H, np.complex128), compute_v=0, lower=1)
     if info != 0:
         raise OracleError("EigenvalueComputationFailed", f"info={info}")
     return w
@@ -333,8 +337,10 @@ def eigh(H):
 
     The reference's ``hamk.eigh(UPLO::Lower)`` hands back ``conj(U)`` because
     the row-major buffer is read column-major (``evec_transform_sanity``,
-    lib.rs:1231-1277: ``E_vec^T H E_vec^* = diag``)."""
-    w, v, info = _lapack.zheev(np.asarray(H, np.complex128), compute_v=1)
+    lib.rs:1231-1277: ``E_vec^T H E_vec^* = diag``); for the same reason its
+    'L' reads the UPPER triangle of H (H[i][j], i <= j), as zheev('U') on H
+    does here."""
+    w, v, info = _lapack.zheev(np.asarray(H, np.complex128), compute_v=1, lower=0)
     if info != 0:
         raise OracleError("EigenvalueComputationFailed", f"info={info}")
     return w, v

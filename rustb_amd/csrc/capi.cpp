@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 
 #include "eigh.hpp"
@@ -253,15 +254,17 @@ int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
 // iteration / back-transformation (the FP64 buffers of `eb` reused as FP32
 // storage) into Z, then FP64 refinement against the FP64 H.  The refined
 // eigenvectors end in W2 as columns that are unit after multiplication by
-// scale[k][j]; Z, W1 and W3 are scratch afterwards.
-int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, double* E, EigBufs& eb, double* Z,
-                    double* W1, double* W2, double* W3, double* scale) {
+// scale[k][j]; Z, W1 and W3 are scratch afterwards.  from_upper: H is first
+// rebuilt from its upper triangle (models whose storage is not exactly
+// Hermitian), so the solve and the refinement see the reference's matrix.
+int eig_mixed_chunk(hipStream_t st, Timer& tmr, bool from_upper, int n, size_t c, double* H, double* E, EigBufs& eb,
+                    double* Z, double* W1, double* W2, double* W3, double* scale) {
   float* Vg = reinterpret_cast<float*>(eb.Vg);
   float* tau = reinterpret_cast<float*>(eb.tau);
   float* Y = reinterpret_cast<float*>(eb.Y);
   float* S = reinterpret_cast<float*>(eb.S);
   float* DE = reinterpret_cast<float*>(eb.DE);
-  RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE));
+  RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE, from_upper));
   RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
   float* Zf = reinterpret_cast<float*>(Z);  // exact: the route's values are FP32
   RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Zf, false, scale));  // |z_j|^2 into scale
@@ -278,9 +281,32 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, int n, size_t c, double* H, doub
 // two products instead of four, and the band sum reads A, B out of C.
 int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const double* V, const double* Z,
                const double* zscale, const double* E, int spin_dir, double eta, double* T1, double* C, double* J,
-               double* omega) {
+               double* omega, double* B = nullptr) {
   const int n = m->nsta;
   const size_t n2 = static_cast<size_t>(n) * n;
+  if (B) {
+    // A = U^H J_a U and B = U^H v_b U as the reference forms them (V holds
+    // V0, V1 apart), then its band sum term by term; U = Z diag(zscale)
+    const double* Ma = V;
+    size_t sMa = 2 * n2;
+    if (m->spin && spin_dir != RTB_SPIN_NONE) {
+      RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+      RTB_TRY(spin_current(st, m->norb, K, spin_dir, V, 2 * n2 * 2, nullptr, J));
+      RTB_TRY(tmr.end(1));
+      Ma = J;
+      sMa = n2;
+    }
+    RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
+    RTB_TRY(zgemm_batched(st, n, K, Ma, sMa, false, Z, n2, T1, n2));
+    RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, C, n2, false, zscale));
+    RTB_TRY(zgemm_batched(st, n, K, V + n2 * 2, 2 * n2, false, Z, n2, T1, n2));
+    RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, B, n2, false, zscale));
+    RTB_TRY(tmr.end(4, 4 * flop_zgemm(n, K), 4 * mflop_zgemm(n, K, false)));
+    RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+    RTB_TRY(kubo_omega_exact(st, n, K, C, B, E, eta, omega));
+    RTB_TRY(tmr.end(1));
+    return RTB_OK;
+  }
   // the operator pair as one matrix M = A_op + i B_op: V holds V0 + i V1
   // (combined by the Fourier stage), or V0, V1 apart when the first
   // operator is the spin current J(V0)
@@ -307,6 +333,31 @@ int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const d
 }  // namespace
 
 void publish_stats(const rtb_stats& s) { g_stats = s; }
+
+// ham[-R] == ham[R]^H exactly for every stored R (the builder's storage, by
+// construction); models read from files are Hermitian only to their printed
+// precision.  Decides whether the velocity operators may be combined into
+// one Kubo product (exact for Hermitian operators only).
+bool storage_is_hermitian(int dim, int n, int n_R, const int64_t* hamR, const double* ham) {
+  std::map<std::vector<int64_t>, int> index;
+  for (int r = 0; r < n_R; ++r) index.emplace(std::vector<int64_t>(hamR + r * dim, hamR + (r + 1) * dim), r);
+  const size_t n2 = static_cast<size_t>(n) * n;
+  for (int r = 0; r < n_R; ++r) {
+    std::vector<int64_t> neg(dim);
+    for (int d = 0; d < dim; ++d) neg[d] = -hamR[r * dim + d];
+    auto it = index.find(neg);
+    if (it == index.end()) return false;
+    const double* a = ham + r * n2 * 2;
+    const double* b = ham + it->second * n2 * 2;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double* x = a + (static_cast<size_t>(i) * n + j) * 2;
+        const double* y = b + (static_cast<size_t>(j) * n + i) * 2;
+        if (x[0] != y[0] || x[1] != -y[1]) return false;
+      }
+  }
+  return true;
+}
 
 void set_error(const char* fmt, ...) {
   char buf[1024];
@@ -368,6 +419,7 @@ int rtb_model_create(int dim, int spin, int norb, int n_R, const int64_t* hamR, 
     return rc;
   };
   if (hipGetDevice(&m->device) != hipSuccess) return cleanup(fail(RTB_ERR_DEVICE, "no HIP device"));
+  m->herm = storage_is_hermitian(dim, n, n_R, hamR, ham);
   std::vector<double> R(static_cast<size_t>(n_R) * 3, 0.0), Rc(static_cast<size_t>(n_R) * 3, 0.0);
   for (int r = 0; r < n_R; ++r)
     for (int d = 0; d < dim; ++d) {
@@ -503,12 +555,12 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     RTB_TRY(tmr.end(1));
     if (f32) {
       // W2, W3: the reflector and inverse-iteration scratch, dead by then
-      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S, dsc));
+      RTB_TRY(eig_mixed_chunk(st, tmr, !m->herm, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S, dsc));
       RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
       RTB_TRY(transpose_batched(st, n, c, eb.Vg, dsc, dZ));
       RTB_TRY(tmr.end(1));
     } else {
-      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
+      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, vec && !m->herm));
       if (vec) {
         RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
         RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, true));
@@ -606,9 +658,13 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   a.gauge = RTB_GAUGE_ATOM;  // berry_curvature_at_impl: gen_v_projected(k, Gauge::Atom, ..)
   const size_t n2 = static_cast<size_t>(n) * n;
   const bool spinj = m->spin && j.spin_dir != RTB_SPIN_NONE;
-  a.combine = spinj ? 0 : 1;  // V0 + i V1 straight from the Fourier stage
-  // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const bool f32 = j.precision == RTB_PREC_MIXED;  // FP32 eigensolve + FP64 refinement
+  // the plain form, and any model whose storage is not exactly Hermitian,
+  // forms the reference's A and B apart (V0, V1 kept apart); the mixed form
+  // of a Hermitian model takes V0 + i V1 straight from the Fourier stage
+  const bool exact_kubo = !f32 || !m->herm;
+  a.combine = spinj || exact_kubo ? 0 : 1;
+  // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const size_t per_k =
       24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 + (f32 ? n2 * 16 + n * 8 : 0);
   const size_t K = chunk_for(j.nk, per_k);
@@ -674,16 +730,16 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     double* Zc = dZ;  // eigenvectors (columns) for the Kubo products
     double* Ab = A;   // scratch of the Kubo products
     if (f32) {
-      RTB_TRY(eig_mixed_chunk(st, tmr, n, c, dH, dE, eb, dZ, T1, A, B, dsc));
+      RTB_TRY(eig_mixed_chunk(st, tmr, !m->herm, n, c, dH, dE, eb, dZ, T1, A, B, dsc));
       Zc = A;  // refined vectors end in the A buffer; dZ is free
       Ab = dZ;
     } else {
-      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE));
+      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm));
       RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
       RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, false));
       RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     }
-    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, dJ, dOm));
+    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, dJ, dOm, exact_kubo ? B : nullptr));
     int l2 = 0;
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     if (j.mode == 0) {
@@ -851,7 +907,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
       RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
       RTB_TRY(tmr.end(1));
     }
-    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr, eb.DE));
+    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr, eb.DE, false));
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     const int nb = dos_blocks(c * n);
     RTB_TRY(dos_partials(st, c * n, dE, Emin, step, static_cast<int>(En), sigma,

@@ -11,6 +11,8 @@
 //   eigh_backtrans_reg_kernel  one workgroup per matrix: the eigenvectors of
 //                              T (generated from the twisted factorisations)
 //                              and Z = Q Y, register-resident
+#include <algorithm>
+
 #include "eigh.hpp"
 
 #include "eigh_core.hpp"
@@ -204,11 +206,70 @@ int launch_trieig(hipStream_t st, int n, size_t nmat, const T* Dg, const T* Eg, 
   return RTB_OK;
 }
 
+// The matrix the reference's eigh(UPLO::Lower) diagonalises for a row-major
+// H (solve_ham.rs:122, response/traits.rs:168): ndarray-linalg hands the
+// buffer to LAPACK as column-major, so LAPACK's lower triangle is the upper
+// triangle of H.  Its eigvalsh_v(UPLO::Upper) (solve_ham.rs:61,
+// ndarray_lapack.rs:412-448) passes the raw buffer, reading the lower one.
+// For an H that is Hermitian only to rounding (e.g. read from a Wannier90
+// file) the two differ, so the vector paths of such models first rebuild, in
+// place,
+// H[i][j] = conj(H[j][i]) for i > j and Im H[i][i] = 0 (what zheev uses of
+// the diagonal).  One 32 x 32 tile pair (I >= J) per workgroup: the upper
+// tile (J, I) is read by rows into LDS and written, conjugate-transposed,
+// by rows into the lower tile (I, J); matrices are strided over grid.y.
+constexpr int kHzTile = 32;
+__global__ void __launch_bounds__(256) hermitize_from_upper_kernel(int n, size_t nmat, double* __restrict__ H) {
+  __shared__ double2 t[kHzTile][kHzTile + 1];
+  int p = blockIdx.x, I = 0;
+  while (p > I) {
+    p -= I + 1;
+    ++I;
+  }
+  const int J = p;
+  const int tx = threadIdx.x % kHzTile, ty = threadIdx.x / kHzTile;  // 32 x 8
+  for (size_t mat = blockIdx.y; mat < nmat; mat += gridDim.y) {
+    double2* A = reinterpret_cast<double2*>(H) + mat * n * n;
+    for (int r = ty; r < kHzTile; r += 8) {
+      const int gi = J * kHzTile + r, gj = I * kHzTile + tx;
+      if (gi < n && gj < n) t[r][tx] = A[static_cast<size_t>(gi) * n + gj];
+    }
+    __syncthreads();
+    for (int r = ty; r < kHzTile; r += 8) {
+      const int gi = I * kHzTile + r, gj = J * kHzTile + tx;
+      if (gi < n && gj < n) {
+        double2* dst = A + static_cast<size_t>(gi) * n + gj;
+        if (gi > gj) {
+          const double2 v = t[tx][r];
+          *dst = make_double2(v.x, -v.y);
+        } else if (gi == gj) {
+          *dst = make_double2(t[tx][r].x, 0.0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int hermitize_from_upper(hipStream_t st, int n, size_t nmat, double* H) {
+  if (nmat == 0 || n < 1) return RTB_OK;
+  const int T = (n + kHzTile - 1) / kHzTile;
+  const unsigned gy = static_cast<unsigned>(std::min<size_t>(nmat, 65535));
+  hipLaunchKernelGGL(hermitize_from_upper_kernel, dim3(T * (T + 1) / 2, gy), dim3(256), 0, st, n, nmat, H);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
 template <typename T>
-int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors, T* Vg,
-                T* taug, T* Y, T* S, T* DE) {
+int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors, T* Vg,
+                T* taug, T* Y, T* S, T* DE, bool from_upper) {
   if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
   if (nmat == 0) return RTB_OK;
+  if (from_upper) {
+    if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REDUCE));
+    RTB_TRY(hermitize_from_upper(st, n, nmat, H));
+    if (tmr) RTB_TRY(tmr->end(1));
+  }
   T* Dg = DE;
   T* Eg = DE + nmat * n;
   T* TW = DE + 2 * nmat * n;
@@ -234,9 +295,9 @@ int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* 
 
 }  // namespace
 
-int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
-                     float* Vg, float* taug, float* Y, float* S, float* DE) {
-  return eigh_launch<float>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE);
+int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
+                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper) {
+  return eigh_launch<float>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper);
 }
 
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
@@ -244,9 +305,9 @@ int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, 
   return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, gdiag);
 }
 
-int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S, double* DE) {
-  return eigh_launch<double>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE);
+int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
+                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper) {
+  return eigh_launch<double>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper);
 }
 
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,

@@ -8,15 +8,20 @@ namespace rtb {
 // ---- eigensolver (eigh.hip) ----
 // Householder + bisection (+ the tridiagonal eigenvector data when
 // `vectors`: twisted factorisations, inverse iteration inside clusters).
+// The matrix is the Hermitian one of H's lower triangle (with Re of the
+// diagonal), as the reference's eigvalsh_v reads it; with from_upper H is
+// first rebuilt in place from its upper triangle, as the reference's eigh
+// reads it (the two agree to rounding when the model storage is exactly
+// Hermitian, so callers ask for it only for the other models).
 //   H: [nmat][n][n] complex; E: [nmat][n]; Vg: [nmat][n][n] complex;
 //   taug: [nmat][n] complex; Y: [nmat][n][n]; S: [nmat][3][n][n];
 //   DE: [3][nmat][n] the tridiagonal (diagonal, off-diagonal) and the twist
 //   indices; Vg, taug, Y, S, DE are the inputs of the back-transformation.
-int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S, double* DE);
+int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
+                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper);
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
-int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, const double* H, double* E, bool vectors,
-                     float* Vg, float* taug, float* Y, float* S, float* DE);
+int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
+                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper);
 // Z: [nmat][n][n] complex FP32 (the values of the FP32 route, exact);
 // gdiag (optional, [nmat][n]): the squared column norms of Z in FP64.
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
@@ -122,6 +127,10 @@ int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double
 // (sum over tiles of the column sums - sum of the row sums) / 2 from the
 // partial sums zgemm_kubo writes (tiles = ceil(n/64)).
 int kubo_omega_from_parts(hipStream_t st, int n, size_t nk, const double* part, double* omega);
+// The same band sum from A and B themselves ([nk][n][n] each, not assumed
+// Hermitian): Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2).
+int kubo_omega_exact(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E, double eta,
+                     double* omega);
 // out[k] = sum_n omega[k][n] f(E[k][n], mu, T)
 int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const double* E, double mu,
                  double T, double* out);

@@ -83,8 +83,10 @@ RTB_HD void tridiag_reg(Team& tm, int n, const double* Hm, RegSmem<T> sm, T* Dg,
   const int r = tm.id % NB, s = slice_of<NB>(tm.id);
   cplx<T> a[C];
   {
-    // A(r, c) = H[r][c] for r >= c, conj(H[c][r]) above the diagonal; the
-    // per-q addresses are base + q * stride
+    // A(r, c) = H[r][c] below the diagonal, conj(H[c][r]) above it and
+    // Re H[r][r] on it: the Hermitian matrix of H's lower triangle, as
+    // zheev('L') sees it (it ignores Im H[r][r]); the per-q addresses are
+    // base + q * stride
     const int rr = r < n ? r : 0;
     const double* lo = Hm + (static_cast<size_t>(rr) * n + s) * 2;  // H[r][s + S q]
     const double* up = Hm + (static_cast<size_t>(s) * n + rr) * 2;  // H[s + S q][r]
@@ -96,7 +98,7 @@ RTB_HD void tridiag_reg(Team& tm, int n, const double* Hm, RegSmem<T> sm, T* Dg,
       if (r < n && c < n) {
         if (r >= c) {
           const double* h = lo + q * (2 * S);
-          x = {static_cast<T>(h[0]), static_cast<T>(h[1])};
+          x = {static_cast<T>(h[0]), r == c ? T(0) : static_cast<T>(h[1])};
         } else {
           const double* h = up + q * ustep;
           x = {static_cast<T>(h[0]), -static_cast<T>(h[1])};

@@ -146,6 +146,64 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
   out[j] = s * scale;
 }
 
+// Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2) from A
+// and B as the reference forms them (berry_curvature_at_impl,
+// response/traits.rs:160-200), with no assumption that A and B are
+// Hermitian.  One k-point per workgroup (n <= 128): 32 x 32 tiles of A (row
+// tile I, column tile J) and of B (tile (J, I)) through LDS, the products
+// summed along each row of A.
+constexpr int kOmTile = 32;
+__global__ void __launch_bounds__(256) kubo_omega_exact_kernel(int n, const double2* __restrict__ A,
+                                                               const double2* __restrict__ B,
+                                                               const double* __restrict__ E, double eta,
+                                                               double* __restrict__ omega) {
+  __shared__ double2 sa[kOmTile][kOmTile + 1];
+  __shared__ double2 sb[kOmTile][kOmTile + 1];
+  __shared__ double se[kMaxEighN];
+  const size_t k = blockIdx.x;
+  const size_t nn = static_cast<size_t>(n) * n;
+  A += k * nn;
+  B += k * nn;
+  const int tx = threadIdx.x % kOmTile, ty = threadIdx.x / kOmTile;  // 32 x 8
+  for (int i = threadIdx.x; i < n; i += blockDim.x) se[i] = E[k * n + i];
+  const int T = (n + kOmTile - 1) / kOmTile;
+  const double eta2 = eta * eta;
+  for (int I = 0; I < T; ++I) {
+    double part[kOmTile / 8] = {};
+    for (int J = 0; J < T; ++J) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kOmTile / 8; ++q) {
+        const int r = ty + 8 * q;
+        const int ar = I * kOmTile + r, ac = J * kOmTile + tx;  // A[ar][ac]
+        sa[r][tx] = ar < n && ac < n ? A[static_cast<size_t>(ar) * n + ac] : make_double2(0.0, 0.0);
+        const int br = J * kOmTile + r, bc = I * kOmTile + tx;  // B[br][bc]
+        sb[r][tx] = br < n && bc < n ? B[static_cast<size_t>(br) * n + bc] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kOmTile / 8; ++q) {
+        const int r = ty + 8 * q;
+        const int row = I * kOmTile + r, col = J * kOmTile + tx;
+        if (row < n && col < n && row != col) {
+          const double2 a = sa[r][tx];  // A[row][col]
+          const double2 b = sb[tx][r];  // B[col][row]
+          const double im = a.x * b.y + a.y * b.x;
+          const double d = se[row] - se[col];
+          part[q] += -2.0 * im / (d * d + eta2);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kOmTile / 8; ++q) {
+      double v = part[q];
+      for (int o = kOmTile / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kOmTile);
+      const int row = I * kOmTile + ty + 8 * q;
+      if (tx == 0 && row < n) omega[k * n + row] = v;
+    }
+  }
+}
+
 }  // namespace
 
 int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride,
@@ -165,6 +223,17 @@ int kubo_omega_from_parts(hipStream_t st, int n, size_t nk, const double* part, 
   if (total == 0) return RTB_OK;
   hipLaunchKernelGGL(kubo_parts_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, n, nk,
                      (n + 63) / 64, part, omega);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int kubo_omega_exact(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E, double eta,
+                     double* omega) {
+  if (nk == 0) return RTB_OK;
+  if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "the band sum supports 1 <= nsta <= 128");
+  if (nk > 0x7fffffffu) return fail(RTB_ERR_UNSUPPORTED, "too many k-points in one launch");
+  hipLaunchKernelGGL(kubo_omega_exact_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
+                     reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B), E, eta, omega);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -115,3 +115,20 @@ def test_reg_clustered_spectra(emul, case):
     assert np.abs(E - w).max() < 1e-13
     assert np.abs(H @ Z - Z * E).max() < 1e-13
     assert np.abs(Z.conj().T @ Z - np.eye(n)).max() < 1e-13
+
+
+@pytest.mark.parametrize("n,fp32", [(9, 0), (64, 0), (128, 0), (100, 1)])
+def test_reg_reads_lower_triangle_and_real_diagonal(emul, n, fp32):
+    """An input Hermitian only to 1e-4 (imaginary diagonal included): the
+    result is that of the Hermitian matrix of H's lower triangle with Re of
+    its diagonal, which is what zheev('L') diagonalises."""
+    rng = np.random.default_rng(n + 3)
+    A = rand_herm(rng, n)
+    H = A + 1e-4 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    low = np.tril(H, -1) + np.tril(H, -1).conj().T + np.diag(H.diagonal().real)
+    w, _, _ = L.zheev(low)
+    E, Z = emul(H, fp32=fp32)
+    scale = np.abs(w).max()
+    tol = 4e-15 * scale * n if not fp32 else 5e-6 * scale
+    assert np.abs(E - w).max() <= tol
+    assert np.abs(low @ Z - Z * E).max() <= tol

@@ -149,7 +149,10 @@ def test_eigenvalues_and_vectors_vs_zheev(n_orb, spin):
     ks = np.random.default_rng(2).random((24, 3))
     E = m.solve_band_all(ks)
     E2, V = m.solve_all(ks)
-    assert np.array_equal(E, E2)
+    # eigenvalue-only solves read H(k)'s lower triangle, eigenvector solves
+    # its upper one (as the reference's eigvalsh_v / eigh): the two triangles'
+    # Fourier sums round differently, so the spectra agree to rounding only
+    assert np.abs(E - E2).max() <= 1e-14 * max(1.0, np.abs(E).max()) * m.nsta
     H = O.gen_ham_batch(o, ks)
     for i in range(len(ks)):
         w = O.eigvalsh(H[i])

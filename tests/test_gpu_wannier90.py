@@ -72,3 +72,35 @@ def test_berry_curvature_and_hall_of_loaded_model(tmp_path):
     s = m.hall_conductivity(ra.Parameters.rank2([4, 4, 3], [1, 0, 0], [0, 1, 0], mu, eta=0.01)).conductivity
     r = O.hall_conductivity_direct(o, [4, 4, 3], dirs, mu, 0.0, 0.01)
     assert np.abs(s - r).max() <= 1e-9 * max(1.0, np.abs(r).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_triangle_conventions_on_non_hermitian_storage(precision):
+    """ham[-R] != ham[R]^H at the 1e-5 level (as in files printed to a few
+    decimals): eigenvalue-only solves follow the lower triangle of H(k)
+    (the reference's eigvalsh_v), eigenvector and Berry paths its upper
+    triangle (the reference's eigh)."""
+    m0 = ra.synth_model(2, 3, False, 10, 1, 55, scale=1.0, decay=1.5)
+    rng = np.random.default_rng(3)
+    ham = m0.ham + 1e-5 * (rng.standard_normal(m0.ham.shape) + 1j * rng.standard_normal(m0.ham.shape))
+    m = ra.Model.from_arrays(m0.lat, m0.orb, False, m0.hamR, ham)
+    o = O.model_from_arrays(m0.lat, m0.orb, False, m0.hamR, ham)
+    print(f"[w90] triangle conventions {precision}", flush=True)
+    ks = np.random.default_rng(4).random((16, 3))
+    scale = max(1.0, np.abs(O.solve_band_all(o, ks)).max())
+    etol = 1e-12 * scale if precision == "fp64" else 1e-9  # eV; the mixed-form bar is 1e-9 eV
+    assert np.abs(m.solve_band_all(ks, precision=precision) - O.solve_band_all(o, ks)).max() <= etol
+    E, V = m.solve_all(ks, precision=precision)
+    for i, k in enumerate(ks):
+        Er, _ = O.eigh(O.gen_ham(o, k, "atom"))
+        assert np.abs(E[i] - Er).max() <= etol
+        U = V[i].T  # eigenvectors as rows
+        up = np.triu(O.gen_ham(o, k, "atom"), 1)
+        M = up + up.conj().T + np.diag(O.gen_ham(o, k, "atom").diagonal().real)
+        assert np.abs(M @ U - U * E[i]).max() <= (1e-12 if precision == "fp64" else 1e-9) * scale
+    dirs = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    p = ra.Parameters(kmesh=(1, 1, 1), direction=dirs, mu=np.array([0.0]), eta=1e-2, precision=precision)
+    om, Eb = m.berry_curvature_on(ks, p)
+    ob, Ebr = O.berry_curvature_batch(o, ks, dirs, 1e-2)
+    assert np.abs(Eb - Ebr).max() <= etol
+    assert np.abs(om - ob).max() <= (1e-8 if precision == "fp64" else 1e-6) * max(1.0, np.abs(ob).max())

@@ -242,3 +242,25 @@ def test_kmesh_order_and_errors():
         O.berry_curvature_at(RM.haldane(make), [0, 0], [[0, 0], [0, 1]], 1e-3)
     with pytest.raises(O.OracleError):
         RM.haldane(make).set_hop(1j, 0, 0, [0, 0])
+
+
+def test_eigen_triangle_conventions():
+    """For an H that is Hermitian only to rounding (a Wannier90 file printed to
+    6 decimals), the reference's eigvalsh_v(UPLO::Upper) on the row-major
+    buffer (solve_ham.rs:61) diagonalises the Hermitian matrix of H's lower
+    triangle, and eigh(UPLO::Lower) (solve_ham.rs:122, response/traits.rs:168)
+    that of its upper triangle: LAPACK reads the buffer column-major (H^T),
+    which is also why eigh hands back conj(U) (lib.rs:1231-1277)."""
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((9, 9)) + 1j * rng.standard_normal((9, 9))
+    H = A + A.conj().T + 1e-4 * (rng.standard_normal((9, 9)) + 1j * rng.standard_normal((9, 9)))
+    low = np.tril(H, -1) + np.tril(H, -1).conj().T + np.diag(H.diagonal().real)
+    up = np.triu(H, 1) + np.triu(H, 1).conj().T + np.diag(H.diagonal().real)
+    assert np.abs(O.eigvalsh(H) - np.linalg.eigvalsh(low)).max() < 1e-12
+    E, U = O.eigh(H)
+    assert np.abs(E - np.linalg.eigvalsh(up)).max() < 1e-12
+    assert np.abs(up @ U - U * E).max() < 1e-12
+    assert np.abs(np.linalg.eigvalsh(low) - np.linalg.eigvalsh(up)).max() > 1e-6
+    # column-major view of the row-major buffer: 'U' of H^T is H's lower triangle
+    assert np.abs(np.linalg.eigvalsh(H.T, UPLO="U") - np.linalg.eigvalsh(low)).max() < 1e-12
+    assert np.abs(np.linalg.eigvalsh(H.T, UPLO="L") - np.linalg.eigvalsh(up)).max() < 1e-12
