@@ -182,6 +182,19 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
                     double T, double eta, const double* dir, int spin_dir, int precision,
                     double* sigma);
 
+/* QuantumGeometry::quantum_geometry_on (quantum_geometry.rs:104-140, kernel
+ * quantum_geometry_at_impl :149-190) for nk k-points: with A = U^H v_a U,
+ * B = U^H v_b U, G_n = sum_{m != n} A_nm B_mn / ((E_n - E_m)^2 + eta^2);
+ * metric = Re G, omega = -2 Im G, E: nk*n each.  dir is 2*dim (rows a, b).
+ * Both precisions form A and B apart (FP64 products). */
+int rtb_quantum_geometry(const rtb_model* m, const double* k, size_t nk, const double* dir, double eta,
+                         int precision, double* metric, double* omega, double* E);
+
+/* Model::quantum_geometry, Integration::Direct (quantum_geometry.rs:196-240):
+ * metric[n_mu], berry[n_mu] = sum_k sum_n {g_n, Omega_n} f(E_n, mu) / N_k / det(lat). */
+int rtb_quantum_geometry_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
+                                double eta, const double* dir, int precision, double* metric, double* berry);
+
 /* Model::dos (model_physics.rs:179-226): E[En] = linspace(Emin, Emax, En), rho[En]. */
 int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, size_t En,
             double sigma, int precision, double* E, double* rho);

@@ -32,6 +32,13 @@ Pinning status (what checks each function, see ``tests/test_oracle.py``):
   antisymmetry ``sigma(t2) + sigma(-t2) <= 1e-10`` (``lib.rs:2665-2740``).
 * ``hall_conductivity_direct``: as above plus single- vs multi-mu agreement
   (``lib.rs:825-845``, ``lib.rs:962-998``).
+* ``quantum_geometry_at``: the Berry part equals ``berry_curvature_at``
+  (pinned above); the metric equals the Fubini-Study metric
+  (1/2) Tr(d_a P_n d_b P_n) of each isolated band from central finite
+  differences of the band projector P_n (gauge invariant, independent of the
+  sum-over-states formula), and for a two-band model g_0 = g_1 and
+  Omega_0 = -Omega_1.  ``quantum_geometry_direct`` sums it like
+  ``hall_conductivity_direct`` (its Berry part equals that function's result).
 * ``dos``: normalisation (integral of rho over a window that contains the whole
   spectrum equals nsta) and the closed-form single-level Gaussian.
 * ``fermi``: the step/half/zero cases of ``thermodynamics.rs:152-158`` and the
@@ -509,6 +516,42 @@ def dos_from_eigenvalues(evals, E, sigma):
 # --------------------------------------------------------------------------
 # Batched helpers for larger fixtures (same formulas; summation by matmul)
 # --------------------------------------------------------------------------
+def quantum_geometry_at(model: Model, k, direction, eta):
+    """quantum_geometry_at_impl (quantum_geometry.rs:149-190): with A = U^H v_a U,
+    B = U^H v_b U (Atom gauge), G_n = sum_{m != n} A_nm B_mn / ((E_n - E_m)^2 +
+    eta^2); returns (metric = Re G, berry = -2 Im G, E)."""
+    direction = validate_direction(direction, 2, model.dim)
+    v, H = gen_v_projected(model, k, "atom", direction)
+    e, U = eigh(H)
+    Uh = U.conj().T
+    A = Uh @ (v[0] @ U)
+    B = Uh @ (v[1] @ U)
+    K = A * B.T
+    dE = e[:, None] - e[None, :]
+    t = K / (dE * dE + eta * eta)
+    np.fill_diagonal(t, 0.0)
+    G = t.sum(axis=1)
+    return G.real, -2.0 * G.imag, e
+
+
+def quantum_geometry_direct(model: Model, kmesh, direction, mu, T=0.0, eta=1e-3):
+    """Model::quantum_geometry, Integration::Direct (quantum_geometry.rs:196-240):
+    (metric[mu], berry[mu]) = sum over the mesh and bands of (g, Omega) f(E, mu)
+    / N_k / det(lat)."""
+    ks = gen_kmesh(kmesh)
+    mu = np.atleast_1d(np.asarray(mu, float))
+    gm = np.zeros(mu.size)
+    bm = np.zeros(mu.size)
+    for k in ks:
+        g, b, e = quantum_geometry_at(model, k, direction, eta)
+        for i, m in enumerate(mu):
+            f = fermi(e, m, T)
+            gm[i] += (g * f).sum()
+            bm[i] += (b * f).sum()
+    norm = 1.0 / len(ks) / np.linalg.det(model.lat)
+    return gm * norm, bm * norm
+
+
 def gen_ham_batch(model: Model, kvec, gauge="atom"):
     """gen_ham for many k at once: phases [N_k, N_R] times ham [N_R, n*n].
     Same formula as gen_ham; the R sum is done by BLAS instead of zaxpy."""

@@ -281,7 +281,7 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, bool from_upper, int n, size_t c
 // two products instead of four, and the band sum reads A, B out of C.
 int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const double* V, const double* Z,
                const double* zscale, const double* E, int spin_dir, double eta, double* T1, double* C, double* J,
-               double* omega, double* B = nullptr) {
+               double* omega, double* B = nullptr, double* metric = nullptr) {
   const int n = m->nsta;
   const size_t n2 = static_cast<size_t>(n) * n;
   if (B) {
@@ -303,7 +303,7 @@ int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const d
     RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, B, n2, false, zscale));
     RTB_TRY(tmr.end(4, 4 * flop_zgemm(n, K), 4 * mflop_zgemm(n, K, false)));
     RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
-    RTB_TRY(kubo_omega_exact(st, n, K, C, B, E, eta, omega));
+    RTB_TRY(kubo_omega_exact(st, n, K, C, B, E, eta, omega, metric));
     RTB_TRY(tmr.end(1));
     return RTB_OK;
   }
@@ -605,6 +605,11 @@ struct BerryJob {
   size_t n_mu = 0;
   double* sigma = nullptr;
   int precision = RTB_PREC_FP64;
+  // quantum geometry (quantum_geometry.rs): the metric beside Omega; mode 0
+  // writes it per k into `metric`, mode 2 integrates it into `msum`
+  bool geometry = false;
+  double* metric = nullptr;
+  double* msum = nullptr;
 };
 
 // A chunk of consecutive pipeline k-points; for the mesh path it covers the
@@ -662,7 +667,7 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   // the plain form, and any model whose storage is not exactly Hermitian,
   // forms the reference's A and B apart (V0, V1 kept apart); the mixed form
   // of a Hermitian model takes V0 + i V1 straight from the Fourier stage
-  const bool exact_kubo = !f32 || !m->herm;
+  const bool exact_kubo = !f32 || !m->herm || j.geometry;
   a.combine = spinj || exact_kubo ? 0 : 1;
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const size_t per_k =
@@ -679,6 +684,8 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   RTB_TRY(w.alloc(&dV, K * n2 * 4));
   RTB_TRY(w.alloc(&dE, K * n));
   RTB_TRY(w.alloc(&dOm, K * n));
+  double* dMet = nullptr;
+  if (j.geometry) RTB_TRY(w.alloc(&dMet, K * n));
   RTB_TRY(alloc_eig(w, n, K, true, eb));
   double* kub;
   RTB_TRY(w.alloc(&kub, K * n2 * 6));  // T1, A, B (complex)
@@ -693,13 +700,17 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   double* T1 = kub;
   double* A = kub + K * n2 * 2;
   double* B = kub + K * n2 * 4;
-  double *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr;
+  double *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr, *dmpart = nullptr, *dmsig = nullptr;
   int total_blk = 0;
   if (j.mode == 2) {
     for (const Chunk& ch : chunks) total_blk += hall_blocks(ch.c);
     RTB_TRY(w.alloc(&dmu, j.n_mu));
     RTB_TRY(w.alloc(&dpart, static_cast<size_t>(total_blk) * j.n_mu));
     RTB_TRY(w.alloc(&dsig, j.n_mu));
+    if (j.geometry) {
+      RTB_TRY(w.alloc(&dmpart, static_cast<size_t>(total_blk) * j.n_mu));
+      RTB_TRY(w.alloc(&dmsig, j.n_mu));
+    }
     RTB_HIP(hipMemcpyAsync(dmu, j.mu, j.n_mu * sizeof(double), hipMemcpyHostToDevice, st));
   }
   if (j.mode == 1) RTB_TRY(w.alloc(&docc, K));
@@ -739,12 +750,15 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
       RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, false));
       RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     }
-    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, dJ, dOm, exact_kubo ? B : nullptr));
+    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, dJ, dOm, exact_kubo ? B : nullptr,
+                       dMet));
     int l2 = 0;
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     if (j.mode == 0) {
       RTB_HIP(hipMemcpyAsync(j.omega + k0 * n, dOm, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
       RTB_HIP(hipMemcpyAsync(j.E + k0 * n, dE, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
+      if (j.geometry)
+        RTB_HIP(hipMemcpyAsync(j.metric + k0 * n, dMet, c * n * sizeof(double), hipMemcpyDeviceToHost, st));
     } else if (j.mode == 1) {
       RTB_TRY(occupied_sum(st, n, c, dOm, dE, j.mu0, j.T, docc));
       ++l2;
@@ -754,6 +768,11 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
       RTB_TRY(hall_partials(st, n, c, dOm, dE, dmu, static_cast<int>(j.n_mu), j.T,
                             dpart + static_cast<size_t>(blk_off) * j.n_mu, nb));
       ++l2;
+      if (j.geometry) {
+        RTB_TRY(hall_partials(st, n, c, dMet, dE, dmu, static_cast<int>(j.n_mu), j.T,
+                              dmpart + static_cast<size_t>(blk_off) * j.n_mu, nb));
+        ++l2;
+      }
       blk_off += nb;
     }
     RTB_TRY(tmr.end(l2));
@@ -762,12 +781,16 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   if (j.mode == 2) {
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     RTB_TRY(reduce_partials(st, dpart, total_blk, static_cast<int>(j.n_mu), 1.0, dsig));
-    RTB_TRY(tmr.end(1));
-    std::vector<double> sums(j.n_mu);
+    if (j.geometry) RTB_TRY(reduce_partials(st, dmpart, total_blk, static_cast<int>(j.n_mu), 1.0, dmsig));
+    RTB_TRY(tmr.end(j.geometry ? 2 : 1));
+    std::vector<double> sums(j.n_mu), msums(j.geometry ? j.n_mu : 0);
     RTB_HIP(hipMemcpyAsync(sums.data(), dsig, j.n_mu * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (j.geometry)
+      RTB_HIP(hipMemcpyAsync(msums.data(), dmsig, j.n_mu * sizeof(double), hipMemcpyDeviceToHost, st));
     RTB_HIP(hipStreamSynchronize(st));
-    // sum / nk / det (response/linear/mod.rs:221)
+    // sum / nk / det (response/linear/mod.rs:221; quantum_geometry.rs:219,234)
     for (size_t i = 0; i < j.n_mu; ++i) j.sigma[i] = sums[i] / static_cast<double>(j.nk) / m->det_lat;
+    for (size_t i = 0; i < msums.size(); ++i) j.msum[i] = msums[i] / static_cast<double>(j.nk) / m->det_lat;
   }
   RTB_HIP(hipStreamSynchronize(st));
   RTB_TRY(tmr.collect(j.nk, n));
@@ -847,6 +870,57 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
   j.mu = mu;
   j.n_mu = n_mu;
   j.sigma = sigma;
+  j.precision = precision;
+  return berry_impl(m, j);
+}
+
+int rtb_quantum_geometry(const rtb_model* m, const double* k, size_t nk, const double* dir, double eta,
+                         int precision, double* metric, double* omega, double* E) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  // quantum_geometry_on (quantum_geometry.rs:117-118): direction, broadening
+  RTB_TRY(check_direction(m, dir));
+  RTB_TRY(check_eta(eta));
+  if (nk > 0 && (!k || !metric || !omega || !E)) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (nk == 0) return RTB_OK;
+  BerryJob j;
+  j.k = k;
+  j.nk = nk;
+  j.dir = dir;
+  j.eta = eta;
+  j.mode = 0;
+  j.omega = omega;
+  j.E = E;
+  j.geometry = true;
+  j.metric = metric;
+  j.precision = precision;
+  return berry_impl(m, j);
+}
+
+int rtb_quantum_geometry_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
+                                double eta, const double* dir, int precision, double* metric, double* berry) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  // validate_rank2 (config.rs:141-147), as Model::quantum_geometry (quantum_geometry.rs:196)
+  size_t nk = 0;
+  RTB_TRY(check_kmesh(m, kmesh, &nk));
+  RTB_TRY(check_direction(m, dir));
+  RTB_TRY(check_mu(mu, n_mu));
+  RTB_TRY(check_eta(eta));
+  RTB_TRY(check_T(T));
+  if (!metric || !berry) return fail(RTB_ERR_INVALID_ARGUMENT, "null output");
+  BerryJob j;
+  j.kmesh = kmesh;
+  j.nk = nk;
+  j.dir = dir;
+  j.eta = eta;
+  j.mode = 2;
+  j.T = T;
+  j.mu = mu;
+  j.n_mu = n_mu;
+  j.sigma = berry;
+  j.geometry = true;
+  j.msum = metric;
   j.precision = precision;
   return berry_impl(m, j);
 }

@@ -128,9 +128,10 @@ int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double
 // partial sums zgemm_kubo writes (tiles = ceil(n/64)).
 int kubo_omega_from_parts(hipStream_t st, int n, size_t nk, const double* part, double* omega);
 // The same band sum from A and B themselves ([nk][n][n] each, not assumed
-// Hermitian): Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2).
+// Hermitian): Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2);
+// metric (optional): g_n = sum_{m != n} Re(A_nm B_mn) / ((E_n - E_m)^2 + eta^2).
 int kubo_omega_exact(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E, double eta,
-                     double* omega);
+                     double* omega, double* metric);
 // out[k] = sum_n omega[k][n] f(E[k][n], mu, T)
 int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const double* E, double mu,
                  double T, double* out);

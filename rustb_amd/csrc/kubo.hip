@@ -149,14 +149,16 @@ __global__ void reduce_partials_kernel(const double* __restrict__ partial, int n
 // Omega_n = sum_{m != n} -2 Im(A_nm B_mn) / ((E_n - E_m)^2 + eta^2) from A
 // and B as the reference forms them (berry_curvature_at_impl,
 // response/traits.rs:160-200), with no assumption that A and B are
-// Hermitian.  One k-point per workgroup (n <= 128): 32 x 32 tiles of A (row
+// Hermitian; with `metric`, also g_n = sum_{m != n} Re(A_nm B_mn) / (...),
+// the real part of the quantum geometric tensor (quantum_geometry.rs:168-186).  One k-point per workgroup (n <= 128): 32 x 32 tiles of A (row
 // tile I, column tile J) and of B (tile (J, I)) through LDS, the products
 // summed along each row of A.
 constexpr int kOmTile = 32;
 __global__ void __launch_bounds__(256) kubo_omega_exact_kernel(int n, const double2* __restrict__ A,
                                                                const double2* __restrict__ B,
                                                                const double* __restrict__ E, double eta,
-                                                               double* __restrict__ omega) {
+                                                               double* __restrict__ omega,
+                                                               double* __restrict__ metric) {
   __shared__ double2 sa[kOmTile][kOmTile + 1];
   __shared__ double2 sb[kOmTile][kOmTile + 1];
   __shared__ double se[kMaxEighN];
@@ -170,6 +172,7 @@ __global__ void __launch_bounds__(256) kubo_omega_exact_kernel(int n, const doub
   const double eta2 = eta * eta;
   for (int I = 0; I < T; ++I) {
     double part[kOmTile / 8] = {};
+    double gpart[kOmTile / 8] = {};
     for (int J = 0; J < T; ++J) {
       __syncthreads();
 #pragma unroll
@@ -190,16 +193,24 @@ __global__ void __launch_bounds__(256) kubo_omega_exact_kernel(int n, const doub
           const double2 b = sb[tx][r];  // B[col][row]
           const double im = a.x * b.y + a.y * b.x;
           const double d = se[row] - se[col];
-          part[q] += -2.0 * im / (d * d + eta2);
+          const double den = d * d + eta2;
+          part[q] += -2.0 * im / den;
+          if (metric) gpart[q] += (a.x * b.x - a.y * b.y) / den;
         }
       }
     }
 #pragma unroll
     for (int q = 0; q < kOmTile / 8; ++q) {
-      double v = part[q];
-      for (int o = kOmTile / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kOmTile);
+      double v = part[q], g = gpart[q];
+      for (int o = kOmTile / 2; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o, kOmTile);
+        g += __shfl_xor(g, o, kOmTile);
+      }
       const int row = I * kOmTile + ty + 8 * q;
-      if (tx == 0 && row < n) omega[k * n + row] = v;
+      if (tx == 0 && row < n) {
+        omega[k * n + row] = v;
+        if (metric) metric[k * n + row] = g;
+      }
     }
   }
 }
@@ -228,12 +239,12 @@ int kubo_omega_from_parts(hipStream_t st, int n, size_t nk, const double* part, 
 }
 
 int kubo_omega_exact(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E, double eta,
-                     double* omega) {
+                     double* omega, double* metric) {
   if (nk == 0) return RTB_OK;
   if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "the band sum supports 1 <= nsta <= 128");
   if (nk > 0x7fffffffu) return fail(RTB_ERR_UNSUPPORTED, "too many k-points in one launch");
   hipLaunchKernelGGL(kubo_omega_exact_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
-                     reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B), E, eta, omega);
+                     reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B), E, eta, omega, metric);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -69,6 +69,34 @@ class HallConductivityResult:
 
 
 @dataclass
+class BandQuantumGeometry:
+    """quantum_geometry.rs:30-40: per band at one k."""
+
+    metric: np.ndarray
+    berry_curvature: np.ndarray
+    energies: np.ndarray
+
+
+@dataclass
+class QuantumGeometryMap:
+    """quantum_geometry.rs:42-51: [nk, nsta] each."""
+
+    metric: np.ndarray
+    berry_curvature: np.ndarray
+    energies: np.ndarray
+
+
+@dataclass
+class QuantumGeometryResult:
+    """quantum_geometry.rs:53-66: occupation-weighted, per chemical potential."""
+
+    chemical_potentials: np.ndarray
+    metric: np.ndarray
+    berry_curvature: np.ndarray
+    diagnostics: Optional[dict] = None
+
+
+@dataclass
 class Atom:
     """An atomic site (atom_struct.rs Atom): species, fractional position and
     the indices of its orbitals."""
@@ -375,6 +403,44 @@ class Model:
                                      ptr(self._dir(params)), SPIN_DIR[params.spin], PRECISION[params.precision],
                                      ptr(sig)))
         return HallConductivityResult(mu.copy(), sig)
+
+    # ---- quantum geometry (quantum_geometry.rs) ----
+    def quantum_geometry_on(self, kvec, params: Parameters) -> "QuantumGeometryMap":
+        """quantum_geometry_on (quantum_geometry.rs:104-140): metric, Berry
+        curvature and energies [nk, nsta] (Atom gauge, rows of params.direction)."""
+        k = self._kvec(kvec)
+        n = self.nsta
+        g = np.zeros((k.shape[0], n))
+        om = np.zeros((k.shape[0], n))
+        E = np.zeros((k.shape[0], n))
+        check(load().rtb_quantum_geometry(self._device(), ptr(k), k.shape[0], ptr(self._dir(params)),
+                                          float(params.eta), PRECISION[params.precision], ptr(g), ptr(om), ptr(E)))
+        return QuantumGeometryMap(g, om, E)
+
+    def quantum_geometry_at(self, k, params: Parameters) -> "BandQuantumGeometry":
+        """quantum_geometry_at (quantum_geometry.rs:88-102)."""
+        r = self.quantum_geometry_on(np.asarray(k, float)[None, :], params)
+        return BandQuantumGeometry(r.metric[0], r.berry_curvature[0], r.energies[0])
+
+    def quantum_geometry(self, params: Parameters) -> "QuantumGeometryResult":
+        """Model::quantum_geometry, Integration::Direct (quantum_geometry.rs:196-240)."""
+        if params.integration == "EnergyCut":
+            raise TbError(-6, "integration: quantum_geometry supports Integration::Direct or Simplex, not EnergyCut")
+        if params.integration != "Direct":
+            raise TbError(-14, "integration: this path implements Integration::Direct")
+        kmesh = np.ascontiguousarray(np.asarray(params.kmesh, dtype=np.uint64))
+        if kmesh.shape != (self.dim,):
+            raise TbError(-3, "kmesh must have dim_r entries")
+        mu = np.ascontiguousarray(np.atleast_1d(np.asarray(params.mu, np.float64)))
+        T = np.atleast_1d(np.asarray(params.T, float))
+        if T.size == 0:
+            raise TbError(-6, "T: must contain at least one value")
+        g = np.zeros(mu.size)
+        b = np.zeros(mu.size)
+        check(load().rtb_quantum_geometry_direct(self._device(), ptr(kmesh), ptr(mu), mu.size, float(T[0]),
+                                                 float(params.eta), ptr(self._dir(params)),
+                                                 PRECISION[params.precision], ptr(g), ptr(b)))
+        return QuantumGeometryResult(mu.copy(), g, b, None)
 
     def dos(self, k_mesh, E_min, E_max, E_n, sigma, precision="fp64"):
         """Model::dos (model_physics.rs:179-226) -> (E, rho)."""

@@ -256,3 +256,28 @@ def test_last_stats_reports_stages():
     herm = 4.0 * 32 * 32 * 33 * 64
     assert st["flop"][i("gemm64")] == pytest.approx(3 * 8.0 * 32**3 * 64 + 2 * herm, rel=1e-12)
     assert all(st["ms"][i(s)] > 0 for s in ("fourier", "tridiag", "trieig", "backtransform", "gemm64", "refine"))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_quantum_geometry_vs_oracle(precision):
+    """quantum_geometry_on / quantum_geometry (Direct) through the C ABI."""
+    m, o = random_pair(3, True, 3, 4, seed=61)
+    ks = np.random.default_rng(5).random((12, 3))
+    dirs = np.array([[1.0, 0.3, 0.0], [0.0, 1.0, -0.2]])
+    p = ra.Parameters(kmesh=(1, 1, 1), direction=dirs, mu=np.array([0.0]), eta=1e-3, precision=precision)
+    r = m.quantum_geometry_on(ks, p)
+    tol = 1e-8 if precision == "fp64" else 1e-7
+    for i, k in enumerate(ks):
+        g, om, E = O.quantum_geometry_at(o, k, dirs, 1e-3)
+        assert np.abs(r.energies[i] - E).max() <= (1e-12 if precision == "fp64" else 1e-9) * max(1.0, np.abs(E).max())
+        assert np.abs(r.metric[i] - g).max() <= tol * max(1.0, np.abs(g).max())
+        assert np.abs(r.berry_curvature[i] - om).max() <= tol * max(1.0, np.abs(om).max())
+    m2, o2 = random_pair(4, False, 2, 5, seed=62)
+    mu = np.array([-0.5, 0.0, 0.7])
+    q = m2.quantum_geometry(ra.Parameters.rank2([7, 9], [1, 0], [0, 1], mu, eta=0.02, precision=precision))
+    gm, bm = O.quantum_geometry_direct(o2, [7, 9], np.eye(2), mu, 0.0, 0.02)
+    rel = 1e-9 if precision == "fp64" else 1e-6
+    assert np.abs(q.metric - gm).max() <= rel * max(1.0, np.abs(gm).max())
+    assert np.abs(q.berry_curvature - bm).max() <= rel * max(1.0, np.abs(bm).max())
+    s = m2.hall_conductivity(ra.Parameters.rank2([7, 9], [1, 0], [0, 1], mu, eta=0.02, precision="fp64"))
+    assert np.abs(q.berry_curvature - s.conductivity).max() <= rel * max(1.0, np.abs(bm).max())

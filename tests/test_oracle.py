@@ -264,3 +264,66 @@ def test_eigen_triangle_conventions():
     # column-major view of the row-major buffer: 'U' of H^T is H's lower triangle
     assert np.abs(np.linalg.eigvalsh(H.T, UPLO="U") - np.linalg.eigvalsh(low)).max() < 1e-12
     assert np.abs(np.linalg.eigvalsh(H.T, UPLO="L") - np.linalg.eigvalsh(up)).max() < 1e-12
+
+
+def test_quantum_geometry_is_fubini_study():
+    """quantum_geometry_at (quantum_geometry.rs:149-190): for isolated bands
+    and eta -> 0 the metric Re G_n^{ab} is the Fubini-Study metric
+    (1/2) Tr(d_a P_n d_b P_n) of the band projector, from central finite
+    differences in Cartesian k (unit lattice, orbitals at the origin so the
+    gauges coincide); the Berry part is berry_curvature_at's."""
+    rng = np.random.default_rng(21)
+    o = O.Model(np.eye(2), np.zeros((3, 2)), False)
+    z = np.zeros(2, int)
+    for i in range(3):
+        o.add_element(rng.uniform(-2, 2), i, i, z)
+    for R in ([1, 0], [0, 1], [1, 1], [1, -1]):
+        for i in range(3):
+            for j in range(3):
+                o.add_element(0.4 * complex(*rng.standard_normal(2)), i, j, np.array(R))
+    k = np.array([0.17, 0.31])
+    h = 1e-5
+
+    def projectors(kc):
+        _, U = O.eigh(O.gen_ham(o, kc / (2 * np.pi), "atom"))
+        return [np.outer(U[:, n], U[:, n].conj()) for n in range(3)]
+
+    kc = 2 * np.pi * k
+    dP = []
+    for a in range(2):
+        e = np.zeros(2)
+        e[a] = h
+        p1, p0 = projectors(kc + e), projectors(kc - e)
+        dP.append([(x - y) / (2 * h) for x, y in zip(p1, p0)])
+    for a, b in ((0, 1), (0, 0), (1, 1)):
+        dirs = np.eye(2)[[a, b]]
+        g, om, E = O.quantum_geometry_at(o, k, dirs, 1e-8)
+        assert np.min(np.abs(np.diff(E))) > 0.1
+        fs = np.array([0.5 * np.trace(dP[a][n] @ dP[b][n]).real for n in range(3)])
+        assert np.abs(g - fs).max() <= 1e-6 * max(1.0, np.abs(fs).max())
+        ob, _ = O.berry_curvature_batch(o, k[None, :], dirs, 1e-8)
+        assert np.array_equal(om, ob[0]) or np.abs(om - ob[0]).max() <= 1e-12 * max(1.0, np.abs(ob).max())
+
+
+def test_quantum_geometry_two_band_symmetry_and_direct_sum():
+    """Two bands: g_0 = g_1 and Omega_0 = -Omega_1; the Direct integral's Berry
+    part is hall_conductivity_direct (same formula, same normalisation)."""
+    o = O.Model(np.eye(2), np.zeros((2, 2)), False)
+    z = np.zeros(2, int)
+    o.add_element(-1.0, 0, 0, z)
+    o.add_element(1.0, 1, 1, z)
+    o.add_element(-0.5j, 0, 1, np.array([1, 0]))
+    o.add_element(0.5j, 0, 1, np.array([-1, 0]))
+    o.add_element(-0.5, 0, 1, np.array([0, 1]))
+    o.add_element(0.5, 0, 1, np.array([0, -1]))
+    for d in ([1, 0], [0, 1]):
+        o.add_element(0.5, 0, 0, np.array(d))
+        o.add_element(-0.5, 1, 1, np.array(d))
+    g, om, _ = O.quantum_geometry_at(o, np.array([0.1, 0.23]), np.eye(2), 1e-6)
+    assert abs(g[0] - g[1]) <= 1e-12 * max(1.0, abs(g[0]))
+    assert abs(om[0] + om[1]) <= 1e-12 * max(1.0, abs(om[0]))
+    gm, bm = O.quantum_geometry_direct(o, [9, 9], np.eye(2), [0.0, 0.5], 0.0, 0.05)
+    sig = O.hall_conductivity_direct(o, [9, 9], np.eye(2), [0.0, 0.5], 0.0, 0.05)
+    assert np.abs(bm - sig).max() <= 1e-12 * max(1.0, np.abs(sig).max())
+    gxx, _ = O.quantum_geometry_direct(o, [9, 9], np.eye(2)[[0, 0]], [0.0], 0.0, 0.05)
+    assert gxx[0] > 0  # the metric of the occupied band is positive semi-definite
