@@ -47,6 +47,7 @@ extern "C" {
 #define RTB_ERR_IO -19                   /* TbError::Io (error.rs:61), e.g. text that is not UTF-8 */
 #define RTB_ERR_NO_ORBITALS -20          /* TbError::NoOrbitals (error.rs:279) */
 #define RTB_ERR_LINALG -21               /* TbError::Linalg (error.rs:83): a singular lattice */
+#define RTB_ERR_INVALID_THERMO -22       /* TbError::InvalidThermodynamicParameter (thermodynamics.rs) */
 
 /* ---- enums ---- */
 #define RTB_GAUGE_LATTICE 0 /* Gauge::Lattice (model.rs:449-453) */
@@ -194,6 +195,16 @@ int rtb_quantum_geometry(const rtb_model* m, const double* k, size_t nk, const d
  * metric[n_mu], berry[n_mu] = sum_k sum_n {g_n, Omega_n} f(E_n, mu) / N_k / det(lat). */
 int rtb_quantum_geometry_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
                                 double eta, const double* dir, int precision, double* metric, double* berry);
+
+/* Model::extrinsic_nonlinear_hall, Integration::Direct (response/nonlinear/mod.rs:256-347):
+ * the Berry curvature dipole sigma[n_mu] = sum_k sum_n d_{f2}E_n Omega_n^{cur,f1}
+ * (-df/dE)(E_n) / N_k / det(lat), per band from berry_curvature_dipole_n_onek
+ * (mod.rs:89-163).  dir is 3*dim (rows current, field_1, field_2); T > 0;
+ * symmetrized != 0 (FieldSymmetry::Symmetrized, the reference's default)
+ * averages the two field orderings when the field rows differ. */
+int rtb_nonlinear_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
+                              double eta, const double* dir, int spin_dir, int symmetrized, int precision,
+                              double* sigma);
 
 /* Model::dos (model_physics.rs:179-226): E[En] = linspace(Emin, Emax, En), rho[En]. */
 int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, size_t En,

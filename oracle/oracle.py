@@ -39,6 +39,13 @@ Pinning status (what checks each function, see ``tests/test_oracle.py``):
   sum-over-states formula), and for a two-band model g_0 = g_1 and
   Omega_0 = -Omega_1.  ``quantum_geometry_direct`` sums it like
   ``hall_conductivity_direct`` (its Berry part equals that function's result).
+* ``berry_curvature_dipole_onek`` / ``extrinsic_nonlinear_hall_direct``: the
+  band velocity Re(U^H v_c U)_nn equals the finite-difference derivative of
+  the band energy (Hellmann-Feynman); the Omega factor equals
+  ``berry_curvature_at``; the dipole response is even under time reversal
+  (Haldane t2 -> -t2, the reference's ``dipole_energy_cut_tr``,
+  lib.rs:2745-2790, here for the Direct sum on the k -> -k symmetric mesh);
+  Ordered with field_1 = field_2 equals Symmetrized.
 * ``dos``: normalisation (integral of rho over a window that contains the whole
   spectrum equals nsta) and the closed-form single-level Gaussian.
 * ``fermi``: the step/half/zero cases of ``thermodynamics.rs:152-158`` and the
@@ -550,6 +557,58 @@ def quantum_geometry_direct(model: Model, kmesh, direction, mu, T=0.0, eta=1e-3)
             bm[i] += (b * f).sum()
     norm = 1.0 / len(ks) / np.linalg.det(model.lat)
     return gm * norm, bm * norm
+
+
+def berry_curvature_dipole_onek(model: Model, k, current, dir_2, dir_3, eta, spin=None):
+    """berry_curvature_dipole_n_onek (response/nonlinear/mod.rs:89-163):
+    per band d_{dir_3}E_n * Omega_n^{current, dir_2}, with
+    d E_n = Re (U^H v_3 U)_nn and Omega from A1 = U^H J U, A2 = U^H v_2 U
+    (J = v_current, or its spin current for a spinful model)."""
+    d = np.array([current, dir_2, dir_3], float)
+    v, H = gen_v_projected(model, k, "atom", d)
+    e, U = eigh(H)
+    S = spin_matrix(model.norb, spin) if (model.spin and spin is not None) else None
+    J = v[0] if S is None else 0.5 * (S @ v[0] + v[0] @ S)
+    Uh = U.conj().T
+    A1 = Uh @ (J @ U)
+    A2 = Uh @ (v[1] @ U)
+    dE = np.real(np.diag(Uh @ (v[2] @ U)))
+    U0 = 1.0 / ((e[:, None] - e[None, :]) ** 2 + eta * eta)
+    np.fill_diagonal(U0, 0.0)
+    om = -2.0 * np.einsum("ij,ji->i", A1 * U0, A2).imag
+    return om * dE, e
+
+
+def fermi_derivative(energy, mu, T_kelvin):
+    """fermi_derivative_from_width (thermodynamics.rs:122-126): f (1 - f) / width."""
+    width = T_kelvin * 8.617333262145e-5
+    f = fermi(energy, mu, T_kelvin)
+    return f * (1.0 - f) / width
+
+
+def extrinsic_nonlinear_hall_direct(model: Model, kmesh, direction, mu, T, eta, spin=None, symmetrized=True):
+    """Model::extrinsic_nonlinear_hall, Integration::Direct
+    (response/nonlinear/mod.rs:256-347): rows (current, field_1, field_2);
+    Symmetrized averages the (field_1, field_2) and (field_2, field_1) kernels
+    when the field rows differ."""
+    direction = np.asarray(direction, float)
+    cur, f1, f2 = direction
+    mu = np.atleast_1d(np.asarray(mu, float))
+    ks = gen_kmesh(kmesh)
+    norm = 1.0 / len(ks) / np.linalg.det(model.lat)
+
+    def component(a, b, c):
+        out = np.zeros(mu.size)
+        for k in ks:
+            kern, e = berry_curvature_dipole_onek(model, k, a, b, c, eta, spin)
+            for i, m in enumerate(mu):
+                out[i] += (kern * fermi_derivative(e, m, T)).sum()
+        return out * norm
+
+    first = component(cur, f1, f2)
+    if symmetrized and not np.array_equal(f1, f2):
+        return 0.5 * (first + component(cur, f2, f1))
+    return first
 
 
 def gen_ham_batch(model: Model, kvec, gauge="atom"):

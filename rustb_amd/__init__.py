@@ -6,13 +6,14 @@ include/rtb_hip.h); this package is the host-side mirror of the reference's
 """
 from ._lib import TbError, last_stats, load
 from .kpoints import gen_kmesh, k_path
-from .model import (Atom, BandBerryCurvature, BandQuantumGeometry, HallConductivityResult, Model, Parameters,
-                    QuantumGeometryMap, QuantumGeometryResult, synth_model)
+from .model import (Atom, BandBerryCurvature, BandQuantumGeometry, HallConductivityResult, Model,
+                    NonlinearHallResult, Parameters, QuantumGeometryMap, QuantumGeometryResult, synth_model)
 from . import workloads
 
 __all__ = [
     "Model",
     "Atom",
+    "NonlinearHallResult",
     "BandQuantumGeometry",
     "QuantumGeometryMap",
     "QuantumGeometryResult",

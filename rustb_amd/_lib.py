@@ -35,6 +35,7 @@ ERRORS = {
     -19: "Io",
     -20: "NoOrbitals",
     -21: "Linalg",
+    -22: "InvalidThermodynamicParameter",
 }
 
 
@@ -93,6 +94,8 @@ SIGNATURES = {
     "rtb_dos": (c_int, [c_void_p, _P, c_double, c_double, c_size_t, c_double, c_int, _P, _P]),
     "rtb_quantum_geometry": (c_int, [c_void_p, _P, c_size_t, _P, c_double, c_int, _P, _P, _P]),
     "rtb_quantum_geometry_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, _P, _P]),
+    "rtb_nonlinear_hall_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, c_int,
+                                          _P]),
     "rtb_w90_read": (c_int, [c_char_p, c_char_p, c_double, c_int, c_int, POINTER(c_void_p)]),
     "rtb_w90_destroy": (None, [c_void_p]),
     "rtb_w90_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int),

@@ -110,9 +110,9 @@ int check_kmesh(const rtb_model* m, const size_t* kmesh, size_t* nk) {
 }
 
 // validate_direction_matrix (response/config.rs:202-236), rank 2
-int check_direction(const rtb_model* m, const double* dir) {
+int check_direction(const rtb_model* m, const double* dir, int rows = 2) {
   if (!dir) return fail(RTB_ERR_INVALID_ARGUMENT, "null direction");
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < rows; ++p) {
     bool allzero = true;
     for (int d = 0; d < m->dim; ++d) {
       double x = dir[p * m->dim + d];
@@ -327,6 +327,41 @@ int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const d
   RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
   RTB_TRY(kubo_omega_from_parts(st, n, K, C, omega));
   RTB_TRY(tmr.end(1));
+  return RTB_OK;
+}
+
+// Berry curvature dipole kernel of a chunk (berry_curvature_dipole_n_onek,
+// response/nonlinear/mod.rs:89-163): V holds v_cur, v_f1, v_f2 per k (stride
+// 3 n^2); A1 = U^H J U (J = v_cur or its spin current), A2 = U^H v_f1 U,
+// A3 = U^H v_f2 U with U = Z diag(zscale); Omega^{cur,f1} (and with sym
+// Omega^{cur,f2}) by the term-by-term band sum, then the per-band kernel.
+int dipole_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const double* V, const double* Z,
+                 const double* zscale, const double* E, int spin_dir, double eta, bool sym, double* T1, double* A1,
+                 double* A2, double* A3, double* J, double* om1, double* om2, double* kern) {
+  const int n = m->nsta;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const double* Ma = V;
+  size_t sMa = 3 * n2;
+  if (m->spin && spin_dir != RTB_SPIN_NONE) {
+    RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+    RTB_TRY(spin_current(st, m->norb, K, spin_dir, V, 3 * n2 * 2, nullptr, J));
+    RTB_TRY(tmr.end(1));
+    Ma = J;
+    sMa = n2;
+  }
+  RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
+  RTB_TRY(zgemm_batched(st, n, K, Ma, sMa, false, Z, n2, T1, n2));
+  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, A1, n2, false, zscale));
+  RTB_TRY(zgemm_batched(st, n, K, V + n2 * 2, 3 * n2, false, Z, n2, T1, n2));
+  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, A2, n2, false, zscale));
+  RTB_TRY(zgemm_batched(st, n, K, V + n2 * 4, 3 * n2, false, Z, n2, T1, n2));
+  RTB_TRY(zgemm_batched_scaled(st, n, K, Z, n2, true, T1, n2, A3, n2, false, zscale));
+  RTB_TRY(tmr.end(6, 6 * flop_zgemm(n, K), 6 * mflop_zgemm(n, K, false)));
+  RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+  RTB_TRY(kubo_omega_exact(st, n, K, A1, A2, E, eta, om1, nullptr));
+  if (sym) RTB_TRY(kubo_omega_exact(st, n, K, A1, A3, E, eta, om2, nullptr));
+  RTB_TRY(dipole_combine(st, n, K, A2, A3, om1, om2, sym, kern));
+  RTB_TRY(tmr.end(sym ? 3 : 2));
   return RTB_OK;
 }
 
@@ -610,6 +645,10 @@ struct BerryJob {
   bool geometry = false;
   double* metric = nullptr;
   double* msum = nullptr;
+  // Berry curvature dipole (response/nonlinear/mod.rs): dir has 3 rows
+  // (current, field_1, field_2); mode 2 only, weights -df/dE
+  bool dipole = false;
+  bool sym = false;
 };
 
 // A chunk of consecutive pipeline k-points; for the mesh path it covers the
@@ -659,33 +698,42 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   Work w(st);
   Timer tmr(st);
   FourierArgs a;
-  RTB_TRY(upload_dirs(st, w, m, j.dir, 2, a));
+  RTB_TRY(upload_dirs(st, w, m, j.dir, j.dipole ? 3 : 2, a));
   a.gauge = RTB_GAUGE_ATOM;  // berry_curvature_at_impl: gen_v_projected(k, Gauge::Atom, ..)
+  const int nv = j.dipole ? 3 : 2;  // projected velocities per k
   const size_t n2 = static_cast<size_t>(n) * n;
   const bool spinj = m->spin && j.spin_dir != RTB_SPIN_NONE;
   const bool f32 = j.precision == RTB_PREC_MIXED;  // FP32 eigensolve + FP64 refinement
   // the plain form, and any model whose storage is not exactly Hermitian,
   // forms the reference's A and B apart (V0, V1 kept apart); the mixed form
   // of a Hermitian model takes V0 + i V1 straight from the Fourier stage
-  const bool exact_kubo = !f32 || !m->herm || j.geometry;
+  const bool exact_kubo = !f32 || !m->herm || j.geometry || j.dipole;
   a.combine = spinj || exact_kubo ? 0 : 1;
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
-  const size_t per_k =
-      24 + n2 * 16 + n2 * 32 + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 + (f32 ? n2 * 16 + n * 8 : 0);
+  const size_t per_k = 24 + n2 * 16 + n2 * 16 * nv + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 +
+                       (f32 ? n2 * 16 + n * 8 : 0) + (j.dipole ? n2 * 16 + n * 24 : 0);
   const size_t K = chunk_for(j.nk, per_k);
   MeshPlan mp;
   double* G = nullptr;
   std::vector<Chunk> chunks;
-  RTB_TRY(mesh_setup(st, w, tmr, m, j.k ? nullptr : j.kmesh, j.precision, j.nk, K, true, mp, &G, chunks));
+  // the separable mesh passes form two projected velocities; the dipole's
+  // three take the dense sum over mesh points
+  RTB_TRY(mesh_setup(st, w, tmr, m, j.k || j.dipole ? nullptr : j.kmesh, j.precision, j.nk, K, true, mp, &G, chunks));
   double *dk, *dH, *dV, *dE, *dOm, *dJ = nullptr, *dZ;
   EigBufs eb;
   RTB_TRY(w.alloc(&dk, K * 3));
   RTB_TRY(w.alloc(&dH, K * n2 * 2));
-  RTB_TRY(w.alloc(&dV, K * n2 * 4));
+  RTB_TRY(w.alloc(&dV, K * n2 * 2 * nv));
   RTB_TRY(w.alloc(&dE, K * n));
   RTB_TRY(w.alloc(&dOm, K * n));
   double* dMet = nullptr;
   if (j.geometry) RTB_TRY(w.alloc(&dMet, K * n));
+  double *dA3 = nullptr, *dOm2 = nullptr, *dKern = nullptr;
+  if (j.dipole) {
+    RTB_TRY(w.alloc(&dA3, K * n2 * 2));
+    RTB_TRY(w.alloc(&dOm2, K * n));
+    RTB_TRY(w.alloc(&dKern, K * n));
+  }
   RTB_TRY(alloc_eig(w, n, K, true, eb));
   double* kub;
   RTB_TRY(w.alloc(&kub, K * n2 * 6));  // T1, A, B (complex)
@@ -750,8 +798,12 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
       RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, false));
       RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     }
-    RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, dJ, dOm, exact_kubo ? B : nullptr,
-                       dMet));
+    if (j.dipole)
+      RTB_TRY(dipole_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, j.sym, T1, Ab, B, dA3, dJ, dOm, dOm2,
+                           dKern));
+    else
+      RTB_TRY(kubo_chunk(st, tmr, m, c, dV, Zc, dsc, dE, j.spin_dir, j.eta, T1, Ab, dJ, dOm, exact_kubo ? B : nullptr,
+                         dMet));
     int l2 = 0;
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     if (j.mode == 0) {
@@ -765,8 +817,8 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
       RTB_HIP(hipMemcpyAsync(j.occ + k0, docc, c * sizeof(double), hipMemcpyDeviceToHost, st));
     } else {
       const int nb = hall_blocks(c);
-      RTB_TRY(hall_partials(st, n, c, dOm, dE, dmu, static_cast<int>(j.n_mu), j.T,
-                            dpart + static_cast<size_t>(blk_off) * j.n_mu, nb));
+      RTB_TRY(hall_partials(st, n, c, j.dipole ? dKern : dOm, dE, dmu, static_cast<int>(j.n_mu), j.T,
+                            dpart + static_cast<size_t>(blk_off) * j.n_mu, nb, j.dipole));
       ++l2;
       if (j.geometry) {
         RTB_TRY(hall_partials(st, n, c, dMet, dE, dmu, static_cast<int>(j.n_mu), j.T,
@@ -921,6 +973,43 @@ int rtb_quantum_geometry_direct(const rtb_model* m, const size_t* kmesh, const d
   j.sigma = berry;
   j.geometry = true;
   j.msum = metric;
+  j.precision = precision;
+  return berry_impl(m, j);
+}
+
+int rtb_nonlinear_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
+                              double eta, const double* dir, int spin_dir, int symmetrized, int precision,
+                              double* sigma) {
+  RTB_TRY(prepare(m));
+  RTB_TRY(check_precision(precision));
+  // extrinsic_nonlinear_hall (response/nonlinear/mod.rs:259-266): validate_rank3
+  // (config.rs:153-158), broadening, spin; the Direct sum needs T > 0 (:318-326)
+  size_t nk = 0;
+  RTB_TRY(check_kmesh(m, kmesh, &nk));
+  RTB_TRY(check_direction(m, dir, 3));
+  RTB_TRY(check_mu(mu, n_mu));
+  RTB_TRY(check_T(T));
+  RTB_TRY(check_eta(eta));
+  RTB_TRY(check_spin(m, spin_dir));
+  if (!(T > 0.0))
+    return fail(RTB_ERR_INVALID_THERMO, "T: direct nonlinear Hall integration requires a finite temperature");
+  if (!sigma) return fail(RTB_ERR_INVALID_ARGUMENT, "null sigma");
+  // symmetrized only when the two field rows differ (mod.rs:284-285)
+  bool differ = false;
+  for (int d = 0; d < m->dim; ++d) differ = differ || dir[m->dim + d] != dir[2 * m->dim + d];
+  BerryJob j;
+  j.kmesh = kmesh;
+  j.nk = nk;
+  j.dir = dir;
+  j.spin_dir = spin_dir;
+  j.eta = eta;
+  j.mode = 2;
+  j.T = T;
+  j.mu = mu;
+  j.n_mu = n_mu;
+  j.sigma = sigma;
+  j.dipole = true;
+  j.sym = symmetrized != 0 && differ;
   j.precision = precision;
   return berry_impl(m, j);
 }

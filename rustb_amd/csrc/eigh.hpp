@@ -136,8 +136,13 @@ int kubo_omega_exact(hipStream_t st, int n, size_t nk, const double* A, const do
 int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const double* E, double mu,
                  double T, double* out);
 // partial[blk][imu] += sum_{k in blk} sum_n omega f(E, mu_imu)   (fixed blocking)
+// deriv: the weights are -df/dE = f (1 - f) / (k_B T) instead of f (T > 0).
 int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const double* E,
-                  const double* mu, int n_mu, double T, double* partial, int n_blk);
+                  const double* mu, int n_mu, double T, double* partial, int n_blk, bool deriv = false);
+// Berry curvature dipole kernel per band: out = om1 * Re(A3_nn), or with sym
+// (om1 * Re(A3_nn) + om2 * Re(A2_nn)) / 2 (A2, A3: [nk][n][n], om: [nk][n]).
+int dipole_combine(hipStream_t st, int n, size_t nk, const double* A2, const double* A3, const double* om1,
+                   const double* om2, bool sym, double* out);
 int hall_blocks(size_t nk);
 // DOS partial sums: partial[blk][j] += sum_{eps in blk} g(E_j - eps)
 int dos_partials(hipStream_t st, size_t ne, const double* evals, double Emin, double dE, int En,

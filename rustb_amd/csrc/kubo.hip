@@ -94,9 +94,11 @@ __global__ void occupied_sum_kernel(int n, size_t nk, const double* __restrict__
 
 constexpr int kHallK = 256;  // k-points per hall partial block
 
+// deriv: weights -df/dE = f (1 - f) / width (fermi_derivative_from_width,
+// thermodynamics.rs:122-126) instead of f
 __global__ void hall_partials_kernel(int n, size_t nk, const double* __restrict__ omega,
                                      const double* __restrict__ E, const double* __restrict__ mu,
-                                     int n_mu, double width, double* __restrict__ partial) {
+                                     int n_mu, double width, int deriv, double* __restrict__ partial) {
   // block = (k-block, mu); 256 threads over k, then a fixed tree
   __shared__ double red[256];
   const int imu = blockIdx.y;
@@ -104,7 +106,10 @@ __global__ void hall_partials_kernel(int n, size_t nk, const double* __restrict_
   double s = 0.0;
   if (k < nk) {
     const double m = mu[imu];
-    for (int b = 0; b < n; ++b) s += omega[k * n + b] * fermi(E[k * n + b], m, width);
+    for (int b = 0; b < n; ++b) {
+      const double f = fermi(E[k * n + b], m, width);
+      s += omega[k * n + b] * (deriv ? f * (1.0 - f) / width : f);
+    }
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -215,6 +220,22 @@ __global__ void __launch_bounds__(256) kubo_omega_exact_kernel(int n, const doub
   }
 }
 
+// Berry curvature dipole kernel per band (berry_curvature_dipole_n_onek,
+// response/nonlinear/mod.rs:140-162): Omega^{c,f1}_n Re(A3_nn), and with
+// `sym` the mean with Omega^{c,f2}_n Re(A2_nn) (FieldSymmetry::Symmetrized,
+// mod.rs:284-300); A2 = U^H v_f1 U, A3 = U^H v_f2 U.
+__global__ void dipole_combine_kernel(int n, size_t nk, const double2* __restrict__ A2,
+                                      const double2* __restrict__ A3, const double* __restrict__ om1,
+                                      const double* __restrict__ om2, int sym, double* __restrict__ out) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= nk * n) return;
+  const size_t k = g / n;
+  const int b = static_cast<int>(g % n);
+  const size_t d = k * n * n + static_cast<size_t>(b) * n + b;
+  const double first = om1[g] * A3[d].x;
+  out[g] = sym ? (first + om2[g] * A2[d].x) * 0.5 : first;
+}
+
 }  // namespace
 
 int spin_current(hipStream_t st, int norb, size_t nb, int spin_dir, const double* v, size_t v_stride,
@@ -258,14 +279,25 @@ int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const do
   return RTB_OK;
 }
 
+int dipole_combine(hipStream_t st, int n, size_t nk, const double* A2, const double* A3, const double* om1,
+                   const double* om2, bool sym, double* out) {
+  const size_t total = nk * n;
+  if (total == 0) return RTB_OK;
+  hipLaunchKernelGGL(dipole_combine_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, n, nk,
+                     reinterpret_cast<const double2*>(A2), reinterpret_cast<const double2*>(A3), om1, om2,
+                     sym ? 1 : 0, out);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
 int hall_blocks(size_t nk) { return static_cast<int>((nk + kHallK - 1) / kHallK); }
 
 int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const double* E, const double* mu,
-                  int n_mu, double T, double* partial, int n_blk) {
+                  int n_mu, double T, double* partial, int n_blk, bool deriv) {
   if (nk == 0) return RTB_OK;
   hipLaunchKernelGGL(hall_partials_kernel, dim3(static_cast<unsigned>(n_blk), static_cast<unsigned>(n_mu)),
                      dim3(256), 0, st, n, nk, omega, E, mu, n_mu, T <= 0.0 ? 0.0 : T * 8.617333262145e-5,
-                     partial);
+                     deriv ? 1 : 0, partial);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -34,11 +34,18 @@ class Parameters:
     spin: Optional[str] = None
     integration: str = "Direct"
     precision: str = "fp64"
+    field_symmetry: str = "Symmetrized"  # FieldSymmetry (config.rs): or "Ordered"
 
     @classmethod
     def rank2(cls, kmesh, direction_a, direction_b, mu, **kw):
         """Parameters::rank2 (config.rs:95-102)."""
         return cls(tuple(kmesh), np.array([direction_a, direction_b], float), np.atleast_1d(np.asarray(mu, float)), **kw)
+
+    @classmethod
+    def rank3(cls, kmesh, current, field_1, field_2, mu, **kw):
+        """Parameters::rank3 (config.rs): direction rows (current, field_1, field_2)."""
+        return cls(tuple(kmesh), np.array([current, field_1, field_2], float),
+                   np.atleast_1d(np.asarray(mu, float)), **kw)
 
     @classmethod
     def at_mu(cls, kmesh, direction, mu: float, **kw):
@@ -66,6 +73,15 @@ class HallConductivityResult:
 
     def single(self):
         return float(self.conductivity[0]) if self.conductivity.size == 1 else None
+
+
+@dataclass
+class NonlinearHallResult:
+    """response/nonlinear/mod.rs:52-61."""
+
+    chemical_potentials: np.ndarray
+    conductivity: np.ndarray
+    diagnostics: Optional[dict] = None
 
 
 @dataclass
@@ -441,6 +457,34 @@ class Model:
                                                  float(params.eta), ptr(self._dir(params)),
                                                  PRECISION[params.precision], ptr(g), ptr(b)))
         return QuantumGeometryResult(mu.copy(), g, b, None)
+
+    def extrinsic_nonlinear_hall(self, params: Parameters) -> NonlinearHallResult:
+        """Model::extrinsic_nonlinear_hall, Integration::Direct
+        (response/nonlinear/mod.rs:256-347): the Berry curvature dipole
+        response; params.direction rows (current, field_1, field_2)."""
+        if params.integration == "Simplex":
+            raise TbError(-6, "integration: extrinsic_nonlinear_hall supports Integration::Direct or EnergyCut, "
+                              "not Simplex")
+        if params.integration != "Direct":
+            raise TbError(-14, "integration: this path implements Integration::Direct")
+        if params.field_symmetry not in ("Symmetrized", "Ordered"):
+            raise TbError(-12, "field_symmetry must be 'Symmetrized' or 'Ordered'")
+        kmesh = np.ascontiguousarray(np.asarray(params.kmesh, dtype=np.uint64))
+        if kmesh.shape != (self.dim,):
+            raise TbError(-3, "kmesh must have dim_r entries")
+        d = np.ascontiguousarray(np.asarray(params.direction, np.float64))
+        if d.shape != (3, self.dim):
+            raise TbError(-2, f"direction must have shape (3, {self.dim})")
+        mu = np.ascontiguousarray(np.atleast_1d(np.asarray(params.mu, np.float64)))
+        T = np.atleast_1d(np.asarray(params.T, float))
+        if T.size == 0:
+            raise TbError(-6, "T: must contain at least one value")
+        sig = np.zeros(mu.size)
+        check(load().rtb_nonlinear_hall_direct(self._device(), ptr(kmesh), ptr(mu), mu.size, float(T[0]),
+                                               float(params.eta), ptr(d), SPIN_DIR[params.spin],
+                                               int(params.field_symmetry == "Symmetrized"),
+                                               PRECISION[params.precision], ptr(sig)))
+        return NonlinearHallResult(mu.copy(), sig)
 
     def dos(self, k_mesh, E_min, E_max, E_n, sigma, precision="fp64"):
         """Model::dos (model_physics.rs:179-226) -> (E, rho)."""

@@ -281,3 +281,32 @@ def test_quantum_geometry_vs_oracle(precision):
     assert np.abs(q.berry_curvature - bm).max() <= rel * max(1.0, np.abs(bm).max())
     s = m2.hall_conductivity(ra.Parameters.rank2([7, 9], [1, 0], [0, 1], mu, eta=0.02, precision="fp64"))
     assert np.abs(q.berry_curvature - s.conductivity).max() <= rel * max(1.0, np.abs(bm).max())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nonlinear_hall_dipole_vs_oracle(precision):
+    """extrinsic_nonlinear_hall (Direct) through the C ABI: a C3-broken Haldane
+    model (symmetrized and ordered), and a spinful 3D model with a spin current."""
+    rel = 1e-9 if precision == "fp64" else 1e-6
+    m = RM.haldane(ra.Model.tb_model, t2_imag=-0.3, t=-1.0, delta=0.7)
+    m.add_hop(-0.4, 0, 1, [0, 0])
+    o = RM.haldane(O.Model, t2_imag=-0.3, t=-1.0, delta=0.7)
+    o.add_hop(-0.4, 0, 1, np.array([0, 0]))
+    mu = np.linspace(-1.5, 1.5, 7)
+    for sym in ("Symmetrized", "Ordered"):
+        p = ra.Parameters.rank3([9, 11], [1.0, 0.0], [0.0, 1.0], [1.0, 0.2], mu, T=np.array([600.0]), eta=0.05,
+                                precision=precision, field_symmetry=sym)
+        got = m.extrinsic_nonlinear_hall(p).conductivity
+        ref = O.extrinsic_nonlinear_hall_direct(o, [9, 11], p.direction, mu, 600.0, 0.05,
+                                                symmetrized=sym == "Symmetrized")
+        assert np.abs(got - ref).max() <= rel * max(1e-3, np.abs(ref).max()), sym
+    m3, o3 = random_pair(3, True, 3, 3, seed=71)
+    d3 = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.3, 1.0]])
+    p3 = ra.Parameters(kmesh=(3, 4, 3), direction=d3, mu=np.array([-0.3, 0.4]), T=np.array([900.0]), eta=0.02,
+                       spin="Z", precision=precision)
+    got = m3.extrinsic_nonlinear_hall(p3).conductivity
+    ref = O.extrinsic_nonlinear_hall_direct(o3, [3, 4, 3], d3, [-0.3, 0.4], 900.0, 0.02, spin="Z")
+    assert np.abs(got - ref).max() <= rel * max(1e-3, np.abs(ref).max())
+    with pytest.raises(ra.TbError) as e:
+        m.extrinsic_nonlinear_hall(ra.Parameters.rank3([3, 3], [1, 0], [0, 1], [1, 0], [0.0], precision=precision))
+    assert e.value.kind == "InvalidThermodynamicParameter"

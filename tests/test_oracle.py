@@ -327,3 +327,47 @@ def test_quantum_geometry_two_band_symmetry_and_direct_sum():
     assert np.abs(bm - sig).max() <= 1e-12 * max(1.0, np.abs(sig).max())
     gxx, _ = O.quantum_geometry_direct(o, [9, 9], np.eye(2)[[0, 0]], [0.0], 0.0, 0.05)
     assert gxx[0] > 0  # the metric of the occupied band is positive semi-definite
+
+
+def _haldane_c3_broken(t2):
+    """The Haldane model of build_haldane_2d (lib.rs:2356-2375) with one
+    nearest-neighbour bond strengthened: C3 broken, so the dipole is nonzero."""
+    m = RM.haldane(O.Model, t2_imag=t2, t=-1.0, delta=0.7)
+    m.add_hop(-0.4, 0, 1, np.array([0, 0]))
+    return m
+
+
+def test_dipole_kernel_is_band_velocity_times_curvature():
+    """berry_curvature_dipole_n_onek (mod.rs:89-163): kernel_n = d_c E_n *
+    Omega_n^{ab}; d_c E_n equals the central finite difference of the band
+    energy along Cartesian c (Hellmann-Feynman), Omega is berry_curvature_at."""
+    m = _haldane_c3_broken(-0.3)
+    k = np.array([0.13, 0.41])
+    a, b, c = np.array([1.0, 0.0]), np.array([0.0, 1.0]), np.array([0.6, 0.8])
+    kern, e = O.berry_curvature_dipole_onek(m, k, a, b, c, 1e-6)
+    om, _ = O.berry_curvature_batch(m, k[None, :], np.array([a, b]), 1e-6)
+    h = 1e-6
+    # a Cartesian step dk along c moves the fractional k by lat @ dk / (2 pi)
+    dkf = m.lat @ (h * c) / (2 * np.pi)
+    ep = O.eigvalsh(O.gen_ham(m, k + dkf, "atom"))
+    em = O.eigvalsh(O.gen_ham(m, k - dkf, "atom"))
+    dE = (ep - em) / (2 * h)
+    assert np.abs(kern - om[0] * dE).max() <= 1e-6 * max(1.0, np.abs(kern).max())
+    assert np.abs(kern).max() > 1e-3
+
+
+def test_dipole_time_reversal_even_and_symmetrization():
+    """The dipole is even under time reversal (Haldane t2 -> -t2), as the
+    reference's dipole_energy_cut_tr (lib.rs:2745-2790) asserts to 1e-10; the
+    Direct sum keeps it on the k -> -k symmetric mesh.  Ordered with equal
+    field rows equals Symmetrized."""
+    mu = np.linspace(-1.5, 1.5, 7)
+    d = np.array([[1.0, 0.0], [0.0, 1.0], [1.0, 0.0]])
+    D = O.extrinsic_nonlinear_hall_direct(_haldane_c3_broken(-0.3), [15, 15], d, mu, 600.0, 0.05)
+    Dtr = O.extrinsic_nonlinear_hall_direct(_haldane_c3_broken(0.3), [15, 15], d, mu, 600.0, 0.05)
+    assert np.abs(D).max() > 1e-4
+    assert np.abs(D - Dtr).max() <= 1e-10 * max(1.0, np.abs(D).max())
+    same = np.array([[1.0, 0.0], [0.0, 1.0], [0.0, 1.0]])
+    s1 = O.extrinsic_nonlinear_hall_direct(_haldane_c3_broken(-0.3), [9, 9], same, mu, 600.0, 0.05, symmetrized=True)
+    s2 = O.extrinsic_nonlinear_hall_direct(_haldane_c3_broken(-0.3), [9, 9], same, mu, 600.0, 0.05, symmetrized=False)
+    assert np.array_equal(s1, s2)
