@@ -48,6 +48,7 @@ extern "C" {
 #define RTB_ERR_NO_ORBITALS -20          /* TbError::NoOrbitals (error.rs:279) */
 #define RTB_ERR_LINALG -21               /* TbError::Linalg (error.rs:83): a singular lattice */
 #define RTB_ERR_INVALID_THERMO -22       /* TbError::InvalidThermodynamicParameter (thermodynamics.rs) */
+#define RTB_ERR_MISSING_CONJUGATE -23    /* TbError::MissingHermitianConjugate (error.rs:285) */
 
 /* ---- enums ---- */
 #define RTB_GAUGE_LATTICE 0 /* Gauge::Lattice (model.rs:449-453) */
@@ -123,7 +124,15 @@ typedef struct rtb_w90 rtb_w90;
  * the _hr.dat body (0: up to 16). */
 int rtb_w90_read(const char* path, const char* seedname, double zero_energy, int spin, int n_threads,
                  rtb_w90** out);
+/* Model::from_hr_with_rmatrix (wannier90.rs:1208-1211), R = HasRMatrix: also
+ * reads {path}{seedname}_r.dat (required) into the position matrices, moved
+ * with the Wigner-Seitz shifts and made Hermitian across +-R. */
+int rtb_w90_read_with_rmatrix(const char* path, const char* seedname, double zero_energy, int spin,
+                              int n_threads, rtb_w90** out);
 void rtb_w90_destroy(rtb_w90* m);
+/* Model::rmatrix (model.rs): n_R x 3 x nsta x nsta complex (RTB_ERR_INVALID_ARGUMENT
+ * for a model read without _r.dat). */
+int rtb_w90_export_rmatrix(const rtb_w90* m, double* rmatrix);
 /* Model::norb / nsta / hamR.nrows() / natom (model.rs:697-709). */
 int rtb_w90_info(const rtb_w90* m, int* spin, int* norb, int* nsta, int* n_R, int* natom);
 /* Model fields lat (3x3 rows, Angstrom), orb (norb x 3 fractional),
@@ -148,8 +157,14 @@ int rtb_model_create(int dim, int spin, int norb, int n_R, const int64_t* hamR,
                      const double* ham /*n_R*n*n complex*/, const double* orb /*norb*dim*/,
                      const double* lat /*dim*dim rows*/, rtb_model** out);
 void rtb_model_destroy(rtb_model* m);
-/* The device model of a loaded Wannier90 model (as rtb_model_create). */
+/* The device model of a loaded Wannier90 model (as rtb_model_create, with
+ * the position matrices when it was read with _r.dat). */
 int rtb_w90_model(const rtb_w90* w, rtb_model** out);
+/* Model<_, _, HasRMatrix>: position matrices rmatrix (n_R x dim x nsta x nsta
+ * complex, rows as hamR) for the velocity term i[H(k), r_p(k)]
+ * (velocity.rs:377-402 Atom gauge, diagonal of r_p zeroed; :645-668 and the
+ * Lattice branch).  Uploaded once; every velocity path then includes it. */
+int rtb_model_set_rmatrix(rtb_model* m, const double* rmatrix);
 int rtb_model_nsta(const rtb_model* m);
 
 /* Model::gen_ham (model_physics.rs:43-148) and Model::gen_v_projected

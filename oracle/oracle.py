@@ -101,6 +101,7 @@ class Model:
         # model_build.rs:244-245: ham starts as one zero block at R = 0
         self.ham = np.zeros((1, self.nsta, self.nsta), dtype=np.complex128)
         self.hamR = np.zeros((1, dim), dtype=np.int64)
+        self.rmatrix = None  # [N_R, dim, n, n] for Model<_, _, HasRMatrix>
 
     # model_utils.rs:18-34 (linear scan)
     def find_R(self, R):
@@ -209,11 +210,34 @@ class Model:
         return np.concatenate([self.orb, self.orb], axis=0) if self.spin else self.orb
 
 
-def model_from_arrays(lat, orb, spin, hamR, ham) -> Model:
+def model_from_arrays(lat, orb, spin, hamR, ham, rmatrix=None) -> Model:
+    """rmatrix: [N_R, dim, n, n] position matrices (Model<_, _, HasRMatrix>)."""
     m = Model(lat, orb, spin)
     m.hamR = np.ascontiguousarray(hamR, dtype=np.int64)
     m.ham = np.ascontiguousarray(ham, dtype=np.complex128)
+    m.rmatrix = None if rmatrix is None else np.ascontiguousarray(rmatrix, dtype=np.complex128)
     return m
+
+
+def _rmatrix_term(model: Model, us, gauge_mat, hamk, directions):
+    """The HasRMatrix term of gen_v_projected (velocity.rs:645-668; Lattice
+    branch without the gauge factor and the diagonal zeroing): rk[d] =
+    sum_R U_R rmatrix[R][d]; for each direction rk_p = sum_d w_d rk[d]
+    (w_d != 0), Atom gauge applied and its diagonal zeroed; returns the
+    i [hamk, rk_p] to add to v_p."""
+    rm = model.rmatrix
+    rk = np.einsum("r,rdij->dij", us[:rm.shape[0]], rm)
+    out = []
+    for w in directions:
+        rp = np.zeros(rk.shape[1:], complex)
+        for d in range(model.dim):
+            if w[d] != 0.0:
+                rp += w[d] * rk[d]
+        if gauge_mat is not None:
+            rp = rp * gauge_mat
+            np.fill_diagonal(rp, 0.0)
+        out.append(1j * (hamk @ rp - rp @ hamk))
+    return out
 
 
 # --------------------------------------------------------------------------
@@ -286,7 +310,8 @@ def gen_ham(model: Model, k, gauge: str = "atom"):
 
 
 def gen_v_projected(model: Model, k, gauge: str, directions):
-    """gen_v_projected (velocity.rs:470-731), NoRMatrix branch.
+    """gen_v_projected (velocity.rs:470-731), NoRMatrix branch, plus the
+    HasRMatrix term when the model carries ``rmatrix`` (velocity.rs:645-668).
 
     v_p = D^* [ sum_R i (w_p . R_cart) U_R h_R + i (w_p.tau_n - w_p.tau_m) H_lat ] D
     with R_cart = R . lat (velocity.rs:530-531) and tau_cart = tau . lat
@@ -313,7 +338,10 @@ def gen_v_projected(model: Model, k, gauge: str, directions):
             if r_dot_w != 0.0:
                 vv += (us[iR] * 1j * r_dot_w) * model.ham[iR]
         v[p] = vv
+    rmat = getattr(model, "rmatrix", None) is not None
     if gauge == "lattice":
+        if rmat:
+            v = v + np.array(_rmatrix_term(model, us, None, hamk, directions))
         return v, hamk
     phi = _phases_orb(model, k)
     gauge_mat = np.conj(phi)[:, None] * phi[None, :]
@@ -321,7 +349,10 @@ def gen_v_projected(model: Model, k, gauge: str, directions):
     for p in range(n_proj):  # velocity.rs:602-610
         diff = tau_proj[p][None, :] - tau_proj[p][:, None]  # [m, n] = tau_n - tau_m
         v[p] = (v[p] + 1j * diff * hamk) * gauge_mat
-    return v, hamk * gauge_mat  # velocity.rs:616-622
+    hamk = hamk * gauge_mat  # velocity.rs:616-622
+    if rmat:
+        v = v + np.array(_rmatrix_term(model, us, gauge_mat, hamk, directions))
+    return v, hamk
 
 
 def gen_v(model: Model, k, gauge: str = "atom"):
@@ -642,7 +673,11 @@ def gen_v_projected_batch(model: Model, kvec, directions):
         vR = ((U * (1j * wR[p])[None, :]) @ hflat).reshape(-1, n, n)
         diff = tau_proj[p][None, :] - tau_proj[p][:, None]
         vs.append((vR + 1j * diff[None] * Hl) * gm)
-    return np.stack(vs, axis=1), Hl * gm
+    v, H = np.stack(vs, axis=1), Hl * gm
+    if getattr(model, "rmatrix", None) is not None:
+        for ik in range(kvec.shape[0]):
+            v[ik] += np.array(_rmatrix_term(model, U[ik], gm[ik], H[ik], directions))
+    return v, H
 
 
 def berry_curvature_batch(model: Model, kvec, direction, eta, spin=None):

@@ -174,14 +174,16 @@ class W90Model:
     hamR: np.ndarray
     ham: np.ndarray
     nsta: int = field(default=0)
+    rmatrix: np.ndarray = None  # [N_R, 3, n, n] with _r.dat
 
     @property
     def norb(self):
         return self.orb.shape[0]
 
 
-def from_hr(path, file_name, zero_energy, spin=None):
-    """Wannier90::from_hr (wannier90.rs:49-1228); spin None takes the file's."""
+def from_hr(path, file_name, zero_energy, spin=None, rmatrix=False):
+    """Wannier90::from_hr (wannier90.rs:49-1228); spin None takes the file's;
+    rmatrix=True is R = HasRMatrix (from_hr_with_rmatrix, wannier90.rs:1208)."""
     prefix = path + file_name
     hr_path = prefix + "_hr.dat"
 
@@ -461,7 +463,64 @@ def from_hr(path, file_name, zero_energy, spin=None):
             fp(win_path, f"species mismatch between begin atoms_frac and the projections block: {detail}")
         orb = np.array(rows, dtype=float).reshape(-1, 3)
 
-    # _wsvec.dat (wannier90.rs:904-1133, the branch without r-matrix)
+    # _r.dat (wannier90.rs:805-902)
+    rmat = None
+    if rmatrix:
+        r_path = prefix + "_r.dat"
+        try:
+            rl = _read_lines(r_path)
+        except UnicodeDecodeError:
+            fp(xyz_path, "Failed to read line: stream did not contain valid UTF-8")
+        except OSError:
+            raise W90Error("FileCreation", f"Failed to create file '{r_path}': R::HAS_RMATRIX=true but _r.dat "
+                                           "file not found")
+        if len(rl) < 3:
+            fp(r_path, "Truncated _r.dat: missing the R-vector count line")
+        nRr, e = _parse_usize(_trim(rl[2]))
+        if e:
+            fp(r_path, f"Failed to parse n_R: {e}")
+        rmat = np.zeros((len(hamR), 3, nsta, nsta), complex)
+        index_of = {}
+        for r, Rv in enumerate(hamR):
+            index_of.setdefault(Rv, r)
+
+        def rline(i):
+            if i >= len(rl):
+                fp(r_path, "Truncated _r.dat")
+            return rl[i]
+
+        for i in range(nRr if nsta else 0):
+            t = _split(_trim(rline(i * nsta * nsta + 3)))
+            Rv = []
+            for d in range(3):
+                if d >= len(t):
+                    fp(r_path, "Missing R vector component")
+                v, e = _parse_isize(t[d])
+                if e:
+                    fp(r_path, f"Failed to parse R vector: {e}")
+                Rv.append(v)
+            if tuple(Rv) not in index_of:
+                fp(r_path, f"R vector [{Rv[0]}, {Rv[1]}, {Rv[2]}] not found in Hamiltonian")
+            idx = index_of[tuple(Rv)]
+            if i >= len(weights):
+                fp(r_path, "Truncated _r.dat: missing degeneracy weight")
+            w = float(weights[i])
+            for ind_i in range(nsta):
+                for ind_j in range(nsta):
+                    t = _split(_trim(rline(i * nsta * nsta + ind_i * nsta + ind_j + 3)))
+                    for d in range(3):
+                        vals = []
+                        for c, part in ((0, "real"), (1, "imaginary")):
+                            pos = 5 + 2 * d + c
+                            if pos >= len(t):
+                                fp(r_path, f"Missing R matrix {part} part")
+                            x = parse_f64(t[pos])
+                            if x is None:
+                                fp(r_path, f"Failed to parse R matrix {part} part: invalid float literal")
+                            vals.append(x)
+                        rmat[idx, d, ind_j, ind_i] = complex(_div(vals[0], w), _div(vals[1], w))
+
+    # _wsvec.dat (wannier90.rs:904-1133; with an r-matrix it moves with the hopping)
     ws_path = prefix + "_wsvec.dat"
     try:
         ws = _read_lines(ws_path)
@@ -474,6 +533,7 @@ def from_hr(path, file_name, zero_energy, spin=None):
             fp(ws_path, "Truncated _wsvec.dat: empty file")
         new_R = [(0, 0, 0)]
         new_ham = [np.zeros((nsta, nsta), complex)]
+        new_rm = [np.zeros((3, nsta, nsta), complex)] if rmat is not None else None
         index_of = {}
         for r, Rv in enumerate(hamR):
             index_of.setdefault(Rv, r)  # find_R: first match
@@ -517,6 +577,9 @@ def from_hr(path, file_name, zero_energy, spin=None):
                 fp(ws_path, f"R vector [{Rv[0]}, {Rv[1]}, {Rv[2]}] not found in Hamiltonian")
             h = ham[index_of[Rt]][oi[0], oi[1]]
             hop = complex(_div(h.real, float(weight)), _div(h.imag, float(weight)))
+            if rmat is not None:
+                hr = [rmat[index_of[Rt], d, oi[0], oi[1]] for d in range(3)]
+                hop_r = [complex(_div(x.real, float(weight)), _div(x.imag, float(weight))) for x in hr]
             for _ in range(weight):
                 i += 1
                 t = _split(_trim(wline(i)))
@@ -530,16 +593,46 @@ def from_hr(path, file_name, zero_energy, spin=None):
                 if nR in new_R:
                     k = new_R.index(nR)
                     new_ham[k][oi[0], oi[1]] += hop
+                    if rmat is not None:
+                        for d in range(3):
+                            new_rm[k][d, oi[0], oi[1]] += hop_r[d]
                 else:
                     blk = np.zeros((nsta, nsta), complex)
-                    blk[oi[0], oi[1]] = hop
+                    if rmat is not None:  # this branch adds to zero (wannier90.rs:1026-1031)
+                        blk[oi[0], oi[1]] += hop
+                        rb = np.zeros((3, nsta, nsta), complex)
+                        for d in range(3):
+                            rb[d, oi[0], oi[1]] += hop_r[d]
+                        new_rm.append(rb)
+                    else:
+                        blk[oi[0], oi[1]] = hop
                     new_R.append(nR)
                     new_ham.append(blk)
         hamR, ham = new_R, new_ham
+        if rmat is not None:
+            rmat = np.array(new_rm)
+
+    if rmat is not None:  # wannier90.rs:1134-1160, rows 0 .. nR - 2 only
+        pos = {}
+        for r, Rv in enumerate(hamR):
+            pos.setdefault(tuple(Rv), r)
+        for r in range(len(hamR) - 1):
+            Rv = tuple(hamR[r])
+            inv = tuple(-x for x in Rv)
+            if inv not in pos:
+                raise W90Error("MissingHermitianConjugate", f"Missing Hermitian conjugate for R vector: {list(Rv)}")
+            q = pos[inv]
+            for i in range(nsta):
+                for j in range(nsta):
+                    for d in range(3):
+                        a, b = rmat[r, d, i, j], rmat[q, d, j, i]
+                        a = complex((a.real + b.real) / 2.0, (a.imag + -b.imag) / 2.0)
+                        rmat[r, d, i, j] = a
+                        rmat[q, d, j, i] = complex(a.real, -a.imag)
 
     m = W90Model(spin=file_spin, lat=lat, orb=np.asarray(orb, float).reshape(-1, 3), orb_projection=orb_proj,
                  atoms=atoms, hamR=np.array(hamR, dtype=np.int64).reshape(-1, 3),
-                 ham=np.array(ham, dtype=complex).reshape(-1, nsta, nsta), nsta=nsta)
+                 ham=np.array(ham, dtype=complex).reshape(-1, nsta, nsta), nsta=nsta, rmatrix=rmat)
     _validate(m)
     return m
 

@@ -36,6 +36,7 @@ ERRORS = {
     -20: "NoOrbitals",
     -21: "Linalg",
     -22: "InvalidThermodynamicParameter",
+    -23: "MissingHermitianConjugate",
 }
 
 
@@ -97,7 +98,10 @@ SIGNATURES = {
     "rtb_nonlinear_hall_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, c_int,
                                           _P]),
     "rtb_w90_read": (c_int, [c_char_p, c_char_p, c_double, c_int, c_int, POINTER(c_void_p)]),
+    "rtb_w90_read_with_rmatrix": (c_int, [c_char_p, c_char_p, c_double, c_int, c_int, POINTER(c_void_p)]),
     "rtb_w90_destroy": (None, [c_void_p]),
+    "rtb_w90_export_rmatrix": (c_int, [c_void_p, _P]),
+    "rtb_model_set_rmatrix": (c_int, [c_void_p, _P]),
     "rtb_w90_info": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int),
                              POINTER(c_int)]),
     "rtb_w90_export": (c_int, [c_void_p, _P, _P, _P, _P, _P]),

@@ -496,7 +496,21 @@ void rtb_model_destroy(rtb_model* m) {
   (void)hipFree(m->d_Rcart);
   (void)hipFree(m->d_tau);
   (void)hipFree(m->d_taucart);
+  (void)hipFree(m->d_rmat);
   delete m;
+}
+
+int rtb_model_set_rmatrix(rtb_model* m, const double* rmatrix) {
+  if (!m || !rmatrix) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  const size_t count = static_cast<size_t>(m->nR) * m->dim * m->nsta * m->nsta * 2;
+  for (size_t i = 0; i < count; ++i)
+    if (!std::isfinite(rmatrix[i])) return fail(RTB_ERR_INVALID_MODEL, "position matrices must be finite");
+  if (!m->d_rmat && hipMalloc(&m->d_rmat, count * sizeof(double)) != hipSuccess) {
+    m->d_rmat = nullptr;
+    return fail(RTB_ERR_DEVICE, "device allocation for the position matrices failed");
+  }
+  RTB_HIP(hipMemcpy(m->d_rmat, rmatrix, count * sizeof(double), hipMemcpyHostToDevice));
+  return RTB_OK;
 }
 
 int rtb_model_nsta(const rtb_model* m) { return m ? m->nsta : RTB_ERR_INVALID_ARGUMENT; }
@@ -519,11 +533,18 @@ int rtb_gen_ham_batch(const rtb_model* m, const double* k, size_t nk, int gauge,
   a.gauge = gauge;
   const int n = m->nsta;
   const size_t n2 = static_cast<size_t>(n) * n;
-  const size_t K = chunk_for(nk, (1 + n_proj) * n2 * 16 + 24);
-  double *dk, *dH, *dV = nullptr;
+  const bool rterm = m->d_rmat && n_proj > 0;  // the position-matrix term of v
+  const size_t K = chunk_for(nk, (1 + n_proj + (rterm ? 3 : 0)) * n2 * 16 + 24);
+  double *dk, *dH, *dV = nullptr, *rproj = nullptr, *Rk = nullptr, *T1 = nullptr, *T2 = nullptr;
   RTB_TRY(w.alloc(&dk, K * 3));
   RTB_TRY(w.alloc(&dH, K * n2 * 2));
   if (n_proj) RTB_TRY(w.alloc(&dV, K * n_proj * n2 * 2));
+  if (rterm) {
+    RTB_TRY(w.alloc(&rproj, static_cast<size_t>(m->nR) * n2 * 2));
+    RTB_TRY(w.alloc(&Rk, K * n2 * 2));
+    RTB_TRY(w.alloc(&T1, K * n2 * 2));
+    RTB_TRY(w.alloc(&T2, K * n2 * 2));
+  }
   std::vector<double> kp;
   for (size_t k0 = 0; k0 < nk; k0 += K) {
     const size_t c = std::min(K, nk - k0);
@@ -534,6 +555,11 @@ int rtb_gen_ham_batch(const rtb_model* m, const double* k, size_t nk, int gauge,
     RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
     RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
     RTB_TRY(tmr.end(1));
+    if (rterm) {
+      RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+      RTB_TRY(rmatrix_velocity(st, a, m->d_rmat, dirs, dk, c, dH, dV, rproj, Rk, T1, T2));
+      RTB_TRY(tmr.end(1));
+    }
     RTB_HIP(hipMemcpyAsync(H + k0 * n2 * 2, dH, c * n2 * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     if (n_proj)
       RTB_HIP(hipMemcpyAsync(V + k0 * n_proj * n2 * 2, dV, c * n_proj * n2 * 2 * sizeof(double),
@@ -707,7 +733,8 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   // the plain form, and any model whose storage is not exactly Hermitian,
   // forms the reference's A and B apart (V0, V1 kept apart); the mixed form
   // of a Hermitian model takes V0 + i V1 straight from the Fourier stage
-  const bool exact_kubo = !f32 || !m->herm || j.geometry || j.dipole;
+  const bool rterm = m->d_rmat != nullptr;  // the position-matrix term of v
+  const bool exact_kubo = !f32 || !m->herm || j.geometry || j.dipole || rterm;
   a.combine = spinj || exact_kubo ? 0 : 1;
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const size_t per_k = 24 + n2 * 16 + n2 * 16 * nv + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 +
@@ -718,7 +745,8 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   std::vector<Chunk> chunks;
   // the separable mesh passes form two projected velocities; the dipole's
   // three take the dense sum over mesh points
-  RTB_TRY(mesh_setup(st, w, tmr, m, j.k || j.dipole ? nullptr : j.kmesh, j.precision, j.nk, K, true, mp, &G, chunks));
+  RTB_TRY(mesh_setup(st, w, tmr, m, j.k || j.dipole || rterm ? nullptr : j.kmesh, j.precision, j.nk, K, true, mp,
+                     &G, chunks));
   double *dk, *dH, *dV, *dE, *dOm, *dJ = nullptr, *dZ;
   EigBufs eb;
   RTB_TRY(w.alloc(&dk, K * 3));
@@ -748,6 +776,8 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   double* T1 = kub;
   double* A = kub + K * n2 * 2;
   double* B = kub + K * n2 * 4;
+  double* rproj = nullptr;
+  if (rterm) RTB_TRY(w.alloc(&rproj, static_cast<size_t>(m->nR) * n2 * 2));
   double *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr, *dmpart = nullptr, *dmsig = nullptr;
   int total_blk = 0;
   if (j.mode == 2) {
@@ -784,6 +814,11 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
       }
       RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
       RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
+      RTB_TRY(tmr.end(1));
+    }
+    if (rterm) {  // the Kubo buffers are free until the eigensolve
+      RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+      RTB_TRY(rmatrix_velocity(st, a, m->d_rmat, j.dir, dk, c, dH, dV, rproj, T1, A, B));
       RTB_TRY(tmr.end(1));
     }
     double* Zc = dZ;  // eigenvectors (columns) for the Kubo products

@@ -56,6 +56,13 @@ struct FourierArgs {
 // np == 0; [nk][n][n] with a.combine)
 int fourier_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t nk, double* H,
                   double* V);
+// The position-matrix term of the projected velocities (velocity.rs:377-402,
+// 627-668): V[k][p] += i [H(k), r_p(k)] with r_p(k) the Bloch sum (same
+// phases and gauge as H) of sum_d dirs[p][d] rmat[R][d] and, in the Atom
+// gauge, its diagonal zeroed.  rmat: [nR][dim][n][n]; V: [nk][np][n][n] (not
+// combined); rproj: nR n^2 complex scratch; Rk, T1, T2: nk n^2 complex.
+int rmatrix_velocity(hipStream_t st, const FourierArgs& a, const double* rmat, const double* dirs, const double* k,
+                     size_t nk, const double* H, double* V, double* rproj, double* Rk, double* T1, double* T2);
 // Axis-separable Bloch sum on a uniform mesh (fourier_mesh.hip).
 struct MeshPlan {
   int dim = 0;

@@ -199,6 +199,81 @@ int fourier_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t 
   }
 }
 
+// ---- the position-matrix term of the velocity (models with an r-matrix) ----
+namespace {
+
+// rproj[R][e] = sum_d w_d rmat[R][d][e] (e over the n^2 elements)
+__global__ void rmatrix_project_kernel(size_t total, int dim, size_t n2, const double2* __restrict__ rmat, double w0,
+                                       double w1, double w2, double2* __restrict__ out) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= total) return;
+  const size_t r = g / n2, e = g % n2;
+  const double w[3] = {w0, w1, w2};
+  double re = 0.0, im = 0.0;
+  for (int d = 0; d < dim; ++d) {
+    const double2 x = rmat[(r * dim + d) * n2 + e];
+    re += w[d] * x.x;
+    im += w[d] * x.y;
+  }
+  out[g] = make_double2(re, im);
+}
+
+__global__ void zero_diag_kernel(size_t nk, int n, double2* __restrict__ M) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= nk * n) return;
+  const size_t k = g / n, i = g % n;
+  M[k * n * n + i * n + i] = make_double2(0.0, 0.0);
+}
+
+// V[k][p] += i (T1 - T2)
+__global__ void add_i_commutator_kernel(size_t total, size_t n2, int np, int p, const double2* __restrict__ T1,
+                                        const double2* __restrict__ T2, double2* __restrict__ V) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= total) return;
+  const size_t k = g / n2, e = g % n2;
+  const double2 a = T1[g], b = T2[g];
+  double2* v = V + (k * np + p) * n2 + e;
+  const double cr = a.x - b.x, ci = a.y - b.y;
+  *v = make_double2(v->x - ci, v->y + cr);
+}
+
+unsigned blocks_for(size_t total) { return static_cast<unsigned>((total + 255) / 256); }
+
+}  // namespace
+
+int rmatrix_velocity(hipStream_t st, const FourierArgs& a, const double* rmat, const double* dirs, const double* k,
+                     size_t nk, const double* H, double* V, double* rproj, double* Rk, double* T1, double* T2) {
+  if (nk == 0 || a.np == 0) return RTB_OK;
+  const int n = a.n, dim = a.dim;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  for (int p = 0; p < a.np; ++p) {
+    double w[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < dim; ++d) w[d] = dirs[p * dim + d];
+    const size_t tot = static_cast<size_t>(a.nR) * n2;
+    hipLaunchKernelGGL(rmatrix_project_kernel, dim3(blocks_for(tot)), dim3(256), 0, st, tot, dim, n2,
+                       reinterpret_cast<const double2*>(rmat), w[0], w[1], w[2], reinterpret_cast<double2*>(rproj));
+    RTB_HIP(hipGetLastError());
+    // r_p(k): the same Bloch sum and gauge factor as H(k)
+    FourierArgs b = a;
+    b.np = 0;
+    b.combine = 0;
+    b.ham = rproj;
+    RTB_TRY(fourier_dense(st, b, k, nk, Rk, nullptr));
+    if (a.gauge == RTB_GAUGE_ATOM) {  // velocity.rs:395, 663: the diagonal of r_p is dropped
+      hipLaunchKernelGGL(zero_diag_kernel, dim3(blocks_for(nk * n)), dim3(256), 0, st, nk, n,
+                         reinterpret_cast<double2*>(Rk));
+      RTB_HIP(hipGetLastError());
+    }
+    RTB_TRY(zgemm_batched(st, n, nk, H, n2, false, Rk, n2, T1, n2));
+    RTB_TRY(zgemm_batched(st, n, nk, Rk, n2, false, H, n2, T2, n2));
+    hipLaunchKernelGGL(add_i_commutator_kernel, dim3(blocks_for(nk * n2)), dim3(256), 0, st, nk * n2, n2, a.np, p,
+                       reinterpret_cast<const double2*>(T1), reinterpret_cast<const double2*>(T2),
+                       reinterpret_cast<double2*>(V));
+    RTB_HIP(hipGetLastError());
+  }
+  return RTB_OK;
+}
+
 int mesh_points(hipStream_t st, int dim, const size_t* kmesh, size_t k0, size_t nk, double* k) {
   if (nk == 0) return RTB_OK;
   unsigned long long n1 = kmesh[0], n2 = dim >= 2 ? kmesh[1] : 1, n3 = dim >= 3 ? kmesh[2] : 1;

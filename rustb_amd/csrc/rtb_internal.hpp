@@ -135,6 +135,7 @@ struct rtb_model {
   std::vector<double> lat;    // host copy [dim*dim]
   double det_lat = 0.0;
   bool herm = true;           // ham[-R] == ham[R]^H bit for bit, for every R
+  double* d_rmat = nullptr;   // [nR][dim][n][n] complex position matrices (or null)
   double* d_ham = nullptr;    // [nR][n][n] complex (re, im)
   double* d_R = nullptr;      // [nR][3]   R as doubles (zero padded)
   double* d_Rcart = nullptr;  // [nR][3]   R . lat

@@ -489,26 +489,88 @@ struct rtb_w90 {
   std::vector<Atom> atoms;
   std::vector<int64_t> hamR;    // [nR][3], row 0 = R = 0
   double* ham = nullptr;        // [nR][nsta][nsta] complex
-  ~rtb_w90() { std::free(ham); }
+  std::vector<uint64_t> weights;  // the _hr.dat degeneracies, file order
+  double* rmat = nullptr;       // [nR][3][nsta][nsta] complex (with _r.dat)
+  ~rtb_w90() {
+    std::free(ham);
+    std::free(rmat);
+  }
 };
 
 namespace {
 
-struct HrBody {
-  int nsta = 0;
-  size_t nR = 0;
-  std::vector<uint64_t> weights;
-  std::vector<int64_t> R;  // [nR][3] in file order
-  double* ham = nullptr;   // [slots][n][n] complex
-  size_t slots = 0;
-  std::vector<size_t> slot_of;  // file block -> slot
-};
-
-// One parse thread's range of the _hr.dat body and its first error.
+// One parse thread's range of a body and its first error (key: the
+// reference's order, line-major, then field group).
 struct RangeErr {
-  uint64_t key = ~0ull;  // 2 * line + (0: R fields, 1: matrix fields)
+  uint64_t key = ~0ull;
   std::string msg;
 };
+
+// The body of a line-oriented file split for host threads: T byte ranges
+// [cut[t], cut[t+1]) that start at line starts, the global line number of
+// each range's first line (a newline count per range; n_line is the number
+// of the line at b0), and whether any range is not UTF-8.
+struct LineRanges {
+  int T = 1;
+  std::vector<size_t> cut;
+  std::vector<uint64_t> first_line;
+  uint64_t total_lines = 0;
+  bool bad_utf8 = false;
+};
+
+template <typename F>
+void run_ranges(int T, F&& fn) {
+  std::vector<std::thread> th;
+  for (int t = 1; t < T; ++t) th.emplace_back(fn, t);
+  fn(0);
+  for (auto& x : th) x.join();
+}
+
+LineRanges split_lines(const char* buf, size_t S, size_t b0, uint64_t n_line, int n_threads) {
+  LineRanges lr;
+  int T = n_threads > 0 ? n_threads : std::min(16, static_cast<int>(std::thread::hardware_concurrency()));
+  T = std::max(1, std::min(T, 64));
+  const size_t body = S - b0;
+  if (body < static_cast<size_t>(T) * 4096) T = 1;
+  lr.T = T;
+  lr.cut.assign(T + 1, 0);
+  lr.cut[0] = b0;
+  lr.cut[T] = S;
+  for (int t = 1; t < T; ++t) {
+    size_t c = b0 + body / T * t;
+    const char* nl = c < S ? static_cast<const char*>(std::memchr(buf + c, '\n', S - c)) : nullptr;
+    c = nl ? static_cast<size_t>(nl - buf) + 1 : S;
+    lr.cut[t] = std::max(c, lr.cut[t - 1]);
+  }
+  std::vector<uint64_t> nlines(T, 0);
+  std::vector<char> bad(T, 0);
+  run_ranges(T, [&](int t) {
+    const char* p = buf + lr.cut[t];
+    const char* e = buf + lr.cut[t + 1];
+    // branch-free so it vectorises: newlines and the OR of all bytes
+    uint64_t c = 0;
+    unsigned char hi = 0;
+    const size_t len = static_cast<size_t>(e - p);
+    const unsigned char* uq = reinterpret_cast<const unsigned char*>(p);
+    for (size_t j = 0; j < len; ++j) {
+      c += uq[j] == '\n';
+      hi |= uq[j];
+    }
+    if (t == T - 1 && e > p && e[-1] != '\n') ++c;  // last line without '\n'
+    nlines[t] = c;
+    if ((hi & 0x80) &&
+        !valid_utf8(reinterpret_cast<const unsigned char*>(p), reinterpret_cast<const unsigned char*>(e)))
+      bad[t] = 1;
+  });
+  lr.first_line.assign(T + 1, 0);
+  lr.first_line[0] = n_line;
+  for (int t = 0; t < T; ++t) {
+    lr.first_line[t + 1] = lr.first_line[t] + nlines[t];
+    lr.bad_utf8 = lr.bad_utf8 || bad[t];
+  }
+  lr.total_lines = lr.first_line[T];
+  return lr;
+}
 
 int parse_hr(const std::string& file, int n_threads, double zero_energy, rtb_w90& out, Error& err) {
   Mapped mf;
@@ -601,59 +663,19 @@ int parse_hr(const std::string& file, int n_threads, double zero_energy, rtb_w90
   }
   (void)header_end;
 
-  // body ranges cut at line ends; pass 1 counts newlines (and checks UTF-8)
   const size_t b0 = n_line ? body_off : 0;
-  int T = n_threads > 0 ? n_threads : std::min(16, static_cast<int>(std::thread::hardware_concurrency()));
-  T = std::max(1, std::min(T, 64));
-  const size_t body = S - b0;
-  if (body < static_cast<size_t>(T) * 4096) T = 1;
-  std::vector<size_t> cut(T + 1);
-  cut[0] = b0;
-  cut[T] = S;
-  for (int t = 1; t < T; ++t) {
-    size_t c = b0 + body / T * t;
-    const char* nl = c < S ? static_cast<const char*>(std::memchr(buf + c, '\n', S - c)) : nullptr;
-    c = nl ? static_cast<size_t>(nl - buf) + 1 : S;
-    cut[t] = std::max(c, cut[t - 1]);
+  const LineRanges lr = split_lines(buf, S, b0, n_line, n_threads);
+  if (lr.bad_utf8) {
+    err.code = RTB_ERR_IO;
+    err.msg = "I/O error: stream did not contain valid UTF-8";
+    return err.code;
   }
-  std::vector<uint64_t> nlines(T, 0);
-  std::vector<char> bad_utf8(T, 0);
-  auto count_range = [&](int t) {
-    const char* p = buf + cut[t];
-    const char* e = buf + cut[t + 1];
-    // branch-free so it vectorises: newlines and the OR of all bytes
-    uint64_t c = 0;
-    unsigned char hi = 0;
-    const size_t len = static_cast<size_t>(e - p);
-    const unsigned char* uq = reinterpret_cast<const unsigned char*>(p);
-    for (size_t j = 0; j < len; ++j) {
-      c += uq[j] == '\n';
-      hi |= uq[j];
-    }
-    const bool high = hi & 0x80;
-    if (t == T - 1 && e > p && e[-1] != '\n') ++c;  // last line without '\n'
-    nlines[t] = c;
-    if (high && !valid_utf8(reinterpret_cast<const unsigned char*>(p), reinterpret_cast<const unsigned char*>(e)))
-      bad_utf8[t] = 1;
-  };
-  {
-    std::vector<std::thread> th;
-    for (int t = 1; t < T; ++t) th.emplace_back(count_range, t);
-    count_range(0);
-    for (auto& x : th) x.join();
-  }
-  for (int t = 0; t < T; ++t)
-    if (bad_utf8[t]) {
-      err.code = RTB_ERR_IO;
-      err.msg = "I/O error: stream did not contain valid UTF-8";
-      return err.code;
-    }
+  const int T = lr.T;
+  const std::vector<size_t>& cut = lr.cut;
+  const std::vector<uint64_t>& first_line = lr.first_line;
   if (!header_err.empty()) return file_parse(err, file, header_err);
   if (!weight_err.empty()) return file_parse(err, file, weight_err);
-  std::vector<uint64_t> first_line(T + 1, 0);
-  first_line[0] = n_line;
-  for (int t = 0; t < T; ++t) first_line[t + 1] = first_line[t] + nlines[t];
-  const uint64_t total_lines = first_line[T];
+  const uint64_t total_lines = lr.total_lines;
 
   const uint64_t n2 = nsta * nsta;
   if (nsta > 0 && (nsta > (1u << 20) || nR > (1ull << 40) / std::max<uint64_t>(n2, 1)))
@@ -757,10 +779,7 @@ int parse_hr(const std::string& file, int n_threads, double zero_energy, rtb_w90
       }
     }
   } else {
-    std::vector<std::thread> th;
-    for (int t = 1; t < T; ++t) th.emplace_back(parse_range, t);
-    parse_range(0);
-    for (auto& x : th) x.join();
+    run_ranges(T, parse_range);
     const RangeErr* first = nullptr;
     for (const auto& r : rerr)
       if (r.key != ~0ull && (!first || r.key < first->key)) first = &r;
@@ -812,6 +831,190 @@ int parse_hr(const std::string& file, int n_threads, double zero_energy, rtb_w90
   out.ham = ham;
   out.nsta = static_cast<int>(nsta);
   out.nR = static_cast<int>(nfinal);
+  out.weights = std::move(weights);
+  return RTB_OK;
+}
+
+// seedname_r.dat (wannier90.rs:805-902): header, num_wann, n_R, then n_R
+// blocks of nsta^2 lines "R1 R2 R3 m n x_re x_im y_re y_im z_re z_im" (n
+// outer, m inner; no degeneracy lines), each block's R looked up in hamR and
+// its values divided by the _hr.dat degeneracy of the same block number:
+// rmat[index(R)][d][m][n].  Parsed like the _hr.dat body: threads over line
+// ranges into a file-order buffer, then each block copied to its R's row.
+int parse_rdat(const std::string& file, const std::string& xyz_file, int n_threads, rtb_w90& m, Error& err) {
+  Mapped mf;
+  {
+    Error e;
+    if (map_file(file, mf, e) != RTB_OK) {
+      if (e.code == RTB_ERR_FILE_OPEN) {
+        err.code = RTB_ERR_FILE_OPEN;
+        err.msg = "Failed to create file '" + file + "': R::HAS_RMATRIX=true but _r.dat file not found";
+      } else {
+        err = e;
+      }
+      return err.code;
+    }
+  }
+  const char* buf = mf.data;
+  const size_t S = mf.size;
+  const size_t n = static_cast<size_t>(m.nsta), n2 = n * n;
+  m.rmat = static_cast<double*>(std::calloc(std::max<size_t>(static_cast<size_t>(m.nR) * 3 * n2 * 2, 2), sizeof(double)));
+  if (!m.rmat) return out_of_memory(err);
+  // the first three lines, then the body from line 3
+  size_t pos = 0, b0 = S;
+  const char* l2p = nullptr;
+  const char* l2e = nullptr;
+  for (int ln = 0; ln < 3 && pos <= S; ++ln) {
+    const char* nl = pos < S ? static_cast<const char*>(std::memchr(buf + pos, '\n', S - pos)) : nullptr;
+    const size_t end = nl ? static_cast<size_t>(nl - buf) : S;
+    if (ln == 2) {
+      l2p = buf + pos;
+      l2e = buf + end;
+    }
+    pos = nl ? end + 1 : S;
+    if (ln == 2) b0 = pos;
+    if (!nl && ln < 2) {
+      pos = S + 1;
+      break;
+    }
+  }
+  const LineRanges lr = split_lines(buf, S, std::min(b0, S), 3, n_threads);
+  if (lr.bad_utf8 || !valid_utf8(reinterpret_cast<const unsigned char*>(buf),
+                                 reinterpret_cast<const unsigned char*>(buf) + std::min(b0, S)))
+    return file_parse(err, xyz_file, "Failed to read line: stream did not contain valid UTF-8");
+  if (!l2p) return file_parse(err, file, "Truncated _r.dat: missing the R-vector count line");
+  uint64_t u;
+  int64_t sv;
+  const char* pe = parse_int(trim(l2p, l2e), false, sv, u);
+  if (*pe) return file_parse(err, file, std::string("Failed to parse n_R: ") + pe);
+  const uint64_t nRr = u;
+  if (n == 0) return RTB_OK;
+  if (nRr > (1ull << 40) / n2 || 3 + nRr * n2 > lr.total_lines)
+    return file_parse(err, file, "Truncated _r.dat: header declares " + std::to_string(nRr) + " R vectors of " +
+                                     std::to_string(n) + "^2 elements, but the file has " +
+                                     std::to_string(lr.total_lines) + " lines");
+  if (m.weights.size() < nRr)
+    return file_parse(err, file, "Truncated _r.dat: " + std::to_string(nRr) + " R vectors but " +
+                                     std::to_string(m.weights.size()) + " _hr.dat degeneracy weights");
+  std::unordered_map<RKey, size_t, RHash> index;
+  for (int r = 0; r < m.nR; ++r)
+    index.emplace(RKey{{m.hamR[r * 3], m.hamR[r * 3 + 1], m.hamR[r * 3 + 2]}}, static_cast<size_t>(r));  // find_R
+  double* tmp = static_cast<double*>(std::malloc(std::max<size_t>(nRr * 3 * n2 * 2, 2) * sizeof(double)));
+  if (!tmp) return out_of_memory(err);
+  std::vector<int64_t> slot(nRr, -1);
+  std::vector<RangeErr> rerr(lr.T);
+  const uint64_t stop = 3 + nRr * n2;
+  run_ranges(lr.T, [&](int t) {
+    const char* p = buf + lr.cut[t];
+    const char* e = buf + lr.cut[t + 1];
+    uint64_t L = lr.first_line[t];
+    RangeErr& re = rerr[t];
+    auto set = [&](uint64_t key, std::string msg) {
+      if (key < re.key) {
+        re.key = key;
+        re.msg = std::move(msg);
+      }
+    };
+    while (p < e && L < stop) {
+      const char* nl = static_cast<const char*>(std::memchr(p, '\n', static_cast<size_t>(e - p)));
+      const char* le = nl ? nl : e;
+      const uint64_t rel = L - 3, ir = rel / n2, w = rel % n2;
+      const uint64_t ii = w / n, jj = w % n;  // ind_i (outer), ind_j (inner)
+      const char* q = p;
+      Tok tk;
+      if (w == 0) {  // R, then its row in hamR (3 L + 0, 3 L + 1)
+        int64_t R[3];
+        bool ok = true;
+        for (int d = 0; d < 3 && ok; ++d) {
+          if (!next_token(q, le, tk)) {
+            set(3 * L, "Missing R vector component");
+            ok = false;
+          } else {
+            int64_t s2;
+            uint64_t u2;
+            const char* me = parse_int(tk, true, s2, u2);
+            if (*me) {
+              set(3 * L, std::string("Failed to parse R vector: ") + me);
+              ok = false;
+            }
+            R[d] = s2;
+          }
+        }
+        if (ok) {
+          auto it = index.find(RKey{{R[0], R[1], R[2]}});
+          if (it == index.end())
+            set(3 * L + 1, "R vector [" + std::to_string(R[0]) + ", " + std::to_string(R[1]) + ", " +
+                               std::to_string(R[2]) + "] not found in Hamiltonian");
+          else
+            slot[ir] = static_cast<int64_t>(it->second);
+        }
+        q = p;
+      }
+      int k = 0;
+      while (k < 5 && next_token(q, le, tk)) ++k;
+      const double wd = static_cast<double>(m.weights[ir]);
+      for (int d = 0; d < 3; ++d) {
+        double v[2] = {0.0, 0.0};
+        for (int c = 0; c < 2; ++c) {
+          const char* part = c == 0 ? "real" : "imaginary";
+          if (k < 5 || !next_token(q, le, tk)) {
+            set(3 * L + 2, std::string("Missing R matrix ") + part + " part");
+            k = 0;
+            break;
+          }
+          if (!parse_f64(tk, v[c])) {
+            set(3 * L + 2, std::string("Failed to parse R matrix ") + part + " part: invalid float literal");
+            break;
+          }
+        }
+        double* dst = tmp + (((ir * 3 + d) * n + jj) * n + ii) * 2;
+        dst[0] = v[0] / wd;
+        dst[1] = v[1] / wd;
+      }
+      p = nl ? nl + 1 : e;
+      ++L;
+    }
+  });
+  const RangeErr* first = nullptr;
+  for (const auto& r : rerr)
+    if (r.key != ~0ull && (!first || r.key < first->key)) first = &r;
+  if (first) {
+    std::free(tmp);
+    return file_parse(err, file, first->msg);
+  }
+  for (uint64_t ir = 0; ir < nRr; ++ir)  // later blocks of one R overwrite earlier ones
+    std::memcpy(m.rmat + static_cast<size_t>(slot[ir]) * 3 * n2 * 2, tmp + ir * 3 * n2 * 2, 3 * n2 * 2 * sizeof(double));
+  std::free(tmp);
+  return RTB_OK;
+}
+
+// The r-matrix made Hermitian across +-R (wannier90.rs:1134-1160): for rows
+// r = 0 .. nR - 2 in order (the reference's loop stops before the last row),
+// rmat[r][d][i][j] = (rmat[r][d][i][j] + conj(rmat[-R][d][j][i])) / 2 and then
+// rmat[-R][d][j][i] = conj(rmat[r][d][i][j]), element by element.
+int hermitize_rmatrix(rtb_w90& m, Error& err) {
+  const size_t n = static_cast<size_t>(m.nsta), n2 = n * n;
+  std::unordered_map<RKey, size_t, RHash> index;
+  for (int r = 0; r < m.nR; ++r) index.emplace(RKey{{m.hamR[r * 3], m.hamR[r * 3 + 1], m.hamR[r * 3 + 2]}}, r);
+  for (int r = 0; r + 1 < m.nR; ++r) {
+    auto it = index.find(RKey{{-m.hamR[r * 3], -m.hamR[r * 3 + 1], -m.hamR[r * 3 + 2]}});
+    if (it == index.end()) {
+      err.code = RTB_ERR_MISSING_CONJUGATE;
+      err.msg = "Missing Hermitian conjugate for R vector: [" + std::to_string(m.hamR[r * 3]) + ", " +
+                std::to_string(m.hamR[r * 3 + 1]) + ", " + std::to_string(m.hamR[r * 3 + 2]) + "]";
+      return err.code;
+    }
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j = 0; j < n; ++j)
+        for (int d = 0; d < 3; ++d) {
+          double* a = m.rmat + ((static_cast<size_t>(r) * 3 + d) * n2 + i * n + j) * 2;
+          double* b = m.rmat + ((it->second * 3 + d) * n2 + j * n + i) * 2;
+          a[0] = (a[0] + b[0]) / 2.0;
+          a[1] = (a[1] + -b[1]) / 2.0;
+          b[0] = a[0];
+          b[1] = -a[1];
+        }
+  }
   return RTB_OK;
 }
 
@@ -1104,6 +1307,8 @@ int apply_wsvec(const std::string& file, const std::string& xyz_file, rtb_w90& m
   }
   std::vector<int64_t> nR_list = {0, 0, 0};
   std::vector<double> nham(blk, 0.0);
+  const bool have_r = m.rmat != nullptr;  // wannier90.rs:929-1041: the r-matrix moves with the hopping
+  std::vector<double> nrmat(have_r ? 3 * blk : 0, 0.0);
   new_idx.emplace(RKey{{0, 0, 0}}, 0);
   if (lines.empty()) return file_parse(err, file, "Truncated _wsvec.dat: empty file");
   size_t i = 0;
@@ -1153,6 +1358,13 @@ int apply_wsvec(const std::string& file, const std::string& xyz_file, rtb_w90& m
     const double* h = m.ham + (it->second * n * n + oi[0] * n + oi[1]) * 2;
     const double wd = static_cast<double>(weight);
     const double hre = h[0] / wd, him = h[1] / wd;
+    double rre[3] = {0, 0, 0}, rim[3] = {0, 0, 0};
+    if (have_r)
+      for (int d = 0; d < 3; ++d) {
+        const double* x = m.rmat + ((it->second * 3 + d) * n * n + oi[0] * n + oi[1]) * 2;
+        rre[d] = x[0] / wd;
+        rim[d] = x[1] / wd;
+      }
     for (uint64_t k = 0; k < weight; ++k) {
       ++i;
       if (!line_at(i, l)) return err.code;
@@ -1171,17 +1383,24 @@ int apply_wsvec(const std::string& file, const std::string& xyz_file, rtb_w90& m
         new_idx.emplace(nk, slot);
         nR_list.insert(nR_list.end(), nk.r, nk.r + 3);
         nham.resize(nham.size() + blk, 0.0);
+        if (have_r) nrmat.resize(nrmat.size() + 3 * blk, 0.0);
       } else {
         slot = f->second;
       }
       double* d = nham.data() + (slot * n * n + oi[0] * n + oi[1]) * 2;
-      if (f == new_idx.end()) {  // a new block takes the value (keeps -0.0)
-        d[0] = hre;
+      if (f == new_idx.end() && !have_r) {  // a new block takes the value (keeps -0.0);
+        d[0] = hre;                           // with _r.dat the reference adds to zero
         d[1] = him;
       } else {
         d[0] += hre;
         d[1] += him;
       }
+      if (have_r)
+        for (int c = 0; c < 3; ++c) {
+          double* x = nrmat.data() + ((slot * 3 + c) * n * n + oi[0] * n + oi[1]) * 2;
+          x[0] += rre[c];  // the reference adds here for new and old blocks alike
+          x[1] += rim[c];
+        }
     }
   }
   double* fresh = static_cast<double*>(std::malloc(std::max<size_t>(nham.size() * sizeof(double), 16)));
@@ -1189,6 +1408,13 @@ int apply_wsvec(const std::string& file, const std::string& xyz_file, rtb_w90& m
   std::memcpy(fresh, nham.data(), nham.size() * sizeof(double));
   std::free(m.ham);
   m.ham = fresh;
+  if (have_r) {
+    double* fr = static_cast<double*>(std::malloc(std::max<size_t>(nrmat.size() * sizeof(double), 16)));
+    if (!fr) return out_of_memory(err);
+    std::memcpy(fr, nrmat.data(), nrmat.size() * sizeof(double));
+    std::free(m.rmat);
+    m.rmat = fr;
+  }
   m.hamR = std::move(nR_list);
   m.nR = static_cast<int>(m.hamR.size() / 3);
   return RTB_OK;
@@ -1245,8 +1471,8 @@ int validate(const rtb_w90& m, Error& err) {
 
 extern "C" {
 
-int rtb_w90_read(const char* path, const char* seedname, double zero_energy, int spin, int n_threads,
-                 rtb_w90** out) {
+static int w90_read(const char* path, const char* seedname, double zero_energy, int spin, int n_threads,
+                    bool with_r, rtb_w90** out) {
   if (!path || !seedname || !out) return rtb::fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
   *out = nullptr;
   if (spin < -1 || spin > 1) return rtb::fail(RTB_ERR_INVALID_ARGUMENT, "spin must be -1, 0 or 1");
@@ -1270,9 +1496,28 @@ int rtb_w90_read(const char* path, const char* seedname, double zero_energy, int
   m->spin = w.spin ? 1 : 0;
   std::copy(w.lat, w.lat + 9, m->lat);
   if (place_orbitals(prefix + "_centres.xyz", prefix + ".win", w, m->nsta, *m, err) != RTB_OK) return bail();
+  if (with_r && parse_rdat(prefix + "_r.dat", prefix + "_centres.xyz", n_threads, *m, err) != RTB_OK) return bail();
   if (apply_wsvec(prefix + "_wsvec.dat", prefix + "_centres.xyz", *m, err) != RTB_OK) return bail();
+  if (with_r && hermitize_rmatrix(*m, err) != RTB_OK) return bail();
   if (validate(*m, err) != RTB_OK) return bail();
   *out = m;
+  return RTB_OK;
+}
+
+int rtb_w90_read(const char* path, const char* seedname, double zero_energy, int spin, int n_threads,
+                 rtb_w90** out) {
+  return w90_read(path, seedname, zero_energy, spin, n_threads, false, out);
+}
+
+int rtb_w90_read_with_rmatrix(const char* path, const char* seedname, double zero_energy, int spin, int n_threads,
+                              rtb_w90** out) {
+  return w90_read(path, seedname, zero_energy, spin, n_threads, true, out);
+}
+
+int rtb_w90_export_rmatrix(const rtb_w90* m, double* rmatrix) {
+  if (!m || !rmatrix) return rtb::fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (!m->rmat) return rtb::fail(RTB_ERR_INVALID_ARGUMENT, "the model was read without _r.dat");
+  std::memcpy(rmatrix, m->rmat, static_cast<size_t>(m->nR) * 3 * m->nsta * m->nsta * 2 * sizeof(double));
   return RTB_OK;
 }
 
@@ -1328,7 +1573,16 @@ const char* rtb_orbital_projection_name(int code) {
 
 int rtb_w90_model(const rtb_w90* m, rtb_model** out) {
   if (!m || !out) return rtb::fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
-  return rtb_model_create(3, m->spin, m->norb, m->nR, m->hamR.data(), m->ham, m->orb.data(), m->lat, out);
+  RTB_TRY(rtb_model_create(3, m->spin, m->norb, m->nR, m->hamR.data(), m->ham, m->orb.data(), m->lat, out));
+  if (m->rmat) {
+    const int rc = rtb_model_set_rmatrix(*out, m->rmat);
+    if (rc != RTB_OK) {
+      rtb_model_destroy(*out);
+      *out = nullptr;
+      return rc;
+    }
+  }
+  return RTB_OK;
 }
 
 }  // extern "C"

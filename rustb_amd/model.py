@@ -143,6 +143,7 @@ class Model:
         self._b = None
         self._dev = None
         self._arrays = None
+        self.rmatrix = None  # [N_R, dim, n, n] position matrices (HasRMatrix)
         if _builder:
             h = ctypes.c_void_p()
             check(lib.rtb_builder_new(self.dim, int(self.spin), self.norb, ptr(self.lat), ptr(self.orb), ctypes.byref(h)))
@@ -155,18 +156,33 @@ class Model:
         return cls(lat, orb, spin)
 
     @classmethod
-    def from_arrays(cls, lat, orb, spin, hamR, ham) -> "Model":
-        """A model from a1 storage arrays (hamR [N_R, dim], ham [N_R, n, n])."""
+    def from_arrays(cls, lat, orb, spin, hamR, ham, rmatrix=None) -> "Model":
+        """A model from a1 storage arrays (hamR [N_R, dim], ham [N_R, n, n];
+        rmatrix [N_R, dim, n, n] for a model with position matrices)."""
         m = cls(lat, orb, spin, _builder=False)
         hamR = np.ascontiguousarray(hamR, dtype=np.int64)
         ham = np.ascontiguousarray(ham, dtype=np.complex128)
         if ham.shape != (hamR.shape[0], m.nsta, m.nsta):
             raise TbError(-11, "hamiltonian_shape")
         m._arrays = (hamR, ham)
+        if rmatrix is not None:
+            rmatrix = np.ascontiguousarray(rmatrix, dtype=np.complex128)
+            if rmatrix.shape != (hamR.shape[0], m.dim, m.nsta, m.nsta):
+                raise TbError(-11, "position_matrix_shape")
+            m.rmatrix = rmatrix
         return m
 
     @classmethod
-    def from_hr(cls, path: str, file_name: str, zero_energy: float, spin=None, n_threads: int = 0) -> "Model":
+    def from_hr_with_rmatrix(cls, path: str, file_name: str, zero_energy: float, spin=None,
+                             n_threads: int = 0) -> "Model":
+        """Model::from_hr_with_rmatrix (wannier90.rs:1208-1211): from_hr plus
+        {path}{file_name}_r.dat (required), kept as ``rmatrix``; every
+        velocity then carries the term i[H, r] (velocity.rs:377-402, 627-668)."""
+        return cls.from_hr(path, file_name, zero_energy, spin, n_threads, _rmatrix=True)
+
+    @classmethod
+    def from_hr(cls, path: str, file_name: str, zero_energy: float, spin=None, n_threads: int = 0,
+                _rmatrix: bool = False) -> "Model":
         """Wannier90::from_hr (wannier90.rs:49-1228), R = NoRMatrix: reads
         {path}{file_name}_hr.dat, .win and, when present, _centres.xyz and
         _wsvec.dat.  spin: the Model's SPIN (True/False; a different `spinors`
@@ -176,8 +192,8 @@ class Model:
         lib = load()
         h = ctypes.c_void_p()
         sp = -1 if spin is None else int(bool(spin))
-        check(lib.rtb_w90_read(path.encode(), file_name.encode(), float(zero_energy), sp, int(n_threads),
-                               ctypes.byref(h)))
+        reader = lib.rtb_w90_read_with_rmatrix if _rmatrix else lib.rtb_w90_read
+        check(reader(path.encode(), file_name.encode(), float(zero_energy), sp, int(n_threads), ctypes.byref(h)))
         try:
             v = [ctypes.c_int() for _ in range(5)]
             check(lib.rtb_w90_info(h, *[ctypes.byref(x) for x in v]))
@@ -188,6 +204,10 @@ class Model:
             lat = np.zeros((3, 3))
             proj = np.zeros(norb, np.int32)
             check(lib.rtb_w90_export(h, ptr(hamR), ptr(ham), ptr(orb), ptr(lat), ptr(proj)))
+            rmat = None
+            if _rmatrix:
+                rmat = np.empty((nR, 3, nsta, nsta), np.complex128)
+                check(lib.rtb_w90_export_rmatrix(h, ptr(rmat)))
             atoms = []
             for i in range(natom):
                 z, first, cnt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -199,6 +219,7 @@ class Model:
             lib.rtb_w90_destroy(h)
         m = cls(lat, orb, bool(file_spin), _builder=False)
         m._arrays = (hamR, ham)
+        m.rmatrix = rmat
         m.atoms = atoms
         m.orb_projection = [lib.rtb_orbital_projection_name(int(c)).decode() for c in proj]
         return m
@@ -285,6 +306,8 @@ class Model:
             check(load().rtb_model_create(self.dim, int(self.spin), self.norb, hamR.shape[0], ptr(hamR), ptr(ham),
                                           ptr(self.orb), ptr(self.lat), ctypes.byref(h)))
             self._dev = h
+            if self.rmatrix is not None:
+                check(load().rtb_model_set_rmatrix(h, ptr(self.rmatrix)))
         return self._dev
 
     def _dev_free(self):

@@ -104,3 +104,79 @@ def test_triangle_conventions_on_non_hermitian_storage(precision):
     ob, Ebr = O.berry_curvature_batch(o, ks, dirs, 1e-2)
     assert np.abs(Eb - Ebr).max() <= etol
     assert np.abs(om - ob).max() <= (1e-8 if precision == "fp64" else 1e-6) * max(1.0, np.abs(ob).max())
+
+
+def rmatrix_pair(seed=91, norb=4, spin=False):
+    """A model with position matrices, Hermitian across +-R, on both sides."""
+    m0 = ra.synth_model(2, 3, spin, norb, 1, seed, scale=1.0, decay=1.5)
+    rng = np.random.default_rng(seed)
+    n, nR = m0.nsta, m0.hamR.shape[0]
+    rm = 0.3 * (rng.standard_normal((nR, 3, n, n)) + 1j * rng.standard_normal((nR, 3, n, n)))
+    idx = {tuple(R): i for i, R in enumerate(m0.hamR)}
+    for i, R in enumerate(m0.hamR):
+        rm[idx[tuple(-R)]] = np.conj(np.swapaxes(rm[i], 1, 2))
+    z = idx[(0, 0, 0)]
+    rm[z] = 0.5 * (rm[z] + np.conj(np.swapaxes(rm[z], 1, 2)))
+    m = ra.Model.from_arrays(m0.lat, m0.orb, spin, m0.hamR, m0.ham, rm)
+    o = O.model_from_arrays(m0.lat, m0.orb, spin, m0.hamR, m0.ham, rm)
+    return m, o
+
+
+@pytest.mark.parametrize("gauge", ["atom", "lattice"])
+def test_rmatrix_velocity_vs_oracle(gauge):
+    """gen_v_projected with the HasRMatrix term i[H, r_p] (velocity.rs:377-402,
+    627-668) through the C ABI."""
+    m, o = rmatrix_pair()
+    print(f"[w90] r-matrix velocity {gauge}", flush=True)
+    ks = np.random.default_rng(12).uniform(-1, 1, (10, 3))
+    dirs = np.array([[1.0, 0.0, 0.0], [0.2, 1.0, -0.3], [0.0, 0.0, 1.0]])
+    V, H = m.gen_v_projected_all(ks, gauge, dirs)
+    scale = (np.abs(o.ham).sum(axis=0).max() + 1.0) * (1 + np.abs(o.rmatrix).sum(axis=0).max())
+    for i, k in enumerate(ks):
+        ov, oh = O.gen_v_projected(o, k, gauge, dirs)
+        assert np.abs(H[i] - oh).max() <= 1e-13 * scale
+        assert np.abs(V[i] - ov).max() <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_rmatrix_berry_dipole_geometry_vs_oracle(precision):
+    m, o = rmatrix_pair(seed=93, norb=3, spin=True)
+    print(f"[w90] r-matrix responses {precision}", flush=True)
+    ks = np.random.default_rng(13).random((8, 3))
+    dirs = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    tol = 1e-8 if precision == "fp64" else 1e-6
+    p = ra.Parameters(kmesh=(1, 1, 1), direction=dirs, mu=np.array([0.0]), eta=1e-2, precision=precision)
+    om, E = m.berry_curvature_on(ks, p)
+    ob, Eb = O.berry_curvature_batch(o, ks, dirs, 1e-2)
+    assert np.abs(om - ob).max() <= tol * max(1.0, np.abs(ob).max())
+    g = m.quantum_geometry_on(ks, p)
+    for i, k in enumerate(ks):
+        gm, gb, _ = O.quantum_geometry_at(o, k, dirs, 1e-2)
+        assert np.abs(g.metric[i] - gm).max() <= tol * max(1.0, np.abs(gm).max())
+    mu = np.array([-0.4, 0.3])
+    s = m.hall_conductivity(ra.Parameters.rank2([3, 3, 2], [1, 0, 0], [0, 1, 0], mu, eta=0.02,
+                                                precision=precision)).conductivity
+    r = O.hall_conductivity_direct(o, [3, 3, 2], dirs, mu, 0.0, 0.02)
+    assert np.abs(s - r).max() <= (1e-9 if precision == "fp64" else 1e-6) * max(1.0, np.abs(r).max())
+    d3 = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.4, 1.0]])
+    pn = ra.Parameters(kmesh=(3, 3, 2), direction=d3, mu=mu, T=np.array([900.0]), eta=0.02, precision=precision)
+    got = m.extrinsic_nonlinear_hall(pn).conductivity
+    ref = O.extrinsic_nonlinear_hall_direct(o, [3, 3, 2], d3, mu, 900.0, 0.02)
+    assert np.abs(got - ref).max() <= (1e-9 if precision == "fp64" else 1e-6) * max(1e-3, np.abs(ref).max())
+
+
+def test_rmatrix_from_files_through_device(tmp_path):
+    """from_hr_with_rmatrix: files -> loader -> device velocities, against the
+    oracle's reading of the same files."""
+    from tests.test_wannier90 import rmatrix_dataset
+    p, *_ = rmatrix_dataset(tmp_path, seed=5, n=4, wsvec=True)
+    m = ra.Model.from_hr_with_rmatrix(p, "seedname", 0.0)
+    w = OW.from_hr(p, "seedname", 0.0, rmatrix=True)
+    o = O.model_from_arrays(w.lat, w.orb, w.spin, w.hamR, w.ham, w.rmatrix)
+    ks = np.random.default_rng(14).random((6, 3))
+    dirs = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    V, _ = m.gen_v_projected_all(ks, "atom", dirs)
+    scale = (np.abs(w.ham).sum(axis=0).max() + 1.0) * (1 + np.abs(w.rmatrix).sum(axis=0).max())
+    for i, k in enumerate(ks):
+        ov, _ = O.gen_v_projected(o, k, "atom", dirs)
+        assert np.abs(V[i] - ov).max() <= 1e-12 * scale

@@ -371,3 +371,46 @@ def test_dipole_time_reversal_even_and_symmetrization():
     s1 = O.extrinsic_nonlinear_hall_direct(_haldane_c3_broken(-0.3), [9, 9], same, mu, 600.0, 0.05, symmetrized=True)
     s2 = O.extrinsic_nonlinear_hall_direct(_haldane_c3_broken(-0.3), [9, 9], same, mu, 600.0, 0.05, symmetrized=False)
     assert np.array_equal(s1, s2)
+
+
+def test_rmatrix_velocity_term():
+    """The HasRMatrix term i[H, r_p] (velocity.rs:645-668): with position
+    matrices diag(tau_cart) at R = 0 only, the Lattice-gauge velocity gains
+    exactly i(tau_n - tau_m) H_mn, i.e. it becomes the Atom-gauge velocity of
+    the plain model with the gauge factor removed (the Atom gauge zeroes the
+    diagonal of r_p, so there the term vanishes); and the term keeps v
+    Hermitian for Hermitian position matrices."""
+    rng = np.random.default_rng(4)
+    lat = np.array([[1.0, 0.2], [-0.1, 1.3]])
+    orb = rng.random((3, 2))
+    o = O.Model(lat, orb, False)
+    z = np.zeros(2, int)
+    for i in range(3):
+        o.add_element(rng.uniform(-1, 1), i, i, z)
+        for j in range(i + 1, 3):
+            o.add_element(complex(*rng.standard_normal(2)), i, j, z)
+    for R in ([1, 0], [0, 1], [1, -1]):
+        for i in range(3):
+            for j in range(3):
+                o.add_element(0.3 * complex(*rng.standard_normal(2)), i, j, np.array(R))
+    tau = orb @ lat
+    rm = np.zeros((o.ham.shape[0], 2, 3, 3), complex)
+    for d in range(2):
+        rm[0, d] = np.diag(tau[:, d])
+    orr = O.model_from_arrays(lat, orb, False, o.hamR, o.ham, rm)
+    k = np.array([0.21, 0.37])
+    dirs = np.array([[1.0, 0.0], [0.3, 0.7]])
+    v_lat_r, _ = O.gen_v_projected(orr, k, "lattice", dirs)
+    v_atom, _ = O.gen_v_projected(o, k, "atom", dirs)
+    phi = np.exp(2j * np.pi * (orb @ k))
+    gm = np.conj(phi)[:, None] * phi[None, :]
+    assert np.abs(v_lat_r - v_atom / gm).max() <= 1e-13
+    v_atom_r, _ = O.gen_v_projected(orr, k, "atom", dirs)
+    assert np.abs(v_atom_r - v_atom).max() <= 1e-14
+    rm2 = rng.standard_normal(rm.shape) + 1j * rng.standard_normal(rm.shape)
+    idx = {tuple(R): i for i, R in enumerate(o.hamR)}
+    for i, R in enumerate(o.hamR):
+        rm2[idx[tuple(-R)]] = np.conj(np.swapaxes(rm2[i], 1, 2))
+    rm2[0] = 0.5 * (rm2[0] + np.conj(np.swapaxes(rm2[0], 1, 2)))  # R = 0 is its own partner
+    v2, _ = O.gen_v_projected(O.model_from_arrays(lat, orb, False, o.hamR, o.ham, rm2), k, "atom", dirs)
+    assert np.abs(v2 - np.conj(np.swapaxes(v2, 1, 2))).max() <= 1e-12

@@ -511,3 +511,103 @@ def test_spinful_model_with_centres_round_trip(tmp_path):
     index = {tuple(R): i for i, R in enumerate(m0.hamR)}
     err = max(np.abs(lo.ham[i] - m0.ham[index[tuple(R)]]).max() for i, R in enumerate(lo.hamR))
     assert err <= 5e-7 * math.sqrt(2) + 1e-12
+
+
+# ------------------------------------------------------------ _r.dat
+def rdat_text(nsta, Rs, rblocks, fmt=repr):
+    """_r.dat of rblocks[r][d][m][n] (as written), lines n outer, m inner."""
+    out = ["r-matrix", str(nsta), str(len(Rs))]
+    for R, b in zip(Rs, rblocks):
+        for n in range(nsta):
+            for m in range(nsta):
+                vals = " ".join(f"{fmt(float(b[d, m, n].real))} {fmt(float(b[d, m, n].imag))}" for d in range(3))
+                out.append(f"{R[0]} {R[1]} {R[2]} {m + 1} {n + 1} {vals}")
+    return "\n".join(out) + "\n"
+
+
+def rmatrix_dataset(tmp_path, seed=3, n=4, wsvec=False, drop_partner=False):
+    rng = np.random.default_rng(seed)
+    Rs = [(a, b, 0) for a in (-1, 0, 1) for b in (-1, 0, 1)]
+    if drop_partner:  # (-1, -1) then lacks its partner; the last stored row would not be checked
+        Rs.remove((1, 1, 0))
+    blocks = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in Rs]
+    rblocks = [rng.standard_normal((3, n, n)) + 1j * rng.standard_normal((3, n, n)) for _ in Rs]
+    weights = [int(w) for w in rng.integers(1, 4, len(Rs))]
+    files = {"_hr.dat": hr_text(n, Rs, blocks, weights), ".win": one_atom_win(n),
+             "_r.dat": rdat_text(n, Rs, rblocks)}
+    if wsvec:
+        ws = ["## header"]
+        for R in Rs:
+            for i in range(1, n + 1):
+                for j in range(1, n + 1):
+                    k = int(rng.integers(1, 3))
+                    ws += [f"{R[0]} {R[1]} {R[2]} {i} {j}", str(k)]
+                    ws += ["0 0 0"] + [f"{int(rng.integers(-1, 2)) * 3} 0 0" for _ in range(k - 1)]
+        files["_wsvec.dat"] = "\n".join(ws) + "\n"
+    return write(str(tmp_path), files), Rs, rblocks, weights
+
+
+@pytest.mark.parametrize("wsvec", [False, True])
+def test_rmatrix_matches_oracle_bitwise(wsvec, tmp_path):
+    """from_hr_with_rmatrix: the loader's position matrices (after the
+    Wigner-Seitz shifts and the +-R hermitization) equal the oracle's bit for
+    bit, at every thread count."""
+    p, *_ = rmatrix_dataset(tmp_path, wsvec=wsvec)
+    ref = OW.from_hr(p, "seedname", 0.0, rmatrix=True)
+    for T in (1, 3, 16):
+        m = ra.Model.from_hr_with_rmatrix(p, "seedname", 0.0, n_threads=T)
+        assert np.array_equal(m.hamR, ref.hamR)
+        assert np.array_equal(bits(m.ham), bits(ref.ham))
+        assert np.array_equal(bits(m.rmatrix), bits(ref.rmatrix))
+
+
+def test_rmatrix_closed_form(tmp_path):
+    """Without shifts: rmatrix[R][d][m][n] = value / ndegen of R's own block,
+    then for rows 0 .. N_R - 2 in order r = (r + r_{-R}^H) / 2 and r_{-R} =
+    r^H (wannier90.rs:1134-1160): every +-R pair ends Hermitian."""
+    p, Rs, rblocks, weights = rmatrix_dataset(tmp_path, seed=8, n=3)
+    m = ra.Model.from_hr_with_rmatrix(p, "seedname", 0.0)
+    raw = {R: np.array([cdiv(rblocks[i][d], weights[i]) for d in range(3)]) for i, R in enumerate(Rs)}
+    rows = [tuple(R) for R in m.hamR]
+    exp = np.array([raw[R] for R in rows])
+    pos = {R: i for i, R in enumerate(rows)}
+    for r in range(len(rows) - 1):
+        q = pos[tuple(-x for x in rows[r])]
+        for i in range(3):
+            for j in range(3):
+                for d in range(3):
+                    a = (exp[r, d, i, j] + np.conj(exp[q, d, j, i])) / 2.0
+                    exp[r, d, i, j] = a
+                    exp[q, d, j, i] = np.conj(a)
+    assert np.abs(m.rmatrix - exp).max() <= 1e-15 * np.abs(exp).max()
+    for r, R in enumerate(rows):
+        q = pos[tuple(-x for x in R)]
+        assert np.array_equal(m.rmatrix[q], np.conj(np.swapaxes(m.rmatrix[r], 1, 2)))
+
+
+@pytest.mark.parametrize("impl", IMPLS)
+@pytest.mark.parametrize("case", ["missing_r", "unknown_R", "missing_partner", "bad_value"])
+def test_rmatrix_errors(impl, case, tmp_path):
+    p, Rs, rblocks, weights = rmatrix_dataset(tmp_path, seed=4, n=2, drop_partner=case == "missing_partner")
+    expect = "FileParse"
+    if case == "missing_r":
+        os.remove(os.path.join(str(tmp_path), "seedname_r.dat"))
+        expect = "FileCreation"
+    elif case == "unknown_R":
+        txt = rdat_text(2, [(5, 5, 0)] + Rs[1:], rblocks)
+        write(str(tmp_path), {"_r.dat": txt})
+    elif case == "missing_partner":
+        expect = "MissingHermitianConjugate"
+    else:
+        txt = rdat_text(2, Rs, rblocks).split("\n")
+        txt[10] = txt[10].rsplit(" ", 1)[0] + " 1.0D-3"
+        write(str(tmp_path), {"_r.dat": "\n".join(txt)})
+    try:
+        if impl == "loader":
+            ra.Model.from_hr_with_rmatrix(p, "seedname", 0.0)
+        else:
+            OW.from_hr(p, "seedname", 0.0, rmatrix=True)
+    except (ra.TbError, OW.W90Error) as e:
+        assert e.kind == expect, str(e)
+    else:
+        raise AssertionError("expected an error")
