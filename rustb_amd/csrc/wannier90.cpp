@@ -609,12 +609,9 @@ int parse_hr(const std::string& file, int n_threads, double zero_energy, rtb_w90
   Line l0, l1, l2;
   if (!next_line(l0) || !next_line(l1) || !next_line(l2))
     return file_parse(err, file, "Truncated _hr.dat: missing the orbital or R-vector count line");
-  const size_t header_end = pos;
   // every line must be UTF-8 before anything is parsed (the reference reads
-  // the whole file through lines() first); checked per range below, the
-  // header range here
-  // header errors are reported after the UTF-8 check, in the reference's
-  // order: nsta, n_R, degeneracies
+  // the whole file through lines() first), so header errors are reported
+  // after the UTF-8 checks, in the reference's order: nsta, n_R, degeneracies
   uint64_t u = 0;
   int64_t s;
   std::string header_err;
@@ -661,7 +658,6 @@ int parse_hr(const std::string& file, int n_threads, double zero_energy, rtb_w90
     err.msg = "I/O error: stream did not contain valid UTF-8";
     return err.code;
   }
-  (void)header_end;
 
   const size_t b0 = n_line ? body_off : 0;
   const LineRanges lr = split_lines(buf, S, b0, n_line, n_threads);
