@@ -1,11 +1,13 @@
 // C ABI entry points: validation (response/config.rs, model_physics.rs),
 // k-chunking, device work buffers and the kernel pipeline of each call.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 
 #include "eigh.hpp"
 
@@ -369,6 +371,55 @@ int dipole_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const
 
 void publish_stats(const rtb_stats& s) { g_stats = s; }
 
+// Host-to-device copy of a large pageable buffer (a model's hoppings): HIP
+// moves pageable memory through its own small staging at a few GB/s, so the
+// copy goes through two pinned 32 MB buffers instead, host threads filling
+// one while the DMA engine drains the other.  Small copies go direct.
+int upload_pageable(void* dst, const void* src, size_t bytes) {
+  constexpr size_t kStage = size_t(32) << 20;
+  if (bytes < 4 * kStage) {
+    RTB_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return RTB_OK;
+  }
+  void* stage[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipStream_t st = nullptr;
+  int rc = RTB_OK;
+  auto check = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == RTB_OK) rc = fail(RTB_ERR_DEVICE, what);
+    return e == hipSuccess;
+  };
+  if (check(hipStreamCreate(&st), "upload stream") && check(hipHostMalloc(&stage[0], kStage), "pinned staging") &&
+      check(hipHostMalloc(&stage[1], kStage), "pinned staging") && check(hipEventCreate(&ev[0]), "event") &&
+      check(hipEventCreate(&ev[1]), "event")) {
+    const char* s8 = static_cast<const char*>(src);
+    char* d8 = static_cast<char*>(dst);
+    size_t i = 0;
+    for (size_t off = 0; off < bytes && rc == RTB_OK; off += kStage, ++i) {
+      const int b = static_cast<int>(i & 1);
+      if (i >= 2 && !check(hipEventSynchronize(ev[b]), "staging wait")) break;
+      const size_t len = std::min(kStage, bytes - off);
+      constexpr int kT = 4;
+      std::vector<std::thread> th;
+      for (int t = 0; t < kT; ++t)
+        th.emplace_back([&, t] {
+          const size_t a = len / kT * t, e = t == kT - 1 ? len : len / kT * (t + 1);
+          std::memcpy(static_cast<char*>(stage[b]) + a, s8 + off + a, e - a);
+        });
+      for (auto& x : th) x.join();
+      if (!check(hipMemcpyAsync(d8 + off, stage[b], len, hipMemcpyHostToDevice, st), "staged copy")) break;
+      check(hipEventRecord(ev[b], st), "event record");
+    }
+    check(hipStreamSynchronize(st), "upload sync");
+  }
+  for (int b = 0; b < 2; ++b) {
+    if (ev[b]) (void)hipEventDestroy(ev[b]);
+    if (stage[b]) (void)hipHostFree(stage[b]);
+  }
+  if (st) (void)hipStreamDestroy(st);
+  return rc;
+}
+
 // ham[-R] == ham[R]^H exactly for every stored R (the builder's storage, by
 // construction); models read from files are Hermitian only to their printed
 // precision.  Decides whether the velocity operators may be combined into
@@ -376,22 +427,38 @@ void publish_stats(const rtb_stats& s) { g_stats = s; }
 bool storage_is_hermitian(int dim, int n, int n_R, const int64_t* hamR, const double* ham) {
   std::map<std::vector<int64_t>, int> index;
   for (int r = 0; r < n_R; ++r) index.emplace(std::vector<int64_t>(hamR + r * dim, hamR + (r + 1) * dim), r);
-  const size_t n2 = static_cast<size_t>(n) * n;
+  std::vector<int> partner(n_R);
   for (int r = 0; r < n_R; ++r) {
     std::vector<int64_t> neg(dim);
     for (int d = 0; d < dim; ++d) neg[d] = -hamR[r * dim + d];
     auto it = index.find(neg);
     if (it == index.end()) return false;
-    const double* a = ham + r * n2 * 2;
-    const double* b = ham + it->second * n2 * 2;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        const double* x = a + (static_cast<size_t>(i) * n + j) * 2;
-        const double* y = b + (static_cast<size_t>(j) * n + i) * 2;
-        if (x[0] != y[0] || x[1] != -y[1]) return false;
-      }
+    partner[r] = it->second;
   }
-  return true;
+  // each pair once (r <= partner), blocks spread over host threads: a full
+  // pass over the storage (349 MB for 1331 R at n = 128)
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const int T = std::max(1, std::min(16, std::min(n_R, static_cast<int>(std::thread::hardware_concurrency()))));
+  std::atomic<bool> ok{true};
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&, t] {
+      for (int r = t; r < n_R && ok.load(std::memory_order_relaxed); r += T) {
+        if (partner[r] < r) continue;
+        const double* a = ham + r * n2 * 2;
+        const double* b = ham + partner[r] * n2 * 2;
+        bool good = true;
+        for (int i = 0; i < n && good; ++i)
+          for (int j = 0; j < n; ++j) {
+            const double* x = a + (static_cast<size_t>(i) * n + j) * 2;
+            const double* y = b + (static_cast<size_t>(j) * n + i) * 2;
+            good = good && x[0] == y[0] && x[1] == -y[1];
+          }
+        if (!good) ok.store(false, std::memory_order_relaxed);
+      }
+    });
+  for (auto& x : th) x.join();
+  return ok.load();
 }
 
 void set_error(const char* fmt, ...) {
@@ -479,8 +546,8 @@ int rtb_model_create(int dim, int spin, int norb, int n_R, const int64_t* hamR, 
       hipMalloc(&m->d_tau, sizeof(double) * tau.size()) != hipSuccess ||
       hipMalloc(&m->d_taucart, sizeof(double) * tauc.size()) != hipSuccess)
     return cleanup(fail(RTB_ERR_DEVICE, "device allocation for the model failed"));
-  if (hipMemcpy(m->d_ham, ham, sizeof(double) * 2 * n2 * n_R, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(m->d_R, R.data(), sizeof(double) * R.size(), hipMemcpyHostToDevice) != hipSuccess ||
+  if (upload_pageable(m->d_ham, ham, sizeof(double) * 2 * n2 * n_R) != RTB_OK) return cleanup(RTB_ERR_DEVICE);
+  if (hipMemcpy(m->d_R, R.data(), sizeof(double) * R.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->d_Rcart, Rc.data(), sizeof(double) * Rc.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->d_tau, tau.data(), sizeof(double) * tau.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(m->d_taucart, tauc.data(), sizeof(double) * tauc.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -509,8 +576,7 @@ int rtb_model_set_rmatrix(rtb_model* m, const double* rmatrix) {
     m->d_rmat = nullptr;
     return fail(RTB_ERR_DEVICE, "device allocation for the position matrices failed");
   }
-  RTB_HIP(hipMemcpy(m->d_rmat, rmatrix, count * sizeof(double), hipMemcpyHostToDevice));
-  return RTB_OK;
+  return upload_pageable(m->d_rmat, rmatrix, count * sizeof(double));
 }
 
 int rtb_model_nsta(const rtb_model* m) { return m ? m->nsta : RTB_ERR_INVALID_ARGUMENT; }
