@@ -181,6 +181,19 @@ int rtb_solve_band_all(const rtb_model* m, const double* k, size_t nk, int preci
 int rtb_solve_all(const rtb_model* m, const double* k, size_t nk, int precision, double* E,
                   double* evec);
 
+/* Solve::solve_band_range_onek (solve_ham.rs:64-80) and solve_range_onek
+ * (solve_ham.rs:130-147), LAPACK zheevr RANGE = 'V' (ndarray_lapack.rs:217-395):
+ * per k the eigenvalues in (vl, vu], ascending, packed to the front of its
+ * row of E (nk*n), their number in count[k]; solve_range also the matching
+ * eigenvectors as the first count[k] rows of evec (nk*n*n complex).  These
+ * read H's lower triangle, as the reference's raw-buffer zheevr call does.
+ * epsilon (zheevr's ABSTOL) only bounds the reference's accuracy; the
+ * eigenvalues here are FP64-accurate whatever it is. */
+int rtb_solve_band_range(const rtb_model* m, const double* k, size_t nk, double vl, double vu, double epsilon,
+                         int precision, double* E, int* count);
+int rtb_solve_range(const rtb_model* m, const double* k, size_t nk, double vl, double vu, double epsilon,
+                    int precision, double* E, double* evec, int* count);
+
 /* BerryCurvature::berry_curvature_at (response/traits.rs:60-78,160-200) for
  * nk k-points: omega, E nk*n.  dir is 2*dim (rows a, b). */
 int rtb_berry_curvature(const rtb_model* m, const double* k, size_t nk, const double* dir,

@@ -391,6 +391,26 @@ def eigh(H):
     return w, v
 
 
+def solve_band_range_onek(model: Model, k, range_, epsilon):
+    """solve_band_range_onek (solve_ham.rs:64-80) -> eigvalsh_r, zheevr with
+    RANGE = 'V' (ndarray_lapack.rs:301-395): the eigenvalues in the half-open
+    interval (vl, vu] of the matrix eigvalsh reads (H's lower triangle).
+    epsilon is zheevr's ABSTOL (accuracy only; not used here)."""
+    e = eigvalsh(gen_ham(model, k, "atom"))
+    return e[(e > range_[0]) & (e <= range_[1])]
+
+
+def solve_range_onek(model: Model, k, range_, epsilon):
+    """solve_range_onek (solve_ham.rs:130-147) -> eigh_r (ndarray_lapack.rs:
+    217-286) on the raw buffer ('U' of H^T, i.e. H's lower triangle), then
+    conjugated: the eigenpairs in (vl, vu] with the eigenvectors as rows."""
+    w, v, info = _lapack.zheev(np.asarray(gen_ham(model, k, "atom"), np.complex128), compute_v=1, lower=1)
+    if info != 0:
+        raise OracleError("EigenvalueComputationFailed", f"info={info}")
+    keep = (w > range_[0]) & (w <= range_[1])
+    return w[keep], v[:, keep].T.copy()
+
+
 def solve_band_onek(model: Model, k):
     """solve_band_onek (solve_ham.rs:52-63): eigenvalues of the Atom-gauge H."""
     return eigvalsh(gen_ham(model, k, "atom"))

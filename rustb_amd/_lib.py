@@ -88,6 +88,8 @@ SIGNATURES = {
     "rtb_gen_ham_batch": (c_int, [c_void_p, _P, c_size_t, c_int, _P, c_int, _P, _P]),
     "rtb_solve_band_all": (c_int, [c_void_p, _P, c_size_t, c_int, _P]),
     "rtb_solve_all": (c_int, [c_void_p, _P, c_size_t, c_int, _P, _P]),
+    "rtb_solve_band_range": (c_int, [c_void_p, _P, c_size_t, c_double, c_double, c_double, c_int, _P, _P]),
+    "rtb_solve_range": (c_int, [c_void_p, _P, c_size_t, c_double, c_double, c_double, c_int, _P, _P, _P]),
     "rtb_berry_curvature": (c_int, [c_void_p, _P, c_size_t, _P, c_int, c_double, c_int, _P, _P]),
     "rtb_occupied_berry_curvature": (c_int, [c_void_p, _P, c_size_t, _P, c_int, c_double, c_double,
                                              c_double, c_int, _P]),

@@ -636,7 +636,10 @@ int rtb_gen_ham_batch(const rtb_model* m, const double* k, size_t nk, int gauge,
   return RTB_OK;
 }
 
-static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precision, double* E, double* evec) {
+// lower_vectors: eigenvectors of the Hermitian matrix of H's lower triangle
+// (the reference's zheevr path) instead of its upper one (its eigh)
+static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precision, double* E, double* evec,
+                      bool lower_vectors = false) {
   RTB_TRY(prepare(m));
   RTB_TRY(check_precision(precision));
   if (nk > 0 && (!k || !E)) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
@@ -682,12 +685,13 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     RTB_TRY(tmr.end(1));
     if (f32) {
       // W2, W3: the reflector and inverse-iteration scratch, dead by then
-      RTB_TRY(eig_mixed_chunk(st, tmr, !m->herm, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S, dsc));
+      RTB_TRY(eig_mixed_chunk(st, tmr, !m->herm && !lower_vectors, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S, dsc));
       RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
       RTB_TRY(transpose_batched(st, n, c, eb.Vg, dsc, dZ));
       RTB_TRY(tmr.end(1));
     } else {
-      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, vec && !m->herm));
+      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE,
+                               vec && !m->herm && !lower_vectors));
       if (vec) {
         RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
         RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, true));
@@ -710,6 +714,47 @@ int rtb_solve_band_all(const rtb_model* m, const double* k, size_t nk, int preci
 int rtb_solve_all(const rtb_model* m, const double* k, size_t nk, int precision, double* E, double* evec) {
   if (nk > 0 && !evec) return fail(RTB_ERR_INVALID_ARGUMENT, "null evec");
   return solve_impl(m, k, nk, precision, E, evec);
+}
+
+// The eigenpairs with E in (vl, vu] (zheevr RANGE = 'V'), packed to the
+// front of each k's row, ascending; count[k] of them.  The whole spectrum
+// comes from the device solve, the selection is a host pass over its output.
+static int solve_range_impl(const rtb_model* m, const double* k, size_t nk, double vl, double vu, double epsilon,
+                            int precision, double* E, double* evec, int* count) {
+  RTB_TRY(prepare(m));
+  if (!(vl < vu) || !std::isfinite(vl) || !std::isfinite(vu))
+    return fail(RTB_ERR_INVALID_ARGUMENT, "range: need finite vl < vu (zheevr RANGE = 'V')");
+  if (!std::isfinite(epsilon)) return fail(RTB_ERR_INVALID_ARGUMENT, "epsilon must be finite");
+  if (nk > 0 && !count) return fail(RTB_ERR_INVALID_ARGUMENT, "null count");
+  RTB_TRY(solve_impl(m, k, nk, precision, E, evec, true));
+  const int n = m->nsta;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  for (size_t i = 0; i < nk; ++i) {
+    double* e = E + i * n;
+    int c = 0;
+    for (int j = 0; j < n; ++j) {
+      if (!(e[j] > vl && e[j] <= vu)) continue;
+      if (c != j) {
+        e[c] = e[j];
+        if (evec) std::memmove(evec + (i * n2 + static_cast<size_t>(c) * n) * 2,
+                               evec + (i * n2 + static_cast<size_t>(j) * n) * 2, n * 2 * sizeof(double));
+      }
+      ++c;
+    }
+    count[i] = c;
+  }
+  return RTB_OK;
+}
+
+int rtb_solve_band_range(const rtb_model* m, const double* k, size_t nk, double vl, double vu, double epsilon,
+                         int precision, double* E, int* count) {
+  return solve_range_impl(m, k, nk, vl, vu, epsilon, precision, E, nullptr, count);
+}
+
+int rtb_solve_range(const rtb_model* m, const double* k, size_t nk, double vl, double vu, double epsilon,
+                    int precision, double* E, double* evec, int* count) {
+  if (nk > 0 && !evec) return fail(RTB_ERR_INVALID_ARGUMENT, "null evec");
+  return solve_range_impl(m, k, nk, vl, vu, epsilon, precision, E, evec, count);
 }
 
 // Shared Berry-curvature pipeline over either a k list (k != null) or a mesh.

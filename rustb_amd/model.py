@@ -384,6 +384,27 @@ class Model:
 
     solve_all_parallel = solve_all
 
+    def solve_band_range_onek(self, k, range_, epsilon, precision="fp64"):
+        """solve_band_range_onek (solve_ham.rs:64-80): eigenvalues in
+        (range_[0], range_[1]] (zheevr RANGE = 'V')."""
+        kk = self._kvec(np.asarray(k, float)[None, :])
+        E = np.zeros((1, self.nsta))
+        cnt = np.zeros(1, np.int32)
+        check(load().rtb_solve_band_range(self._device(), ptr(kk), 1, float(range_[0]), float(range_[1]),
+                                          float(epsilon), PRECISION[precision], ptr(E), ptr(cnt)))
+        return E[0, :cnt[0]].copy()
+
+    def solve_range_onek(self, k, range_, epsilon, precision="fp64"):
+        """solve_range_onek (solve_ham.rs:130-147): eigenvalues in (range_[0],
+        range_[1]] and their eigenvectors as rows."""
+        kk = self._kvec(np.asarray(k, float)[None, :])
+        E = np.zeros((1, self.nsta))
+        V = np.zeros((1, self.nsta, self.nsta), np.complex128)
+        cnt = np.zeros(1, np.int32)
+        check(load().rtb_solve_range(self._device(), ptr(kk), 1, float(range_[0]), float(range_[1]), float(epsilon),
+                                     PRECISION[precision], ptr(E), ptr(V), ptr(cnt)))
+        return E[0, :cnt[0]].copy(), V[0, :cnt[0]].copy()
+
     def solve_onek(self, k, precision="fp64"):
         E, V = self.solve_all(np.asarray(k, float)[None, :], precision)
         return E[0], V[0]

@@ -310,3 +310,30 @@ def test_nonlinear_hall_dipole_vs_oracle(precision):
     with pytest.raises(ra.TbError) as e:
         m.extrinsic_nonlinear_hall(ra.Parameters.rank3([3, 3], [1, 0], [0, 1], [1, 0], [0.0], precision=precision))
     assert e.value.kind == "InvalidThermodynamicParameter"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_range_solves_vs_oracle(precision):
+    """solve_band_range_onek / solve_range_onek through the C ABI: the same
+    eigenvalues in (vl, vu] as the oracle, and eigenvectors (rows) of the
+    lower-triangle matrix; the half-open edges on exact eigenvalues."""
+    m, o = random_pair(5, True, 3, 3, seed=81)
+    tol = 1e-12 if precision == "fp64" else 1e-9
+    for k in np.random.default_rng(9).random((5, 3)):
+        full = O.solve_band_onek(o, k)
+        vl, vu = 0.5 * (full[2] + full[3]), 0.5 * (full[7] + full[8])
+        e = m.solve_band_range_onek(k, (vl, vu), 1e-10, precision=precision)
+        er = O.solve_band_range_onek(o, k, (vl, vu), 1e-10)
+        assert e.shape == er.shape and np.abs(e - er).max() <= tol * max(1.0, np.abs(full).max())
+        w, V = m.solve_range_onek(k, (vl, vu), 1e-10, precision=precision)
+        H = O.gen_ham(o, k, "atom")
+        low = np.tril(H, -1) + np.tril(H, -1).conj().T + np.diag(H.diagonal().real)
+        assert w.shape == er.shape and np.abs(low @ V.T - V.T * w).max() <= tol * max(1.0, np.abs(full).max())
+    # edges between eigenvalues (one exactly at an edge is decided by rounding
+    # in any FP64 solver; the half-open rule itself is pinned on the oracle)
+    d = ra.Model.tb_model(np.eye(2), np.zeros((4, 2)), False)
+    d.set_onsite([-1.0, 0.0, 1.0, 2.0])
+    e = d.solve_band_range_onek([0.1, 0.2], (-0.5, 1.5), 1e-10, precision=precision)
+    assert e.shape == (2,) and np.abs(e - [0.0, 1.0]).max() <= 1e-14
+    with pytest.raises(ra.TbError):
+        d.solve_band_range_onek([0.1, 0.2], (1.0, 0.0), 1e-10)

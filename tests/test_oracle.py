@@ -414,3 +414,17 @@ def test_rmatrix_velocity_term():
     rm2[0] = 0.5 * (rm2[0] + np.conj(np.swapaxes(rm2[0], 1, 2)))  # R = 0 is its own partner
     v2, _ = O.gen_v_projected(O.model_from_arrays(lat, orb, False, o.hamR, o.ham, rm2), k, "atom", dirs)
     assert np.abs(v2 - np.conj(np.swapaxes(v2, 1, 2))).max() <= 1e-12
+
+
+def test_range_solves_half_open_interval():
+    """solve_band_range_onek / solve_range_onek (zheevr RANGE = 'V'): the
+    eigenvalues in (vl, vu] -- an eigenvalue at vl is excluded, one at vu
+    included -- with the eigenvectors of the lower-triangle matrix as rows."""
+    o = O.Model(np.eye(2), np.zeros((4, 2)), False)
+    o.set_onsite([-1.0, 0.0, 1.0, 2.0])
+    e = O.solve_band_range_onek(o, np.array([0.1, 0.2]), (0.0, 1.0), 1e-10)
+    assert np.array_equal(e, [1.0])
+    w, V = O.solve_range_onek(o, np.array([0.1, 0.2]), (-1.0, 2.0), 1e-10)
+    assert np.array_equal(w, [0.0, 1.0, 2.0]) and V.shape == (3, 4)
+    H = O.gen_ham(o, np.array([0.1, 0.2]), "atom")
+    assert np.abs(H @ V.T - V.T * w).max() < 1e-14
