@@ -49,6 +49,7 @@ extern "C" {
 #define RTB_ERR_LINALG -21               /* TbError::Linalg (error.rs:83): a singular lattice */
 #define RTB_ERR_INVALID_THERMO -22       /* TbError::InvalidThermodynamicParameter (thermodynamics.rs) */
 #define RTB_ERR_MISSING_CONJUGATE -23    /* TbError::MissingHermitianConjugate (error.rs:285) */
+#define RTB_ERR_INVALID_DIMENSION -24    /* TbError::InvalidDimension (error.rs:239) */
 
 /* ---- enums ---- */
 #define RTB_GAUGE_LATTICE 0 /* Gauge::Lattice (model.rs:449-453) */
@@ -210,6 +211,25 @@ int rtb_occupied_berry_curvature(const rtb_model* m, const double* k, size_t nk,
 int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu,
                     double T, double eta, const double* dir, int spin_dir, int precision,
                     double* sigma);
+
+/* Model::hall_conductivity, Integration::EnergyCut (response/linear/mod.rs:177-185,
+ * 226-268) for a 2D model: per-vertex E and K^{ab}_nm = v^a_nm v^b_mn
+ * (compute_velocity_kernel, response/primitives.rs:33-137, FP64), band tracking
+ * over the mesh (global_band_track, response/tracking.rs:123-163), the Fermi cut
+ * of each cell's four triangles (integrate_fermi_cut_2d, response/energy_cut.rs:
+ * 1231-1409), / det(lat).  mu must be sorted ascending (RTB_ERR_INVALID_RESPONSE);
+ * a 1D model gives RTB_ERR_INVALID_DIMENSION, a 3D one RTB_ERR_UNSUPPORTED. */
+int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu,
+                        double T, double eta, const double* dir, int spin_dir, double* sigma);
+
+/* The host part of rtb_hall_energy_cut from given vertex data (no device
+ * work): E [nx ny][nsta], K [nx ny][nsta][nsta] complex, the raw eigenvector
+ * overlaps ovx [(nx-1) ny][nsta][nsta] = U_i^H U_{i+ny} and ovy [nx ny - 1]
+ * [nsta][nsta] = U_i^H U_{i+1} (complex, U: eigenvectors as columns), sorted
+ * mu; sigma[n_mu] = integrate_fermi_cut_2d after global_band_track, / det. */
+int rtb_energy_cut_host(size_t nx, size_t ny, int nsta, const double* E, const double* K,
+                        const double* ovx, const double* ovy, const double* mu, size_t n_mu,
+                        double T, double eta, double det, double* sigma);
 
 /* QuantumGeometry::quantum_geometry_on (quantum_geometry.rs:104-140, kernel
  * quantum_geometry_at_impl :149-190) for nk k-points: with A = U^H v_a U,

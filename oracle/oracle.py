@@ -467,6 +467,8 @@ def spin_matrix(norb, spin):
     elif spin == "Z":
         m[idx, idx] = 0.5
         m[idx + norb, idx + norb] = -0.5
+    else:
+        raise OracleError("InvalidSpinDirection", repr(spin))
     return m
 
 

@@ -37,6 +37,7 @@ ERRORS = {
     -21: "Linalg",
     -22: "InvalidThermodynamicParameter",
     -23: "MissingHermitianConjugate",
+    -24: "InvalidDimension",
 }
 
 
@@ -94,6 +95,9 @@ SIGNATURES = {
     "rtb_occupied_berry_curvature": (c_int, [c_void_p, _P, c_size_t, _P, c_int, c_double, c_double,
                                              c_double, c_int, _P]),
     "rtb_hall_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, _P]),
+    "rtb_hall_energy_cut": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, _P]),
+    "rtb_energy_cut_host": (c_int, [c_size_t, c_size_t, c_int, _P, _P, _P, _P, _P, c_size_t, c_double, c_double,
+                                    c_double, _P]),
     "rtb_dos": (c_int, [c_void_p, _P, c_double, c_double, c_size_t, c_double, c_int, _P, _P]),
     "rtb_quantum_geometry": (c_int, [c_void_p, _P, c_size_t, _P, c_double, c_int, _P, _P, _P]),
     "rtb_quantum_geometry_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, _P, _P]),

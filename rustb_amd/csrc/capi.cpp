@@ -1072,6 +1072,129 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
   return berry_impl(m, j);
 }
 
+int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T, double eta,
+                        const double* dir, int spin_dir, double* sigma) {
+  RTB_TRY(prepare(m));
+  // validate_rank2 and the spin check as the Direct form, then EnergyCut's
+  // own checks (linear/mod.rs:177-185)
+  size_t nk = 0;
+  RTB_TRY(check_kmesh(m, kmesh, &nk));
+  RTB_TRY(check_direction(m, dir));
+  RTB_TRY(check_mu(mu, n_mu));
+  RTB_TRY(check_eta(eta));
+  RTB_TRY(check_T(T));
+  RTB_TRY(check_spin(m, spin_dir));
+  for (size_t i = 0; i + 1 < n_mu; ++i)
+    if (mu[i] > mu[i + 1]) return fail(RTB_ERR_INVALID_RESPONSE, "mu: must be sorted in ascending order");
+  if (m->dim != 2 && m->dim != 3) {
+    set_error("Invalid dimension for operation: %d. Supported dimensions: [2, 3]", m->dim);
+    return RTB_ERR_INVALID_DIMENSION;
+  }
+  if (m->dim == 3) return fail(RTB_ERR_UNSUPPORTED, "EnergyCut integration is implemented for 2D models only");
+  if (!sigma) return fail(RTB_ERR_INVALID_ARGUMENT, "null sigma");
+  const int n = m->nsta;
+  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  const size_t nx = kmesh[0], ny = kmesh[1];
+  const size_t n2 = static_cast<size_t>(n) * n;
+  Stream s;
+  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
+  hipStream_t st = s.get();
+  Work w(st);
+  Timer tmr(st);
+  FourierArgs a;
+  RTB_TRY(upload_dirs(st, w, m, dir, 2, a));
+  a.gauge = RTB_GAUGE_ATOM;  // compute_velocity_kernel(.., Gauge::Atom, ..) (linear/mod.rs:235-242)
+  a.combine = 0;
+  const bool spinj = m->spin && spin_dir != RTB_SPIN_NONE;
+  const bool rterm = m->d_rmat != nullptr;
+  // resident for the whole mesh: eigenvectors U, K and E per vertex, and the
+  // neighbour overlaps; per chunk: H, V (2), eigensolver scratch, T1/A/B, J
+  double *dU, *dKv, *dEv, *dOx = nullptr, *dOy = nullptr;
+  RTB_TRY(w.alloc(&dU, nk * n2 * 2));
+  RTB_TRY(w.alloc(&dKv, nk * n2 * 2));
+  RTB_TRY(w.alloc(&dEv, nk * n));
+  if (nx > 1) RTB_TRY(w.alloc(&dOx, (nx - 1) * ny * n2 * 2));
+  if (nk > 1) RTB_TRY(w.alloc(&dOy, (nk - 1) * n2 * 2));
+  const size_t per_k = 24 + n2 * 16 * 3 + n2 * 48 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56;
+  const size_t K = chunk_for(nk, per_k);
+  double *dk, *dH, *dV, *kub, *dJ = nullptr, *rproj = nullptr;
+  EigBufs eb;
+  RTB_TRY(w.alloc(&dk, K * 3));
+  RTB_TRY(w.alloc(&dH, K * n2 * 2));
+  RTB_TRY(w.alloc(&dV, K * n2 * 4));
+  RTB_TRY(alloc_eig(w, n, K, true, eb));
+  RTB_TRY(w.alloc(&kub, K * n2 * 6));
+  if (spinj) RTB_TRY(w.alloc(&dJ, K * n2 * 2));
+  if (rterm) RTB_TRY(w.alloc(&rproj, static_cast<size_t>(m->nR) * n2 * 2));
+  double* T1 = kub;
+  double* A = kub + K * n2 * 2;
+  double* B = kub + K * n2 * 4;
+  for (size_t k0 = 0; k0 < nk; k0 += K) {
+    const size_t c = std::min(K, nk - k0);
+    double* Zc = dU + k0 * n2 * 2;
+    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+    RTB_TRY(mesh_points(st, m->dim, kmesh, k0, c, dk));
+    RTB_TRY(fourier_dense(st, a, dk, c, dH, dV));
+    if (rterm) RTB_TRY(rmatrix_velocity(st, a, m->d_rmat, dir, dk, c, dH, dV, rproj, T1, A, B));
+    RTB_TRY(tmr.end(rterm ? 3 : 2));
+    // hamk.eigh(UPLO::Lower) (primitives.rs:87): the upper triangle of a
+    // storage that is not exactly Hermitian
+    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dEv + k0 * n, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm));
+    RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
+    RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, Zc, false));
+    RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
+    // v^a = U^H J_a U, v^b = U^H v_b U (the reference's evec^T v evec^* with
+    // evec = conj(U)), the spin current on the first direction only
+    const double* Ma = dV;
+    size_t sMa = 2 * n2;
+    if (spinj) {
+      RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+      RTB_TRY(spin_current(st, m->norb, c, spin_dir, dV, 2 * n2 * 2, nullptr, dJ));
+      RTB_TRY(tmr.end(1));
+      Ma = dJ;
+      sMa = n2;
+    }
+    RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
+    RTB_TRY(zgemm_batched(st, n, c, Ma, sMa, false, Zc, n2, T1, n2));
+    RTB_TRY(zgemm_batched(st, n, c, Zc, n2, true, T1, n2, A, n2));
+    RTB_TRY(zgemm_batched(st, n, c, dV + n2 * 2, 2 * n2, false, Zc, n2, T1, n2));
+    RTB_TRY(zgemm_batched(st, n, c, Zc, n2, true, T1, n2, B, n2));
+    RTB_TRY(tmr.end(4, 4 * flop_zgemm(n, c), 4 * mflop_zgemm(n, c, false)));
+    RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+    RTB_TRY(energy_cut_vertex(st, n, c, A, B, dKv + k0 * n2 * 2));
+    RTB_TRY(tmr.end(1));
+  }
+  // raw overlaps of mesh neighbours (i, i + ny) and (i, i + 1): U_i^H U_j
+  RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
+  int nl = 0;
+  const size_t piece = 32768;
+  for (size_t i0 = 0; nx > 1 && i0 < (nx - 1) * ny; i0 += piece, ++nl) {
+    const size_t c = std::min(piece, (nx - 1) * ny - i0);
+    RTB_TRY(zgemm_batched(st, n, c, dU + i0 * n2 * 2, n2, true, dU + (i0 + ny) * n2 * 2, n2, dOx + i0 * n2 * 2, n2));
+  }
+  for (size_t i0 = 0; nk > 1 && i0 < nk - 1; i0 += piece, ++nl) {
+    const size_t c = std::min(piece, nk - 1 - i0);
+    RTB_TRY(zgemm_batched(st, n, c, dU + i0 * n2 * 2, n2, true, dU + (i0 + 1) * n2 * 2, n2, dOy + i0 * n2 * 2, n2));
+  }
+  RTB_TRY(tmr.end(nl, flop_zgemm(n, (nx > 1 ? (nx - 1) * ny : 0) + nk - 1)));
+  std::vector<double> hE(nk * n), hK(nk * n2 * 2), hOx(nx > 1 ? (nx - 1) * ny * n2 * 2 : 0),
+      hOy(nk > 1 ? (nk - 1) * n2 * 2 : 0);
+  RTB_HIP(hipMemcpyAsync(hE.data(), dEv, hE.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipMemcpyAsync(hK.data(), dKv, hK.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (!hOx.empty()) RTB_HIP(hipMemcpyAsync(hOx.data(), dOx, hOx.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (!hOy.empty()) RTB_HIP(hipMemcpyAsync(hOy.data(), dOy, hOy.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipStreamSynchronize(st));
+  RTB_TRY(tmr.collect(nk, n));
+  // band tracking and the Fermi cut on the host (global_band_track is a
+  // sequential breadth-first walk)
+  std::vector<int> perm;
+  energy_cut_track(nx, ny, n, hOx.data(), hOy.data(), perm);
+  const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
+  energy_cut_sigma_2d(nx, ny, n, hE.data(), hK.data(), perm.data(), mu, n_mu, width, eta, sigma);
+  for (size_t i = 0; i < n_mu; ++i) sigma[i] /= m->det_lat;  // linear/mod.rs:267
+  return RTB_OK;
+}
+
 int rtb_quantum_geometry(const rtb_model* m, const double* k, size_t nk, const double* dir, double eta,
                          int precision, double* metric, double* omega, double* E) {
   RTB_TRY(prepare(m));

@@ -1,0 +1,420 @@
+// EnergyCut Hall conductivity in two dimensions, host part: band tracking over
+// the k-mesh and the per-triangle Fermi cut, from vertex data the device
+// forms (band energies, K^{ab}_nm = v^a_nm v^b_mn, and the raw eigenvector
+// overlaps of mesh neighbours).  Follows hall_conductivity,
+// Integration::EnergyCut (src/response/linear/mod.rs:226-268),
+// global_band_track (src/response/tracking.rs:25-163) and
+// integrate_fermi_cut_2d (src/response/energy_cut.rs:1102-1409).
+//
+// Vertex data stays in the order the eigensolver produced; tracking keeps a
+// permutation per vertex (band n of vertex i is raw band perm[i][n]) instead
+// of moving matrices.
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <thread>
+#include <vector>
+
+#include "rtb_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rtb {
+namespace {
+
+constexpr double kCutEps = 1e-12;  // ENERGY_CUT_EPS (energy_cut.rs:47)
+constexpr double kXCut = 18.0;     // FERMI_X_CUT (energy_cut.rs:48)
+constexpr int kXSteps = 72;        // FERMI_X_STEPS (energy_cut.rs:49)
+// degree-2 symmetric 3-point rule, barycentric (quadrature.rs TRI_QUAD_PTS_3)
+constexpr double kQ[3][3] = {{2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                             {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0},
+                             {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0}};
+constexpr double kQW = 1.0 / 3.0;
+
+struct Mesh {
+  size_t nx, ny;
+  int n;
+  const double* E;    // [nk][n]
+  const double* K;    // [nk][n][n] complex
+  const double* ovx;  // [(nx-1) ny][n][n] complex, U_i^H U_{i+ny}
+  const double* ovy;  // [nk-1][n][n] complex, U_i^H U_{i+1}
+  const int* perm;    // [nk][n]
+};
+
+// |<u_a(i)|u_b(j)>|^2 for mesh neighbours i, j (build_overlap_matrix,
+// tracking.rs:25-42, one element; the reference's evec is conj(U), which
+// leaves the modulus unchanged).
+double overlap2(const Mesh& M, size_t i, size_t j, int a, int b) {
+  const size_t n2 = static_cast<size_t>(M.n) * M.n;
+  const double* s;
+  if (j == i + M.ny) {
+    s = M.ovx + 2 * (i * n2 + static_cast<size_t>(a) * M.n + b);
+  } else if (i == j + M.ny) {
+    s = M.ovx + 2 * (j * n2 + static_cast<size_t>(b) * M.n + a);
+  } else if (j == i + 1) {
+    s = M.ovy + 2 * (i * n2 + static_cast<size_t>(a) * M.n + b);
+  } else {
+    s = M.ovy + 2 * (j * n2 + static_cast<size_t>(b) * M.n + a);
+  }
+  return s[0] * s[0] + s[1] * s[1];
+}
+
+// neighbour_indices (tracking.rs:165-224), 2D: +x, -x, +y, -y, not periodic.
+int neighbours(size_t i, size_t nx, size_t ny, size_t out[4]) {
+  const size_t ix = i / ny, iy = i % ny;
+  int c = 0;
+  if (ix + 1 < nx) out[c++] = i + ny;
+  if (ix > 0) out[c++] = i - ny;
+  if (iy + 1 < ny) out[c++] = i + 1;
+  if (iy > 0) out[c++] = i - 1;
+  return c;
+}
+
+// greedy_assign (tracking.rs:44-68).
+void greedy_assign(int n, const std::vector<double>& ov, std::vector<int>& perm, std::vector<char>& assigned,
+                   std::vector<std::pair<int, double>>& rows) {
+  rows.clear();
+  for (int i = 0; i < n; ++i) {
+    double best = -1.0;
+    for (int j = 0; j < n; ++j) best = std::fmax(best, ov[static_cast<size_t>(i) * n + j]);
+    rows.emplace_back(i, best);
+  }
+  std::stable_sort(rows.begin(), rows.end(), [](const auto& x, const auto& y) { return y.second < x.second; });
+  std::fill(assigned.begin(), assigned.end(), 0);
+  for (const auto& r : rows) {
+    int best_j = 0;
+    double best_val = -1.0;
+    for (int j = 0; j < n; ++j) {
+      const double v = ov[static_cast<size_t>(r.first) * n + j];
+      if (!assigned[j] && v > best_val) {
+        best_val = v;
+        best_j = j;
+      }
+    }
+    assigned[best_j] = 1;
+    perm[r.first] = best_j;
+  }
+}
+
+double kre(const Mesh& M, size_t v, int a, int b) {
+  const size_t n = M.n;
+  return M.K[2 * (v * n * n + static_cast<size_t>(M.perm[v * n + a]) * n + M.perm[v * n + b])];
+}
+double kim(const Mesh& M, size_t v, int a, int b) {
+  const size_t n = M.n;
+  return M.K[2 * (v * n * n + static_cast<size_t>(M.perm[v * n + a]) * n + M.perm[v * n + b]) + 1];
+}
+double band(const Mesh& M, size_t v, int a) { return M.E[v * M.n + M.perm[v * M.n + a]]; }
+
+// eval_berry_kernel (kernel.rs:54-80), Omega only.
+void vertex_omega(const Mesh& M, size_t v, double eta, double* om) {
+  const double eta2 = eta * eta;
+  for (int i = 0; i < M.n; ++i) {
+    double im = 0.0;
+    for (int m = 0; m < M.n; ++m) {
+      if (m == i) continue;
+      const double de = band(M, v, i) - band(M, v, m);
+      const double denom = de * de + eta2;
+      if (denom < 1e-30) continue;
+      im += kim(M, v, i, m) / denom;
+    }
+    om[i] = -2.0 * im;
+  }
+}
+
+// eval_berry_band_at_lam_buf (kernel.rs:138-182).
+double band_at_lam(const Mesh& M, const size_t vs[3], int n, const double lam[3], double eta, std::vector<double>& e,
+                   std::vector<double>& ki) {
+  std::fill(e.begin(), e.end(), 0.0);
+  std::fill(ki.begin(), ki.end(), 0.0);
+  for (int v = 0; v < 3; ++v) {
+    if (lam[v] == 0.0) continue;
+    for (int m = 0; m < M.n; ++m) e[m] += band(M, vs[v], m) * lam[v];
+  }
+  for (int v = 0; v < 3; ++v) {
+    if (lam[v] == 0.0) continue;
+    for (int m = 0; m < M.n; ++m) ki[m] += kim(M, vs[v], n, m) * lam[v];
+  }
+  const double eta2 = eta * eta;
+  double im = 0.0;
+  for (int m = 0; m < M.n; ++m) {
+    if (m == n) continue;
+    const double de = e[n] - e[m];
+    const double denom = de * de + eta2;
+    if (denom < 1e-30) continue;
+    im += ki[m] / denom;
+  }
+  return -2.0 * im;
+}
+
+double tri_area(const double c[3][2]) {  // triangle_area (energy_cut.rs:69-78)
+  return 0.5 * std::fabs((c[1][0] - c[0][0]) * (c[2][1] - c[0][1]) - (c[2][0] - c[0][0]) * (c[1][1] - c[0][1]));
+}
+
+// sub_tri_area_2d with bary_to_phys_2d (energy_cut.rs:1102-1116).
+double sub_area(const double c[3][2], const double t[3][3]) {
+  double p[3][2];
+  for (int q = 0; q < 3; ++q)
+    for (int d = 0; d < 2; ++d) p[q][d] = t[q][0] * c[0][d] + t[q][1] * c[1][d] + t[q][2] * c[2][d];
+  return 0.5 * std::fabs((p[1][0] - p[0][0]) * (p[2][1] - p[0][1]) - (p[2][0] - p[0][0]) * (p[1][1] - p[0][1]));
+}
+
+// clip_triangle (energy_cut.rs:1120-1169): the occupied part E <= mu of the
+// linearly interpolated triangle as 0-2 sub-triangles (barycentric rows).
+int clip(const double e[3], double mu, double out[2][3][3]) {
+  const double eps = 1e-12;
+  const double e_min = std::fmin(std::fmin(e[0], e[1]), e[2]);
+  const double e_max = std::fmax(std::fmax(e[0], e[1]), e[2]);
+  for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < 3; ++q)
+      for (int d = 0; d < 3; ++d) out[s][q][d] = 0.0;
+  if (mu <= e_min + eps) return 0;
+  if (mu >= e_max - eps) {
+    for (int q = 0; q < 3; ++q) out[0][q][q] = 1.0;
+    return 1;
+  }
+  int idx[3] = {0, 1, 2};
+  std::stable_sort(idx, idx + 3, [&](int i, int j) { return e[i] < e[j]; });
+  const int a = idx[0], b = idx[1], c = idx[2];
+  auto cut = [&](int i, int j, double* lam) {
+    const double de = e[j] - e[i];
+    double t = 0.5;
+    if (!(std::fabs(de) < eps)) {
+      t = (mu - e[i]) / de;
+      t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    }
+    lam[i] = 1.0 - t;
+    lam[j] = t;
+  };
+  if (mu <= e[b] + eps) {
+    out[0][0][a] = 1.0;
+    cut(a, b, out[0][1]);
+    cut(a, c, out[0][2]);
+    return 1;
+  }
+  out[0][0][a] = 1.0;
+  out[0][1][b] = 1.0;
+  cut(a, c, out[0][2]);
+  out[1][0][b] = 1.0;
+  cut(a, c, out[1][1]);
+  cut(b, c, out[1][2]);
+  return 2;
+}
+
+// triangle_occupied_hybrid (energy_cut.rs:1184-1228).
+double hybrid(const Mesh& M, const double c[3][2], const size_t vs[3], const double e_v[3], const double om_v[3],
+              double mu, double eta, int n, std::vector<double>& e, std::vector<double>& ki) {
+  const double e_min = std::fmin(std::fmin(e_v[0], e_v[1]), e_v[2]);
+  const double e_max = std::fmax(std::fmax(e_v[0], e_v[1]), e_v[2]);
+  const double area = tri_area(c);
+  if (mu <= e_min + kCutEps) return 0.0;
+  if (mu >= e_max - kCutEps) return area * (om_v[0] + om_v[1] + om_v[2]) / 3.0;
+  double subs[2][3][3];
+  const int ns = clip(e_v, mu, subs);
+  double total = 0.0;
+  for (int s = 0; s < ns; ++s) {
+    const double sa = sub_area(c, subs[s]);
+    if (sa < 1e-30) continue;
+    for (int iq = 0; iq < 3; ++iq) {
+      double lam[3];
+      for (int d = 0; d < 3; ++d)  // combine_bary (energy_cut.rs:1172-1178)
+        lam[d] = kQ[iq][0] * subs[s][0][d] + kQ[iq][1] * subs[s][1][d] + kQ[iq][2] * subs[s][2][d];
+      total += sa * kQW * band_at_lam(M, vs, n, lam, eta, e, ki);
+    }
+  }
+  return total;
+}
+
+// integrate_fermi_cut_2d_t0 (energy_cut.rs:1231-1345) for the cells
+// idx in [i0, i1), over both diagonals' triangles (build_triangles_2d_diagavg,
+// tracking.rs:284-342), into acc [n_mu].
+void cut_t0_cells(const Mesh& M, size_t i0, size_t i1, const double* mu, size_t n_mu, double eta, double* acc) {
+  const size_t nx = M.nx, ny = M.ny;
+  const double inv_nx = 1.0 / static_cast<double>(nx), inv_ny = 1.0 / static_cast<double>(ny);
+  const double half_vol = inv_nx * inv_ny / 4.0;
+  const int n = M.n;
+  std::vector<double> e(n), ki(n), om(3 * static_cast<size_t>(n));
+  for (size_t idx = i0; idx < i1; ++idx) {
+    const size_t iy = idx % ny, ix = idx / ny;
+    const size_t ixp = (ix + 1) % nx, iyp = (iy + 1) % ny;
+    const size_t i00 = ix * ny + iy, i10 = ixp * ny + iy, i11 = ixp * ny + iyp, i01 = ix * ny + iyp;
+    auto coord_of = [&](size_t i, double* c) {
+      const double fx = static_cast<double>(ix), fy = static_cast<double>(iy);
+      if (i == i00) {
+        c[0] = fx * inv_nx, c[1] = fy * inv_ny;
+      } else if (i == i10) {
+        c[0] = (fx + 1.0) * inv_nx, c[1] = fy * inv_ny;
+      } else if (i == i11) {
+        c[0] = (fx + 1.0) * inv_nx, c[1] = (fy + 1.0) * inv_ny;
+      } else {
+        c[0] = fx * inv_nx, c[1] = (fy + 1.0) * inv_ny;
+      }
+    };
+    const size_t tris[4][3] = {{i00, i10, i01}, {i11, i10, i01}, {i00, i10, i11}, {i00, i11, i01}};
+    for (const auto& tv : tris) {
+      double c[3][2];
+      for (int q = 0; q < 3; ++q) coord_of(tv[q], c[q]);
+      const double area = tri_area(c);
+      if (area < kCutEps) continue;
+      const double scale = half_vol / area;
+      for (int q = 0; q < 3; ++q) vertex_omega(M, tv[q], eta, om.data() + static_cast<size_t>(q) * n);
+      for (int b = 0; b < n; ++b) {
+        const double e_v[3] = {band(M, tv[0], b), band(M, tv[1], b), band(M, tv[2], b)};
+        const double om_v[3] = {om[b], om[n + b], om[2 * static_cast<size_t>(n) + b]};
+        const double e_min = std::fmin(std::fmin(e_v[0], e_v[1]), e_v[2]);
+        const double e_max = std::fmax(std::fmax(e_v[0], e_v[1]), e_v[2]);
+        const double full_val = area * (om_v[0] + om_v[1] + om_v[2]) / 3.0;
+        const size_t i_partial =
+            std::partition_point(mu, mu + n_mu, [&](double x) { return x <= e_min + kCutEps; }) - mu;
+        const size_t i_full = std::partition_point(mu, mu + n_mu, [&](double x) { return x < e_max - kCutEps; }) - mu;
+        for (size_t im = i_partial; im < i_full; ++im)
+          acc[im] += scale * hybrid(M, c, tv, e_v, om_v, mu[im], eta, b, e, ki);
+        if (i_full < n_mu) {
+          const double add = scale * full_val;
+          for (size_t im = i_full; im < n_mu; ++im) acc[im] += add;
+        }
+      }
+    }
+  }
+}
+
+// The zero-width cut summed over all cells: fixed row blocks on up to 16
+// threads, partial sums added in block order (deterministic).
+void cut_t0(const Mesh& M, const double* mu, size_t n_mu, double eta, double* out) {
+  const size_t cells = M.nx * M.ny;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, cells / 64));
+  std::vector<std::vector<double>> part(nt, std::vector<double>(n_mu, 0.0));
+  std::vector<std::thread> th;
+  for (size_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { cut_t0_cells(M, cells * t / nt, cells * (t + 1) / nt, mu, n_mu, eta, part[t].data()); });
+  for (auto& x : th) x.join();
+  for (size_t i = 0; i < n_mu; ++i) {
+    double s = 0.0;
+    for (size_t t = 0; t < nt; ++t) s += part[t][i];
+    out[i] = s;
+  }
+}
+
+double fermi_window_x(double x) {  // energy_cut.rs:59-66
+  if (std::fabs(x) > 50.0) return 0.0;
+  const double ex = std::exp(x);
+  return ex / ((1.0 + ex) * (1.0 + ex));
+}
+
+}  // namespace
+
+// global_band_track (tracking.rs:123-163): breadth-first from vertex 0; a new
+// vertex takes greedy_assign of its overlaps with its visited neighbours
+// (summed, then divided by their count).
+void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, std::vector<int>& perm) {
+  const size_t nk = nx * ny;
+  perm.resize(nk * n);
+  for (size_t i = 0; i < nk; ++i)
+    for (int b = 0; b < n; ++b) perm[i * n + b] = b;
+  if (nk <= 1) return;
+  Mesh M{nx, ny, n, nullptr, nullptr, ovx, ovy, perm.data()};
+  std::vector<char> visited(nk, 0), assigned(n);
+  std::vector<double> ov(static_cast<size_t>(n) * n);
+  std::vector<int> p(n);
+  std::vector<std::pair<int, double>> rows;
+  std::deque<size_t> queue;
+  visited[0] = 1;
+  queue.push_back(0);
+  size_t nbs[4], vns[4];
+  while (!queue.empty()) {
+    const size_t cur = queue.front();
+    queue.pop_front();
+    const int cn = neighbours(cur, nx, ny, nbs);
+    for (int q = 0; q < cn; ++q) {
+      const size_t nb = nbs[q];
+      if (visited[nb]) continue;
+      int cnt = 0;
+      const int vc = neighbours(nb, nx, ny, vns);
+      for (int r = 0; r < vc; ++r) {
+        const size_t vn = vns[r];
+        if (!visited[vn]) continue;
+        // ov[a][b] = |<tracked band a of vn | raw band b of nb>|^2
+        for (int a = 0; a < n; ++a)
+          for (int b = 0; b < n; ++b) {
+            const double x = overlap2(M, vn, nb, perm[vn * n + a], b);
+            double& o = ov[static_cast<size_t>(a) * n + b];
+            o = cnt == 0 ? x : o + x;
+          }
+        ++cnt;
+      }
+      if (cnt > 0) {
+        for (double& o : ov) o /= static_cast<double>(cnt);
+        greedy_assign(n, ov, p, assigned, rows);
+        for (int b = 0; b < n; ++b) perm[nb * n + b] = p[b];
+      }
+      visited[nb] = 1;
+      queue.push_back(nb);
+    }
+  }
+}
+
+// integrate_fermi_cut_2d (energy_cut.rs:1355-1409): the zero-width cut, or
+// its convolution with the Fermi window for width > 0.  Fractional measure.
+void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* K, const int* perm,
+                         const double* mu, size_t n_mu, double width, double eta, double* out) {
+  const Mesh M{nx, ny, n, E, K, nullptr, nullptr, perm};
+  if (width == 0.0) {
+    cut_t0(M, mu, n_mu, eta, out);
+    return;
+  }
+  const double beta = 1.0 / width;
+  const double x_max = 12.0;
+  double mu_min = INFINITY, mu_max = -INFINITY;
+  for (size_t i = 0; i < n_mu; ++i) {
+    mu_min = std::fmin(mu_min, mu[i]);
+    mu_max = std::fmax(mu_max, mu[i]);
+  }
+  const double mu_lo = mu_min - x_max / beta;
+  const double mu_hi = mu_max + x_max / beta;
+  const double dmu_fine = 0.1 / beta;
+  const size_t n_ext = static_cast<size_t>(std::ceil((mu_hi - mu_lo) / dmu_fine)) + 1;
+  // ndarray's linspace: a + i (b - a) / (n - 1)
+  const double step = n_ext > 1 ? (mu_hi - mu_lo) / static_cast<double>(n_ext - 1) : 0.0;
+  std::vector<double> mu_ext(n_ext), s0(n_ext);
+  for (size_t i = 0; i < n_ext; ++i) mu_ext[i] = mu_lo + step * static_cast<double>(i);
+  cut_t0(M, mu_ext.data(), n_ext, eta, s0.data());
+  const double dx = 2.0 * kXCut / static_cast<double>(kXSteps);
+  for (size_t i = 0; i < n_mu; ++i) {
+    double sum = 0.0;
+    for (int iq = 0; iq < kXSteps; ++iq) {
+      const double x = -kXCut + (static_cast<double>(iq) + 0.5) * dx;
+      const double w = fermi_window_x(x);
+      const double i_f = (mu[i] + x / beta - mu_lo) / dmu_fine;
+      const double fl = std::floor(i_f);
+      const size_t i_lo = fl > 0.0 ? static_cast<size_t>(fl) : 0;
+      const size_t i_hi = std::min(i_lo + 1, n_ext - 1);
+      if (i_hi > i_lo) {
+        const double t = i_f - static_cast<double>(i_lo);
+        const double val = s0[i_lo] + t * (s0[i_hi] - s0[i_lo]);
+        sum += dx * w * val;
+      }
+    }
+    out[i] = sum;
+  }
+}
+
+}  // namespace rtb
+
+int rtb_energy_cut_host(size_t nx, size_t ny, int nsta, const double* E, const double* K, const double* ovx,
+                        const double* ovy, const double* mu, size_t n_mu, double T, double eta, double det,
+                        double* sigma) {
+  using namespace rtb;
+  if (nx == 0 || ny == 0 || nsta <= 0) return fail(RTB_ERR_INVALID_KMESH, "kmesh and nsta must be positive");
+  if (!E || !K || !mu || !sigma || (nx > 1 && !ovx) || (nx * ny > 1 && !ovy))
+    return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (n_mu == 0) return fail(RTB_ERR_INVALID_RESPONSE, "chemical_potentials: must contain at least one value");
+  for (size_t i = 0; i + 1 < n_mu; ++i)  // validate_sorted (config.rs:291-303)
+    if (mu[i] > mu[i + 1]) return fail(RTB_ERR_INVALID_RESPONSE, "mu: must be sorted in ascending order");
+  std::vector<int> perm;
+  energy_cut_track(nx, ny, nsta, ovx, ovy, perm);
+  const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
+  energy_cut_sigma_2d(nx, ny, nsta, E, K, perm.data(), mu, n_mu, width, eta, sigma);
+  for (size_t i = 0; i < n_mu; ++i) sigma[i] /= det;  // linear/mod.rs:267
+  return RTB_OK;
+}

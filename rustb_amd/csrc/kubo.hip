@@ -290,6 +290,32 @@ int dipole_combine(hipStream_t st, int n, size_t nk, const double* A2, const dou
   return RTB_OK;
 }
 
+namespace {
+// K[b][i][j] = A[b][i][j] B[b][j][i] (compute_velocity_kernel,
+// response/primitives.rs:106-111): one thread per element.
+__global__ void cut_vertex_kernel(size_t total, int n, const double2* __restrict__ A, const double2* __restrict__ B,
+                                  double2* __restrict__ K) {
+  const size_t g = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  const size_t b = g / n2, r = g % n2;
+  const size_t i = r / n, j = r % n;
+  const double2 a = A[g];
+  const double2 c = B[b * n2 + j * n + i];
+  K[g] = make_double2(a.x * c.x - a.y * c.y, a.x * c.y + a.y * c.x);
+}
+}  // namespace
+
+int energy_cut_vertex(hipStream_t st, int n, size_t nk, const double* A, const double* B, double* K) {
+  const size_t total = nk * n * n;
+  if (total == 0) return RTB_OK;
+  hipLaunchKernelGGL(cut_vertex_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, total, n,
+                     reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B),
+                     reinterpret_cast<double2*>(K));
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
 int hall_blocks(size_t nk) { return static_cast<int>((nk + kHallK - 1) / kHallK); }
 
 int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const double* E, const double* mu,

@@ -119,6 +119,15 @@ inline double mflop_zgemm(int n, size_t nb, bool herm) {
   return tiles * 64.0 * 64.0 * kpad * 6.0 * static_cast<double>(nb);
 }
 
+// EnergyCut Hall conductivity, host part (energy_cut.cpp).  Band tracking:
+// perm [nx ny][n] (band b of vertex i is raw band perm[i][b]) from the raw
+// overlaps ovx [(nx-1) ny][n][n] = U_i^H U_{i+ny}, ovy [nx ny - 1][n][n] =
+// U_i^H U_{i+1} (complex).  The 2D Fermi cut of E [nk][n] and K [nk][n][n]
+// (complex) for sorted mu, fractional-coordinate measure, thermal width in eV.
+void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, std::vector<int>& perm);
+void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* K, const int* perm,
+                         const double* mu, size_t n_mu, double width, double eta, double* out);
+
 // Largest nsta the batched eigensolver handles (packed lower triangle of the
 // FP64 matrix in LDS: n(n+1)/2 * 16 B <= 160 KB).
 constexpr int kMaxEighN = 128;

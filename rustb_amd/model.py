@@ -448,9 +448,12 @@ class Model:
         return float(self.occupied_berry_curvature_on(np.asarray(k, float)[None, :], params)[0])
 
     def hall_conductivity(self, params: Parameters) -> HallConductivityResult:
-        """hall_conductivity, Integration::Direct (response/linear/mod.rs:160-225)."""
-        if params.integration != "Direct":
-            raise TbError(-6, "integration: this path implements Integration::Direct")
+        """hall_conductivity (response/linear/mod.rs:160-276): Integration::Direct
+        (the k-sum of occupied Omega), or EnergyCut for a 2D model (band tracking
+        and the per-triangle Fermi cut, always FP64); Simplex is rejected as the
+        reference does (:166-176)."""
+        if params.integration not in ("Direct", "EnergyCut"):
+            raise TbError(-6, "integration: hall_conductivity supports Integration::Direct or EnergyCut, not Simplex")
         kmesh = np.ascontiguousarray(np.asarray(params.kmesh, dtype=np.uint64))
         if kmesh.shape != (self.dim,):
             raise TbError(-3, "kmesh must have dim_r entries")
@@ -459,6 +462,11 @@ class Model:
         if T.size == 0:
             raise TbError(-6, "T: must contain at least one value")
         sig = np.zeros(mu.size)
+        if params.integration == "EnergyCut":
+            check(load().rtb_hall_energy_cut(self._device(), ptr(kmesh), ptr(mu), mu.size, float(T[0]),
+                                             float(params.eta), ptr(self._dir(params)), SPIN_DIR[params.spin],
+                                             ptr(sig)))
+            return HallConductivityResult(mu.copy(), sig)
         check(load().rtb_hall_direct(self._device(), ptr(kmesh), ptr(mu), mu.size, float(T[0]), float(params.eta),
                                      ptr(self._dir(params)), SPIN_DIR[params.spin], PRECISION[params.precision],
                                      ptr(sig)))
