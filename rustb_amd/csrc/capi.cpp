@@ -1107,14 +1107,15 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   a.combine = 0;
   const bool spinj = m->spin && spin_dir != RTB_SPIN_NONE;
   const bool rterm = m->d_rmat != nullptr;
-  // resident for the whole mesh: eigenvectors U, K and E per vertex, and the
-  // neighbour overlaps; per chunk: H, V (2), eigensolver scratch, T1/A/B, J
+  // resident for the whole mesh: eigenvectors U, Im K and E per vertex, and
+  // the squared neighbour overlaps (what the host reads); per chunk: H, V (2),
+  // eigensolver scratch, T1/A/B, J
   double *dU, *dKv, *dEv, *dOx = nullptr, *dOy = nullptr;
   RTB_TRY(w.alloc(&dU, nk * n2 * 2));
-  RTB_TRY(w.alloc(&dKv, nk * n2 * 2));
+  RTB_TRY(w.alloc(&dKv, nk * n2));
   RTB_TRY(w.alloc(&dEv, nk * n));
-  if (nx > 1) RTB_TRY(w.alloc(&dOx, (nx - 1) * ny * n2 * 2));
-  if (nk > 1) RTB_TRY(w.alloc(&dOy, (nk - 1) * n2 * 2));
+  if (nx > 1) RTB_TRY(w.alloc(&dOx, (nx - 1) * ny * n2));
+  if (nk > 1) RTB_TRY(w.alloc(&dOy, (nk - 1) * n2));
   const size_t per_k = 24 + n2 * 16 * 3 + n2 * 48 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56;
   const size_t K = chunk_for(nk, per_k);
   double *dk, *dH, *dV, *kub, *dJ = nullptr, *rproj = nullptr;
@@ -1161,24 +1162,27 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
     RTB_TRY(zgemm_batched(st, n, c, Zc, n2, true, T1, n2, B, n2));
     RTB_TRY(tmr.end(4, 4 * flop_zgemm(n, c), 4 * mflop_zgemm(n, c, false)));
     RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
-    RTB_TRY(energy_cut_vertex(st, n, c, A, B, dKv + k0 * n2 * 2));
+    RTB_TRY(energy_cut_vertex(st, n, c, A, B, dKv + k0 * n2));
     RTB_TRY(tmr.end(1));
   }
-  // raw overlaps of mesh neighbours (i, i + ny) and (i, i + 1): U_i^H U_j
-  RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
-  int nl = 0;
-  const size_t piece = 32768;
-  for (size_t i0 = 0; nx > 1 && i0 < (nx - 1) * ny; i0 += piece, ++nl) {
-    const size_t c = std::min(piece, (nx - 1) * ny - i0);
-    RTB_TRY(zgemm_batched(st, n, c, dU + i0 * n2 * 2, n2, true, dU + (i0 + ny) * n2 * 2, n2, dOx + i0 * n2 * 2, n2));
+  // overlaps of mesh neighbours (i, i + ny) and (i, i + 1): U_i^H U_j in
+  // pieces of at most K products through T1, then their squared moduli
+  const size_t piece = std::min<size_t>(K, 32768);
+  const size_t nxp = nx > 1 ? (nx - 1) * ny : 0, nyp = nk - 1;
+  for (int dir2 = 0; dir2 < 2; ++dir2) {
+    const size_t cnt = dir2 ? nyp : nxp, step = dir2 ? 1 : ny;
+    double* dst = dir2 ? dOy : dOx;
+    for (size_t i0 = 0; i0 < cnt; i0 += piece) {
+      const size_t c = std::min(piece, cnt - i0);
+      RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
+      RTB_TRY(zgemm_batched(st, n, c, dU + i0 * n2 * 2, n2, true, dU + (i0 + step) * n2 * 2, n2, T1, n2));
+      RTB_TRY(tmr.end(1, flop_zgemm(n, c), mflop_zgemm(n, c, false)));
+      RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+      RTB_TRY(abs2(st, c * n2, T1, dst + i0 * n2));
+      RTB_TRY(tmr.end(1));
+    }
   }
-  for (size_t i0 = 0; nk > 1 && i0 < nk - 1; i0 += piece, ++nl) {
-    const size_t c = std::min(piece, nk - 1 - i0);
-    RTB_TRY(zgemm_batched(st, n, c, dU + i0 * n2 * 2, n2, true, dU + (i0 + 1) * n2 * 2, n2, dOy + i0 * n2 * 2, n2));
-  }
-  RTB_TRY(tmr.end(nl, flop_zgemm(n, (nx > 1 ? (nx - 1) * ny : 0) + nk - 1)));
-  std::vector<double> hE(nk * n), hK(nk * n2 * 2), hOx(nx > 1 ? (nx - 1) * ny * n2 * 2 : 0),
-      hOy(nk > 1 ? (nk - 1) * n2 * 2 : 0);
+  std::vector<double> hE(nk * n), hK(nk * n2), hOx(nxp * n2), hOy(nyp * n2);
   RTB_HIP(hipMemcpyAsync(hE.data(), dEv, hE.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   RTB_HIP(hipMemcpyAsync(hK.data(), dKv, hK.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   if (!hOx.empty()) RTB_HIP(hipMemcpyAsync(hOx.data(), dOx, hOx.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1188,9 +1192,9 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   // band tracking and the Fermi cut on the host (global_band_track is a
   // sequential breadth-first walk)
   std::vector<int> perm;
-  energy_cut_track(nx, ny, n, hOx.data(), hOy.data(), perm);
+  energy_cut_track(nx, ny, n, hOx.data(), hOy.data(), true, perm);
   const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
-  energy_cut_sigma_2d(nx, ny, n, hE.data(), hK.data(), perm.data(), mu, n_mu, width, eta, sigma);
+  energy_cut_sigma_2d(nx, ny, n, hE.data(), hK.data(), 1, perm.data(), mu, n_mu, width, eta, sigma);
   for (size_t i = 0; i < n_mu; ++i) sigma[i] /= m->det_lat;  // linear/mod.rs:267
   return RTB_OK;
 }

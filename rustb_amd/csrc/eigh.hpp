@@ -151,8 +151,10 @@ int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const d
 int dipole_combine(hipStream_t st, int n, size_t nk, const double* A2, const double* A3, const double* om1,
                    const double* om2, bool sym, double* out);
 int hall_blocks(size_t nk);
-// EnergyCut vertex kernel K[b][i][j] = A[b][i][j] B[b][j][i] ([nk][n][n] complex).
-int energy_cut_vertex(hipStream_t st, int n, size_t nk, const double* A, const double* B, double* K);
+// EnergyCut vertex kernel Im K[b][i][j] = Im(A[b][i][j] B[b][j][i]) ([nk][n][n] real).
+int energy_cut_vertex(hipStream_t st, int n, size_t nk, const double* A, const double* B, double* Ki);
+// x = |z|^2 elementwise (total complex z, real x).
+int abs2(hipStream_t st, size_t total, const double* z, double* x);
 // DOS partial sums: partial[blk][j] += sum_{eps in blk} g(E_j - eps)
 int dos_partials(hipStream_t st, size_t ne, const double* evals, double Emin, double dE, int En,
                  double sigma, double* partial, int n_blk);

@@ -35,29 +35,17 @@ struct Mesh {
   size_t nx, ny;
   int n;
   const double* E;    // [nk][n]
-  const double* K;    // [nk][n][n] complex
+  const double* Ki;   // Im K [nk][n][n] at stride ks (2: inside complex K, 1: Im K alone)
+  size_t ks;
   const double* ovx;  // [(nx-1) ny][n][n] complex, U_i^H U_{i+ny}
   const double* ovy;  // [nk-1][n][n] complex, U_i^H U_{i+1}
   const int* perm;    // [nk][n]
+  const double* om;   // [nk][n] vertex Omega in tracked band order (or null)
 };
 
-// |<u_a(i)|u_b(j)>|^2 for mesh neighbours i, j (build_overlap_matrix,
-// tracking.rs:25-42, one element; the reference's evec is conj(U), which
-// leaves the modulus unchanged).
-double overlap2(const Mesh& M, size_t i, size_t j, int a, int b) {
-  const size_t n2 = static_cast<size_t>(M.n) * M.n;
-  const double* s;
-  if (j == i + M.ny) {
-    s = M.ovx + 2 * (i * n2 + static_cast<size_t>(a) * M.n + b);
-  } else if (i == j + M.ny) {
-    s = M.ovx + 2 * (j * n2 + static_cast<size_t>(b) * M.n + a);
-  } else if (j == i + 1) {
-    s = M.ovy + 2 * (i * n2 + static_cast<size_t>(a) * M.n + b);
-  } else {
-    s = M.ovy + 2 * (j * n2 + static_cast<size_t>(b) * M.n + a);
-  }
-  return s[0] * s[0] + s[1] * s[1];
-}
+// Overlaps |<u_a(i)|u_b(j)>|^2 of mesh neighbours (build_overlap_matrix,
+// tracking.rs:25-42) come from the raw products U_i^H U_j; the reference's
+// evec is conj(U), which leaves each modulus unchanged.
 
 // neighbour_indices (tracking.rs:165-224), 2D: +x, -x, +y, -y, not periodic.
 int neighbours(size_t i, size_t nx, size_t ny, size_t out[4]) {
@@ -96,13 +84,10 @@ void greedy_assign(int n, const std::vector<double>& ov, std::vector<int>& perm,
   }
 }
 
-double kre(const Mesh& M, size_t v, int a, int b) {
-  const size_t n = M.n;
-  return M.K[2 * (v * n * n + static_cast<size_t>(M.perm[v * n + a]) * n + M.perm[v * n + b])];
-}
+// Only Im K enters Omega (kernel.rs:54-80, 138-182).
 double kim(const Mesh& M, size_t v, int a, int b) {
   const size_t n = M.n;
-  return M.K[2 * (v * n * n + static_cast<size_t>(M.perm[v * n + a]) * n + M.perm[v * n + b]) + 1];
+  return M.Ki[M.ks * (v * n * n + static_cast<size_t>(M.perm[v * n + a]) * n + M.perm[v * n + b])];
 }
 double band(const Mesh& M, size_t v, int a) { return M.E[v * M.n + M.perm[v * M.n + a]]; }
 
@@ -229,11 +214,12 @@ double hybrid(const Mesh& M, const double c[3][2], const size_t vs[3], const dou
 // idx in [i0, i1), over both diagonals' triangles (build_triangles_2d_diagavg,
 // tracking.rs:284-342), into acc [n_mu].
 void cut_t0_cells(const Mesh& M, size_t i0, size_t i1, const double* mu, size_t n_mu, double eta, double* acc) {
+  const double* om_all = M.om;
   const size_t nx = M.nx, ny = M.ny;
   const double inv_nx = 1.0 / static_cast<double>(nx), inv_ny = 1.0 / static_cast<double>(ny);
   const double half_vol = inv_nx * inv_ny / 4.0;
   const int n = M.n;
-  std::vector<double> e(n), ki(n), om(3 * static_cast<size_t>(n));
+  std::vector<double> e(n), ki(n);
   for (size_t idx = i0; idx < i1; ++idx) {
     const size_t iy = idx % ny, ix = idx / ny;
     const size_t ixp = (ix + 1) % nx, iyp = (iy + 1) % ny;
@@ -257,10 +243,14 @@ void cut_t0_cells(const Mesh& M, size_t i0, size_t i1, const double* mu, size_t 
       const double area = tri_area(c);
       if (area < kCutEps) continue;
       const double scale = half_vol / area;
-      for (int q = 0; q < 3; ++q) vertex_omega(M, tv[q], eta, om.data() + static_cast<size_t>(q) * n);
+      // eval_berry_kernel of the three vertices (energy_cut.rs:1268-1285),
+      // formed once per vertex beforehand
+      const double* o0 = om_all + tv[0] * n;
+      const double* o1 = om_all + tv[1] * n;
+      const double* o2 = om_all + tv[2] * n;
       for (int b = 0; b < n; ++b) {
         const double e_v[3] = {band(M, tv[0], b), band(M, tv[1], b), band(M, tv[2], b)};
-        const double om_v[3] = {om[b], om[n + b], om[2 * static_cast<size_t>(n) + b]};
+        const double om_v[3] = {o0[b], o1[b], o2[b]};
         const double e_min = std::fmin(std::fmin(e_v[0], e_v[1]), e_v[2]);
         const double e_max = std::fmax(std::fmax(e_v[0], e_v[1]), e_v[2]);
         const double full_val = area * (om_v[0] + om_v[1] + om_v[2]) / 3.0;
@@ -278,12 +268,16 @@ void cut_t0_cells(const Mesh& M, size_t i0, size_t i1, const double* mu, size_t 
   }
 }
 
+size_t host_threads(size_t items) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, items / 64));
+}
+
 // The zero-width cut summed over all cells: fixed row blocks on up to 16
 // threads, partial sums added in block order (deterministic).
 void cut_t0(const Mesh& M, const double* mu, size_t n_mu, double eta, double* out) {
   const size_t cells = M.nx * M.ny;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, cells / 64));
+  const size_t nt = host_threads(cells);
   std::vector<std::vector<double>> part(nt, std::vector<double>(n_mu, 0.0));
   std::vector<std::thread> th;
   for (size_t t = 0; t < nt; ++t)
@@ -307,13 +301,14 @@ double fermi_window_x(double x) {  // energy_cut.rs:59-66
 // global_band_track (tracking.rs:123-163): breadth-first from vertex 0; a new
 // vertex takes greedy_assign of its overlaps with its visited neighbours
 // (summed, then divided by their count).
-void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, std::vector<int>& perm) {
+void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, bool squared,
+                      std::vector<int>& perm) {
+  const size_t zs = squared ? 1 : 2;  // doubles per overlap element
   const size_t nk = nx * ny;
   perm.resize(nk * n);
   for (size_t i = 0; i < nk; ++i)
     for (int b = 0; b < n; ++b) perm[i * n + b] = b;
   if (nk <= 1) return;
-  Mesh M{nx, ny, n, nullptr, nullptr, ovx, ovy, perm.data()};
   std::vector<char> visited(nk, 0), assigned(n);
   std::vector<double> ov(static_cast<size_t>(n) * n);
   std::vector<int> p(n);
@@ -334,13 +329,29 @@ void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const doub
       for (int r = 0; r < vc; ++r) {
         const size_t vn = vns[r];
         if (!visited[vn]) continue;
-        // ov[a][b] = |<tracked band a of vn | raw band b of nb>|^2
-        for (int a = 0; a < n; ++a)
+        // ov[a][b] = |<tracked band a of vn | raw band b of nb>|^2: element
+        // (raw row r, column b) of U_vn^H U_nb sits at base + 2 (r rs + b cs)
+        const size_t n2 = static_cast<size_t>(n) * n;
+        const double* base;
+        size_t rs = n, cs = 1;
+        if (nb == vn + ny) {
+          base = ovx + zs * vn * n2;
+        } else if (vn == nb + ny) {
+          base = ovx + zs * nb * n2, rs = 1, cs = n;
+        } else if (nb == vn + 1) {
+          base = ovy + zs * vn * n2;
+        } else {
+          base = ovy + zs * nb * n2, rs = 1, cs = n;
+        }
+        for (int a = 0; a < n; ++a) {
+          const double* row = base + zs * static_cast<size_t>(perm[vn * n + a]) * rs;
+          double* o = ov.data() + static_cast<size_t>(a) * n;
           for (int b = 0; b < n; ++b) {
-            const double x = overlap2(M, vn, nb, perm[vn * n + a], b);
-            double& o = ov[static_cast<size_t>(a) * n + b];
-            o = cnt == 0 ? x : o + x;
+            const double* z = row + zs * b * cs;
+            const double x = squared ? z[0] : z[0] * z[0] + z[1] * z[1];
+            o[b] = cnt == 0 ? x : o[b] + x;
           }
+        }
         ++cnt;
       }
       if (cnt > 0) {
@@ -356,9 +367,22 @@ void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const doub
 
 // integrate_fermi_cut_2d (energy_cut.rs:1355-1409): the zero-width cut, or
 // its convolution with the Fermi window for width > 0.  Fractional measure.
-void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* K, const int* perm,
+void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* Ki, size_t ks, const int* perm,
                          const double* mu, size_t n_mu, double width, double eta, double* out) {
-  const Mesh M{nx, ny, n, E, K, nullptr, nullptr, perm};
+  // vertex Omega of every vertex in its tracked band order, once
+  const size_t nk = nx * ny;
+  std::vector<double> om(nk * n);
+  {
+    const Mesh P{nx, ny, n, E, Ki, ks, nullptr, nullptr, perm, nullptr};
+    const size_t nt = host_threads(nk);
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; ++t)
+      th.emplace_back([&, t] {
+        for (size_t v = nk * t / nt; v < nk * (t + 1) / nt; ++v) vertex_omega(P, v, eta, om.data() + v * n);
+      });
+    for (auto& x : th) x.join();
+  }
+  const Mesh M{nx, ny, n, E, Ki, ks, nullptr, nullptr, perm, om.data()};
   if (width == 0.0) {
     cut_t0(M, mu, n_mu, eta, out);
     return;
@@ -412,9 +436,9 @@ int rtb_energy_cut_host(size_t nx, size_t ny, int nsta, const double* E, const d
   for (size_t i = 0; i + 1 < n_mu; ++i)  // validate_sorted (config.rs:291-303)
     if (mu[i] > mu[i + 1]) return fail(RTB_ERR_INVALID_RESPONSE, "mu: must be sorted in ascending order");
   std::vector<int> perm;
-  energy_cut_track(nx, ny, nsta, ovx, ovy, perm);
+  energy_cut_track(nx, ny, nsta, ovx, ovy, false, perm);
   const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
-  energy_cut_sigma_2d(nx, ny, nsta, E, K, perm.data(), mu, n_mu, width, eta, sigma);
+  energy_cut_sigma_2d(nx, ny, nsta, E, K + 1, 2, perm.data(), mu, n_mu, width, eta, sigma);
   for (size_t i = 0; i < n_mu; ++i) sigma[i] /= det;  // linear/mod.rs:267
   return RTB_OK;
 }

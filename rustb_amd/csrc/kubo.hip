@@ -291,10 +291,11 @@ int dipole_combine(hipStream_t st, int n, size_t nk, const double* A2, const dou
 }
 
 namespace {
-// K[b][i][j] = A[b][i][j] B[b][j][i] (compute_velocity_kernel,
-// response/primitives.rs:106-111): one thread per element.
+// Im K[b][i][j] = Im(A[b][i][j] B[b][j][i]) (compute_velocity_kernel,
+// response/primitives.rs:106-111; only Im K enters Omega): one thread per
+// element.
 __global__ void cut_vertex_kernel(size_t total, int n, const double2* __restrict__ A, const double2* __restrict__ B,
-                                  double2* __restrict__ K) {
+                                  double* __restrict__ Ki) {
   const size_t g = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= total) return;
   const size_t n2 = static_cast<size_t>(n) * n;
@@ -302,16 +303,31 @@ __global__ void cut_vertex_kernel(size_t total, int n, const double2* __restrict
   const size_t i = r / n, j = r % n;
   const double2 a = A[g];
   const double2 c = B[b * n2 + j * n + i];
-  K[g] = make_double2(a.x * c.x - a.y * c.y, a.x * c.y + a.y * c.x);
+  Ki[g] = a.x * c.y + a.y * c.x;
+}
+
+// x[g] = |z[g]|^2 (complex z, real x; may not alias).
+__global__ void abs2_kernel(size_t total, const double2* __restrict__ z, double* __restrict__ x) {
+  const size_t g = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  const double2 v = z[g];
+  x[g] = v.x * v.x + v.y * v.y;
 }
 }  // namespace
 
-int energy_cut_vertex(hipStream_t st, int n, size_t nk, const double* A, const double* B, double* K) {
+int energy_cut_vertex(hipStream_t st, int n, size_t nk, const double* A, const double* B, double* Ki) {
   const size_t total = nk * n * n;
   if (total == 0) return RTB_OK;
   hipLaunchKernelGGL(cut_vertex_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, total, n,
-                     reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B),
-                     reinterpret_cast<double2*>(K));
+                     reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B), Ki);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int abs2(hipStream_t st, size_t total, const double* z, double* x) {
+  if (total == 0) return RTB_OK;
+  hipLaunchKernelGGL(abs2_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, total,
+                     reinterpret_cast<const double2*>(z), x);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -122,10 +122,12 @@ inline double mflop_zgemm(int n, size_t nb, bool herm) {
 // EnergyCut Hall conductivity, host part (energy_cut.cpp).  Band tracking:
 // perm [nx ny][n] (band b of vertex i is raw band perm[i][b]) from the raw
 // overlaps ovx [(nx-1) ny][n][n] = U_i^H U_{i+ny}, ovy [nx ny - 1][n][n] =
-// U_i^H U_{i+1} (complex).  The 2D Fermi cut of E [nk][n] and K [nk][n][n]
-// (complex) for sorted mu, fractional-coordinate measure, thermal width in eV.
-void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, std::vector<int>& perm);
-void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* K, const int* perm,
+// U_i^H U_{i+1} (complex, or their squared moduli when squared).  The 2D
+// Fermi cut of E [nk][n] and Im K [nk][n][n] (Ki, element stride ks) for
+// sorted mu, fractional-coordinate measure, thermal width in eV.
+void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, bool squared,
+                      std::vector<int>& perm);
+void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* Ki, size_t ks, const int* perm,
                          const double* mu, size_t n_mu, double width, double eta, double* out);
 
 // Largest nsta the batched eigensolver handles (packed lower triangle of the
