@@ -1183,16 +1183,25 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
     }
   }
   std::vector<double> hE(nk * n), hK(nk * n2), hOx(nxp * n2), hOy(nyp * n2);
-  RTB_HIP(hipMemcpyAsync(hE.data(), dEv, hE.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  RTB_HIP(hipMemcpyAsync(hK.data(), dKv, hK.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  // the overlaps first: tracking (a sequential breadth-first walk on the
+  // host) runs while Im K is still being copied
   if (!hOx.empty()) RTB_HIP(hipMemcpyAsync(hOx.data(), dOx, hOx.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   if (!hOy.empty()) RTB_HIP(hipMemcpyAsync(hOy.data(), dOy, hOy.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  RTB_HIP(hipStreamSynchronize(st));
-  RTB_TRY(tmr.collect(nk, n));
-  // band tracking and the Fermi cut on the host (global_band_track is a
-  // sequential breadth-first walk)
+  struct Ev {
+    hipEvent_t e = nullptr;
+    ~Ev() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } ov_done;
+  RTB_HIP(hipEventCreateWithFlags(&ov_done.e, hipEventDisableTiming));
+  RTB_HIP(hipEventRecord(ov_done.e, st));
+  RTB_HIP(hipMemcpyAsync(hE.data(), dEv, hE.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipMemcpyAsync(hK.data(), dKv, hK.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipEventSynchronize(ov_done.e));
   std::vector<int> perm;
   energy_cut_track(nx, ny, n, hOx.data(), hOy.data(), true, perm);
+  RTB_HIP(hipStreamSynchronize(st));
+  RTB_TRY(tmr.collect(nk, n));
   const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
   energy_cut_sigma_2d(nx, ny, n, hE.data(), hK.data(), 1, perm.data(), mu, n_mu, width, eta, sigma);
   for (size_t i = 0; i < n_mu; ++i) sigma[i] /= m->det_lat;  // linear/mod.rs:267
