@@ -302,3 +302,36 @@ def test_gpu_energy_cut_errors():
     with pytest.raises(ra.TbError) as e:
         cube.hall_conductivity(p)
     assert e.value.code == -14
+
+
+RAGGED = [(1, 5), (5, 1), (1, 1), (2, 2), (3, 7)]
+
+
+@pytest.mark.parametrize("mesh", RAGGED)
+def test_host_part_ragged_meshes(mesh):
+    """Meshes with one row, one column or one point: cells whose corners
+    coincide give zero-area triangles, which the cut skips as the reference
+    does; tracking walks whatever neighbours exist."""
+    m = haldane_broken(O.Model)
+    nx, ny = mesh
+    mu = np.linspace(-3, 3, 9)
+    pts = EC.vertices(m, mesh, np.eye(2))
+    ref = EC.hall_conductivity_energy_cut(m, mesh, np.eye(2), mu, 0.0, 0.05)
+    E, K, ovx, ovy = vertex_arrays(pts, nx, ny)
+    # empty overlap arrays still need a valid pointer
+    ovx = ovx if ovx.size else np.zeros((1, 2, 2), np.complex128)
+    ovy = ovy if ovy.size else np.zeros((1, 2, 2), np.complex128)
+    rc, out = host_sigma(E, K, ovx, ovy, nx, ny, 2, mu, 0.0, 0.05, float(np.linalg.det(m.lat)))
+    assert rc == 0
+    np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mesh", RAGGED)
+def test_gpu_ragged_meshes_vs_oracle(mesh):
+    m, o = lib_pair(haldane_broken)
+    mu = np.linspace(-3, 3, 9)
+    p = ra.Parameters.rank2(mesh, [1.0, 0.0], [0.0, 1.0], mu, eta=0.05, integration="EnergyCut")
+    got = m.hall_conductivity(p).conductivity
+    ref = EC.hall_conductivity_energy_cut(o, mesh, np.eye(2), mu, 0.0, 0.05)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * max(np.abs(ref).max(), 1e-3))
