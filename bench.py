@@ -102,6 +102,24 @@ def copy_probe(nbytes=1 << 30, reps=3):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
 
 
+def host_info():
+    """CPU model and core counts of the host the CPU baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count() or 1, "usable_cpus": usable}
+
+
 def cpu_baseline(model, kmesh, dirs, eta, target_s):
     from oracle import cpu_ref
 
@@ -125,6 +143,7 @@ def cpu_baseline(model, kmesh, dirs, eta, target_s):
         "unit": "k-points/s",
         "cores": threads,
         "kind": "port",
+        **host_info(),
         "sample": f"{done} seeded random k-points of the same {'x'.join(map(str, kmesh))} mesh "
                   f"(H + dH/dk zaxpy sum, zheev 'V', 4 zgemm, Kubo loop) in {t_used:.1f} s on {threads} threads, "
                   f"LAPACK {cpu_ref.lapack_name()}",
@@ -229,7 +248,7 @@ def ingest_bench(args):
         t0 = time.perf_counter()
         OW.from_hr(d, "sample", 0.0)
         tc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(ss / tc / 1e6, 2), "unit": "MB/s", "cores": 1, "kind": "port",
+        out["cpu_baseline"] = {"value": round(ss / tc / 1e6, 2), "unit": "MB/s", "cores": 1, "kind": "port", **host_info(),
                                "sample": f"oracle/wannier90_ref.py on the |R_i|<=1 cut ({ss} bytes, {tc:.1f} s)"}
     else:
         out["cpu_baseline"] = None
@@ -289,7 +308,7 @@ def energy_cut_bench(args):
         t0 = time.perf_counter()
         EC.hall_conductivity_energy_cut(o, (8, 8), np.eye(2), mu, 0.0, eta)
         tc = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(64 / tc, 2), "unit": "k-points/s", "cores": 1, "kind": "port",
+        out["cpu_baseline"] = {"value": round(64 / tc, 2), "unit": "k-points/s", "cores": 1, "kind": "port", **host_info(),
                                "sample": f"oracle/energy_cut_ref.py on an 8 x 8 mesh of the same model ({tc:.1f} s)"}
     else:
         out["cpu_baseline"] = None
@@ -353,13 +372,18 @@ def main():
     stage_mflop = np.zeros(ns)
     t0 = time.perf_counter()
     sigma = None
+    step_s = []
+    ws_gb = []
     for _ in range(args.steps):
-        sigma = model.hall_conductivity(params).conductivity[0]
+        ts = time.perf_counter()
+        sigma = model.hall_conductivity(params).conductivity[0]  # synchronous call
+        step_s.append(time.perf_counter() - ts)
         st = _lib.last_stats()
         stage_ms += st["ms"]
         stage_launch += st["launches"]
         stage_flop += st["flop"]
         stage_mflop += st["mflop"]
+        ws_gb.append(_lib.workspace_bytes()[0] / 1e9)
     device_sync()
     elapsed = time.perf_counter() - t0
     value = nk * args.steps / elapsed
@@ -416,6 +440,9 @@ def main():
         "config": {"workload": "wannier3d-n128", "nsta": n, "n_R": nR, "kmesh": list(kmesh), "n_k": nk,
                    "precision": args.precision, "eta": eta, "mu": [0.0], "T": 0.0, "outputs": "E, Omega_z, AHC"},
         "stage_ms_per_step": {names[i]: round(stage_ms[i] / args.steps, 2) for i in range(ns)},
+        "step_ms": {"min": round(min(step_s) * 1e3, 2), "median": round(float(np.median(step_s)) * 1e3, 2),
+                    "max": round(max(step_s) * 1e3, 2)},
+        "workspace_gb_after_each_step": [round(x, 2) for x in ws_gb],
         "sigma_xy_mu0": float(sigma),
         "roofline": roofline,
     }

@@ -9,10 +9,18 @@
  *   - Complex numbers are interleaved (re, im) doubles; matrices are C-order.
  *   - k-points are fractional reciprocal coordinates, one row of `dim` doubles.
  *   - n = nsta = norb (spinless) or 2*norb (spinful, [up block, down block]).
- *   - Thread safety: concurrent calls on one rtb_model are allowed (each call
- *     uses its own HIP stream and device work buffers).
- *   - No call allocates on the host beyond small bookkeeping; device memory is
- *     stream-ordered (hipMallocAsync) and released before the call returns.
+ *   - Thread safety: concurrent calls on one rtb_model are allowed.  Each
+ *     call leases a workspace slot of its device (a HIP stream and one device
+ *     block); at most RTB_MAX_CONCURRENT_CALLS (environment, default 4) calls
+ *     run on one device at once and further callers wait for a slot.
+ *   - Device memory: a slot's block grows to the largest workspace a call on
+ *     it has needed (for a k-mesh call, a chunk budget of min(device/8,
+ *     32 GiB) or RTB_WORKSPACE_MB, plus the mesh intermediates) and is kept
+ *     for later calls until rtb_workspace_release() or process exit, so that
+ *     repeated calls allocate nothing.  Concurrent large calls each hold
+ *     their own block: an allocation the device cannot satisfy is
+ *     RTB_ERR_DEVICE.  Host allocation is per call and proportional to the
+ *     outputs it stages (EnergyCut: its vertex data).
  */
 #ifndef RTB_HIP_H
 #define RTB_HIP_H
@@ -257,6 +265,15 @@ int rtb_nonlinear_hall_direct(const rtb_model* m, const size_t* kmesh, const dou
 /* Model::dos (model_physics.rs:179-226): E[En] = linspace(Emin, Emax, En), rho[En]. */
 int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, size_t En,
             double sigma, int precision, double* E, double* rho);
+
+/* Frees the workspace blocks and streams of the idle slots of the current
+ * device (slots in use by running calls are kept).  No counterpart in the
+ * reference (whose calls hold only host memory): for hosts that want the
+ * device memory back between bursts of calls. */
+int rtb_workspace_release(void);
+/* Device bytes held by the workspace slots of the current device and their
+ * number (slots may be null).  No counterpart in the reference. */
+int rtb_workspace_bytes(size_t* bytes, int* slots);
 
 /* Thread-local message of the last failure on this thread ("" if none). */
 const char* rtb_last_error(void);

@@ -117,6 +117,8 @@ SIGNATURES = {
     "rtb_w90_model": (c_int, [c_void_p, POINTER(c_void_p)]),
     "rtb_last_error": (c_char_p, []),
     "rtb_last_stats": (c_int, [POINTER(Stats)]),
+    "rtb_workspace_release": (c_int, []),
+    "rtb_workspace_bytes": (c_int, [POINTER(c_size_t), POINTER(c_int)]),
 }
 
 _lib = None
@@ -152,6 +154,14 @@ def last_stats() -> dict:
     check(load().rtb_last_stats(ctypes.byref(s)))
     return {"ms": list(s.ms), "launches": list(s.launches), "flop": list(s.flop), "nk": s.nk, "nsta": s.nsta,
             "mflop": list(s.mflop)}
+
+
+def workspace_bytes() -> tuple:
+    """Device bytes held by the workspace slots of the current device, and
+    their number."""
+    b, n = c_size_t(), c_int()
+    check(load().rtb_workspace_bytes(ctypes.byref(b), ctypes.byref(n)))
+    return b.value, n.value
 
 
 STAGES = ["fourier", "tridiag", "trieig", "backtransform", "gemm64", "gemm32", "refine", "kubo", "reduce", "total"]

@@ -13,7 +13,7 @@ LIB = os.path.join(HERE, "librtb_hip.so")
 PROBE_LIB = os.path.join(HERE, "librtb_probe.so")  # matrix-core ceiling probe used by bench.py
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["builder.cpp", "capi.cpp", "wannier90.cpp", "energy_cut.cpp", "fourier.hip", "fourier_mesh.hip", "eigh.hip", "refine.hip", "kubo.hip", "zgemm.hip"]
+SOURCES = ["builder.cpp", "capi.cpp", "workspace.cpp", "wannier90.cpp", "energy_cut.cpp", "fourier.hip", "fourier_mesh.hip", "eigh.hip", "refine.hip", "kubo.hip", "zgemm.hip"]
 
 COMMON = [
     "-O3",

@@ -18,75 +18,6 @@ namespace {
 thread_local std::string g_error;
 thread_local rtb_stats g_stats;
 
-// Stream-ordered device allocations released at scope exit.
-class Work {
- public:
-  explicit Work(hipStream_t s) : st_(s) {}
-  ~Work() {
-    for (void* p : ptrs_) (void)hipFreeAsync(p, st_);
-  }
-  template <typename T>
-  int alloc(T** out, size_t count) {
-    *out = nullptr;
-    if (count == 0) count = 1;
-    void* p = nullptr;
-    hipError_t e = hipMallocAsync(&p, count * sizeof(T), st_);
-    if (e != hipSuccess) {
-      set_error("device allocation of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
-      return RTB_ERR_DEVICE;
-    }
-    ptrs_.push_back(p);
-    *out = static_cast<T*>(p);
-    return RTB_OK;
-  }
-
- private:
-  hipStream_t st_;
-  std::vector<void*> ptrs_;
-};
-
-// The workspace of a call (tens of GB for a large mesh) comes from the
-// device's default stream-ordered pool.  By default the pool returns freed
-// memory to the driver at every synchronisation, and mapping it again costs
-// more than a call's computation; the first call on a device therefore lets
-// the pool keep what it has (the process holds its peak workspace until it
-// exits).
-void retain_pool_memory() {
-  static std::mutex mu;
-  static bool done[64] = {};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
-  std::lock_guard<std::mutex> lock(mu);
-  if (done[dev]) return;
-  hipMemPool_t pool;
-  if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-    uint64_t keep = UINT64_MAX;
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-  }
-  done[dev] = true;
-}
-
-// A non-blocking stream owned by one call (default priority).
-class Stream {
- public:
-  Stream() {
-    retain_pool_memory();
-    ok_ = hipStreamCreateWithFlags(&s_, hipStreamNonBlocking) == hipSuccess;
-  }
-  ~Stream() {
-    if (ok_) {
-      (void)hipStreamSynchronize(s_);
-      (void)hipStreamDestroy(s_);
-    }
-  }
-  bool ok() const { return ok_; }
-  hipStream_t get() const { return s_; }
-
- private:
-  hipStream_t s_{};
-  bool ok_ = false;
-};
-
 std::once_flag g_smem_once;
 int g_smem_rc = RTB_OK;
 
@@ -461,6 +392,24 @@ bool storage_is_hermitian(int dim, int n, int n_R, const int64_t* hamR, const do
   return ok.load();
 }
 
+// Every value of a large array finite (a model's hoppings), spread over
+// host threads.
+bool all_finite(const double* x, size_t count) {
+  const int T = static_cast<int>(std::max<size_t>(1, std::min<size_t>({16, count / (size_t(1) << 20) + 1,
+                                                                          std::thread::hardware_concurrency()})));
+  std::atomic<bool> ok{true};
+  std::vector<std::thread> th;
+  for (int t = 0; t < T; ++t)
+    th.emplace_back([&, t] {
+      const size_t a = count / T * t, e = t == T - 1 ? count : count / T * (t + 1);
+      bool good = true;
+      for (size_t i = a; i < e; ++i) good = good && std::isfinite(x[i]);
+      if (!good) ok.store(false, std::memory_order_relaxed);
+    });
+  for (auto& x : th) x.join();
+  return ok.load();
+}
+
 void set_error(const char* fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -499,6 +448,10 @@ int rtb_model_create(int dim, int spin, int norb, int n_R, const int64_t* hamR, 
     if (!std::isfinite(orb[i])) return fail(RTB_ERR_INVALID_MODEL, "orbital positions must be finite");
   const int n = spin ? 2 * norb : norb;
   const size_t n2 = static_cast<size_t>(n) * n;
+  // a NaN or infinite hopping would come back as eigenvalues that are not
+  // numbers (the reference's zheev reports such input as an error)
+  if (!all_finite(ham, 2 * n2 * static_cast<size_t>(n_R)))
+    return fail(RTB_ERR_INVALID_MODEL, "hoppings must be finite");
   auto* m = new rtb_model();
   m->dim = dim;
   m->spin = spin ? 1 : 0;
@@ -589,11 +542,11 @@ int rtb_gen_ham_batch(const rtb_model* m, const double* k, size_t nk, int gauge,
   if (gauge != RTB_GAUGE_ATOM && gauge != RTB_GAUGE_LATTICE) return fail(RTB_ERR_INVALID_ARGUMENT, "bad gauge");
   if (n_proj < 0 || n_proj > 3) return fail(RTB_ERR_UNSUPPORTED, "n_proj must be 0..3");
   if (nk == 0) return RTB_OK;
-  Stream s;
-  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
-  hipStream_t st = s.get();
-  Work w(st);
-  Timer tmr(st);
+  Lease lease;
+  RTB_TRY(lease.status());
+  hipStream_t st = lease.stream();
+  Work w(lease);
+  Timer tmr(lease);
   FourierArgs a;
   RTB_TRY(upload_dirs(st, w, m, dirs, n_proj, a));
   a.gauge = gauge;
@@ -647,11 +600,11 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
   const int n = m->nsta;
   if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
   const bool vec = evec != nullptr;
-  Stream s;
-  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
-  hipStream_t st = s.get();
-  Work w(st);
-  Timer tmr(st);
+  Lease lease;
+  RTB_TRY(lease.status());
+  hipStream_t st = lease.stream();
+  Work w(lease);
+  Timer tmr(lease);
   FourierArgs a;
   RTB_TRY(upload_dirs(st, w, m, nullptr, 0, a));
   a.gauge = RTB_GAUGE_ATOM;  // solve_band_onek / solve_onek use Gauge::Atom
@@ -829,11 +782,11 @@ static int mesh_setup(hipStream_t st, Work& w, Timer& tmr, const rtb_model* m, c
 static int berry_impl(const rtb_model* m, BerryJob& j) {
   const int n = m->nsta;
   if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
-  Stream s;
-  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
-  hipStream_t st = s.get();
-  Work w(st);
-  Timer tmr(st);
+  Lease lease;
+  RTB_TRY(lease.status());
+  hipStream_t st = lease.stream();
+  Work w(lease);
+  Timer tmr(lease);
   FourierArgs a;
   RTB_TRY(upload_dirs(st, w, m, j.dir, j.dipole ? 3 : 2, a));
   a.gauge = RTB_GAUGE_ATOM;  // berry_curvature_at_impl: gen_v_projected(k, Gauge::Atom, ..)
@@ -1096,11 +1049,11 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
   const size_t nx = kmesh[0], ny = kmesh[1];
   const size_t n2 = static_cast<size_t>(n) * n;
-  Stream s;
-  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
-  hipStream_t st = s.get();
-  Work w(st);
-  Timer tmr(st);
+  Lease lease;
+  RTB_TRY(lease.status());
+  hipStream_t st = lease.stream();
+  Work w(lease);
+  Timer tmr(lease);
   FourierArgs a;
   RTB_TRY(upload_dirs(st, w, m, dir, 2, a));
   a.gauge = RTB_GAUGE_ATOM;  // compute_velocity_kernel(.., Gauge::Atom, ..) (linear/mod.rs:235-242)
@@ -1312,11 +1265,11 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
   const double step = En > 1 ? (Emax - Emin) / static_cast<double>(En - 1) : 0.0;
   for (size_t i = 0; i < En; ++i) E[i] = Emin + step * static_cast<double>(i);
-  Stream s;
-  if (!s.ok()) return fail(RTB_ERR_DEVICE, "stream creation failed");
-  hipStream_t st = s.get();
-  Work w(st);
-  Timer tmr(st);
+  Lease lease;
+  RTB_TRY(lease.status());
+  hipStream_t st = lease.stream();
+  Work w(lease);
+  Timer tmr(lease);
   FourierArgs a;
   RTB_TRY(upload_dirs(st, w, m, nullptr, 0, a));
   a.gauge = RTB_GAUGE_ATOM;

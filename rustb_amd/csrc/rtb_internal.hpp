@@ -38,30 +38,78 @@ inline int fail(int code, const char* msg) {
     }                        \
   } while (0)
 
+// ---- call workspaces (workspace.cpp) ----
+// A slot is a non-blocking HIP stream (default priority) plus one device
+// block that grows to the largest workspace a call on it has needed; calls
+// lease a slot for their duration, so that steady-state calls allocate no
+// device memory.  At most max_concurrent_calls() calls (RTB_MAX_CONCURRENT_CALLS,
+// default 4) hold a slot on one device at once; further callers wait.
+struct Slot;
+int max_concurrent_calls();
+class Lease {
+ public:
+  Lease();   // takes a slot on the current device (waits while all are busy)
+  ~Lease();  // synchronises the slot's stream, then returns the slot
+  Lease(const Lease&) = delete;
+  Lease& operator=(const Lease&) = delete;
+  int status() const { return rc_; }
+  hipStream_t stream() const;
+  Slot* slot() const { return s_; }
+  // the slot's i-th timing event (created on first use, then reused)
+  int event(size_t i, hipEvent_t* out);
+
+ private:
+  Slot* s_ = nullptr;
+  int rc_ = RTB_OK;
+};
+
+// A call's device buffers, carved with a bump pointer (256-byte aligned)
+// from a leased slot's block (which grows when a call needs more) or from a
+// fixed buffer (no growth: a request beyond it is RTB_ERR_UNSUPPORTED).
+class Work {
+ public:
+  explicit Work(Lease& l);
+  Work(void* base, size_t cap);
+  Work(const Work&) = delete;
+  Work& operator=(const Work&) = delete;
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    void* p = nullptr;
+    const int rc = take(count * sizeof(T), &p);
+    *out = static_cast<T*>(p);
+    return rc;
+  }
+  size_t used() const { return used_; }
+
+ private:
+  int take(size_t bytes, void** out);
+  Slot* slot_ = nullptr;
+  char* base_ = nullptr;
+  size_t cap_ = 0, off_ = 0, used_ = 0;
+};
+
 // Event-pair timing of kernel launches on one stream, by stage (see
-// rtb_stats).  collect() runs after the stream is synchronised and publishes
-// the sums as the calling thread's rtb_last_stats().
+// rtb_stats), with the leased slot's reusable events; a Timer made without
+// a lease times nothing.  collect() runs after the stream is synchronised
+// and publishes the sums as the calling thread's rtb_last_stats().
 void publish_stats(const rtb_stats& s);
 class Timer {
  public:
-  explicit Timer(hipStream_t s) : st_(s) {}
+  explicit Timer(Lease& l) : lease_(&l), st_(l.stream()) {}
+  Timer() = default;
   Timer(const Timer&) = delete;
   Timer& operator=(const Timer&) = delete;
-  ~Timer() {
-    for (auto& r : recs_) {
-      (void)hipEventDestroy(r.a);
-      (void)hipEventDestroy(r.b);
-    }
-  }
   int begin(int stage) {
+    if (!lease_) return RTB_OK;
     Rec r{stage, nullptr, nullptr, 0, 0.0, 0.0};
-    RTB_HIP(hipEventCreate(&r.a));
-    RTB_HIP(hipEventCreate(&r.b));
+    RTB_TRY(lease_->event(2 * recs_.size(), &r.a));
+    RTB_TRY(lease_->event(2 * recs_.size() + 1, &r.b));
     RTB_HIP(hipEventRecord(r.a, st_));
     recs_.push_back(r);
     return RTB_OK;
   }
   int end(int launches, double flop = 0.0, double mflop = 0.0) {
+    if (!lease_) return RTB_OK;
     recs_.back().launches = launches;
     recs_.back().flop = flop;
     recs_.back().mflop = mflop;
@@ -101,7 +149,8 @@ class Timer {
     int launches;
     double flop, mflop;
   };
-  hipStream_t st_;
+  Lease* lease_ = nullptr;
+  hipStream_t st_ = nullptr;
   std::vector<Rec> recs_;
 };
 
