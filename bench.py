@@ -443,6 +443,8 @@ def main():
         "step_ms": {"min": round(min(step_s) * 1e3, 2), "median": round(float(np.median(step_s)) * 1e3, 2),
                     "max": round(max(step_s) * 1e3, 2)},
         "workspace_gb_after_each_step": [round(x, 2) for x in ws_gb],
+        "eigenpair_check": {"max_residual_eV": st["max_residual"], "resolved_in_fp64": int(st["resolved"]),
+                            "of_k": nk},
         "sigma_xy_mu0": float(sigma),
         "roofline": roofline,
     }

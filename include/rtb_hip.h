@@ -69,7 +69,15 @@ extern "C" {
 #define RTB_SPIN_Z 3
 
 #define RTB_PREC_FP64 0  /* plain form: the reference's formulas in FP64 */
-#define RTB_PREC_MIXED 1 /* mixed form: FP32 eigensolve + FP64 refinement */
+/* mixed form: FP32 eigensolve + FP64 refinement.  Every refined eigenpair is
+ * checked: a k-point with an eigenvalue that is not finite or a residual
+ * |H u - lambda u| of a unit eigenvector above RTB_MIXED_RESIDUAL (environment,
+ * eV per max(1, max |lambda|), default 1e-9) is re-solved by the FP64
+ * eigensolver; with RTB_MIXED_FALLBACK=0 it fails the call with
+ * RTB_ERR_EIGEN_FAILED instead.  Any call whose final eigenvalues are not all
+ * finite returns RTB_ERR_EIGEN_FAILED (the reference's zheev info != 0,
+ * ndarray_lapack.rs:443-447, and TbError from eigh, response/traits.rs:168). */
+#define RTB_PREC_MIXED 1
 
 /* =====================================================================
  * Model builder (host only, no GPU).  Mirrors the reference's hopping API
@@ -316,6 +324,11 @@ typedef struct rtb_stats {
   size_t nk;
   int nsta;
   double mflop[RTB_N_STAGES];
+  /* mixed form: the largest residual |H u - lambda u| (eV) of a unit refined
+   * eigenvector over the call, and the number of k-points whose refined
+   * eigenpairs failed the check and were re-solved in FP64 */
+  double max_residual;
+  size_t resolved;
 } rtb_stats;
 int rtb_last_stats(rtb_stats* out);
 

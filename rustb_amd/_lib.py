@@ -67,7 +67,7 @@ class SynthSpec(ctypes.Structure):
 
 class Stats(ctypes.Structure):
     _fields_ = [("ms", c_double * 10), ("launches", c_int * 10), ("flop", c_double * 10), ("nk", c_size_t),
-                ("nsta", c_int), ("mflop", c_double * 10)]
+                ("nsta", c_int), ("mflop", c_double * 10), ("max_residual", c_double), ("resolved", c_size_t)]
 
 
 _P = c_void_p
@@ -153,7 +153,7 @@ def last_stats() -> dict:
     s = Stats()
     check(load().rtb_last_stats(ctypes.byref(s)))
     return {"ms": list(s.ms), "launches": list(s.launches), "flop": list(s.flop), "nk": s.nk, "nsta": s.nsta,
-            "mflop": list(s.mflop)}
+            "mflop": list(s.mflop), "max_residual": s.max_residual, "resolved": s.resolved}
 
 
 def workspace_bytes() -> tuple:

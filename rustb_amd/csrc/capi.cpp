@@ -183,6 +183,138 @@ int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
   return RTB_OK;
 }
 
+// ---- eigenpair checks ----
+// Options of the mixed form's check, read from the environment per call:
+//   RTB_MIXED_RESIDUAL  the accepted residual |H u - lambda u| of a unit
+//                       refined eigenvector, in eV per max(1, max |lambda|)
+//                       (default 1e-9);
+//   RTB_MIXED_FALLBACK  0: a flagged k-point fails the call with
+//                       RTB_ERR_EIGEN_FAILED instead of being re-solved in FP64.
+struct MixedOpts {
+  double res_tol = 1e-9;
+  bool fallback = true;
+};
+
+MixedOpts mixed_opts() {
+  MixedOpts o;
+  if (const char* e = std::getenv("RTB_MIXED_RESIDUAL")) {
+    char* end = nullptr;
+    const double v = std::strtod(e, &end);
+    if (end != e && v >= 0.0) o.res_tol = v;
+  }
+  if (const char* e = std::getenv("RTB_MIXED_FALLBACK")) o.fallback = std::atoi(e) != 0;
+  return o;
+}
+
+// Device counters of a call: the k-points the mixed check flagged in the
+// current chunk (count, local indices), the k-points whose FP64 eigenvalues
+// are not finite, the largest refined residual (bits of a double); and the
+// buffers of the FP64 re-solve, taken from the workspace on first use.
+struct Checks {
+  int* flag_count = nullptr;
+  int* flag_list = nullptr;
+  int* nonfinite = nullptr;
+  unsigned long long* max_res = nullptr;
+  size_t resolved = 0;
+  MixedOpts opt;
+  size_t cap = 0;
+  double *dk = nullptr, *H = nullptr, *E = nullptr, *Z = nullptr;
+  EigBufs eb{};
+  RefineCheck refine() const {
+    RefineCheck r;
+    r.res_tol = opt.res_tol;
+    r.fail_count = flag_count;
+    r.fail_list = flag_list;
+    r.max_res = max_res;
+    return r;
+  }
+};
+
+int checks_init(hipStream_t st, Work& w, size_t K, Checks& ck) {
+  ck.opt = mixed_opts();
+  int* ints;
+  RTB_TRY(w.alloc(&ints, 4 + K));  // flag count, non-finite count, max residual (8 bytes), list
+  ck.flag_count = ints;
+  ck.nonfinite = ints + 1;
+  ck.max_res = reinterpret_cast<unsigned long long*>(ints + 2);
+  ck.flag_list = ints + 4;
+  return zero_ints(st, ints, 4);
+}
+
+// The k-points of a chunk the mixed check flagged, re-solved in FP64: H by
+// the dense Fourier sum at kof(local index), the FP64 eigensolver, and the
+// results written over the refined ones (E, unit columns into Zc, scale 1).
+// With the fallback switched off a flagged k-point fails the call.
+template <typename KOf>
+int resolve_flagged(hipStream_t st, Work& w, Timer& tmr, const rtb_model* m, const FourierArgs& a, bool from_upper,
+                    size_t c, KOf kof, Checks& ck, double* E, double* Zc, double* scale) {
+  int cnt = 0;
+  RTB_HIP(hipMemcpyAsync(&cnt, ck.flag_count, sizeof(int), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipStreamSynchronize(st));
+  if (cnt == 0) return RTB_OK;
+  if (!ck.opt.fallback) {
+    set_error("EigenvalueComputationFailed: the refined eigenpairs of %d k-point(s) fail the residual check", cnt);
+    return RTB_ERR_EIGEN_FAILED;
+  }
+  std::vector<int> list(static_cast<size_t>(cnt));
+  RTB_HIP(hipMemcpyAsync(list.data(), ck.flag_list, list.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipStreamSynchronize(st));
+  const int n = m->nsta;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  if (ck.cap == 0) {
+    ck.cap = std::min<size_t>(c, 1024);
+    RTB_TRY(w.alloc(&ck.dk, ck.cap * 3));
+    RTB_TRY(w.alloc(&ck.H, ck.cap * n2 * 2));
+    RTB_TRY(w.alloc(&ck.E, ck.cap * n));
+    RTB_TRY(w.alloc(&ck.Z, ck.cap * n2 * 2));
+    RTB_TRY(alloc_eig(w, n, ck.cap, true, ck.eb));
+  }
+  FourierArgs a0 = a;
+  a0.np = 0;
+  a0.combine = 0;
+  std::vector<double> kp;
+  for (size_t b0 = 0; b0 < list.size(); b0 += ck.cap) {
+    const size_t bc = std::min(ck.cap, list.size() - b0);
+    kp.assign(bc * 3, 0.0);
+    for (size_t i = 0; i < bc; ++i) kof(static_cast<size_t>(list[b0 + i]), &kp[3 * i]);
+    RTB_HIP(hipMemcpyAsync(ck.dk, kp.data(), kp.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    RTB_TRY(tmr.begin(RTB_STAGE_FOURIER));
+    RTB_TRY(fourier_dense(st, a0, ck.dk, bc, ck.H, nullptr));
+    RTB_TRY(tmr.end(1));
+    RTB_TRY(eigh_launch_fp64(st, &tmr, n, bc, ck.H, ck.E, true, ck.eb.Vg, ck.eb.tau, ck.eb.Y, ck.eb.S, ck.eb.DE,
+                             from_upper));
+    RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
+    RTB_TRY(eigh_backtransform_fp64(st, n, bc, ck.eb.Y, ck.eb.DE, ck.eb.S, ck.eb.Vg, ck.eb.tau, ck.Z, false));
+    RTB_TRY(tmr.end(1, flop_backtrans(n, bc)));
+    RTB_TRY(tmr.begin(RTB_STAGE_REFINE));
+    RTB_TRY(count_nonfinite(st, n, bc, ck.E, ck.nonfinite));
+    RTB_TRY(scatter_resolved(st, n, static_cast<int>(bc), ck.flag_list + b0, ck.E, ck.Z, E, Zc, scale));
+    RTB_TRY(tmr.end(2));
+    RTB_HIP(hipStreamSynchronize(st));  // kp is refilled next
+  }
+  RTB_TRY(zero_ints(st, ck.flag_count, 1));
+  ck.resolved += list.size();
+  return RTB_OK;
+}
+
+// End of a call: the largest refined residual and the re-solved count into
+// the stats; RTB_ERR_EIGEN_FAILED when an eigenvalue is not finite.
+int checks_finish(hipStream_t st, Checks& ck, Timer& tmr) {
+  int nf = 0;
+  unsigned long long mr = 0;
+  RTB_HIP(hipMemcpyAsync(&nf, ck.nonfinite, sizeof(int), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipMemcpyAsync(&mr, ck.max_res, sizeof(mr), hipMemcpyDeviceToHost, st));
+  RTB_HIP(hipStreamSynchronize(st));
+  double r;
+  std::memcpy(&r, &mr, sizeof(r));
+  tmr.note_check(r, ck.resolved);
+  if (nf > 0) {
+    set_error("EigenvalueComputationFailed: the eigenvalues of %d k-point(s) are not finite", nf);
+    return RTB_ERR_EIGEN_FAILED;
+  }
+  return RTB_OK;
+}
+
 // Mixed-form eigenpairs of a chunk: FP32 Householder / bisection / inverse
 // iteration / back-transformation (the FP64 buffers of `eb` reused as FP32
 // storage) into Z, then FP64 refinement against the FP64 H.  The refined
@@ -191,7 +323,7 @@ int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
 // rebuilt from its upper triangle (models whose storage is not exactly
 // Hermitian), so the solve and the refinement see the reference's matrix.
 int eig_mixed_chunk(hipStream_t st, Timer& tmr, bool from_upper, int n, size_t c, double* H, double* E, EigBufs& eb,
-                    double* Z, double* W1, double* W2, double* W3, double* scale) {
+                    double* Z, double* W1, double* W2, double* W3, double* scale, const RefineCheck& chk) {
   float* Vg = reinterpret_cast<float*>(eb.Vg);
   float* tau = reinterpret_cast<float*>(eb.tau);
   float* Y = reinterpret_cast<float*>(eb.Y);
@@ -204,7 +336,7 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, bool from_upper, int n, size_t c
   RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
   // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
   // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius
-  RTB_TRY(refine_eigenpairs(st, &tmr, n, c, H, Zf, W1, W2, W3, E, scale, 1e-3));
+  RTB_TRY(refine_eigenpairs(st, &tmr, n, c, H, Zf, W1, W2, W3, E, scale, 1e-3, chk));
   return RTB_OK;
 }
 
@@ -626,6 +758,8 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     RTB_TRY(w.alloc(&dHZ, K * n2 * 2));
     RTB_TRY(w.alloc(&dsc, K * n));
   }
+  Checks ck;
+  RTB_TRY(checks_init(st, w, K, ck));
   std::vector<double> kp;
   for (size_t k0 = 0; k0 < nk; k0 += K) {
     const size_t c = std::min(K, nk - k0);
@@ -638,13 +772,19 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
     RTB_TRY(tmr.end(1));
     if (f32) {
       // W2, W3: the reflector and inverse-iteration scratch, dead by then
-      RTB_TRY(eig_mixed_chunk(st, tmr, !m->herm && !lower_vectors, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S, dsc));
+      const bool up = !m->herm && !lower_vectors;
+      RTB_TRY(eig_mixed_chunk(st, tmr, up, n, c, dH, dE, eb, dZc, dHZ, eb.Vg, eb.S, dsc, ck.refine()));
+      auto kof = [&](size_t i, double* k3) {
+        for (int d = 0; d < m->dim; ++d) k3[d] = k[(k0 + i) * m->dim + d];
+      };
+      RTB_TRY(resolve_flagged(st, w, tmr, m, a, up, c, kof, ck, dE, eb.Vg, dsc));
       RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
       RTB_TRY(transpose_batched(st, n, c, eb.Vg, dsc, dZ));
       RTB_TRY(tmr.end(1));
     } else {
       RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE,
                                vec && !m->herm && !lower_vectors));
+      RTB_TRY(count_nonfinite(st, n, c, dE, ck.nonfinite));
       if (vec) {
         RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
         RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, true));
@@ -656,6 +796,7 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
       RTB_HIP(hipMemcpyAsync(evec + k0 * n2 * 2, dZ, c * n2 * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
     RTB_HIP(hipStreamSynchronize(st));
   }
+  RTB_TRY(checks_finish(st, ck, tmr));
   RTB_TRY(tmr.collect(nk, n));
   return RTB_OK;
 }
@@ -856,6 +997,8 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     RTB_HIP(hipMemcpyAsync(dmu, j.mu, j.n_mu * sizeof(double), hipMemcpyHostToDevice, st));
   }
   if (j.mode == 1) RTB_TRY(w.alloc(&docc, K));
+  Checks ck;
+  RTB_TRY(checks_init(st, w, K, ck));
   std::vector<double> kp;
   int blk_off = 0;
   for (const Chunk& ch : chunks) {
@@ -888,11 +1031,34 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     double* Zc = dZ;  // eigenvectors (columns) for the Kubo products
     double* Ab = A;   // scratch of the Kubo products
     if (f32) {
-      RTB_TRY(eig_mixed_chunk(st, tmr, !m->herm, n, c, dH, dE, eb, dZ, T1, A, B, dsc));
+      RTB_TRY(eig_mixed_chunk(st, tmr, !m->herm, n, c, dH, dE, eb, dZ, T1, A, B, dsc, ck.refine()));
       Zc = A;  // refined vectors end in the A buffer; dZ is free
       Ab = dZ;
+      // the k-point of local index i, in the order the Fourier stage used
+      auto kof = [&](size_t i, double* k3) {
+        size_t x[3] = {0, 0, 0};
+        if (G) {  // k1 innermost within the chunk's (k2, k3) pairs
+          const size_t pair = ch.p0 + i / mp.N[0];
+          x[0] = i % mp.N[0];
+          x[1] = pair / mp.N[2];
+          x[2] = pair % mp.N[2];
+        } else if (j.k) {
+          for (int d = 0; d < m->dim; ++d) k3[d] = j.k[(k0 + i) * m->dim + d];
+          return;
+        } else {  // gen_kmesh order: first axis outermost
+          size_t f = k0 + i;
+          for (int d = m->dim - 1; d >= 0; --d) {
+            x[d] = f % j.kmesh[d];
+            f /= j.kmesh[d];
+          }
+        }
+        for (int d = 0; d < m->dim; ++d)
+          k3[d] = static_cast<double>(x[d]) / static_cast<double>(G ? mp.N[d] : static_cast<int>(j.kmesh[d]));
+      };
+      RTB_TRY(resolve_flagged(st, w, tmr, m, a, !m->herm, c, kof, ck, dE, Zc, dsc));
     } else {
       RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm));
+      RTB_TRY(count_nonfinite(st, n, c, dE, ck.nonfinite));
       RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
       RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, false));
       RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
@@ -944,6 +1110,7 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
     for (size_t i = 0; i < msums.size(); ++i) j.msum[i] = msums[i] / static_cast<double>(j.nk) / m->det_lat;
   }
   RTB_HIP(hipStreamSynchronize(st));
+  RTB_TRY(checks_finish(st, ck, tmr));
   RTB_TRY(tmr.collect(j.nk, n));
   return RTB_OK;
 }
@@ -1083,6 +1250,8 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   double* T1 = kub;
   double* A = kub + K * n2 * 2;
   double* B = kub + K * n2 * 4;
+  Checks ck;
+  RTB_TRY(checks_init(st, w, 0, ck));
   for (size_t k0 = 0; k0 < nk; k0 += K) {
     const size_t c = std::min(K, nk - k0);
     double* Zc = dU + k0 * n2 * 2;
@@ -1094,6 +1263,7 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
     // hamk.eigh(UPLO::Lower) (primitives.rs:87): the upper triangle of a
     // storage that is not exactly Hermitian
     RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dEv + k0 * n, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm));
+    RTB_TRY(count_nonfinite(st, n, c, dEv + k0 * n, ck.nonfinite));
     RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
     RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, Zc, false));
     RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
@@ -1154,6 +1324,7 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   std::vector<int> perm;
   energy_cut_track(nx, ny, n, hOx.data(), hOy.data(), true, perm);
   RTB_HIP(hipStreamSynchronize(st));
+  RTB_TRY(checks_finish(st, ck, tmr));
   RTB_TRY(tmr.collect(nk, n));
   const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
   energy_cut_sigma_2d(nx, ny, n, hE.data(), hK.data(), 1, perm.data(), mu, n_mu, width, eta, sigma);
@@ -1290,6 +1461,8 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   double *dpart, *drho;
   RTB_TRY(w.alloc(&dpart, static_cast<size_t>(total_blk) * En));
   RTB_TRY(w.alloc(&drho, En));
+  Checks ck;
+  RTB_TRY(checks_init(st, w, 0, ck));
   int blk_off = 0;
   for (const Chunk& ch : chunks) {
     const size_t k0 = ch.k0, c = ch.c;
@@ -1306,6 +1479,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
       RTB_TRY(tmr.end(1));
     }
     RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr, eb.DE, false));
+    RTB_TRY(count_nonfinite(st, n, c, dE, ck.nonfinite));
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     const int nb = dos_blocks(c * n);
     RTB_TRY(dos_partials(st, c * n, dE, Emin, step, static_cast<int>(En), sigma,
@@ -1319,6 +1493,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   RTB_HIP(hipMemcpyAsync(rho, drho, En * sizeof(double), hipMemcpyDeviceToHost, st));
   RTB_HIP(hipStreamSynchronize(st));
   for (size_t i = 0; i < En; ++i) rho[i] /= static_cast<double>(nk);  // model_physics.rs:224
+  RTB_TRY(checks_finish(st, ck, tmr));
   RTB_TRY(tmr.collect(nk, n));
   return RTB_OK;
 }

@@ -32,8 +32,32 @@ int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, 
 // H is overwritten; the refined columns are written to W2 (FP64), unit
 // after multiplication by scale[k][j] ([nk][n]: the squared column norms of
 // Z on entry, as eigh_backtransform_fp32 writes them); W1, W3 scratch.
+//
+// Check: per eigenpair the residual |H u - lambda u| of the unit refined
+// vector u (from the correction product's epilogue, recomputed after the
+// Rayleigh-Ritz rotation inside clusters); a k-point with an eigenvalue that
+// is not finite, a cluster the Rayleigh-Ritz step cannot handle, or a
+// residual above res_tol * max(1, max |lambda|) eV gets its local index
+// appended to fail_list (fail_count: an int the caller zeroes); max_res
+// (the bits of a non-negative double, zeroed by the caller) receives the
+// largest finite residual.
+struct RefineCheck {
+  double res_tol = 1e-9;
+  int* fail_count = nullptr;
+  int* fail_list = nullptr;
+  unsigned long long* max_res = nullptr;
+};
 int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, const float* Z, double* W1,
-                      double* W2, double* W3, double* E, double* scale, double gap_rel);
+                      double* W2, double* W3, double* E, double* scale, double gap_rel, const RefineCheck& chk);
+// Count of k-points ([nk][n] eigenvalues) with an eigenvalue that is not
+// finite, added to *count.
+int count_nonfinite(hipStream_t st, int n, size_t nk, const double* E, int* count);
+// For i < cnt: E[list[i]] = Ef[i], Z[list[i]] = Zf[i] ([n][n] complex
+// columns), scale[list[i]][:] = 1 (the FP64 re-solve of flagged k-points).
+int scatter_resolved(hipStream_t st, int n, int cnt, const int* list, const double* Ef, const double* Zf,
+                     double* E, double* Z, double* scale);
+// p[0..count) = 0 (a fill kernel: usable where hipMemsetAsync is not).
+int zero_ints(hipStream_t st, int* p, size_t count);
 // out[k][r][c] = Z[k][c][r] * scale[k][r] (eigenvector columns to rows;
 // scale may be null)
 int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, double* out);
@@ -100,9 +124,10 @@ int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B,
 // The Gram product G = Z^H Z (FP64, Z stored in FP32, stride n^2) of the
 // Ogita-Aishima step with G consumed in the epilogue: F (FP32, [nb][n][n])
 // receives the correction F' - I from G, S = Z^H H Z ([nb][n][n]) and the
-// exact diagonal gd ([nb][n]) of G (n <= 128).
+// exact diagonal gd ([nb][n]) of G (n <= 128); lt ([nb][n]) the Rayleigh
+// quotients S_ii / G_ii.
 int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double* S, const double* gd, float* F,
-                  double gap_rel);
+                  double gap_rel, double* lt);
 // The Kubo product C = diag(s) U^H T1 diag(s) (U, T1: [nb][n][n] FP64; s
 // may be null) consumed by its epilogue: with Q_ij = |C_ij|^2 /
 // ((E_i - E_j)^2 + eta^2), i != j, part ([nb][2][ceil(n/64)][n]) receives the
@@ -110,10 +135,11 @@ int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double
 int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* T1, const double* s,
                const double* E, double eta, double* part);
 // Both refinement corrections in one pass: Zp = Z + Z F (FP64 out) and the
-// Rayleigh sums of the Zp columns against HZ + HZ F (not stored), per 64-row
-// tile into part ([nb][ceil(n/64)][2][n]); products on the FP32 matrix cores.
+// Rayleigh sums of the Zp columns against HZ + HZ F (not stored) with the
+// squared residuals |HZ + HZ F - lt_j Zp_j|^2, per 64-row tile into part
+// ([nb][ceil(n/64)][3][n]); products on the FP32 matrix cores.
 int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, const double* HZ, const float* F,
-                          double* Zp, double* part);
+                          const double* lt, double* Zp, double* part);
 // C[b] = op(A[b]) * B[b], n x n complex; conj_a: op = conjugate transpose.
 // sA, sB, sC: batch strides in complex elements.  herm: the product is known
 // to be Hermitian; the upper triangle is filled from the lower one.

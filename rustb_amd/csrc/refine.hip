@@ -25,38 +25,44 @@ __device__ inline double wave_sum(double x) {
   return x;
 }
 
-// One wave per k-point: first the Rayleigh quotients and column scales from
-// the per-tile sums; then the lanes test every eigenvalue gap in parallel; a
-// k-point with a cluster forms the cluster's Gram and projection sums with
-// the lanes over rows, lane 0 solves the c x c problem (LDS), and the lanes
-// rotate the cluster's columns row by row.  Column j of Z (and of HZ = H Z)
+// One wave per k-point: first the Rayleigh quotients, column scales and
+// residuals from the per-tile sums; then the lanes test every eigenvalue gap
+// in parallel; a k-point with a cluster forms the cluster's Gram and
+// projection sums with the lanes over rows, lane 0 solves the c x c problem
+// (LDS), and the lanes rotate the cluster's columns row by row, forming the
+// rotated columns' residuals on the way.  Column j of Z (and of HZ = H Z)
 // times scale[j] is the unit vector; rotated columns are written unit.
+// Last, the k-point's check (see RefineCheck).
 __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restrict__ Z,
                                                         const double2* __restrict__ HZ,
                                                         const float2* __restrict__ F,
                                                         const double* __restrict__ part, int tiles,
                                                         double* __restrict__ E, double* __restrict__ scale,
-                                                        double gap_rel) {
+                                                        double gap_rel, RefineCheck chk) {
   __shared__ C2 lds[7][kMaxCluster][kMaxCluster];
   __shared__ double2 hzc[kMaxCluster][kMaxEighN];  // refined H z' of the cluster's columns
   __shared__ double ev[kMaxCluster];
+  __shared__ double s_res[kMaxEighN];  // residual of unit column j
   __shared__ int solved;
   const size_t k = blockIdx.x;
   const int lane = threadIdx.x;
   double* Ek = E + k * n;
   double* sk = scale + k * n;
   {
-    // Rayleigh quotients and column scales from the per-tile sums of the
-    // correction product's epilogue
-    const double* pk = part + k * tiles * 2 * static_cast<size_t>(n);
+    // Rayleigh quotients, column scales and residuals (against the
+    // Ogita-Aishima quotients) from the per-tile sums of the correction
+    // product's epilogue
+    const double* pk = part + k * tiles * 3 * static_cast<size_t>(n);
     for (int j = lane; j < n; j += 64) {
-      double num = 0.0, den = 0.0;
+      double num = 0.0, den = 0.0, res = 0.0;
       for (int t = 0; t < tiles; ++t) {
-        num += pk[(2 * t) * static_cast<size_t>(n) + j];
-        den += pk[(2 * t + 1) * static_cast<size_t>(n) + j];
+        num += pk[(3 * t) * static_cast<size_t>(n) + j];
+        den += pk[(3 * t + 1) * static_cast<size_t>(n) + j];
+        res += pk[(3 * t + 2) * static_cast<size_t>(n) + j];
       }
       Ek[j] = num / den;
       sk[j] = 1.0 / sqrt(den);
+      s_res[j] = sqrt(res / den);
     }
     __threadfence_block();
     __syncthreads();
@@ -68,87 +74,165 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
   const double thr = gap_rel * fmax(m, 1.0);
   int close = 0;
   for (int j = lane; j + 1 < n; j += 64) close |= (Ek[j + 1] - Ek[j] <= thr);
-  if (!__any(close)) return;
-  double2* Zk = Z + k * n * n;
-  const double2* Hk = HZ + k * n * n;
-  const float2* Fk = F + k * n * n;
-  int j0 = 0;
-  while (j0 < n) {  // the same cluster sequence in every lane
-    const int j1 = cluster_end(n, Ek, j0, thr);
-    const int c = j1 - j0 + 1;
-    if (c >= 2 && c <= kMaxCluster) {
-      // H z'_b = HZ (e_b + (F' - I)_b) for the cluster's columns, in FP64
-      for (int b = 0; b < c; ++b)
-        for (int r = lane; r < n; r += 64) {
-          const double2* hr = Hk + static_cast<size_t>(r) * n;
-          double2 acc = hr[j0 + b];
-          for (int q = 0; q < n; ++q) {
-            const double2 h = hr[q];
-            const float2 f = Fk[static_cast<size_t>(q) * n + j0 + b];
-            acc.x += h.x * f.x - h.y * f.y;
-            acc.y += h.x * f.y + h.y * f.x;
+  if (__any(close)) {
+    double2* Zk = Z + k * n * n;
+    const double2* Hk = HZ + k * n * n;
+    const float2* Fk = F + k * n * n;
+    int j0 = 0;
+    while (j0 < n) {  // the same cluster sequence in every lane
+      const int j1 = cluster_end(n, Ek, j0, thr);
+      const int c = j1 - j0 + 1;
+      // (a run of more than kMaxCluster close eigenvalues is split; the
+      // residuals of the columns at a split fail the check)
+      if (c >= 2) {
+        // H z'_b = HZ (e_b + (F' - I)_b) for the cluster's columns, in FP64
+        for (int b = 0; b < c; ++b)
+          for (int r = lane; r < n; r += 64) {
+            const double2* hr = Hk + static_cast<size_t>(r) * n;
+            double2 acc = hr[j0 + b];
+            for (int q = 0; q < n; ++q) {
+              const double2 h = hr[q];
+              const float2 f = Fk[static_cast<size_t>(q) * n + j0 + b];
+              acc.x += h.x * f.x - h.y * f.y;
+              acc.y += h.x * f.y + h.y * f.x;
+            }
+            hzc[b][r] = acc;
           }
-          hzc[b][r] = acc;
+        __syncthreads();
+        for (int p = 0; p < c * c; ++p) {
+          const int a = p / c, b = p % c;
+          double gr = 0.0, gi = 0.0, mr = 0.0, mi = 0.0;
+          for (int r = lane; r < n; r += 64) {
+            const double2 za = Zk[static_cast<size_t>(r) * n + j0 + a];
+            const double2 zb = Zk[static_cast<size_t>(r) * n + j0 + b];
+            const double2 hb = hzc[b][r];
+            gr += za.x * zb.x + za.y * zb.y;  // conj(za) zb
+            gi += za.x * zb.y - za.y * zb.x;
+            mr += za.x * hb.x + za.y * hb.y;  // conj(za) hb
+            mi += za.x * hb.y - za.y * hb.x;
+          }
+          // sums over the unit columns s_a z_a, s_b z_b (H z_b scales with z_b)
+          const double sab = sk[j0 + a] * sk[j0 + b];
+          gr = wave_sum(gr) * sab;
+          gi = wave_sum(gi) * sab;
+          mr = wave_sum(mr) * sab;
+          mi = wave_sum(mi) * sab;
+          if (lane == 0) {
+            lds[0][a][b] = {gr, gi};
+            lds[1][a][b] = {mr, mi};
+          }
         }
-      __syncthreads();
-      for (int p = 0; p < c * c; ++p) {
-        const int a = p / c, b = p % c;
-        double gr = 0.0, gi = 0.0, mr = 0.0, mi = 0.0;
-        for (int r = lane; r < n; r += 64) {
-          const double2 za = Zk[static_cast<size_t>(r) * n + j0 + a];
-          const double2 zb = Zk[static_cast<size_t>(r) * n + j0 + b];
-          const double2 hb = hzc[b][r];
-          gr += za.x * zb.x + za.y * zb.y;  // conj(za) zb
-          gi += za.x * zb.y - za.y * zb.x;
-          mr += za.x * hb.x + za.y * hb.y;  // conj(za) hb
-          mi += za.x * hb.y - za.y * hb.x;
-        }
-        // sums over the unit columns s_a z_a, s_b z_b (H z_b scales with z_b)
-        const double sab = sk[j0 + a] * sk[j0 + b];
-        gr = wave_sum(gr) * sab;
-        gi = wave_sum(gi) * sab;
-        mr = wave_sum(mr) * sab;
-        mi = wave_sum(mi) * sab;
-        if (lane == 0) {
-          lds[0][a][b] = {gr, gi};
-          lds[1][a][b] = {mr, mi};
-        }
-      }
-      if (lane == 0) solved = cluster_rr_small(c, lds, ev) ? 1 : 0;
-      __syncthreads();
-      if (solved) {
-        for (int r = lane; r < n; r += 64) {
-          double2* zr = Zk + static_cast<size_t>(r) * n + j0;
-          C2 row[kMaxCluster];
+        if (lane == 0) solved = cluster_rr_small(c, lds, ev) ? 1 : 0;
+        __syncthreads();
+        if (solved) {
+          double res2[kMaxCluster];
 #pragma unroll
-          for (int q = 0; q < kMaxCluster; ++q)
-            if (q < c) row[q] = {zr[q].x * sk[j0 + q], zr[q].y * sk[j0 + q]};
+          for (int q = 0; q < kMaxCluster; ++q) res2[q] = 0.0;
+          for (int r = lane; r < n; r += 64) {
+            double2* zr = Zk + static_cast<size_t>(r) * n + j0;
+            C2 row[kMaxCluster], hrow[kMaxCluster];
 #pragma unroll
-          for (int bb = 0; bb < kMaxCluster; ++bb) {
-            if (bb < c) {
-              C2 acc = {0.0, 0.0};
-#pragma unroll
-              for (int q = 0; q < kMaxCluster; ++q) {
-                if (q < c) {
-                  const C2 t = cm(row[q], lds[6][q][bb]);
-                  acc.re += t.re;
-                  acc.im += t.im;
-                }
+            for (int q = 0; q < kMaxCluster; ++q)
+              if (q < c) {
+                const double sq = sk[j0 + q];
+                row[q] = {zr[q].x * sq, zr[q].y * sq};
+                hrow[q] = {hzc[q][r].x * sq, hzc[q][r].y * sq};
               }
-              zr[bb] = make_double2(acc.re, acc.im);
+#pragma unroll
+            for (int bb = 0; bb < kMaxCluster; ++bb) {
+              if (bb < c) {
+                C2 acc = {0.0, 0.0}, hacc = {0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < kMaxCluster; ++q) {
+                  if (q < c) {
+                    const C2 t = cm(row[q], lds[6][q][bb]);
+                    const C2 u = cm(hrow[q], lds[6][q][bb]);
+                    acc.re += t.re;
+                    acc.im += t.im;
+                    hacc.re += u.re;
+                    hacc.im += u.im;
+                  }
+                }
+                zr[bb] = make_double2(acc.re, acc.im);
+                const double rr = fma(-ev[bb], acc.re, hacc.re), ri = fma(-ev[bb], acc.im, hacc.im);
+                res2[bb] += rr * rr + ri * ri;
+              }
             }
           }
+#pragma unroll
+          for (int bb = 0; bb < kMaxCluster; ++bb)
+            if (bb < c) res2[bb] = wave_sum(res2[bb]);
+          __syncthreads();  // every lane has read the old scales
+          if (lane < c) {
+            Ek[j0 + lane] = ev[lane];
+            sk[j0 + lane] = 1.0;  // the rotated columns are unit vectors
+          }
+#pragma unroll
+          for (int bb = 0; bb < kMaxCluster; ++bb)
+            if (bb < c && lane == bb) s_res[j0 + bb] = sqrt(res2[bb]);
+        } else {
+          for (int q = lane; q < c; q += 64) s_res[j0 + q] = HUGE_VAL;
         }
-        __syncthreads();  // every lane has read the old scales
-        if (lane < c) {
-          Ek[j0 + lane] = ev[lane];
-          sk[j0 + lane] = 1.0;  // the rotated columns are unit vectors
-        }
+        __syncthreads();
       }
-      __syncthreads();
+      j0 = j1 + 1;
     }
-    j0 = j1 + 1;
   }
+  __threadfence_block();
+  __syncthreads();
+  // the check: every eigenvalue finite, every residual within the bound
+  double mx = 0.0, rmax = 0.0;
+  int bad = 0;
+  for (int j = lane; j < n; j += 64) {
+    const double e = Ek[j];
+    bad |= !isfinite(e);
+    mx = fmax(mx, fabs(e));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  const double bound = chk.res_tol * fmax(1.0, mx);
+  for (int j = lane; j < n; j += 64) {
+    const double r = s_res[j];
+    bad |= !(r <= bound);
+    if (isfinite(r)) rmax = fmax(rmax, r);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+  const int any_bad = __any(bad);
+  if (lane == 0) {
+    if (any_bad && chk.fail_count) {
+      const int slot = atomicAdd(chk.fail_count, 1);
+      chk.fail_list[slot] = static_cast<int>(k);
+    }
+    if (chk.max_res) atomicMax(chk.max_res, static_cast<unsigned long long>(__double_as_longlong(rmax)));
+  }
+}
+
+__global__ void count_nonfinite_kernel(int n, size_t nk, const double* __restrict__ E, int* __restrict__ count) {
+  const size_t k = blockIdx.x * 4ull + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= nk) return;
+  int bad = 0;
+  for (int j = lane; j < n; j += 64) bad |= !isfinite(E[k * n + j]);
+  if (__any(bad) && lane == 0) atomicAdd(count, 1);
+}
+
+__global__ void scatter_resolved_kernel(int n, const int* __restrict__ list, const double* __restrict__ Ef,
+                                        const double2* __restrict__ Zf, double* __restrict__ E,
+                                        double2* __restrict__ Z, double* __restrict__ scale) {
+  const size_t i = blockIdx.x;
+  const size_t k = static_cast<size_t>(list[i]);
+  const size_t n2 = static_cast<size_t>(n) * n;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    E[k * n + j] = Ef[i * n + j];
+    scale[k * n + j] = 1.0;
+  }
+  for (size_t e = threadIdx.x; e < n2; e += blockDim.x) Z[k * n2 + e] = Zf[i * n2 + e];
+}
+
+__global__ void zero_ints_kernel(int* p, size_t count) {
+  const size_t i = blockIdx.x * 256ull + threadIdx.x;
+  if (i < count) p[i] = 0;
 }
 
 __global__ void transpose_kernel(int n, size_t nk, const double2* __restrict__ Z, const double* __restrict__ scale,
@@ -193,7 +277,7 @@ int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const d
 // W2 as columns that become unit vectors when multiplied by scale[k][j]
 // ([nk][n]; on entry the squared column norms of Z).
 int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, const float* Z, double* W1,
-                      double* W2, double* W3, double* E, double* scale, double gap_rel) {
+                      double* W2, double* W3, double* E, double* scale, double gap_rel, const RefineCheck& chk) {
   if (nk == 0) return RTB_OK;
   const size_t n2 = static_cast<size_t>(n) * n;
   // FP64 products: HZ, S = Z^H HZ, and G = Z^H Z consumed by its epilogue,
@@ -203,7 +287,7 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, c
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM64));
   RTB_TRY(zgemm(st, n, nk, zop(H, n2), false, zop(Z, n2), W1, n2, false, nullptr));
   RTB_TRY(zgemm(st, n, nk, zop(Z, n2), true, zop(W1, n2), W2, n2, true, nullptr));
-  RTB_TRY(zgemm_gram_oa(st, n, nk, Z, W2, scale, F, gap_rel));
+  RTB_TRY(zgemm_gram_oa(st, n, nk, Z, W2, scale, F, gap_rel, E));  // E <- the quotients lt
   if (tmr)
     RTB_TRY(tmr->end(3, flop_zgemm(n, nk) + 2 * flop_zgemm_herm(n, nk),
                      mflop_zgemm(n, nk, false) + 2 * mflop_zgemm(n, nk, true)));
@@ -213,14 +297,38 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, c
   const int tiles = (n + 63) / 64;
   double* part = W3;
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_GEMM32));
-  RTB_TRY(zgemm_corrections_f32(st, n, nk, Z, W1, F, W2, part));  // both products, one pass over F
+  RTB_TRY(zgemm_corrections_f32(st, n, nk, Z, W1, F, E, W2, part));  // both products, one pass over F
   if (tmr) RTB_TRY(tmr->end(1, 2 * flop_zgemm(n, nk), 2 * mflop_zgemm(n, nk, false)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
   hipLaunchKernelGGL(cluster_rr_kernel, dim3(static_cast<unsigned>(nk)), dim3(64), 0, st, n,
                      reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W1),
-                     reinterpret_cast<const float2*>(F), part, tiles, E, scale, gap_rel);
+                     reinterpret_cast<const float2*>(F), part, tiles, E, scale, gap_rel, chk);
   RTB_HIP(hipGetLastError());
   if (tmr) RTB_TRY(tmr->end(1));
+  return RTB_OK;
+}
+
+int count_nonfinite(hipStream_t st, int n, size_t nk, const double* E, int* count) {
+  if (nk == 0) return RTB_OK;
+  hipLaunchKernelGGL(count_nonfinite_kernel, dim3(static_cast<unsigned>((nk + 3) / 4)), dim3(256), 0, st, n, nk, E,
+                     count);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int scatter_resolved(hipStream_t st, int n, int cnt, const int* list, const double* Ef, const double* Zf, double* E,
+                     double* Z, double* scale) {
+  if (cnt <= 0) return RTB_OK;
+  hipLaunchKernelGGL(scatter_resolved_kernel, dim3(static_cast<unsigned>(cnt)), dim3(256), 0, st, n, list, Ef,
+                     reinterpret_cast<const double2*>(Zf), E, reinterpret_cast<double2*>(Z), scale);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int zero_ints(hipStream_t st, int* p, size_t count) {
+  if (count == 0) return RTB_OK;
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256), 0, st, p, count);
+  RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
 

@@ -116,10 +116,18 @@ class Timer {
     RTB_HIP(hipEventRecord(recs_.back().b, st_));
     return RTB_OK;
   }
+  // the eigenpair checks of the call: largest refined residual (eV) and the
+  // number of k-points re-solved in FP64
+  void note_check(double max_residual, size_t resolved) {
+    max_res_ = max_residual;
+    resolved_ = resolved;
+  }
   int collect(size_t nk, int n) {
     rtb_stats s{};
     s.nk = nk;
     s.nsta = n;
+    s.max_residual = max_res_;
+    s.resolved = resolved_;
     for (auto& r : recs_) {
       float ms = 0.f;
       RTB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
@@ -152,6 +160,8 @@ class Timer {
   Lease* lease_ = nullptr;
   hipStream_t st_ = nullptr;
   std::vector<Rec> recs_;
+  double max_res_ = 0.0;
+  size_t resolved_ = 0;
 };
 
 // Algorithmic flop counts used in the stats.

@@ -94,11 +94,14 @@ __device__ inline V zero2() {
 //                = (1 - gd_i) / 2                      (i == j)
 // from S = Z^H H Z (already stored), the exact diagonal gd of G and the
 // Rayleigh quotients lt_i = S_ii / gd_i; thr = gap_rel * max(1, max |lt|).
+// The tile at (0, 0) also writes the lt_i to lt ([nmat][n]) for the
+// residuals of the correction products.
 struct OaEpilogue {
   const double2* S = nullptr;  // [nmat][n][n], stride sC
   const double* gd = nullptr;  // [nmat][n]
   float2* F = nullptr;         // [nmat][n][n], stride sC
   double gap_rel = 0.0;
+  double* lt = nullptr;        // [nmat][n]
 };
 
 // Optional epilogue of the Kubo product C = U^H (v_a + i v_b) U: C is not
@@ -146,9 +149,11 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
     oa.gd += mat * n;
     oa.F += mat * sC;
     double m = 0.0;
+    const bool put_lt = oa.lt && ti == 0 && tj == 0;
     for (int i = tid; i < n; i += 256) {
       const double l = oa.S[static_cast<size_t>(i) * n + i].x / oa.gd[i];
       s_lt[i] = l;
+      if (put_lt) oa.lt[mat * n + i] = l;
       m = fmax(m, fabs(l));
     }
 #pragma unroll
@@ -359,17 +364,19 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
 // Z' = Z + Z F and (HZ)' = HZ + HZ F on the FP32 matrix cores (Z, F in FP32
 // storage, HZ in FP64), one 64 x 64 tile of each per workgroup sharing the
 // staged F tile.  Z' is stored (FP64); (HZ)' is consumed in the epilogue:
-// the Rayleigh sums Re(z'^H (HZ)') and z'^H z' of the tile's columns over its
-// rows go to part[mat][tile row][2][n].  Same lane layouts and pipelining as
-// zgemm3m_kernel.
+// the Rayleigh sums Re(z'^H (HZ)') and z'^H z' and the squared residual
+// |(HZ)' - lt_j z'|^2 of the tile's columns over its rows go to
+// part[mat][tile row][3][n] (lt: [nmat][n], the Ogita-Aishima Rayleigh
+// quotients).  Same lane layouts and pipelining as zgemm3m_kernel.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z, const double2* __restrict__ HZ,
-                    const float2* __restrict__ F, double2* __restrict__ Zp, double* __restrict__ part) {
+                    const float2* __restrict__ F, const double* __restrict__ lt, double2* __restrict__ Zp,
+                    double* __restrict__ part) {
   typedef Mfma<float>::v4 v4;
   __shared__ float sA3[3][kKc][kLd];  // Z tile, k-major: re, im, re + im
   __shared__ float sH3[3][kKc][kLd];  // HZ tile (rounded to FP32)
   __shared__ float sB3[3][kKc][kLd];  // F tile
-  __shared__ double s_rq[2][2][kTile];
+  __shared__ double s_rq[2][3][kTile];
   const size_t g = blockIdx.x;
   const size_t loc = g / kXcd;
   const size_t mat = (loc / tm.tiles) * kXcd + g % kXcd;
@@ -466,7 +473,12 @@ zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z
           }
     }
   }
-  double num[2] = {0.0, 0.0}, den[2] = {0.0, 0.0};
+  double num[2] = {0.0, 0.0}, den[2] = {0.0, 0.0}, res[2] = {0.0, 0.0}, ltc[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int j = col0 + wc + 16 * b + fr;
+    ltc[b] = j < n ? lt[mat * n + j] : 0.0;
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -488,6 +500,8 @@ zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z
           Zp[o] = make_double2(zr, zi);
           num[b] += zr * hr + zi * hi;  // Re(conj(z') h')
           den[b] += zr * zr + zi * zi;
+          const double rr = fma(-ltc[b], zr, hr), ri = fma(-ltc[b], zi, hi);
+          res[b] += rr * rr + ri * ri;
         }
       }
 #pragma unroll
@@ -496,16 +510,20 @@ zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z
     num[b] += __shfl_xor(num[b], 32, 64);
     den[b] += __shfl_xor(den[b], 16, 64);
     den[b] += __shfl_xor(den[b], 32, 64);
+    res[b] += __shfl_xor(res[b], 16, 64);
+    res[b] += __shfl_xor(res[b], 32, 64);
     if (fk == 0) {
       s_rq[wave >> 1][0][wc + 16 * b + fr] = num[b];
       s_rq[wave >> 1][1][wc + 16 * b + fr] = den[b];
+      s_rq[wave >> 1][2][wc + 16 * b + fr] = res[b];
     }
   }
   __syncthreads();
   if (tid < kTile && col0 + tid < n) {
-    double* pt = part + (mat * tm.side + ti) * 2 * static_cast<size_t>(n);
+    double* pt = part + (mat * tm.side + ti) * 3 * static_cast<size_t>(n);
     pt[col0 + tid] = s_rq[0][0][tid] + s_rq[1][0][tid];
     pt[n + col0 + tid] = s_rq[0][1][tid] + s_rq[1][1][tid];
+    pt[2 * n + col0 + tid] = s_rq[0][2][tid] + s_rq[1][2][tid];
   }
 }
 
@@ -533,7 +551,7 @@ int zgemm_launch(hipStream_t st, int n, size_t nb, const ZOperand& A, bool conj_
                        pD ? pD + off * D.stride : nullptr, D.stride,
                        C ? reinterpret_cast<double2*>(C) + off * sC : nullptr, sC, sc ? sc + off * n : nullptr,
                        OaEpilogue{oa.S ? oa.S + off * sC : nullptr, oa.gd ? oa.gd + off * n : nullptr,
-                                  oa.F ? oa.F + off * sC : nullptr, oa.gap_rel},
+                                  oa.F ? oa.F + off * sC : nullptr, oa.gap_rel, oa.lt ? oa.lt + off * n : nullptr},
                        KuboEpilogue{kb.E ? kb.E + off * n : nullptr, kb.eta2,
                                     kb.part ? kb.part + off * tm.side * 2 * static_cast<size_t>(n) : nullptr});
   }
@@ -553,7 +571,7 @@ int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B,
 }
 
 int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double* S, const double* gd, float* F,
-                  double gap_rel) {
+                  double gap_rel, double* lt) {
   if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "zgemm_gram_oa supports n <= 128");
   const size_t n2 = static_cast<size_t>(n) * n;
   OaEpilogue oa;
@@ -561,6 +579,7 @@ int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double
   oa.gd = gd;
   oa.F = reinterpret_cast<float2*>(F);
   oa.gap_rel = gap_rel;
+  oa.lt = lt;
   return zgemm_launch<double, false, float2, float2>(st, n, nb, zop(Z, n2), true, zop(Z, n2), ZOperand{}, nullptr,
                                                      n2, true, nullptr, oa);
 }
@@ -577,7 +596,7 @@ int zgemm_kubo(hipStream_t st, int n, size_t nb, const double* U, const double* 
 }
 
 int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, const double* HZ, const float* F,
-                          double* Zp, double* part) {
+                          const double* lt, double* Zp, double* part) {
   if (nb == 0 || n <= 0) return RTB_OK;
   TileMap tm;
   tm.side = (n + kTile - 1) / kTile;
@@ -590,8 +609,8 @@ int zgemm_corrections_f32(hipStream_t st, int n, size_t nb, const float* Z, cons
     const size_t wgs = (cnt + kXcd - 1) / kXcd * tm.tiles * kXcd;
     hipLaunchKernelGGL(zgemm3m_dual_kernel, dim3(static_cast<unsigned>(wgs)), dim3(256), 0, st, n, cnt, tm,
                        reinterpret_cast<const float2*>(Z) + off * n2, reinterpret_cast<const double2*>(HZ) + off * n2,
-                       reinterpret_cast<const float2*>(F) + off * n2, reinterpret_cast<double2*>(Zp) + off * n2,
-                       part + off * tm.side * 2 * static_cast<size_t>(n));
+                       reinterpret_cast<const float2*>(F) + off * n2, lt + off * n,
+                       reinterpret_cast<double2*>(Zp) + off * n2, part + off * tm.side * 3 * static_cast<size_t>(n));
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;

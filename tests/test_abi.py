@@ -34,3 +34,24 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     # the ctypes table covers the whole header
     assert declared_functions() <= set(_lib.SIGNATURES)
+
+
+def test_non_finite_hoppings_rejected_before_any_device_work():
+    """rtb_model_create checks the hoppings first (no device is touched when
+    they are rejected, so this runs without a GPU)."""
+    import numpy as np
+
+    import rustb_amd as ra
+
+    for bad in (float("nan"), float("inf")):
+        m = ra.Model.from_arrays(np.eye(2), [[0.0, 0.0], [0.5, 0.5]], False, np.array([[0, 0], [1, 0], [-1, 0]]),
+                                 np.zeros((3, 2, 2), complex))
+        h = m.ham.copy()
+        h[1, 0, 1] = complex(bad, 0.0)
+        m2 = ra.Model.from_arrays(m.lat, m.orb, False, m.hamR, h)
+        try:
+            m2._device()
+        except ra.TbError as e:
+            assert e.code == -11 and "finite" in str(e)  # RTB_ERR_INVALID_MODEL
+        else:
+            raise AssertionError("non-finite hopping accepted")

@@ -2,6 +2,7 @@
 band fixture, closed forms and invariants (CPU only)."""
 import math
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -428,3 +429,130 @@ def test_range_solves_half_open_interval():
     assert np.array_equal(w, [0.0, 1.0, 2.0]) and V.shape == (3, 4)
     H = O.gen_ham(o, np.array([0.1, 0.2]), "atom")
     assert np.abs(H @ V.T - V.T * w).max() < 1e-14
+
+
+# ------------------------------------------------------------ spin currents
+def test_spin_matrix_algebra_and_layout():
+    """build_spin_matrix (response/helpers.rs:11-39) is S = sigma/2 (x) I_norb
+    with the spin index outer ([up block, down block], model_build.rs:71-98):
+    Hermitian, S_a^2 = 1/4, [S_x, S_y] = i S_z and cyclic, S_z = +1/2 on the
+    up block; and an onsite term J decorated with Pauli a by the builder's
+    table (model_build.rs:118-145, pinned by the graphene band fixture for Z)
+    is 2 J S_a."""
+    for norb in (1, 3):
+        S = {a: O.spin_matrix(norb, a) for a in "XYZ"}
+        eye = np.eye(2 * norb)
+        for a in "XYZ":
+            assert np.array_equal(S[a], S[a].conj().T)
+            assert np.array_equal(S[a] @ S[a], eye / 4)
+            assert np.trace(S[a]) == 0
+        for a, b, c in (("X", "Y", "Z"), ("Y", "Z", "X"), ("Z", "X", "Y")):
+            assert np.array_equal(S[a] @ S[b] - S[b] @ S[a], 1j * S[c])
+        assert np.array_equal(np.diag(S["Z"]).real, [0.5] * norb + [-0.5] * norb)
+        assert np.array_equal(O.spin_matrix(norb, None), eye)
+        orb = np.random.default_rng(norb).random((norb, 2))
+        for a in "XYZ":
+            m = O.Model(np.eye(2), orb, True)
+            m.add_onsite([0.37] * norb, a)
+            H = O.gen_ham(m, [0.1, 0.2], "lattice")  # only R = 0 is stored
+            assert np.allclose(H, 2 * 0.37 * S[a], rtol=0, atol=1e-15)
+
+
+def _kane_mele(make, axis, t2, delta=0.7, t=-1.0):
+    """Honeycomb model H = 1 (x) H0 + sigma_axis (x) H_so: spin-independent
+    nearest-neighbour hopping and staggered onsite (H0), and the Haldane
+    next-neighbour term i t2 decorated with the Pauli matrix `axis` (H_so).
+    It conserves S_axis; its S_axis = +-1/2 sectors are the spinless Haldane
+    models with next-neighbour term +-t2 (H0 +- H_so)."""
+    m = make(RM.HEX_LAT, RM.HEX_ORB, True)
+    m.set_onsite([-delta, delta])
+    for R in ([0, 0], [-1, 0], [0, -1]):
+        m.add_hop(t, 0, 1, R)
+    for R in ([1, 0], [-1, 1], [0, -1]):
+        m.add_hop(complex(0.0, t2), 0, 0, R, axis)
+    for R in ([-1, 0], [1, -1], [0, 1]):
+        m.add_hop(complex(0.0, t2), 1, 1, R, axis)
+    return m
+
+
+@pytest.mark.parametrize("axis", ["X", "Y", "Z"])
+def test_spin_berry_curvature_of_conserved_axis(axis):
+    """The spin-current Berry curvature (J = {S, v_a}/2, traits.rs:170-175)
+    of a model that conserves S_axis is +-1/2 times the spinless curvature of
+    each sector: per k, the occupied sum equals (Omega_occ(H0 + H_so) -
+    Omega_occ(H0 - H_so)) / 2 from the pinned spinless function; over the
+    Brillouin zone 2 pi sigma_spin = (C_+ - C_-) / 2 = +-1.  A wrong sign of
+    sigma_y, a missing 1/2 or the spin index taken inner fails this."""
+    t2 = -0.3
+    m = _kane_mele(make, axis, t2)
+    up, dn = RM.haldane(make, t2), RM.haldane(make, -t2)
+    d = np.eye(2)
+    for k in np.random.default_rng(5).random((6, 2)):
+        om, E = O.berry_curvature_at(m, k, d, 1e-3, spin=axis)
+        ou, Eu = O.berry_curvature_at(up, k, d, 1e-3)
+        od, Ed = O.berry_curvature_at(dn, k, d, 1e-3)
+        assert np.allclose(E, np.sort(np.concatenate([Eu, Ed])), rtol=0, atol=1e-12)
+        occ = om[E < 0].sum()
+        ref = 0.5 * (ou[Eu < 0].sum() - od[Ed < 0].sum())
+        assert abs(occ - ref) <= 1e-10 * max(1.0, abs(ref))
+        # the charge curvature of the same model is the sum of the sectors
+        oc, _ = O.berry_curvature_at(m, k, d, 1e-3)
+        assert abs(oc[E < 0].sum() - (ou[Eu < 0].sum() + od[Ed < 0].sum())) <= 1e-10 * max(1.0, abs(ref))
+    s = O.hall_conductivity_direct(m, [31, 31], d, [0.0], 0.0, 1e-3, spin=axis)[0]
+    su = O.hall_conductivity_direct(up, [31, 31], d, [0.0], 0.0, 1e-3)[0]
+    sd = O.hall_conductivity_direct(dn, [31, 31], d, [0.0], 0.0, 1e-3)[0]
+    assert abs(s - 0.5 * (su - sd)) < 1e-10
+    assert abs(abs(2 * math.pi * s) - 1.0) < 1e-3
+
+
+def test_fermi_derivative_is_minus_df_dE():
+    """fermi_derivative_from_width (thermodynamics.rs:122-126) against a
+    central difference of the pinned fermi(), and its unit integral over E."""
+    mu = 0.013
+    for T in (10.0, 300.0, 2000.0):
+        w = T * O.BOLTZMANN_CONSTANT_EV_PER_K
+        E = mu + np.linspace(-30.0, 30.0, 601) * w
+        h = 1e-5 * w
+        fd = -(O.fermi(E + h, mu, T) - O.fermi(E - h, mu, T)) / (2.0 * h)
+        d = O.fermi_derivative(E, mu, T)
+        assert np.abs(d - fd).max() <= 1e-6 * d.max()
+        assert abs(d.max() - 0.25 / w) <= 1e-12 / w  # at E = mu: f (1 - f) = 1/4
+        Eg = mu + np.linspace(-60.0, 60.0, 200001) * w
+        assert abs(np.trapezoid(O.fermi_derivative(Eg, mu, T), Eg) - 1.0) < 1e-8
+
+
+# ------------------------------------------- the C++ restatement (checker)
+def test_cpu_ref_matches_oracle():
+    """oracle/cpu_ref.cpp (the benchmark's accuracy checker and CPU baseline)
+    against the pinned oracle on a random spinful 3D model: eigenvalues of
+    both modes and Omega (gauge-invariant, so LAPACK phase choices drop out)."""
+    from oracle import cpu_ref
+
+    try:
+        cpu_ref.load()
+    except (RuntimeError, OSError, subprocess.CalledProcessError) as e:
+        pytest.skip(f"oracle/cpu_ref.cpp not buildable or no host LAPACK: {e}")
+    rng = np.random.default_rng(17)
+    lat = np.eye(3) + 0.2 * rng.standard_normal((3, 3))
+    o = O.Model(lat, rng.random((3, 3)), True)
+    n = o.nsta
+    z = np.zeros(3, int)
+    for i in range(n):
+        o.add_element(rng.uniform(-1, 1), i, i, z)
+        for j in range(i + 1, n):
+            o.add_element(complex(*rng.standard_normal(2)), i, j, z)
+    for _ in range(5):
+        R = rng.integers(-2, 3, 3)
+        if R.any():
+            for i in range(n):
+                for j in range(n):
+                    o.add_element(complex(*rng.standard_normal(2)) * 0.4, i, j, R)
+    ks = rng.random((12, 3))
+    d = np.array([[1.0, 0.2, 0.0], [0.0, 1.0, 0.3]])
+    om, E = cpu_ref.run(2, o.lat, o.orb, o.spin, o.hamR, o.ham, ks, d, 1e-3, threads=2)
+    ob, Eb = O.berry_curvature_batch(o, ks, d, 1e-3)
+    E0 = cpu_ref.run(0, o.lat, o.orb, o.spin, o.hamR, o.ham, ks, threads=2)
+    scale = np.abs(Eb).max()
+    assert np.abs(E - Eb).max() <= 1e-12 * scale
+    assert np.abs(E0 - Eb).max() <= 1e-12 * scale
+    assert np.abs(om - ob).max() <= 1e-9 * np.abs(ob).max()
