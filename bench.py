@@ -74,10 +74,10 @@ def traffic_from_profiles(stage):
 
 def mfma_ceiling():
     """Attainable FP64 and FP32 matrix-core rates of this device (TFLOP/s),
-    measured by rustb_amd/csrc/probe.hip: independent MFMA chains, no memory
+    measured by bench_probe/probe.hip: independent MFMA chains, no memory
     traffic.  None when the probe library is absent or fails."""
     try:
-        lib = ctypes.CDLL(os.path.join(ROOT, "rustb_amd", "librtb_probe.so"))
+        lib = ctypes.CDLL(os.path.join(ROOT, "bench_probe", "librtb_probe.so"))
     except OSError:
         return None
     f64, f32 = ctypes.c_double(), ctypes.c_double()
@@ -189,7 +189,7 @@ def ingest_bench(args):
     size = os.path.getsize(hr)
     threads = min(16, os.cpu_count() or 1)
     lib = _lib.load()
-    probe = ctypes.CDLL(os.path.join(ROOT, "rustb_amd", "librtb_probe.so"))
+    probe = ctypes.CDLL(os.path.join(ROOT, "bench_probe", "librtb_probe.so"))
     probe.rtb_probe_read.restype = ctypes.c_int
     probe.rtb_probe_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     ceil = ctypes.c_double()

@@ -10,7 +10,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "librtb_hip.so")
-PROBE_LIB = os.path.join(HERE, "librtb_probe.so")  # matrix-core ceiling probe used by bench.py
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["builder.cpp", "capi.cpp", "workspace.cpp", "wannier90.cpp", "energy_cut.cpp", "fourier.hip", "fourier_mesh.hip", "eigh.hip", "refine.hip", "kubo.hip", "zgemm.hip"]
@@ -37,15 +36,11 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=min(len(SOURCES) + 1, 8)) as ex:
-        objs = list(ex.map(_compile, SOURCES + ["probe.hip"]))
-    probe_obj = objs.pop()
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
+        objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], check=True)
     os.replace(tmp, LIB)
-    tmp = PROBE_LIB + ".tmp"
-    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", probe_obj, "-o", tmp], check=True)
-    os.replace(tmp, PROBE_LIB)
     if verbose:
         print(f"built {LIB}")
     return LIB
