@@ -228,6 +228,20 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
                     double T, double eta, const double* dir, int spin_dir, int precision,
                     double* sigma);
 
+/* rtb_hall_direct without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * device buffers are a fixed static device array of 256 MiB per GPU the
+ * library runs on (held until the process exits); the k-mesh is processed
+ * in chunks sized to fit it, with the dense Fourier sum in both precision
+ * forms.  A call whose buffers cannot fit (even one k-point per chunk, plus
+ * the direction tables, position matrices and the n_mu-long sums) returns
+ * RTB_ERR_UNSUPPORTED.  Runs on the device's null stream; every call on one
+ * GPU uses the same array, so calls that overlap on one GPU race. */
+int rtb_hall_direct_noalloc(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu,
+                            double T, double eta, const double* dir, int spin_dir, int precision,
+                            double* sigma);
+
 /* Model::hall_conductivity, Integration::EnergyCut (response/linear/mod.rs:177-185,
  * 226-268) for a 2D model: per-vertex E and K^{ab}_nm = v^a_nm v^b_mn
  * (compute_velocity_kernel, response/primitives.rs:33-137, FP64), band tracking

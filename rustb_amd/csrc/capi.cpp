@@ -230,7 +230,9 @@ struct Checks {
   }
 };
 
-int checks_init(hipStream_t st, Work& w, size_t K, Checks& ck) {
+// The counters (and, with reserve > 0, the FP64 re-solve buffers for that
+// many k-points at a time) from the workspace; checks_zero clears the counters.
+int checks_alloc(Work& w, int n, size_t K, size_t reserve, Checks& ck) {
   ck.opt = mixed_opts();
   int* ints;
   RTB_TRY(w.alloc(&ints, 4 + K));  // flag count, non-finite count, max residual (8 bytes), list
@@ -238,7 +240,23 @@ int checks_init(hipStream_t st, Work& w, size_t K, Checks& ck) {
   ck.nonfinite = ints + 1;
   ck.max_res = reinterpret_cast<unsigned long long*>(ints + 2);
   ck.flag_list = ints + 4;
-  return zero_ints(st, ints, 4);
+  if (reserve > 0) {
+    const size_t n2 = static_cast<size_t>(n) * n;
+    ck.cap = reserve;
+    RTB_TRY(w.alloc(&ck.dk, ck.cap * 3));
+    RTB_TRY(w.alloc(&ck.H, ck.cap * n2 * 2));
+    RTB_TRY(w.alloc(&ck.E, ck.cap * n));
+    RTB_TRY(w.alloc(&ck.Z, ck.cap * n2 * 2));
+    RTB_TRY(alloc_eig(w, n, ck.cap, true, ck.eb));
+  }
+  return RTB_OK;
+}
+
+int checks_zero(hipStream_t st, Checks& ck) { return zero_ints(st, ck.flag_count, 4); }
+
+int checks_init(hipStream_t st, Work& w, size_t K, Checks& ck) {
+  RTB_TRY(checks_alloc(w, 0, K, 0, ck));
+  return checks_zero(st, ck);
 }
 
 // The k-points of a chunk the mixed check flagged, re-solved in FP64: H by
@@ -920,14 +938,87 @@ static int mesh_setup(hipStream_t st, Work& w, Timer& tmr, const rtb_model* m, c
   return RTB_OK;
 }
 
-static int berry_impl(const rtb_model* m, BerryJob& j) {
+// Device buffers of a Berry-curvature call (besides the direction tables
+// and the mesh intermediates), for chunks of K k-points.
+struct BerryBufs {
+  double *dk = nullptr, *dH = nullptr, *dV = nullptr, *dE = nullptr, *dOm = nullptr, *dJ = nullptr, *dZ = nullptr;
+  double *dMet = nullptr, *dA3 = nullptr, *dOm2 = nullptr, *dKern = nullptr, *kub = nullptr, *dsc = nullptr;
+  double *rproj = nullptr, *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr;
+  double *dmpart = nullptr, *dmsig = nullptr;
+  EigBufs eb{};
+  int total_blk = 0;
+};
+
+static int berry_alloc(Work& w, const rtb_model* m, const BerryJob& j, size_t K, const std::vector<Chunk>& chunks,
+                       bool f32, bool spinj, bool rterm, BerryBufs& b) {
+  const int n = m->nsta;
+  const int nv = j.dipole ? 3 : 2;
+  const size_t n2 = static_cast<size_t>(n) * n;
+  RTB_TRY(w.alloc(&b.dk, K * 3));
+  RTB_TRY(w.alloc(&b.dH, K * n2 * 2));
+  RTB_TRY(w.alloc(&b.dV, K * n2 * 2 * nv));
+  RTB_TRY(w.alloc(&b.dE, K * n));
+  RTB_TRY(w.alloc(&b.dOm, K * n));
+  if (j.geometry) RTB_TRY(w.alloc(&b.dMet, K * n));
+  if (j.dipole) {
+    RTB_TRY(w.alloc(&b.dA3, K * n2 * 2));
+    RTB_TRY(w.alloc(&b.dOm2, K * n));
+    RTB_TRY(w.alloc(&b.dKern, K * n));
+  }
+  RTB_TRY(alloc_eig(w, n, K, true, b.eb));
+  RTB_TRY(w.alloc(&b.kub, K * n2 * 6));  // T1, A, B (complex)
+  if (spinj) RTB_TRY(w.alloc(&b.dJ, K * n2 * 2));
+  if (f32) {
+    RTB_TRY(w.alloc(&b.dZ, K * n2 * 2));  // the refinement needs H after the solve
+    RTB_TRY(w.alloc(&b.dsc, K * n));      // column scales of the refined eigenvectors
+  } else {
+    b.dZ = b.dH;  // H is consumed by the Householder kernel before Z is written
+  }
+  if (rterm) RTB_TRY(w.alloc(&b.rproj, static_cast<size_t>(m->nR) * n2 * 2));
+  b.total_blk = 0;
+  if (j.mode == 2) {
+    for (const Chunk& ch : chunks) b.total_blk += hall_blocks(ch.c);
+    RTB_TRY(w.alloc(&b.dmu, j.n_mu));
+    RTB_TRY(w.alloc(&b.dpart, static_cast<size_t>(b.total_blk) * j.n_mu));
+    RTB_TRY(w.alloc(&b.dsig, j.n_mu));
+    if (j.geometry) {
+      RTB_TRY(w.alloc(&b.dmpart, static_cast<size_t>(b.total_blk) * j.n_mu));
+      RTB_TRY(w.alloc(&b.dmsig, j.n_mu));
+    }
+  }
+  if (j.mode == 1) RTB_TRY(w.alloc(&b.docc, K));
+  return RTB_OK;
+}
+
+// Equal chunks of at most K consecutive k-points (a short last chunk would
+// run its launches under-filled).
+static void equal_chunks(size_t nk, size_t K, std::vector<Chunk>& chunks) {
+  chunks.clear();
+  const size_t nc = (nk + K - 1) / K, Kb = (nk + nc - 1) / nc;
+  for (size_t k0 = 0; k0 < nk; k0 += Kb) chunks.push_back({k0, std::min(Kb, nk - k0), 0, 0});
+}
+
+// Where a Berry-curvature call runs: a leased workspace slot (grown as
+// needed), or a fixed device buffer (the allocation-free entry point: the
+// dense Fourier sum, chunks sized so that every buffer of the call fits).
+struct Exec {
+  hipStream_t st = nullptr;
+  Work* w = nullptr;
+  Timer* tmr = nullptr;
+  bool fixed = false;
+  size_t cap = 0;  // fixed: bytes of the buffer
+};
+
+// k-points at a time of the FP64 re-solve buffers reserved up front in a
+// fixed buffer
+constexpr size_t kFixedResolve = 16;
+
+static int berry_run(const rtb_model* m, BerryJob& j, Exec& ex) {
   const int n = m->nsta;
   if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
-  Lease lease;
-  RTB_TRY(lease.status());
-  hipStream_t st = lease.stream();
-  Work w(lease);
-  Timer tmr(lease);
+  hipStream_t st = ex.st;
+  Work& w = *ex.w;
+  Timer& tmr = *ex.tmr;
   FourierArgs a;
   RTB_TRY(upload_dirs(st, w, m, j.dir, j.dipole ? 3 : 2, a));
   a.gauge = RTB_GAUGE_ATOM;  // berry_curvature_at_impl: gen_v_projected(k, Gauge::Atom, ..)
@@ -944,61 +1035,55 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const size_t per_k = 24 + n2 * 16 + n2 * 16 * nv + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 +
                        (f32 ? n2 * 16 + n * 8 : 0) + (j.dipole ? n2 * 16 + n * 24 : 0);
-  const size_t K = chunk_for(j.nk, per_k);
   MeshPlan mp;
   double* G = nullptr;
   std::vector<Chunk> chunks;
-  // the separable mesh passes form two projected velocities; the dipole's
-  // three take the dense sum over mesh points
-  RTB_TRY(mesh_setup(st, w, tmr, m, j.k || j.dipole || rterm ? nullptr : j.kmesh, j.precision, j.nk, K, true, mp,
-                     &G, chunks));
-  double *dk, *dH, *dV, *dE, *dOm, *dJ = nullptr, *dZ;
-  EigBufs eb;
-  RTB_TRY(w.alloc(&dk, K * 3));
-  RTB_TRY(w.alloc(&dH, K * n2 * 2));
-  RTB_TRY(w.alloc(&dV, K * n2 * 2 * nv));
-  RTB_TRY(w.alloc(&dE, K * n));
-  RTB_TRY(w.alloc(&dOm, K * n));
-  double* dMet = nullptr;
-  if (j.geometry) RTB_TRY(w.alloc(&dMet, K * n));
-  double *dA3 = nullptr, *dOm2 = nullptr, *dKern = nullptr;
-  if (j.dipole) {
-    RTB_TRY(w.alloc(&dA3, K * n2 * 2));
-    RTB_TRY(w.alloc(&dOm2, K * n));
-    RTB_TRY(w.alloc(&dKern, K * n));
-  }
-  RTB_TRY(alloc_eig(w, n, K, true, eb));
-  double* kub;
-  RTB_TRY(w.alloc(&kub, K * n2 * 6));  // T1, A, B (complex)
-  if (spinj) RTB_TRY(w.alloc(&dJ, K * n2 * 2));
-  double* dsc = nullptr;  // column scales of the refined eigenvectors
-  if (f32) {
-    RTB_TRY(w.alloc(&dZ, K * n2 * 2));  // the refinement needs H after the solve
-    RTB_TRY(w.alloc(&dsc, K * n));
-  } else {
-    dZ = dH;  // H is consumed by the Householder kernel before Z is written
-  }
-  double* T1 = kub;
-  double* A = kub + K * n2 * 2;
-  double* B = kub + K * n2 * 4;
-  double* rproj = nullptr;
-  if (rterm) RTB_TRY(w.alloc(&rproj, static_cast<size_t>(m->nR) * n2 * 2));
-  double *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr, *dmpart = nullptr, *dmsig = nullptr;
-  int total_blk = 0;
-  if (j.mode == 2) {
-    for (const Chunk& ch : chunks) total_blk += hall_blocks(ch.c);
-    RTB_TRY(w.alloc(&dmu, j.n_mu));
-    RTB_TRY(w.alloc(&dpart, static_cast<size_t>(total_blk) * j.n_mu));
-    RTB_TRY(w.alloc(&dsig, j.n_mu));
-    if (j.geometry) {
-      RTB_TRY(w.alloc(&dmpart, static_cast<size_t>(total_blk) * j.n_mu));
-      RTB_TRY(w.alloc(&dmsig, j.n_mu));
-    }
-    RTB_HIP(hipMemcpyAsync(dmu, j.mu, j.n_mu * sizeof(double), hipMemcpyHostToDevice, st));
-  }
-  if (j.mode == 1) RTB_TRY(w.alloc(&docc, K));
+  BerryBufs bb;
   Checks ck;
-  RTB_TRY(checks_init(st, w, K, ck));
+  size_t K = 0;
+  if (!ex.fixed) {
+    K = chunk_for(j.nk, per_k);
+    // the separable mesh passes form two projected velocities; the dipole's
+    // three take the dense sum over mesh points
+    RTB_TRY(mesh_setup(st, w, tmr, m, j.k || j.dipole || rterm ? nullptr : j.kmesh, j.precision, j.nk, K, true, mp,
+                       &G, chunks));
+    RTB_TRY(berry_alloc(w, m, j, K, chunks, f32, spinj, rterm, bb));
+    RTB_TRY(checks_alloc(w, n, K, 0, ck));
+  } else {
+    // the largest chunk for which every buffer of the call fits what is left
+    const size_t left = ex.cap > w.used() ? ex.cap - w.used() : 0;
+    K = std::max<size_t>(1, std::min(j.nk, left / per_k));
+    for (;;) {
+      equal_chunks(j.nk, K, chunks);
+      Work probe(nullptr, SIZE_MAX);
+      BerryBufs pb;
+      Checks pc;
+      RTB_TRY(berry_alloc(probe, m, j, K, chunks, f32, spinj, rterm, pb));
+      RTB_TRY(checks_alloc(probe, n, K, f32 ? std::min(K, kFixedResolve) : 0, pc));
+      if (probe.used() <= left) break;
+      if (K == 1) {
+        set_error("the allocation-free call needs %zu bytes of device buffers for this model, above the %zu "
+                  "bytes left of its fixed %zu-byte scratch",
+                  probe.used(), left, ex.cap);
+        return RTB_ERR_UNSUPPORTED;
+      }
+      K = std::max<size_t>(1, static_cast<size_t>(static_cast<double>(K) * static_cast<double>(left) /
+                                                  static_cast<double>(probe.used()) * 0.98));
+    }
+    RTB_TRY(berry_alloc(w, m, j, K, chunks, f32, spinj, rterm, bb));
+    RTB_TRY(checks_alloc(w, n, K, f32 ? std::min(K, kFixedResolve) : 0, ck));
+  }
+  RTB_TRY(checks_zero(st, ck));
+  double *dk = bb.dk, *dH = bb.dH, *dV = bb.dV, *dE = bb.dE, *dOm = bb.dOm, *dJ = bb.dJ, *dZ = bb.dZ;
+  double *dMet = bb.dMet, *dA3 = bb.dA3, *dOm2 = bb.dOm2, *dKern = bb.dKern, *dsc = bb.dsc, *rproj = bb.rproj;
+  double *dmu = bb.dmu, *dpart = bb.dpart, *dsig = bb.dsig, *docc = bb.docc, *dmpart = bb.dmpart;
+  double* dmsig = bb.dmsig;
+  EigBufs& eb = bb.eb;
+  const int total_blk = bb.total_blk;
+  double* T1 = bb.kub;
+  double* A = bb.kub + K * n2 * 2;
+  double* B = bb.kub + K * n2 * 4;
+  if (j.mode == 2) RTB_HIP(hipMemcpyAsync(dmu, j.mu, j.n_mu * sizeof(double), hipMemcpyHostToDevice, st));
   std::vector<double> kp;
   int blk_off = 0;
   for (const Chunk& ch : chunks) {
@@ -1115,6 +1200,18 @@ static int berry_impl(const rtb_model* m, BerryJob& j) {
   return RTB_OK;
 }
 
+static int berry_impl(const rtb_model* m, BerryJob& j) {
+  Lease lease;
+  RTB_TRY(lease.status());
+  Work w(lease);
+  Timer tmr(lease);
+  Exec ex;
+  ex.st = lease.stream();
+  ex.w = &w;
+  ex.tmr = &tmr;
+  return berry_run(m, j, ex);
+}
+
 int rtb_berry_curvature(const rtb_model* m, const double* k, size_t nk, const double* dir, int spin_dir, double eta,
                         int precision, double* omega, double* E) {
   RTB_TRY(prepare(m));
@@ -1164,11 +1261,13 @@ int rtb_occupied_berry_curvature(const rtb_model* m, const double* k, size_t nk,
   return berry_impl(m, j);
 }
 
-int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T, double eta,
-                    const double* dir, int spin_dir, int precision, double* sigma) {
-  RTB_TRY(prepare(m));
+}  // extern "C"
+
+// The job of hall_conductivity, Integration::Direct, after its validation:
+// validate_rank2 (config.rs:141-147) then the spin check (linear/mod.rs:162-165).
+static int hall_direct_job(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
+                           double eta, const double* dir, int spin_dir, int precision, double* sigma, BerryJob& j) {
   RTB_TRY(check_precision(precision));
-  // validate_rank2 (config.rs:141-147) then the spin check (linear/mod.rs:162-165)
   size_t nk = 0;
   RTB_TRY(check_kmesh(m, kmesh, &nk));
   RTB_TRY(check_direction(m, dir));
@@ -1177,7 +1276,6 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
   RTB_TRY(check_T(T));
   RTB_TRY(check_spin(m, spin_dir));
   if (!sigma) return fail(RTB_ERR_INVALID_ARGUMENT, "null sigma");
-  BerryJob j;
   j.kmesh = kmesh;
   j.nk = nk;
   j.dir = dir;
@@ -1189,7 +1287,64 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
   j.n_mu = n_mu;
   j.sigma = sigma;
   j.precision = precision;
+  return RTB_OK;
+}
+
+// Device buffers of the allocation-free entry point: a static device array,
+// one copy per GPU the library runs on, held until the process exits.  Host
+// code never uses the array's name as an address: each device's copy has its
+// own address, taken once per device with hipGetSymbolAddress.
+constexpr size_t kNoAllocBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// the device address of `scratch` on device `dev` (the current device)
+int scratch_address(int dev, void** out) {
+  if (dev < 0 || dev >= 64) return fail(RTB_ERR_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress of the allocation-free scratch failed: %s", hipGetErrorString(e));
+      return RTB_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return RTB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T, double eta,
+                    const double* dir, int spin_dir, int precision, double* sigma) {
+  RTB_TRY(prepare(m));
+  BerryJob j;
+  RTB_TRY(hall_direct_job(m, kmesh, mu, n_mu, T, eta, dir, spin_dir, precision, sigma, j));
   return berry_impl(m, j);
+}
+
+int rtb_hall_direct_noalloc(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
+                            double eta, const double* dir, int spin_dir, int precision, double* sigma) {
+  RTB_TRY(prepare(m));
+  BerryJob j;
+  RTB_TRY(hall_direct_job(m, kmesh, mu, n_mu, T, eta, dir, spin_dir, precision, sigma, j));
+  void* base = nullptr;
+  RTB_TRY(scratch_address(m->device, &base));
+  Work w(base, kNoAllocBytes);
+  Timer tmr;  // no timing events
+  Exec ex;
+  ex.st = nullptr;  // the device's null stream
+  ex.w = &w;
+  ex.tmr = &tmr;
+  ex.fixed = true;
+  ex.cap = kNoAllocBytes;
+  return berry_run(m, j, ex);
 }
 
 int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T, double eta,

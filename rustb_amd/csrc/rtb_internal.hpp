@@ -65,7 +65,8 @@ class Lease {
 
 // A call's device buffers, carved with a bump pointer (256-byte aligned)
 // from a leased slot's block (which grows when a call needs more) or from a
-// fixed buffer (no growth: a request beyond it is RTB_ERR_UNSUPPORTED).
+// fixed buffer (no growth: a request beyond it is RTB_ERR_UNSUPPORTED).  A
+// fixed buffer with a null base only counts the bytes requested (sizing).
 class Work {
  public:
   explicit Work(Lease& l);

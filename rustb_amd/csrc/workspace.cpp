@@ -147,7 +147,7 @@ int Work::take(size_t bytes, void** out) {
     cap_ = sz;
     off_ = 0;
   }
-  *out = base_ + off_;
+  *out = base_ ? base_ + off_ : nullptr;  // no base: only counting bytes
   off_ += bytes;
   if (slot_) slot_->need += bytes;
   used_ += bytes;
