@@ -299,7 +299,8 @@ RTB_HD void backtrans_reg(Team& tm, int n, const T* Y, int ldy, const T* twist, 
           const int k = t + S * q;
           if (k >= 1) {
             const int kc = k < n ? k : n - 1;
-            const T nv = -es[kc - 1] * recip(Dm[static_cast<size_t>(kc) * ldy]) * v;
+            const int ke = kc >= 1 ? kc - 1 : 0;  // in range also for n = 1 (value unused there)
+            const T nv = -es[ke] * recip(Dm[static_cast<size_t>(kc) * ldy]) * v;
             const bool in = r >= 0 && k > r && k < n;
             v = in ? nv : v;
             nrm2 += in ? nv * nv : T(0);

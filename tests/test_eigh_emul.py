@@ -132,3 +132,22 @@ def test_reg_reads_lower_triangle_and_real_diagonal(emul, n, fp32):
     tol = 4e-15 * scale * n if not fp32 else 5e-6 * scale
     assert np.abs(E - w).max() <= tol
     assert np.abs(low @ Z - Z * E).max() <= tol
+
+
+@pytest.mark.parametrize("san", [["-fsanitize=address,undefined", "-fno-sanitize-recover=all"],
+                                 ["-fsanitize=thread"]], ids=["asan_ubsan", "tsan"])
+def test_emulation_under_sanitizers(tmp_path, san):
+    """The eigensolver workgroups and the cluster Rayleigh-Ritz (the device
+    source run by host threads) under AddressSanitizer + UBSan and under
+    ThreadSanitizer: tests/native/sanitize_main.cpp checks residuals and
+    orthonormality itself; any sanitizer report fails the run."""
+    exe = str(tmp_path / "sanitize_main")
+    src = os.path.join(HERE, "native", "sanitize_main.cpp")
+    subprocess.run(["g++", "-std=c++20", "-O1", "-g", "-fno-omit-frame-pointer", *san, src, "-o", exe, "-lpthread"],
+                   check=True)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", TSAN_OPTIONS="halt_on_error=1",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    env.pop("LD_PRELOAD", None)
+    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "FAIL" not in r.stdout and "Sanitizer" not in r.stderr, r.stderr[-3000:]
