@@ -24,7 +24,10 @@ int g_smem_rc = RTB_OK;
 int prepare(const rtb_model* m) {
   if (!m) return fail(RTB_ERR_INVALID_ARGUMENT, "null model");
   RTB_HIP(hipSetDevice(m->device));
-  std::call_once(g_smem_once, [] { g_smem_rc = mesh_set_smem_limits(); });
+  std::call_once(g_smem_once, [] {
+    g_smem_rc = mesh_set_smem_limits();
+    if (g_smem_rc == RTB_OK) g_smem_rc = eigh_set_smem_limits();
+  });
   return g_smem_rc;
 }
 
@@ -165,15 +168,24 @@ size_t chunk_for(size_t nk, size_t per_k_bytes) {
 // The eigen + Kubo pipeline for one chunk of k-points whose H (and V when
 // omega is wanted) are already on the device.
 struct EigBufs {
-  double *Vg, *tau, *Y, *S, *DE;
+  double *Vg, *tau, *Y, *S, *DE, *band;
 };
+
+// Bytes per k-point of alloc_eig's buffers beyond those of n <= 128 (the
+// two-stage route's reflector store, and its matrix when no vectors are
+// asked for).
+size_t eig_extra_bytes(int n, bool vectors) {
+  if (n <= kMaxEighN) return 0;
+  return eigh_band_elems(n) * 8 + (vectors ? 0 : static_cast<size_t>(n) * n * 16);
+}
 
 int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
   const size_t n2 = static_cast<size_t>(n) * n;
-  b.Vg = b.tau = nullptr;
+  b.Vg = b.tau = b.band = nullptr;
   RTB_TRY(w.alloc(&b.DE, 3 * K * n));
+  if (vectors || n > kMaxEighN) RTB_TRY(w.alloc(&b.Vg, K * n2 * 2));
+  if (n > kMaxEighN) RTB_TRY(w.alloc(&b.band, K * eigh_band_elems(n)));
   if (vectors) {
-    RTB_TRY(w.alloc(&b.Vg, K * n2 * 2));
     RTB_TRY(w.alloc(&b.tau, K * n * 2));
     RTB_TRY(w.alloc(&b.Y, K * n2));
     RTB_TRY(w.alloc(&b.S, K * n2 * 3));
@@ -300,9 +312,10 @@ int resolve_flagged(hipStream_t st, Work& w, Timer& tmr, const rtb_model* m, con
     RTB_TRY(fourier_dense(st, a0, ck.dk, bc, ck.H, nullptr));
     RTB_TRY(tmr.end(1));
     RTB_TRY(eigh_launch_fp64(st, &tmr, n, bc, ck.H, ck.E, true, ck.eb.Vg, ck.eb.tau, ck.eb.Y, ck.eb.S, ck.eb.DE,
-                             from_upper));
+                             from_upper, ck.eb.band));
     RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-    RTB_TRY(eigh_backtransform_fp64(st, n, bc, ck.eb.Y, ck.eb.DE, ck.eb.S, ck.eb.Vg, ck.eb.tau, ck.Z, false));
+    RTB_TRY(eigh_backtransform_fp64(st, n, bc, ck.eb.Y, ck.eb.DE, ck.eb.S, ck.eb.Vg, ck.eb.tau, ck.Z, false,
+                                    ck.eb.band));
     RTB_TRY(tmr.end(1, flop_backtrans(n, bc)));
     RTB_TRY(tmr.begin(RTB_STAGE_REFINE));
     RTB_TRY(count_nonfinite(st, n, bc, ck.E, ck.nonfinite));
@@ -347,10 +360,11 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, bool from_upper, int n, size_t c
   float* Y = reinterpret_cast<float*>(eb.Y);
   float* S = reinterpret_cast<float*>(eb.S);
   float* DE = reinterpret_cast<float*>(eb.DE);
-  RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE, from_upper));
+  float* band = reinterpret_cast<float*>(eb.band);
+  RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE, from_upper, band));
   RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
   float* Zf = reinterpret_cast<float*>(Z);  // exact: the route's values are FP32
-  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Zf, false, scale));  // |z_j|^2 into scale
+  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Zf, false, scale, band));  // |z_j|^2 into scale
   RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
   // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
   // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius
@@ -748,7 +762,7 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
   if (nk > 0 && (!k || !E)) return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
   if (nk == 0) return RTB_OK;
   const int n = m->nsta;
-  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  if (n > kMaxN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 256 is not supported by the eigensolver");
   const bool vec = evec != nullptr;
   Lease lease;
   RTB_TRY(lease.status());
@@ -762,7 +776,8 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
   // the mixed form runs the FP32 route (with FP64 refinement) when
   // eigenvectors are requested; eigenvalue-only solves stay FP64
   const bool f32 = vec && precision == RTB_PREC_MIXED;
-  const size_t per_k = 24 + n2 * 16 * 2 + n * 32 + (vec ? n2 * 32 + n2 * 16 : 0) + (f32 ? n2 * 32 + n * 8 : 0);
+  const size_t per_k = 24 + n2 * 16 * 2 + n * 32 + (vec ? n2 * 32 + n2 * 16 : 0) + (f32 ? n2 * 32 + n * 8 : 0) +
+                       eig_extra_bytes(n, vec);
   const size_t K = chunk_for(nk, per_k);
   double *dk, *dH, *dE, *dZ = nullptr, *dZc = nullptr, *dHZ = nullptr, *dsc = nullptr;
   EigBufs eb;
@@ -801,11 +816,11 @@ static int solve_impl(const rtb_model* m, const double* k, size_t nk, int precis
       RTB_TRY(tmr.end(1));
     } else {
       RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, vec, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE,
-                               vec && !m->herm && !lower_vectors));
+                               vec && !m->herm && !lower_vectors, eb.band));
       RTB_TRY(count_nonfinite(st, n, c, dE, ck.nonfinite));
       if (vec) {
         RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-        RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, true));
+        RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, true, eb.band));
         RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
       }
     }
@@ -1015,7 +1030,7 @@ constexpr size_t kFixedResolve = 16;
 
 static int berry_run(const rtb_model* m, BerryJob& j, Exec& ex) {
   const int n = m->nsta;
-  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  if (n > kMaxN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 256 is not supported by the eigensolver");
   hipStream_t st = ex.st;
   Work& w = *ex.w;
   Timer& tmr = *ex.tmr;
@@ -1034,7 +1049,7 @@ static int berry_run(const rtb_model* m, BerryJob& j, Exec& ex) {
   a.combine = spinj || exact_kubo ? 0 : 1;
   // H/Z, V(2), Vg, Y+S aliased with T1/A/B, J
   const size_t per_k = 24 + n2 * 16 + n2 * 16 * nv + n2 * 16 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 +
-                       (f32 ? n2 * 16 + n * 8 : 0) + (j.dipole ? n2 * 16 + n * 24 : 0);
+                       (f32 ? n2 * 16 + n * 8 : 0) + (j.dipole ? n2 * 16 + n * 24 : 0) + eig_extra_bytes(n, true);
   MeshPlan mp;
   double* G = nullptr;
   std::vector<Chunk> chunks;
@@ -1142,10 +1157,10 @@ static int berry_run(const rtb_model* m, BerryJob& j, Exec& ex) {
       };
       RTB_TRY(resolve_flagged(st, w, tmr, m, a, !m->herm, c, kof, ck, dE, Zc, dsc));
     } else {
-      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm));
+      RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm, eb.band));
       RTB_TRY(count_nonfinite(st, n, c, dE, ck.nonfinite));
       RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, false));
+      RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, dZ, false, eb.band));
       RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     }
     if (j.dipole)
@@ -1368,7 +1383,7 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   if (m->dim == 3) return fail(RTB_ERR_UNSUPPORTED, "EnergyCut integration is implemented for 2D models only");
   if (!sigma) return fail(RTB_ERR_INVALID_ARGUMENT, "null sigma");
   const int n = m->nsta;
-  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  if (n > kMaxN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 256 is not supported by the eigensolver");
   const size_t nx = kmesh[0], ny = kmesh[1];
   const size_t n2 = static_cast<size_t>(n) * n;
   Lease lease;
@@ -1391,7 +1406,8 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   RTB_TRY(w.alloc(&dEv, nk * n));
   if (nx > 1) RTB_TRY(w.alloc(&dOx, (nx - 1) * ny * n2));
   if (nk > 1) RTB_TRY(w.alloc(&dOy, (nk - 1) * n2));
-  const size_t per_k = 24 + n2 * 16 * 3 + n2 * 48 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56;
+  const size_t per_k = 24 + n2 * 16 * 3 + n2 * 48 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 +
+                       eig_extra_bytes(n, true);
   const size_t K = chunk_for(nk, per_k);
   double *dk, *dH, *dV, *kub, *dJ = nullptr, *rproj = nullptr;
   EigBufs eb;
@@ -1417,10 +1433,11 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
     RTB_TRY(tmr.end(rterm ? 3 : 2));
     // hamk.eigh(UPLO::Lower) (primitives.rs:87): the upper triangle of a
     // storage that is not exactly Hermitian
-    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dEv + k0 * n, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm));
+    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dEv + k0 * n, true, eb.Vg, eb.tau, eb.Y, eb.S, eb.DE, !m->herm,
+                             eb.band));
     RTB_TRY(count_nonfinite(st, n, c, dEv + k0 * n, ck.nonfinite));
     RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
-    RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, Zc, false));
+    RTB_TRY(eigh_backtransform_fp64(st, n, c, eb.Y, eb.DE, eb.S, eb.Vg, eb.tau, Zc, false, eb.band));
     RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
     // v^a = U^H J_a U, v^b = U^H v_b U (the reference's evec^T v evec^* with
     // evec = conj(U)), the spin current on the first direction only
@@ -1588,7 +1605,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   RTB_TRY(check_kmesh(m, kmesh, &nk));
   if (!E || !rho) return fail(RTB_ERR_INVALID_ARGUMENT, "null output");
   const int n = m->nsta;
-  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 128 is not supported by the eigensolver");
+  if (n > kMaxN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 256 is not supported by the eigensolver");
   const double step = En > 1 ? (Emax - Emin) / static_cast<double>(En - 1) : 0.0;
   for (size_t i = 0; i < En; ++i) E[i] = Emin + step * static_cast<double>(i);
   Lease lease;
@@ -1600,7 +1617,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
   RTB_TRY(upload_dirs(st, w, m, nullptr, 0, a));
   a.gauge = RTB_GAUGE_ATOM;
   const size_t n2 = static_cast<size_t>(n) * n;
-  const size_t K = chunk_for(nk, 24 + n2 * 32 + n * 32);
+  const size_t K = chunk_for(nk, 24 + n2 * 32 + n * 32 + eig_extra_bytes(n, false));
   MeshPlan mp;
   double* G = nullptr;
   std::vector<Chunk> chunks;
@@ -1633,7 +1650,7 @@ int rtb_dos(const rtb_model* m, const size_t* kmesh, double Emin, double Emax, s
       RTB_TRY(fourier_dense(st, a, dk, c, dH, nullptr));
       RTB_TRY(tmr.end(1));
     }
-    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, false, nullptr, nullptr, nullptr, nullptr, eb.DE, false));
+    RTB_TRY(eigh_launch_fp64(st, &tmr, n, c, dH, dE, false, eb.Vg, nullptr, nullptr, nullptr, eb.DE, false, eb.band));
     RTB_TRY(count_nonfinite(st, n, c, dE, ck.nonfinite));
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
     const int nb = dos_blocks(c * n);

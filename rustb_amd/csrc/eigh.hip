@@ -15,6 +15,7 @@
 
 #include "eigh.hpp"
 
+#include "eigh_band.hpp"
 #include "eigh_core.hpp"
 #include "eigh_reg.hpp"
 #include "gpu_team.hpp"
@@ -117,6 +118,97 @@ void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, cons
   } else {
     launch_backtrans_reg<T, 128, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out, gdiag);
   }
+}
+
+// ---- two-stage route (128 < n <= 256), see eigh_band.hpp ----
+constexpr int kBandReduceThreads = 256;
+constexpr int kBandChaseThreads = 64;
+
+template <typename T>
+size_t band_reduce_smem(int n) {
+  const size_t pan = static_cast<size_t>(n - kBandB) * kBandB * sizeof(cplx<T>);
+  return 2 * pan + 3 * kBandB * kBandB * sizeof(cplx<T>) + kBandB * sizeof(cplx<T>) +
+         2 * GpuTeam::kRedStride * sizeof(T);
+}
+
+// Dense -> band, one workgroup per matrix (matrix in global memory).
+template <typename T>
+__global__ void __launch_bounds__(kBandReduceThreads) band_reduce_kernel(int n, const double* __restrict__ H,
+                                                                        cplx<T>* __restrict__ A,
+                                                                        cplx<T>* __restrict__ band) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  GpuTeam tm;
+  const size_t mat = blockIdx.x;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const BandLayout L = band_layout(n);
+  cplx<T>* base = reinterpret_cast<cplx<T>*>(smem_raw);
+  const size_t pan = static_cast<size_t>(n - kBandB) * kBandB;
+  BandReduceSmem<T> sm;
+  sm.V = base;
+  sm.Y = base + pan;
+  sm.Tm = sm.Y + pan;
+  sm.W = sm.Tm + kBandB * kBandB;
+  sm.M = sm.W + kBandB * kBandB;
+  sm.tau = sm.M + kBandB * kBandB;
+  sm.red = reinterpret_cast<T*>(sm.tau + kBandB);
+  cplx<T>* bm = band + mat * L.total;
+  band_reduce<T>(tm, n, H + mat * nn * 2, A + mat * nn, bm + L.v1, bm + L.t1, sm);
+}
+
+// Band -> tridiagonal, one wave per matrix.
+template <typename T>
+__global__ void __launch_bounds__(kBandChaseThreads) band_chase_kernel(int n, cplx<T>* __restrict__ A,
+                                                                      T* __restrict__ Dg, T* __restrict__ Eg,
+                                                                      cplx<T>* __restrict__ band) {
+  __shared__ cplx<T> hv[kBandB];
+  __shared__ T red[2 * GpuTeam::kRedStride];
+  GpuTeam tm;
+  const size_t mat = blockIdx.x;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const BandLayout L = band_layout(n);
+  cplx<T>* bm = band + mat * L.total;
+  band_chase<T>(tm, n, A + mat * nn, Dg + mat * n, Eg + mat * n, bm + L.v2, bm + L.tau2, hv, red);
+}
+
+// The eigenvectors of T of the isolated eigenvalues written into Y (the
+// cluster members are there already): one thread per (matrix, column).
+template <typename T>
+__global__ void __launch_bounds__(256) twisted_columns_kernel(int n, size_t nmat, const T* __restrict__ Eg,
+                                                             const T* __restrict__ S, const T* __restrict__ TW,
+                                                             T* __restrict__ Y) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= nmat * n) return;
+  const size_t mat = g / n;
+  const int j = static_cast<int>(g % n);
+  const size_t nn = static_cast<size_t>(n) * n;
+  twisted_column<T>(n, Eg + mat * n, S + mat * nn * 3, n, j, TW[mat * n + j], Y + mat * nn);
+}
+
+// Z = Q1 Q2 Y for W columns per workgroup (W * S threads, Y block in LDS).
+template <typename T>
+struct BandBt;
+template <> struct BandBt<float> { static constexpr int W = 64, S = 8; };
+template <> struct BandBt<double> { static constexpr int W = 32, S = 8; };
+
+template <typename T, typename TO>
+__global__ void __launch_bounds__(BandBt<T>::W * BandBt<T>::S) band_backtrans_kernel(
+    int n, const T* __restrict__ Y, const cplx<T>* __restrict__ band, TO* __restrict__ Z, int rows_out,
+    double* __restrict__ gdiag) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int W = BandBt<T>::W, S = BandBt<T>::S;
+  GpuTeam tm;
+  const size_t mat = blockIdx.y;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const BandLayout L = band_layout(n);
+  const cplx<T>* bm = band + mat * L.total;
+  band_backtrans<T, W, S>(tm, n, blockIdx.x * W, Y + mat * nn, n, bm + L.v1, bm + L.t1, bm + L.v2, bm + L.tau2,
+                          reinterpret_cast<cplx<T>*>(smem_raw), Z + mat * nn * 2, rows_out,
+                          gdiag ? gdiag + mat * n : nullptr);
+}
+
+template <typename T>
+size_t band_backtrans_smem(int n) {
+  return static_cast<size_t>(n) * BandBt<T>::W * sizeof(cplx<T>);
 }
 
 constexpr int kTeBlock = 256;
@@ -262,8 +354,9 @@ int hermitize_from_upper(hipStream_t st, int n, size_t nmat, double* H) {
 
 template <typename T>
 int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors, T* Vg,
-                T* taug, T* Y, T* S, T* DE, bool from_upper) {
-  if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 128");
+                T* taug, T* Y, T* S, T* DE, bool from_upper, T* band) {
+  if (n < 1 || n > kMaxBandN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 256");
+  if (n > kMaxEighN && !band) return fail(RTB_ERR_INVALID_ARGUMENT, "nsta above 128 needs the band buffers");
   if (nmat == 0) return RTB_OK;
   if (from_upper) {
     if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REDUCE));
@@ -274,21 +367,55 @@ int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, doubl
   T* Eg = DE + nmat * n;
   T* TW = DE + 2 * nmat * n;
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIDIAG));
-  tridiag_reg_dispatch<T>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<T>*>(Vg), reinterpret_cast<cplx<T>*>(taug));
-  RTB_HIP(hipGetLastError());
-  if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
+  if (n <= kMaxEighN) {
+    tridiag_reg_dispatch<T>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<T>*>(Vg),
+                            reinterpret_cast<cplx<T>*>(taug));
+    RTB_HIP(hipGetLastError());
+    if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
+  } else {
+    // the matrix itself is reduced in the Vg buffer (n x n complex)
+    cplx<T>* A = reinterpret_cast<cplx<T>*>(Vg);
+    cplx<T>* bnd = reinterpret_cast<cplx<T>*>(band);
+    hipLaunchKernelGGL((band_reduce_kernel<T>), dim3(static_cast<unsigned>(nmat)), dim3(kBandReduceThreads),
+                       band_reduce_smem<T>(n), st, n, H, A, bnd);
+    RTB_HIP(hipGetLastError());
+    hipLaunchKernelGGL((band_chase_kernel<T>), dim3(static_cast<unsigned>(nmat)), dim3(kBandChaseThreads), 0, st,
+                       n, A, Dg, Eg, bnd);
+    RTB_HIP(hipGetLastError());
+    if (tmr) RTB_TRY(tmr->end(2, flop_tridiag(n, nmat)));
+  }
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIEIG));
   RTB_TRY(launch_trieig<T>(st, n, nmat, Dg, Eg, E, vectors, Y, S, TW));
-  if (tmr) RTB_TRY(tmr->end(vectors ? 2 : 1));
+  if (vectors && n > kMaxEighN) {
+    const size_t tot = nmat * n;
+    hipLaunchKernelGGL((twisted_columns_kernel<T>), dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0,
+                       st, n, nmat, Eg, S, TW, Y);
+    RTB_HIP(hipGetLastError());
+  }
+  if (tmr) RTB_TRY(tmr->end(vectors ? (n > kMaxEighN ? 3 : 2) : 1));
   return RTB_OK;
 }
 
 template <typename T, typename TO>
 int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* S, const T* Vg,
-                       const T* taug, TO* Z, bool rows_out, double* gdiag) {
+                       const T* taug, TO* Z, bool rows_out, double* gdiag, const T* band) {
   if (nmat == 0) return RTB_OK;
-  backtrans_reg_dispatch<T, TO>(st, n, nmat, Y, DE, S, reinterpret_cast<const cplx<T>*>(Vg),
-                                reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0, gdiag);
+  if (n <= kMaxEighN) {
+    backtrans_reg_dispatch<T, TO>(st, n, nmat, Y, DE, S, reinterpret_cast<const cplx<T>*>(Vg),
+                                  reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0, gdiag);
+  } else {
+    if (!band) return fail(RTB_ERR_INVALID_ARGUMENT, "nsta above 128 needs the band buffers");
+    constexpr int W = BandBt<T>::W, S2 = BandBt<T>::S;
+    const unsigned gx = static_cast<unsigned>((n + W - 1) / W);
+    for (size_t m0 = 0; m0 < nmat; m0 += 65535) {
+      const size_t cnt = std::min<size_t>(65535, nmat - m0);
+      const size_t nn = static_cast<size_t>(n) * n;
+      hipLaunchKernelGGL((band_backtrans_kernel<T, TO>), dim3(gx, static_cast<unsigned>(cnt)), dim3(W * S2),
+                         band_backtrans_smem<T>(n), st, n, Y + m0 * nn,
+                         reinterpret_cast<const cplx<T>*>(band) + m0 * band_layout(n).total, Z + m0 * nn * 2,
+                         rows_out ? 1 : 0, gdiag ? gdiag + m0 * n : nullptr);
+    }
+  }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
@@ -296,23 +423,42 @@ int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* 
 }  // namespace
 
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper) {
-  return eigh_launch<float>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper);
+                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper, float* band) {
+  return eigh_launch<float>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper, band);
 }
 
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
-                            const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag) {
-  return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, gdiag);
+                            const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag,
+                            const float* band) {
+  return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, gdiag, band);
 }
 
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper) {
-  return eigh_launch<double>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper);
+                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper, double* band) {
+  return eigh_launch<double>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper, band);
 }
 
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
-                            const double* Vg, const double* taug, double* Z, bool rows_out) {
-  return eigh_backtransform<double, double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, nullptr);
+                            const double* Vg, const double* taug, double* Z, bool rows_out, const double* band) {
+  return eigh_backtransform<double, double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, nullptr, band);
+}
+
+size_t eigh_band_elems(int n) { return n > kMaxEighN ? band_layout(n).total * 2 : 0; }
+
+int eigh_set_smem_limits() {
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&band_reduce_kernel<float>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(band_reduce_smem<float>(kMaxBandN))));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&band_reduce_kernel<double>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(band_reduce_smem<double>(kMaxBandN))));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&band_backtrans_kernel<float, float>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(band_backtrans_smem<float>(kMaxBandN))));
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&band_backtrans_kernel<double, double>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(band_backtrans_smem<double>(kMaxBandN))));
+  return RTB_OK;
 }
 
 }  // namespace rtb

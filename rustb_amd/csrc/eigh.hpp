@@ -17,15 +17,24 @@ namespace rtb {
 //   taug: [nmat][n] complex; Y: [nmat][n][n]; S: [nmat][3][n][n];
 //   DE: [3][nmat][n] the tridiagonal (diagonal, off-diagonal) and the twist
 //   indices; Vg, taug, Y, S, DE are the inputs of the back-transformation.
+//   n > 128 (two-stage route): Vg also holds the matrix being reduced (also
+//   without vectors), and `band` ([nmat][eigh_band_elems(n)] in T) the
+//   reflectors of both stages; null for n <= 128.
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper);
+                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper, double* band);
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper);
+                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper, float* band);
 // Z: [nmat][n][n] complex FP32 (the values of the FP32 route, exact);
 // gdiag (optional, [nmat][n]): the squared column norms of Z in FP64.
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
-                            const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag);
+                            const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag,
+                            const float* band);
+// Real elements of T per matrix the two-stage route keeps in `band` (0 for
+// n <= 128).
+size_t eigh_band_elems(int n);
+// Raises the dynamic LDS limits of the two-stage route's kernels (once).
+int eigh_set_smem_limits();
 // FP64 refinement of FP32 eigenpairs (refine.hip): one Ogita-Aishima step,
 // Rayleigh quotients, unit columns, Rayleigh-Ritz in clusters with gaps
 // below gap_rel * max(1, spectral radius).  Z: [nk][n][n] FP32 columns (in);
@@ -63,7 +72,7 @@ int zero_ints(hipStream_t st, int* p, size_t count);
 int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, double* out);
 // Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
-                            const double* Vg, const double* taug, double* Z, bool rows_out);
+                            const double* Vg, const double* taug, double* Z, bool rows_out, const double* band);
 
 // ---- Fourier sum (fourier.hip) ----
 struct FourierArgs {

@@ -43,11 +43,39 @@ __device__ inline T wave_sum64(T x) {
   return (lane_value(x, 0) + lane_value(x, 16)) + (lane_value(x, 32) + lane_value(x, 48));
 }
 
+// Sum over the aligned groups of 8 lanes (quad swaps, then the half-row
+// mirror), returned to every lane of the group.
+template <typename T>
+__device__ inline T group_sum8(T x) {
+  x += dpp_mov<0xB1>(x);   // quad_perm [1, 0, 3, 2]
+  x += dpp_mov<0x4E>(x);   // quad_perm [2, 3, 0, 1]
+  x += dpp_mov<0x141>(x);  // row_half_mirror
+  return x;
+}
+
 struct GpuTeam {
   static constexpr int kRedStride = 32;  // >= 2 x the waves of a workgroup
   int id, nt;
   __device__ GpuTeam() : id(threadIdx.x), nt(blockDim.x) {}
   __device__ void sync() { __syncthreads(); }
+  // Orders the LDS accesses of the lanes of one wave (they execute in
+  // lockstep; only the compiler's reordering has to be fenced).
+  __device__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // Sum over the aligned group of G (8 or 16) consecutive lanes, to each of
+  // them; every lane of the wave calls it.
+  template <int G, typename T>
+  __device__ T group_sum(T x) {
+    static_assert(G == 8 || G == 16, "group size");
+    if constexpr (G == 16) {
+      return row_sum16(x);
+    } else {
+      return group_sum8(x);
+    }
+  }
 
   // Sum over the workgroup, returned to every thread, without a trailing
   // barrier: `red` must not be rewritten until every thread has passed

@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(256) kubo_omega_exact_kernel(int n, const doub
                                                                double* __restrict__ metric) {
   __shared__ double2 sa[kOmTile][kOmTile + 1];
   __shared__ double2 sb[kOmTile][kOmTile + 1];
-  __shared__ double se[kMaxEighN];
+  __shared__ double se[kMaxN];
   const size_t k = blockIdx.x;
   const size_t nn = static_cast<size_t>(n) * n;
   A += k * nn;
@@ -262,7 +262,7 @@ int kubo_omega_from_parts(hipStream_t st, int n, size_t nk, const double* part, 
 int kubo_omega_exact(hipStream_t st, int n, size_t nk, const double* A, const double* B, const double* E, double eta,
                      double* omega, double* metric) {
   if (nk == 0) return RTB_OK;
-  if (n < 1 || n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "the band sum supports 1 <= nsta <= 128");
+  if (n < 1 || n > kMaxN) return fail(RTB_ERR_UNSUPPORTED, "the band sum supports 1 <= nsta <= 256");
   if (nk > 0x7fffffffu) return fail(RTB_ERR_UNSUPPORTED, "too many k-points in one launch");
   hipLaunchKernelGGL(kubo_omega_exact_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
                      reinterpret_cast<const double2*>(A), reinterpret_cast<const double2*>(B), E, eta, omega, metric);

@@ -40,9 +40,9 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
                                                         double* __restrict__ E, double* __restrict__ scale,
                                                         double gap_rel, RefineCheck chk) {
   __shared__ C2 lds[7][kMaxCluster][kMaxCluster];
-  __shared__ double2 hzc[kMaxCluster][kMaxEighN];  // refined H z' of the cluster's columns
+  __shared__ double2 hzc[kMaxCluster][kMaxN];  // refined H z' of the cluster's columns
   __shared__ double ev[kMaxCluster];
-  __shared__ double s_res[kMaxEighN];  // residual of unit column j
+  __shared__ double s_res[kMaxN];  // residual of unit column j
   __shared__ int solved;
   const size_t k = blockIdx.x;
   const int lane = threadIdx.x;

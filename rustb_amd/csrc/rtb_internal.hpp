@@ -190,9 +190,11 @@ void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const doub
 void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* Ki, size_t ks, const int* perm,
                          const double* mu, size_t n_mu, double width, double eta, double* out);
 
-// Largest nsta the batched eigensolver handles (packed lower triangle of the
-// FP64 matrix in LDS: n(n+1)/2 * 16 B <= 160 KB).
+// Largest nsta of the register-resident eigensolver route (one workgroup
+// holds the matrix in registers); above it, up to kMaxN, the two-stage route
+// (dense -> band -> tridiagonal) with the matrix in global memory.
 constexpr int kMaxEighN = 128;
+constexpr int kMaxN = 256;
 
 }  // namespace rtb
 

@@ -123,7 +123,7 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
                const SB* __restrict__ B, size_t sB, const SA* D, size_t sD, double2* __restrict__ C,
                size_t sC, const double* __restrict__ sc, OaEpilogue oa, KuboEpilogue kb) {
   typedef typename Mfma<T>::v4 v4;
-  __shared__ double s_lt[kMaxEighN];  // Rayleigh quotients (OA epilogue)
+  __shared__ double s_lt[kMaxN];  // Rayleigh quotients (OA epilogue)
   __shared__ double s_red[4];
   __shared__ double s_rq[2][kTile];  // Kubo column sums by wave row
   __shared__ double s_kr[2][kTile];     // Kubo row sums by wave column
@@ -572,7 +572,7 @@ int zgemm(hipStream_t st, int n, size_t nb, ZOperand A, bool conj_a, ZOperand B,
 
 int zgemm_gram_oa(hipStream_t st, int n, size_t nb, const float* Z, const double* S, const double* gd, float* F,
                   double gap_rel, double* lt) {
-  if (n > kMaxEighN) return fail(RTB_ERR_UNSUPPORTED, "zgemm_gram_oa supports n <= 128");
+  if (n > kMaxN) return fail(RTB_ERR_UNSUPPORTED, "zgemm_gram_oa supports n <= 256");
   const size_t n2 = static_cast<size_t>(n) * n;
   OaEpilogue oa;
   oa.S = reinterpret_cast<const double2*>(S);

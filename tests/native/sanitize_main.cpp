@@ -63,9 +63,10 @@ void check(int n, const std::vector<cd>& H, const std::vector<double>& E, const 
 
 int main() {
   int bad = 0;
-  const int sizes[] = {1, 2, 17, 33, 64, 100, 128};
+  // sizes of each slice shape (32, 64, 128 rows) and the smallest ones
+  const int sizes[2][4] = {{1, 2, 65, 128}, {17, 33, 100, 128}};
   for (int fp32 = 0; fp32 < 2; ++fp32)
-    for (int n : sizes) {
+    for (int n : sizes[fp32]) {
       const std::vector<cd> H = hermitian(n, 1000 + n);
       std::vector<double> E(n);
       std::vector<cd> Z(static_cast<size_t>(n) * n);
@@ -76,6 +77,22 @@ int main() {
       const double tol_r = fp32 ? 2e-5 * n : 1e-13 * n, tol_o = fp32 ? 1e-4 : 1e-13 * n;
       const bool ok = rc == 0 && res <= tol_r && orth <= tol_o;
       std::printf("eigh n=%d fp32=%d residual %.2e orthonormality %.2e %s\n", n, fp32, res, orth, ok ? "ok" : "FAIL");
+      bad += !ok;
+    }
+  // the two-stage route (eigh_band.hpp) at sizes with one, two and several
+  // panels
+  for (int fp32 = 0; fp32 < 2; ++fp32)
+    for (int n : {fp32 ? 33 : 17, 50}) {
+      const std::vector<cd> H = hermitian(n, 2000 + n);
+      std::vector<double> E(n);
+      std::vector<cd> Z(static_cast<size_t>(n) * n);
+      const int rc = emul_eigh_band(n, fp32, reinterpret_cast<const double*>(H.data()), E.data(),
+                                    reinterpret_cast<double*>(Z.data()), 1, 0);
+      double res = 0.0, orth = 0.0;
+      check(n, H, E, Z, res, orth);
+      const double tol_r = fp32 ? 2e-5 * n : 1e-13 * n, tol_o = fp32 ? 1e-4 : 1e-13 * n;
+      const bool ok = rc == 0 && res <= tol_r && orth <= tol_o;
+      std::printf("band n=%d fp32=%d residual %.2e orthonormality %.2e %s\n", n, fp32, res, orth, ok ? "ok" : "FAIL");
       bad += !ok;
     }
   // cluster Rayleigh-Ritz: exact eigenvectors of a matrix with a degenerate
