@@ -57,16 +57,20 @@ ROOFLINE_STAGES = {
     "tridiag": ("eigh_tridiag_reg_kernel", "vector ALU", None),
     "backtransform": ("eigh_backtrans_reg_kernel", "vector ALU", None),
 }
+# above 128 orbitals the eigensolver's stages are the two-stage route's kernels
+BAND_KERNELS = {"tridiag": "band_reduce_kernel + band_chase_kernel", "backtransform": "band_backtrans_kernel"}
 
 
-def traffic_from_profiles(stage):
-    """HBM bytes per launch of the kernels of `stage` measured with rocprofv3
-    --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), as
-    committed under profiles/; None when no such measurement exists."""
+def traffic_from_profiles(stage, workload="wannier3d-n128"):
+    """HBM bytes per launch of the kernels of `stage` on `workload` measured
+    with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel
+    calibration), as committed under profiles/; None when no such
+    measurement exists."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
+    key = "stage:" + stage if workload == "wannier3d-n128" else workload + "/stage:" + stage
     try:
         with open(path) as f:
-            rec = json.load(f).get("stage:" + stage)
+            rec = json.load(f).get(key)
     except (OSError, ValueError):
         return None
     return None if rec is None else rec.get("bytes_per_launch")
@@ -322,7 +326,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-fp64-compare", action="store_true")
-    ap.add_argument("--kmesh", default="48,48,48")
+    ap.add_argument("--workload", default="wannier3d-n128", choices=["wannier3d-n128", "slab2d-n256"],
+                    help="wannier3d-n128: the headline configuration; slab2d-n256: 32 layers x 8 orbitals "
+                         "(n = 256, 49 R) on 200 x 200")
+    ap.add_argument("--kmesh", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ingest", action="store_true",
@@ -351,10 +358,17 @@ def main():
     import rustb_amd as ra
     from rustb_amd import _lib
 
-    kmesh = tuple(int(x) for x in args.kmesh.split(","))
     eta = 1e-4
-    dirs = [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]
-    model = ra.workloads.wannier3d_n128()
+    if args.workload == "slab2d-n256":
+        kmesh = (200, 200)
+        dirs = [[1.0, 0.0], [0.0, 1.0]]
+        model = ra.workloads.slab2d_n256()
+    else:
+        kmesh = (48, 48, 48)
+        dirs = [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]
+        model = ra.workloads.wannier3d_n128()
+    if args.kmesh:
+        kmesh = tuple(int(x) for x in args.kmesh.split(","))
     n, nR = model.nsta, model.hamR.shape[0]
     nk = int(np.prod(kmesh))
     params = ra.Parameters(kmesh=kmesh, direction=np.array(dirs), mu=np.array([0.0]), eta=eta,
@@ -393,6 +407,8 @@ def main():
     dom = max(cand, key=lambda i: stage_ms[i])
     dname = names[dom]
     kernel, unit_name, peak = ROOFLINE_STAGES[dname]
+    if n > 128 and dname in BAND_KERNELS:
+        kernel = BAND_KERNELS[dname]
     f32 = args.precision == "mixed" and dname in ("tridiag", "backtransform")
     if peak is None:  # vector ALU: FP32 in the mixed form, FP64 otherwise
         peak = PEAK_FP32 if f32 else PEAK_FP64
@@ -405,7 +421,7 @@ def main():
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": round(achieved / peak, 4),
-        "traffic": traffic_from_profiles(dname),
+        "traffic": traffic_from_profiles(dname, args.workload),
         "kernel": kernel,
         "stage": dname,
         "arithmetic": unit_name,
@@ -437,7 +453,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "wannier3d-n128", "nsta": n, "n_R": nR, "kmesh": list(kmesh), "n_k": nk,
+        "config": {"workload": args.workload, "nsta": n, "n_R": nR, "kmesh": list(kmesh), "n_k": nk,
                    "precision": args.precision, "eta": eta, "mu": [0.0], "T": 0.0, "outputs": "E, Omega_z, AHC"},
         "stage_ms_per_step": {names[i]: round(stage_ms[i] / args.steps, 2) for i in range(ns)},
         "step_ms": {"min": round(min(step_s) * 1e3, 2), "median": round(float(np.median(step_s)) * 1e3, 2),

@@ -228,6 +228,19 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
                     double T, double eta, const double* dir, int spin_dir, int precision,
                     double* sigma);
 
+/* The AHC of rtb_hall_direct resolved by layer, for layered (slab) models.
+ * No counterpart in the reference, which defines no layer-resolved
+ * curvature: here sigma_layers[m][l] = sum_k sum_n f(E_n, mu_m) Omega_n(k)
+ * sum_{i in layer l} |u_n(k)_i|^2 / N_k / det(lat), the band curvature
+ * weighted by the band's weight on the layer's orbitals (both spin copies of
+ * orbital o belong to layer orbital_layer[o], o < norb); summed over l it is
+ * sigma (also returned).  For degenerate bands the split between layers
+ * depends on the basis chosen inside the degenerate subspace.
+ * n_mu * n_layers <= 1024. */
+int rtb_hall_direct_layers(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu,
+                           double T, double eta, const double* dir, int spin_dir, int precision,
+                           int n_layers, const int* orbital_layer, double* sigma, double* sigma_layers);
+
 /* rtb_hall_direct without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its

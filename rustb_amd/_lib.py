@@ -95,6 +95,8 @@ SIGNATURES = {
     "rtb_occupied_berry_curvature": (c_int, [c_void_p, _P, c_size_t, _P, c_int, c_double, c_double,
                                              c_double, c_int, _P]),
     "rtb_hall_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, _P]),
+    "rtb_hall_direct_layers": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, c_int, _P,
+                                       _P, _P]),
     "rtb_hall_direct_noalloc": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, _P]),
     "rtb_hall_energy_cut": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, _P]),
     "rtb_energy_cut_host": (c_int, [c_size_t, c_size_t, c_int, _P, _P, _P, _P, _P, c_size_t, c_double, c_double,

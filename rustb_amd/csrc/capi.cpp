@@ -913,6 +913,12 @@ struct BerryJob {
   // (current, field_1, field_2); mode 2 only, weights -df/dE
   bool dipole = false;
   bool sym = false;
+  // layer-resolved AHC (mode 2): rows of H by layer (perm, bounds: host,
+  // nsta and n_layers + 1 entries), output [n_mu][n_layers]
+  const int* layer_perm = nullptr;
+  const int* layer_bounds = nullptr;
+  int n_layers = 0;
+  double* sigma_layers = nullptr;
 };
 
 // A chunk of consecutive pipeline k-points; for the mesh path it covers the
@@ -960,6 +966,8 @@ struct BerryBufs {
   double *dMet = nullptr, *dA3 = nullptr, *dOm2 = nullptr, *dKern = nullptr, *kub = nullptr, *dsc = nullptr;
   double *rproj = nullptr, *dmu = nullptr, *dpart = nullptr, *dsig = nullptr, *docc = nullptr;
   double *dmpart = nullptr, *dmsig = nullptr;
+  int *dperm = nullptr, *dbounds = nullptr;
+  double *dlpart = nullptr, *dlsum = nullptr, *dlay = nullptr;
   EigBufs eb{};
   int total_blk = 0;
 };
@@ -1002,6 +1010,14 @@ static int berry_alloc(Work& w, const rtb_model* m, const BerryJob& j, size_t K,
     }
   }
   if (j.mode == 1) RTB_TRY(w.alloc(&b.docc, K));
+  if (j.n_layers > 0) {
+    const size_t ml = j.n_mu * static_cast<size_t>(j.n_layers);
+    RTB_TRY(w.alloc(&b.dperm, static_cast<size_t>(n)));
+    RTB_TRY(w.alloc(&b.dbounds, static_cast<size_t>(j.n_layers) + 1));
+    RTB_TRY(w.alloc(&b.dlpart, K * ml));
+    RTB_TRY(w.alloc(&b.dlsum, chunks.size() * ml));
+    RTB_TRY(w.alloc(&b.dlay, ml));
+  }
   return RTB_OK;
 }
 
@@ -1099,6 +1115,12 @@ static int berry_run(const rtb_model* m, BerryJob& j, Exec& ex) {
   double* A = bb.kub + K * n2 * 2;
   double* B = bb.kub + K * n2 * 4;
   if (j.mode == 2) RTB_HIP(hipMemcpyAsync(dmu, j.mu, j.n_mu * sizeof(double), hipMemcpyHostToDevice, st));
+  const size_t ml = j.n_mu * static_cast<size_t>(j.n_layers);
+  if (j.n_layers > 0) {
+    RTB_HIP(hipMemcpyAsync(bb.dperm, j.layer_perm, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    RTB_HIP(hipMemcpyAsync(bb.dbounds, j.layer_bounds, sizeof(int) * (j.n_layers + 1), hipMemcpyHostToDevice, st));
+  }
+  size_t chunk_no = 0;
   std::vector<double> kp;
   int blk_off = 0;
   for (const Chunk& ch : chunks) {
@@ -1190,10 +1212,18 @@ static int berry_run(const rtb_model* m, BerryJob& j, Exec& ex) {
                               dmpart + static_cast<size_t>(blk_off) * j.n_mu, nb));
         ++l2;
       }
+      if (j.n_layers > 0) {  // Zc still holds the eigenvector columns
+        RTB_TRY(layer_partials(st, n, c, Zc, f32 ? dsc : nullptr, dE, dOm, dmu, static_cast<int>(j.n_mu), j.T,
+                               bb.dperm, bb.dbounds, j.n_layers, bb.dlpart));
+        RTB_TRY(reduce_partials(st, bb.dlpart, static_cast<int>(c), static_cast<int>(ml), 1.0,
+                                bb.dlsum + chunk_no * ml));
+        l2 += 2;
+      }
       blk_off += nb;
     }
     RTB_TRY(tmr.end(l2));
     if (j.mode != 2) RTB_HIP(hipStreamSynchronize(st));
+    ++chunk_no;
   }
   if (j.mode == 2) {
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
@@ -1208,6 +1238,12 @@ static int berry_run(const rtb_model* m, BerryJob& j, Exec& ex) {
     // sum / nk / det (response/linear/mod.rs:221; quantum_geometry.rs:219,234)
     for (size_t i = 0; i < j.n_mu; ++i) j.sigma[i] = sums[i] / static_cast<double>(j.nk) / m->det_lat;
     for (size_t i = 0; i < msums.size(); ++i) j.msum[i] = msums[i] / static_cast<double>(j.nk) / m->det_lat;
+    if (j.n_layers > 0) {
+      RTB_TRY(reduce_partials(st, bb.dlsum, static_cast<int>(chunks.size()), static_cast<int>(ml), 1.0, bb.dlay));
+      RTB_HIP(hipMemcpyAsync(j.sigma_layers, bb.dlay, ml * sizeof(double), hipMemcpyDeviceToHost, st));
+      RTB_HIP(hipStreamSynchronize(st));
+      for (size_t i = 0; i < ml; ++i) j.sigma_layers[i] /= static_cast<double>(j.nk) * m->det_lat;
+    }
   }
   RTB_HIP(hipStreamSynchronize(st));
   RTB_TRY(checks_finish(st, ck, tmr));
@@ -1341,6 +1377,34 @@ int rtb_hall_direct(const rtb_model* m, const size_t* kmesh, const double* mu, s
   RTB_TRY(prepare(m));
   BerryJob j;
   RTB_TRY(hall_direct_job(m, kmesh, mu, n_mu, T, eta, dir, spin_dir, precision, sigma, j));
+  return berry_impl(m, j);
+}
+
+int rtb_hall_direct_layers(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu, double T,
+                           double eta, const double* dir, int spin_dir, int precision, int n_layers,
+                           const int* orbital_layer, double* sigma, double* sigma_layers) {
+  RTB_TRY(prepare(m));
+  BerryJob j;
+  RTB_TRY(hall_direct_job(m, kmesh, mu, n_mu, T, eta, dir, spin_dir, precision, sigma, j));
+  if (n_layers < 1 || !orbital_layer || !sigma_layers)
+    return fail(RTB_ERR_INVALID_ARGUMENT, "layers: need n_layers >= 1, orbital_layer and sigma_layers");
+  if (static_cast<long long>(n_mu) * n_layers > 1024)
+    return fail(RTB_ERR_UNSUPPORTED, "layers: n_mu * n_layers above 1024");
+  for (int o = 0; o < m->norb; ++o)
+    if (orbital_layer[o] < 0 || orbital_layer[o] >= n_layers)
+      return fail(RTB_ERR_INVALID_ARGUMENT, "layers: orbital_layer entries must be in [0, n_layers)");
+  // rows of H grouped by layer (both spin copies of an orbital in its layer)
+  std::vector<int> perm, bounds(static_cast<size_t>(n_layers) + 1, 0);
+  for (int l = 0; l < n_layers; ++l) {
+    bounds[l] = static_cast<int>(perm.size());
+    for (int r = 0; r < m->nsta; ++r)
+      if (orbital_layer[r % m->norb] == l) perm.push_back(r);
+  }
+  bounds[n_layers] = static_cast<int>(perm.size());
+  j.layer_perm = perm.data();
+  j.layer_bounds = bounds.data();
+  j.n_layers = n_layers;
+  j.sigma_layers = sigma_layers;
   return berry_impl(m, j);
 }
 

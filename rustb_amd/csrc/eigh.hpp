@@ -186,6 +186,12 @@ int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const d
 int dipole_combine(hipStream_t st, int n, size_t nk, const double* A2, const double* A3, const double* om1,
                    const double* om2, bool sym, double* out);
 int hall_blocks(size_t nk);
+// Layer-resolved occupied curvature: partial[k][m][l] = sum_j f(E_j, mu_m)
+// Omega_j sum_{i in layer l} |U_ij|^2 (U = Z diag(scale), Z: [nk][n][n]
+// columns; the rows of layer l are perm[bounds[l] .. bounds[l+1])).
+int layer_partials(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, const double* E,
+                   const double* omega, const double* mu, int n_mu, double T, const int* perm, const int* bounds,
+                   int n_layers, double* partial);
 // EnergyCut vertex kernel Im K[b][i][j] = Im(A[b][i][j] B[b][j][i]) ([nk][n][n] real).
 int energy_cut_vertex(hipStream_t st, int n, size_t nk, const double* A, const double* B, double* Ki);
 // x = |z|^2 elementwise (total complex z, real x).

@@ -2,6 +2,7 @@
 // (thermodynamics.rs:86-118, response/linear/mod.rs:204-221) and the
 // Gaussian DOS (model_physics.rs:179-226), FP64.
 #include "eigh.hpp"
+#include "gpu_team.hpp"
 
 namespace rtb {
 
@@ -90,6 +91,52 @@ __global__ void occupied_sum_kernel(int n, size_t nk, const double* __restrict__
   double s = 0.0;
   for (int b = 0; b < n; ++b) s += omega[k * n + b] * fermi(E[k * n + b], mu, width);
   out[k] = s;
+}
+
+// Layer-resolved occupied Berry curvature: per k-point and chemical
+// potential mu_m, partial[k][m][l] = sum_j f(E_j, mu_m) Omega_j w_jl with
+// w_jl = sum_{i in layer l} |U_ij|^2 for the unit eigenvector columns
+// U = Z diag(scale) (scale may be null).  perm lists the rows of H by layer,
+// rows of layer l at perm[bounds[l] .. bounds[l+1]).  One workgroup of 256
+// threads per k-point, thread j per band (coalesced column walks); per-wave
+// sums by DPP, then the waves in order.  n_mu * n_layers <= kLayerOut.
+constexpr int kLayerOut = 1024;
+__global__ void __launch_bounds__(256) layer_partials_kernel(int n, const double2* __restrict__ Z,
+                                                            const double* __restrict__ scale,
+                                                            const double* __restrict__ E,
+                                                            const double* __restrict__ omega,
+                                                            const double* __restrict__ mu, int n_mu, double width,
+                                                            const int* __restrict__ perm,
+                                                            const int* __restrict__ bounds, int n_layers,
+                                                            double* __restrict__ partial) {
+  __shared__ double wred[4][kLayerOut];
+  const size_t k = blockIdx.x;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double2* Zk = Z + k * nn;
+  for (int j0 = 0; j0 < n; j0 += 256) {
+    const int j = j0 + tid;
+    const bool live = j < n;
+    const double sj = live && scale ? scale[k * n + j] : 1.0;
+    const double om = live ? omega[k * n + j] : 0.0;
+    const double ej = live ? E[k * n + j] : 0.0;
+    for (int l = 0; l < n_layers; ++l) {
+      double w = 0.0;
+      if (live)
+        for (int q = bounds[l]; q < bounds[l + 1]; ++q) {
+          const double2 z = Zk[static_cast<size_t>(perm[q]) * n + j];
+          w += z.x * z.x + z.y * z.y;
+        }
+      w *= sj * sj * om;
+      for (int m = 0; m < n_mu; ++m) {
+        const double v = wave_sum64(live ? w * fermi(ej, mu[m], width) : 0.0);
+        if (lane == 0) wred[wave][m * n_layers + l] = (j0 == 0 ? 0.0 : wred[wave][m * n_layers + l]) + v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int o = tid; o < n_mu * n_layers; o += 256)
+    partial[k * n_mu * n_layers + o] = (wred[0][o] + wred[1][o]) + (wred[2][o] + wred[3][o]);
 }
 
 constexpr int kHallK = 256;  // k-points per hall partial block
@@ -275,6 +322,20 @@ int occupied_sum(hipStream_t st, int n, size_t nk, const double* omega, const do
   if (nk == 0) return RTB_OK;
   hipLaunchKernelGGL(occupied_sum_kernel, dim3(static_cast<unsigned>((nk + 255) / 256)), dim3(256), 0, st, n, nk,
                      omega, E, mu, T <= 0.0 ? 0.0 : T * 8.617333262145e-5, out);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int layer_partials(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, const double* E,
+                   const double* omega, const double* mu, int n_mu, double T, const int* perm, const int* bounds,
+                   int n_layers, double* partial) {
+  if (nk == 0) return RTB_OK;
+  if (static_cast<long long>(n_mu) * n_layers > kLayerOut)
+    return fail(RTB_ERR_UNSUPPORTED, "n_mu * n_layers above 1024");
+  hipLaunchKernelGGL(layer_partials_kernel, dim3(static_cast<unsigned>(nk)), dim3(256), 0, st, n,
+                     reinterpret_cast<const double2*>(Z), scale, E, omega, mu, n_mu,
+                     T <= 0.0 ? 0.0 : T * 8.617333262145e-5, perm, bounds,
+                     n_layers, partial);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -472,6 +472,26 @@ class Model:
                                      ptr(sig)))
         return HallConductivityResult(mu.copy(), sig)
 
+    def hall_conductivity_layers(self, params: Parameters, orbital_layer, n_layers=None):
+        """The Direct AHC resolved by layer (no reference counterpart): returns
+        (sigma [n_mu], sigma_layers [n_mu, n_layers]) with sigma_layers[m, l] =
+        sum_k sum_n f(E_n, mu_m) Omega_n sum_{i in layer l} |u_n(i)|^2 / N_k / det,
+        orbital o (both spin copies) in layer orbital_layer[o]; the layers sum
+        to sigma."""
+        lay = np.ascontiguousarray(np.asarray(orbital_layer, dtype=np.int32))
+        if lay.shape != (self.norb,):
+            raise TbError(-2, "orbital_layer must have norb entries")
+        nl = int(lay.max()) + 1 if n_layers is None else int(n_layers)
+        kmesh = np.ascontiguousarray(np.asarray(params.kmesh, dtype=np.uint64))
+        mu = np.ascontiguousarray(np.atleast_1d(np.asarray(params.mu, np.float64)))
+        T = float(np.atleast_1d(np.asarray(params.T, float))[0])
+        sig = np.zeros(mu.size)
+        sl = np.zeros((mu.size, nl))
+        check(load().rtb_hall_direct_layers(self._device(), ptr(kmesh), ptr(mu), mu.size, T, float(params.eta),
+                                            ptr(self._dir(params)), SPIN_DIR[params.spin],
+                                            PRECISION[params.precision], nl, ptr(lay), ptr(sig), ptr(sl)))
+        return sig, sl
+
     # ---- quantum geometry (quantum_geometry.rs) ----
     def quantum_geometry_on(self, kvec, params: Parameters) -> "QuantumGeometryMap":
         """quantum_geometry_on (quantum_geometry.rs:104-140): metric, Berry

@@ -108,3 +108,41 @@ def test_slab_n256_dos_vs_oracle(slab):
     E, rho = m.dos([4, 3], -6.0, 6.0, 121, 0.1)
     Er, rr = O.dos(o, [4, 3], -6.0, 6.0, 121, 0.1)
     assert np.abs(rho - rr).max() <= 1e-9 * rr.max()
+
+
+def layer_definition(o, kmesh, mu, eta, layer_of_orbital, n_layers):
+    """The layer-resolved AHC as rtb_hall_direct_layers defines it (no
+    reference counterpart, so parity is unpinned; restated here from the
+    definition): sum_k sum_n f(E_n) Omega_n sum_{i in l} |u_n(i)|^2 / N_k / det,
+    from the oracle's Omega and its zheev eigenvectors (|u_n(i)|^2 does not
+    depend on the eigenvector phases of non-degenerate bands)."""
+    ks = O.gen_kmesh(kmesh)
+    lay = np.asarray(layer_of_orbital)[np.arange(o.nsta) % o.norb]
+    out = np.zeros((len(mu), n_layers))
+    for k in ks:
+        om, E = O.berry_curvature_at(o, k, D2, eta)
+        _, U = O.eigh(O.gen_ham(o, k, "atom"))
+        w = np.zeros((o.nsta, n_layers))
+        for l in range(n_layers):
+            w[:, l] = (np.abs(U[lay == l, :]) ** 2).sum(axis=0)
+        for i, m in enumerate(mu):
+            out[i] += ((om * O.fermi(E, m, 0.0))[:, None] * w).sum(axis=0)
+    return out / len(ks) / np.linalg.det(o.lat)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_slab_n256_layer_resolved_ahc(slab, precision):
+    """Sum over the 32 layers equals the AHC, the AHC equals the plain
+    Direct call, and each layer equals the definition from the oracle."""
+    m, o = slab
+    mu = np.array([-0.3, 0.0, 0.4])
+    layer = np.arange(m.norb) // 8  # 32 layers of 8 orbitals, layer-major
+    p = ra.Parameters(kmesh=(5, 4), direction=D2, mu=mu, eta=1e-4, precision=precision)
+    s, sl = m.hall_conductivity_layers(p, layer)
+    assert sl.shape == (3, 32)
+    s0 = m.hall_conductivity(p).conductivity
+    assert np.abs(s - s0).max() <= 1e-13 * np.abs(s0).max()
+    assert np.abs(sl.sum(axis=1) - s).max() <= 1e-10 * np.abs(s).max()
+    ref = layer_definition(o, (5, 4), mu, 1e-4, layer, 32)
+    tol = 1e-9 if precision == "fp64" else 1e-6
+    assert np.abs(sl - ref).max() <= tol * np.abs(ref).max()
