@@ -126,7 +126,7 @@ constexpr int kBandChaseThreads = 64;
 
 template <typename T>
 size_t band_reduce_smem(int n) {
-  const size_t pan = static_cast<size_t>(n - kBandB) * kBandB * sizeof(cplx<T>);
+  const size_t pan = static_cast<size_t>(n - kBandB) * kBandLd * sizeof(cplx<T>);
   return 2 * pan + 3 * kBandB * kBandB * sizeof(cplx<T>) + kBandB * sizeof(cplx<T>) +
          2 * GpuTeam::kRedStride * sizeof(T);
 }
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(kBandReduceThreads) band_reduce_kernel(int n, 
   const size_t nn = static_cast<size_t>(n) * n;
   const BandLayout L = band_layout(n);
   cplx<T>* base = reinterpret_cast<cplx<T>*>(smem_raw);
-  const size_t pan = static_cast<size_t>(n - kBandB) * kBandB;
+  const size_t pan = static_cast<size_t>(n - kBandB) * kBandLd;
   BandReduceSmem<T> sm;
   sm.V = base;
   sm.Y = base + pan;
@@ -208,7 +208,7 @@ __global__ void __launch_bounds__(BandBt<T>::W * BandBt<T>::S) band_backtrans_ke
 
 template <typename T>
 size_t band_backtrans_smem(int n) {
-  return static_cast<size_t>(n) * BandBt<T>::W * sizeof(cplx<T>);
+  return static_cast<size_t>(n) * (BandBt<T>::W + 1) * sizeof(cplx<T>);
 }
 
 constexpr int kTeBlock = 256;

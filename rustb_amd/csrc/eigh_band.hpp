@@ -31,6 +31,7 @@
 namespace rtb {
 
 constexpr int kBandB = 16;       // half-bandwidth of the intermediate band matrix
+constexpr int kBandLd = kBandB + 1;  // padded row stride of the panels in team memory
 constexpr int kMaxBandN = 256;   // largest order of the two-stage route
 
 // Per-matrix storage of the reflectors (complex elements):
@@ -97,8 +98,8 @@ RTB_HD inline void cfmac(cplx<T>& acc, cplx<T> a, cplx<T> b) {  // acc += conj(a
 // 2 * Team::kRedStride.
 template <typename T>
 struct BandReduceSmem {
-  cplx<T>* V;    // [n - b][b]  the panel, then its reflectors
-  cplx<T>* Y;    // [n - b][b]  A22 V, then Y, then Z
+  cplx<T>* V;    // [n - b][b + 1]  the panel, then its reflectors
+  cplx<T>* Y;    // [n - b][b + 1]  A22 V, then Y, then Z
   cplx<T>* Tm;   // [b][b]
   cplx<T>* W;    // [b][b]
   cplx<T>* M;    // [b][b]
@@ -118,6 +119,7 @@ template <typename T, typename Team>
 RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V1, cplx<T>* T1,
                         BandReduceSmem<T> sm) {
   constexpr int b = kBandB;
+  constexpr int ld = kBandLd;  // team-memory row stride of the panels (padded: no bank conflicts)
   const int nt = tm.nt, id = tm.id;
   for (size_t e = id; e < static_cast<size_t>(n) * n; e += nt) {
     const int r = static_cast<int>(e / n), c = static_cast<int>(e % n);
@@ -139,7 +141,7 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
     const int c0 = p * b, r0 = c0 + b, m = n - r0;
     cplx<T>* V1p = V1 + static_cast<size_t>(p) * n * b;
     // the panel A[r0 .. n-1][c0 .. c0+b-1] into team memory
-    for (int e = id; e < m * b; e += nt) sm.V[e] = A[static_cast<size_t>(r0 + e / b) * n + c0 + e % b];
+    for (int e = id; e < m * b; e += nt) sm.V[(e / b) * ld + e % b] = A[static_cast<size_t>(r0 + e / b) * n + c0 + e % b];
     tm.sync();
     // QR of the panel (zgeqr2): reflector c annihilates V[c+1 .., c]; the
     // remaining columns get H_c^H = I - conj(tau) v v^H.  Thread group g of
@@ -152,17 +154,17 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
       }
       T part = T(0);
       for (int i = c + 1 + id; i < m; i += nt) {
-        const cplx<T> x = sm.V[i * b + c];
+        const cplx<T> x = sm.V[i * ld + c];
         part += x.re * x.re + x.im * x.im;
       }
       T* red = sm.red + red_sel * Team::kRedStride;
       red_sel ^= 1;
       const T xn2 = tm.reduce_sum_ns(part, red);
-      const Refl<T> h = make_reflector(sm.V[c * b + c], xn2);
+      const Refl<T> h = make_reflector(sm.V[c * ld + c], xn2);
       tm.sync();  // every thread has read V[c][c]
-      for (int i = c + 1 + id; i < m; i += nt) sm.V[i * b + c] = cmul(sm.V[i * b + c], h.scale);
+      for (int i = c + 1 + id; i < m; i += nt) sm.V[i * ld + c] = cmul(sm.V[i * ld + c], h.scale);
       if (id == 0) {
-        sm.V[c * b + c] = {h.beta, T(0)};
+        sm.V[c * ld + c] = {h.beta, T(0)};
         sm.tau[c] = h.tau;
       }
       tm.sync();
@@ -171,8 +173,8 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
       cplx<T> w = {T(0), T(0)};
       if (live) {
         for (int i = c + l; i < m; i += 16) {
-          const cplx<T> v = i == c ? cplx<T>{T(1), T(0)} : sm.V[i * b + c];
-          cfmac(w, v, sm.V[i * b + c2]);
+          const cplx<T> v = i == c ? cplx<T>{T(1), T(0)} : sm.V[i * ld + c];
+          cfmac(w, v, sm.V[i * ld + c2]);
         }
       }
       w.re = tm.template group_sum<16>(w.re);
@@ -180,10 +182,10 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
       if (live) {
         const cplx<T> f = cmul(conjc(h.tau), w);  // conj(tau) v^H x
         for (int i = c + l; i < m; i += 16) {
-          const cplx<T> v = i == c ? cplx<T>{T(1), T(0)} : sm.V[i * b + c];
+          const cplx<T> v = i == c ? cplx<T>{T(1), T(0)} : sm.V[i * ld + c];
           const cplx<T> d = cmul(v, f);
-          sm.V[i * b + c2].re -= d.re;
-          sm.V[i * b + c2].im -= d.im;
+          sm.V[i * ld + c2].re -= d.re;
+          sm.V[i * ld + c2].im -= d.im;
         }
       }
       tm.sync();
@@ -192,7 +194,7 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
     // T[c][c] = tau_c, T[0:c, c] = -tau_c T[0:c, 0:c] V[:, 0:c]^H v_c.
     // First W = V^H V (strictly upper part), one thread per pair.
     auto vat = [&](int i, int c) -> cplx<T> {
-      return i < c ? cplx<T>{T(0), T(0)} : (i == c ? cplx<T>{T(1), T(0)} : sm.V[i * b + c]);
+      return i < c ? cplx<T>{T(0), T(0)} : (i == c ? cplx<T>{T(1), T(0)} : sm.V[i * ld + c]);
     };
     for (int q = id; q < b * b; q += nt) {
       const int a = q / b, c = q % b;
@@ -220,7 +222,7 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
     // zeros below, mirrored into the upper part; V and T to the panel store
     for (int e = id; e < m * b; e += nt) {
       const int i = e / b, c = e % b;
-      const cplx<T> x = i <= c ? sm.V[e] : cplx<T>{T(0), T(0)};
+      const cplx<T> x = i <= c ? sm.V[i * ld + c] : cplx<T>{T(0), T(0)};
       A[static_cast<size_t>(r0 + i) * n + c0 + c] = x;
       A[static_cast<size_t>(c0 + c) * n + r0 + i] = conjc(x);
       V1p[e] = vat(i, c);
@@ -228,7 +230,7 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
     for (int q = id; q < b * b; q += nt) T1[static_cast<size_t>(p) * b * b + q] = sm.Tm[q];
     tm.sync();
     // V as a full m x b block (unit diagonal, zeros above) for the products
-    for (int e = id; e < m * b; e += nt) sm.V[e] = vat(e / b, e % b);
+    for (int e = id; e < m * b; e += nt) sm.V[(e / b) * ld + e % b] = vat(e / b, e % b);
     tm.sync();
     // Y = A22 V: row i of Y from column i of A22 (Hermitian: conj(A22[k][i])),
     // read across threads i; all b columns per thread
@@ -240,7 +242,7 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
       for (int k = 0; k < m; ++k) {
         const cplx<T> a = col[static_cast<size_t>(k) * n];
 #pragma unroll
-        for (int c = 0; c < b; ++c) cfmac(acc[c], a, sm.V[k * b + c]);
+        for (int c = 0; c < b; ++c) cfmac(acc[c], a, sm.V[k * ld + c]);
       }
       // Y = (A22 V) T, T upper triangular
 #pragma unroll
@@ -248,7 +250,7 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
         cplx<T> s = {T(0), T(0)};
 #pragma unroll
         for (int a = 0; a <= c; ++a) cfma(s, acc[a], sm.Tm[a * b + c]);
-        sm.Y[i * b + c] = s;
+        sm.Y[i * ld + c] = s;
       }
     }
     tm.sync();
@@ -256,7 +258,7 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
     for (int q = id; q < b * b; q += nt) {
       const int a = q / b, c = q % b;
       cplx<T> s = {T(0), T(0)};
-      for (int i = a; i < m; ++i) cfmac(s, sm.V[i * b + a], sm.Y[i * b + c]);
+      for (int i = a; i < m; ++i) cfmac(s, sm.V[i * ld + a], sm.Y[i * ld + c]);
       sm.W[q] = s;
     }
     tm.sync();
@@ -271,9 +273,9 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
     for (int e = id; e < m * b; e += nt) {
       const int i = e / b, c = e % b;
       cplx<T> s = {T(0), T(0)};
-      for (int a = 0; a < b; ++a) cfma(s, sm.V[i * b + a], sm.M[a * b + c]);
-      sm.Y[e].re -= T(0.5) * s.re;
-      sm.Y[e].im -= T(0.5) * s.im;
+      for (int a = 0; a < b; ++a) cfma(s, sm.V[i * ld + a], sm.M[a * b + c]);
+      sm.Y[i * ld + c].re -= T(0.5) * s.re;
+      sm.Y[i * ld + c].im -= T(0.5) * s.im;
     }
     tm.sync();
     // A22 -= V Z^H + Z V^H, threads along the columns j (coalesced rows)
@@ -281,15 +283,15 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
       cplx<T> vj[b], zj[b];
 #pragma unroll
       for (int c = 0; c < b; ++c) {
-        vj[c] = conjc(sm.V[j * b + c]);
-        zj[c] = conjc(sm.Y[j * b + c]);
+        vj[c] = conjc(sm.V[j * ld + c]);
+        zj[c] = conjc(sm.Y[j * ld + c]);
       }
       for (int i = 0; i < m; ++i) {
         cplx<T> s = {T(0), T(0)};
 #pragma unroll
         for (int c = 0; c < b; ++c) {
-          cfma(s, sm.V[i * b + c], zj[c]);
-          cfma(s, sm.Y[i * b + c], vj[c]);
+          cfma(s, sm.V[i * ld + c], zj[c]);
+          cfma(s, sm.Y[i * ld + c], vj[c]);
         }
         cplx<T>& a = A[static_cast<size_t>(r0 + i) * n + r0 + j];
         a.re -= s.re;
@@ -307,7 +309,8 @@ RTB_HD void band_reduce(Team& tm, int n, const double* H, cplx<T>* A, cplx<T>* V
 // annihilates column `col` below row r0 (col = i for k = 0, else the first
 // row of the previous step) is applied from the left to columns
 // [col, r1 + b] and from the right to rows [col, r1 + b]; stored in V2, tau2.
-// hv: team memory of b complex.
+// hv: team memory of b complex.  nt a multiple of 16 (the right
+// application takes 16 threads per row).
 // --------------------------------------------------------------------------
 template <typename T, typename Team>
 RTB_HD void band_chase(Team& tm, int n, cplx<T>* A, T* D, T* E, cplx<T>* V2, cplx<T>* tau2, cplx<T>* hv, T* red) {
@@ -363,16 +366,22 @@ RTB_HD void band_chase(Team& tm, int n, cplx<T>* A, T* D, T* E, cplx<T>* V2, cpl
       }
       tm.sync();
       if (apply) {
-        // right: rows [col, hi], columns [r0, r1]: y <- y - tau (y v) v^H
-        for (int r = col + id; r <= hi; r += nt) {
-          cplx<T>* row = A + static_cast<size_t>(r) * n + r0;
-          cplx<T> w = {T(0), T(0)};
-          for (int q = 0; q < len; ++q) cfma(w, row[q], hv[q]);
-          const cplx<T> f = cmul(w, h.tau);
-          for (int q = 0; q < len; ++q) {
-            const cplx<T> d = cmul(f, conjc(hv[q]));
-            row[q].re -= d.re;
-            row[q].im -= d.im;
+        // right: rows [col, hi], columns [r0, r1]: y <- y - tau (y v) v^H;
+        // 16 threads per row, one column each (contiguous in the row), the
+        // row's dot product by a group sum
+        const int g = id / 16, q = id % 16, rows_per = nt / 16;
+        const cplx<T> vq = q < len ? hv[q] : cplx<T>{T(0), T(0)};
+        for (int r0w = col; r0w <= hi; r0w += rows_per) {  // same trip count in every thread
+          const int r = r0w + g;
+          const bool in = r <= hi && q < len;
+          cplx<T> a = {T(0), T(0)};
+          if (in) a = A[static_cast<size_t>(r) * n + r0 + q];
+          cplx<T> w = cmul(a, vq);
+          w.re = tm.template group_sum<16>(w.re);
+          w.im = tm.template group_sum<16>(w.im);
+          if (in) {
+            const cplx<T> d = cmul(cmul(w, h.tau), conjc(vq));
+            A[static_cast<size_t>(r) * n + r0 + q] = {a.re - d.re, a.im - d.im};
           }
         }
       }
@@ -418,7 +427,7 @@ RTB_HD void twisted_column(int n, const T* e, const T* Ts, int ldy, int j, T twi
 // Y of T (Y[i * ldy + j]).  Team of W * S threads: thread (c, s) =
 // (id / S, id % S) takes rows s, s + S, .. of each reflector's rows for
 // column j0 + c; the S threads of a column are consecutive ids (one wave).
-// Ys: team memory [n][W] complex.  Z out as in backtrans_reg: rows_out = 0
+// Ys: team memory [n][W + 1] complex.  Z out as in backtrans_reg: rows_out = 0
 // Zout[r][c], rows_out = 1 Zout[c][r]; gdiag (optional): squared norms of
 // the stored columns (FP64).
 // --------------------------------------------------------------------------
@@ -427,6 +436,7 @@ RTB_HD void band_backtrans(Team& tm, int n, int j0, const T* Y, int ldy, const c
                            const cplx<T>* V2, const cplx<T>* tau2, cplx<T>* Ys, TO* Zout, int rows_out,
                            double* gdiag) {
   constexpr int b = kBandB;
+  constexpr int Wp = W + 1;  // padded row stride of Ys (the S threads of a column hit distinct banks)
   static_assert(b % S == 0 && (S == 8 || S == 16), "rows per thread");
   const BandLayout L = band_layout(n);
   const int c = tm.id / S, s = tm.id % S;
@@ -434,38 +444,63 @@ RTB_HD void band_backtrans(Team& tm, int n, int j0, const T* Y, int ldy, const c
   const bool live = c < W && j < n;
   for (int e = tm.id; e < n * W; e += tm.nt) {
     const int i = e / W, cc = e % W;
-    Ys[e] = {j0 + cc < n ? Y[static_cast<size_t>(i) * ldy + j0 + cc] : T(0), T(0)};
+    Ys[i * Wp + cc] = {j0 + cc < n ? Y[static_cast<size_t>(i) * ldy + j0 + cc] : T(0), T(0)};
   }
   tm.sync();
-  // Q2: the chasing reflectors, last sweep first, last step first
-  for (int i = n - 2; i >= 0; --i) {
-    const int K = (n - 1 - i + b - 1) / b;
-    for (int k = K - 1; k >= 0; --k) {
+  // Q2: the chasing reflectors, last sweep first, last step first; the
+  // next reflector's tau and this thread's entries of its v are loaded
+  // while the current one is applied (the chain is otherwise one global
+  // memory latency per reflector)
+  constexpr int Q = b / S;  // rows of a reflector per thread
+  auto nsteps = [&](int i) { return (n - 1 - i + b - 1) / b; };
+  struct Pre {
+    cplx<T> tau, v[Q];
+  };
+  auto fetch = [&](int i, int k) {
+    Pre p;
+    const size_t r = static_cast<size_t>(i) * L.K + k;
+    p.tau = tau2[r];
+#pragma unroll
+    for (int u = 0; u < Q; ++u) p.v[u] = V2[r * b + s + S * u];
+    return p;
+  };
+  int i = n - 2, k = n >= 2 ? nsteps(n - 2) - 1 : -1;
+  Pre cur{};
+  if (i >= 0) cur = fetch(i, k);
+  while (i >= 0) {
+    int ni = i, nk = k - 1;
+    if (nk < 0) {
+      ni = i - 1;
+      nk = ni >= 0 ? nsteps(ni) - 1 : -1;
+    }
+    Pre nxt{};
+    if (ni >= 0) nxt = fetch(ni, nk);
+    if (cur.tau.re != T(0) || cur.tau.im != T(0)) {  // team-uniform
       const int r0 = i + 1 + k * b;
       const int len = r0 + b - 1 < n - 1 ? b : n - r0;
-      const cplx<T> tau = tau2[static_cast<size_t>(i) * L.K + k];
-      if (tau.re == T(0) && tau.im == T(0)) continue;  // team-uniform
-      const cplx<T>* v = V2 + (static_cast<size_t>(i) * L.K + k) * b;
       cplx<T> w = {T(0), T(0)};
 #pragma unroll
-      for (int q0 = 0; q0 < b; q0 += S) {
-        const int q = q0 + s;
-        if (q < len) cfmac(w, v[q], Ys[(r0 + q) * W + c]);
+      for (int u = 0; u < Q; ++u) {
+        const int q = s + S * u;
+        if (q < len) cfmac(w, cur.v[u], Ys[(r0 + q) * Wp + c]);
       }
       w.re = tm.template group_sum<S>(w.re);
       w.im = tm.template group_sum<S>(w.im);
-      const cplx<T> f = cmul(tau, w);
+      const cplx<T> f = cmul(cur.tau, w);
 #pragma unroll
-      for (int q0 = 0; q0 < b; q0 += S) {
-        const int q = q0 + s;
+      for (int u = 0; u < Q; ++u) {
+        const int q = s + S * u;
         if (q < len) {
-          const cplx<T> d = cmul(v[q], f);
-          Ys[(r0 + q) * W + c].re -= d.re;
-          Ys[(r0 + q) * W + c].im -= d.im;
+          const cplx<T> d = cmul(cur.v[u], f);
+          Ys[(r0 + q) * Wp + c].re -= d.re;
+          Ys[(r0 + q) * Wp + c].im -= d.im;
         }
       }
       tm.wave_sync();
     }
+    cur = nxt;
+    i = ni;
+    k = nk;
   }
   // Q1: panels last first, Z <- Z - V (T (V^H Z)) on rows [r0, n)
   for (int p = L.P - 1; p >= 0; --p) {
@@ -476,7 +511,7 @@ RTB_HD void band_backtrans(Team& tm, int n, int j0, const T* Y, int ldy, const c
 #pragma unroll
     for (int a = 0; a < b; ++a) w[a] = {T(0), T(0)};
     for (int i = s; i < m; i += S) {
-      const cplx<T> z = Ys[(r0 + i) * W + c];
+      const cplx<T> z = Ys[(r0 + i) * Wp + c];
 #pragma unroll
       for (int a = 0; a < b; ++a) cfmac(w[a], V[i * b + a], z);
     }
@@ -496,8 +531,8 @@ RTB_HD void band_backtrans(Team& tm, int n, int j0, const T* Y, int ldy, const c
       cplx<T> d = {T(0), T(0)};
 #pragma unroll
       for (int a = 0; a < b; ++a) cfma(d, V[i * b + a], u[a]);
-      Ys[(r0 + i) * W + c].re -= d.re;
-      Ys[(r0 + i) * W + c].im -= d.im;
+      Ys[(r0 + i) * Wp + c].re -= d.re;
+      Ys[(r0 + i) * Wp + c].im -= d.im;
     }
     tm.wave_sync();
   }
@@ -505,7 +540,7 @@ RTB_HD void band_backtrans(Team& tm, int n, int j0, const T* Y, int ldy, const c
   double g2 = 0.0;
   if (live) {
     for (int i = s; i < n; i += S) {
-      const cplx<T> z = Ys[i * W + c];
+      const cplx<T> z = Ys[i * Wp + c];
       const size_t o = rows_out ? static_cast<size_t>(j) * n + i : static_cast<size_t>(i) * n + j;
       Zout[o * 2] = static_cast<TO>(z.re);
       Zout[o * 2 + 1] = static_cast<TO>(z.im);
