@@ -255,6 +255,71 @@ int rtb_hall_direct_noalloc(const rtb_model* m, const size_t* kmesh, const doubl
                             double T, double eta, const double* dir, int spin_dir, int precision,
                             double* sigma);
 
+/* ---------------------------------------------------------------------
+ * Many models in one call.  The reference evaluates Model::hall_conductivity
+ * (Integration::Direct) one model at a time (response/linear/mod.rs:160-225);
+ * a batch runs that loop for every model of the batch, each on its own
+ * uniform mesh, with the Atom gauge and no spin current, as device work
+ * planned once and replayed: the models' k-points in passes that fit the
+ * workspace budget, the Fourier sums of all models of a pass in persistent
+ * work-queue launches (tiles longest first), the eigensolve, Kubo band sum
+ * and per-k band sums per matrix size, captured in a HIP graph per
+ * (precision, n_mu, T, eta).  Results equal rtb_hall_direct per model up to
+ * the order of the k sum (FP64), or within the mixed form's bars.
+ * Switched off unless the environment sets RTB_BATCH_ENABLE=1 (otherwise
+ * rtb_batch_hall_direct returns RTB_ERR_UNSUPPORTED): its first device run
+ * stopped making progress and the cause is not known; its host logic is
+ * checked by emulation.
+ * --------------------------------------------------------------------- */
+typedef struct rtb_batch rtb_batch;
+/* A per-size choice of kernel variants (the autotuned table): fourier_tk,
+ * k-points per thread of the work-queue Fourier sum (1, 2, 4 or 8); eigh_nb,
+ * rows of the register eigensolver's block (0: the smallest of 32, 64, 128
+ * that holds nsta; else one of them, at least nsta, nsta <= 128).  Sizes not
+ * in the table take fourier_tk 2, eigh_nb 0. */
+typedef struct rtb_batch_variant {
+  int nsta;
+  int fourier_tk;
+  int eigh_nb;
+} rtb_batch_variant;
+/* models: n_models handles on one device (no position matrices), which must
+ * outlive the batch; kmesh: [n_models][3] mesh sizes (entries beyond a
+ * model's dim ignored); dirs: [n_models][2][3] the two directions of each
+ * model's conductivity (first dim entries of each row used); table may be
+ * null (n_table 0).  Checks as rtb_hall_direct's. */
+int rtb_batch_create(const rtb_model* const* models, int n_models, const size_t* kmesh, const double* dirs,
+                     const rtb_batch_variant* table, int n_table, rtb_batch** out);
+/* sigma: [n_models][n_mu] in the order of `models`; sum / nk / det(lat) per
+ * model.  The first call of a parameter set plans (per precision) and
+ * captures its graph; later ones replay it.  Calls on one batch are
+ * serialised.  Mixed form: k-points whose refined eigenpairs fail the check
+ * are re-solved in FP64 after the replay (RTB_MIXED_FALLBACK=0: the call
+ * fails with RTB_ERR_EIGEN_FAILED); rtb_last_stats gives nk (all models),
+ * ms[RTB_STAGE_TOTAL] (the graph's device time), launches[RTB_STAGE_TOTAL]
+ * (graph nodes), max_residual and resolved. */
+int rtb_batch_hall_direct(rtb_batch* b, const double* mu, size_t n_mu, double T, double eta, int precision,
+                          double* sigma);
+/* use_graph 1 (the default): capture once per parameter set, replay;
+ * 0: launch the same sequence directly at every call, with per-stage times
+ * in rtb_last_stats (event nodes of a graph give no elapsed times). */
+int rtb_batch_set_graph(rtb_batch* b, int use_graph);
+typedef struct rtb_batch_info {
+  int n_models;
+  int n_passes;      /* of the last call's precision */
+  int n_portions;    /* (pass, matrix size) pairs */
+  int n_tiles;       /* work-queue Fourier tiles */
+  int n_queues;      /* work-queue launches */
+  int n_mesh_models; /* models on the separable mesh route (mixed form) */
+  int graphs_cached;
+  int captures;      /* graphs captured over the batch's life */
+  int graph_nodes;   /* of the last call's graph */
+  size_t nk_total;
+  size_t arena_bytes; /* device memory the batch holds */
+  double graph_ms;    /* the last replay */
+} rtb_batch_info;
+int rtb_batch_info_get(rtb_batch* b, rtb_batch_info* out);
+void rtb_batch_destroy(rtb_batch* b);
+
 /* Model::hall_conductivity, Integration::EnergyCut (response/linear/mod.rs:177-185,
  * 226-268) for a 2D model: per-vertex E and K^{ab}_nm = v^a_nm v^b_mn
  * (compute_velocity_kernel, response/primitives.rs:33-137, FP64), band tracking

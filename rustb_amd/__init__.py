@@ -8,10 +8,12 @@ from ._lib import TbError, last_stats, load
 from .kpoints import gen_kmesh, k_path
 from .model import (Atom, BandBerryCurvature, BandQuantumGeometry, HallConductivityResult, Model,
                     NonlinearHallResult, Parameters, QuantumGeometryMap, QuantumGeometryResult, synth_model)
+from .batch import ModelBatch
 from . import workloads
 
 __all__ = [
     "Model",
+    "ModelBatch",
     "Atom",
     "NonlinearHallResult",
     "BandQuantumGeometry",

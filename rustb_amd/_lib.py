@@ -70,6 +70,16 @@ class Stats(ctypes.Structure):
                 ("nsta", c_int), ("mflop", c_double * 10), ("max_residual", c_double), ("resolved", c_size_t)]
 
 
+class BatchVariant(ctypes.Structure):
+    _fields_ = [("nsta", c_int), ("fourier_tk", c_int), ("eigh_nb", c_int)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("n_models", c_int), ("n_passes", c_int), ("n_portions", c_int), ("n_tiles", c_int),
+                ("n_queues", c_int), ("n_mesh_models", c_int), ("graphs_cached", c_int), ("captures", c_int),
+                ("graph_nodes", c_int), ("nk_total", c_size_t), ("arena_bytes", c_size_t), ("graph_ms", c_double)]
+
+
 _P = c_void_p
 SIGNATURES = {
     "rtb_builder_new": (c_int, [c_int, c_int, c_int, _P, _P, POINTER(c_void_p)]),
@@ -98,6 +108,11 @@ SIGNATURES = {
     "rtb_hall_direct_layers": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, c_int, _P,
                                        _P, _P]),
     "rtb_hall_direct_noalloc": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, c_int, _P]),
+    "rtb_batch_create": (c_int, [_P, c_int, _P, _P, _P, c_int, POINTER(c_void_p)]),
+    "rtb_batch_hall_direct": (c_int, [c_void_p, _P, c_size_t, c_double, c_double, c_int, _P]),
+    "rtb_batch_set_graph": (c_int, [c_void_p, c_int]),
+    "rtb_batch_info_get": (c_int, [c_void_p, POINTER(BatchInfo)]),
+    "rtb_batch_destroy": (None, [c_void_p]),
     "rtb_hall_energy_cut": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, _P]),
     "rtb_energy_cut_host": (c_int, [c_size_t, c_size_t, c_int, _P, _P, _P, _P, _P, c_size_t, c_double, c_double,
                                     c_double, _P]),

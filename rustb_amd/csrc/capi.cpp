@@ -10,6 +10,7 @@
 #include <thread>
 
 #include "eigh.hpp"
+#include "pipeline.hpp"
 
 namespace rtb {
 
@@ -21,6 +22,10 @@ thread_local rtb_stats g_stats;
 std::once_flag g_smem_once;
 int g_smem_rc = RTB_OK;
 
+}  // namespace
+
+// The helpers below are shared with the multi-model batch (batch.cpp)
+// through pipeline.hpp.
 int prepare(const rtb_model* m) {
   if (!m) return fail(RTB_ERR_INVALID_ARGUMENT, "null model");
   RTB_HIP(hipSetDevice(m->device));
@@ -165,12 +170,6 @@ size_t chunk_for(size_t nk, size_t per_k_bytes) {
   return std::min(nk, c);
 }
 
-// The eigen + Kubo pipeline for one chunk of k-points whose H (and V when
-// omega is wanted) are already on the device.
-struct EigBufs {
-  double *Vg, *tau, *Y, *S, *DE, *band;
-};
-
 // Bytes per k-point of alloc_eig's buffers beyond those of n <= 128 (the
 // two-stage route's reflector store, and its matrix when no vectors are
 // asked for).
@@ -202,11 +201,6 @@ int alloc_eig(Work& w, int n, size_t K, bool vectors, EigBufs& b) {
 //                       (default 1e-9);
 //   RTB_MIXED_FALLBACK  0: a flagged k-point fails the call with
 //                       RTB_ERR_EIGEN_FAILED instead of being re-solved in FP64.
-struct MixedOpts {
-  double res_tol = 1e-9;
-  bool fallback = true;
-};
-
 MixedOpts mixed_opts() {
   MixedOpts o;
   if (const char* e = std::getenv("RTB_MIXED_RESIDUAL")) {
@@ -354,17 +348,17 @@ int checks_finish(hipStream_t st, Checks& ck, Timer& tmr) {
 // rebuilt from its upper triangle (models whose storage is not exactly
 // Hermitian), so the solve and the refinement see the reference's matrix.
 int eig_mixed_chunk(hipStream_t st, Timer& tmr, bool from_upper, int n, size_t c, double* H, double* E, EigBufs& eb,
-                    double* Z, double* W1, double* W2, double* W3, double* scale, const RefineCheck& chk) {
+                    double* Z, double* W1, double* W2, double* W3, double* scale, const RefineCheck& chk, int nb) {
   float* Vg = reinterpret_cast<float*>(eb.Vg);
   float* tau = reinterpret_cast<float*>(eb.tau);
   float* Y = reinterpret_cast<float*>(eb.Y);
   float* S = reinterpret_cast<float*>(eb.S);
   float* DE = reinterpret_cast<float*>(eb.DE);
   float* band = reinterpret_cast<float*>(eb.band);
-  RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE, from_upper, band));
+  RTB_TRY(eigh_launch_fp32(st, &tmr, n, c, H, E, true, Vg, tau, Y, S, DE, from_upper, band, nb));
   RTB_TRY(tmr.begin(RTB_STAGE_BACKTRANS));
   float* Zf = reinterpret_cast<float*>(Z);  // exact: the route's values are FP32
-  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Zf, false, scale, band));  // |z_j|^2 into scale
+  RTB_TRY(eigh_backtransform_fp32(st, n, c, Y, DE, S, Vg, tau, Zf, false, scale, band, nb));  // |z_j|^2 into scale
   RTB_TRY(tmr.end(1, flop_backtrans(n, c)));
   // one Ogita-Aishima step in FP64 (vector error squared), Rayleigh quotients,
   // and Rayleigh-Ritz inside clusters closer than 1e-3 of the spectral radius
@@ -378,7 +372,7 @@ int eig_mixed_chunk(hipStream_t st, Timer& tmr, bool from_upper, int n, size_t c
 // two products instead of four, and the band sum reads A, B out of C.
 int kubo_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const double* V, const double* Z,
                const double* zscale, const double* E, int spin_dir, double eta, double* T1, double* C, double* J,
-               double* omega, double* B = nullptr, double* metric = nullptr) {
+               double* omega, double* B, double* metric) {
   const int n = m->nsta;
   const size_t n2 = static_cast<size_t>(n) * n;
   if (B) {
@@ -461,8 +455,6 @@ int dipole_chunk(hipStream_t st, Timer& tmr, const rtb_model* m, size_t K, const
   RTB_TRY(tmr.end(sym ? 3 : 2));
   return RTB_OK;
 }
-
-}  // namespace
 
 void publish_stats(const rtb_stats& s) { g_stats = s; }
 
@@ -1313,6 +1305,31 @@ int rtb_occupied_berry_curvature(const rtb_model* m, const double* k, size_t nk,
 }
 
 }  // extern "C"
+
+namespace rtb {
+
+int berry_points_fp64(const rtb_model* m, const double* k, size_t nk, const double* dir, double eta, double* omega,
+                      double* E, hipStream_t st, Work& w) {
+  if (nk == 0) return RTB_OK;
+  BerryJob j;
+  j.k = k;
+  j.nk = nk;
+  j.dir = dir;
+  j.eta = eta;
+  j.mode = 0;
+  j.omega = omega;
+  j.E = E;
+  j.precision = RTB_PREC_FP64;
+  Timer tmr;  // no timing events
+  Exec ex;
+  ex.st = st;
+  ex.w = &w;
+  ex.tmr = &tmr;
+  return berry_run(m, j, ex);
+}
+
+}  // namespace rtb
+
 
 // The job of hall_conductivity, Integration::Direct, after its validation:
 // validate_rank2 (config.rs:141-147) then the spin check (linear/mod.rs:162-165).

@@ -88,12 +88,17 @@ void launch_tridiag_reg(hipStream_t st, int n, size_t nmat, const double* H, T* 
                      H, Dg, Eg, Vg, taug);
 }
 
+// nb: the block rows (32, 64 or 128, at least n), or 0 for the smallest
+// that holds n.
+inline int reg_block(int n, int nb) { return nb > 0 ? nb : (n <= 32 ? 32 : (n <= 64 ? 64 : 128)); }
+
 template <typename T>
 void tridiag_reg_dispatch(hipStream_t st, int n, size_t nmat, const double* H, T* Dg, T* Eg, cplx<T>* Vg,
-                          cplx<T>* taug) {
-  if (n <= 32) {
+                          cplx<T>* taug, int nb) {
+  nb = reg_block(n, nb);
+  if (nb == 32) {
     launch_tridiag_reg<T, 32>(st, n, nmat, H, Dg, Eg, Vg, taug);
-  } else if (n <= 64) {
+  } else if (nb == 64) {
     launch_tridiag_reg<T, 64>(st, n, nmat, H, Dg, Eg, Vg, taug);
   } else {
     launch_tridiag_reg<T, 128>(st, n, nmat, H, Dg, Eg, Vg, taug);
@@ -110,10 +115,11 @@ void launch_backtrans_reg(hipStream_t st, int n, size_t nmat, const T* Y, const 
 
 template <typename T, typename TO>
 void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* Ts,
-                            const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out, double* gdiag) {
-  if (n <= 32) {
+                            const cplx<T>* Vg, const cplx<T>* taug, TO* Z, int rows_out, double* gdiag, int nb) {
+  nb = reg_block(n, nb);
+  if (nb == 32) {
     launch_backtrans_reg<T, 32, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out, gdiag);
-  } else if (n <= 64) {
+  } else if (nb == 64) {
     launch_backtrans_reg<T, 64, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out, gdiag);
   } else {
     launch_backtrans_reg<T, 128, TO>(st, n, nmat, Y, DE, Ts, Vg, taug, Z, rows_out, gdiag);
@@ -354,8 +360,9 @@ int hermitize_from_upper(hipStream_t st, int n, size_t nmat, double* H) {
 
 template <typename T>
 int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors, T* Vg,
-                T* taug, T* Y, T* S, T* DE, bool from_upper, T* band) {
+                T* taug, T* Y, T* S, T* DE, bool from_upper, T* band, int nb) {
   if (n < 1 || n > kMaxBandN) return fail(RTB_ERR_UNSUPPORTED, "eigensolver supports 1 <= nsta <= 256");
+  if (!eigh_block_valid(n, nb)) return fail(RTB_ERR_INVALID_ARGUMENT, "eigensolver block rows must be 0 or 32, 64, 128 >= nsta");
   if (n > kMaxEighN && !band) return fail(RTB_ERR_INVALID_ARGUMENT, "nsta above 128 needs the band buffers");
   if (nmat == 0) return RTB_OK;
   if (from_upper) {
@@ -369,7 +376,7 @@ int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, doubl
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_TRIDIAG));
   if (n <= kMaxEighN) {
     tridiag_reg_dispatch<T>(st, n, nmat, H, Dg, Eg, reinterpret_cast<cplx<T>*>(Vg),
-                            reinterpret_cast<cplx<T>*>(taug));
+                            reinterpret_cast<cplx<T>*>(taug), nb);
     RTB_HIP(hipGetLastError());
     if (tmr) RTB_TRY(tmr->end(1, flop_tridiag(n, nmat)));
   } else {
@@ -398,11 +405,12 @@ int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, doubl
 
 template <typename T, typename TO>
 int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* DE, const T* S, const T* Vg,
-                       const T* taug, TO* Z, bool rows_out, double* gdiag, const T* band) {
+                       const T* taug, TO* Z, bool rows_out, double* gdiag, const T* band, int nb) {
   if (nmat == 0) return RTB_OK;
+  if (!eigh_block_valid(n, nb)) return fail(RTB_ERR_INVALID_ARGUMENT, "eigensolver block rows must be 0 or 32, 64, 128 >= nsta");
   if (n <= kMaxEighN) {
     backtrans_reg_dispatch<T, TO>(st, n, nmat, Y, DE, S, reinterpret_cast<const cplx<T>*>(Vg),
-                                  reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0, gdiag);
+                                  reinterpret_cast<const cplx<T>*>(taug), Z, rows_out ? 1 : 0, gdiag, nb);
   } else {
     if (!band) return fail(RTB_ERR_INVALID_ARGUMENT, "nsta above 128 needs the band buffers");
     constexpr int W = BandBt<T>::W, S2 = BandBt<T>::S;
@@ -423,25 +431,34 @@ int eigh_backtransform(hipStream_t st, int n, size_t nmat, const T* Y, const T* 
 }  // namespace
 
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper, float* band) {
-  return eigh_launch<float>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper, band);
+                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper, float* band, int nb) {
+  return eigh_launch<float>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper, band, nb);
 }
 
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
                             const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag,
-                            const float* band) {
-  return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, gdiag, band);
+                            const float* band, int nb) {
+  return eigh_backtransform<float, float>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, gdiag, band, nb);
 }
 
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper, double* band) {
-  return eigh_launch<double>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper, band);
+                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper, double* band,
+                     int nb) {
+  return eigh_launch<double>(st, tmr, n, nmat, H, E, vectors, Vg, taug, Y, S, DE, from_upper, band, nb);
 }
 
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
-                            const double* Vg, const double* taug, double* Z, bool rows_out, const double* band) {
-  return eigh_backtransform<double, double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, nullptr, band);
+                            const double* Vg, const double* taug, double* Z, bool rows_out, const double* band,
+                            int nb) {
+  return eigh_backtransform<double, double>(st, n, nmat, Y, DE, S, Vg, taug, Z, rows_out, nullptr, band, nb);
 }
+
+bool eigh_block_valid(int n, int nb) {
+  if (nb == 0) return true;
+  return n <= kMaxEighN && (nb == 32 || nb == 64 || nb == 128) && nb >= n;
+}
+
+int hermitize_upper(hipStream_t st, int n, size_t nmat, double* H) { return hermitize_from_upper(st, n, nmat, H); }
 
 size_t eigh_band_elems(int n) { return n > kMaxEighN ? band_layout(n).total * 2 : 0; }
 

@@ -20,16 +20,24 @@ namespace rtb {
 //   n > 128 (two-stage route): Vg also holds the matrix being reduced (also
 //   without vectors), and `band` ([nmat][eigh_band_elems(n)] in T) the
 //   reflectors of both stages; null for n <= 128.
+//   nb (n <= 128): rows of the register route's block, 32, 64 or 128 and at
+//   least n, or 0 for the smallest that holds n (eigh_block_valid).
 int eigh_launch_fp64(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper, double* band);
+                     double* Vg, double* taug, double* Y, double* S, double* DE, bool from_upper, double* band,
+                     int nb = 0);
+bool eigh_block_valid(int n, int nb);
 // FP32 route (mixed form): H read as FP64 and rounded; E written as FP64.
 int eigh_launch_fp32(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, double* E, bool vectors,
-                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper, float* band);
+                     float* Vg, float* taug, float* Y, float* S, float* DE, bool from_upper, float* band,
+                     int nb = 0);
 // Z: [nmat][n][n] complex FP32 (the values of the FP32 route, exact);
 // gdiag (optional, [nmat][n]): the squared column norms of Z in FP64.
 int eigh_backtransform_fp32(hipStream_t st, int n, size_t nmat, const float* Y, const float* DE, const float* S,
                             const float* Vg, const float* taug, float* Z, bool rows_out, double* gdiag,
-                            const float* band);
+                            const float* band, int nb = 0);
+// H[k] (complex [nmat][n][n]) rebuilt in place from its upper triangle as a
+// Hermitian matrix (what from_upper does before a solve).
+int hermitize_upper(hipStream_t st, int n, size_t nmat, double* H);
 // Real elements of T per matrix the two-stage route keeps in `band` (0 for
 // n <= 128).
 size_t eigh_band_elems(int n);
@@ -72,7 +80,8 @@ int zero_ints(hipStream_t st, int* p, size_t count);
 int transpose_batched(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, double* out);
 // Z = Q Y (complex, [nmat][n][n]); rows_out stores eigenvectors as rows.
 int eigh_backtransform_fp64(hipStream_t st, int n, size_t nmat, const double* Y, const double* DE, const double* S,
-                            const double* Vg, const double* taug, double* Z, bool rows_out, const double* band);
+                            const double* Vg, const double* taug, double* Z, bool rows_out, const double* band,
+                            int nb = 0);
 
 // ---- Fourier sum (fourier.hip) ----
 struct FourierArgs {
@@ -89,6 +98,34 @@ struct FourierArgs {
 // np == 0; [nk][n][n] with a.combine)
 int fourier_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t nk, double* H,
                   double* V);
+// Many models' H and two projected velocities on their uniform meshes in one
+// persistent launch over a table of tiles (fourier.hip).  QueueModel: the
+// Fourier arguments of a model (np == 2) and its mesh; QueueTile: cnt
+// k-points from flat mesh index f0 (gen_kmesh order) of model `model`,
+// elements [e0, e0 + 256) (all n^2 when n^2 <= 256), outputs H [cnt][n][n]
+// and V [cnt][2][n][n] ([cnt][n][n] with combine) from the given pointers.
+// Tiles hold at most fourier_queue_groups(n, tk) * tk k-points; workgroups
+// take tiles from *next (zeroed beforehand) in table order; max_groups: the
+// largest group count among the launch's tiles.
+constexpr int kQueueSlots = 64;  // k-points per tile at most
+struct QueueModel {
+  FourierArgs a;
+  int N[3];
+};
+struct QueueTile {
+  int model;
+  int e0;
+  long long f0;
+  int cnt;
+  double* H;
+  double* V;
+};
+int fourier_queue_groups(int n, int tk);
+int fourier_queue(hipStream_t st, int tk, const QueueModel* models, const QueueTile* tiles, int ntiles,
+                  int max_groups, unsigned* next, int workgroups);
+// Workgroups of the queue kernel that fit one CU at once (dynamic LDS of
+// max_groups * tk k-point slots).
+int fourier_queue_occupancy(int tk, int max_groups, int* per_cu);
 // The position-matrix term of the projected velocities (velocity.rs:377-402,
 // 627-668): V[k][p] += i [H(k), r_p(k)] with r_p(k) the Bloch sum (same
 // phases and gauge as H) of sum_d dirs[p][d] rmat[R][d] and, in the Atom
@@ -108,6 +145,11 @@ int mesh_plan(const rtb_model* m, const size_t* kmesh, MeshPlan& p);
 // passes A/B: work >= box_elems + ga_elems complex; G = gb_elems complex
 int mesh_prepare(hipStream_t st, const rtb_model* m, const MeshPlan& p, int* d_boxidx, double* work, double* G,
                  bool want_v);
+// The same from a device copy of the flat box index of every stored R
+// (mesh_box_index); kernels only (no copies from the host, no memset).
+void mesh_box_index(const rtb_model* m, const MeshPlan& p, std::vector<int>& idx);
+int mesh_prepare_dev(hipStream_t st, const rtb_model* m, const MeshPlan& p, const int* d_boxidx, double* work,
+                     double* G, bool want_v);
 // pass C for pairs [p0, p0 + np_pairs): H [np_pairs*N1][n][n], V [..][2][n][n]
 // ([..][n][n] holding V0 + i V1 with a.combine) in k1-innermost order: local
 // k = (pair - p0) * N1 + i1
@@ -186,6 +228,12 @@ int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const d
 int dipole_combine(hipStream_t st, int n, size_t nk, const double* A2, const double* A3, const double* om1,
                    const double* om2, bool sym, double* out);
 int hall_blocks(size_t nk);
+// out[k][m] = sum_n omega[k][n] f(E[k][n], mu_m, T) per k-point ([nk][n_mu]).
+int kterms(hipStream_t st, int n, size_t nk, const double* omega, const double* E, const double* mu, int n_mu,
+           double T, double* out);
+// out[s][m] = sum over k in [start[s], start[s + 1]) of term[k][m] (start:
+// device, n_seg + 1 entries), in a fixed order.
+int segment_sums(hipStream_t st, const double* term, const long long* start, int n_seg, int n_mu, double* out);
 // Layer-resolved occupied curvature: partial[k][m][l] = sum_j f(E_j, mu_m)
 // Omega_j sum_{i in layer l} |U_ij|^2 (U = Z diag(scale), Z: [nk][n][n]
 // columns; the rows of layer l are perm[bounds[l] .. bounds[l+1])).

@@ -8,6 +8,8 @@
 // in FP64 before sincospi.  h_R[e] is read once per R and reused by the TK
 // k-points of the tile (coalesced along e).  Epilogue: the Atom-gauge
 // similarity conj(phi_m) phi_c and the tau term i (w.tau_c - w.tau_m) H_lat.
+#include <algorithm>
+
 #include "eigh.hpp"
 
 namespace rtb {
@@ -21,6 +23,57 @@ __device__ inline void phase_of(double arg, double& c, double& s) {
   // exp(i 2 pi arg), arg reduced to [-1/2, 1/2] first
   double t = arg - rint(arg);
   sincospi(2.0 * t, &s, &c);
+}
+
+// The epilogue of one k-point kv and element e = (m, c) of its matrices:
+// the Atom-gauge factor conj(phi_m) phi_c, the tau term of the projected
+// velocities, the stores.  ho: complex index of element e of H(k); vo: of
+// element e of the first V matrix of k (NP matrices per k, or one holding
+// V0 + i V1 with a.combine).
+template <int NP>
+__device__ inline void store_elem(const FourierArgs& a, const double* kv, int m, int c, double hr, double hi,
+                                  const double* vr, const double* vi, double* __restrict__ H, size_t ho,
+                                  double* __restrict__ V, size_t vo, size_t n2) {
+  double gr = 1.0, gi = 0.0;
+  if (a.gauge == RTB_GAUGE_ATOM) {
+    // conj(phi_m) phi_c = exp(i 2 pi k.(tau_c - tau_m)); each phase as in the
+    // reference, exp(i 2 pi k.tau) per orbital, multiplied
+    const double* tm = a.tau + static_cast<size_t>(m) * 3;
+    const double* tc = a.tau + static_cast<size_t>(c) * 3;
+    double cm, sm, cc, sc;
+    phase_of(tm[0] * kv[0] + tm[1] * kv[1] + tm[2] * kv[2], cm, sm);
+    phase_of(tc[0] * kv[0] + tc[1] * kv[1] + tc[2] * kv[2], cc, sc);
+    gr = cm * cc + sm * sc;
+    gi = cm * sc - sm * cc;
+  }
+  H[2 * ho] = hr * gr - hi * gi;
+  H[2 * ho + 1] = hr * gi + hi * gr;
+  double cr = 0.0, ci = 0.0;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    // i * acc  (+ i (w.tau_c - w.tau_m) H_lat in the Atom gauge)
+    double xr = -vi[p], xi = vr[p];
+    if (a.gauge == RTB_GAUGE_ATOM) {
+      const double diff = a.wtau[static_cast<size_t>(p) * a.n + c] - a.wtau[static_cast<size_t>(p) * a.n + m];
+      xr += -diff * hi;
+      xi += diff * hr;
+      const double yr = xr * gr - xi * gi, yi = xr * gi + xi * gr;
+      xr = yr;
+      xi = yi;
+    }
+    if (NP == 2 && a.combine) {  // V0 + i V1
+      cr += p == 0 ? xr : -xi;
+      ci += p == 0 ? xi : xr;
+    } else {
+      const size_t o = vo + static_cast<size_t>(p) * n2;
+      V[2 * o] = xr;
+      V[2 * o + 1] = xi;
+    }
+  }
+  if (NP == 2 && a.combine) {
+    V[2 * vo] = cr;
+    V[2 * vo + 1] = ci;
+  }
 }
 
 template <int TK, int NP>
@@ -92,52 +145,19 @@ __global__ void __launch_bounds__(kBlock) fourier_dense_kernel(FourierArgs a, co
   }
   if (e >= n2) return;
   const int m = e / a.n, c = e % a.n;
-  const double* tm = a.tau + static_cast<size_t>(m) * 3;
-  const double* tc = a.tau + static_cast<size_t>(c) * 3;
 #pragma unroll
   for (int kk = 0; kk < TK; ++kk) {
     const int kg = k0 + kk;
     if (kg >= nk) break;
-    double gr = 1.0, gi = 0.0;
-    if (a.gauge == RTB_GAUGE_ATOM) {
-      // conj(phi_m) phi_c = exp(i 2 pi k.(tau_c - tau_m)); each phase as in the
-      // reference, exp(i 2 pi k.tau) per orbital, multiplied
-      double cm, sm, cc, sc;
-      phase_of(tm[0] * Ks[kk][0] + tm[1] * Ks[kk][1] + tm[2] * Ks[kk][2], cm, sm);
-      phase_of(tc[0] * Ks[kk][0] + tc[1] * Ks[kk][1] + tc[2] * Ks[kk][2], cc, sc);
-      gr = cm * cc + sm * sc;
-      gi = cm * sc - sm * cc;
-    }
-    const size_t ho = (static_cast<size_t>(kg) * n2 + e) * 2;
-    H[ho] = hr[kk] * gr - hi[kk] * gi;
-    H[ho + 1] = hr[kk] * gi + hi[kk] * gr;
-    double cr = 0.0, ci = 0.0;
+    double pr[NP > 0 ? NP : 1], pi[NP > 0 ? NP : 1];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-      // i * acc  (+ i (w.tau_c - w.tau_m) H_lat in the Atom gauge)
-      double xr = -vi[p][kk], xi = vr[p][kk];
-      if (a.gauge == RTB_GAUGE_ATOM) {
-        const double diff = a.wtau[static_cast<size_t>(p) * a.n + c] - a.wtau[static_cast<size_t>(p) * a.n + m];
-        xr += -diff * hi[kk];
-        xi += diff * hr[kk];
-        const double yr = xr * gr - xi * gi, yi = xr * gi + xi * gr;
-        xr = yr;
-        xi = yi;
-      }
-      if (NP == 2 && a.combine) {  // V0 + i V1
-        cr += p == 0 ? xr : -xi;
-        ci += p == 0 ? xi : xr;
-      } else {
-        const size_t vo = ((static_cast<size_t>(kg) * NP + p) * n2 + e) * 2;
-        V[vo] = xr;
-        V[vo + 1] = xi;
-      }
+      pr[p] = vr[p][kk];
+      pi[p] = vi[p][kk];
     }
-    if (NP == 2 && a.combine) {
-      const size_t vo = (static_cast<size_t>(kg) * n2 + e) * 2;
-      V[vo] = cr;
-      V[vo + 1] = ci;
-    }
+    const size_t ho = static_cast<size_t>(kg) * n2 + e;
+    const size_t vo = static_cast<size_t>(kg) * (NP == 2 && a.combine ? 1 : NP) * n2 + e;
+    store_elem<NP>(a, Ks[kk], m, c, hr[kk], hi[kk], pr, pi, H, ho, V, vo, n2);
   }
 }
 
@@ -154,6 +174,119 @@ int launch_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t n
   }
   RTB_HIP(hipGetLastError());
   return RTB_OK;
+}
+
+// ---- many models in one launch: a persistent work queue of tiles ----
+//
+// A tile is TK-deep k-point groups of one model on the uniform mesh (k-points
+// generated from flat gen_kmesh indices, first axis outermost) times a block
+// of matrix elements.  With n^2 <= 256 a workgroup holds G = min(256 / n^2,
+// kQueueSlots / TK) groups of TK k-points over all n^2 elements (thread t:
+// element t % n^2, group t / n^2), so small models fill the workgroup; above,
+// G = 1 and the tile is one 256-element block.  Workgroups take tile indices from
+// one atomic counter in the order of the table (the host sorts it by cost,
+// longest first) until the table is exhausted, so every workgroup exits once
+// the counter passes the tile count.  Per (R, element, k) the arithmetic is
+// the dense kernel's: the same phases, stored R order and epilogue.
+template <int TK>
+__global__ void __launch_bounds__(kBlock) fourier_queue_kernel(const QueueModel* __restrict__ models,
+                                                               const QueueTile* __restrict__ tiles, int ntiles,
+                                                               unsigned* __restrict__ next) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* Ur = reinterpret_cast<double*>(smem_raw);  // [ks][kRC], ks = G * TK of the launch's largest G
+  __shared__ double W[2][kRC];
+  __shared__ double Ks[kQueueSlots][3];
+  __shared__ int s_tile;
+  for (;;) {
+    __syncthreads();  // the previous tile's reads of the shared tables are done
+    if (threadIdx.x == 0) s_tile = static_cast<int>(atomicAdd(next, 1u));
+    __syncthreads();
+    const int t = s_tile;
+    if (t >= ntiles) return;
+    const QueueTile tile = tiles[t];
+    const QueueModel& md = models[tile.model];
+    const FourierArgs& a = md.a;
+    const int n2 = a.n * a.n;
+    const int EB = n2 <= kBlock ? n2 : kBlock;
+    const int G = n2 <= kBlock ? min(kBlock / n2, kQueueSlots / TK) : 1;
+    const int ks = G * TK;
+    double* Ui = Ur + static_cast<size_t>(ks) * kRC;
+    const int grp = threadIdx.x / EB;
+    const int e = tile.e0 + threadIdx.x % EB;
+    const bool live = grp < G && e < n2;
+    const int kn = tile.cnt;  // <= ks
+    if (threadIdx.x < kn) {
+      // gen_kmesh (kpoints.rs:49-87): first axis outermost; k_d = i_d / N_d
+      long long f = tile.f0 + threadIdx.x;
+      long long x[3] = {0, 0, 0};
+      for (int d = a.dim - 1; d >= 0; --d) {
+        x[d] = f % md.N[d];
+        f /= md.N[d];
+      }
+      for (int d = 0; d < 3; ++d)
+        Ks[threadIdx.x][d] = d < a.dim ? static_cast<double>(x[d]) / static_cast<double>(md.N[d]) : 0.0;
+    }
+    __syncthreads();
+    double hr[TK], hi[TK], vr[2][TK], vi[2][TK];
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) {
+      hr[kk] = hi[kk] = 0.0;
+      vr[0][kk] = vi[0][kk] = vr[1][kk] = vi[1][kk] = 0.0;
+    }
+    const double2* ham2 = reinterpret_cast<const double2*>(a.ham);
+    for (int R0 = 0; R0 < a.nR; R0 += kRC) {
+      __syncthreads();
+      const int rc = min(kRC, a.nR - R0);
+      for (int q = threadIdx.x; q < ks * kRC; q += kBlock) {
+        const int kk = q / kRC, r = q % kRC;
+        double c = 0.0, s = 0.0;
+        if (r < rc && kk < kn) {
+          const double* Rv = a.R + static_cast<size_t>(R0 + r) * 3;
+          // same association as the reference: R0*k0 + R1*k1 + R2*k2
+          const double arg = Rv[0] * Ks[kk][0] + Rv[1] * Ks[kk][1] + Rv[2] * Ks[kk][2];
+          phase_of(arg, c, s);
+        }
+        Ur[q] = c;
+        Ui[q] = s;
+      }
+      if (threadIdx.x < 2 * kRC) {
+        const int p = threadIdx.x / kRC, r = threadIdx.x % kRC;
+        W[p][r] = r < rc ? a.wR[static_cast<size_t>(p) * a.nR + R0 + r] : 0.0;
+      }
+      __syncthreads();
+      if (live) {
+        const double* Urg = Ur + static_cast<size_t>(grp) * TK * kRC;
+        const double* Uig = Ui + static_cast<size_t>(grp) * TK * kRC;
+        for (int r = 0; r < rc; ++r) {
+          const double2 h = ham2[static_cast<size_t>(R0 + r) * n2 + e];
+          const double w0 = W[0][r], w1 = W[1][r];
+#pragma unroll
+          for (int kk = 0; kk < TK; ++kk) {
+            const double ur = Urg[kk * kRC + r], ui = Uig[kk * kRC + r];
+            const double tr = ur * h.x - ui * h.y;
+            const double ti = ur * h.y + ui * h.x;
+            hr[kk] += tr;
+            hi[kk] += ti;
+            vr[0][kk] += w0 * tr;
+            vi[0][kk] += w0 * ti;
+            vr[1][kk] += w1 * tr;
+            vi[1][kk] += w1 * ti;
+          }
+        }
+      }
+    }
+    if (!live) continue;
+    const int m = e / a.n, c = e % a.n;
+    const size_t vper = a.combine ? 1 : 2;
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) {
+      const int kq = grp * TK + kk;
+      if (kq >= kn) break;
+      const double pr[2] = {vr[0][kk], vr[1][kk]}, pi[2] = {vi[0][kk], vi[1][kk]};
+      store_elem<2>(a, Ks[kq], m, c, hr[kk], hi[kk], pr, pi, tile.H, static_cast<size_t>(kq) * n2 + e, tile.V,
+                    static_cast<size_t>(kq) * vper * n2 + e, static_cast<size_t>(n2));
+    }
+  }
 }
 
 __global__ void mesh_points_kernel(int dim, unsigned long long n1, unsigned long long n2,
@@ -197,6 +330,68 @@ int fourier_dense(hipStream_t st, const FourierArgs& a, const double* k, size_t 
     default:
       return fail(RTB_ERR_UNSUPPORTED, "at most 3 projected directions");
   }
+}
+
+size_t fourier_queue_smem(int tk, int max_groups) {
+  return 2 * sizeof(double) * static_cast<size_t>(tk) * max_groups * kRC;
+}
+
+int fourier_queue_groups(int n, int tk) {
+  const int n2 = n * n;
+  return n2 <= kBlock ? std::min(kBlock / n2, kQueueSlots / tk) : 1;
+}
+
+int fourier_queue(hipStream_t st, int tk, const QueueModel* models, const QueueTile* tiles, int ntiles,
+                  int max_groups, unsigned* next, int workgroups) {
+  if (ntiles == 0) return RTB_OK;
+  if (tk < 1 || max_groups < 1 || max_groups * tk > kQueueSlots || workgroups < 1)
+    return fail(RTB_ERR_INVALID_ARGUMENT, "work queue: bad launch shape");
+  const size_t smem = fourier_queue_smem(tk, max_groups);
+  switch (tk) {
+    case 1:
+      hipLaunchKernelGGL(fourier_queue_kernel<1>, dim3(workgroups), dim3(kBlock), smem, st, models, tiles, ntiles,
+                         next);
+      break;
+    case 2:
+      hipLaunchKernelGGL(fourier_queue_kernel<2>, dim3(workgroups), dim3(kBlock), smem, st, models, tiles, ntiles,
+                         next);
+      break;
+    case 4:
+      hipLaunchKernelGGL(fourier_queue_kernel<4>, dim3(workgroups), dim3(kBlock), smem, st, models, tiles, ntiles,
+                         next);
+      break;
+    case 8:
+      hipLaunchKernelGGL(fourier_queue_kernel<8>, dim3(workgroups), dim3(kBlock), smem, st, models, tiles, ntiles,
+                         next);
+      break;
+    default:
+      return fail(RTB_ERR_INVALID_ARGUMENT, "work queue: k-points per thread must be 1, 2, 4 or 8");
+  }
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int fourier_queue_occupancy(int tk, int max_groups, int* per_cu) {
+  const size_t smem = fourier_queue_smem(tk, max_groups);
+  int b = 0;
+  switch (tk) {
+    case 1:
+      RTB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fourier_queue_kernel<1>, kBlock, smem));
+      break;
+    case 2:
+      RTB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fourier_queue_kernel<2>, kBlock, smem));
+      break;
+    case 4:
+      RTB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fourier_queue_kernel<4>, kBlock, smem));
+      break;
+    case 8:
+      RTB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, fourier_queue_kernel<8>, kBlock, smem));
+      break;
+    default:
+      return fail(RTB_ERR_INVALID_ARGUMENT, "work queue: k-points per thread must be 1, 2, 4 or 8");
+  }
+  *per_cu = std::max(b, 1);
+  return RTB_OK;
 }
 
 // ---- the position-matrix term of the velocity (models with an r-matrix) ----

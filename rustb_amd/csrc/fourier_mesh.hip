@@ -32,6 +32,11 @@ __device__ inline int dmod(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
+__global__ void copy_kernel(size_t count, const double2* __restrict__ x, double2* __restrict__ y) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g < count) y[g] = x[g];
+}
+
 __global__ void scatter_box_kernel(const double2* __restrict__ ham, const int* __restrict__ boxidx, int nR,
                                    size_t n2, double2* __restrict__ box) {
   const size_t g = blockIdx.x * 256ull + threadIdx.x;
@@ -307,16 +312,8 @@ int mesh_plan(const rtb_model* m, const size_t* kmesh, MeshPlan& p) {
   return RTB_OK;
 }
 
-// Passes A and B for the whole mesh (plus the box scatter).
-//   work: >= box_elems + ga_elems complex scratch; G: gb_elems complex output
-int mesh_prepare(hipStream_t st, const rtb_model* m, const MeshPlan& p, int* d_boxidx, double* work, double* G,
-                 bool want_v) {
-  const size_t n2 = static_cast<size_t>(m->nsta) * m->nsta;
-  double2* box = reinterpret_cast<double2*>(work);
-  double2* GA = box + p.box_elems;
-  double2* GB = reinterpret_cast<double2*>(G);
-  // box scatter (R -> flat box index computed on the host)
-  std::vector<int> idx(m->nR);
+void mesh_box_index(const rtb_model* m, const MeshPlan& p, std::vector<int>& idx) {
+  idx.resize(static_cast<size_t>(m->nR));
   for (int r = 0; r < m->nR; ++r) {
     size_t f = 0;
     for (int d = 0; d < 3; ++d) {
@@ -325,8 +322,25 @@ int mesh_prepare(hipStream_t st, const rtb_model* m, const MeshPlan& p, int* d_b
     }
     idx[r] = static_cast<int>(f);
   }
+}
+
+// Passes A and B for the whole mesh (plus the box scatter).
+//   work: >= box_elems + ga_elems complex scratch; G: gb_elems complex output
+int mesh_prepare(hipStream_t st, const rtb_model* m, const MeshPlan& p, int* d_boxidx, double* work, double* G,
+                 bool want_v) {
+  std::vector<int> idx;
+  mesh_box_index(m, p, idx);
   RTB_HIP(hipMemcpyAsync(d_boxidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  RTB_HIP(hipMemsetAsync(box, 0, p.box_elems * sizeof(double2), st));
+  return mesh_prepare_dev(st, m, p, d_boxidx, work, G, want_v);
+}
+
+int mesh_prepare_dev(hipStream_t st, const rtb_model* m, const MeshPlan& p, const int* d_boxidx, double* work,
+                     double* G, bool want_v) {
+  const size_t n2 = static_cast<size_t>(m->nsta) * m->nsta;
+  double2* box = reinterpret_cast<double2*>(work);
+  double2* GA = box + p.box_elems;
+  double2* GB = reinterpret_cast<double2*>(G);
+  RTB_TRY(zero_ints(st, reinterpret_cast<int*>(box), p.box_elems * (sizeof(double2) / sizeof(int))));
   const size_t tot = static_cast<size_t>(m->nR) * n2;
   hipLaunchKernelGGL(scatter_box_kernel, dim3(static_cast<unsigned>((tot + 255) / 256)), dim3(256), 0, st,
                      reinterpret_cast<const double2*>(m->d_ham), d_boxidx, m->nR, n2, box);
@@ -347,7 +361,9 @@ int mesh_prepare(hipStream_t st, const rtb_model* m, const MeshPlan& p, int* d_b
     RTB_TRY(launch_axis(st, box, p.L[0], p.L[1], p.lo[1], n2, p.N[1], want_v ? 2 : 1, 0, 1, GB,
                         want_v ? GB + gsz : nullptr));
   } else {
-    RTB_HIP(hipMemcpyAsync(GB, box, p.box_elems * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(copy_kernel, dim3(static_cast<unsigned>((p.box_elems + 255) / 256)), dim3(256), 0, st,
+                       p.box_elems, box, GB);
+    RTB_HIP(hipGetLastError());
   }
   return RTB_OK;
 }

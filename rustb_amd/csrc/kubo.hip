@@ -167,6 +167,38 @@ __global__ void hall_partials_kernel(int n, size_t nk, const double* __restrict_
   if (threadIdx.x == 0) partial[static_cast<size_t>(blockIdx.x) * n_mu + imu] = red[0];
 }
 
+// Per k-point and chemical potential: out[k][m] = sum_n omega[k][n] f(E[k][n], mu_m)
+// (the band sum of hall_partials, kept per k-point).
+__global__ void kterm_kernel(int n, size_t nk, const double* __restrict__ omega, const double* __restrict__ E,
+                             const double* __restrict__ mu, int n_mu, double width, double* __restrict__ out) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g >= nk * n_mu) return;
+  const size_t k = g / n_mu;
+  const double m = mu[g % n_mu];
+  double s = 0.0;
+  for (int b = 0; b < n; ++b) s += omega[k * n + b] * fermi(E[k * n + b], m, width);
+  out[g] = s;
+}
+
+// out[s][m] = sum_{k in [start[s], start[s + 1])} term[k][m]: one workgroup per
+// (segment, mu), strided partial sums then a fixed tree.
+__global__ void __launch_bounds__(256) segment_sums_kernel(const double* __restrict__ term,
+                                                           const long long* __restrict__ start, int n_mu,
+                                                           double* __restrict__ out) {
+  __shared__ double red[256];
+  const int sg = blockIdx.x, m = blockIdx.y;
+  const long long a = start[sg], b = start[sg + 1];
+  double acc = 0.0;
+  for (long long k = a + threadIdx.x; k < b; k += 256) acc += term[k * n_mu + m];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[static_cast<size_t>(sg) * n_mu + m] = red[0];
+}
+
 constexpr int kDosE = 256;   // energies per block
 constexpr int kDosEig = 2048; // eigenvalues per block
 
@@ -401,6 +433,25 @@ int hall_partials(hipStream_t st, int n, size_t nk, const double* omega, const d
   hipLaunchKernelGGL(hall_partials_kernel, dim3(static_cast<unsigned>(n_blk), static_cast<unsigned>(n_mu)),
                      dim3(256), 0, st, n, nk, omega, E, mu, n_mu, T <= 0.0 ? 0.0 : T * 8.617333262145e-5,
                      deriv ? 1 : 0, partial);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int kterms(hipStream_t st, int n, size_t nk, const double* omega, const double* E, const double* mu, int n_mu,
+           double T, double* out) {
+  const size_t total = nk * static_cast<size_t>(n_mu);
+  if (total == 0) return RTB_OK;
+  hipLaunchKernelGGL(kterm_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, n, nk, omega,
+                     E, mu, n_mu, T <= 0.0 ? 0.0 : T * 8.617333262145e-5, out);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int segment_sums(hipStream_t st, const double* term, const long long* start, int n_seg, int n_mu, double* out) {
+  if (n_seg == 0 || n_mu == 0) return RTB_OK;
+  if (n_mu > 65535) return fail(RTB_ERR_UNSUPPORTED, "at most 65535 chemical potentials");
+  hipLaunchKernelGGL(segment_sums_kernel, dim3(static_cast<unsigned>(n_seg), static_cast<unsigned>(n_mu)), dim3(256),
+                     0, st, term, start, n_mu, out);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }

@@ -125,6 +125,13 @@ class Timer {
   }
   int collect(size_t nk, int n) {
     rtb_stats s{};
+    RTB_TRY(gather(nk, n, &s));
+    publish_stats(s);
+    return RTB_OK;
+  }
+  // the sums without publishing them
+  int gather(size_t nk, int n, rtb_stats* out) {
+    rtb_stats s{};
     s.nk = nk;
     s.nsta = n;
     s.max_residual = max_res_;
@@ -147,7 +154,7 @@ class Timer {
         s.mflop[RTB_STAGE_TOTAL] += s.mflop[i];
       }
     }
-    publish_stats(s);
+    *out = s;
     return RTB_OK;
   }
 
