@@ -321,12 +321,13 @@ int rtb_batch_info_get(rtb_batch* b, rtb_batch_info* out);
 void rtb_batch_destroy(rtb_batch* b);
 
 /* Model::hall_conductivity, Integration::EnergyCut (response/linear/mod.rs:177-185,
- * 226-268) for a 2D model: per-vertex E and K^{ab}_nm = v^a_nm v^b_mn
+ * 226-268) for a 2D or 3D model: per-vertex E and K^{ab}_nm = v^a_nm v^b_mn
  * (compute_velocity_kernel, response/primitives.rs:33-137, FP64), band tracking
- * over the mesh (global_band_track, response/tracking.rs:123-163), the Fermi cut
+ * over the mesh (global_band_track, response/tracking.rs:123-224), the Fermi cut
  * of each cell's four triangles (integrate_fermi_cut_2d, response/energy_cut.rs:
- * 1231-1409), / det(lat).  mu must be sorted ascending (RTB_ERR_INVALID_RESPONSE);
- * a 1D model gives RTB_ERR_INVALID_DIMENSION, a 3D one RTB_ERR_UNSUPPORTED. */
+ * 1231-1409) or five tetrahedra (integrate_fermi_cut_3d, response/energy_cut.rs:
+ * 1650-1850), / det(lat).  mu must be sorted ascending (RTB_ERR_INVALID_RESPONSE);
+ * a 1D model gives RTB_ERR_INVALID_DIMENSION. */
 int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu,
                         double T, double eta, const double* dir, int spin_dir, double* sigma);
 
@@ -338,6 +339,13 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
 int rtb_energy_cut_host(size_t nx, size_t ny, int nsta, const double* E, const double* K,
                         const double* ovx, const double* ovy, const double* mu, size_t n_mu,
                         double T, double eta, double det, double* sigma);
+/* The same for a 3D mesh (vertex (ix ny + iy) nz + iz): ovx [nk - ny nz] =
+ * U_i^H U_{i+ny nz}, ovy [nk - nz] = U_i^H U_{i+nz}, ovz [nk - 1] = U_i^H U_{i+1}
+ * ([nsta][nsta] complex each; ovx, ovy, ovz may be null when nx, ny, nz is 1);
+ * sigma = integrate_fermi_cut_3d after global_band_track, / det. */
+int rtb_energy_cut_host_3d(size_t nx, size_t ny, size_t nz, int nsta, const double* E, const double* K,
+                           const double* ovx, const double* ovy, const double* ovz, const double* mu,
+                           size_t n_mu, double T, double eta, double det, double* sigma);
 
 /* QuantumGeometry::quantum_geometry_on (quantum_geometry.rs:104-140, kernel
  * quantum_geometry_at_impl :149-190) for nk k-points: with A = U^H v_a U,

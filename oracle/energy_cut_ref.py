@@ -1,6 +1,6 @@
 """Test infrastructure, not product code: a CPU restatement of the reference's
-EnergyCut Hall conductivity in two dimensions, the checker for the library's
-rtb_hall_energy_cut.  Pure-Python loops: small meshes only.
+EnergyCut Hall conductivity in two and three dimensions, the checker for the
+library's rtb_hall_energy_cut.  Pure-Python loops: small meshes only.
 
 Followed step by step:
   hall_conductivity, Integration::EnergyCut   src/response/linear/mod.rs:177-185, 226-268
@@ -13,6 +13,13 @@ Followed step by step:
   triangle_area, clip_triangle, combine_bary,
   triangle_occupied_hybrid,
   integrate_fermi_cut_2d(_t0)                 src/response/energy_cut.rs:59-78, 1102-1409
+  3D: build_tetrahedra_3d (CUBE_TETS,
+  TET_VOL_FACTOR)                             src/response/tracking.rs:8-23, 344-412
+  TET_QUAD_PTS_4 / TET_QUAD_WTS_4             src/response/quadrature.rs
+  tet_vol_from_pts, bary_to_phys_3d,
+  sub_tet_vol_3d, combine_bary_4,
+  sub_tet_k_quad, tetrahedron_occupied_hybrid,
+  integrate_fermi_cut_3d(_t0)                 src/response/energy_cut.rs:1446-1850
 ndarray's Array1::linspace(a, b, n) is a + i (b - a) / (n - 1).
 
 Pinned (tests/test_energy_cut.py) by: the clipped-triangle area against a
@@ -22,7 +29,10 @@ constant and of a linear sigma_0; band tracking recovering vertex data whose
 band order was shuffled; greedy assignment of permutation matrices; and the
 reference's own checks (EnergyCut within 3e-2 of Direct on the Haldane
 model, time reversal sigma(t2) = -sigma(-t2), lib.rs:2640-2760) plus the
-Chern plateau in the gap.
+Chern plateau in the gap.  3D: the occupied volume of the tetrahedron cut
+against a brute-force count, the band sum rule, time reversal on a stacked
+Haldane model, and the reference's stacked-QWZ check (plateau 1/(2 pi) within
+3e-3 and Direct within 5e-2, lib.rs:3196-3270).
 """
 from __future__ import annotations
 
@@ -98,14 +108,24 @@ def permute_vertex(v: Vertex, p) -> Vertex:
 
 
 def neighbour_indices(i, k_mesh):
-    """tracking.rs:165-224 (2D): non-periodic +x, -x, +y, -y neighbours."""
-    nx, ny = k_mesh
-    ix, iy = i // ny, i % ny
+    """tracking.rs:165-224 (2D, 3D): non-periodic neighbours in the order
+    +x, -x, +y, -y (+z, -z)."""
+    if len(k_mesh) == 2:
+        nx, ny = k_mesh
+        ix, iy = i // ny, i % ny
+        out = []
+        for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+            jx, jy = ix + dx, iy + dy
+            if 0 <= jx < nx and 0 <= jy < ny:
+                out.append(jx * ny + jy)
+        return out
+    nx, ny, nz = k_mesh
+    ix, iy, iz = i // (ny * nz), (i % (ny * nz)) // nz, i % nz
     out = []
-    for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1)):
-        jx, jy = ix + dx, iy + dy
-        if 0 <= jx < nx and 0 <= jy < ny:
-            out.append(jx * ny + jy)
+    for dx, dy, dz in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+        jx, jy, jz = ix + dx, iy + dy, iz + dz
+        if 0 <= jx < nx and 0 <= jy < ny and 0 <= jz < nz:
+            out.append(jx * ny * nz + jy * nz + jz)
     return out
 
 
@@ -365,6 +385,174 @@ def integrate_fermi_cut_2d(pts, k_mesh, mu, width, eta):
     return thermal_convolution(mu, width, lambda m: integrate_fermi_cut_2d_t0(pts, k_mesh, m, eta))
 
 
+# ------------------------------------------------------------------ 3D
+CUBE_TETS = ((0, 1, 2, 4), (3, 1, 2, 7), (5, 1, 4, 7), (6, 2, 4, 7), (1, 2, 4, 7))  # tracking.rs:8-14
+TET_VOL_FACTOR = (1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 3.0)  # tracking.rs:23
+_QA, _QB = 0.5854101966249685, 0.1381966011250105
+TET_QUAD_PTS_4 = ((_QA, _QB, _QB, _QB), (_QB, _QA, _QB, _QB), (_QB, _QB, _QA, _QB), (_QB, _QB, _QB, _QA))
+TET_QUAD_WTS_4 = (0.25, 0.25, 0.25, 0.25)
+
+
+def tet_vol_from_pts(p0, p1, p2, p3):
+    """energy_cut.rs:1448-1458: |det| / 6 of the edge vectors, expanded along
+    the first row as the reference writes it."""
+    x0, y0, z0 = p0
+    x1, y1, z1 = p1
+    x2, y2, z2 = p2
+    x3, y3, z3 = p3
+    det = ((x1 - x0) * ((y2 - y0) * (z3 - z0) - (z2 - z0) * (y3 - y0))
+           - (y1 - y0) * ((x2 - x0) * (z3 - z0) - (z2 - z0) * (x3 - x0))
+           + (z1 - z0) * ((x2 - x0) * (y3 - y0) - (y2 - y0) * (x3 - x0)))
+    return abs(det) / 6.0
+
+
+def bary_to_phys_3d(c, lam):
+    """energy_cut.rs:1461-1478."""
+    return [lam[0] * c[0][d] + lam[1] * c[1][d] + lam[2] * c[2][d] + lam[3] * c[3][d] for d in range(3)]
+
+
+def sub_tet_vol_3d(c, sub):
+    """energy_cut.rs:1481-1488."""
+    return tet_vol_from_pts(*[bary_to_phys_3d(c, l) for l in sub])
+
+
+def combine_bary_4(alpha, lam):
+    """energy_cut.rs:1492-1500: sum_k alpha_k lam_k."""
+    out = [0.0] * 4
+    for k in range(4):
+        for i in range(4):
+            out[i] += alpha[k] * lam[k][i]
+    return out
+
+
+def sub_tet_k_quad(sub, bands, kmats, n, eta, c):
+    """energy_cut.rs:1503-1526: the 4-point rule on one sub-tetrahedron."""
+    sv = sub_tet_vol_3d(c, sub)
+    if sv < 1e-30:
+        return 0.0
+    total = 0.0
+    for iq in range(4):
+        lam = combine_bary_4(TET_QUAD_PTS_4[iq], sub)
+        total += sv * TET_QUAD_WTS_4[iq] * eval_berry_band_at_lam(n, bands, kmats, lam, eta)
+    return total
+
+
+def tet_cut(e, mu):
+    """The occupied part E <= mu of a linearly interpolated tetrahedron as the
+    sub-tetrahedra (barycentric rows) of tetrahedron_occupied_hybrid
+    (energy_cut.rs:1529-1640), and the sign each enters with (the 3-below case
+    is the full tetrahedron minus one); None when mu is outside (e_min, e_max)
+    within the cut's eps."""
+    eps = ENERGY_CUT_EPS
+    if mu <= min(e) + eps or mu >= max(e) - eps:
+        return None
+    a, b, c, d = sorted(range(4), key=lambda i: e[i])
+
+    def unit(i):
+        l = [0.0] * 4
+        l[i] = 1.0
+        return l
+
+    def cut(i, j):
+        de = e[j] - e[i]
+        t = 0.5 if abs(de) < eps else min(max((mu - e[i]) / de, 0.0), 1.0)
+        l = [0.0] * 4
+        l[i] = 1.0 - t
+        l[j] = t
+        return l
+
+    if mu <= e[b] + eps:
+        return [(1.0, [unit(a), cut(a, b), cut(a, c), cut(a, d)])]
+    if mu <= e[c] + eps:
+        return [(1.0, [unit(a), unit(b), cut(a, c), cut(b, d)]),
+                (1.0, [unit(a), cut(a, c), cut(a, d), cut(b, d)]),
+                (1.0, [unit(b), cut(b, c), cut(b, d), cut(a, c)])]
+    return [(-1.0, [unit(d), cut(a, d), cut(b, d), cut(c, d)])]
+
+
+def tetrahedron_occupied_hybrid(c, e_v, omega_v, bands, kmats, mu, eta, n):
+    """energy_cut.rs:1529-1640: vertex-average Omega for a full tetrahedron,
+    the 4-point rule on the sub-tetrahedra of a partial one."""
+    full_vol = tet_vol_from_pts(*c)
+    if full_vol < ENERGY_CUT_EPS:
+        return 0.0
+    if mu <= min(e_v) + ENERGY_CUT_EPS:
+        return 0.0
+    full_val = full_vol * (omega_v[0] + omega_v[1] + omega_v[2] + omega_v[3]) / 4.0
+    if mu >= max(e_v) - ENERGY_CUT_EPS:
+        return full_val
+    subs = tet_cut(e_v, mu)
+    if subs[0][0] < 0:
+        return full_val - sub_tet_k_quad(subs[0][1], bands, kmats, n, eta, c)
+    total = 0.0
+    for _, sub in subs:
+        total += sub_tet_k_quad(sub, bands, kmats, n, eta, c)
+    return total
+
+
+def build_tetrahedra_3d(ix, iy, iz, nx, ny, nz):
+    """tracking.rs:344-412: the five tetrahedra of cell (ix, iy, iz) on the
+    0-7 body diagonal, as (vertex indices, corner coordinates, volume)."""
+    ixp, iyp, izp = (ix + 1) % nx, (iy + 1) % ny, (iz + 1) % nz
+
+    def idx3(x, y, z):
+        return x * ny * nz + y * nz + z
+
+    cidx = [idx3(ix, iy, iz), idx3(ixp, iy, iz), idx3(ix, iyp, iz), idx3(ixp, iyp, iz),
+            idx3(ix, iy, izp), idx3(ixp, iy, izp), idx3(ix, iyp, izp), idx3(ixp, iyp, izp)]
+
+    def frac(x, y, z):
+        return (x / nx, y / ny, z / nz)
+
+    corners = [frac(ix, iy, iz), frac(ix + 1, iy, iz), frac(ix, iy + 1, iz), frac(ix + 1, iy + 1, iz),
+               frac(ix, iy, iz + 1), frac(ix + 1, iy, iz + 1), frac(ix, iy + 1, iz + 1),
+               frac(ix + 1, iy + 1, iz + 1)]
+    cube_vol = (1.0 / nx) * (1.0 / ny) * (1.0 / nz)
+    return [([cidx[l] for l in t], [corners[l] for l in t], cube_vol * TET_VOL_FACTOR[i])
+            for i, t in enumerate(CUBE_TETS)]
+
+
+def integrate_fermi_cut_3d_t0(pts, k_mesh, mu, eta):
+    """energy_cut.rs:1650-1797, fractional-coordinate measure."""
+    nx, ny, nz = k_mesh
+    n_mu = len(mu)
+    acc = np.zeros(n_mu)
+    for idx in range(nx * ny * nz):
+        iz, iy, ix = idx % nz, (idx // nz) % ny, idx // (ny * nz)
+        for vidx, c, volume in build_tetrahedra_3d(ix, iy, iz, nx, ny, nz):
+            vol = tet_vol_from_pts(*c)
+            if vol < ENERGY_CUT_EPS:
+                continue
+            scale = volume / vol
+            vs = [pts[v] for v in vidx]
+            nsta = vs[0].band.shape[0]
+            om = [eval_berry_kernel(v.band, v.k_ab, eta) for v in vs]
+            bands = [v.band for v in vs]
+            kmats = [v.k_ab for v in vs]
+            for n in range(nsta):
+                e_v = [float(v.band[n]) for v in vs]
+                omega_v = [float(o[n]) for o in om]
+                e_min, e_max = min(e_v), max(e_v)
+                full_val = vol * (omega_v[0] + omega_v[1] + omega_v[2] + omega_v[3]) / 4.0
+                i_partial = _partition_point(mu, lambda x: x <= e_min + ENERGY_CUT_EPS)
+                i_full = _partition_point(mu, lambda x: x < e_max - ENERGY_CUT_EPS)
+                for im in range(i_partial, i_full):
+                    acc[im] += scale * tetrahedron_occupied_hybrid(c, e_v, omega_v, bands, kmats, mu[im], eta, n)
+                if i_full < n_mu:
+                    add = scale * full_val
+                    for im in range(i_full, n_mu):
+                        acc[im] += add
+    return acc
+
+
+def integrate_fermi_cut_3d(pts, k_mesh, mu, width, eta):
+    """energy_cut.rs:1798-1848 (the thermal convolution as in 2D)."""
+    mu = [float(x) for x in mu]
+    if width == 0.0:
+        return integrate_fermi_cut_3d_t0(pts, k_mesh, mu, eta)
+    return thermal_convolution(mu, width, lambda m: integrate_fermi_cut_3d_t0(pts, k_mesh, m, eta))
+
+
 def vertices(model, kmesh, direction, spin=None):
     """The mesh's vertex kernels in gen_kmesh order (last axis fastest)."""
     direction = np.asarray(direction, float)
@@ -372,16 +560,18 @@ def vertices(model, kmesh, direction, spin=None):
 
 
 def hall_conductivity_energy_cut(model, kmesh, direction, mu, T=0.0, eta=1e-3, spin=None, pts=None):
-    """hall_conductivity with Integration::EnergyCut, 2D (linear/mod.rs:177-185,
-    226-268): vertex kernels, band tracking, Fermi cut, divided by det(lat)."""
+    """hall_conductivity with Integration::EnergyCut, 2D and 3D
+    (linear/mod.rs:177-185, 226-268): vertex kernels, band tracking, Fermi
+    cut, divided by det(lat)."""
     direction = O.validate_direction(direction, 2, model.dim)
     mu = np.atleast_1d(np.asarray(mu, float))
     if any(mu[i] > mu[i + 1] for i in range(mu.size - 1)):  # config.rs:291-303
         raise O.OracleError("InvalidResponseParameter", "mu: must be sorted in ascending order")
-    if model.dim != 2:
+    if model.dim not in (2, 3):
         raise O.OracleError("InvalidDimension", str(model.dim))
     if pts is None:
         pts = vertices(model, kmesh, direction, spin)
         global_band_track(pts, list(kmesh))
     width = 0.0 if T <= 0.0 else T * O.BOLTZMANN_CONSTANT_EV_PER_K
-    return np.asarray(integrate_fermi_cut_2d(pts, tuple(kmesh), mu, width, eta)) / np.linalg.det(model.lat)
+    cut = integrate_fermi_cut_2d if model.dim == 2 else integrate_fermi_cut_3d
+    return np.asarray(cut(pts, tuple(kmesh), mu, width, eta)) / np.linalg.det(model.lat)

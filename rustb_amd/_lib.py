@@ -116,6 +116,8 @@ SIGNATURES = {
     "rtb_hall_energy_cut": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, _P]),
     "rtb_energy_cut_host": (c_int, [c_size_t, c_size_t, c_int, _P, _P, _P, _P, _P, c_size_t, c_double, c_double,
                                     c_double, _P]),
+    "rtb_energy_cut_host_3d": (c_int, [c_size_t, c_size_t, c_size_t, c_int, _P, _P, _P, _P, _P, _P, c_size_t,
+                                       c_double, c_double, c_double, _P]),
     "rtb_dos": (c_int, [c_void_p, _P, c_double, c_double, c_size_t, c_double, c_int, _P, _P]),
     "rtb_quantum_geometry": (c_int, [c_void_p, _P, c_size_t, _P, c_double, c_int, _P, _P, _P]),
     "rtb_quantum_geometry_direct": (c_int, [c_void_p, _P, _P, c_size_t, c_double, c_double, _P, c_int, _P, _P]),

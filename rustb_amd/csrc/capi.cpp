@@ -1461,11 +1461,11 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
     set_error("Invalid dimension for operation: %d. Supported dimensions: [2, 3]", m->dim);
     return RTB_ERR_INVALID_DIMENSION;
   }
-  if (m->dim == 3) return fail(RTB_ERR_UNSUPPORTED, "EnergyCut integration is implemented for 2D models only");
   if (!sigma) return fail(RTB_ERR_INVALID_ARGUMENT, "null sigma");
   const int n = m->nsta;
   if (n > kMaxN) return fail(RTB_ERR_UNSUPPORTED, "nsta above 256 is not supported by the eigensolver");
-  const size_t nx = kmesh[0], ny = kmesh[1];
+  // vertex (ix ny + iy) nz + iz of gen_kmesh's order; a 2D mesh is nz = 1
+  const size_t nx = kmesh[0], ny = kmesh[1], nz = m->dim == 3 ? kmesh[2] : 1;
   const size_t n2 = static_cast<size_t>(n) * n;
   Lease lease;
   RTB_TRY(lease.status());
@@ -1481,12 +1481,17 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   // resident for the whole mesh: eigenvectors U, Im K and E per vertex, and
   // the squared neighbour overlaps (what the host reads); per chunk: H, V (2),
   // eigensolver scratch, T1/A/B, J
-  double *dU, *dKv, *dEv, *dOx = nullptr, *dOy = nullptr;
+  // the neighbour pairs of each axis, indexed by the lower vertex: x (i, i +
+  // ny nz), y (i, i + nz), z (i, i + 1), every i whose partner is on the mesh
+  // (the pairs across a row end are formed and never read)
+  const size_t step[3] = {ny * nz, nz, 1};
+  const size_t cnt_ax[3] = {nx > 1 ? nk - step[0] : 0, ny > 1 ? nk - step[1] : 0, nz > 1 ? nk - step[2] : 0};
+  double *dU, *dKv, *dEv, *dOv[3] = {nullptr, nullptr, nullptr};
   RTB_TRY(w.alloc(&dU, nk * n2 * 2));
   RTB_TRY(w.alloc(&dKv, nk * n2));
   RTB_TRY(w.alloc(&dEv, nk * n));
-  if (nx > 1) RTB_TRY(w.alloc(&dOx, (nx - 1) * ny * n2));
-  if (nk > 1) RTB_TRY(w.alloc(&dOy, (nk - 1) * n2));
+  for (int ax = 0; ax < 3; ++ax)
+    if (cnt_ax[ax]) RTB_TRY(w.alloc(&dOv[ax], cnt_ax[ax] * n2));
   const size_t per_k = 24 + n2 * 16 * 3 + n2 * 48 + n2 * 48 + (spinj ? n2 * 16 : 0) + n * 56 +
                        eig_extra_bytes(n, true);
   const size_t K = chunk_for(nk, per_k);
@@ -1541,28 +1546,28 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
     RTB_TRY(energy_cut_vertex(st, n, c, A, B, dKv + k0 * n2));
     RTB_TRY(tmr.end(1));
   }
-  // overlaps of mesh neighbours (i, i + ny) and (i, i + 1): U_i^H U_j in
-  // pieces of at most K products through T1, then their squared moduli
+  // overlaps of mesh neighbours U_i^H U_j in pieces of at most K products
+  // through T1, then their squared moduli
   const size_t piece = std::min<size_t>(K, 32768);
-  const size_t nxp = nx > 1 ? (nx - 1) * ny : 0, nyp = nk - 1;
-  for (int dir2 = 0; dir2 < 2; ++dir2) {
-    const size_t cnt = dir2 ? nyp : nxp, step = dir2 ? 1 : ny;
-    double* dst = dir2 ? dOy : dOx;
-    for (size_t i0 = 0; i0 < cnt; i0 += piece) {
-      const size_t c = std::min(piece, cnt - i0);
+  for (int ax = 0; ax < 3; ++ax) {
+    for (size_t i0 = 0; i0 < cnt_ax[ax]; i0 += piece) {
+      const size_t c = std::min(piece, cnt_ax[ax] - i0);
       RTB_TRY(tmr.begin(RTB_STAGE_GEMM64));
-      RTB_TRY(zgemm_batched(st, n, c, dU + i0 * n2 * 2, n2, true, dU + (i0 + step) * n2 * 2, n2, T1, n2));
+      RTB_TRY(zgemm_batched(st, n, c, dU + i0 * n2 * 2, n2, true, dU + (i0 + step[ax]) * n2 * 2, n2, T1, n2));
       RTB_TRY(tmr.end(1, flop_zgemm(n, c), mflop_zgemm(n, c, false)));
       RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
-      RTB_TRY(abs2(st, c * n2, T1, dst + i0 * n2));
+      RTB_TRY(abs2(st, c * n2, T1, dOv[ax] + i0 * n2));
       RTB_TRY(tmr.end(1));
     }
   }
-  std::vector<double> hE(nk * n), hK(nk * n2), hOx(nxp * n2), hOy(nyp * n2);
+  std::vector<double> hE(nk * n), hK(nk * n2), hOv[3];
   // the overlaps first: tracking (a sequential breadth-first walk on the
   // host) runs while Im K is still being copied
-  if (!hOx.empty()) RTB_HIP(hipMemcpyAsync(hOx.data(), dOx, hOx.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  if (!hOy.empty()) RTB_HIP(hipMemcpyAsync(hOy.data(), dOy, hOy.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  for (int ax = 0; ax < 3; ++ax) {
+    hOv[ax].resize(cnt_ax[ax] * n2);
+    if (!hOv[ax].empty())
+      RTB_HIP(hipMemcpyAsync(hOv[ax].data(), dOv[ax], hOv[ax].size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
   struct Ev {
     hipEvent_t e = nullptr;
     ~Ev() {
@@ -1575,12 +1580,12 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   RTB_HIP(hipMemcpyAsync(hK.data(), dKv, hK.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   RTB_HIP(hipEventSynchronize(ov_done.e));
   std::vector<int> perm;
-  energy_cut_track(nx, ny, n, hOx.data(), hOy.data(), true, perm);
+  energy_cut_track(nx, ny, nz, n, hOv[0].data(), hOv[1].data(), hOv[2].data(), true, perm);
   RTB_HIP(hipStreamSynchronize(st));
   RTB_TRY(checks_finish(st, ck, tmr));
   RTB_TRY(tmr.collect(nk, n));
   const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
-  energy_cut_sigma_2d(nx, ny, n, hE.data(), hK.data(), 1, perm.data(), mu, n_mu, width, eta, sigma);
+  energy_cut_sigma(nx, ny, nz, m->dim, n, hE.data(), hK.data(), 1, perm.data(), mu, n_mu, width, eta, sigma);
   for (size_t i = 0; i < n_mu; ++i) sigma[i] /= m->det_lat;  // linear/mod.rs:267
   return RTB_OK;
 }

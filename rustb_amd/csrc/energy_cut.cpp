@@ -1,10 +1,11 @@
-// EnergyCut Hall conductivity in two dimensions, host part: band tracking over
-// the k-mesh and the per-triangle Fermi cut, from vertex data the device
-// forms (band energies, K^{ab}_nm = v^a_nm v^b_mn, and the raw eigenvector
-// overlaps of mesh neighbours).  Follows hall_conductivity,
-// Integration::EnergyCut (src/response/linear/mod.rs:226-268),
-// global_band_track (src/response/tracking.rs:25-163) and
-// integrate_fermi_cut_2d (src/response/energy_cut.rs:1102-1409).
+// EnergyCut Hall conductivity in two and three dimensions, host part: band
+// tracking over the k-mesh and the per-triangle (per-tetrahedron) Fermi cut,
+// from vertex data the device forms (band energies, K^{ab}_nm = v^a_nm v^b_mn,
+// and the raw eigenvector overlaps of mesh neighbours).  Follows
+// hall_conductivity, Integration::EnergyCut (src/response/linear/mod.rs:226-268),
+// global_band_track (src/response/tracking.rs:25-224), integrate_fermi_cut_2d
+// (src/response/energy_cut.rs:1102-1409) and integrate_fermi_cut_3d
+// (src/response/energy_cut.rs:1446-1850, build_tetrahedra_3d tracking.rs:344-412).
 //
 // Vertex data stays in the order the eigensolver produced; tracking keeps a
 // permutation per vertex (band n of vertex i is raw band perm[i][n]) instead
@@ -30,6 +31,15 @@ constexpr double kQ[3][3] = {{2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
                              {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0},
                              {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0}};
 constexpr double kQW = 1.0 / 3.0;
+// degree-2 symmetric 4-point rule for the tetrahedron (quadrature.rs
+// TET_QUAD_PTS_4, weights 1/4 each)
+constexpr double kTA = 0.5854101966249685, kTB = 0.1381966011250105;
+constexpr double kTQ[4][4] = {{kTA, kTB, kTB, kTB}, {kTB, kTA, kTB, kTB}, {kTB, kTB, kTA, kTB}, {kTB, kTB, kTB, kTA}};
+constexpr double kTW = 0.25;
+// the five tetrahedra of a cube on its 0-7 body diagonal, corners numbered
+// x + 2 y + 4 z, and their volume fractions (tracking.rs:8-23)
+constexpr int kCubeTets[5][4] = {{0, 1, 2, 4}, {3, 1, 2, 7}, {5, 1, 4, 7}, {6, 2, 4, 7}, {1, 2, 4, 7}};
+constexpr double kTetVolFactor[5] = {1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 3.0};
 
 struct Mesh {
   size_t nx, ny;
@@ -41,20 +51,24 @@ struct Mesh {
   const double* ovy;  // [nk-1][n][n] complex, U_i^H U_{i+1}
   const int* perm;    // [nk][n]
   const double* om;   // [nk][n] vertex Omega in tracked band order (or null)
+  size_t nz = 1;      // 3D meshes (vertex index (ix ny + iy) nz + iz)
 };
 
 // Overlaps |<u_a(i)|u_b(j)>|^2 of mesh neighbours (build_overlap_matrix,
 // tracking.rs:25-42) come from the raw products U_i^H U_j; the reference's
 // evec is conj(U), which leaves each modulus unchanged.
 
-// neighbour_indices (tracking.rs:165-224), 2D: +x, -x, +y, -y, not periodic.
-int neighbours(size_t i, size_t nx, size_t ny, size_t out[4]) {
-  const size_t ix = i / ny, iy = i % ny;
+// neighbour_indices (tracking.rs:165-224): +x, -x, +y, -y (+z, -z), not
+// periodic; a 2D mesh is nz = 1.
+int neighbours(size_t i, size_t nx, size_t ny, size_t nz, size_t out[6]) {
+  const size_t ix = i / (ny * nz), iy = (i / nz) % ny, iz = i % nz;
   int c = 0;
-  if (ix + 1 < nx) out[c++] = i + ny;
-  if (ix > 0) out[c++] = i - ny;
-  if (iy + 1 < ny) out[c++] = i + 1;
-  if (iy > 0) out[c++] = i - 1;
+  if (ix + 1 < nx) out[c++] = i + ny * nz;
+  if (ix > 0) out[c++] = i - ny * nz;
+  if (iy + 1 < ny) out[c++] = i + nz;
+  if (iy > 0) out[c++] = i - nz;
+  if (iz + 1 < nz) out[c++] = i + 1;
+  if (iz > 0) out[c++] = i - 1;
   return c;
 }
 
@@ -107,16 +121,17 @@ void vertex_omega(const Mesh& M, size_t v, double eta, double* om) {
   }
 }
 
-// eval_berry_band_at_lam_buf (kernel.rs:138-182).
-double band_at_lam(const Mesh& M, const size_t vs[3], int n, const double lam[3], double eta, std::vector<double>& e,
-                   std::vector<double>& ki) {
+// eval_berry_band_at_lam_buf (kernel.rs:138-182) over NV vertices.
+template <int NV>
+double band_at_lam(const Mesh& M, const size_t vs[NV], int n, const double lam[NV], double eta,
+                   std::vector<double>& e, std::vector<double>& ki) {
   std::fill(e.begin(), e.end(), 0.0);
   std::fill(ki.begin(), ki.end(), 0.0);
-  for (int v = 0; v < 3; ++v) {
+  for (int v = 0; v < NV; ++v) {
     if (lam[v] == 0.0) continue;
     for (int m = 0; m < M.n; ++m) e[m] += band(M, vs[v], m) * lam[v];
   }
-  for (int v = 0; v < 3; ++v) {
+  for (int v = 0; v < NV; ++v) {
     if (lam[v] == 0.0) continue;
     for (int m = 0; m < M.n; ++m) ki[m] += kim(M, vs[v], n, m) * lam[v];
   }
@@ -204,7 +219,7 @@ double hybrid(const Mesh& M, const double c[3][2], const size_t vs[3], const dou
       double lam[3];
       for (int d = 0; d < 3; ++d)  // combine_bary (energy_cut.rs:1172-1178)
         lam[d] = kQ[iq][0] * subs[s][0][d] + kQ[iq][1] * subs[s][1][d] + kQ[iq][2] * subs[s][2][d];
-      total += sa * kQW * band_at_lam(M, vs, n, lam, eta, e, ki);
+      total += sa * kQW * band_at_lam<3>(M, vs, n, lam, eta, e, ki);
     }
   }
   return total;
@@ -268,6 +283,142 @@ void cut_t0_cells(const Mesh& M, size_t i0, size_t i1, const double* mu, size_t 
   }
 }
 
+// ---- three dimensions: tetrahedra ----
+double tet_vol(const double p0[3], const double p1[3], const double p2[3], const double p3[3]) {
+  // tet_vol_from_pts (energy_cut.rs:1448-1458): |det| / 6, first-row expansion
+  const double det = (p1[0] - p0[0]) * ((p2[1] - p0[1]) * (p3[2] - p0[2]) - (p2[2] - p0[2]) * (p3[1] - p0[1])) -
+                     (p1[1] - p0[1]) * ((p2[0] - p0[0]) * (p3[2] - p0[2]) - (p2[2] - p0[2]) * (p3[0] - p0[0])) +
+                     (p1[2] - p0[2]) * ((p2[0] - p0[0]) * (p3[1] - p0[1]) - (p2[1] - p0[1]) * (p3[0] - p0[0]));
+  return std::fabs(det) / 6.0;
+}
+
+// sub_tet_vol_3d with bary_to_phys_3d (energy_cut.rs:1461-1488).
+double sub_tet_vol(const double c[4][3], const double t[4][4]) {
+  double p[4][3];
+  for (int q = 0; q < 4; ++q)
+    for (int d = 0; d < 3; ++d)
+      p[q][d] = t[q][0] * c[0][d] + t[q][1] * c[1][d] + t[q][2] * c[2][d] + t[q][3] * c[3][d];
+  return tet_vol(p[0], p[1], p[2], p[3]);
+}
+
+// sub_tet_k_quad (energy_cut.rs:1503-1526) with combine_bary_4 (:1492-1500).
+double sub_tet_quad(const Mesh& M, const double c[4][3], const size_t vs[4], const double sub[4][4], int n, double eta,
+                    std::vector<double>& e, std::vector<double>& ki) {
+  const double sv = sub_tet_vol(c, sub);
+  if (sv < 1e-30) return 0.0;
+  double total = 0.0;
+  for (int iq = 0; iq < 4; ++iq) {
+    double lam[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 4; ++k)
+      for (int i = 0; i < 4; ++i) lam[i] += kTQ[iq][k] * sub[k][i];
+    total += sv * kTW * band_at_lam<4>(M, vs, n, lam, eta, e, ki);
+  }
+  return total;
+}
+
+// tetrahedron_occupied_hybrid (energy_cut.rs:1529-1640): 0 when empty, the
+// vertex-mean Omega times the volume when full, else the 4-point rule on the
+// occupied sub-tetrahedra (1, 2 or 3 vertices below mu; with 3 below, the full
+// tetrahedron minus the one above).
+double hybrid_tet(const Mesh& M, const double c[4][3], const size_t vs[4], const double e_v[4], const double om_v[4],
+                  double mu, double eta, int n, std::vector<double>& e, std::vector<double>& ki) {
+  const double eps = kCutEps;
+  const double full_vol = tet_vol(c[0], c[1], c[2], c[3]);
+  if (full_vol < eps) return 0.0;
+  const double e_min = std::fmin(std::fmin(e_v[0], e_v[1]), std::fmin(e_v[2], e_v[3]));
+  const double e_max = std::fmax(std::fmax(e_v[0], e_v[1]), std::fmax(e_v[2], e_v[3]));
+  if (mu <= e_min + eps) return 0.0;
+  const double full_val = full_vol * (om_v[0] + om_v[1] + om_v[2] + om_v[3]) / 4.0;
+  if (mu >= e_max - eps) return full_val;
+  int idx[4] = {0, 1, 2, 3};
+  std::stable_sort(idx, idx + 4, [&](int i, int j) { return e_v[i] < e_v[j]; });
+  const int a = idx[0], b = idx[1], cc = idx[2], d = idx[3];
+  auto unit = [](int i, double* l) {
+    for (int q = 0; q < 4; ++q) l[q] = 0.0;
+    l[i] = 1.0;
+  };
+  auto cut = [&](int i, int j, double* l) {
+    for (int q = 0; q < 4; ++q) l[q] = 0.0;
+    const double de = e_v[j] - e_v[i];
+    double t = 0.5;
+    if (!(std::fabs(de) < eps)) {
+      t = (mu - e_v[i]) / de;
+      t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    }
+    l[i] = 1.0 - t;
+    l[j] = t;
+  };
+  double s[4][4];
+  if (mu <= e_v[b] + eps) {
+    unit(a, s[0]), cut(a, b, s[1]), cut(a, cc, s[2]), cut(a, d, s[3]);
+    return sub_tet_quad(M, c, vs, s, n, eta, e, ki);
+  }
+  if (mu <= e_v[cc] + eps) {  // three sub-tetrahedra through the p_ac - p_bd diagonal
+    double total = 0.0;
+    unit(a, s[0]), unit(b, s[1]), cut(a, cc, s[2]), cut(b, d, s[3]);
+    total += sub_tet_quad(M, c, vs, s, n, eta, e, ki);
+    unit(a, s[0]), cut(a, cc, s[1]), cut(a, d, s[2]), cut(b, d, s[3]);
+    total += sub_tet_quad(M, c, vs, s, n, eta, e, ki);
+    unit(b, s[0]), cut(b, cc, s[1]), cut(b, d, s[2]), cut(a, cc, s[3]);
+    total += sub_tet_quad(M, c, vs, s, n, eta, e, ki);
+    return total;
+  }
+  unit(d, s[0]), cut(a, d, s[1]), cut(b, d, s[2]), cut(cc, d, s[3]);
+  return full_val - sub_tet_quad(M, c, vs, s, n, eta, e, ki);
+}
+
+// integrate_fermi_cut_3d_t0 (energy_cut.rs:1650-1797) for the cells idx in
+// [i0, i1) (ix outermost), over the five tetrahedra of build_tetrahedra_3d,
+// into acc [n_mu].
+void cut_t0_cells_3d(const Mesh& M, size_t i0, size_t i1, const double* mu, size_t n_mu, double eta, double* acc) {
+  const size_t nx = M.nx, ny = M.ny, nz = M.nz;
+  const double inv_nx = 1.0 / static_cast<double>(nx), inv_ny = 1.0 / static_cast<double>(ny),
+               inv_nz = 1.0 / static_cast<double>(nz);
+  const double cube_vol = inv_nx * inv_ny * inv_nz;
+  const int n = M.n;
+  std::vector<double> e(n), ki(n);
+  for (size_t idx = i0; idx < i1; ++idx) {
+    const size_t iz = idx % nz, iy = (idx / nz) % ny, ix = idx / (ny * nz);
+    const size_t ixp = (ix + 1) % nx, iyp = (iy + 1) % ny, izp = (iz + 1) % nz;
+    size_t cid[8];
+    double corner[8][3];
+    for (int q = 0; q < 8; ++q) {
+      const size_t x = q & 1 ? ixp : ix, y = q & 2 ? iyp : iy, z = q & 4 ? izp : iz;
+      cid[q] = (x * ny + y) * nz + z;
+      corner[q][0] = static_cast<double>(ix + (q & 1 ? 1 : 0)) * inv_nx;
+      corner[q][1] = static_cast<double>(iy + (q & 2 ? 1 : 0)) * inv_ny;
+      corner[q][2] = static_cast<double>(iz + (q & 4 ? 1 : 0)) * inv_nz;
+    }
+    for (int t = 0; t < 5; ++t) {
+      size_t vs[4];
+      double c[4][3];
+      for (int q = 0; q < 4; ++q) {
+        vs[q] = cid[kCubeTets[t][q]];
+        for (int d = 0; d < 3; ++d) c[q][d] = corner[kCubeTets[t][q]][d];
+      }
+      const double vol = tet_vol(c[0], c[1], c[2], c[3]);
+      if (vol < kCutEps) continue;
+      const double scale = cube_vol * kTetVolFactor[t] / vol;
+      for (int b = 0; b < n; ++b) {
+        const double e_v[4] = {band(M, vs[0], b), band(M, vs[1], b), band(M, vs[2], b), band(M, vs[3], b)};
+        const double om_v[4] = {M.om[vs[0] * n + b], M.om[vs[1] * n + b], M.om[vs[2] * n + b], M.om[vs[3] * n + b]};
+        const double e_min = std::fmin(std::fmin(e_v[0], e_v[1]), std::fmin(e_v[2], e_v[3]));
+        const double e_max = std::fmax(std::fmax(e_v[0], e_v[1]), std::fmax(e_v[2], e_v[3]));
+        const double full_val = vol * (om_v[0] + om_v[1] + om_v[2] + om_v[3]) / 4.0;
+        const size_t i_partial =
+            std::partition_point(mu, mu + n_mu, [&](double x) { return x <= e_min + kCutEps; }) - mu;
+        const size_t i_full = std::partition_point(mu, mu + n_mu, [&](double x) { return x < e_max - kCutEps; }) - mu;
+        for (size_t im = i_partial; im < i_full; ++im)
+          acc[im] += scale * hybrid_tet(M, c, vs, e_v, om_v, mu[im], eta, b, e, ki);
+        if (i_full < n_mu) {
+          const double add = scale * full_val;
+          for (size_t im = i_full; im < n_mu; ++im) acc[im] += add;
+        }
+      }
+    }
+  }
+}
+
 size_t host_threads(size_t items) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   return std::min<size_t>(std::min<size_t>(hw, 16), std::max<size_t>(1, items / 64));
@@ -275,13 +426,19 @@ size_t host_threads(size_t items) {
 
 // The zero-width cut summed over all cells: fixed row blocks on up to 16
 // threads, partial sums added in block order (deterministic).
-void cut_t0(const Mesh& M, const double* mu, size_t n_mu, double eta, double* out) {
-  const size_t cells = M.nx * M.ny;
+void cut_t0(const Mesh& M, int dim, const double* mu, size_t n_mu, double eta, double* out) {
+  const size_t cells = M.nx * M.ny * M.nz;
   const size_t nt = host_threads(cells);
   std::vector<std::vector<double>> part(nt, std::vector<double>(n_mu, 0.0));
   std::vector<std::thread> th;
   for (size_t t = 0; t < nt; ++t)
-    th.emplace_back([&, t] { cut_t0_cells(M, cells * t / nt, cells * (t + 1) / nt, mu, n_mu, eta, part[t].data()); });
+    th.emplace_back([&, t] {
+      const size_t c0 = cells * t / nt, c1 = cells * (t + 1) / nt;
+      if (dim == 3)
+        cut_t0_cells_3d(M, c0, c1, mu, n_mu, eta, part[t].data());
+      else
+        cut_t0_cells(M, c0, c1, mu, n_mu, eta, part[t].data());
+    });
   for (auto& x : th) x.join();
   for (size_t i = 0; i < n_mu; ++i) {
     double s = 0.0;
@@ -301,10 +458,10 @@ double fermi_window_x(double x) {  // energy_cut.rs:59-66
 // global_band_track (tracking.rs:123-163): breadth-first from vertex 0; a new
 // vertex takes greedy_assign of its overlaps with its visited neighbours
 // (summed, then divided by their count).
-void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, bool squared,
-                      std::vector<int>& perm) {
+void energy_cut_track(size_t nx, size_t ny, size_t nz, int n, const double* ovx, const double* ovy,
+                      const double* ovz, bool squared, std::vector<int>& perm) {
   const size_t zs = squared ? 1 : 2;  // doubles per overlap element
-  const size_t nk = nx * ny;
+  const size_t nk = nx * ny * nz;
   perm.resize(nk * n);
   for (size_t i = 0; i < nk; ++i)
     for (int b = 0; b < n; ++b) perm[i * n + b] = b;
@@ -316,33 +473,29 @@ void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const doub
   std::deque<size_t> queue;
   visited[0] = 1;
   queue.push_back(0);
-  size_t nbs[4], vns[4];
+  size_t nbs[6], vns[6];
   while (!queue.empty()) {
     const size_t cur = queue.front();
     queue.pop_front();
-    const int cn = neighbours(cur, nx, ny, nbs);
+    const int cn = neighbours(cur, nx, ny, nz, nbs);
     for (int q = 0; q < cn; ++q) {
       const size_t nb = nbs[q];
       if (visited[nb]) continue;
       int cnt = 0;
-      const int vc = neighbours(nb, nx, ny, vns);
+      const int vc = neighbours(nb, nx, ny, nz, vns);
       for (int r = 0; r < vc; ++r) {
         const size_t vn = vns[r];
         if (!visited[vn]) continue;
         // ov[a][b] = |<tracked band a of vn | raw band b of nb>|^2: element
-        // (raw row r, column b) of U_vn^H U_nb sits at base + 2 (r rs + b cs)
+        // (raw row r, column b) of U_vn^H U_nb sits at base + 2 (r rs + b cs);
+        // the pair's axis from its index step (x first: with ny = 1 the x
+        // and y steps coincide and only x neighbours exist)
         const size_t n2 = static_cast<size_t>(n) * n;
-        const double* base;
+        const size_t lo = std::min(nb, vn), step = std::max(nb, vn) - lo;
+        const double* ov_axis = step == ny * nz ? ovx : (step == nz ? ovy : ovz);
+        const double* base = ov_axis + zs * lo * n2;
         size_t rs = n, cs = 1;
-        if (nb == vn + ny) {
-          base = ovx + zs * vn * n2;
-        } else if (vn == nb + ny) {
-          base = ovx + zs * nb * n2, rs = 1, cs = n;
-        } else if (nb == vn + 1) {
-          base = ovy + zs * vn * n2;
-        } else {
-          base = ovy + zs * nb * n2, rs = 1, cs = n;
-        }
+        if (nb == lo) rs = 1, cs = n;
         for (int a = 0; a < n; ++a) {
           const double* row = base + zs * static_cast<size_t>(perm[vn * n + a]) * rs;
           double* o = ov.data() + static_cast<size_t>(a) * n;
@@ -365,15 +518,16 @@ void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const doub
   }
 }
 
-// integrate_fermi_cut_2d (energy_cut.rs:1355-1409): the zero-width cut, or
-// its convolution with the Fermi window for width > 0.  Fractional measure.
-void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* Ki, size_t ks, const int* perm,
-                         const double* mu, size_t n_mu, double width, double eta, double* out) {
+// integrate_fermi_cut_2d / _3d (energy_cut.rs:1355-1409, 1798-1850): the
+// zero-width cut, or its convolution with the Fermi window for width > 0.
+// Fractional measure; a 2D mesh is nz = 1.
+void energy_cut_sigma(size_t nx, size_t ny, size_t nz, int dim, int n, const double* E, const double* Ki, size_t ks,
+                      const int* perm, const double* mu, size_t n_mu, double width, double eta, double* out) {
   // vertex Omega of every vertex in its tracked band order, once
-  const size_t nk = nx * ny;
+  const size_t nk = nx * ny * nz;
   std::vector<double> om(nk * n);
   {
-    const Mesh P{nx, ny, n, E, Ki, ks, nullptr, nullptr, perm, nullptr};
+    const Mesh P{nx, ny, n, E, Ki, ks, nullptr, nullptr, perm, nullptr, nz};
     const size_t nt = host_threads(nk);
     std::vector<std::thread> th;
     for (size_t t = 0; t < nt; ++t)
@@ -382,9 +536,9 @@ void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const dou
       });
     for (auto& x : th) x.join();
   }
-  const Mesh M{nx, ny, n, E, Ki, ks, nullptr, nullptr, perm, om.data()};
+  const Mesh M{nx, ny, n, E, Ki, ks, nullptr, nullptr, perm, om.data(), nz};
   if (width == 0.0) {
-    cut_t0(M, mu, n_mu, eta, out);
+    cut_t0(M, dim, mu, n_mu, eta, out);
     return;
   }
   const double beta = 1.0 / width;
@@ -402,7 +556,7 @@ void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const dou
   const double step = n_ext > 1 ? (mu_hi - mu_lo) / static_cast<double>(n_ext - 1) : 0.0;
   std::vector<double> mu_ext(n_ext), s0(n_ext);
   for (size_t i = 0; i < n_ext; ++i) mu_ext[i] = mu_lo + step * static_cast<double>(i);
-  cut_t0(M, mu_ext.data(), n_ext, eta, s0.data());
+  cut_t0(M, dim, mu_ext.data(), n_ext, eta, s0.data());
   const double dx = 2.0 * kXCut / static_cast<double>(kXSteps);
   for (size_t i = 0; i < n_mu; ++i) {
     double sum = 0.0;
@@ -436,9 +590,28 @@ int rtb_energy_cut_host(size_t nx, size_t ny, int nsta, const double* E, const d
   for (size_t i = 0; i + 1 < n_mu; ++i)  // validate_sorted (config.rs:291-303)
     if (mu[i] > mu[i + 1]) return fail(RTB_ERR_INVALID_RESPONSE, "mu: must be sorted in ascending order");
   std::vector<int> perm;
-  energy_cut_track(nx, ny, nsta, ovx, ovy, false, perm);
+  energy_cut_track(nx, ny, 1, nsta, ovx, ovy, nullptr, false, perm);
   const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
-  energy_cut_sigma_2d(nx, ny, nsta, E, K + 1, 2, perm.data(), mu, n_mu, width, eta, sigma);
+  energy_cut_sigma(nx, ny, 1, 2, nsta, E, K + 1, 2, perm.data(), mu, n_mu, width, eta, sigma);
+  for (size_t i = 0; i < n_mu; ++i) sigma[i] /= det;  // linear/mod.rs:267
+  return RTB_OK;
+}
+
+int rtb_energy_cut_host_3d(size_t nx, size_t ny, size_t nz, int nsta, const double* E, const double* K,
+                           const double* ovx, const double* ovy, const double* ovz, const double* mu, size_t n_mu,
+                           double T, double eta, double det, double* sigma) {
+  using namespace rtb;
+  if (nx == 0 || ny == 0 || nz == 0 || nsta <= 0) return fail(RTB_ERR_INVALID_KMESH, "kmesh and nsta must be positive");
+  const size_t nk = nx * ny * nz;
+  if (!E || !K || !mu || !sigma || (nx > 1 && !ovx) || (nk > nz && ny > 1 && !ovy) || (nz > 1 && !ovz))
+    return fail(RTB_ERR_INVALID_ARGUMENT, "null argument");
+  if (n_mu == 0) return fail(RTB_ERR_INVALID_RESPONSE, "chemical_potentials: must contain at least one value");
+  for (size_t i = 0; i + 1 < n_mu; ++i)  // validate_sorted (config.rs:291-303)
+    if (mu[i] > mu[i + 1]) return fail(RTB_ERR_INVALID_RESPONSE, "mu: must be sorted in ascending order");
+  std::vector<int> perm;
+  energy_cut_track(nx, ny, nz, nsta, ovx, ovy, ovz, false, perm);
+  const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
+  energy_cut_sigma(nx, ny, nz, 3, nsta, E, K + 1, 2, perm.data(), mu, n_mu, width, eta, sigma);
   for (size_t i = 0; i < n_mu; ++i) sigma[i] /= det;  // linear/mod.rs:267
   return RTB_OK;
 }

@@ -192,10 +192,13 @@ inline double mflop_zgemm(int n, size_t nb, bool herm) {
 // U_i^H U_{i+1} (complex, or their squared moduli when squared).  The 2D
 // Fermi cut of E [nk][n] and Im K [nk][n][n] (Ki, element stride ks) for
 // sorted mu, fractional-coordinate measure, thermal width in eV.
-void energy_cut_track(size_t nx, size_t ny, int n, const double* ovx, const double* ovy, bool squared,
-                      std::vector<int>& perm);
-void energy_cut_sigma_2d(size_t nx, size_t ny, int n, const double* E, const double* Ki, size_t ks, const int* perm,
-                         const double* mu, size_t n_mu, double width, double eta, double* out);
+// 3D (nz > 1, vertex (ix ny + iy) nz + iz): ovx pairs (i, i + ny nz), ovy
+// (i, i + nz), ovz (i, i + 1), each indexed by its lower vertex; the cut is
+// over tetrahedra.  A 2D mesh is nz = 1 (ovz unused).
+void energy_cut_track(size_t nx, size_t ny, size_t nz, int n, const double* ovx, const double* ovy,
+                      const double* ovz, bool squared, std::vector<int>& perm);
+void energy_cut_sigma(size_t nx, size_t ny, size_t nz, int dim, int n, const double* E, const double* Ki, size_t ks,
+                      const int* perm, const double* mu, size_t n_mu, double width, double eta, double* out);
 
 // Largest nsta of the register-resident eigensolver route (one workgroup
 // holds the matrix in registers); above it, up to kMaxN, the two-stage route

@@ -449,8 +449,9 @@ class Model:
 
     def hall_conductivity(self, params: Parameters) -> HallConductivityResult:
         """hall_conductivity (response/linear/mod.rs:160-276): Integration::Direct
-        (the k-sum of occupied Omega), or EnergyCut for a 2D model (band tracking
-        and the per-triangle Fermi cut, always FP64); Simplex is rejected as the
+        (the k-sum of occupied Omega), or EnergyCut for a 2D or 3D model (band
+        tracking and the per-triangle / per-tetrahedron Fermi cut, always FP64);
+        Simplex is rejected as the
         reference does (:166-176)."""
         if params.integration not in ("Direct", "EnergyCut"):
             raise TbError(-6, "integration: hall_conductivity supports Integration::Direct or EnergyCut, not Simplex")

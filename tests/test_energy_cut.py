@@ -1,7 +1,8 @@
-"""EnergyCut Hall conductivity (2D): the oracle pinned by closed forms and the
-reference's own checks, the library's host part (band tracking + Fermi cut,
-rtb_energy_cut_host, no device work) against the oracle on the CPU, and the
-full device path (rtb_hall_energy_cut) against the oracle on the GPU.
+"""EnergyCut Hall conductivity (2D and 3D): the oracle pinned by closed forms
+and the reference's own checks, the library's host part (band tracking +
+Fermi cut, rtb_energy_cut_host / _3d, no device work) against the oracle on
+the CPU, and the full device path (rtb_hall_energy_cut) against the oracle on
+the GPU.
 
 GPU tolerance: 1e-9 relative to max|sigma| (the vertex data differ from the
 oracle's by FP64 rounding of the products and the eigensolver; band tracking
@@ -298,10 +299,10 @@ def test_gpu_energy_cut_errors():
         chain.hall_conductivity(p)
     assert e.value.code == -24
     cube = RM.qwz_stacked_3d(ra.Model.tb_model)
-    p = ra.Parameters.rank2((3, 3, 3), [1.0, 0, 0], [0, 1.0, 0], [0.0], integration="EnergyCut")
+    p = ra.Parameters.rank2((3, 3, 3), [1.0, 0, 0], [0, 1.0, 0], [0.2, 0.1], integration="EnergyCut")
     with pytest.raises(ra.TbError) as e:
         cube.hall_conductivity(p)
-    assert e.value.code == -14
+    assert e.value.code == -6
 
 
 RAGGED = [(1, 5), (5, 1), (1, 1), (2, 2), (3, 7)]
@@ -335,3 +336,163 @@ def test_gpu_ragged_meshes_vs_oracle(mesh):
     got = m.hall_conductivity(p).conductivity
     ref = EC.hall_conductivity_energy_cut(o, mesh, np.eye(2), mu, 0.0, 0.05)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * max(np.abs(ref).max(), 1e-3))
+
+
+# ------------------------------------------------------------------ 3D
+def qwz_broken_3d(make):
+    """The stacked QWZ model with interlayer and diagonal hoppings, so that
+    the bands depend on kz and no two vertices of a tetrahedron share an
+    energy (the smallest gap between them is 1.4e-3 on the 5 x 4 x 3 mesh)."""
+    m = RM.qwz_stacked_3d(make)
+    m.add_hop(0.23, 0, 0, [0, 0, 1])
+    m.add_hop(-0.11, 1, 1, [0, 0, 1])
+    m.add_hop(0.07 + 0.05j, 0, 1, [1, 0, 1])
+    m.add_hop(0.05, 0, 0, [1, 1, 0])
+    m.add_hop(0.031 + 0.017j, 0, 1, [0, 1, 1])
+    m.add_hop(0.043, 1, 1, [1, -1, 0])
+    m.add_hop(0.029j, 0, 1, [1, 2, 0])
+    return m
+
+
+def min_vertex_tie_3d(pts, nx, ny, nz):
+    best = np.inf
+    for ix in range(nx):
+        for iy in range(ny):
+            for iz in range(nz):
+                for vidx, _, _ in EC.build_tetrahedra_3d(ix, iy, iz, nx, ny, nz):
+                    e = np.array([pts[v].band for v in vidx])
+                    for a in range(4):
+                        for b in range(a + 1, 4):
+                            best = min(best, float(np.abs(e[a] - e[b]).min()))
+    return best
+
+
+def bloechl_occupied(V, e, mu):
+    """Volume of {E <= mu} for E linear on a tetrahedron of volume V with
+    vertex energies e (Bloechl, Jepsen and Andersen, PRB 49, 16223 (1994),
+    eqs. B1-B4, the integrated number of states)."""
+    e1, e2, e3, e4 = sorted(e)
+    if mu <= e1:
+        return 0.0
+    if mu <= e2:
+        return V * (mu - e1) ** 3 / ((e2 - e1) * (e3 - e1) * (e4 - e1))
+    if mu <= e3:
+        e21, e31, e41, e32, e42 = e2 - e1, e3 - e1, e4 - e1, e3 - e2, e4 - e2
+        x = mu - e2
+        return V / (e31 * e41) * (e21 ** 2 + 3 * e21 * x + 3 * x ** 2 - (e31 + e42) / (e32 * e42) * x ** 3)
+    if mu < e4:
+        return V * (1.0 - (e4 - mu) ** 3 / ((e4 - e1) * (e4 - e2) * (e4 - e3)))
+    return V
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_tetrahedron_cut_volume_closed_form(seed):
+    """The sub-tetrahedra of the cut (1, 2 or 3 vertices below mu) cover
+    exactly the occupied volume of the linear interpolation."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((4, 3))
+    V = EC.tet_vol_from_pts(*c)
+    e = list(rng.normal(size=4))
+    for mu in np.linspace(min(e) + 1e-6, max(e) - 1e-6, 23):
+        subs = EC.tet_cut(e, mu)
+        got = 0.0
+        for sign, sub in subs:
+            got += EC.sub_tet_vol_3d(c, sub) if sign > 0 else V - EC.sub_tet_vol_3d(c, sub)
+        assert abs(got - bloechl_occupied(V, e, mu)) <= 1e-12 * V
+
+
+def test_3d_sum_rule_all_bands_occupied():
+    m = qwz_broken_3d(O.Model)
+    s = EC.hall_conductivity_energy_cut(m, (4, 3, 3), np.eye(3)[:2], [-0.2, 10.0], 0.0, 0.05)
+    assert abs(s[1]) < 1e-12 and abs(s[0]) > 1e-3
+
+
+def test_3d_qwz_plateau_and_direct():
+    """lib.rs:3219-3270 (nk = 10): the EnergyCut plateau is 1/(2 pi) within
+    3e-3 and EnergyCut is within 5e-2 of Direct, eta = 0.1."""
+    m = RM.qwz_stacked_3d(O.Model)
+    mu = np.array([-2.0, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0])
+    ec = EC.hall_conductivity_energy_cut(m, (10, 10, 4), np.eye(3)[:2], mu, 0.0, 0.1)
+    dr = O.hall_conductivity_direct(m, (10, 10, 4), np.eye(3)[:2], mu, 0.0, 0.1)
+    assert abs(ec[3] - 1 / (2 * math.pi)) < 3e-3
+    assert np.abs(ec - dr).max() < 5e-2
+
+
+def vertex_arrays_3d(pts, nx, ny, nz):
+    U = np.array([p.evec.conj() for p in pts])
+    E = np.ascontiguousarray([p.band for p in pts])
+    K = np.ascontiguousarray([p.k_ab for p in pts], dtype=np.complex128)
+
+    def ov(step):
+        if len(pts) <= step:
+            return np.zeros((1, E.shape[1], E.shape[1]), np.complex128)
+        return np.ascontiguousarray(np.einsum("ian,iam->inm", U[:-step].conj(), U[step:]))
+
+    return E, K, ov(ny * nz), ov(nz), ov(1)
+
+
+def host_sigma_3d(E, K, ovx, ovy, ovz, mesh, n, mu, T, eta, det):
+    mu = np.ascontiguousarray(mu, np.float64)
+    out = np.zeros(mu.size)
+    rc = load().rtb_energy_cut_host_3d(*mesh, n, P(E), P(K), P(ovx), P(ovy), P(ovz), P(mu), mu.size, T, eta, det,
+                                       P(out))
+    return rc, out
+
+
+@pytest.mark.parametrize("T,shuffle", [(0.0, False), (0.0, True), (300.0, True)])
+def test_host_part_3d_matches_oracle(T, shuffle):
+    """rtb_energy_cut_host_3d on oracle vertex data equals the oracle bit for
+    bit; shuffled band orders are restored by the tracking.  (At T > 0 a
+    narrow mu window on a smaller mesh: the oracle's zero-width cut runs on a
+    grid of width / 10 steps.)"""
+    m = qwz_broken_3d(O.Model)
+    mesh = (4, 3, 3) if T == 0.0 else (3, 3, 2)
+    mu = np.linspace(-2.5, 2.5, 21) if T == 0.0 else np.linspace(-0.35, 0.35, 5)
+    pts = EC.vertices(m, mesh, np.eye(3)[:2])
+    ref = EC.hall_conductivity_energy_cut(m, mesh, np.eye(3)[:2], mu, T, 0.05)
+    assert np.abs(ref).max() > 1e-3
+    if shuffle:
+        rng = np.random.default_rng(5)
+        pts = [pts[0]] + [EC.permute_vertex(p, rng.permutation(2)) for p in pts[1:]]
+    rc, out = host_sigma_3d(*vertex_arrays_3d(pts, *mesh), mesh, 2, mu, T, 0.05, float(np.linalg.det(m.lat)))
+    assert rc == 0
+    np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("mesh", [(1, 3, 2), (3, 1, 2), (2, 3, 1), (1, 1, 1)])
+def test_host_part_3d_ragged_meshes(mesh):
+    m = qwz_broken_3d(O.Model)
+    mu = np.linspace(-2.5, 2.5, 9)
+    pts = EC.vertices(m, mesh, np.eye(3)[:2])
+    ref = EC.hall_conductivity_energy_cut(m, mesh, np.eye(3)[:2], mu, 0.0, 0.05)
+    rc, out = host_sigma_3d(*vertex_arrays_3d(pts, *mesh), mesh, 2, mu, 0.0, 0.05, float(np.linalg.det(m.lat)))
+    assert rc == 0
+    np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("T", [0.0, 300.0])
+def test_gpu_3d_vs_oracle(T):
+    m, o = lib_pair(qwz_broken_3d)
+    mesh = (5, 4, 3)
+    mu = np.linspace(-2.5, 2.5, 21)
+    assert min_vertex_tie_3d(EC.vertices(o, mesh, np.eye(3)[:2]), *mesh) > 1e-6
+    p = ra.Parameters.rank2(mesh, [1.0, 0, 0], [0, 1.0, 0], mu, eta=0.05, integration="EnergyCut")
+    p.with_temperature(T)
+    got = m.hall_conductivity(p).conductivity
+    ref = EC.hall_conductivity_energy_cut(o, mesh, np.eye(3)[:2], mu, T, 0.05)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+@pytest.mark.gpu
+def test_gpu_3d_qwz_plateau_and_direct():
+    """lib.rs:3219-3270 on the device, nk = 10 and 14."""
+    m = RM.qwz_stacked_3d(ra.Model.tb_model)
+    mu = np.linspace(-3.0, 3.0, 61)
+    for nk in (10, 14):
+        res = {}
+        for integ in ("Direct", "EnergyCut"):
+            p = ra.Parameters.rank2((nk, nk, 4), [1.0, 0, 0], [0, 1.0, 0], mu, eta=0.1, integration=integ)
+            res[integ] = m.hall_conductivity(p).conductivity
+        assert np.abs(res["Direct"] - res["EnergyCut"]).max() < 5e-2
+        assert abs(res["EnergyCut"][30] - 1 / (2 * math.pi)) < 3e-3
