@@ -12,7 +12,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "librtb_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["builder.cpp", "capi.cpp", "batch.cpp", "workspace.cpp", "wannier90.cpp", "energy_cut.cpp", "fourier.hip", "fourier_mesh.hip", "eigh.hip", "refine.hip", "kubo.hip", "zgemm.hip"]
+SOURCES = ["builder.cpp", "capi.cpp", "batch.cpp", "workspace.cpp", "wannier90.cpp", "energy_cut.cpp", "fourier.hip", "fourier_mesh.hip", "eigh.hip", "refine.hip", "kubo.hip", "zgemm.hip", "energy_cut_dev.hip"]
 
 COMMON = [
     "-O3",

@@ -1560,32 +1560,45 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
       RTB_TRY(tmr.end(1));
     }
   }
-  std::vector<double> hE(nk * n), hK(nk * n2), hOv[3];
-  // the overlaps first: tracking (a sequential breadth-first walk on the
-  // host) runs while Im K is still being copied
+  // the overlaps to the host for the tracking walk; E and Im K stay on the
+  // device for the cut
+  std::vector<double> hOv[3];
   for (int ax = 0; ax < 3; ++ax) {
     hOv[ax].resize(cnt_ax[ax] * n2);
     if (!hOv[ax].empty())
       RTB_HIP(hipMemcpyAsync(hOv[ax].data(), dOv[ax], hOv[ax].size() * sizeof(double), hipMemcpyDeviceToHost, st));
   }
-  struct Ev {
-    hipEvent_t e = nullptr;
-    ~Ev() {
-      if (e) (void)hipEventDestroy(e);
-    }
-  } ov_done;
-  RTB_HIP(hipEventCreateWithFlags(&ov_done.e, hipEventDisableTiming));
-  RTB_HIP(hipEventRecord(ov_done.e, st));
-  RTB_HIP(hipMemcpyAsync(hE.data(), dEv, hE.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  RTB_HIP(hipMemcpyAsync(hK.data(), dKv, hK.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  RTB_HIP(hipEventSynchronize(ov_done.e));
+  RTB_HIP(hipStreamSynchronize(st));
   std::vector<int> perm;
   energy_cut_track(nx, ny, nz, n, hOv[0].data(), hOv[1].data(), hOv[2].data(), true, perm);
-  RTB_HIP(hipStreamSynchronize(st));
+  // vertex Omega in tracked band order, then the cut on the device
+  int* dperm;
+  double* dom;
+  RTB_TRY(w.alloc(&dperm, perm.size()));
+  RTB_TRY(w.alloc(&dom, nk * n));
+  RTB_HIP(hipMemcpyAsync(dperm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
+  RTB_TRY(energy_cut_omega(st, n, nk, dEv, dKv, dperm, eta, dom));
+  RTB_TRY(tmr.end(1));
+  const size_t waves = energy_cut_waves(m->dim, nx, ny, nz, n);
+  const size_t chunk = std::max<size_t>(1, std::min<size_t>(256, (size_t(512) << 20) / (waves * sizeof(double))));
+  double* dpart;
+  RTB_TRY(w.alloc(&dpart, waves * chunk));
+  const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
+  RTB_TRY(energy_cut_convolve(mu, n_mu, width, [&](const double* mv, size_t nm, double* s0) -> int {
+    double *dmu, *ds0;
+    RTB_TRY(w.alloc(&dmu, nm));
+    RTB_TRY(w.alloc(&ds0, nm));
+    RTB_HIP(hipMemcpyAsync(dmu, mv, nm * sizeof(double), hipMemcpyHostToDevice, st));
+    RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
+    RTB_TRY(energy_cut_t0(st, m->dim, nx, ny, nz, n, dEv, dKv, dperm, dom, dmu, nm, eta, dpart, chunk, ds0));
+    RTB_TRY(tmr.end(static_cast<int>(2 * ((nm + chunk - 1) / chunk))));
+    RTB_HIP(hipMemcpyAsync(s0, ds0, nm * sizeof(double), hipMemcpyDeviceToHost, st));
+    RTB_HIP(hipStreamSynchronize(st));
+    return RTB_OK;
+  }, sigma));
   RTB_TRY(checks_finish(st, ck, tmr));
   RTB_TRY(tmr.collect(nk, n));
-  const double width = T <= 0.0 ? 0.0 : T * 8.617333262145e-5;  // thermodynamics.rs:10, 50-59
-  energy_cut_sigma(nx, ny, nz, m->dim, n, hE.data(), hK.data(), 1, perm.data(), mu, n_mu, width, eta, sigma);
   for (size_t i = 0; i < n_mu; ++i) sigma[i] /= m->det_lat;  // linear/mod.rs:267
   return RTB_OK;
 }

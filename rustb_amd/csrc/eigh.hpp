@@ -240,6 +240,17 @@ int segment_sums(hipStream_t st, const double* term, const long long* start, int
 int layer_partials(hipStream_t st, int n, size_t nk, const double* Z, const double* scale, const double* E,
                    const double* omega, const double* mu, int n_mu, double T, const int* perm, const int* bounds,
                    int n_layers, double* partial);
+// EnergyCut Fermi cut on the device (energy_cut_dev.hip): vertex Omega in tracked
+// band order (om [nk][n]; E [nk][n], Ki = Im K [nk][n][n], perm [nk][n]), and
+// the zero-width cut of every (cell, simplex, band) for the sorted mu (device,
+// n_mu) into out (device, n_mu), in launches of `chunk` chemical potentials
+// with partial [energy_cut_waves(...)][chunk] of scratch.
+int energy_cut_omega(hipStream_t st, int n, size_t nk, const double* E, const double* Ki, const int* perm, double eta,
+                     double* om);
+size_t energy_cut_waves(int dim, size_t nx, size_t ny, size_t nz, int n);
+int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int n, const double* E, const double* Ki,
+                  const int* perm, const double* om, const double* mu, size_t n_mu, double eta, double* partial,
+                  size_t chunk, double* out);
 // EnergyCut vertex kernel Im K[b][i][j] = Im(A[b][i][j] B[b][j][i]) ([nk][n][n] real).
 int energy_cut_vertex(hipStream_t st, int n, size_t nk, const double* A, const double* B, double* Ki);
 // x = |z|^2 elementwise (total complex z, real x).

@@ -1,0 +1,334 @@
+// The per-simplex Fermi cut of the EnergyCut Hall conductivity, shared by the
+// host part (energy_cut.cpp) and the device kernel (energy_cut_dev.hip): the same
+// operations in the same order on both, so that the host part reproduces the
+// reference's arithmetic step by step.  Follows triangle_occupied_hybrid and
+// tetrahedron_occupied_hybrid (src/response/energy_cut.rs:1102-1228,
+// 1446-1640), eval_berry_kernel and eval_berry_band_at_lam_buf
+// (src/response/kernel.rs:54-80, 138-182), the quadrature rules of
+// src/response/quadrature.rs and the cells of build_triangles_2d_diagavg /
+// build_tetrahedra_3d (src/response/tracking.rs:8-23, 284-412).
+#pragma once
+
+#include <cmath>
+#include <cstddef>
+
+#if defined(__HIP__)
+#define RTB_CUT __host__ __device__
+#else
+#define RTB_CUT
+#endif
+
+#pragma clang fp contract(off)
+
+namespace rtb {
+namespace cut {
+
+constexpr double kEps = 1e-12;  // ENERGY_CUT_EPS (energy_cut.rs:47)
+// degree-2 symmetric 3-point rule of the triangle (TRI_QUAD_PTS_3, weights 1/3)
+constexpr double kQ3[3][3] = {{2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                              {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0},
+                              {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0}};
+constexpr double kW3 = 1.0 / 3.0;
+// degree-2 symmetric 4-point rule of the tetrahedron (TET_QUAD_PTS_4, weights 1/4)
+constexpr double kTA = 0.5854101966249685, kTB = 0.1381966011250105;
+constexpr double kQ4[4][4] = {{kTA, kTB, kTB, kTB}, {kTB, kTA, kTB, kTB}, {kTB, kTB, kTA, kTB}, {kTB, kTB, kTB, kTA}};
+constexpr double kW4 = 0.25;
+// the five tetrahedra of a cube on its 0-7 body diagonal (corners numbered
+// x + 2 y + 4 z) and their volume fractions
+constexpr int kCubeTets[5][4] = {{0, 1, 2, 4}, {3, 1, 2, 7}, {5, 1, 4, 7}, {6, 2, 4, 7}, {1, 2, 4, 7}};
+constexpr double kTetVolFactor[5] = {1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 3.0};
+
+// Vertex data in the eigensolver's band order, read through the tracking
+// permutation (band a of vertex v is raw band perm[v][a]).
+struct View {
+  int n;
+  const double* E;   // [nk][n]
+  const double* Ki;  // Im K [nk][n][n] at stride ks
+  size_t ks;
+  const int* perm;  // [nk][n]
+};
+
+RTB_CUT inline double band(const View& M, size_t v, int a) { return M.E[v * M.n + M.perm[v * M.n + a]]; }
+RTB_CUT inline double kim(const View& M, size_t v, int a, int b) {
+  const size_t n = M.n;
+  return M.Ki[M.ks * (v * n * n + static_cast<size_t>(M.perm[v * n + a]) * n + M.perm[v * n + b])];
+}
+RTB_CUT inline double dmin(double a, double b) { return a < b ? a : (b < a ? b : a); }
+RTB_CUT inline double dmax(double a, double b) { return a > b ? a : (b > a ? b : a); }
+
+// eval_berry_kernel (kernel.rs:54-80), Omega of tracked band i at vertex v.
+RTB_CUT inline double vertex_omega(const View& M, size_t v, int i, double eta) {
+  const double eta2 = eta * eta;
+  double im = 0.0;
+  for (int m = 0; m < M.n; ++m) {
+    if (m == i) continue;
+    const double de = band(M, v, i) - band(M, v, m);
+    const double denom = de * de + eta2;
+    if (denom < 1e-30) continue;
+    im += kim(M, v, i, m) / denom;
+  }
+  return -2.0 * im;
+}
+
+// eval_berry_band_at_lam_buf (kernel.rs:138-182) over NV vertices: E and row
+// n of Im K interpolated at the barycentric point lam, then the band sum; each
+// interpolated value is formed where it is used (the same sums as with the
+// reference's buffers).
+template <int NV>
+RTB_CUT inline double band_at_lam(const View& M, const size_t* vs, int n, const double* lam, double eta) {
+  double en = 0.0;
+  for (int v = 0; v < NV; ++v)
+    if (lam[v] != 0.0) en += band(M, vs[v], n) * lam[v];
+  const double eta2 = eta * eta;
+  double im = 0.0;
+  for (int m = 0; m < M.n; ++m) {
+    if (m == n) continue;
+    double em = 0.0, km = 0.0;
+    for (int v = 0; v < NV; ++v)
+      if (lam[v] != 0.0) em += band(M, vs[v], m) * lam[v];
+    for (int v = 0; v < NV; ++v)
+      if (lam[v] != 0.0) km += kim(M, vs[v], n, m) * lam[v];
+    const double de = en - em;
+    const double denom = de * de + eta2;
+    if (denom < 1e-30) continue;
+    im += km / denom;
+  }
+  return -2.0 * im;
+}
+
+// indices of e[0..N) in ascending order, ties in index order (a stable sort)
+template <int N>
+RTB_CUT inline void sort_idx(const double* e, int* idx) {
+  for (int i = 0; i < N; ++i) idx[i] = i;
+  for (int i = 1; i < N; ++i) {
+    const int x = idx[i];
+    int j = i;
+    while (j > 0 && e[x] < e[idx[j - 1]]) {
+      idx[j] = idx[j - 1];
+      --j;
+    }
+    idx[j] = x;
+  }
+}
+
+// the point of edge (i, j) where the linear E crosses mu, as barycentrics
+template <int N>
+RTB_CUT inline void cut_point(const double* e, int i, int j, double mu, double* l) {
+  for (int q = 0; q < N; ++q) l[q] = 0.0;
+  const double de = e[j] - e[i];
+  double t = 0.5;
+  if (!(fabs(de) < kEps)) {
+    t = (mu - e[i]) / de;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+  }
+  l[i] = 1.0 - t;
+  l[j] = t;
+}
+
+// ---- triangles ----
+RTB_CUT inline double tri_area(const double c[3][2]) {  // triangle_area (energy_cut.rs:69-78)
+  return 0.5 * fabs((c[1][0] - c[0][0]) * (c[2][1] - c[0][1]) - (c[2][0] - c[0][0]) * (c[1][1] - c[0][1]));
+}
+
+// sub_tri_area_2d with bary_to_phys_2d (energy_cut.rs:1102-1116).
+RTB_CUT inline double sub_area(const double c[3][2], const double t[3][3]) {
+  double p[3][2];
+  for (int q = 0; q < 3; ++q)
+    for (int d = 0; d < 2; ++d) p[q][d] = t[q][0] * c[0][d] + t[q][1] * c[1][d] + t[q][2] * c[2][d];
+  return 0.5 * fabs((p[1][0] - p[0][0]) * (p[2][1] - p[0][1]) - (p[2][0] - p[0][0]) * (p[1][1] - p[0][1]));
+}
+
+// clip_triangle (energy_cut.rs:1120-1169): the occupied part E <= mu as 0-2
+// sub-triangles (barycentric rows).
+RTB_CUT inline int clip(const double e[3], double mu, double out[2][3][3]) {
+  const double e_min = dmin(dmin(e[0], e[1]), e[2]);
+  const double e_max = dmax(dmax(e[0], e[1]), e[2]);
+  for (int s = 0; s < 2; ++s)
+    for (int q = 0; q < 3; ++q)
+      for (int d = 0; d < 3; ++d) out[s][q][d] = 0.0;
+  if (mu <= e_min + kEps) return 0;
+  if (mu >= e_max - kEps) {
+    for (int q = 0; q < 3; ++q) out[0][q][q] = 1.0;
+    return 1;
+  }
+  int idx[3];
+  sort_idx<3>(e, idx);
+  const int a = idx[0], b = idx[1], c = idx[2];
+  if (mu <= e[b] + kEps) {
+    out[0][0][a] = 1.0;
+    cut_point<3>(e, a, b, mu, out[0][1]);
+    cut_point<3>(e, a, c, mu, out[0][2]);
+    return 1;
+  }
+  out[0][0][a] = 1.0;
+  out[0][1][b] = 1.0;
+  cut_point<3>(e, a, c, mu, out[0][2]);
+  out[1][0][b] = 1.0;
+  cut_point<3>(e, a, c, mu, out[1][1]);
+  cut_point<3>(e, b, c, mu, out[1][2]);
+  return 2;
+}
+
+// triangle_occupied_hybrid (energy_cut.rs:1184-1228): 0 when empty, the
+// vertex-mean Omega times the area when full, else the 3-point rule on each
+// clipped sub-triangle.
+RTB_CUT inline double hybrid_tri(const View& M, const double c[3][2], const size_t vs[3], const double e_v[3],
+                                 const double om_v[3], double mu, double eta, int n) {
+  const double e_min = dmin(dmin(e_v[0], e_v[1]), e_v[2]);
+  const double e_max = dmax(dmax(e_v[0], e_v[1]), e_v[2]);
+  const double area = tri_area(c);
+  if (mu <= e_min + kEps) return 0.0;
+  if (mu >= e_max - kEps) return area * (om_v[0] + om_v[1] + om_v[2]) / 3.0;
+  double subs[2][3][3];
+  const int ns = clip(e_v, mu, subs);
+  double total = 0.0;
+  for (int s = 0; s < ns; ++s) {
+    const double sa = sub_area(c, subs[s]);
+    if (sa < 1e-30) continue;
+    for (int iq = 0; iq < 3; ++iq) {
+      double lam[3];
+      for (int d = 0; d < 3; ++d)  // combine_bary (energy_cut.rs:1172-1178)
+        lam[d] = kQ3[iq][0] * subs[s][0][d] + kQ3[iq][1] * subs[s][1][d] + kQ3[iq][2] * subs[s][2][d];
+      total += sa * kW3 * band_at_lam<3>(M, vs, n, lam, eta);
+    }
+  }
+  return total;
+}
+
+// ---- tetrahedra ----
+// tet_vol_from_pts (energy_cut.rs:1448-1458): |det| / 6, first-row expansion.
+RTB_CUT inline double tet_vol(const double p0[3], const double p1[3], const double p2[3], const double p3[3]) {
+  const double det = (p1[0] - p0[0]) * ((p2[1] - p0[1]) * (p3[2] - p0[2]) - (p2[2] - p0[2]) * (p3[1] - p0[1])) -
+                     (p1[1] - p0[1]) * ((p2[0] - p0[0]) * (p3[2] - p0[2]) - (p2[2] - p0[2]) * (p3[0] - p0[0])) +
+                     (p1[2] - p0[2]) * ((p2[0] - p0[0]) * (p3[1] - p0[1]) - (p2[1] - p0[1]) * (p3[0] - p0[0]));
+  return fabs(det) / 6.0;
+}
+
+// sub_tet_vol_3d with bary_to_phys_3d (energy_cut.rs:1461-1488).
+RTB_CUT inline double sub_tet_vol(const double c[4][3], const double t[4][4]) {
+  double p[4][3];
+  for (int q = 0; q < 4; ++q)
+    for (int d = 0; d < 3; ++d)
+      p[q][d] = t[q][0] * c[0][d] + t[q][1] * c[1][d] + t[q][2] * c[2][d] + t[q][3] * c[3][d];
+  return tet_vol(p[0], p[1], p[2], p[3]);
+}
+
+// sub_tet_k_quad (energy_cut.rs:1503-1526) with combine_bary_4 (:1492-1500).
+RTB_CUT inline double sub_tet_quad(const View& M, const double c[4][3], const size_t vs[4], const double sub[4][4],
+                                   int n, double eta) {
+  const double sv = sub_tet_vol(c, sub);
+  if (sv < 1e-30) return 0.0;
+  double total = 0.0;
+  for (int iq = 0; iq < 4; ++iq) {
+    double lam[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 4; ++k)
+      for (int i = 0; i < 4; ++i) lam[i] += kQ4[iq][k] * sub[k][i];
+    total += sv * kW4 * band_at_lam<4>(M, vs, n, lam, eta);
+  }
+  return total;
+}
+
+// tetrahedron_occupied_hybrid (energy_cut.rs:1529-1640): 0 when empty, the
+// vertex-mean Omega times the volume when full, else the 4-point rule on the
+// occupied sub-tetrahedra (1, 2 or 3 vertices below mu; with 3 below, the full
+// tetrahedron minus the one above).
+RTB_CUT inline double hybrid_tet(const View& M, const double c[4][3], const size_t vs[4], const double e_v[4],
+                                 const double om_v[4], double mu, double eta, int n) {
+  const double full_vol = tet_vol(c[0], c[1], c[2], c[3]);
+  if (full_vol < kEps) return 0.0;
+  const double e_min = dmin(dmin(e_v[0], e_v[1]), dmin(e_v[2], e_v[3]));
+  const double e_max = dmax(dmax(e_v[0], e_v[1]), dmax(e_v[2], e_v[3]));
+  if (mu <= e_min + kEps) return 0.0;
+  const double full_val = full_vol * (om_v[0] + om_v[1] + om_v[2] + om_v[3]) / 4.0;
+  if (mu >= e_max - kEps) return full_val;
+  int idx[4];
+  sort_idx<4>(e_v, idx);
+  const int a = idx[0], b = idx[1], cc = idx[2], d = idx[3];
+  double s[4][4];
+  auto unit = [](int i, double* l) {
+    for (int q = 0; q < 4; ++q) l[q] = 0.0;
+    l[i] = 1.0;
+  };
+  if (mu <= e_v[b] + kEps) {
+    unit(a, s[0]);
+    cut_point<4>(e_v, a, b, mu, s[1]);
+    cut_point<4>(e_v, a, cc, mu, s[2]);
+    cut_point<4>(e_v, a, d, mu, s[3]);
+    return sub_tet_quad(M, c, vs, s, n, eta);
+  }
+  if (mu <= e_v[cc] + kEps) {  // three sub-tetrahedra through the p_ac - p_bd diagonal
+    double total = 0.0;
+    unit(a, s[0]);
+    unit(b, s[1]);
+    cut_point<4>(e_v, a, cc, mu, s[2]);
+    cut_point<4>(e_v, b, d, mu, s[3]);
+    total += sub_tet_quad(M, c, vs, s, n, eta);
+    unit(a, s[0]);
+    cut_point<4>(e_v, a, cc, mu, s[1]);
+    cut_point<4>(e_v, a, d, mu, s[2]);
+    cut_point<4>(e_v, b, d, mu, s[3]);
+    total += sub_tet_quad(M, c, vs, s, n, eta);
+    unit(b, s[0]);
+    cut_point<4>(e_v, b, cc, mu, s[1]);
+    cut_point<4>(e_v, b, d, mu, s[2]);
+    cut_point<4>(e_v, a, cc, mu, s[3]);
+    total += sub_tet_quad(M, c, vs, s, n, eta);
+    return total;
+  }
+  unit(d, s[0]);
+  cut_point<4>(e_v, a, d, mu, s[1]);
+  cut_point<4>(e_v, b, d, mu, s[2]);
+  cut_point<4>(e_v, cc, d, mu, s[3]);
+  return full_val - sub_tet_quad(M, c, vs, s, n, eta);
+}
+
+// ---- cells ----
+// The triangles of 2D cell (ix, iy) over both diagonals
+// (build_triangles_2d_diagavg, tracking.rs:284-342): vertices, fractional
+// coordinates; each triangle carries a quarter of the cell.
+RTB_CUT inline void cell_triangles(size_t ix, size_t iy, size_t nx, size_t ny, size_t vs[4][3],
+                                   double c[4][3][2]) {
+  const double inv_nx = 1.0 / static_cast<double>(nx), inv_ny = 1.0 / static_cast<double>(ny);
+  const size_t ixp = (ix + 1) % nx, iyp = (iy + 1) % ny;
+  const size_t i00 = ix * ny + iy, i10 = ixp * ny + iy, i11 = ixp * ny + iyp, i01 = ix * ny + iyp;
+  const double fx = static_cast<double>(ix), fy = static_cast<double>(iy);
+  const double x0 = fx * inv_nx, x1 = (fx + 1.0) * inv_nx, y0 = fy * inv_ny, y1 = (fy + 1.0) * inv_ny;
+  // corners 00, 10, 11, 01 (a corner shared by two vertices of a ragged
+  // mesh keeps the coordinates of its first match, as the reference's lookup)
+  const size_t id[4] = {i00, i10, i11, i01};
+  const double px[4] = {x0, x1, x1, x0}, py[4] = {y0, y0, y1, y1};
+  const int tri[4][3] = {{0, 1, 3}, {2, 1, 3}, {0, 1, 2}, {0, 2, 3}};
+  for (int t = 0; t < 4; ++t)
+    for (int q = 0; q < 3; ++q) {
+      const size_t v = id[tri[t][q]];
+      int k = 0;
+      while (id[k] != v) ++k;  // first corner with this vertex
+      vs[t][q] = v;
+      c[t][q][0] = px[k];
+      c[t][q][1] = py[k];
+    }
+}
+
+// The five tetrahedra of 3D cell (ix, iy, iz) (build_tetrahedra_3d,
+// tracking.rs:344-412): vertices, fractional coordinates, volumes.
+RTB_CUT inline void cell_tetrahedra(size_t ix, size_t iy, size_t iz, size_t nx, size_t ny, size_t nz, size_t vs[5][4],
+                                    double c[5][4][3], double vol[5]) {
+  const double inv_nx = 1.0 / static_cast<double>(nx), inv_ny = 1.0 / static_cast<double>(ny),
+               inv_nz = 1.0 / static_cast<double>(nz);
+  const double cube_vol = inv_nx * inv_ny * inv_nz;
+  const size_t ixp = (ix + 1) % nx, iyp = (iy + 1) % ny, izp = (iz + 1) % nz;
+  for (int t = 0; t < 5; ++t) {
+    for (int q = 0; q < 4; ++q) {
+      const int k = kCubeTets[t][q];
+      const size_t x = k & 1 ? ixp : ix, y = k & 2 ? iyp : iy, z = k & 4 ? izp : iz;
+      vs[t][q] = (x * ny + y) * nz + z;
+      c[t][q][0] = static_cast<double>(ix + (k & 1 ? 1 : 0)) * inv_nx;
+      c[t][q][1] = static_cast<double>(iy + (k & 2 ? 1 : 0)) * inv_ny;
+      c[t][q][2] = static_cast<double>(iz + (k & 4 ? 1 : 0)) * inv_nz;
+    }
+    vol[t] = cube_vol * kTetVolFactor[t];
+  }
+}
+
+}  // namespace cut
+}  // namespace rtb

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -199,6 +200,10 @@ void energy_cut_track(size_t nx, size_t ny, size_t nz, int n, const double* ovx,
                       const double* ovz, bool squared, std::vector<int>& perm);
 void energy_cut_sigma(size_t nx, size_t ny, size_t nz, int dim, int n, const double* E, const double* Ki, size_t ks,
                       const int* perm, const double* mu, size_t n_mu, double width, double eta, double* out);
+// The thermal convolution of a zero-width cut t0(mu, n_mu, out) (t0 itself
+// for width 0).
+int energy_cut_convolve(const double* mu, size_t n_mu, double width,
+                        const std::function<int(const double*, size_t, double*)>& t0, double* out);
 
 // Largest nsta of the register-resident eigensolver route (one workgroup
 // holds the matrix in registers); above it, up to kMaxN, the two-stage route
