@@ -261,5 +261,8 @@ int dos_partials(hipStream_t st, size_t ne, const double* evals, double Emin, do
 int dos_blocks(size_t ne);
 // out[j] = scale * sum_blk partial[blk][j]
 int reduce_partials(hipStream_t st, const double* partial, int n_blk, int m, double scale, double* out);
+// out[j] = sum over n_rows rows of partial[r][j] (one workgroup per column, a
+// fixed order): for many rows and few columns.
+int column_sums(hipStream_t st, const double* partial, size_t n_rows, int m, double* out);
 
 }  // namespace rtb

@@ -7,8 +7,8 @@
 //
 // One lane per (cell, simplex, band), lanes of a wave on consecutive bands;
 // for each mu the 64 lanes' contributions are summed by DPP in a fixed order
-// into a per-wave partial, and the partials are summed in wave order
-// (reduce_partials): the result does not depend on scheduling.
+// into a per-wave partial, and the partials are summed per mu in a fixed
+// order (column_sums): the result does not depend on scheduling.
 #include "eigh.hpp"
 #include "energy_cut_core.hpp"
 #include "gpu_team.hpp"
@@ -135,7 +135,7 @@ int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int 
     else
       hipLaunchKernelGGL(cut_kernel<3>, dim3(blocks), dim3(256), 0, st, V, om, nx, ny, nz, mu + m0, c, eta, partial);
     RTB_HIP(hipGetLastError());
-    RTB_TRY(reduce_partials(st, partial, static_cast<int>(waves), c, 1.0, out + m0));
+    RTB_TRY(column_sums(st, partial, waves, c, out + m0));
   }
   return RTB_OK;
 }

@@ -199,6 +199,23 @@ __global__ void __launch_bounds__(256) segment_sums_kernel(const double* __restr
   if (threadIdx.x == 0) out[static_cast<size_t>(sg) * n_mu + m] = red[0];
 }
 
+// out[j] = sum_r partial[r][j] for r < n_rows: one workgroup per column,
+// strided partial sums then a fixed tree (for many rows).
+__global__ void __launch_bounds__(256) column_sums_kernel(const double* __restrict__ partial, size_t n_rows, int m,
+                                                          double* __restrict__ out) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  double acc = 0.0;
+  for (size_t r = threadIdx.x; r < n_rows; r += 256) acc += partial[r * m + j];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
 constexpr int kDosE = 256;   // energies per block
 constexpr int kDosEig = 2048; // eigenvalues per block
 
@@ -452,6 +469,13 @@ int segment_sums(hipStream_t st, const double* term, const long long* start, int
   if (n_mu > 65535) return fail(RTB_ERR_UNSUPPORTED, "at most 65535 chemical potentials");
   hipLaunchKernelGGL(segment_sums_kernel, dim3(static_cast<unsigned>(n_seg), static_cast<unsigned>(n_mu)), dim3(256),
                      0, st, term, start, n_mu, out);
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int column_sums(hipStream_t st, const double* partial, size_t n_rows, int m, double* out) {
+  if (m == 0) return RTB_OK;
+  hipLaunchKernelGGL(column_sums_kernel, dim3(static_cast<unsigned>(m)), dim3(256), 0, st, partial, n_rows, m, out);
   RTB_HIP(hipGetLastError());
   return RTB_OK;
 }
