@@ -169,16 +169,34 @@ RTB_CUT inline int clip(const double e[3], double mu, double out[2][3][3]) {
   return 2;
 }
 
+// The band sum of eval_berry_band_at_lam_buf in the reference's order, for the
+// hybrid rules below.  Those take any evaluator `f` whose f.at<NV>(lam) is the
+// band sum at the barycentric point lam, or one lane's share of it when the
+// sum over m is split across lanes; `one` weighs the rules' constant terms
+// (1 here, 1 on one lane and 0 on the others of a split sum), so that the
+// lanes' results add up to the rule.
+struct SeqBandSum {
+  View M;
+  const size_t* vs;
+  int n;
+  double eta;
+  template <int NV>
+  RTB_CUT double at(const double* lam) const {
+    return band_at_lam<NV>(M, vs, n, lam, eta);
+  }
+};
+
 // triangle_occupied_hybrid (energy_cut.rs:1184-1228): 0 when empty, the
 // vertex-mean Omega times the area when full, else the 3-point rule on each
 // clipped sub-triangle.
-RTB_CUT inline double hybrid_tri(const View& M, const double c[3][2], const size_t vs[3], const double e_v[3],
-                                 const double om_v[3], double mu, double eta, int n) {
+template <class F>
+RTB_CUT inline double hybrid_tri_by(const double c[3][2], const double e_v[3], const double om_v[3], double mu,
+                                    const F& f, double one) {
   const double e_min = dmin(dmin(e_v[0], e_v[1]), e_v[2]);
   const double e_max = dmax(dmax(e_v[0], e_v[1]), e_v[2]);
   const double area = tri_area(c);
   if (mu <= e_min + kEps) return 0.0;
-  if (mu >= e_max - kEps) return area * (om_v[0] + om_v[1] + om_v[2]) / 3.0;
+  if (mu >= e_max - kEps) return one * (area * (om_v[0] + om_v[1] + om_v[2]) / 3.0);
   double subs[2][3][3];
   const int ns = clip(e_v, mu, subs);
   double total = 0.0;
@@ -189,10 +207,15 @@ RTB_CUT inline double hybrid_tri(const View& M, const double c[3][2], const size
       double lam[3];
       for (int d = 0; d < 3; ++d)  // combine_bary (energy_cut.rs:1172-1178)
         lam[d] = kQ3[iq][0] * subs[s][0][d] + kQ3[iq][1] * subs[s][1][d] + kQ3[iq][2] * subs[s][2][d];
-      total += sa * kW3 * band_at_lam<3>(M, vs, n, lam, eta);
+      total += sa * kW3 * f.template at<3>(lam);
     }
   }
   return total;
+}
+
+RTB_CUT inline double hybrid_tri(const View& M, const double c[3][2], const size_t vs[3], const double e_v[3],
+                                 const double om_v[3], double mu, double eta, int n) {
+  return hybrid_tri_by(c, e_v, om_v, mu, SeqBandSum{M, vs, n, eta}, 1.0);
 }
 
 // ---- tetrahedra ----
@@ -214,8 +237,8 @@ RTB_CUT inline double sub_tet_vol(const double c[4][3], const double t[4][4]) {
 }
 
 // sub_tet_k_quad (energy_cut.rs:1503-1526) with combine_bary_4 (:1492-1500).
-RTB_CUT inline double sub_tet_quad(const View& M, const double c[4][3], const size_t vs[4], const double sub[4][4],
-                                   int n, double eta) {
+template <class F>
+RTB_CUT inline double sub_tet_quad(const double c[4][3], const double sub[4][4], const F& f) {
   const double sv = sub_tet_vol(c, sub);
   if (sv < 1e-30) return 0.0;
   double total = 0.0;
@@ -223,7 +246,7 @@ RTB_CUT inline double sub_tet_quad(const View& M, const double c[4][3], const si
     double lam[4] = {0.0, 0.0, 0.0, 0.0};
     for (int k = 0; k < 4; ++k)
       for (int i = 0; i < 4; ++i) lam[i] += kQ4[iq][k] * sub[k][i];
-    total += sv * kW4 * band_at_lam<4>(M, vs, n, lam, eta);
+    total += sv * kW4 * f.template at<4>(lam);
   }
   return total;
 }
@@ -232,15 +255,16 @@ RTB_CUT inline double sub_tet_quad(const View& M, const double c[4][3], const si
 // vertex-mean Omega times the volume when full, else the 4-point rule on the
 // occupied sub-tetrahedra (1, 2 or 3 vertices below mu; with 3 below, the full
 // tetrahedron minus the one above).
-RTB_CUT inline double hybrid_tet(const View& M, const double c[4][3], const size_t vs[4], const double e_v[4],
-                                 const double om_v[4], double mu, double eta, int n) {
+template <class F>
+RTB_CUT inline double hybrid_tet_by(const double c[4][3], const double e_v[4], const double om_v[4], double mu,
+                                    const F& f, double one) {
   const double full_vol = tet_vol(c[0], c[1], c[2], c[3]);
   if (full_vol < kEps) return 0.0;
   const double e_min = dmin(dmin(e_v[0], e_v[1]), dmin(e_v[2], e_v[3]));
   const double e_max = dmax(dmax(e_v[0], e_v[1]), dmax(e_v[2], e_v[3]));
   if (mu <= e_min + kEps) return 0.0;
   const double full_val = full_vol * (om_v[0] + om_v[1] + om_v[2] + om_v[3]) / 4.0;
-  if (mu >= e_max - kEps) return full_val;
+  if (mu >= e_max - kEps) return one * full_val;
   int idx[4];
   sort_idx<4>(e_v, idx);
   const int a = idx[0], b = idx[1], cc = idx[2], d = idx[3];
@@ -254,7 +278,7 @@ RTB_CUT inline double hybrid_tet(const View& M, const double c[4][3], const size
     cut_point<4>(e_v, a, b, mu, s[1]);
     cut_point<4>(e_v, a, cc, mu, s[2]);
     cut_point<4>(e_v, a, d, mu, s[3]);
-    return sub_tet_quad(M, c, vs, s, n, eta);
+    return sub_tet_quad(c, s, f);
   }
   if (mu <= e_v[cc] + kEps) {  // three sub-tetrahedra through the p_ac - p_bd diagonal
     double total = 0.0;
@@ -262,24 +286,29 @@ RTB_CUT inline double hybrid_tet(const View& M, const double c[4][3], const size
     unit(b, s[1]);
     cut_point<4>(e_v, a, cc, mu, s[2]);
     cut_point<4>(e_v, b, d, mu, s[3]);
-    total += sub_tet_quad(M, c, vs, s, n, eta);
+    total += sub_tet_quad(c, s, f);
     unit(a, s[0]);
     cut_point<4>(e_v, a, cc, mu, s[1]);
     cut_point<4>(e_v, a, d, mu, s[2]);
     cut_point<4>(e_v, b, d, mu, s[3]);
-    total += sub_tet_quad(M, c, vs, s, n, eta);
+    total += sub_tet_quad(c, s, f);
     unit(b, s[0]);
     cut_point<4>(e_v, b, cc, mu, s[1]);
     cut_point<4>(e_v, b, d, mu, s[2]);
     cut_point<4>(e_v, a, cc, mu, s[3]);
-    total += sub_tet_quad(M, c, vs, s, n, eta);
+    total += sub_tet_quad(c, s, f);
     return total;
   }
   unit(d, s[0]);
   cut_point<4>(e_v, a, d, mu, s[1]);
   cut_point<4>(e_v, b, d, mu, s[2]);
   cut_point<4>(e_v, cc, d, mu, s[3]);
-  return full_val - sub_tet_quad(M, c, vs, s, n, eta);
+  return one * full_val - sub_tet_quad(c, s, f);
+}
+
+RTB_CUT inline double hybrid_tet(const View& M, const double c[4][3], const size_t vs[4], const double e_v[4],
+                                 const double om_v[4], double mu, double eta, int n) {
+  return hybrid_tet_by(c, e_v, om_v, mu, SeqBandSum{M, vs, n, eta}, 1.0);
 }
 
 // ---- cells ----

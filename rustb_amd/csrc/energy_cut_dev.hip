@@ -1,14 +1,19 @@
 // The EnergyCut Fermi cut on the device: vertex Omega in tracked band order,
 // then the zero-width cut of every (cell, simplex, band) for a list of
 // chemical potentials (integrate_fermi_cut_2d_t0 / _3d_t0,
-// src/response/energy_cut.rs:1231-1345, 1650-1797).  The per-simplex
-// arithmetic is energy_cut_core.hpp, the same code as the host part
-// (energy_cut.cpp).
+// src/response/energy_cut.rs:1231-1345, 1650-1797).  The per-simplex rules
+// are energy_cut_core.hpp, the same code as the host part (energy_cut.cpp).
 //
-// One lane per (cell, simplex, band), lanes of a wave on consecutive bands;
-// for each mu the 64 lanes' contributions are summed by DPP in a fixed order
-// into a per-wave partial, and the partials are summed per mu in a fixed
-// order (column_sums): the result does not depend on scheduling.
+// One lane per (cell, simplex, band), lanes of a wave on consecutive bands.
+// For each mu a lane settles the empty and full cases itself; the few that
+// are cut (mu strictly inside the band's range on the simplex) are taken by
+// the whole wave, two at a time for n <= 32: the 32 or 64 lanes of a group
+// split the band sum over m of each quadrature point and add their shares in
+// a fixed butterfly, so the loads of E and Im K rows are coalesced instead of
+// one lane walking n bands.  The 64 lanes' contributions are then summed by
+// DPP in a fixed order into a per-wave partial, and the partials are summed
+// per mu in a fixed order (column_sums): the result does not depend on
+// scheduling.
 #include "eigh.hpp"
 #include "energy_cut_core.hpp"
 #include "gpu_team.hpp"
@@ -23,79 +28,154 @@ __global__ void cut_omega_kernel(cut::View V, size_t nk, double eta, double* __r
   om[g] = cut::vertex_omega(V, g / V.n, static_cast<int>(g % V.n), eta);
 }
 
+// simplex s of a cell with band b, item g = (cell * S + s) * n + b
+template <int DIM>
+struct Simplex {
+  static constexpr int NV = DIM + 1;
+  size_t vs[NV];
+  double c[NV][DIM];
+  double e_v[NV], om_v[NV];
+  double measure, scale;
+  int b;
+};
+
+template <int DIM>
+__device__ inline void load_simplex(const cut::View& V, const double* __restrict__ om, size_t g, size_t nx, size_t ny,
+                                    size_t nz, Simplex<DIM>& x) {
+  constexpr int S = DIM == 2 ? 4 : 5;  // simplices per cell
+  const int n = V.n;
+  const size_t cell = g / (static_cast<size_t>(S) * n);
+  const int s = static_cast<int>((g / n) % S);
+  x.b = static_cast<int>(g % n);
+  if constexpr (DIM == 2) {
+    size_t vv[4][3];
+    double cc[4][3][2];
+    cut::cell_triangles(cell / ny, cell % ny, nx, ny, vv, cc);
+    for (int q = 0; q < 3; ++q) {
+      x.vs[q] = vv[s][q];
+      x.c[q][0] = cc[s][q][0];
+      x.c[q][1] = cc[s][q][1];
+    }
+    x.measure = cut::tri_area(x.c);
+    const double half_vol = (1.0 / static_cast<double>(nx)) * (1.0 / static_cast<double>(ny)) / 4.0;
+    x.scale = half_vol / x.measure;
+  } else {
+    size_t vv[5][4];
+    double cc[5][4][3], vol[5];
+    cut::cell_tetrahedra(cell / (ny * nz), (cell / nz) % ny, cell % nz, nx, ny, nz, vv, cc, vol);
+    for (int q = 0; q < 4; ++q) {
+      x.vs[q] = vv[s][q];
+      for (int d = 0; d < 3; ++d) x.c[q][d] = cc[s][q][d];
+    }
+    x.measure = cut::tet_vol(x.c[0], x.c[1], x.c[2], x.c[3]);
+    x.scale = vol[s] / x.measure;
+  }
+  for (int q = 0; q < Simplex<DIM>::NV; ++q) {
+    x.e_v[q] = cut::band(V, x.vs[q], x.b);
+    x.om_v[q] = om[x.vs[q] * n + x.b];
+  }
+}
+
+// Lane t's share of the band sum of eval_berry_band_at_lam_buf
+// (kernel.rs:138-182): the terms m = t, t + G, ...; the G lanes' shares add up
+// to the sum.
+struct LaneBandSum {
+  cut::View M;
+  const size_t* vs;
+  int b;
+  double eta;
+  int t, G;
+  template <int NV>
+  __device__ double at(const double* lam) const {
+    double en = 0.0;
+    for (int v = 0; v < NV; ++v)
+      if (lam[v] != 0.0) en += cut::band(M, vs[v], b) * lam[v];
+    const double eta2 = eta * eta;
+    double im = 0.0;
+    for (int m = t; m < M.n; m += G) {
+      if (m == b) continue;
+      double em = 0.0, km = 0.0;
+      for (int v = 0; v < NV; ++v)
+        if (lam[v] != 0.0) em += cut::band(M, vs[v], m) * lam[v];
+      for (int v = 0; v < NV; ++v)
+        if (lam[v] != 0.0) km += cut::kim(M, vs[v], b, m) * lam[v];
+      const double de = en - em;
+      const double denom = de * de + eta2;
+      if (denom < 1e-30) continue;
+      im += km / denom;
+    }
+    return -2.0 * im;
+  }
+};
+
 template <int DIM>
 __global__ void __launch_bounds__(256) cut_kernel(cut::View V, const double* __restrict__ om, size_t nx, size_t ny,
                                                   size_t nz, const double* __restrict__ mu, int n_mu, double eta,
                                                   double* __restrict__ partial) {
-  constexpr int S = DIM == 2 ? 4 : 5;  // simplices per cell
+  constexpr int S = DIM == 2 ? 4 : 5;
   constexpr int NV = DIM + 1;
   const int n = V.n;
+  const int G = n <= 32 ? 32 : 64;  // lanes per cut item
+  const int lane = threadIdx.x & 63;
+  const int t = lane & (G - 1);
   const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  const size_t g0 = g - lane;  // the wave's first item
   const size_t total = nx * ny * nz * S * static_cast<size_t>(n);
   bool live = g < total;
-  size_t vs[NV] = {};
-  double c[NV][DIM] = {};
-  double e_v[NV] = {}, om_v[NV] = {};
-  double scale = 0.0, full_val = 0.0, e_min = 0.0, e_max = 0.0;
-  int b = 0;
+  Simplex<DIM> x{};
+  double full_val = 0.0, e_min = 0.0, e_max = 0.0;
   if (live) {
-    const size_t cell = g / (static_cast<size_t>(S) * n);
-    const int s = static_cast<int>((g / n) % S);
-    b = static_cast<int>(g % n);
-    double measure = 0.0;
-    if constexpr (DIM == 2) {
-      size_t vv[4][3];
-      double cc[4][3][2];
-      cut::cell_triangles(cell / ny, cell % ny, nx, ny, vv, cc);
-      for (int q = 0; q < 3; ++q) {
-        vs[q] = vv[s][q];
-        c[q][0] = cc[s][q][0];
-        c[q][1] = cc[s][q][1];
-      }
-      measure = cut::tri_area(c);
-      const double half_vol = (1.0 / static_cast<double>(nx)) * (1.0 / static_cast<double>(ny)) / 4.0;
-      scale = half_vol / measure;
-    } else {
-      size_t vv[5][4];
-      double cc[5][4][3], vol[5];
-      cut::cell_tetrahedra(cell / (ny * nz), (cell / nz) % ny, cell % nz, nx, ny, nz, vv, cc, vol);
-      for (int q = 0; q < 4; ++q) {
-        vs[q] = vv[s][q];
-        for (int d = 0; d < 3; ++d) c[q][d] = cc[s][q][d];
-      }
-      measure = cut::tet_vol(c[0], c[1], c[2], c[3]);
-      scale = vol[s] / measure;
-    }
-    live = !(measure < cut::kEps);  // degenerate simplices of ragged meshes
-    for (int q = 0; q < NV; ++q) {
-      e_v[q] = cut::band(V, vs[q], b);
-      om_v[q] = om[vs[q] * n + b];
-    }
-    e_min = e_max = e_v[0];
-    double osum = om_v[0];
+    load_simplex<DIM>(V, om, g, nx, ny, nz, x);
+    live = !(x.measure < cut::kEps);  // degenerate simplices of ragged meshes
+    e_min = e_max = x.e_v[0];
+    double osum = x.om_v[0];
     for (int q = 1; q < NV; ++q) {
-      e_min = cut::dmin(e_min, e_v[q]);
-      e_max = cut::dmax(e_max, e_v[q]);
-      osum += om_v[q];
+      e_min = cut::dmin(e_min, x.e_v[q]);
+      e_max = cut::dmax(e_max, x.e_v[q]);
+      osum += x.om_v[q];
     }
-    full_val = measure * osum / static_cast<double>(NV);
+    full_val = x.measure * osum / static_cast<double>(NV);
   }
-  const int lane = threadIdx.x & 63;
   double* out = partial + (g / 64) * static_cast<size_t>(n_mu);
   for (int im = 0; im < n_mu; ++im) {
     const double m = mu[im];
     double v = 0.0;
+    bool need = false;
     if (live) {
       // the partition points of the host loop: mu >= e_max - eps is the full
       // range, mu <= e_min + eps below it is empty, the rest is cut
-      if (!(m < e_max - cut::kEps)) {
-        v = scale * full_val;
-      } else if (!(m <= e_min + cut::kEps)) {
-        if constexpr (DIM == 2)
-          v = scale * cut::hybrid_tri(V, c, vs, e_v, om_v, m, eta, b);
-        else
-          v = scale * cut::hybrid_tet(V, c, vs, e_v, om_v, m, eta, b);
+      if (!(m < e_max - cut::kEps))
+        v = x.scale * full_val;
+      else
+        need = !(m <= e_min + cut::kEps);
+    }
+    unsigned long long mask = __ballot(need);
+    while (mask) {  // wave-uniform: every lane runs every pass
+      const int l0 = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      int l1 = -1;
+      if (G == 32 && mask) {
+        l1 = __builtin_ctzll(mask);
+        mask &= mask - 1;
       }
+      const int src = (G == 32 && lane >= 32) ? l1 : l0;  // this group's item
+      double r = 0.0;
+      if (src >= 0) {
+        Simplex<DIM> y;
+        load_simplex<DIM>(V, om, g0 + src, nx, ny, nz, y);
+        const LaneBandSum f{V, y.vs, y.b, eta, t, G};
+        const double one = t == 0 ? 1.0 : 0.0;
+        if constexpr (DIM == 2)
+          r = cut::hybrid_tri_by(y.c, y.e_v, y.om_v, m, f, one);
+        else
+          r = cut::hybrid_tet_by(y.c, y.e_v, y.om_v, m, f, one);
+      }
+      for (int o = G / 2; o > 0; o >>= 1) r += __shfl_xor(r, o);
+      // group 0 (lanes 0..G-1) holds item l0's rule, group 1 item l1's; the
+      // owners may sit in either half
+      const double r0 = __shfl(r, 0), r1 = __shfl(r, 32);
+      if (lane == l0) v = x.scale * r0;
+      if (lane == l1) v = x.scale * r1;
     }
     v = wave_sum64(v);
     if (lane == 0) out[im] = v;

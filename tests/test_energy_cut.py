@@ -338,6 +338,54 @@ def test_gpu_ragged_meshes_vs_oracle(mesh):
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * max(np.abs(ref).max(), 1e-3))
 
 
+
+def wide_model(make, seed, n, dim):
+    """A seeded random complex model of n orbitals (on-site block and four
+    hopping vectors, no lattice or time-reversal symmetry), wide enough that
+    the device cut splits each band sum over 64 lanes (n > 32) and, for
+    n > 64, over several terms per lane."""
+    rng = np.random.default_rng(seed)
+    R = [[0, 0], [1, 0], [0, 1], [1, 1], [1, -1]] if dim == 2 else [[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1],
+                                                                     [1, 1, 0]]
+    w = np.array([1.0, 0.5, 0.5, 0.25, 0.25])[:, None, None]
+    ham = (rng.standard_normal((5, n, n)) + 1j * rng.standard_normal((5, n, n))) * w
+    ham[0] = 0.5 * (ham[0] + ham[0].conj().T)
+    orb = rng.random((n, dim))
+    if make is O.Model:
+        return O.model_from_arrays(np.eye(dim), orb, False, np.array(R), ham)
+    return ra.Model.from_arrays(np.eye(dim), orb, False, np.array(R), ham)
+
+
+def overlap_margin(ovs):
+    """Smallest gap between the two largest |<u_i|u_j>|^2 of a row over all
+    neighbour overlaps: band tracking is decided far from ties above ~1e-8."""
+    best = np.inf
+    for ov in ovs:
+        if ov.shape[0] == 0:
+            continue
+        a = np.sort(np.abs(ov) ** 2, axis=-1)
+        best = min(best, float((a[..., -1] - a[..., -2]).min()))
+    return best
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,n", [(1, 40), (5, 72)])
+def test_gpu_wide_model_vs_host_part(seed, n):
+    """n = 40 and 72 against the host part (rtb_energy_cut_host, bit-exact
+    with the oracle on the same vertex data) fed the oracle's vertex data."""
+    m, o = wide_model(ra.Model, seed, n, 2), wide_model(O.Model, seed, n, 2)
+    mesh = (7, 6)
+    mu = np.linspace(-6, 6, 13)
+    pts = EC.vertices(o, mesh, np.eye(2))
+    assert min_vertex_tie(pts, *mesh) > 1e-5
+    E, K, ovx, ovy = vertex_arrays(pts, *mesh)
+    assert overlap_margin([ovx, ovy]) > 1e-8
+    rc, ref = host_sigma(E, K, ovx, ovy, *mesh, n, mu, 0.0, 0.05, 1.0)
+    assert rc == 0
+    p = ra.Parameters.rank2(mesh, [1.0, 0.0], [0.0, 1.0], mu, eta=0.05, integration="EnergyCut")
+    got = m.hall_conductivity(p).conductivity
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
 # ------------------------------------------------------------------ 3D
 def qwz_broken_3d(make):
     """The stacked QWZ model with interlayer and diagonal hoppings, so that
@@ -496,3 +544,21 @@ def test_gpu_3d_qwz_plateau_and_direct():
             res[integ] = m.hall_conductivity(p).conductivity
         assert np.abs(res["Direct"] - res["EnergyCut"]).max() < 5e-2
         assert abs(res["EnergyCut"][30] - 1 / (2 * math.pi)) < 3e-3
+
+
+@pytest.mark.gpu
+def test_gpu_3d_wide_model_vs_host_part():
+    """n = 40 in 3D (tetrahedra, 64-lane band sums) against
+    rtb_energy_cut_host_3d fed the oracle's vertex data."""
+    m, o = wide_model(ra.Model, 3, 40, 3), wide_model(O.Model, 3, 40, 3)
+    mesh = (4, 3, 3)
+    mu = np.linspace(-6, 6, 13)
+    pts = EC.vertices(o, mesh, np.eye(3)[:2])
+    assert min_vertex_tie_3d(pts, *mesh) > 1e-5
+    E, K, ovx, ovy, ovz = vertex_arrays_3d(pts, *mesh)
+    assert overlap_margin([ovx, ovy, ovz]) > 1e-8
+    rc, ref = host_sigma_3d(E, K, ovx, ovy, ovz, mesh, 40, mu, 0.0, 0.05, 1.0)
+    assert rc == 0
+    p = ra.Parameters.rank2(mesh, [1.0, 0, 0], [0, 1.0, 0], mu, eta=0.05, integration="EnergyCut")
+    got = m.hall_conductivity(p).conductivity
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
