@@ -32,6 +32,7 @@ int prepare(const rtb_model* m) {
   std::call_once(g_smem_once, [] {
     g_smem_rc = mesh_set_smem_limits();
     if (g_smem_rc == RTB_OK) g_smem_rc = eigh_set_smem_limits();
+    if (g_smem_rc == RTB_OK) g_smem_rc = energy_cut_track_smem_limits();
   });
   return g_smem_rc;
 }
@@ -1507,6 +1508,24 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   double* T1 = kub;
   double* A = kub + K * n2 * 2;
   double* B = kub + K * n2 * 4;
+  // the tracking walk's schedule (mesh only) for the device walk, uploaded
+  // before the vertex stage; RTB_TRACK_HOST=1 takes the host walk instead
+  const char* th = std::getenv("RTB_TRACK_HOST");
+  const bool dev_track = n <= kTrackDevMaxN && nk <= 0xffffffffull && !(th && std::atoi(th) == 1);
+  TrackSchedule ts;
+  unsigned *sched_v = nullptr, *sched_d = nullptr;
+  unsigned char* sched_n = nullptr;
+  if (dev_track) {
+    energy_cut_schedule(nx, ny, nz, ts);
+    const std::vector<unsigned> hv(ts.by_level.begin(), ts.by_level.end()), hd(ts.dep.begin(), ts.dep.end());
+    RTB_TRY(w.alloc(&sched_v, nk));
+    RTB_TRY(w.alloc(&sched_d, nk * 6));
+    RTB_TRY(w.alloc(&sched_n, nk));
+    RTB_HIP(hipMemcpyAsync(sched_v, hv.data(), nk * sizeof(unsigned), hipMemcpyHostToDevice, st));
+    RTB_HIP(hipMemcpyAsync(sched_d, hd.data(), nk * 6 * sizeof(unsigned), hipMemcpyHostToDevice, st));
+    RTB_HIP(hipMemcpyAsync(sched_n, ts.ndep.data(), nk, hipMemcpyHostToDevice, st));
+    RTB_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope here
+  }
   Checks ck;
   RTB_TRY(checks_init(st, w, 0, ck));
   for (size_t k0 = 0; k0 < nk; k0 += K) {
@@ -1560,23 +1579,32 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
       RTB_TRY(tmr.end(1));
     }
   }
-  // the overlaps to the host for the tracking walk; E and Im K stay on the
-  // device for the cut
-  std::vector<double> hOv[3];
-  for (int ax = 0; ax < 3; ++ax) {
-    hOv[ax].resize(cnt_ax[ax] * n2);
-    if (!hOv[ax].empty())
-      RTB_HIP(hipMemcpyAsync(hOv[ax].data(), dOv[ax], hOv[ax].size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  }
-  RTB_HIP(hipStreamSynchronize(st));
-  std::vector<int> perm;
-  energy_cut_track(nx, ny, nz, n, hOv[0].data(), hOv[1].data(), hOv[2].data(), true, perm);
-  // vertex Omega in tracked band order, then the cut on the device
+  // band tracking: on the device for n <= kTrackDevMaxN, level by level over
+  // the walk's schedule (which depends on the mesh only); above, the squared
+  // overlaps go to the host walk.  Either way the permutations are the same.
   int* dperm;
+  RTB_TRY(w.alloc(&dperm, nk * n));
+  std::vector<int> perm;
+  if (dev_track) {
+    RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
+    RTB_TRY(energy_cut_track_dev(st, nx, ny, nz, n, dOv[0], dOv[1], dOv[2], sched_v, ts.level_start, sched_n,
+                                 sched_d, dperm));
+    RTB_TRY(tmr.end(static_cast<int>(ts.level_start.size() - 1)));
+  } else {
+    std::vector<double> hOv[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      hOv[ax].resize(cnt_ax[ax] * n2);
+      if (!hOv[ax].empty())
+        RTB_HIP(hipMemcpyAsync(hOv[ax].data(), dOv[ax], hOv[ax].size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    RTB_HIP(hipStreamSynchronize(st));
+    energy_cut_track(nx, ny, nz, n, hOv[0].data(), hOv[1].data(), hOv[2].data(), true, perm);
+    RTB_HIP(hipMemcpyAsync(dperm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  }
+  // vertex Omega in tracked band order, then the cut on the device (E and
+  // Im K stayed on the device)
   double* dom;
-  RTB_TRY(w.alloc(&dperm, perm.size()));
   RTB_TRY(w.alloc(&dom, nk * n));
-  RTB_HIP(hipMemcpyAsync(dperm, perm.data(), perm.size() * sizeof(int), hipMemcpyHostToDevice, st));
   RTB_TRY(tmr.begin(RTB_STAGE_KUBO));
   RTB_TRY(energy_cut_omega(st, n, nk, dEv, dKv, dperm, eta, dom));
   RTB_TRY(tmr.end(1));

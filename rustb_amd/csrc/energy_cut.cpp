@@ -187,16 +187,7 @@ double fermi_window_x(double x) {  // energy_cut.rs:59-66
 // highest level among them); then the levels run in order, the vertices of
 // one level on host threads at once.  Every vertex computes exactly what the
 // sequential walk computes.
-namespace {
-
-struct TrackSchedule {
-  std::vector<size_t> by_level;  // vertices grouped by level, walk order inside a level
-  std::vector<size_t> level_start;
-  std::vector<unsigned char> ndep;
-  std::vector<size_t> dep;  // [nk][6] visited neighbours at assignment, neighbour order
-};
-
-void track_schedule(size_t nx, size_t ny, size_t nz, TrackSchedule& ts) {
+void energy_cut_schedule(size_t nx, size_t ny, size_t nz, TrackSchedule& ts) {
   const size_t nk = nx * ny * nz;
   ts.ndep.assign(nk, 0);
   ts.dep.assign(nk * 6, 0);
@@ -239,6 +230,8 @@ void track_schedule(size_t nx, size_t ny, size_t nz, TrackSchedule& ts) {
   for (size_t v : order) ts.by_level[fill[level[v]]++] = v;
 }
 
+namespace {
+
 // A reusable barrier for a fixed set of host threads (generation counting).
 class SpinBarrier {
  public:
@@ -271,7 +264,7 @@ void energy_cut_track(size_t nx, size_t ny, size_t nz, int n, const double* ovx,
     for (int b = 0; b < n; ++b) perm[i * n + b] = b;
   if (nk <= 1) return;
   TrackSchedule ts;
-  track_schedule(nx, ny, nz, ts);
+  energy_cut_schedule(nx, ny, nz, ts);
   // the assignment of vertex nb from its scheduled neighbours
   auto assign = [&](size_t nb, std::vector<double>& ov, std::vector<int>& p, std::vector<char>& assigned,
                     std::vector<std::pair<int, double>>& rows) {

@@ -182,6 +182,153 @@ __global__ void __launch_bounds__(256) cut_kernel(cut::View V, const double* __r
   }
 }
 
+// ---- band tracking (global_band_track, tracking.rs:123-163) ----
+// One workgroup per vertex of a level: the squared overlaps of its scheduled
+// neighbours (rows in the neighbour's tracked order) summed in neighbour
+// order and divided by their count into LDS, as the host walk forms them;
+// then greedy_assign (tracking.rs:44-68): rows by descending row maximum
+// (stable), each taking its largest unassigned column, the lowest column
+// among equal values.  A row whose overall best column (same scan rule) is
+// still free takes it, which is what the scan over the free columns returns;
+// otherwise wave 0 scans the free columns.  The same operations on the same
+// values as the host walk, so the permutations are identical.
+constexpr int kTrackThreads = 256;
+constexpr int kTrackMaxDep = 6;  // visited neighbours of a 3D vertex at most
+
+size_t track_smem(int n) {
+  return static_cast<size_t>(n) * n * sizeof(double) + n * sizeof(double) +
+         (3 + kTrackMaxDep) * static_cast<size_t>(n) * sizeof(int);
+}
+
+__global__ void identity_perm_kernel(size_t total, int n, int* __restrict__ perm) {
+  const size_t g = blockIdx.x * 256ull + threadIdx.x;
+  if (g < total) perm[g] = static_cast<int>(g % n);
+}
+
+__global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __restrict__ ovx,
+                                                              const double* __restrict__ ovy,
+                                                              const double* __restrict__ ovz, size_t ny, size_t nz,
+                                                              int n, const unsigned* __restrict__ verts,
+                                                              const unsigned char* __restrict__ ndep,
+                                                              const unsigned* __restrict__ dep, int* __restrict__ perm) {
+  extern __shared__ double smem[];
+  double* M = smem;                                // [n][n]
+  double* rmax = M + static_cast<size_t>(n) * n;  // [n]
+  int* best = reinterpret_cast<int*>(rmax + n);    // [n] each row's best column over all columns
+  int* order = best + n;                           // [n] rows by descending maximum
+  int* bsort = order + n;                          // [n] best[order[s]]
+  int* pv = bsort + n;                             // [kTrackMaxDep][n] the neighbours' permutations
+  const size_t nb = verts[blockIdx.x];
+  const int cnt = ndep[nb];
+  if (cnt == 0) return;  // vertex 0 keeps the identity
+  const int nn = n * n;
+  const size_t n2 = static_cast<size_t>(nn);
+  for (int e = threadIdx.x; e < cnt * n; e += kTrackThreads) {
+    const int r = e / n, a = e - r * n;
+    pv[e] = perm[static_cast<size_t>(dep[nb * kTrackMaxDep + r]) * n + a];
+  }
+  // each neighbour's overlap block and orientation; the pair's axis from its
+  // index step, x first (with ny = 1 the x and y steps coincide and only x
+  // pairs exist)
+  const double* base[kTrackMaxDep];
+  bool lower[kTrackMaxDep];  // nb is the pair's lower vertex: its raw band is the row
+#pragma unroll
+  for (int r = 0; r < kTrackMaxDep; ++r) {
+    base[r] = nullptr;
+    lower[r] = false;
+    if (r < cnt) {
+      const size_t vn = dep[nb * kTrackMaxDep + r];
+      const size_t lo = nb < vn ? nb : vn, step = (nb < vn ? vn : nb) - lo;
+      const double* axis = step == ny * nz ? ovx : (step == nz ? ovy : ovz);
+      base[r] = axis + lo * n2;
+      lower[r] = nb == lo;
+    }
+  }
+  __syncthreads();
+  const double dc = static_cast<double>(cnt);
+  for (int e = threadIdx.x; e < nn; e += kTrackThreads) {
+    const int a = e / n, b = e - a * n;
+    double x = 0.0;
+#pragma unroll
+    for (int r = 0; r < kTrackMaxDep; ++r) {
+      if (r < cnt) {
+        const size_t pa = static_cast<size_t>(pv[r * n + a]);
+        const double y = lower[r] ? base[r][static_cast<size_t>(b) * n + pa] : base[r][pa * n + b];
+        x = r == 0 ? y : x + y;
+      }
+    }
+    M[e] = x / dc;
+  }
+  __syncthreads();
+  // each row's maximum (fmax from -1: NaN dropped) and best column, the
+  // lowest among equal values (the scan is rotated to spread the LDS banks)
+  for (int i = threadIdx.x; i < n; i += kTrackThreads) {
+    double bv = -1.0;
+    int bj = 0;
+    for (int q = 0; q < n; ++q) {
+      const int j = q + i < n ? q + i : q + i - n;
+      const double v = M[i * n + j];
+      if (v > bv || (v == bv && j < bj)) {
+        bv = v;
+        bj = j;
+      }
+    }
+    rmax[i] = bv;
+    best[i] = bj;
+  }
+  __syncthreads();
+  // the stable descending sort as ranks
+  for (int i = threadIdx.x; i < n; i += kTrackThreads) {
+    const double x = rmax[i];
+    int rank = 0;
+    for (int k = 0; k < n; ++k) {
+      const double y = rmax[k];
+      rank += (y > x || (y == x && k < i)) ? 1 : 0;
+    }
+    order[rank] = i;
+    bsort[rank] = best[i];
+  }
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  unsigned long long tk0 = 0, tk1 = 0;  // assigned columns 0..63, 64..127 (the same in every lane)
+  int* out = perm + nb * n;
+  for (int s = 0; s < n; ++s) {
+    const int r = order[s];
+    int j = bsort[s];
+    const bool busy = j < 64 ? ((tk0 >> j) & 1ull) : ((tk1 >> (j - 64)) & 1ull);
+    if (busy) {  // the same in every lane: the scan over the free columns
+      // no candidate: (-1, 0), as the host scan's start; every real
+      // candidate is >= 0 and NaN never wins
+      double bv = -1.0;
+      int bj = 0;
+      for (int c = 0; lane + 64 * c < n; ++c) {
+        const int jj = lane + 64 * c;
+        const bool t = c == 0 ? ((tk0 >> lane) & 1ull) : ((tk1 >> lane) & 1ull);
+        const double v = M[r * n + jj];
+        if (!t && v > bv) {
+          bv = v;
+          bj = jj;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o);
+        const int oj = __shfl_xor(bj, o);
+        if (ov > bv || (ov == bv && oj < bj)) {
+          bv = ov;
+          bj = oj;
+        }
+      }
+      j = bj;
+    }
+    if (j < 64)
+      tk0 |= 1ull << j;
+    else
+      tk1 |= 1ull << (j - 64);
+    if (lane == 0) out[r] = j;
+  }
+}
+
 }  // namespace
 
 int energy_cut_omega(hipStream_t st, int n, size_t nk, const double* E, const double* Ki, const int* perm, double eta,
@@ -216,6 +363,34 @@ int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int 
       hipLaunchKernelGGL(cut_kernel<3>, dim3(blocks), dim3(256), 0, st, V, om, nx, ny, nz, mu + m0, c, eta, partial);
     RTB_HIP(hipGetLastError());
     RTB_TRY(column_sums(st, partial, waves, c, out + m0));
+  }
+  return RTB_OK;
+}
+
+int energy_cut_track_smem_limits() {
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&track_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(track_smem(kTrackDevMaxN))));
+  return RTB_OK;
+}
+
+int energy_cut_track_dev(hipStream_t st, size_t nx, size_t ny, size_t nz, int n, const double* ovx,
+                         const double* ovy, const double* ovz, const unsigned* by_level,
+                         const std::vector<size_t>& level_start, const unsigned char* ndep, const unsigned* dep,
+                         int* perm) {
+  if (n < 1 || n > kTrackDevMaxN) return fail(RTB_ERR_UNSUPPORTED, "device band tracking is for n <= 128");
+  const size_t nk = nx * ny * nz;
+  if (nk == 0) return RTB_OK;
+  const size_t total = nk * static_cast<size_t>(n);
+  hipLaunchKernelGGL(identity_perm_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, total,
+                     n, perm);
+  RTB_HIP(hipGetLastError());
+  // level 0 is vertex 0 alone (no neighbours visited before it)
+  for (size_t l = 1; l + 1 < level_start.size(); ++l) {
+    const size_t a = level_start[l], b = level_start[l + 1];
+    if (b == a) continue;
+    hipLaunchKernelGGL(track_kernel, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads), track_smem(n), st, ovx,
+                       ovy, ovz, ny, nz, n, by_level + a, ndep, dep, perm);
+    RTB_HIP(hipGetLastError());
   }
   return RTB_OK;
 }

@@ -198,6 +198,17 @@ inline double mflop_zgemm(int n, size_t nb, bool herm) {
 // over tetrahedra.  A 2D mesh is nz = 1 (ovz unused).
 void energy_cut_track(size_t nx, size_t ny, size_t nz, int n, const double* ovx, const double* ovy,
                       const double* ovz, bool squared, std::vector<int>& perm);
+// The walk's schedule, which depends on the mesh only: per vertex the
+// neighbours already visited when the walk assigns it (neighbour order) and a
+// level one above theirs; the vertices grouped by level (walk order inside a
+// level).  Vertex 0 alone is level 0.
+struct TrackSchedule {
+  std::vector<size_t> by_level;
+  std::vector<size_t> level_start;  // [levels + 1]
+  std::vector<unsigned char> ndep;  // [nk]
+  std::vector<size_t> dep;          // [nk][6]
+};
+void energy_cut_schedule(size_t nx, size_t ny, size_t nz, TrackSchedule& ts);
 void energy_cut_sigma(size_t nx, size_t ny, size_t nz, int dim, int n, const double* E, const double* Ki, size_t ks,
                       const int* perm, const double* mu, size_t n_mu, double width, double eta, double* out);
 // The thermal convolution of a zero-width cut t0(mu, n_mu, out) (t0 itself

@@ -562,3 +562,25 @@ def test_gpu_3d_wide_model_vs_host_part():
     p = ra.Parameters.rank2(mesh, [1.0, 0, 0], [0, 1.0, 0], mu, eta=0.05, integration="EnergyCut")
     got = m.hall_conductivity(p).conductivity
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-9 * np.abs(ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,n,mesh", [(7, 32, (16, 12)), (8, 24, (5, 4, 4)), (9, 100, (6, 5)), (10, 128, (4, 3))])
+def test_gpu_device_walk_equals_host_walk(seed, n, mesh, monkeypatch):
+    """Band tracking on the device (n <= 128) against the host walk
+    (RTB_TRACK_HOST=1) on the same device vertex data: identical sigma, bit
+    for bit.  On these random models the walk reorders the bands of nearly
+    every vertex (counted with the oracle's global_band_track)."""
+    dim = len(mesh)
+    pts = list(EC.vertices(wide_model(O.Model, seed, n, dim), mesh, np.eye(dim)[:2]))
+    EC.global_band_track(pts, mesh)
+    assert sum(bool(np.any(np.diff(q.band) < 0)) for q in pts) > len(pts) // 2
+    m = wide_model(ra.Model, seed, n, dim)
+    d = np.eye(dim)
+    p = ra.Parameters.rank2(mesh, d[0], d[1], np.linspace(-6, 6, 25), eta=0.05, integration="EnergyCut")
+    monkeypatch.delenv("RTB_TRACK_HOST", raising=False)
+    dev = m.hall_conductivity(p).conductivity
+    monkeypatch.setenv("RTB_TRACK_HOST", "1")
+    host = m.hall_conductivity(p).conductivity
+    assert np.abs(host).max() > 1e-3
+    np.testing.assert_array_equal(dev, host)
