@@ -270,8 +270,8 @@ def energy_cut_workload():
 
 def energy_cut_bench(args):
     """hall_conductivity with Integration::EnergyCut (response/linear/mod.rs:226-268):
-    device vertex stage, band tracking on host threads, the triangle Fermi cut
-    on the device."""
+    device vertex stage, band tracking on the device (one launch per level of
+    the walk's schedule), the triangle Fermi cut on the device."""
     import rustb_amd as ra
     from rustb_amd import _lib
 
@@ -287,8 +287,9 @@ def energy_cut_bench(args):
     wall = (time.perf_counter() - t0) / args.steps
     st = _lib.last_stats()
     names = _lib.STAGES
-    # device time = the sum of the stage timers (the event span also brackets
-    # the host tracking walk between the vertex stage and the cut)
+    # device time = the sum of the stage timers (for nsta above 128 the event
+    # span also brackets the host tracking walk between the vertex stage and
+    # the cut)
     dev_ms = sum(st["ms"][i] for i in range(len(names)) if names[i] != "total")
     g = names.index("gemm64")
     tf = st["flop"][g] / (st["ms"][g] * 1e-3) / 1e12 if st["ms"][g] > 0 else 0.0
@@ -304,7 +305,7 @@ def energy_cut_bench(args):
         "sigma_mu0": float(sig[mu.size // 2]),
         "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
                      "frac": round(tf / 78.6, 4), "traffic": None, "kernel": "zgemm3m_kernel<double, ...>",
-                     "stage": "gemm64", "note": "device part only; host_ms = copies of the overlaps and the tracking walk"},
+                     "stage": "gemm64", "note": "the overlap and Kubo products; host_ms = the walk schedule, launches and synchronisations"},
     }
     if not args.no_cpu:
         # the oracle (pure-Python loops, one core) on an 8 x 8 mesh of the same model
