@@ -190,7 +190,8 @@ __global__ void __launch_bounds__(256) twisted_columns_kernel(int n, size_t nmat
   twisted_column<T>(n, Eg + mat * n, S + mat * nn * 3, n, j, TW[mat * n + j], Y + mat * nn);
 }
 
-// Z = Q1 Q2 Y for W columns per workgroup (W * S threads, Y block in LDS).
+// Z = Q1 Q2 Y for W columns per workgroup (W * S threads, Y block and the
+// reflector staging buffers in LDS).
 template <typename T>
 struct BandBt;
 template <> struct BandBt<float> { static constexpr int W = 64, S = 8; };
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(BandBt<T>::W * BandBt<T>::S) band_backtrans_ke
 
 template <typename T>
 size_t band_backtrans_smem(int n) {
-  return static_cast<size_t>(n) * (BandBt<T>::W + 1) * sizeof(cplx<T>);
+  return band_backtrans_team_elems<T, BandBt<T>::W>(n) * sizeof(cplx<T>);
 }
 
 constexpr int kTeBlock = 256;

@@ -39,6 +39,24 @@ constexpr int kMaxBandN = 256;   // largest order of the two-stage route
 //   T1 [P][b][b]   panel p's T (upper triangular)
 //   V2 [n - 1][K][b]  sweep i, step k: reflector on rows i + 1 + k b ..
 //   tau2 [n - 1][K]
+// band_backtrans's team memory: the column block Ys [n][W + pad], then two
+// staging buffers of NB slots (reflectors or panel rows) of b + 1 complex
+// elements (the b entries, then the reflector's tau).  The strides keep the
+// lanes of one LDS access on distinct banks: thread (c, s) reads row s + S u
+// of column c, so a row stride of 2 mod 16 complex (FP32) or odd (FP64,
+// 16-byte elements) spreads the 8 rows x 2 columns of a 16-lane access; the
+// 8 lanes of a column read staged rows 17 elements apart.
+template <typename T>
+struct BandBtStage {
+  static constexpr int NB = sizeof(T) == 4 ? 64 : 32;
+  static constexpr int pad = sizeof(T) == 4 ? 2 : 1;
+};
+template <typename T, int W>
+RTB_HD inline size_t band_backtrans_team_elems(int n) {
+  return static_cast<size_t>(n) * (W + BandBtStage<T>::pad) +
+         2 * static_cast<size_t>(BandBtStage<T>::NB) * (kBandB + 1);
+}
+
 struct BandLayout {
   int n, P, K;
   size_t v1, t1, v2, tau2, total;
@@ -427,7 +445,7 @@ RTB_HD void twisted_column(int n, const T* e, const T* Ts, int ldy, int j, T twi
 // Y of T (Y[i * ldy + j]).  Team of W * S threads: thread (c, s) =
 // (id / S, id % S) takes rows s, s + S, .. of each reflector's rows for
 // column j0 + c; the S threads of a column are consecutive ids (one wave).
-// Ys: team memory [n][W + 1] complex.  Z out as in backtrans_reg: rows_out = 0
+// Ys: team memory of band_backtrans_team_elems.  Z out as in backtrans_reg: rows_out = 0
 // Zout[r][c], rows_out = 1 Zout[c][r]; gdiag (optional): squared norms of
 // the stored columns (FP64).
 // --------------------------------------------------------------------------
@@ -436,7 +454,7 @@ RTB_HD void band_backtrans(Team& tm, int n, int j0, const T* Y, int ldy, const c
                            const cplx<T>* V2, const cplx<T>* tau2, cplx<T>* Ys, TO* Zout, int rows_out,
                            double* gdiag) {
   constexpr int b = kBandB;
-  constexpr int Wp = W + 1;  // padded row stride of Ys (the S threads of a column hit distinct banks)
+  constexpr int Wp = W + BandBtStage<T>::pad;  // padded row stride of Ys
   static_assert(b % S == 0 && (S == 8 || S == 16), "rows per thread");
   const BandLayout L = band_layout(n);
   const int c = tm.id / S, s = tm.id % S;
@@ -447,94 +465,195 @@ RTB_HD void band_backtrans(Team& tm, int n, int j0, const T* Y, int ldy, const c
     Ys[i * Wp + cc] = {j0 + cc < n ? Y[static_cast<size_t>(i) * ldy + j0 + cc] : T(0), T(0)};
   }
   tm.sync();
-  // Q2: the chasing reflectors, last sweep first, last step first; the
-  // next reflector's tau and this thread's entries of its v are loaded
-  // while the current one is applied (the chain is otherwise one global
-  // memory latency per reflector)
-  constexpr int Q = b / S;  // rows of a reflector per thread
+  // The reflectors stream through team memory: a sequence of chunks, each
+  // loaded by the whole team with coalesced reads into registers while the
+  // previous chunk is applied from team memory (one memory latency per chunk
+  // rather than one per reflector).  Chunks: first the chasing reflectors
+  // (Q2) in batches of NB slots, slot r = i K + k holding sweep i, step k
+  // (k < nsteps(i) used), taken in descending order (last sweep first, last
+  // step first); then per panel of Q1 (last first) its rows in chunks of NB,
+  // twice: for w = V^H Z, then for Z <- Z - V (T w).  Thread (c, s) keeps
+  // rows s, s + S, .. of every reflector and panel in increasing order, so
+  // each column's arithmetic is the same as applying them one by one.
+  constexpr int NB = BandBtStage<T>::NB;
+  constexpr int NT = W * S;  // team size
+  constexpr int SL = b + 1;                       // staged slot: b entries, then tau
+  constexpr int EV = (NB * SL + NT - 1) / NT;      // staging registers per thread
+  constexpr int Q = b / S;                        // rows of a reflector per thread
+  cplx<T>* Sv = Ys + static_cast<size_t>(n) * Wp;  // [2][NB][SL]
   auto nsteps = [&](int i) { return (n - 1 - i + b - 1) / b; };
-  struct Pre {
-    cplx<T> tau, v[Q];
+  // whole sweeps per chasing chunk (a sweep's K slots are contiguous)
+  const int spc = NB / L.K > 1 ? NB / L.K : 1;
+  const int nq2 = n >= 2 ? (n - 1 + spc - 1) / spc : 0;
+  auto prow = [&](int p) { return n - (p + 1) * b; };  // rows of panel p
+  auto pch = [&](int p) { return (prow(p) + NB - 1) / NB; };
+  int nchunks = nq2;
+  for (int p = 0; p < L.P; ++p) nchunks += 2 * pch(p);
+  struct Chunk {
+    int kind;      // 0: chasing slots [lo, hi) (whole sweeps); 1, 2: panel p rows [lo, hi), first / second pass
+    int p, lo, hi;
+    bool first, last;  // first / last chunk of its pass
   };
-  auto fetch = [&](int i, int k) {
-    Pre p;
-    const size_t r = static_cast<size_t>(i) * L.K + k;
-    p.tau = tau2[r];
-#pragma unroll
-    for (int u = 0; u < Q; ++u) p.v[u] = V2[r * b + s + S * u];
-    return p;
+  auto chunk = [&](int t) {
+    Chunk ch{0, 0, 0, 0, false, false};
+    if (t < nq2) {
+      const int ihi = n - 1 - t * spc, ilo = ihi > spc ? ihi - spc : 0;
+      ch.lo = ilo * L.K;
+      ch.hi = ihi * L.K;
+      return ch;
+    }
+    t -= nq2;
+    for (int p = L.P - 1; p >= 0; --p) {
+      const int c2 = pch(p);
+      if (t < 2 * c2) {
+        const int ci = t < c2 ? t : t - c2;
+        ch.kind = t < c2 ? 1 : 2;
+        ch.p = p;
+        ch.lo = ci * NB;
+        ch.hi = ch.lo + NB < prow(p) ? ch.lo + NB : prow(p);
+        ch.first = ci == 0;
+        ch.last = ci == c2 - 1;
+        return ch;
+      }
+      t -= 2 * c2;
+    }
+    return ch;
   };
-  int i = n - 2, k = n >= 2 ? nsteps(n - 2) - 1 : -1;
-  Pre cur{};
-  if (i >= 0) cur = fetch(i, k);
-  while (i >= 0) {
-    int ni = i, nk = k - 1;
-    if (nk < 0) {
-      ni = i - 1;
-      nk = ni >= 0 ? nsteps(ni) - 1 : -1;
-    }
-    Pre nxt{};
-    if (ni >= 0) nxt = fetch(ni, nk);
-    if (cur.tau.re != T(0) || cur.tau.im != T(0)) {  // team-uniform
-      const int r0 = i + 1 + k * b;
-      const int len = r0 + b - 1 < n - 1 ? b : n - r0;
-      cplx<T> w = {T(0), T(0)};
+  cplx<T> rv[EV];
+  auto load = [&](int t) {
+    const Chunk ch = chunk(t);
+    const cplx<T>* v = ch.kind == 0 ? V2 + static_cast<size_t>(ch.lo) * b
+                                    : V1 + (static_cast<size_t>(ch.p) * n + ch.lo) * b;
+    const int rows = ch.hi - ch.lo;
 #pragma unroll
-      for (int u = 0; u < Q; ++u) {
-        const int q = s + S * u;
-        if (q < len) cfmac(w, cur.v[u], Ys[(r0 + q) * Wp + c]);
+    for (int u = 0; u < EV; ++u) {
+      const int e = tm.id + u * NT;
+      const int r = e / SL, q = e - r * SL;
+      cplx<T> x = {T(0), T(0)};
+      if (r < rows) {
+        if (q < b)
+          x = v[r * b + q];
+        else if (ch.kind == 0)
+          x = tau2[ch.lo + r];
       }
-      w.re = tm.template group_sum<S>(w.re);
-      w.im = tm.template group_sum<S>(w.im);
-      const cplx<T> f = cmul(cur.tau, w);
-#pragma unroll
-      for (int u = 0; u < Q; ++u) {
-        const int q = s + S * u;
-        if (q < len) {
-          const cplx<T> d = cmul(cur.v[u], f);
-          Ys[(r0 + q) * Wp + c].re -= d.re;
-          Ys[(r0 + q) * Wp + c].im -= d.im;
-        }
-      }
-      tm.wave_sync();
+      rv[u] = x;
     }
-    cur = nxt;
-    i = ni;
-    k = nk;
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < EV; ++u) {
+      const int e = tm.id + u * NT;
+      if (e < NB * SL) Sv[buf * NB * SL + e] = rv[u];
+    }
+  };
+  cplx<T> w[b];  // a panel's V^H Z, then T (V^H Z), for this column
+  if (nchunks > 0) {
+    load(0);
+    store(0);
   }
-  // Q1: panels last first, Z <- Z - V (T (V^H Z)) on rows [r0, n)
-  for (int p = L.P - 1; p >= 0; --p) {
-    const int r0 = (p + 1) * b, m = n - r0;
-    const cplx<T>* V = V1 + static_cast<size_t>(p) * n * b;
-    const cplx<T>* Tp = T1 + static_cast<size_t>(p) * b * b;
-    cplx<T> w[b];
+  tm.sync();
+  for (int t = 0; t < nchunks; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < nchunks) load(t + 1);
+    const Chunk ch = chunk(t);
+    const cplx<T>* sv = Sv + buf * NB * SL;
+    if (ch.kind == 0) {
+      // sweep i's windows [i + 1 + k b, i + 1 + (k + 1) b) are disjoint, so
+      // its reflectors commute: all window sums, then all updates (K
+      // independent chains per thread instead of one)
+      constexpr int KM = (kMaxBandN - 1 + b - 1) / b;  // steps per sweep at most
+      const int ilo = ch.lo / L.K, ihi = ch.hi / L.K;
+      for (int i = ihi - 1; i >= ilo; --i) {
+        const int ki = nsteps(i);
+        const cplx<T>* vs = sv + (i - ilo) * L.K * SL;  // slot (i, k) at vs + k SL
+        cplx<T> x[KM];
 #pragma unroll
-    for (int a = 0; a < b; ++a) w[a] = {T(0), T(0)};
-    for (int i = s; i < m; i += S) {
-      const cplx<T> z = Ys[(r0 + i) * Wp + c];
+        for (int k = 0; k < KM; ++k) {
+          x[k] = {T(0), T(0)};
+          if (k < ki) {
+            const cplx<T>* v = vs + k * SL;
+            const int r0 = i + 1 + k * b;
+            const int len = r0 + b - 1 < n - 1 ? b : n - r0;
 #pragma unroll
-      for (int a = 0; a < b; ++a) cfmac(w[a], V[i * b + a], z);
+            for (int u = 0; u < Q; ++u) {
+              const int q = s + S * u;
+              if (q < len) cfmac(x[k], v[q], Ys[(r0 + q) * Wp + c]);
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < KM; ++k)
+          if (k < ki) {
+            x[k].re = tm.template group_sum<S>(x[k].re);
+            x[k].im = tm.template group_sum<S>(x[k].im);
+          }
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+          if (k < ki) {
+            const cplx<T>* v = vs + k * SL;
+            const cplx<T> tau = v[b];
+            if (tau.re != T(0) || tau.im != T(0)) {  // team-uniform
+              const cplx<T> f = cmul(tau, x[k]);
+              const int r0 = i + 1 + k * b;
+              const int len = r0 + b - 1 < n - 1 ? b : n - r0;
+#pragma unroll
+              for (int u = 0; u < Q; ++u) {
+                const int q = s + S * u;
+                if (q < len) {
+                  const cplx<T> d = cmul(v[q], f);
+                  Ys[(r0 + q) * Wp + c].re -= d.re;
+                  Ys[(r0 + q) * Wp + c].im -= d.im;
+                }
+              }
+            }
+          }
+        }
+        tm.wave_sync();
+      }
+    } else {
+      // Q1 panel p: Z <- Z - V (T (V^H Z)) on rows [r0, n)
+      const int r0 = (ch.p + 1) * b;
+      if (ch.kind == 1) {
+        if (ch.first) {
+#pragma unroll
+          for (int a = 0; a < b; ++a) w[a] = {T(0), T(0)};
+        }
+        for (int i = ch.lo + s; i < ch.hi; i += S) {  // NB is a multiple of S: rows s mod S
+          const cplx<T> z = Ys[(r0 + i) * Wp + c];
+          const cplx<T>* vr = sv + (i - ch.lo) * SL;
+#pragma unroll
+          for (int a = 0; a < b; ++a) cfmac(w[a], vr[a], z);
+        }
+        if (ch.last) {
+#pragma unroll
+          for (int a = 0; a < b; ++a) {
+            w[a].re = tm.template group_sum<S>(w[a].re);
+            w[a].im = tm.template group_sum<S>(w[a].im);
+          }
+          // w <- T w in place: row a of the upper triangular T reads w[q >= a]
+          const cplx<T>* Tp = T1 + static_cast<size_t>(ch.p) * b * b;
+#pragma unroll
+          for (int a = 0; a < b; ++a) {
+            cplx<T> x = {T(0), T(0)};
+#pragma unroll
+            for (int q = a; q < b; ++q) cfma(x, Tp[a * b + q], w[q]);
+            w[a] = x;
+          }
+        }
+      } else {
+        for (int i = ch.lo + s; i < ch.hi; i += S) {
+          const cplx<T>* vr = sv + (i - ch.lo) * SL;
+          cplx<T> d = {T(0), T(0)};
+#pragma unroll
+          for (int a = 0; a < b; ++a) cfma(d, vr[a], w[a]);
+          Ys[(r0 + i) * Wp + c].re -= d.re;
+          Ys[(r0 + i) * Wp + c].im -= d.im;
+        }
+        tm.wave_sync();
+      }
     }
-#pragma unroll
-    for (int a = 0; a < b; ++a) {
-      w[a].re = tm.template group_sum<S>(w[a].re);
-      w[a].im = tm.template group_sum<S>(w[a].im);
-    }
-    cplx<T> u[b];  // T w
-#pragma unroll
-    for (int a = 0; a < b; ++a) {
-      u[a] = {T(0), T(0)};
-#pragma unroll
-      for (int q = a; q < b; ++q) cfma(u[a], Tp[a * b + q], w[q]);
-    }
-    for (int i = s; i < m; i += S) {
-      cplx<T> d = {T(0), T(0)};
-#pragma unroll
-      for (int a = 0; a < b; ++a) cfma(d, V[i * b + a], u[a]);
-      Ys[(r0 + i) * Wp + c].re -= d.re;
-      Ys[(r0 + i) * Wp + c].im -= d.im;
-    }
-    tm.wave_sync();
+    if (t + 1 < nchunks) store(buf ^ 1);
+    tm.sync();
   }
   // out
   double g2 = 0.0;

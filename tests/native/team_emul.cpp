@@ -88,7 +88,7 @@ int run_band(int n, const double* H, double* E, double* Z, int vectors, int rows
   std::vector<T> d(n), e(n), e2(n), Ev(n), cnt(n), twist(n), Y(static_cast<size_t>(n) * n),
       Sw(static_cast<size_t>(n) * n * 3);
   std::vector<cplx<T>> pV(static_cast<size_t>(n) * kBandLd), pY(static_cast<size_t>(n) * kBandLd),
-      small(3 * kBandB * kBandB + kBandB), hv(kBandB), Ys(static_cast<size_t>(n) * (W + 1));
+      small(3 * kBandB * kBandB + kBandB), hv(kBandB), Ys(band_backtrans_team_elems<T, W>(n));
   std::vector<T> red(2 * 2048);
   std::vector<double> grp(2048);
   auto team = [&](int nt, auto&& f) {
