@@ -176,6 +176,23 @@ __global__ void __launch_bounds__(kBandChaseThreads) band_chase_kernel(int n, cp
   band_chase<T>(tm, n, A + mat * nn, Dg + mat * n, Eg + mat * n, bm + L.v2, bm + L.tau2, hv, red);
 }
 
+// FP32 stage 2 with the band region in LDS (one workgroup of 768 threads
+// per matrix).
+constexpr int kBandChaseLdsThreads = 768;  // 48 groups of 16: one pass over the 3b columns (rows) of a step
+__global__ void __launch_bounds__(kBandChaseLdsThreads) band_chase_lds_kernel(int n, const cplx<float>* __restrict__ A,
+                                                                            float* __restrict__ Dg,
+                                                                            float* __restrict__ Eg,
+                                                                            cplx<float>* __restrict__ band) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  GpuTeam tm;
+  const size_t mat = blockIdx.x;
+  const size_t nn = static_cast<size_t>(n) * n;
+  const BandLayout L = band_layout(n);
+  cplx<float>* bm = band + mat * L.total;
+  band_chase_lds<float>(tm, n, A + mat * nn, Dg + mat * n, Eg + mat * n, bm + L.v2, bm + L.tau2,
+                        reinterpret_cast<cplx<float>*>(smem_raw));
+}
+
 // The eigenvectors of T of the isolated eigenvalues written into Y (the
 // cluster members are there already): one thread per (matrix, column).
 template <typename T>
@@ -387,8 +404,12 @@ int eigh_launch(hipStream_t st, Timer* tmr, int n, size_t nmat, double* H, doubl
     hipLaunchKernelGGL((band_reduce_kernel<T>), dim3(static_cast<unsigned>(nmat)), dim3(kBandReduceThreads),
                        band_reduce_smem<T>(n), st, n, H, A, bnd);
     RTB_HIP(hipGetLastError());
-    hipLaunchKernelGGL((band_chase_kernel<T>), dim3(static_cast<unsigned>(nmat)), dim3(kBandChaseThreads), 0, st,
-                       n, A, Dg, Eg, bnd);
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL(band_chase_lds_kernel, dim3(static_cast<unsigned>(nmat)), dim3(kBandChaseLdsThreads),
+                         band_chase_team_elems(n) * sizeof(cplx<float>), st, n, A, Dg, Eg, bnd);
+    else
+      hipLaunchKernelGGL((band_chase_kernel<T>), dim3(static_cast<unsigned>(nmat)), dim3(kBandChaseThreads), 0,
+                         st, n, A, Dg, Eg, bnd);
     RTB_HIP(hipGetLastError());
     if (tmr) RTB_TRY(tmr->end(2, flop_tridiag(n, nmat)));
   }
@@ -464,6 +485,9 @@ int hermitize_upper(hipStream_t st, int n, size_t nmat, double* H) { return herm
 size_t eigh_band_elems(int n) { return n > kMaxEighN ? band_layout(n).total * 2 : 0; }
 
 int eigh_set_smem_limits() {
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&band_chase_lds_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(band_chase_team_elems(kMaxBandN) * sizeof(cplx<float>))));
   RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&band_reduce_kernel<float>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(band_reduce_smem<float>(kMaxBandN))));

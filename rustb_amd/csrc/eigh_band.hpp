@@ -414,6 +414,100 @@ RTB_HD void band_chase(Team& tm, int n, cplx<T>* A, T* D, T* E, cplx<T>* V2, cpl
   tm.sync();
 }
 
+// Stage 2 with the band region in team memory (the FP32 route: at n = 256 it
+// is 135 KB): AB[r][d] holds A[r][r + d - 2b] for |d - 2b| <= 2b (the chase
+// touches |c - r| <= 2b - 1), rows kChaseLd = 4b + 2 apart so the 16 rows a
+// reflector spans fall on distinct banks.  The steps and formulas of
+// band_chase; the team (nt a multiple of 16) works in groups of 16 threads:
+// every group forms the reflector from the column (thread q of a group holds
+// v_q), then for the left application each group takes a column of
+// [col, hi] and its 16 rows, for the right one a row of [col, hi] and its 16
+// columns, the dot products by group sums.  A[r][c] for |c - r| <= 2b is read
+// from A as band_reduce left it.
+constexpr int kChaseLd = 4 * kBandB + 2;
+RTB_HD inline size_t band_chase_team_elems(int n) { return static_cast<size_t>(n) * kChaseLd; }
+
+template <typename T, typename Team>
+RTB_HD void band_chase_lds(Team& tm, int n, const cplx<T>* A, T* D, T* E, cplx<T>* V2, cplx<T>* tau2,
+                           cplx<T>* AB) {
+  constexpr int b = kBandB, W2 = 2 * kBandB, ND = 4 * kBandB + 1;
+  const int nt = tm.nt, id = tm.id;
+  const int g = id / 16, q = id % 16, groups = nt / 16;
+  const BandLayout L = band_layout(n);
+  auto at = [&](int r, int c) -> cplx<T>& { return AB[r * kChaseLd + (c - r + W2)]; };
+  for (int e = id; e < n * ND; e += nt) {
+    const int r = e / ND, d = e - r * ND, c = r + d - W2;
+    AB[r * kChaseLd + d] = c >= 0 && c < n ? A[static_cast<size_t>(r) * n + c] : cplx<T>{T(0), T(0)};
+  }
+  tm.sync();
+  for (int i = 0; i + 1 < n; ++i) {
+    int col = i;
+    for (int k = 0;; ++k) {
+      const int r0 = i + 1 + k * b;
+      if (r0 > n - 1) break;
+      const int r1 = r0 + b - 1 < n - 1 ? r0 + b - 1 : n - 1;
+      const int len = r1 - r0 + 1;
+      const int hi = r1 + b < n - 1 ? r1 + b : n - 1;
+      // reflector from A[r0 .. r1][col], formed by every group
+      const cplx<T> x = q < len ? at(r0 + q, col) : cplx<T>{T(0), T(0)};
+      const T part = q >= 1 ? x.re * x.re + x.im * x.im : T(0);
+      const T xn2 = tm.template group_sum<16>(part);
+      const Refl<T> h = make_reflector(at(r0, col), xn2);
+      cplx<T> vq = {T(0), T(0)};
+      if (q == 0)
+        vq = {T(1), T(0)};
+      else if (q < len)
+        vq = cmul(x, h.scale);
+      if (id < b) V2[(static_cast<size_t>(i) * L.K + k) * b + id] = vq;
+      if (id == 0) tau2[static_cast<size_t>(i) * L.K + k] = h.tau;
+      tm.sync();  // every group has read the column before it changes
+      const bool apply = h.tau.re != T(0) || h.tau.im != T(0);
+      if (apply) {
+        // left: rows [r0, r1], columns [col, hi]: x <- x - conj(tau) v (v^H x)
+        const cplx<T> ct = conjc(h.tau);
+        for (int c0 = col; c0 <= hi; c0 += groups) {  // same trip count in every thread
+          const int c = c0 + g;
+          const bool in = c <= hi && q < len;
+          cplx<T> a = {T(0), T(0)};
+          if (in) a = at(r0 + q, c);
+          cplx<T> w = {T(0), T(0)};
+          cfmac(w, vq, a);
+          w.re = tm.template group_sum<16>(w.re);
+          w.im = tm.template group_sum<16>(w.im);
+          if (in) {
+            const cplx<T> d = cmul(vq, cmul(ct, w));
+            at(r0 + q, c) = {a.re - d.re, a.im - d.im};
+          }
+        }
+      }
+      tm.sync();
+      if (apply) {
+        // right: rows [col, hi], columns [r0, r1]: y <- y - tau (y v) v^H
+        for (int rr = col; rr <= hi; rr += groups) {  // same trip count in every thread
+          const int r = rr + g;
+          const bool in = r <= hi && q < len;
+          cplx<T> a = {T(0), T(0)};
+          if (in) a = at(r, r0 + q);
+          cplx<T> w = cmul(a, vq);
+          w.re = tm.template group_sum<16>(w.re);
+          w.im = tm.template group_sum<16>(w.im);
+          if (in) {
+            const cplx<T> d = cmul(cmul(w, h.tau), conjc(vq));
+            at(r, r0 + q) = {a.re - d.re, a.im - d.im};
+          }
+        }
+      }
+      tm.sync();
+      col = r0;
+    }
+  }
+  for (int j = id; j < n; j += nt) {
+    D[j] = at(j, j).re;
+    E[j] = j + 1 < n ? at(j + 1, j).re : T(0);
+  }
+  tm.sync();
+}
+
 // Column j of the eigenvectors of T (Y[i * ldy + j], unit) from its twisted
 // factorisation (Ts = [D+ | D-], [2][n][ldy], as twisted_all stores them);
 // twist < 0: a cluster member, already in Y.

@@ -78,7 +78,8 @@ int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_
 }
 
 // Two-stage route (eigh_band.hpp) with the team sizes of the GPU dispatch:
-// band_reduce (256 threads), band_chase (64), the tridiagonal eigenpairs,
+// band_reduce (256 threads), band_chase (64; FP32: band_chase_lds, 768), the
+// tridiagonal eigenpairs,
 // twisted_column, band_backtrans (W * S threads per block of W columns).
 template <typename T, int W, int S>
 int run_band(int n, const double* H, double* E, double* Z, int vectors, int rows_out) {
@@ -103,10 +104,17 @@ int run_band(int n, const double* H, double* E, double* Z, int vectors, int rows
   BandReduceSmem<T> sm{pV.data(), pY.data(), small.data(), small.data() + kBandB * kBandB,
                        small.data() + 2 * kBandB * kBandB, small.data() + 3 * kBandB * kBandB, red.data()};
   team(256, [&](auto& tm) { band_reduce<T>(tm, n, H, A.data(), band.data() + L.v1, band.data() + L.t1, sm); });
-  team(64, [&](auto& tm) {
-    band_chase<T>(tm, n, A.data(), d.data(), e.data(), band.data() + L.v2, band.data() + L.tau2, hv.data(),
-                  red.data());
-  });
+  if constexpr (sizeof(T) == 4) {  // the GPU's FP32 chase: band region in team memory, 768 threads
+    std::vector<cplx<T>> AB(band_chase_team_elems(n));
+    team(768, [&](auto& tm) {
+      band_chase_lds<T>(tm, n, A.data(), d.data(), e.data(), band.data() + L.v2, band.data() + L.tau2, AB.data());
+    });
+  } else {
+    team(64, [&](auto& tm) {
+      band_chase<T>(tm, n, A.data(), d.data(), e.data(), band.data() + L.v2, band.data() + L.tau2, hv.data(),
+                    red.data());
+    });
+  }
   team(n, [&](auto& tm) {
     bisect_all<T>(tm, n, d.data(), e.data(), e2.data(), Ev.data(), cnt.data());
     if (vectors) tridiag_vectors<T>(tm, n, d.data(), e.data(), Ev.data(), Y.data(), n, Sw.data(), twist.data());
