@@ -25,35 +25,54 @@ __device__ inline double wave_sum(double x) {
   return x;
 }
 
-// One wave per k-point: first the Rayleigh quotients, column scales and
-// residuals from the per-tile sums; then the lanes test every eigenvalue gap
-// in parallel; a k-point with a cluster forms the cluster's Gram and
-// projection sums with the lanes over rows, lane 0 solves the c x c problem
-// (LDS), and the lanes rotate the cluster's columns row by row, forming the
-// rotated columns' residuals on the way.  Column j of Z (and of HZ = H Z)
-// times scale[j] is the unit vector; rotated columns are written unit.
-// Last, the k-point's check (see RefineCheck).
-__global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restrict__ Z,
+// One workgroup of NT threads (64, 128 or 256 by n) per k-point: first the
+// Rayleigh quotients, column scales and residuals from the per-tile sums;
+// then the threads test every eigenvalue gap in parallel; a k-point with a
+// cluster forms H z' of all the cluster's columns in one pass over each row
+// of HZ (a thread per row), the cluster's Gram and projection sums with the
+// threads over rows (wave sums, then the waves' partials in wave order),
+// thread 0 solves the c x c problem (LDS), and the threads rotate the
+// cluster's columns row by row, forming the rotated columns' residuals on
+// the way.  Column j of Z (and of HZ = H Z) times scale[j] is the unit
+// vector; rotated columns are written unit.  Last, the k-point's check (see
+// RefineCheck).
+template <int NT>
+__global__ void __launch_bounds__(NT) cluster_rr_kernel(int n, double2* __restrict__ Z,
                                                         const double2* __restrict__ HZ,
                                                         const float2* __restrict__ F,
                                                         const double* __restrict__ part, int tiles,
                                                         double* __restrict__ E, double* __restrict__ scale,
                                                         double gap_rel, RefineCheck chk) {
+  constexpr int NW = NT / 64;
+  constexpr int NP = kMaxCluster * kMaxCluster * 4;  // per-wave partials of the Gram / projection sums
   __shared__ C2 lds[7][kMaxCluster][kMaxCluster];
   __shared__ double2 hzc[kMaxCluster][kMaxN];  // refined H z' of the cluster's columns
   __shared__ double ev[kMaxCluster];
   __shared__ double s_res[kMaxN];  // residual of unit column j
+  __shared__ double wred[NW][NP];
   __shared__ int solved;
   const size_t k = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double* Ek = E + k * n;
   double* sk = scale + k * n;
+  // the maximum over the workgroup, to every thread
+  auto block_max = [&](double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    if (lane == 0) wred[wave][0] = x;
+    __syncthreads();
+    double m = wred[0][0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) m = fmax(m, wred[w][0]);
+    __syncthreads();
+    return m;
+  };
   {
     // Rayleigh quotients, column scales and residuals (against the
     // Ogita-Aishima quotients) from the per-tile sums of the correction
     // product's epilogue
     const double* pk = part + k * tiles * 3 * static_cast<size_t>(n);
-    for (int j = lane; j < n; j += 64) {
+    for (int j = tid; j < n; j += NT) {
       double num = 0.0, den = 0.0, res = 0.0;
       for (int t = 0; t < tiles; ++t) {
         num += pk[(3 * t) * static_cast<size_t>(n) + j];
@@ -68,41 +87,49 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
     __syncthreads();
   }
   double m = 0.0;
-  for (int j = lane; j < n; j += 64) m = fmax(m, fabs(Ek[j]));
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  for (int j = tid; j < n; j += NT) m = fmax(m, fabs(Ek[j]));
+  m = block_max(m);
   const double thr = gap_rel * fmax(m, 1.0);
   int close = 0;
-  for (int j = lane; j + 1 < n; j += 64) close |= (Ek[j + 1] - Ek[j] <= thr);
-  if (__any(close)) {
+  for (int j = tid; j + 1 < n; j += NT) close |= (Ek[j + 1] - Ek[j] <= thr);
+  if (__syncthreads_or(close)) {
     double2* Zk = Z + k * n * n;
     const double2* Hk = HZ + k * n * n;
     const float2* Fk = F + k * n * n;
     int j0 = 0;
-    while (j0 < n) {  // the same cluster sequence in every lane
+    while (j0 < n) {  // the same cluster sequence in every thread
       const int j1 = cluster_end(n, Ek, j0, thr);
       const int c = j1 - j0 + 1;
       // (a run of more than kMaxCluster close eigenvalues is split; the
       // residuals of the columns at a split fail the check)
       if (c >= 2) {
-        // H z'_b = HZ (e_b + (F' - I)_b) for the cluster's columns, in FP64
-        for (int b = 0; b < c; ++b)
-          for (int r = lane; r < n; r += 64) {
-            const double2* hr = Hk + static_cast<size_t>(r) * n;
-            double2 acc = hr[j0 + b];
-            for (int q = 0; q < n; ++q) {
-              const double2 h = hr[q];
-              const float2 f = Fk[static_cast<size_t>(q) * n + j0 + b];
-              acc.x += h.x * f.x - h.y * f.y;
-              acc.y += h.x * f.y + h.y * f.x;
-            }
-            hzc[b][r] = acc;
+        // H z'_b = HZ (e_b + (F' - I)_b) for the cluster's columns, in FP64:
+        // one pass over row r of HZ for all of them
+        for (int r = tid; r < n; r += NT) {
+          const double2* hr = Hk + static_cast<size_t>(r) * n;
+          double2 acc[kMaxCluster];
+#pragma unroll
+          for (int b = 0; b < kMaxCluster; ++b) acc[b] = b < c ? hr[j0 + b] : make_double2(0.0, 0.0);
+          for (int q = 0; q < n; ++q) {
+            const double2 h = hr[q];
+            const float2* fq = Fk + static_cast<size_t>(q) * n + j0;
+#pragma unroll
+            for (int b = 0; b < kMaxCluster; ++b)
+              if (b < c) {
+                const float2 f = fq[b];
+                acc[b].x += h.x * f.x - h.y * f.y;
+                acc[b].y += h.x * f.y + h.y * f.x;
+              }
           }
+#pragma unroll
+          for (int b = 0; b < kMaxCluster; ++b)
+            if (b < c) hzc[b][r] = acc[b];
+        }
         __syncthreads();
         for (int p = 0; p < c * c; ++p) {
           const int a = p / c, b = p % c;
           double gr = 0.0, gi = 0.0, mr = 0.0, mi = 0.0;
-          for (int r = lane; r < n; r += 64) {
+          for (int r = tid; r < n; r += NT) {
             const double2 za = Zk[static_cast<size_t>(r) * n + j0 + a];
             const double2 zb = Zk[static_cast<size_t>(r) * n + j0 + b];
             const double2 hb = hzc[b][r];
@@ -111,24 +138,42 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
             mr += za.x * hb.x + za.y * hb.y;  // conj(za) hb
             mi += za.x * hb.y - za.y * hb.x;
           }
-          // sums over the unit columns s_a z_a, s_b z_b (H z_b scales with z_b)
-          const double sab = sk[j0 + a] * sk[j0 + b];
-          gr = wave_sum(gr) * sab;
-          gi = wave_sum(gi) * sab;
-          mr = wave_sum(mr) * sab;
-          mi = wave_sum(mi) * sab;
+          gr = wave_sum(gr);
+          gi = wave_sum(gi);
+          mr = wave_sum(mr);
+          mi = wave_sum(mi);
           if (lane == 0) {
-            lds[0][a][b] = {gr, gi};
-            lds[1][a][b] = {mr, mi};
+            wred[wave][4 * p] = gr;
+            wred[wave][4 * p + 1] = gi;
+            wred[wave][4 * p + 2] = mr;
+            wred[wave][4 * p + 3] = mi;
           }
         }
-        if (lane == 0) solved = cluster_rr_small(c, lds, ev) ? 1 : 0;
+        __syncthreads();
+        if (tid < c * c) {
+          // sums over the unit columns s_a z_a, s_b z_b (H z_b scales with z_b)
+          const int a = tid / c, b = tid % c;
+          double gr = wred[0][4 * tid], gi = wred[0][4 * tid + 1], mr = wred[0][4 * tid + 2],
+                 mi = wred[0][4 * tid + 3];
+#pragma unroll
+          for (int w = 1; w < NW; ++w) {
+            gr += wred[w][4 * tid];
+            gi += wred[w][4 * tid + 1];
+            mr += wred[w][4 * tid + 2];
+            mi += wred[w][4 * tid + 3];
+          }
+          const double sab = sk[j0 + a] * sk[j0 + b];
+          lds[0][a][b] = {gr * sab, gi * sab};
+          lds[1][a][b] = {mr * sab, mi * sab};
+        }
+        __syncthreads();
+        if (tid == 0) solved = cluster_rr_small(c, lds, ev) ? 1 : 0;
         __syncthreads();
         if (solved) {
           double res2[kMaxCluster];
 #pragma unroll
           for (int q = 0; q < kMaxCluster; ++q) res2[q] = 0.0;
-          for (int r = lane; r < n; r += 64) {
+          for (int r = tid; r < n; r += NT) {
             double2* zr = Zk + static_cast<size_t>(r) * n + j0;
             C2 row[kMaxCluster], hrow[kMaxCluster];
 #pragma unroll
@@ -161,18 +206,23 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
           }
 #pragma unroll
           for (int bb = 0; bb < kMaxCluster; ++bb)
-            if (bb < c) res2[bb] = wave_sum(res2[bb]);
-          __syncthreads();  // every lane has read the old scales
-          if (lane < c) {
-            Ek[j0 + lane] = ev[lane];
-            sk[j0 + lane] = 1.0;  // the rotated columns are unit vectors
-          }
+            if (bb < c) {
+              const double x = wave_sum(res2[bb]);
+              if (lane == 0) wred[wave][bb] = x;
+            }
+          __syncthreads();  // every thread has read the old scales
+          if (tid < c) {
+            double r2 = wred[0][tid];
 #pragma unroll
-          for (int bb = 0; bb < kMaxCluster; ++bb)
-            if (bb < c && lane == bb) s_res[j0 + bb] = sqrt(res2[bb]);
+            for (int w = 1; w < NW; ++w) r2 += wred[w][tid];
+            Ek[j0 + tid] = ev[tid];
+            sk[j0 + tid] = 1.0;  // the rotated columns are unit vectors
+            s_res[j0 + tid] = sqrt(r2);
+          }
         } else {
-          for (int q = lane; q < c; q += 64) s_res[j0 + q] = HUGE_VAL;
+          for (int q = tid; q < c; q += NT) s_res[j0 + q] = HUGE_VAL;
         }
+        __threadfence_block();
         __syncthreads();
       }
       j0 = j1 + 1;
@@ -183,23 +233,21 @@ __global__ void __launch_bounds__(64) cluster_rr_kernel(int n, double2* __restri
   // the check: every eigenvalue finite, every residual within the bound
   double mx = 0.0, rmax = 0.0;
   int bad = 0;
-  for (int j = lane; j < n; j += 64) {
+  for (int j = tid; j < n; j += NT) {
     const double e = Ek[j];
     bad |= !isfinite(e);
     mx = fmax(mx, fabs(e));
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  mx = block_max(mx);
   const double bound = chk.res_tol * fmax(1.0, mx);
-  for (int j = lane; j < n; j += 64) {
+  for (int j = tid; j < n; j += NT) {
     const double r = s_res[j];
     bad |= !(r <= bound);
     if (isfinite(r)) rmax = fmax(rmax, r);
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
-  const int any_bad = __any(bad);
-  if (lane == 0) {
+  rmax = block_max(rmax);
+  const int any_bad = __syncthreads_or(bad);
+  if (tid == 0) {
     if (any_bad && chk.fail_count) {
       const int slot = atomicAdd(chk.fail_count, 1);
       chk.fail_list[slot] = static_cast<int>(k);
@@ -300,9 +348,18 @@ int refine_eigenpairs(hipStream_t st, Timer* tmr, int n, size_t nk, double* H, c
   RTB_TRY(zgemm_corrections_f32(st, n, nk, Z, W1, F, E, W2, part));  // both products, one pass over F
   if (tmr) RTB_TRY(tmr->end(1, 2 * flop_zgemm(n, nk), 2 * mflop_zgemm(n, nk, false)));
   if (tmr) RTB_TRY(tmr->begin(RTB_STAGE_REFINE));
-  hipLaunchKernelGGL(cluster_rr_kernel, dim3(static_cast<unsigned>(nk)), dim3(64), 0, st, n,
-                     reinterpret_cast<double2*>(W2), reinterpret_cast<const double2*>(W1),
-                     reinterpret_cast<const float2*>(F), part, tiles, E, scale, gap_rel, chk);
+  // a thread per row of the k-point's matrices
+  auto rr = [&](auto kernel, int nt) {
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(nk)), dim3(nt), 0, st, n, reinterpret_cast<double2*>(W2),
+                       reinterpret_cast<const double2*>(W1), reinterpret_cast<const float2*>(F), part, tiles, E,
+                       scale, gap_rel, chk);
+  };
+  if (n <= 64)
+    rr(cluster_rr_kernel<64>, 64);
+  else if (n <= 128)
+    rr(cluster_rr_kernel<128>, 128);
+  else
+    rr(cluster_rr_kernel<256>, 256);
   RTB_HIP(hipGetLastError());
   if (tmr) RTB_TRY(tmr->end(1));
   return RTB_OK;
