@@ -215,6 +215,30 @@ def test_host_part_matches_oracle(T, shuffle):
     np.testing.assert_array_equal(out, ref)
 
 
+def test_host_part_undoes_three_cycles():
+    """Every vertex but the first with three of its six bands rotated (a
+    3-cycle, which is not its own inverse): tracking must restore the
+    unshuffled result bit for bit, so a permutation applied as its inverse
+    (or a transposed overlap) would fail here where 2-band swaps cannot."""
+    m = wide_model(O.Model, 21, 6, 2)
+    nx, ny = 6, 5
+    mu = np.linspace(-3, 3, 13)
+    pts = EC.vertices(m, (nx, ny), np.eye(2))
+    assert min_vertex_tie(pts, nx, ny) > 1e-6
+    ref = EC.hall_conductivity_energy_cut(m, (nx, ny), np.eye(2), mu, 0.0, 0.05)
+    rng = np.random.default_rng(4)
+    shuffled = [pts[0]]
+    for q in pts[1:]:
+        p = np.arange(6)
+        idx = rng.choice(6, 3, replace=False)
+        p[idx] = idx[[1, 2, 0]]
+        assert np.any(p[p] != np.arange(6))
+        shuffled.append(EC.permute_vertex(q, p))
+    rc, out = host_sigma(*vertex_arrays(shuffled, nx, ny), nx, ny, 6, mu, 0.0, 0.05, 1.0)
+    assert rc == 0 and np.abs(ref).max() > 1e-3
+    np.testing.assert_array_equal(out, ref)
+
+
 def test_host_part_spinful_matches_oracle():
     m = soc_broken(O.Model)
     nx, ny = 5, 4
@@ -312,7 +336,9 @@ RAGGED = [(1, 5), (5, 1), (1, 1), (2, 2), (3, 7)]
 def test_host_part_ragged_meshes(mesh):
     """Meshes with one row, one column or one point: cells whose corners
     coincide give zero-area triangles, which the cut skips as the reference
-    does; tracking walks whatever neighbours exist."""
+    does, so sigma is 0 there (the (1, 5), (5, 1) and (1, 1) cases check the
+    skipping and the walk, not the cut); tracking walks whatever neighbours
+    exist."""
     m = haldane_broken(O.Model)
     nx, ny = mesh
     mu = np.linspace(-3, 3, 9)
@@ -573,8 +599,13 @@ def test_gpu_device_walk_equals_host_walk(seed, n, mesh, monkeypatch):
     every vertex (counted with the oracle's global_band_track)."""
     dim = len(mesh)
     pts = list(EC.vertices(wide_model(O.Model, seed, n, dim), mesh, np.eye(dim)[:2]))
+    sorted_bands = [q.band.copy() for q in pts]
     EC.global_band_track(pts, mesh)
     assert sum(bool(np.any(np.diff(q.band) < 0)) for q in pts) > len(pts) // 2
+    # the walk's permutation of each vertex (tracked band i = sorted band p[i]);
+    # some are not involutions, so a permutation applied as its inverse fails
+    perms = [np.searchsorted(sb, q.band) for sb, q in zip(sorted_bands, pts)]
+    assert any(np.any(p[p] != np.arange(n)) for p in perms)
     m = wide_model(ra.Model, seed, n, dim)
     d = np.eye(dim)
     p = ra.Parameters.rank2(mesh, d[0], d[1], np.linspace(-6, 6, 25), eta=0.05, integration="EnergyCut")
