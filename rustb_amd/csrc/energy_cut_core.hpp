@@ -96,33 +96,66 @@ RTB_CUT inline double band_at_lam(const View& M, const size_t* vs, int n, const 
   return -2.0 * im;
 }
 
-// indices of e[0..N) in ascending order, ties in index order (a stable sort)
+// e[i] for a run-time i, as a chain of selects, so the small arrays here stay
+// in registers on the device.  The values are made opaque to the optimizer
+// first (an empty asm statement on each, no instruction), else it folds the
+// selects of loads back into one load at a run-time index, which keeps the
+// array in scratch memory.
+template <int N>
+RTB_CUT inline double pick(const double* e, int i) {
+  double v[N];
+  for (int q = 0; q < N; ++q) {
+    v[q] = e[q];
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(v[q]));
+#endif
+  }
+  double x = v[0];
+  for (int q = 1; q < N; ++q) x = q == i ? v[q] : x;
+  return x;
+}
+
+// l = the unit vector e_i (N entries)
+template <int N>
+RTB_CUT inline void unit_vec(double* l, int i) {
+  for (int q = 0; q < N; ++q) l[q] = q == i ? 1.0 : 0.0;
+}
+
+// indices of e[0..N) in ascending order, ties in index order (a stable sort):
+// compare-exchanges of adjacent positions only (N rounds of odd-even
+// transposition), each swapping on a strict <, keep equal keys in index order
 template <int N>
 RTB_CUT inline void sort_idx(const double* e, int* idx) {
-  for (int i = 0; i < N; ++i) idx[i] = i;
-  for (int i = 1; i < N; ++i) {
-    const int x = idx[i];
-    int j = i;
-    while (j > 0 && e[x] < e[idx[j - 1]]) {
-      idx[j] = idx[j - 1];
-      --j;
-    }
-    idx[j] = x;
+  double k[N];
+  int o[N];
+  for (int i = 0; i < N; ++i) {
+    k[i] = e[i];
+    o[i] = i;
   }
+  for (int round = 0; round < N; ++round)
+    for (int i = round % 2; i + 1 < N; i += 2)
+      if (k[i + 1] < k[i]) {
+        const double tk = k[i];
+        k[i] = k[i + 1];
+        k[i + 1] = tk;
+        const int to = o[i];
+        o[i] = o[i + 1];
+        o[i + 1] = to;
+      }
+  for (int i = 0; i < N; ++i) idx[i] = o[i];
 }
 
 // the point of edge (i, j) where the linear E crosses mu, as barycentrics
 template <int N>
 RTB_CUT inline void cut_point(const double* e, int i, int j, double mu, double* l) {
-  for (int q = 0; q < N; ++q) l[q] = 0.0;
-  const double de = e[j] - e[i];
+  const double ei = pick<N>(e, i);
+  const double de = pick<N>(e, j) - ei;
   double t = 0.5;
   if (!(fabs(de) < kEps)) {
-    t = (mu - e[i]) / de;
+    t = (mu - ei) / de;
     t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
   }
-  l[i] = 1.0 - t;
-  l[j] = t;
+  for (int q = 0; q < N; ++q) l[q] = q == i ? 1.0 - t : (q == j ? t : 0.0);
 }
 
 // ---- triangles ----
@@ -154,16 +187,16 @@ RTB_CUT inline int clip(const double e[3], double mu, double out[2][3][3]) {
   int idx[3];
   sort_idx<3>(e, idx);
   const int a = idx[0], b = idx[1], c = idx[2];
-  if (mu <= e[b] + kEps) {
-    out[0][0][a] = 1.0;
+  if (mu <= pick<3>(e, b) + kEps) {
+    unit_vec<3>(out[0][0], a);
     cut_point<3>(e, a, b, mu, out[0][1]);
     cut_point<3>(e, a, c, mu, out[0][2]);
     return 1;
   }
-  out[0][0][a] = 1.0;
-  out[0][1][b] = 1.0;
+  unit_vec<3>(out[0][0], a);
+  unit_vec<3>(out[0][1], b);
   cut_point<3>(e, a, c, mu, out[0][2]);
-  out[1][0][b] = 1.0;
+  unit_vec<3>(out[1][0], b);
   cut_point<3>(e, a, c, mu, out[1][1]);
   cut_point<3>(e, b, c, mu, out[1][2]);
   return 2;
@@ -200,7 +233,9 @@ RTB_CUT inline double hybrid_tri_by(const double c[3][2], const double e_v[3], c
   double subs[2][3][3];
   const int ns = clip(e_v, mu, subs);
   double total = 0.0;
-  for (int s = 0; s < ns; ++s) {
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {  // unrolled: subs is indexed at compile time
+    if (s >= ns) break;
     const double sa = sub_area(c, subs[s]);
     if (sa < 1e-30) continue;
     for (int iq = 0; iq < 3; ++iq) {
@@ -269,18 +304,15 @@ RTB_CUT inline double hybrid_tet_by(const double c[4][3], const double e_v[4], c
   sort_idx<4>(e_v, idx);
   const int a = idx[0], b = idx[1], cc = idx[2], d = idx[3];
   double s[4][4];
-  auto unit = [](int i, double* l) {
-    for (int q = 0; q < 4; ++q) l[q] = 0.0;
-    l[i] = 1.0;
-  };
-  if (mu <= e_v[b] + kEps) {
+  auto unit = [](int i, double* l) { unit_vec<4>(l, i); };
+  if (mu <= pick<4>(e_v, b) + kEps) {
     unit(a, s[0]);
     cut_point<4>(e_v, a, b, mu, s[1]);
     cut_point<4>(e_v, a, cc, mu, s[2]);
     cut_point<4>(e_v, a, d, mu, s[3]);
     return sub_tet_quad(c, s, f);
   }
-  if (mu <= e_v[cc] + kEps) {  // three sub-tetrahedra through the p_ac - p_bd diagonal
+  if (mu <= pick<4>(e_v, cc) + kEps) {  // three sub-tetrahedra through the p_ac - p_bd diagonal
     double total = 0.0;
     unit(a, s[0]);
     unit(b, s[1]);
@@ -312,51 +344,55 @@ RTB_CUT inline double hybrid_tet(const View& M, const double c[4][3], const size
 }
 
 // ---- cells ----
-// The triangles of 2D cell (ix, iy) over both diagonals
-// (build_triangles_2d_diagavg, tracking.rs:284-342): vertices, fractional
-// coordinates; each triangle carries a quarter of the cell.
-RTB_CUT inline void cell_triangles(size_t ix, size_t iy, size_t nx, size_t ny, size_t vs[4][3],
-                                   double c[4][3][2]) {
+// Triangle t of 2D cell (ix, iy) over both diagonals
+// (build_triangles_2d_diagavg, tracking.rs:284-342): vertices and fractional
+// coordinates; each triangle carries a quarter of the cell.  Corners 00, 10,
+// 11, 01 are numbered 0..3; a corner shared by two vertices of a ragged mesh
+// keeps the coordinates of its first match, as the reference's lookup.
+RTB_CUT inline void cell_triangle(size_t ix, size_t iy, size_t nx, size_t ny, int t, size_t vs[3], double c[3][2]) {
+  constexpr int kTri[4][3] = {{0, 1, 3}, {2, 1, 3}, {0, 1, 2}, {0, 2, 3}};
   const double inv_nx = 1.0 / static_cast<double>(nx), inv_ny = 1.0 / static_cast<double>(ny);
   const size_t ixp = (ix + 1) % nx, iyp = (iy + 1) % ny;
-  const size_t i00 = ix * ny + iy, i10 = ixp * ny + iy, i11 = ixp * ny + iyp, i01 = ix * ny + iyp;
   const double fx = static_cast<double>(ix), fy = static_cast<double>(iy);
   const double x0 = fx * inv_nx, x1 = (fx + 1.0) * inv_nx, y0 = fy * inv_ny, y1 = (fy + 1.0) * inv_ny;
-  // corners 00, 10, 11, 01 (a corner shared by two vertices of a ragged
-  // mesh keeps the coordinates of its first match, as the reference's lookup)
-  const size_t id[4] = {i00, i10, i11, i01};
-  const double px[4] = {x0, x1, x1, x0}, py[4] = {y0, y0, y1, y1};
-  const int tri[4][3] = {{0, 1, 3}, {2, 1, 3}, {0, 1, 2}, {0, 2, 3}};
-  for (int t = 0; t < 4; ++t)
-    for (int q = 0; q < 3; ++q) {
-      const size_t v = id[tri[t][q]];
-      int k = 0;
-      while (id[k] != v) ++k;  // first corner with this vertex
-      vs[t][q] = v;
-      c[t][q][0] = px[k];
-      c[t][q][1] = py[k];
-    }
+  auto vid = [&](int k) { return (k == 1 || k == 2 ? ixp : ix) * ny + (k >= 2 ? iyp : iy); };
+  for (int q = 0; q < 3; ++q) {
+    const size_t v = vid(kTri[t][q]);
+    int k = 0;
+    while (vid(k) != v) ++k;  // first corner with this vertex
+    vs[q] = v;
+    c[q][0] = k == 1 || k == 2 ? x1 : x0;
+    c[q][1] = k >= 2 ? y1 : y0;
+  }
 }
 
-// The five tetrahedra of 3D cell (ix, iy, iz) (build_tetrahedra_3d,
-// tracking.rs:344-412): vertices, fractional coordinates, volumes.
-RTB_CUT inline void cell_tetrahedra(size_t ix, size_t iy, size_t iz, size_t nx, size_t ny, size_t nz, size_t vs[5][4],
-                                    double c[5][4][3], double vol[5]) {
+RTB_CUT inline void cell_triangles(size_t ix, size_t iy, size_t nx, size_t ny, size_t vs[4][3],
+                                   double c[4][3][2]) {
+  for (int t = 0; t < 4; ++t) cell_triangle(ix, iy, nx, ny, t, vs[t], c[t]);
+}
+
+// Tetrahedron t of 3D cell (ix, iy, iz) (build_tetrahedra_3d,
+// tracking.rs:344-412): vertices, fractional coordinates, volume.
+RTB_CUT inline void cell_tetrahedron(size_t ix, size_t iy, size_t iz, size_t nx, size_t ny, size_t nz, int t,
+                                     size_t vs[4], double c[4][3], double* vol) {
   const double inv_nx = 1.0 / static_cast<double>(nx), inv_ny = 1.0 / static_cast<double>(ny),
                inv_nz = 1.0 / static_cast<double>(nz);
   const double cube_vol = inv_nx * inv_ny * inv_nz;
   const size_t ixp = (ix + 1) % nx, iyp = (iy + 1) % ny, izp = (iz + 1) % nz;
-  for (int t = 0; t < 5; ++t) {
-    for (int q = 0; q < 4; ++q) {
-      const int k = kCubeTets[t][q];
-      const size_t x = k & 1 ? ixp : ix, y = k & 2 ? iyp : iy, z = k & 4 ? izp : iz;
-      vs[t][q] = (x * ny + y) * nz + z;
-      c[t][q][0] = static_cast<double>(ix + (k & 1 ? 1 : 0)) * inv_nx;
-      c[t][q][1] = static_cast<double>(iy + (k & 2 ? 1 : 0)) * inv_ny;
-      c[t][q][2] = static_cast<double>(iz + (k & 4 ? 1 : 0)) * inv_nz;
-    }
-    vol[t] = cube_vol * kTetVolFactor[t];
+  for (int q = 0; q < 4; ++q) {
+    const int k = kCubeTets[t][q];
+    const size_t x = k & 1 ? ixp : ix, y = k & 2 ? iyp : iy, z = k & 4 ? izp : iz;
+    vs[q] = (x * ny + y) * nz + z;
+    c[q][0] = static_cast<double>(ix + (k & 1 ? 1 : 0)) * inv_nx;
+    c[q][1] = static_cast<double>(iy + (k & 2 ? 1 : 0)) * inv_ny;
+    c[q][2] = static_cast<double>(iz + (k & 4 ? 1 : 0)) * inv_nz;
   }
+  *vol = cube_vol * kTetVolFactor[t];
+}
+
+RTB_CUT inline void cell_tetrahedra(size_t ix, size_t iy, size_t iz, size_t nx, size_t ny, size_t nz, size_t vs[5][4],
+                                    double c[5][4][3], double vol[5]) {
+  for (int t = 0; t < 5; ++t) cell_tetrahedron(ix, iy, iz, nx, ny, nz, t, vs[t], c[t], &vol[t]);
 }
 
 }  // namespace cut

@@ -48,27 +48,15 @@ __device__ inline void load_simplex(const cut::View& V, const double* __restrict
   const int s = static_cast<int>((g / n) % S);
   x.b = static_cast<int>(g % n);
   if constexpr (DIM == 2) {
-    size_t vv[4][3];
-    double cc[4][3][2];
-    cut::cell_triangles(cell / ny, cell % ny, nx, ny, vv, cc);
-    for (int q = 0; q < 3; ++q) {
-      x.vs[q] = vv[s][q];
-      x.c[q][0] = cc[s][q][0];
-      x.c[q][1] = cc[s][q][1];
-    }
+    cut::cell_triangle(cell / ny, cell % ny, nx, ny, s, x.vs, x.c);
     x.measure = cut::tri_area(x.c);
     const double half_vol = (1.0 / static_cast<double>(nx)) * (1.0 / static_cast<double>(ny)) / 4.0;
     x.scale = half_vol / x.measure;
   } else {
-    size_t vv[5][4];
-    double cc[5][4][3], vol[5];
-    cut::cell_tetrahedra(cell / (ny * nz), (cell / nz) % ny, cell % nz, nx, ny, nz, vv, cc, vol);
-    for (int q = 0; q < 4; ++q) {
-      x.vs[q] = vv[s][q];
-      for (int d = 0; d < 3; ++d) x.c[q][d] = cc[s][q][d];
-    }
+    double vol;
+    cut::cell_tetrahedron(cell / (ny * nz), (cell / nz) % ny, cell % nz, nx, ny, nz, s, x.vs, x.c, &vol);
     x.measure = cut::tet_vol(x.c[0], x.c[1], x.c[2], x.c[3]);
-    x.scale = vol[s] / x.measure;
+    x.scale = vol / x.measure;
   }
   for (int q = 0; q < Simplex<DIM>::NV; ++q) {
     x.e_v[q] = cut::band(V, x.vs[q], x.b);
