@@ -615,3 +615,27 @@ def test_gpu_device_walk_equals_host_walk(seed, n, mesh, monkeypatch):
     host = m.hall_conductivity(p).conductivity
     assert np.abs(host).max() > 1e-3
     np.testing.assert_array_equal(dev, host)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed,mesh", [(11, (1, 6, 4)), (12, (6, 1, 4)), (13, (6, 4, 1)), (14, (1, 9)), (15, (9, 1))])
+def test_gpu_device_walk_ragged_meshes(seed, mesh, monkeypatch):
+    """Meshes with a single point along an axis, where index steps of two
+    axes coincide and the walk must take each pair's overlaps from the axis
+    the host walk takes them from: the device walk equals the host walk
+    (RTB_TRACK_HOST=1) bit for bit on a 24-band model whose bands the walk
+    permutes at nearly every vertex (non-involutions included)."""
+    dim, n = len(mesh), 24
+    pts = list(EC.vertices(wide_model(O.Model, seed, n, dim), mesh, np.eye(dim)[:2]))
+    sorted_bands = [q.band.copy() for q in pts]
+    EC.global_band_track(pts, mesh)
+    perms = [np.searchsorted(sb, q.band) for sb, q in zip(sorted_bands, pts)]
+    assert any(np.any(p[p] != np.arange(n)) for p in perms)
+    m = wide_model(ra.Model, seed, n, dim)
+    d = np.eye(dim)
+    p = ra.Parameters.rank2(mesh, d[0], d[1], np.linspace(-6, 6, 25), eta=0.05, integration="EnergyCut")
+    monkeypatch.delenv("RTB_TRACK_HOST", raising=False)
+    dev = m.hall_conductivity(p).conductivity
+    monkeypatch.setenv("RTB_TRACK_HOST", "1")
+    host = m.hall_conductivity(p).conductivity
+    np.testing.assert_array_equal(dev, host)
