@@ -639,3 +639,25 @@ def test_gpu_device_walk_ragged_meshes(seed, mesh, monkeypatch):
     monkeypatch.setenv("RTB_TRACK_HOST", "1")
     host = m.hall_conductivity(p).conductivity
     np.testing.assert_array_equal(dev, host)
+
+
+@pytest.mark.gpu
+def test_gpu_n136_two_stage_and_host_walk():
+    """n = 136: the vertex stage takes the two-stage FP64 eigensolver and,
+    above 128 bands, the squared overlaps go to the host walk; against the
+    host part fed the oracle's vertex data.  Tolerance 1e-8 of max|sigma|:
+    the two-stage reduction rounds more than the n <= 128 route, and the
+    smallest energy gap inside a triangle is 1.1e-4 here."""
+    n = 136
+    m, o = wide_model(ra.Model, 32, n, 2), wide_model(O.Model, 32, n, 2)
+    mesh = (4, 3)
+    mu = np.linspace(-6, 6, 13)
+    pts = EC.vertices(o, mesh, np.eye(2))
+    assert min_vertex_tie(pts, *mesh) > 5e-5
+    E, K, ovx, ovy = vertex_arrays(pts, *mesh)
+    assert overlap_margin([ovx, ovy]) > 1e-7
+    rc, ref = host_sigma(E, K, ovx, ovy, *mesh, n, mu, 0.0, 0.05, 1.0)
+    assert rc == 0
+    p = ra.Parameters.rank2(mesh, [1.0, 0.0], [0.0, 1.0], mu, eta=0.05, integration="EnergyCut")
+    got = m.hall_conductivity(p).conductivity
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-8 * np.abs(ref).max())
