@@ -1513,18 +1513,20 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   const char* th = std::getenv("RTB_TRACK_HOST");
   const bool dev_track = n <= kTrackDevMaxN && nk <= 0xffffffffull && !(th && std::atoi(th) == 1);
   TrackSchedule ts;
-  unsigned *sched_v = nullptr, *sched_d = nullptr;
-  unsigned char* sched_n = nullptr;
+  TrackEntry* sched = nullptr;
   if (dev_track) {
     energy_cut_schedule(nx, ny, nz, ts);
-    const std::vector<unsigned> hv(ts.by_level.begin(), ts.by_level.end()), hd(ts.dep.begin(), ts.dep.end());
-    RTB_TRY(w.alloc(&sched_v, nk));
-    RTB_TRY(w.alloc(&sched_d, nk * 6));
-    RTB_TRY(w.alloc(&sched_n, nk));
-    RTB_HIP(hipMemcpyAsync(sched_v, hv.data(), nk * sizeof(unsigned), hipMemcpyHostToDevice, st));
-    RTB_HIP(hipMemcpyAsync(sched_d, hd.data(), nk * 6 * sizeof(unsigned), hipMemcpyHostToDevice, st));
-    RTB_HIP(hipMemcpyAsync(sched_n, ts.ndep.data(), nk, hipMemcpyHostToDevice, st));
-    RTB_HIP(hipStreamSynchronize(st));  // the host vectors go out of scope here
+    std::vector<TrackEntry> he(nk);  // level order: vertex, count, visited neighbours
+    for (size_t i = 0; i < nk; ++i) {
+      const size_t v = ts.by_level[i];
+      TrackEntry& e = he[i];
+      e.v = static_cast<unsigned>(v);
+      e.cnt = ts.ndep[v];
+      for (int r = 0; r < 6; ++r) e.dep[r] = r < static_cast<int>(e.cnt) ? static_cast<unsigned>(ts.dep[v * 6 + r]) : 0u;
+    }
+    RTB_TRY(w.alloc(&sched, nk));
+    RTB_HIP(hipMemcpyAsync(sched, he.data(), nk * sizeof(TrackEntry), hipMemcpyHostToDevice, st));
+    RTB_HIP(hipStreamSynchronize(st));  // the host vector goes out of scope here
   }
   Checks ck;
   RTB_TRY(checks_init(st, w, 0, ck));
@@ -1587,8 +1589,7 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   std::vector<int> perm;
   if (dev_track) {
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
-    RTB_TRY(energy_cut_track_dev(st, nx, ny, nz, n, dOv[0], dOv[1], dOv[2], sched_v, ts.level_start, sched_n,
-                                 sched_d, dperm));
+    RTB_TRY(energy_cut_track_dev(st, nx, ny, nz, n, dOv[0], dOv[1], dOv[2], sched, ts.level_start, dperm));
     RTB_TRY(tmr.end(static_cast<int>(ts.level_start.size() - 1)));
   } else {
     std::vector<double> hOv[3];

@@ -249,14 +249,18 @@ int energy_cut_omega(hipStream_t st, int n, size_t nk, const double* E, const do
                      double* om);
 size_t energy_cut_waves(int dim, size_t nx, size_t ny, size_t nz, int n);
 // Band tracking on the device for n <= kTrackDevMaxN: the walk of
-// energy_cut_track over the schedule of energy_cut_schedule (by_level [nk],
-// ndep [nk], dep [nk][6] as 32-bit vertex indices on the device, level_start
-// on the host), from the squared overlaps; perm [nk][n] on the device.
+// energy_cut_track over the schedule of energy_cut_schedule, given as one
+// entry per vertex in level order (the vertex, its count of visited
+// neighbours and those neighbours, 32-bit indices) on the device and the
+// level bounds on the host; from the squared overlaps; perm [nk][n] on the
+// device.
 constexpr int kTrackDevMaxN = 128;
+struct TrackEntry {
+  unsigned v, cnt, dep[6];
+};
 int energy_cut_track_dev(hipStream_t st, size_t nx, size_t ny, size_t nz, int n, const double* ovx,
-                         const double* ovy, const double* ovz, const unsigned* by_level,
-                         const std::vector<size_t>& level_start, const unsigned char* ndep, const unsigned* dep,
-                         int* perm);
+                         const double* ovy, const double* ovz, const TrackEntry* sched,
+                         const std::vector<size_t>& level_start, int* perm);
 int energy_cut_track_smem_limits();
 
 int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int n, const double* E, const double* Ki,

@@ -180,8 +180,8 @@ __global__ void __launch_bounds__(256) cut_kernel(cut::View V, const double* __r
 // still free takes it, which is what the scan over the free columns returns;
 // otherwise wave 0 scans the free columns.  The same operations on the same
 // values as the host walk, so the permutations are identical.
-constexpr int kTrackThreads = 256;
 constexpr int kTrackMaxDep = 6;  // visited neighbours of a 3D vertex at most
+static_assert(sizeof(TrackEntry) == 8 * sizeof(unsigned), "schedule entry layout");
 
 size_t track_smem(int n) {
   return static_cast<size_t>(n) * n * sizeof(double) + n * sizeof(double) +
@@ -193,12 +193,12 @@ __global__ void identity_perm_kernel(size_t total, int n, int* __restrict__ perm
   if (g < total) perm[g] = static_cast<int>(g % n);
 }
 
-__global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __restrict__ ovx,
-                                                              const double* __restrict__ ovy,
-                                                              const double* __restrict__ ovz, size_t ny, size_t nz,
-                                                              int n, const unsigned* __restrict__ verts,
-                                                              const unsigned char* __restrict__ ndep,
-                                                              const unsigned* __restrict__ dep, int* __restrict__ perm) {
+// kTrackThreads per vertex (one wave of 64 was slower at n = 32: the matrix
+// build loses its parallelism).
+constexpr int kTrackThreads = 256;
+__global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __restrict__ ovx, const double* __restrict__ ovy,
+                                                   const double* __restrict__ ovz, size_t ny, size_t nz, int n,
+                                                   const TrackEntry* __restrict__ sched, int* __restrict__ perm) {
   extern __shared__ double smem[];
   double* M = smem;                                // [n][n]
   double* rmax = M + static_cast<size_t>(n) * n;  // [n]
@@ -206,15 +206,17 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   int* order = best + n;                           // [n] rows by descending maximum
   int* bsort = order + n;                          // [n] best[order[s]]
   int* pv = bsort + n;                             // [kTrackMaxDep][n] the neighbours' permutations
-  const size_t nb = verts[blockIdx.x];
-  const int cnt = ndep[nb];
+  const TrackEntry te = sched[blockIdx.x];        // uniform: one scalar load
+  const size_t nb = te.v;
+  const int cnt = static_cast<int>(te.cnt);
   if (cnt == 0) return;  // vertex 0 keeps the identity
   const int nn = n * n;
   const size_t n2 = static_cast<size_t>(nn);
-  for (int e = threadIdx.x; e < cnt * n; e += kTrackThreads) {
-    const int r = e / n, a = e - r * n;
-    pv[e] = perm[static_cast<size_t>(dep[nb * kTrackMaxDep + r]) * n + a];
-  }
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kTrackMaxDep; ++r)
+    if (r < cnt)
+      for (int a = tid; a < n; a += kTrackThreads) pv[r * n + a] = perm[static_cast<size_t>(te.dep[r]) * n + a];
   // each neighbour's overlap block and orientation; the pair's axis from its
   // index step, x first (with ny = 1 the x and y steps coincide and only x
   // pairs exist)
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
     base[r] = nullptr;
     lower[r] = false;
     if (r < cnt) {
-      const size_t vn = dep[nb * kTrackMaxDep + r];
+      const size_t vn = te.dep[r];
       const size_t lo = nb < vn ? nb : vn, step = (nb < vn ? vn : nb) - lo;
       const double* axis = step == ny * nz ? ovx : (step == nz ? ovy : ovz);
       base[r] = axis + lo * n2;
@@ -234,7 +236,7 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   }
   __syncthreads();
   const double dc = static_cast<double>(cnt);
-  for (int e = threadIdx.x; e < nn; e += kTrackThreads) {
+  for (int e = tid; e < nn; e += kTrackThreads) {
     const int a = e / n, b = e - a * n;
     double x = 0.0;
 #pragma unroll
@@ -250,7 +252,7 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   __syncthreads();
   // each row's maximum (fmax from -1: NaN dropped) and best column, the
   // lowest among equal values (the scan is rotated to spread the LDS banks)
-  for (int i = threadIdx.x; i < n; i += kTrackThreads) {
+  for (int i = tid; i < n; i += kTrackThreads) {
     double bv = -1.0;
     int bj = 0;
     for (int q = 0; q < n; ++q) {
@@ -266,7 +268,7 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   }
   __syncthreads();
   // the stable descending sort as ranks
-  for (int i = threadIdx.x; i < n; i += kTrackThreads) {
+  for (int i = tid; i < n; i += kTrackThreads) {
     const double x = rmax[i];
     int rank = 0;
     for (int k = 0; k < n; ++k) {
@@ -277,8 +279,8 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
     bsort[rank] = best[i];
   }
   __syncthreads();
-  if (threadIdx.x >= 64) return;
-  const int lane = threadIdx.x;
+  if (tid >= 64) return;
+  const int lane = tid;
   unsigned long long tk0 = 0, tk1 = 0;  // assigned columns 0..63, 64..127 (the same in every lane)
   int* out = perm + nb * n;
   for (int s = 0; s < n; ++s) {
@@ -356,15 +358,15 @@ int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int 
 }
 
 int energy_cut_track_smem_limits() {
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&track_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&track_kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(track_smem(kTrackDevMaxN))));
   return RTB_OK;
 }
 
 int energy_cut_track_dev(hipStream_t st, size_t nx, size_t ny, size_t nz, int n, const double* ovx,
-                         const double* ovy, const double* ovz, const unsigned* by_level,
-                         const std::vector<size_t>& level_start, const unsigned char* ndep, const unsigned* dep,
-                         int* perm) {
+                         const double* ovy, const double* ovz, const TrackEntry* sched,
+                         const std::vector<size_t>& level_start, int* perm) {
   if (n < 1 || n > kTrackDevMaxN) return fail(RTB_ERR_UNSUPPORTED, "device band tracking is for n <= 128");
   const size_t nk = nx * ny * nz;
   if (nk == 0) return RTB_OK;
@@ -376,8 +378,8 @@ int energy_cut_track_dev(hipStream_t st, size_t nx, size_t ny, size_t nz, int n,
   for (size_t l = 1; l + 1 < level_start.size(); ++l) {
     const size_t a = level_start[l], b = level_start[l + 1];
     if (b == a) continue;
-    hipLaunchKernelGGL(track_kernel, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads), track_smem(n), st, ovx,
-                       ovy, ovz, ny, nz, n, by_level + a, ndep, dep, perm);
+    hipLaunchKernelGGL(track_kernel, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads), track_smem(n), st, ovx, ovy,
+                       ovz, ny, nz, n, sched + a, perm);
     RTB_HIP(hipGetLastError());
   }
   return RTB_OK;
