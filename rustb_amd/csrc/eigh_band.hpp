@@ -519,18 +519,22 @@ RTB_HD void twisted_column(int n, const T* e, const T* Ts, int ldy, int j, T twi
   const T* Dm = Ts + static_cast<size_t>(n) * ldy + j;
   T nrm2 = T(1), v = T(1);
   Y[static_cast<size_t>(r) * ldy + j] = T(1);
+  // unrolled: the factor loads do not depend on v, so several are in flight
+#pragma unroll 8
   for (int k = r - 1; k >= 0; --k) {
     v = -e[k] * recip(Dp[static_cast<size_t>(k) * ldy]) * v;
     Y[static_cast<size_t>(k) * ldy + j] = v;
     nrm2 += v * v;
   }
   v = T(1);
+#pragma unroll 8
   for (int k = r + 1; k < n; ++k) {
     v = -e[k - 1] * recip(Dm[static_cast<size_t>(k) * ldy]) * v;
     Y[static_cast<size_t>(k) * ldy + j] = v;
     nrm2 += v * v;
   }
   const T inv = T(1) / sqrt(nrm2);
+#pragma unroll 8
   for (int k = 0; k < n; ++k) Y[static_cast<size_t>(k) * ldy + j] *= inv;
 }
 
