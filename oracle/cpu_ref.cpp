@@ -1,4 +1,4 @@
-// Reference-equivalent CPU path (TEST/BASELINE INFRASTRUCTURE ONLY).
+// Reference-equivalent CPU path (test and CPU-baseline infrastructure only).
 //
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
 // this library, as the checker / CPU baseline; the product never links it.

@@ -1,5 +1,5 @@
 """Loader for oracle/build/libcpu_ref.so, the reference-equivalent CPU path
-(OpenMP over k + host LAPACK).  TEST/BASELINE INFRASTRUCTURE ONLY: used by
+(OpenMP over k + host LAPACK).  TEST AND CPU-BASELINE INFRASTRUCTURE ONLY: used by
 tests/ and by bench.py's cpu_baseline leg, never by the product."""
 from __future__ import annotations
 
