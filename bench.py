@@ -287,9 +287,7 @@ def energy_cut_bench(args):
     wall = (time.perf_counter() - t0) / args.steps
     st = _lib.last_stats()
     names = _lib.STAGES
-    # device time = the sum of the stage timers (for nsta above 128 the event
-    # span also brackets the host tracking walk between the vertex stage and
-    # the cut)
+    # device time = the sum of the stage timers
     dev_ms = sum(st["ms"][i] for i in range(len(names)) if names[i] != "total")
     g = names.index("gemm64")
     tf = st["flop"][g] / (st["ms"][g] * 1e-3) / 1e12 if st["ms"][g] > 0 else 0.0

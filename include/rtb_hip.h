@@ -327,11 +327,13 @@ void rtb_batch_destroy(rtb_batch* b);
  * of each cell's four triangles (integrate_fermi_cut_2d, response/energy_cut.rs:
  * 1231-1409) or five tetrahedra (integrate_fermi_cut_3d, response/energy_cut.rs:
  * 1650-1850), / det(lat).  mu must be sorted ascending (RTB_ERR_INVALID_RESPONSE);
- * a 1D model gives RTB_ERR_INVALID_DIMENSION.  Everything runs on the device for
- * nsta <= 128 (the tracking walk one launch per level of its breadth-first
- * schedule); above, the squared neighbour overlaps go to the host walk.  The
- * environment variable RTB_TRACK_HOST=1 selects the host walk at any nsta (the
- * same permutations: for checking). */
+ * a 1D model gives RTB_ERR_INVALID_DIMENSION.  Everything runs on the device
+ * (the tracking walk one launch per level of its breadth-first schedule;
+ * above nsta = 128 each vertex's overlap matrix is in a workspace slice of
+ * nsta^2 doubles, at most 1024 vertices of a level per launch, or
+ * RTB_TRACK_SLICES=s in the environment).  RTB_TRACK_HOST=1 selects the host
+ * walk instead (the squared overlaps copied to the host; the same
+ * permutations: for checking). */
 int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* mu, size_t n_mu,
                         double T, double eta, const double* dir, int spin_dir, double* sigma);
 

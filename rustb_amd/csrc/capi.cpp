@@ -1480,7 +1480,7 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
   const bool spinj = m->spin && spin_dir != RTB_SPIN_NONE;
   const bool rterm = m->d_rmat != nullptr;
   // resident for the whole mesh: eigenvectors U, Im K and E per vertex, and
-  // the squared neighbour overlaps (what the host reads); per chunk: H, V (2),
+  // the squared neighbour overlaps (what the walk reads); per chunk: H, V (2),
   // eigensolver scratch, T1/A/B, J
   // the neighbour pairs of each axis, indexed by the lower vertex: x (i, i +
   // ny nz), y (i, i + nz), z (i, i + 1), every i whose partner is on the mesh
@@ -1581,16 +1581,36 @@ int rtb_hall_energy_cut(const rtb_model* m, const size_t* kmesh, const double* m
       RTB_TRY(tmr.end(1));
     }
   }
-  // band tracking: on the device for n <= kTrackDevMaxN, level by level over
-  // the walk's schedule (which depends on the mesh only); above, the squared
+  // band tracking: on the device, level by level over the walk's schedule
+  // (which depends on the mesh only); with RTB_TRACK_HOST=1 the squared
   // overlaps go to the host walk.  Either way the permutations are the same.
   int* dperm;
   RTB_TRY(w.alloc(&dperm, nk * n));
   std::vector<int> perm;
   if (dev_track) {
+    // above kTrackLdsMaxN bands each vertex's overlap matrix is in a
+    // workspace slice: at most 1024 vertices of a level per launch
+    // (RTB_TRACK_SLICES=s sets another bound, s >= 1)
+    double* mwork = nullptr;
+    size_t mblocks = 0;
+    if (n > kTrackLdsMaxN) {
+      for (size_t l = 0; l + 1 < ts.level_start.size(); ++l)
+        mblocks = std::max(mblocks, ts.level_start[l + 1] - ts.level_start[l]);
+      const char* sl = std::getenv("RTB_TRACK_SLICES");
+      const long cap = sl ? std::atol(sl) : 1024;
+      if (cap < 1) return fail(RTB_ERR_INVALID_ARGUMENT, "RTB_TRACK_SLICES must be at least 1");
+      mblocks = std::min<size_t>(mblocks, static_cast<size_t>(cap));
+      RTB_TRY(w.alloc(&mwork, mblocks * n2));
+    }
     RTB_TRY(tmr.begin(RTB_STAGE_REDUCE));
-    RTB_TRY(energy_cut_track_dev(st, nx, ny, nz, n, dOv[0], dOv[1], dOv[2], sched, ts.level_start, dperm));
-    RTB_TRY(tmr.end(static_cast<int>(ts.level_start.size() - 1)));
+    RTB_TRY(energy_cut_track_dev(st, nx, ny, nz, n, dOv[0], dOv[1], dOv[2], sched, ts.level_start, mwork, mblocks,
+                                 dperm));
+    size_t launches = 1;  // the identity permutations, then each level's launches
+    for (size_t l = 1; l + 1 < ts.level_start.size(); ++l) {
+      const size_t lw = ts.level_start[l + 1] - ts.level_start[l];
+      launches += mblocks ? (lw + mblocks - 1) / mblocks : (lw ? 1 : 0);
+    }
+    RTB_TRY(tmr.end(static_cast<int>(launches)));
   } else {
     std::vector<double> hOv[3];
     for (int ax = 0; ax < 3; ++ax) {

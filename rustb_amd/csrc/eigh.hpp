@@ -253,14 +253,17 @@ size_t energy_cut_waves(int dim, size_t nx, size_t ny, size_t nz, int n);
 // entry per vertex in level order (the vertex, its count of visited
 // neighbours and those neighbours, 32-bit indices) on the device and the
 // level bounds on the host; from the squared overlaps; perm [nk][n] on the
-// device.
-constexpr int kTrackDevMaxN = 128;
+// device.  Up to kTrackLdsMaxN bands a vertex's averaged overlap matrix is
+// in LDS; above, mwork (device, [mwork_blocks][n][n]) holds it for up to
+// mwork_blocks vertices of a level at a time (null below).
+constexpr int kTrackDevMaxN = 256;
+constexpr int kTrackLdsMaxN = 128;
 struct TrackEntry {
   unsigned v, cnt, dep[6];
 };
 int energy_cut_track_dev(hipStream_t st, size_t nx, size_t ny, size_t nz, int n, const double* ovx,
                          const double* ovy, const double* ovz, const TrackEntry* sched,
-                         const std::vector<size_t>& level_start, int* perm);
+                         const std::vector<size_t>& level_start, double* mwork, size_t mwork_blocks, int* perm);
 int energy_cut_track_smem_limits();
 
 int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int n, const double* E, const double* Ki,

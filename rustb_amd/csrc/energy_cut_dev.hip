@@ -14,6 +14,8 @@
 // DPP in a fixed order into a per-wave partial, and the partials are summed
 // per mu in a fixed order (column_sums): the result does not depend on
 // scheduling.
+#include <limits>
+
 #include "eigh.hpp"
 #include "energy_cut_core.hpp"
 #include "gpu_team.hpp"
@@ -173,19 +175,25 @@ __global__ void __launch_bounds__(256) cut_kernel(cut::View V, const double* __r
 // ---- band tracking (global_band_track, tracking.rs:123-163) ----
 // One workgroup per vertex of a level: the squared overlaps of its scheduled
 // neighbours (rows in the neighbour's tracked order) summed in neighbour
-// order and divided by their count into LDS, as the host walk forms them;
+// order and divided by their count into M, as the host walk forms them;
 // then greedy_assign (tracking.rs:44-68): rows by descending row maximum
 // (stable), each taking its largest unassigned column, the lowest column
 // among equal values.  A row whose overall best column (same scan rule) is
 // still free takes it, which is what the scan over the free columns returns;
 // otherwise wave 0 scans the free columns.  The same operations on the same
-// values as the host walk, so the permutations are identical.
+// values as the host walk, so the permutations are identical.  M [n][n] sits
+// in LDS up to kTrackLdsMaxN bands and above in a global slice per
+// workgroup of the launch (n = 256: 512 KiB), which the rows' scans then
+// read with one wave per row.
 constexpr int kTrackMaxDep = 6;  // visited neighbours of a 3D vertex at most
 static_assert(sizeof(TrackEntry) == 8 * sizeof(unsigned), "schedule entry layout");
+static_assert(kTrackDevMaxN <= 256, "the greedy's assigned-column mask is four 64-bit words");
 
+// LDS bytes: M when it is in LDS, then rmax [n], best/order/bsort [n] each
+// and the neighbours' permutations [kTrackMaxDep][n]
 size_t track_smem(int n) {
-  return static_cast<size_t>(n) * n * sizeof(double) + n * sizeof(double) +
-         (3 + kTrackMaxDep) * static_cast<size_t>(n) * sizeof(int);
+  const size_t m = n <= kTrackLdsMaxN ? static_cast<size_t>(n) * n * sizeof(double) : 0;
+  return m + n * sizeof(double) + (3 + kTrackMaxDep) * static_cast<size_t>(n) * sizeof(int);
 }
 
 __global__ void identity_perm_kernel(size_t total, int n, int* __restrict__ perm) {
@@ -196,22 +204,24 @@ __global__ void identity_perm_kernel(size_t total, int n, int* __restrict__ perm
 // kTrackThreads per vertex (one wave of 64 was slower at n = 32: the matrix
 // build loses its parallelism).
 constexpr int kTrackThreads = 256;
+template <bool kLdsM>
 __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __restrict__ ovx, const double* __restrict__ ovy,
                                                    const double* __restrict__ ovz, size_t ny, size_t nz, int n,
-                                                   const TrackEntry* __restrict__ sched, int* __restrict__ perm) {
+                                                   const TrackEntry* __restrict__ sched, double* __restrict__ mwork,
+                                                   int* __restrict__ perm) {
   extern __shared__ double smem[];
-  double* M = smem;                                // [n][n]
-  double* rmax = M + static_cast<size_t>(n) * n;  // [n]
-  int* best = reinterpret_cast<int*>(rmax + n);    // [n] each row's best column over all columns
-  int* order = best + n;                           // [n] rows by descending maximum
-  int* bsort = order + n;                          // [n] best[order[s]]
-  int* pv = bsort + n;                             // [kTrackMaxDep][n] the neighbours' permutations
-  const TrackEntry te = sched[blockIdx.x];        // uniform: one scalar load
+  const int nn = n * n;
+  const size_t n2 = static_cast<size_t>(nn);
+  double* M = kLdsM ? smem : mwork + blockIdx.x * n2;  // [n][n]
+  double* rmax = kLdsM ? smem + n2 : smem;               // [n]
+  int* best = reinterpret_cast<int*>(rmax + n);          // [n] each row's best column over all columns
+  int* order = best + n;                                 // [n] rows by descending maximum
+  int* bsort = order + n;                                // [n] best[order[s]]
+  int* pv = bsort + n;                                   // [kTrackMaxDep][n] the neighbours' permutations
+  const TrackEntry te = sched[blockIdx.x];              // uniform: one scalar load
   const size_t nb = te.v;
   const int cnt = static_cast<int>(te.cnt);
   if (cnt == 0) return;  // vertex 0 keeps the identity
-  const int nn = n * n;
-  const size_t n2 = static_cast<size_t>(nn);
   const int tid = threadIdx.x;
 #pragma unroll
   for (int r = 0; r < kTrackMaxDep; ++r)
@@ -251,20 +261,52 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   }
   __syncthreads();
   // each row's maximum (fmax from -1: NaN dropped) and best column, the
-  // lowest among equal values (the scan is rotated to spread the LDS banks)
-  for (int i = tid; i < n; i += kTrackThreads) {
-    double bv = -1.0;
-    int bj = 0;
-    for (int q = 0; q < n; ++q) {
-      const int j = q + i < n ? q + i : q + i - n;
-      const double v = M[i * n + j];
-      if (v > bv || (v == bv && j < bj)) {
-        bv = v;
-        bj = j;
+  // lowest among equal values
+  if constexpr (kLdsM) {
+    // a row per thread, the scan rotated to spread the LDS banks
+    for (int i = tid; i < n; i += kTrackThreads) {
+      double bv = -1.0;
+      int bj = 0;
+      for (int q = 0; q < n; ++q) {
+        const int j = q + i < n ? q + i : q + i - n;
+        const double v = M[i * n + j];
+        if (v > bv || (v == bv && j < bj)) {
+          bv = v;
+          bj = j;
+        }
+      }
+      rmax[i] = bv;
+      best[i] = bj;
+    }
+  } else {
+    // a row per wave: each lane scans its columns in ascending order (the
+    // first of equal values stays), then the lanes' (value, column) pairs
+    // merge by larger value, lower column; a lane with no candidate keeps
+    // (-1, 0), which every candidate (>= 0) beats
+    const int wv = tid >> 6, ln = tid & 63;
+    for (int i = wv; i < n; i += kTrackThreads / 64) {
+      double bv = -1.0;
+      int bj = 0;
+      for (int j = ln; j < n; j += 64) {
+        const double v = M[static_cast<size_t>(i) * n + j];
+        if (v > bv) {
+          bv = v;
+          bj = j;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o);
+        const int oj = __shfl_xor(bj, o);
+        if (ov > bv || (ov == bv && oj < bj)) {
+          bv = ov;
+          bj = oj;
+        }
+      }
+      if (ln == 0) {
+        rmax[i] = bv;
+        best[i] = bj;
       }
     }
-    rmax[i] = bv;
-    best[i] = bj;
   }
   __syncthreads();
   // the stable descending sort as ranks
@@ -281,24 +323,30 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   __syncthreads();
   if (tid >= 64) return;
   const int lane = tid;
-  unsigned long long tk0 = 0, tk1 = 0;  // assigned columns 0..63, 64..127 (the same in every lane)
+  // assigned columns 64 c .. 64 c + 63 in word c (the same in every lane);
+  // four named words rather than an array indexed at run time
+  unsigned long long tk0 = 0, tk1 = 0, tk2 = 0, tk3 = 0;
+  const auto word = [&](int c) { return c == 0 ? tk0 : (c == 1 ? tk1 : (c == 2 ? tk2 : tk3)); };
   int* out = perm + nb * n;
   for (int s = 0; s < n; ++s) {
     const int r = order[s];
     int j = bsort[s];
-    const bool busy = j < 64 ? ((tk0 >> j) & 1ull) : ((tk1 >> (j - 64)) & 1ull);
+    const bool busy = (word(j >> 6) >> (j & 63)) & 1ull;
     if (busy) {  // the same in every lane: the scan over the free columns
       // no candidate: (-1, 0), as the host scan's start; every real
       // candidate is >= 0 and NaN never wins
       double bv = -1.0;
       int bj = 0;
-      for (int c = 0; lane + 64 * c < n; ++c) {
+#pragma unroll
+      for (int c = 0; c < kTrackDevMaxN / 64; ++c) {
         const int jj = lane + 64 * c;
-        const bool t = c == 0 ? ((tk0 >> lane) & 1ull) : ((tk1 >> lane) & 1ull);
-        const double v = M[r * n + jj];
-        if (!t && v > bv) {
-          bv = v;
-          bj = jj;
+        if (jj < n) {
+          const bool t = (word(c) >> lane) & 1ull;
+          const double v = M[static_cast<size_t>(r) * n + jj];
+          if (!t && v > bv) {
+            bv = v;
+            bj = jj;
+          }
         }
       }
       for (int o = 32; o > 0; o >>= 1) {
@@ -311,10 +359,13 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
       }
       j = bj;
     }
-    if (j < 64)
-      tk0 |= 1ull << j;
-    else
-      tk1 |= 1ull << (j - 64);
+    const unsigned long long bit = 1ull << (j & 63);
+    switch (j >> 6) {
+      case 0: tk0 |= bit; break;
+      case 1: tk1 |= bit; break;
+      case 2: tk2 |= bit; break;
+      default: tk3 |= bit; break;
+    }
     if (lane == 0) out[r] = j;
   }
 }
@@ -358,29 +409,40 @@ int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int 
 }
 
 int energy_cut_track_smem_limits() {
-  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&track_kernel),
+  RTB_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&track_kernel<true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(track_smem(kTrackDevMaxN))));
+                              static_cast<int>(track_smem(kTrackLdsMaxN))));
   return RTB_OK;
 }
 
 int energy_cut_track_dev(hipStream_t st, size_t nx, size_t ny, size_t nz, int n, const double* ovx,
                          const double* ovy, const double* ovz, const TrackEntry* sched,
-                         const std::vector<size_t>& level_start, int* perm) {
-  if (n < 1 || n > kTrackDevMaxN) return fail(RTB_ERR_UNSUPPORTED, "device band tracking is for n <= 128");
+                         const std::vector<size_t>& level_start, double* mwork, size_t mwork_blocks, int* perm) {
+  if (n < 1 || n > kTrackDevMaxN) return fail(RTB_ERR_UNSUPPORTED, "device band tracking is for n <= 256");
+  const bool lds = n <= kTrackLdsMaxN;
+  if (!lds && (!mwork || mwork_blocks == 0))
+    return fail(RTB_ERR_INVALID_ARGUMENT, "device band tracking above 128 bands needs its matrix workspace");
   const size_t nk = nx * ny * nz;
   if (nk == 0) return RTB_OK;
   const size_t total = nk * static_cast<size_t>(n);
   hipLaunchKernelGGL(identity_perm_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, total,
                      n, perm);
   RTB_HIP(hipGetLastError());
-  // level 0 is vertex 0 alone (no neighbours visited before it)
+  // level 0 is vertex 0 alone (no neighbours visited before it); above
+  // kTrackLdsMaxN a level goes in launches of at most mwork_blocks vertices,
+  // one workspace slice each
+  const size_t step = lds ? std::numeric_limits<unsigned>::max() : mwork_blocks;
   for (size_t l = 1; l + 1 < level_start.size(); ++l) {
-    const size_t a = level_start[l], b = level_start[l + 1];
-    if (b == a) continue;
-    hipLaunchKernelGGL(track_kernel, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads), track_smem(n), st, ovx, ovy,
-                       ovz, ny, nz, n, sched + a, perm);
-    RTB_HIP(hipGetLastError());
+    for (size_t a = level_start[l]; a < level_start[l + 1]; a += step) {
+      const size_t b = std::min(level_start[l + 1], a + step);
+      if (lds)
+        hipLaunchKernelGGL(track_kernel<true>, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads), track_smem(n),
+                           st, ovx, ovy, ovz, ny, nz, n, sched + a, nullptr, perm);
+      else
+        hipLaunchKernelGGL(track_kernel<false>, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads),
+                           track_smem(n), st, ovx, ovy, ovz, ny, nz, n, sched + a, mwork, perm);
+      RTB_HIP(hipGetLastError());
+    }
   }
   return RTB_OK;
 }

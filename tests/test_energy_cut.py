@@ -591,12 +591,21 @@ def test_gpu_3d_wide_model_vs_host_part():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("seed,n,mesh", [(7, 32, (16, 12)), (8, 24, (5, 4, 4)), (9, 100, (6, 5)), (10, 128, (4, 3))])
-def test_gpu_device_walk_equals_host_walk(seed, n, mesh, monkeypatch):
-    """Band tracking on the device (n <= 128) against the host walk
-    (RTB_TRACK_HOST=1) on the same device vertex data: identical sigma, bit
-    for bit.  On these random models the walk reorders the bands of nearly
-    every vertex (counted with the oracle's global_band_track)."""
+@pytest.mark.parametrize("seed,n,mesh,slices", [(7, 32, (16, 12), None), (8, 24, (5, 4, 4), None),
+                                                (9, 100, (6, 5), None), (10, 128, (4, 3), None),
+                                                (16, 136, (4, 3), None), (17, 256, (3, 3), None),
+                                                (18, 160, (3, 2, 2), None), (19, 144, (5, 4), "2")])
+def test_gpu_device_walk_equals_host_walk(seed, n, mesh, slices, monkeypatch):
+    """Band tracking on the device against the host walk (RTB_TRACK_HOST=1)
+    on the same device vertex data: identical sigma, bit for bit.  Up to 128
+    bands the overlap matrix is in LDS, above in a workspace slice per vertex
+    (RTB_TRACK_SLICES=2: levels in launches of two vertices).  On these
+    random models the walk reorders the bands of nearly every vertex (counted
+    with the oracle's global_band_track)."""
+    if slices is None:
+        monkeypatch.delenv("RTB_TRACK_SLICES", raising=False)
+    else:
+        monkeypatch.setenv("RTB_TRACK_SLICES", slices)
     dim = len(mesh)
     pts = list(EC.vertices(wide_model(O.Model, seed, n, dim), mesh, np.eye(dim)[:2]))
     sorted_bands = [q.band.copy() for q in pts]
@@ -642,9 +651,9 @@ def test_gpu_device_walk_ragged_meshes(seed, mesh, monkeypatch):
 
 
 @pytest.mark.gpu
-def test_gpu_n136_two_stage_and_host_walk():
-    """n = 136: the vertex stage takes the two-stage FP64 eigensolver and,
-    above 128 bands, the squared overlaps go to the host walk; against the
+def test_gpu_n136_two_stage_vs_host_part():
+    """n = 136: the vertex stage takes the two-stage FP64 eigensolver and the
+    walk keeps each vertex's overlap matrix in a workspace slice; against the
     host part fed the oracle's vertex data.  Tolerance 1e-8 of max|sigma|:
     the two-stage reduction rounds more than the n <= 128 route, and the
     smallest energy gap inside a triangle is 1.1e-4 here."""
