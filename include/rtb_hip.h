@@ -20,7 +20,8 @@
  *     repeated calls allocate nothing.  Concurrent large calls each hold
  *     their own block: an allocation the device cannot satisfy is
  *     RTB_ERR_DEVICE.  Host allocation is per call and proportional to the
- *     outputs it stages (EnergyCut: its vertex data).
+ *     outputs it stages (EnergyCut: the tracking walk's schedule, 32 bytes
+ *     per k-point; its vertex data only under RTB_TRACK_HOST=1).
  */
 #ifndef RTB_HIP_H
 #define RTB_HIP_H
