@@ -183,17 +183,22 @@ __global__ void __launch_bounds__(256) cut_kernel(cut::View V, const double* __r
 // otherwise wave 0 scans the free columns.  The same operations on the same
 // values as the host walk, so the permutations are identical.  M [n][n] sits
 // in LDS up to kTrackLdsMaxN bands and above in a global slice per
-// workgroup of the launch (n = 256: 512 KiB), which the rows' scans then
-// read with one wave per row.
+// workgroup of the launch (n = 256: 512 KiB); there a wave builds each row
+// and ranks its first kTrackTop columns in registers, a row whose best
+// column is taken takes the first free one of them, and only when all are
+// taken does wave 0 scan the row in global memory.
 constexpr int kTrackMaxDep = 6;  // visited neighbours of a 3D vertex at most
 static_assert(sizeof(TrackEntry) == 8 * sizeof(unsigned), "schedule entry layout");
 static_assert(kTrackDevMaxN <= 256, "the greedy's assigned-column mask is four 64-bit words");
 
-// LDS bytes: M when it is in LDS, then rmax [n], best/order/bsort [n] each
-// and the neighbours' permutations [kTrackMaxDep][n]
+// LDS bytes: M when it is in LDS, then rmax [n], best/order/bsort [n] each,
+// the neighbours' permutations [kTrackMaxDep][n] and, with M in global
+// memory, each row's first kTrackTop columns [n][kTrackTop]
+constexpr int kTrackTop = 4;
 size_t track_smem(int n) {
-  const size_t m = n <= kTrackLdsMaxN ? static_cast<size_t>(n) * n * sizeof(double) : 0;
-  return m + n * sizeof(double) + (3 + kTrackMaxDep) * static_cast<size_t>(n) * sizeof(int);
+  const bool lds = n <= kTrackLdsMaxN;
+  const size_t m = lds ? static_cast<size_t>(n) * n * sizeof(double) : 0;
+  return m + n * sizeof(double) + (3 + kTrackMaxDep + (lds ? 0 : kTrackTop)) * static_cast<size_t>(n) * sizeof(int);
 }
 
 __global__ void identity_perm_kernel(size_t total, int n, int* __restrict__ perm) {
@@ -201,14 +206,25 @@ __global__ void identity_perm_kernel(size_t total, int n, int* __restrict__ perm
   if (g < total) perm[g] = static_cast<int>(g % n);
 }
 
-// kTrackThreads per vertex (one wave of 64 was slower at n = 32: the matrix
-// build loses its parallelism).
+// The overlap sum over the neighbours divided by their count: x / 1, x * 0.5
+// and x * 0.25 are the correctly rounded x / c for c = 1, 2, 4 (exact
+// scalings), so only 3, 5 and 6 take the division.
+__device__ inline double by_count(double x, int cnt, double dc) {
+  return cnt == 1 ? x : (cnt == 2 ? x * 0.5 : (cnt == 4 ? x * 0.25 : x / dc));
+}
+
+// kTrackThreads per vertex with M in LDS (one wave of 64 was slower at
+// n = 32: the matrix build loses its parallelism); with M in global memory
+// kTrackThreadsWide, one row per wave in flight: the build waits on global
+// loads, and a level holds few vertices.
 constexpr int kTrackThreads = 256;
+constexpr int kTrackThreadsWide = 1024;
 template <bool kLdsM>
-__global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __restrict__ ovx, const double* __restrict__ ovy,
+__global__ void __launch_bounds__(kLdsM ? kTrackThreads : kTrackThreadsWide) track_kernel(const double* __restrict__ ovx, const double* __restrict__ ovy,
                                                    const double* __restrict__ ovz, size_t ny, size_t nz, int n,
                                                    const TrackEntry* __restrict__ sched, double* __restrict__ mwork,
                                                    int* __restrict__ perm) {
+  constexpr int NT = kLdsM ? kTrackThreads : kTrackThreadsWide;
   extern __shared__ double smem[];
   const int nn = n * n;
   const size_t n2 = static_cast<size_t>(nn);
@@ -218,6 +234,7 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   int* order = best + n;                                 // [n] rows by descending maximum
   int* bsort = order + n;                                // [n] best[order[s]]
   int* pv = bsort + n;                                   // [kTrackMaxDep][n] the neighbours' permutations
+  int* top = pv + kTrackMaxDep * n;                      // [n][kTrackTop] (M in global memory only)
   const TrackEntry te = sched[blockIdx.x];              // uniform: one scalar load
   const size_t nb = te.v;
   const int cnt = static_cast<int>(te.cnt);
@@ -226,7 +243,7 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
 #pragma unroll
   for (int r = 0; r < kTrackMaxDep; ++r)
     if (r < cnt)
-      for (int a = tid; a < n; a += kTrackThreads) pv[r * n + a] = perm[static_cast<size_t>(te.dep[r]) * n + a];
+      for (int a = tid; a < n; a += NT) pv[r * n + a] = perm[static_cast<size_t>(te.dep[r]) * n + a];
   // each neighbour's overlap block and orientation; the pair's axis from its
   // index step, x first (with ny = 1 the x and y steps coincide and only x
   // pairs exist)
@@ -246,8 +263,8 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   }
   __syncthreads();
   const double dc = static_cast<double>(cnt);
-  for (int e = tid; e < nn; e += kTrackThreads) {
-    const int a = e / n, b = e - a * n;
+  // entry (a, b) of M: the neighbours' values in neighbour order
+  const auto entry = [&](int a, int b) {
     double x = 0.0;
 #pragma unroll
     for (int r = 0; r < kTrackMaxDep; ++r) {
@@ -257,14 +274,18 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
         x = r == 0 ? y : x + y;
       }
     }
-    M[e] = x / dc;
-  }
-  __syncthreads();
-  // each row's maximum (fmax from -1: NaN dropped) and best column, the
-  // lowest among equal values
+    return by_count(x, cnt, dc);
+  };
   if constexpr (kLdsM) {
-    // a row per thread, the scan rotated to spread the LDS banks
-    for (int i = tid; i < n; i += kTrackThreads) {
+    for (int e = tid; e < nn; e += NT) {
+      const int a = e / n;
+      M[e] = entry(a, e - a * n);
+    }
+    __syncthreads();
+    // each row's maximum (fmax from -1: NaN dropped) and best column, the
+    // lowest among equal values: a row per thread, the scan rotated to
+    // spread the LDS banks
+    for (int i = tid; i < n; i += NT) {
       double bv = -1.0;
       int bj = 0;
       for (int q = 0; q < n; ++q) {
@@ -279,38 +300,65 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
       best[i] = bj;
     }
   } else {
-    // a row per wave: each lane scans its columns in ascending order (the
-    // first of equal values stays), then the lanes' (value, column) pairs
-    // merge by larger value, lower column; a lane with no candidate keeps
-    // (-1, 0), which every candidate (>= 0) beats
+    // a row per wave, each lane its columns b = lane + 64 c: the row is
+    // built, stored, and ranked from registers.  The columns in the order
+    // of the host scan (larger value first, then lower column; NaN never
+    // ranked) are found kTrackTop times over, each round among the columns
+    // after the previous round's; the lanes' (value, column) pairs merge by
+    // that order, and a lane or row with no candidate gives (-1, 0), which
+    // every candidate (>= 0) beats.  Round 0 is the row's maximum and best
+    // column as the host scan gives them.
     const int wv = tid >> 6, ln = tid & 63;
-    for (int i = wv; i < n; i += kTrackThreads / 64) {
-      double bv = -1.0;
-      int bj = 0;
-      for (int j = ln; j < n; j += 64) {
-        const double v = M[static_cast<size_t>(i) * n + j];
-        if (v > bv) {
-          bv = v;
-          bj = j;
+    for (int a = wv; a < n; a += NT / 64) {
+      double xv[kTrackDevMaxN / 64];
+#pragma unroll
+      for (int c = 0; c < kTrackDevMaxN / 64; ++c) {
+        const int b = ln + 64 * c;
+        xv[c] = __builtin_nan("");
+        if (b < n) {
+          xv[c] = entry(a, b);
+          M[static_cast<size_t>(a) * n + b] = xv[c];
         }
       }
-      for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(bv, o);
-        const int oj = __shfl_xor(bj, o);
-        if (ov > bv || (ov == bv && oj < bj)) {
-          bv = ov;
-          bj = oj;
+      double pvl = 0.0;
+      int pcl = -1;
+#pragma unroll
+      for (int k = 0; k < kTrackTop; ++k) {
+        double bv = -1.0;
+        int bj = 0;
+#pragma unroll
+        for (int c = 0; c < kTrackDevMaxN / 64; ++c) {
+          const double x = xv[c];
+          const int b = ln + 64 * c;
+          const bool after = k == 0 || x < pvl || (x == pvl && b > pcl);
+          if (after && x > bv) {
+            bv = x;
+            bj = b;
+          }
         }
-      }
-      if (ln == 0) {
-        rmax[i] = bv;
-        best[i] = bj;
+        for (int o = 32; o > 0; o >>= 1) {
+          const double ov = __shfl_xor(bv, o);
+          const int oj = __shfl_xor(bj, o);
+          if (ov > bv || (ov == bv && oj < bj)) {
+            bv = ov;
+            bj = oj;
+          }
+        }
+        if (ln == 0) {
+          if (k == 0) {
+            rmax[a] = bv;
+            best[a] = bj;
+          }
+          top[a * kTrackTop + k] = bv >= 0.0 ? bj : -1;  // -1: no further candidate
+        }
+        pvl = bv;
+        pcl = bj;
       }
     }
   }
   __syncthreads();
   // the stable descending sort as ranks
-  for (int i = tid; i < n; i += kTrackThreads) {
+  for (int i = tid; i < n; i += NT) {
     const double x = rmax[i];
     int rank = 0;
     for (int k = 0; k < n; ++k) {
@@ -324,21 +372,48 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
   if (tid >= 64) return;
   const int lane = tid;
   // assigned columns 64 c .. 64 c + 63 in word c (the same in every lane);
-  // four named words rather than an array indexed at run time
+  // NW named words rather than an array indexed at run time
+  constexpr int NW = kLdsM ? kTrackLdsMaxN / 64 : kTrackDevMaxN / 64;
+  static_assert(NW == 2 || NW == 4, "two or four assigned-column words");
   unsigned long long tk0 = 0, tk1 = 0, tk2 = 0, tk3 = 0;
-  const auto word = [&](int c) { return c == 0 ? tk0 : (c == 1 ? tk1 : (c == 2 ? tk2 : tk3)); };
+  const auto word = [&](int c) {
+    if constexpr (NW == 2) return c == 0 ? tk0 : tk1;
+    return c == 0 ? tk0 : (c == 1 ? tk1 : (c == 2 ? tk2 : tk3));
+  };
+  const auto taken = [&](int j) { return ((word(j >> 6) >> (j & 63)) & 1ull) != 0; };
   int* out = perm + nb * n;
+  int r_next = order[0], j_next = bsort[0];  // read one step ahead
   for (int s = 0; s < n; ++s) {
-    const int r = order[s];
-    int j = bsort[s];
-    const bool busy = (word(j >> 6) >> (j & 63)) & 1ull;
-    if (busy) {  // the same in every lane: the scan over the free columns
+    const int r = r_next;
+    int j = j_next;
+    if (s + 1 < n) {
+      r_next = order[s + 1];
+      j_next = bsort[s + 1];
+    }
+    bool scan = taken(j);  // the same in every lane
+    if constexpr (!kLdsM) {
+      // the first free column of the row's ranked list (entry 0 is j) is the
+      // free columns' maximum; the full scan only when the list runs out
+      if (scan) {
+#pragma unroll
+        for (int k = 1; k < kTrackTop; ++k) {
+          const int c = top[r * kTrackTop + k];
+          if (c < 0) break;
+          if (!taken(c)) {
+            j = c;
+            scan = false;
+            break;
+          }
+        }
+      }
+    }
+    if (scan) {  // the scan over the free columns
       // no candidate: (-1, 0), as the host scan's start; every real
       // candidate is >= 0 and NaN never wins
       double bv = -1.0;
       int bj = 0;
 #pragma unroll
-      for (int c = 0; c < kTrackDevMaxN / 64; ++c) {
+      for (int c = 0; c < NW; ++c) {
         const int jj = lane + 64 * c;
         if (jj < n) {
           const bool t = (word(c) >> lane) & 1ull;
@@ -360,11 +435,18 @@ __global__ void __launch_bounds__(kTrackThreads) track_kernel(const double* __re
       j = bj;
     }
     const unsigned long long bit = 1ull << (j & 63);
-    switch (j >> 6) {
-      case 0: tk0 |= bit; break;
-      case 1: tk1 |= bit; break;
-      case 2: tk2 |= bit; break;
-      default: tk3 |= bit; break;
+    if constexpr (NW == 2) {
+      if (j < 64)
+        tk0 |= bit;
+      else
+        tk1 |= bit;
+    } else {
+      switch (j >> 6) {
+        case 0: tk0 |= bit; break;
+        case 1: tk1 |= bit; break;
+        case 2: tk2 |= bit; break;
+        default: tk3 |= bit; break;
+      }
     }
     if (lane == 0) out[r] = j;
   }
@@ -439,7 +521,7 @@ int energy_cut_track_dev(hipStream_t st, size_t nx, size_t ny, size_t nz, int n,
         hipLaunchKernelGGL(track_kernel<true>, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads), track_smem(n),
                            st, ovx, ovy, ovz, ny, nz, n, sched + a, nullptr, perm);
       else
-        hipLaunchKernelGGL(track_kernel<false>, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreads),
+        hipLaunchKernelGGL(track_kernel<false>, dim3(static_cast<unsigned>(b - a)), dim3(kTrackThreadsWide),
                            track_smem(n), st, ovx, ovy, ovz, ny, nz, n, sched + a, mwork, perm);
       RTB_HIP(hipGetLastError());
     }
