@@ -67,38 +67,61 @@ __device__ inline void load_simplex(const cut::View& V, const double* __restrict
 }
 
 // Lane t's share of the band sum of eval_berry_band_at_lam_buf
-// (kernel.rs:138-182): the terms m = t, t + G, ...; the G lanes' shares add up
-// to the sum.
-struct LaneBandSum {
-  cut::View M;
-  const size_t* vs;
-  int b;
-  double eta;
-  int t, G;
-  template <int NV>
+// (kernel.rs:138-182): the terms m = t, t + G, ..; the G lanes' shares add up
+// to the sum.  The lane's KM terms (E and Im K behind the tracking
+// permutation) are gathered once per cut item, and every quadrature point of
+// the item is evaluated from registers.
+template <int NV, int KM>
+struct CachedLaneBandSum {
+  double eb[NV];
+  double em[KM][NV], km[KM][NV];
+  bool use[KM];
+  double eta2;
+  __device__ CachedLaneBandSum(const cut::View& M, const size_t* vs, int b, double eta, int t, int G) {
+    eta2 = eta * eta;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) eb[v] = cut::band(M, vs[v], b);
+#pragma unroll
+    for (int u = 0; u < KM; ++u) {
+      const int m = t + G * u;
+      use[u] = m < M.n && m != b;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        em[u][v] = use[u] ? cut::band(M, vs[v], m) : 0.0;
+        km[u][v] = use[u] ? cut::kim(M, vs[v], b, m) : 0.0;
+      }
+    }
+  }
+  template <int NVV>
   __device__ double at(const double* lam) const {
+    static_assert(NVV == NV, "vertex count");
     double en = 0.0;
+#pragma unroll
     for (int v = 0; v < NV; ++v)
-      if (lam[v] != 0.0) en += cut::band(M, vs[v], b) * lam[v];
-    const double eta2 = eta * eta;
+      if (lam[v] != 0.0) en += eb[v] * lam[v];
     double im = 0.0;
-    for (int m = t; m < M.n; m += G) {
-      if (m == b) continue;
-      double em = 0.0, km = 0.0;
+#pragma unroll
+    for (int u = 0; u < KM; ++u) {
+      if (!use[u]) continue;
+      double e = 0.0, k = 0.0;
+#pragma unroll
       for (int v = 0; v < NV; ++v)
-        if (lam[v] != 0.0) em += cut::band(M, vs[v], m) * lam[v];
+        if (lam[v] != 0.0) e += em[u][v] * lam[v];
+#pragma unroll
       for (int v = 0; v < NV; ++v)
-        if (lam[v] != 0.0) km += cut::kim(M, vs[v], b, m) * lam[v];
-      const double de = en - em;
+        if (lam[v] != 0.0) k += km[u][v] * lam[v];
+      const double de = en - e;
       const double denom = de * de + eta2;
       if (denom < 1e-30) continue;
-      im += km / denom;
+      im += k / denom;
     }
     return -2.0 * im;
   }
 };
 
-template <int DIM>
+// KM: terms per lane, ceil(n / G) with G = 32 lanes per item for n <= 32,
+// 64 above.
+template <int DIM, int KM>
 __global__ void __launch_bounds__(256) cut_kernel(cut::View V, const double* __restrict__ om, size_t nx, size_t ny,
                                                   size_t nz, const double* __restrict__ mu, int n_mu, double eta,
                                                   double* __restrict__ partial) {
@@ -153,7 +176,7 @@ __global__ void __launch_bounds__(256) cut_kernel(cut::View V, const double* __r
       if (src >= 0) {
         Simplex<DIM> y;
         load_simplex<DIM>(V, om, g0 + src, nx, ny, nz, y);
-        const LaneBandSum f{V, y.vs, y.b, eta, t, G};
+        const CachedLaneBandSum<NV, KM> f(V, y.vs, y.b, eta, t, G);
         const double one = t == 0 ? 1.0 : 0.0;
         if constexpr (DIM == 2)
           r = cut::hybrid_tri_by(y.c, y.e_v, y.om_v, m, f, one);
@@ -470,6 +493,34 @@ size_t energy_cut_waves(int dim, size_t nx, size_t ny, size_t nz, int n) {
   return (total + 255) / 256 * 4;
 }
 
+namespace {
+
+template <int DIM>
+int launch_cut_dim(hipStream_t st, int n, unsigned blocks, const cut::View& V, const double* om, size_t nx, size_t ny,
+                   size_t nz, const double* mu, int c, double eta, double* partial) {
+  // the kernel's lanes per item: 32 for n <= 32, else 64; KM = ceil(n / G)
+  const int G = n <= 32 ? 32 : 64;
+  const int km = (n + G - 1) / G;
+  if (km <= 1)
+    hipLaunchKernelGGL((cut_kernel<DIM, 1>), dim3(blocks), dim3(256), 0, st, V, om, nx, ny, nz, mu, c, eta, partial);
+  else if (km == 2)
+    hipLaunchKernelGGL((cut_kernel<DIM, 2>), dim3(blocks), dim3(256), 0, st, V, om, nx, ny, nz, mu, c, eta, partial);
+  else if (km <= 4)
+    hipLaunchKernelGGL((cut_kernel<DIM, 4>), dim3(blocks), dim3(256), 0, st, V, om, nx, ny, nz, mu, c, eta, partial);
+  else
+    return fail(RTB_ERR_UNSUPPORTED, "the Fermi cut is for n <= 256");
+  RTB_HIP(hipGetLastError());
+  return RTB_OK;
+}
+
+int launch_cut(hipStream_t st, int dim, int n, unsigned blocks, const cut::View& V, const double* om, size_t nx,
+               size_t ny, size_t nz, const double* mu, int c, double eta, double* partial) {
+  if (dim == 2) return launch_cut_dim<2>(st, n, blocks, V, om, nx, ny, nz, mu, c, eta, partial);
+  return launch_cut_dim<3>(st, n, blocks, V, om, nx, ny, nz, mu, c, eta, partial);
+}
+
+}  // namespace
+
 int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int n, const double* E, const double* Ki,
                   const int* perm, const double* om, const double* mu, size_t n_mu, double eta, double* partial,
                   size_t chunk, double* out) {
@@ -480,11 +531,7 @@ int energy_cut_t0(hipStream_t st, int dim, size_t nx, size_t ny, size_t nz, int 
   const cut::View V{n, E, Ki, 1, perm};
   for (size_t m0 = 0; m0 < n_mu; m0 += chunk) {
     const int c = static_cast<int>(std::min(chunk, n_mu - m0));
-    if (dim == 2)
-      hipLaunchKernelGGL(cut_kernel<2>, dim3(blocks), dim3(256), 0, st, V, om, nx, ny, nz, mu + m0, c, eta, partial);
-    else
-      hipLaunchKernelGGL(cut_kernel<3>, dim3(blocks), dim3(256), 0, st, V, om, nx, ny, nz, mu + m0, c, eta, partial);
-    RTB_HIP(hipGetLastError());
+    RTB_TRY(launch_cut(st, dim, n, blocks, V, om, nx, ny, nz, mu + m0, c, eta, partial));
     RTB_TRY(column_sums(st, partial, waves, c, out + m0));
   }
   return RTB_OK;
