@@ -236,12 +236,17 @@ zgemm3m_kernel(int n, size_t nmat, TileMap tm, const SA* __restrict__ A, size_t 
       for (int b = 0; b < 2; ++b) acc[p][a][b] = v4{T(0), T(0), T(0), T(0)};
 
   const int fr = lane & 15, fk = lane >> 4;
+  // a wave whose 32 x 32 quadrant lies outside the matrix (small n, edge
+  // tiles) stages its share of the tiles but issues no MFMAs: its outputs
+  // are never stored, and the matrix cores of its SIMD serve other waves
+  const bool quad_live = __builtin_amdgcn_readfirstlane(row0 + wr < n && col0 + wc < n ? 1 : 0) != 0;
   load_chunk(0);
   for (int k0 = 0; k0 < n; k0 += kKc) {
     __syncthreads();  // previous chunk fully consumed
     store_chunk();
     __syncthreads();
     if (k0 + kKc < n) load_chunk(k0 + kKc);  // in flight during the MFMAs
+    if (!quad_live) continue;
     // the fragments of k-step s + 1 are read from LDS while the 12 MFMAs of
     // k-step s issue (two fragment register sets)
     T av[2][3][2], bv[2][3][2];
@@ -438,12 +443,15 @@ zgemm3m_dual_kernel(int n, size_t nmat, TileMap tm, const float2* __restrict__ Z
         ach[p][a][b] = v4{0.f, 0.f, 0.f, 0.f};
       }
   const int fr = lane & 15, fk = lane >> 4;
+  // quadrants outside the matrix issue no MFMAs (as in zgemm3m_kernel)
+  const bool quad_live = __builtin_amdgcn_readfirstlane(row0 + wr < n && col0 + wc < n ? 1 : 0) != 0;
   load_chunk(0);
   for (int k0 = 0; k0 < n; k0 += kKc) {
     __syncthreads();
     store_chunk();
     __syncthreads();
     if (k0 + kKc < n) load_chunk(k0 + kKc);
+    if (!quad_live) continue;
     // fragments of k-step s + 1 read from LDS while the 24 MFMAs of k-step s
     // issue (two register sets)
     float az[2][3][2], ah[2][3][2], bv[2][3][2];
