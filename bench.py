@@ -259,11 +259,17 @@ def ingest_bench(args):
     print(json.dumps(out))
 
 
-def energy_cut_workload():
-    """A random 2D model, 32 orbitals spinless (n = 32), |R_i| <= 3 (49 R),
-    on a 128 x 128 mesh; 41 chemical potentials in [-2, 2] eV, eta = 1e-3, T = 0."""
+def energy_cut_workload(name="random2d-n32"):
+    """random2d-n32: a random 2D model, 32 orbitals spinless (n = 32), |R_i|
+    <= 3 (49 R), on a 128 x 128 mesh; random2d-n256: 256 orbitals spinless,
+    |R_i| <= 2 (25 R), on a 32 x 32 mesh (the two-stage eigensolver and the
+    band tracking with its matrices in global memory).  Both: 41 chemical
+    potentials in [-2, 2] eV, eta = 1e-3, T = 0."""
     from rustb_amd.model import synth_model
 
+    if name == "random2d-n256":
+        model = synth_model(3, 2, False, 256, 2, 7, scale=1.0, decay=1.0, n_R_target=25)
+        return model, (32, 32), np.linspace(-2.0, 2.0, 41), 1e-3
     model = synth_model(3, 2, False, 32, 3, 7, scale=1.0, decay=1.0, n_R_target=49)
     return model, (128, 128), np.linspace(-2.0, 2.0, 41), 1e-3
 
@@ -275,7 +281,7 @@ def energy_cut_bench(args):
     import rustb_amd as ra
     from rustb_amd import _lib
 
-    model, mesh, mu, eta = energy_cut_workload()
+    model, mesh, mu, eta = energy_cut_workload(args.ec_workload)
     nk = mesh[0] * mesh[1]
     p = ra.Parameters.rank2(mesh, [1.0, 0.0], [0.0, 1.0], mu, eta=eta, integration="EnergyCut")
     for _ in range(args.warmup):
@@ -296,7 +302,7 @@ def energy_cut_bench(args):
         "value": round(nk / wall, 2), "unit": "k-points/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 2), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "random2d-n32-energycut", "nsta": model.nsta, "n_R": int(model.hamR.shape[0]),
+        "config": {"workload": args.ec_workload + "-energycut", "nsta": model.nsta, "n_R": int(model.hamR.shape[0]),
                    "kmesh": list(mesh), "n_mu": int(mu.size), "eta": eta, "T": 0.0},
         "device_ms": round(dev_ms, 2), "host_ms": round(wall * 1e3 - dev_ms, 2),
         "stage_ms_per_step": {names[i]: round(st["ms"][i], 2) for i in range(len(names))},
@@ -305,7 +311,7 @@ def energy_cut_bench(args):
                      "frac": round(tf / 78.6, 4), "traffic": None, "kernel": "zgemm3m_kernel<double, ...>",
                      "stage": "gemm64", "note": "the overlap and Kubo products; host_ms = the walk schedule, launches and synchronisations"},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and args.ec_workload == "random2d-n32":
         # the oracle (pure-Python loops, one core) on an 8 x 8 mesh of the same model
         from oracle import energy_cut_ref as EC
         from oracle import oracle as O
@@ -339,6 +345,9 @@ def main():
     ap.add_argument("--ingest-dir", default="/tmp/rtb_ingest")
     ap.add_argument("--energy-cut", action="store_true",
                     help="measure the EnergyCut Hall conductivity of a 2D model instead of the k-mesh path")
+    ap.add_argument("--ec-workload", default="random2d-n32", choices=["random2d-n32", "random2d-n256"],
+                    help="the EnergyCut workload (with --energy-cut); random2d-n256 has no CPU leg (the "
+                         "oracle's pure-Python loops take minutes on any mesh of that model)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 3 device copies of 1 GiB (counter calibration) and exit")
     args = ap.parse_args()
