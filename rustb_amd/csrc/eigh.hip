@@ -128,7 +128,11 @@ void backtrans_reg_dispatch(hipStream_t st, int n, size_t nmat, const T* Y, cons
 
 // ---- two-stage route (128 < n <= 256), see eigh_band.hpp ----
 constexpr int kBandReduceThreads = 256;
-constexpr int kBandChaseThreads = 64;
+// FP64 stage 2 team: 16 groups of 16, the right update of a step's 3b rows
+// in three passes (each a dependent global round trip).  64 threads (12
+// passes) took 29 ms per 1024 matrices at n = 256, 256 threads 23.7 ms; 128
+// and 768 were slower (barriers over more waves, fewer matrices resident).
+constexpr int kBandChaseThreads = 256;
 
 template <typename T>
 size_t band_reduce_smem(int n) {
@@ -161,7 +165,7 @@ __global__ void __launch_bounds__(kBandReduceThreads) band_reduce_kernel(int n, 
   band_reduce<T>(tm, n, H + mat * nn * 2, A + mat * nn, bm + L.v1, bm + L.t1, sm);
 }
 
-// Band -> tridiagonal, one wave per matrix.
+// Band -> tridiagonal, one workgroup per matrix (band in global memory).
 template <typename T>
 __global__ void __launch_bounds__(kBandChaseThreads) band_chase_kernel(int n, cplx<T>* __restrict__ A,
                                                                       T* __restrict__ Dg, T* __restrict__ Eg,

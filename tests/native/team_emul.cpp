@@ -78,7 +78,7 @@ int run_reg(int n, const double* H, double* E, double* Z, int vectors, int rows_
 }
 
 // Two-stage route (eigh_band.hpp) with the team sizes of the GPU dispatch:
-// band_reduce (256 threads), band_chase (64; FP32: band_chase_lds, 768), the
+// band_reduce (256 threads), band_chase (256; FP32: band_chase_lds, 768), the
 // tridiagonal eigenpairs,
 // twisted_column, band_backtrans (W * S threads per block of W columns).
 template <typename T, int W, int S>
@@ -110,7 +110,7 @@ int run_band(int n, const double* H, double* E, double* Z, int vectors, int rows
       band_chase_lds<T>(tm, n, A.data(), d.data(), e.data(), band.data() + L.v2, band.data() + L.tau2, AB.data());
     });
   } else {
-    team(64, [&](auto& tm) {
+    team(256, [&](auto& tm) {
       band_chase<T>(tm, n, A.data(), d.data(), e.data(), band.data() + L.v2, band.data() + L.tau2, hv.data(),
                     red.data());
     });
