@@ -215,9 +215,10 @@ __global__ void __launch_bounds__(256) twisted_columns_kernel(int n, size_t nmat
 // reflector staging buffers in LDS).
 template <typename T>
 struct BandBt;
-template <> struct BandBt<float> { static constexpr int W = 64, S = 8; };
-// FP64: 16 threads per column (one row of each reflector per thread): 65 ms
-// per 1024 matrices at n = 256 against 72 ms with 8
+// Threads per column S: one row of each chasing reflector per thread (FP32:
+// 2489 -> 2400 ms per slab pass against 8; FP64: 65 ms per 1024 matrices at
+// n = 256 against 72 ms with 8)
+template <> struct BandBt<float> { static constexpr int W = 64, S = 16; };
 template <> struct BandBt<double> { static constexpr int W = 32, S = 16; };
 
 template <typename T, typename TO>

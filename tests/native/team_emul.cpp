@@ -134,7 +134,7 @@ int run_band(int n, const double* H, double* E, double* Z, int vectors, int rows
 
 extern "C" int emul_eigh_band(int n, int fp32, const double* H, double* E, double* Z, int vectors, int rows_out) {
   if (n < 2 || n > 256) return -1;
-  if (fp32) return run_band<float, 64, 8>(n, H, E, Z, vectors, rows_out);
+  if (fp32) return run_band<float, 64, 16>(n, H, E, Z, vectors, rows_out);
   return run_band<double, 32, 16>(n, H, E, Z, vectors, rows_out);
 }
 
